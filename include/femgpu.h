@@ -1,0 +1,208 @@
+/*
+ * femgpu.h -- C ABI of the MI355X (gfx950) assemble + solve path for LMT++-style
+ * linear elasticity (Tetra, Tetra_10, Hexa).
+ *
+ * The entry points replace the hot-path virtuals of LMT++'s
+ * FormulationAncestor<double> (include/formulation/formulation_ancestor.h) as
+ * implemented by Formulation<...> (include/formulation/formulation.h); each
+ * declaration cites the reference interface it stands in for.  Paths are
+ * relative to the LMT++ source tree.
+ *
+ * Conventions
+ *   - every call returns an int status (FEMGPU_OK == 0, see femgpu_status);
+ *   - host buffers are caller-owned and copied; "_device" variants take device
+ *     pointers on the problem's GPU and are read/written on the problem's stream
+ *     after the call has synchronised that stream;
+ *   - a problem owns its device buffers and one HIP stream; a problem is not
+ *     thread-safe, distinct problems are independent;
+ *   - DOF numbering is DOF(node n, component d) = 3n + d (formulation.h:376).
+ */
+#ifndef FEMGPU_H
+#define FEMGPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEMGPU_API __attribute__((visibility("default")))
+
+typedef enum {
+    FEMGPU_OK = 0,
+    FEMGPU_BAD_INPUT = 1,     /* sizes, indices or options out of range */
+    FEMGPU_NEG_JACOBIAN = 2,  /* an element has det J <= 0 at a Gauss point */
+    FEMGPU_NOT_SPD = 3,       /* non-positive pivot / curvature (p.Ap <= 0) */
+    FEMGPU_MAX_ITER = 4,      /* iteration cap reached before the stop test */
+    FEMGPU_HIP_ERROR = 5,     /* a HIP runtime call failed */
+    FEMGPU_NAN = 6,           /* NaN or Inf met in a reduction */
+    FEMGPU_NOT_READY = 7      /* call order violated (e.g. solve before assemble) */
+} femgpu_status;
+
+typedef enum { FEMGPU_TETRA = 0, FEMGPU_TETRA10 = 1, FEMGPU_HEXA = 2 } femgpu_elem_kind;
+typedef enum { FEMGPU_TRI3 = 0, FEMGPU_TRI6 = 1, FEMGPU_QUAD4 = 2 } femgpu_face_kind;
+
+typedef enum { FEMGPU_FP64 = 0, FEMGPU_MIXED = 1 } femgpu_precision;
+typedef enum { FEMGPU_JACOBI = 0, FEMGPU_BLOCK_JACOBI = 1 } femgpu_precond;
+/* stop tests: relative L2 residual, or the reference's max_i |r_i| <= crit
+ * (containers/MatWithTinyBlocks.h:1115,1136) */
+typedef enum { FEMGPU_REL_L2 = 0, FEMGPU_ABS_MAXNORM = 1 } femgpu_crit_kind;
+
+typedef struct femgpu_problem femgpu_problem;
+
+typedef struct {
+    int elem_kind;           /* femgpu_elem_kind */
+    uint32_t n_nodes;
+    const double *coords;    /* n_nodes * 3 */
+    uint32_t n_elem;
+    const uint32_t *conn;    /* n_elem * nodes_per_elem, reference local node order */
+} femgpu_mesh;
+
+typedef struct {
+    double nu;                /* poisson_ratio (formulations/formulation_elasticity_iso.py:6) */
+    double E;                 /* elastic_modulus when E_elem == NULL (global interpolation) */
+    const double *E_elem;     /* per-element modulus (interpolation='elementary'), or NULL */
+    double body_force[3];     /* density * f_vol (formulation_elasticity_iso.py:52) */
+} femgpu_material;
+
+typedef struct {
+    int precision;            /* femgpu_precision */
+    int precond;              /* femgpu_precond */
+    int crit_kind;            /* femgpu_crit_kind */
+    double tol;               /* REL_L2: ||r|| <= tol ||b||;  ABS_MAXNORM: max|r_i| <= tol */
+    int max_iter;             /* inner iterations cap (the reference has none) */
+    double reliable_delta;    /* MIXED: FP64 residual refresh when the FP32 residual fell by this factor */
+    int chunk;                /* iterations enqueued between host checks */
+} femgpu_solve_opts;
+
+typedef struct {
+    int status;
+    int iters;                /* CG iterations (inner FP32 iterations in MIXED mode) */
+    int refreshes;            /* MIXED: FP64 residual refreshes (outer refinement steps) */
+    double rel_res;           /* final ||b - K u||_2 / ||b||_2, FP64 */
+    double max_abs_res;       /* final max_i |b - K u|_i, FP64 */
+    double solve_ms;          /* device time of the solve (HIP events on the problem's stream) */
+} femgpu_solve_stats;
+
+/* ---- lifecycle ---------------------------------------------------------------------------- */
+
+/* Stands in for `new Formulation<TM,elasticity_iso,...>(m)` made by the generated factory
+ * Problem_<name>::new_formulation_<f> (__init__.py:167-168,205-209) plus
+ * update_connectivity / allocate_matrices (formulation.h:283-412): copies the mesh, builds the
+ * DOF numbering, the 3x3-block pattern (ResizeMat::GetNZ, formulation.h:200-230), the element
+ * colouring and the scatter map, and uploads everything to the current HIP device. */
+FEMGPU_API int femgpu_create(const femgpu_mesh *mesh, femgpu_problem **out);
+/* ProblemAncestor deletes its formulations (problem_ancestor.h:298-301). */
+FEMGPU_API int femgpu_destroy(femgpu_problem *p);
+
+/* ---- inputs ------------------------------------------------------------------------------- */
+
+/* Material fields: elastic_modulus / poisson_ratio / density*f_vol of
+ * formulations/formulation_elasticity_iso.py:4-7 (global or elementary interpolation). */
+FEMGPU_API int femgpu_set_material(femgpu_problem *p, const femgpu_material *mat);
+/* Per-element modulus from a device array of n_elem doubles. */
+FEMGPU_API int femgpu_set_elastic_modulus_device(femgpu_problem *p, const double *d_E);
+/* Skin loads: f_surf (interpolation='skin_elementary', formulation_elasticity_iso.py:22) on
+ * boundary faces, integrated by the skin loop AssembleElem::ass_skin_elem
+ * (formulation.h:584-594).  faces: n_faces * nodes_per_face in the reference's child order. */
+FEMGPU_API int femgpu_set_traction(femgpu_problem *p, uint32_t n_faces, const uint32_t *faces, const double *trac);
+/* add_constraint(txt, penalty) (formulation_ancestor.h:197) for u[dof] = value with a unit
+ * coefficient, penalty branch of assemble_constraints (formulation.h:1028-1070). */
+FEMGPU_API int femgpu_add_constraint(femgpu_problem *p, uint32_t dof, double value, double penalty);
+FEMGPU_API int femgpu_add_constraints(femgpu_problem *p, uint32_t n, const uint32_t *dofs, const double *values,
+                                      double penalty);
+FEMGPU_API int femgpu_clear_constraints(femgpu_problem *p);
+
+/* ---- hot path ----------------------------------------------------------------------------- */
+
+/* assemble(assemble_mat, assemble_vec) (formulation_ancestor.h:63; formulation.h:1112-1125):
+ * element loop (formulation.h:626-647) + skin loads + penalty constraints. */
+FEMGPU_API int femgpu_assemble(femgpu_problem *p, int assemble_mat, int assemble_vec);
+/* solve_system(crit) (formulation_ancestor.h:64; formulation.h:1494-1504 and the CG branch
+ * :1487-1492, containers/MatWithTinyBlocks.h:1101-1151).  The initial guess is the current
+ * result vector (vectors[0] after shift(), formulation.h:424-433). */
+FEMGPU_API int femgpu_solve_system(femgpu_problem *p, const femgpu_solve_opts *opts, femgpu_solve_stats *stats);
+/* solve(crit) (formulation_ancestor.h:49; formulation.h:1517-1568): assemble + solve_system. */
+FEMGPU_API int femgpu_solve(femgpu_problem *p, const femgpu_solve_opts *opts, femgpu_solve_stats *stats);
+FEMGPU_API void femgpu_default_solve_opts(femgpu_solve_opts *opts);
+
+/* ---- results ------------------------------------------------------------------------------ */
+
+/* get_result(0) / update_variables (formulation_ancestor.h:237,179; formulation.h:2571-2576). */
+FEMGPU_API int femgpu_get_result(femgpu_problem *p, double *u, size_t n);
+FEMGPU_API int femgpu_get_result_device(femgpu_problem *p, double *d_u);
+/* Writes vectors[0], the initial guess of the next solve (shift(), formulation.h:424-433). */
+FEMGPU_API int femgpu_set_result(femgpu_problem *p, const double *u, size_t n);
+/* get_sollicitation() (formulation_ancestor.h:236). */
+FEMGPU_API int femgpu_get_sollicitation(femgpu_problem *p, double *F, size_t n);
+/* get_mat() (formulation_ancestor.h:230): the assembled K as full 3x3-block CSR, block row-major
+ * values (both triangles; the reference stores the lower one).  Any pointer may be NULL. */
+FEMGPU_API int femgpu_get_matrix(femgpu_problem *p, uint32_t *row_ptr, uint32_t *col, double *vals);
+FEMGPU_API int femgpu_get_sizes(femgpu_problem *p, uint32_t *n_nodes, uint32_t *n_elem, uint64_t *nnzb,
+                                uint32_t *n_colors);
+/* max|diag K| taken before the constraints (formulation.h:1029-1030). */
+FEMGPU_API int femgpu_get_max_diag(femgpu_problem *p, double *max_diag);
+
+/* ---- differentiation ---------------------------------------------------------------------- */
+
+/* Adjoint of the assembled system: solves K lambda = dJ/du and returns
+ * dJ/dE_e = -lambda_e^T (dK_e/dE_e) u_e for every element.  Replaces the forward-sensitivity
+ * loop of FormulationFit::exec / solve_and_get_derivatives (FormulationFit.h:19-85,
+ * formulation.h:1588-1652), which costs one solve per parameter.  The penalty scale
+ * max|diag K| is held fixed. */
+FEMGPU_API int femgpu_adjoint(femgpu_problem *p, const double *dJdu, double *dJdE, const femgpu_solve_opts *opts,
+                              femgpu_solve_stats *stats);
+FEMGPU_API int femgpu_adjoint_device(femgpu_problem *p, const double *d_dJdu, double *d_dJdE,
+                                     const femgpu_solve_opts *opts, femgpu_solve_stats *stats);
+/* Element-gradient kernel alone for given u and lambda (host arrays, 3*n_nodes each). */
+FEMGPU_API int femgpu_elem_grad(femgpu_problem *p, const double *u, const double *lam, double *dJdE);
+
+/* ---- generic block systems ---------------------------------------------------------------- */
+
+/* MatWithTinyBlocks<double,Sym<3>> built from a caller's matrix (MatWithTinyBlocks.h:224-269)
+ * and solved by the same PCG: full 3x3-block CSR (both triangles, rows sorted). */
+FEMGPU_API int femgpu_create_from_bsr(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col,
+                                      const double *vals, const double *rhs, femgpu_problem **out);
+
+/* ---- mesh utilities (host, no GPU) -------------------------------------------------------- */
+
+/* make_rect(m, Elem(), X0, X1, nb_points) (mesh/make_rect.h:389-430 Tetra, :443-544 Tetra_10,
+ * :653-679 Hexa).  Call with coords == conn == NULL to get the counts. */
+FEMGPU_API int femgpu_make_rect(int elem_kind, const double X0[3], const double X1[3], const uint32_t nb_points[3],
+                                double *coords, uint32_t *conn, uint32_t *n_nodes, uint32_t *n_elem);
+/* Skin faces (faces with one parent element, mesh.h:151-188) in the reference child order;
+ * call with faces == NULL to get the count. */
+FEMGPU_API int femgpu_skin_faces(int elem_kind, uint32_t n_nodes, uint32_t n_elem, const uint32_t *conn,
+                                 uint32_t *faces, uint32_t *n_faces);
+
+/* ---- batched uneven meshes ---------------------------------------------------------------- */
+
+typedef struct femgpu_batch femgpu_batch;
+/* n independent problems assembled and solved together (no reference counterpart: the
+ * reference solves one formulation at a time).  Each problem keeps its own scalars. */
+FEMGPU_API int femgpu_batch_create(uint32_t n, const femgpu_mesh *meshes, femgpu_batch **out);
+FEMGPU_API int femgpu_batch_set_material(femgpu_batch *b, uint32_t i, const femgpu_material *mat);
+FEMGPU_API int femgpu_batch_set_traction(femgpu_batch *b, uint32_t i, uint32_t n_faces, const uint32_t *faces,
+                                         const double *trac);
+FEMGPU_API int femgpu_batch_add_constraints(femgpu_batch *b, uint32_t i, uint32_t n, const uint32_t *dofs,
+                                            const double *values, double penalty);
+FEMGPU_API int femgpu_batch_assemble(femgpu_batch *b);
+FEMGPU_API int femgpu_batch_solve(femgpu_batch *b, const femgpu_solve_opts *opts, femgpu_solve_stats *per_problem,
+                                  double *total_ms);
+FEMGPU_API int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, size_t n);
+FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);
+
+/* ---- diagnostics -------------------------------------------------------------------------- */
+
+FEMGPU_API const char *femgpu_status_string(int status);
+FEMGPU_API const char *femgpu_last_error(void);
+/* Device time (ms) of the last assembly (element + skin + constraint kernels). */
+FEMGPU_API int femgpu_get_assemble_ms(femgpu_problem *p, double *ms);
+/* Average device time (ms) of one block-SpMV launch (precision FP64 or the FP32 copy), measured
+ * with HIP events around reps back-to-back launches on the problem's stream. */
+FEMGPU_API int femgpu_time_spmv(femgpu_problem *p, int precision, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

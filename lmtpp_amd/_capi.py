@@ -1,0 +1,121 @@
+"""ctypes binding of libfemgpu.so, the C ABI declared in include/femgpu.h.
+
+The library is the only compute path: importing this module on a machine with a GPU and
+without the built library raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemgpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "femgpu.h")
+
+OK, BAD_INPUT, NEG_JACOBIAN, NOT_SPD, MAX_ITER, HIP_ERROR, NAN, NOT_READY = range(8)
+TETRA, TETRA10, HEXA = 0, 1, 2
+TRI3, TRI6, QUAD4 = 0, 1, 2
+FP64, MIXED = 0, 1
+JACOBI, BLOCK_JACOBI = 0, 1
+REL_L2, ABS_MAXNORM = 0, 1
+NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
+FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
+NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}
+
+
+class Mesh(C.Structure):
+    _fields_ = [("elem_kind", C.c_int), ("n_nodes", C.c_uint32), ("coords", C.c_void_p),
+                ("n_elem", C.c_uint32), ("conn", C.c_void_p)]
+
+
+class Material(C.Structure):
+    _fields_ = [("nu", C.c_double), ("E", C.c_double), ("E_elem", C.c_void_p), ("body_force", C.c_double * 3)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("crit_kind", C.c_int), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("reliable_delta", C.c_double), ("chunk", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("refreshes", C.c_int), ("rel_res", C.c_double),
+                ("max_abs_res", C.c_double), ("solve_ms", C.c_double)]
+
+
+class FemGpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s (status %d)" % (msg, status))
+        self.status = status
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "femgpu_create": (C.c_int, [C.POINTER(Mesh), C.POINTER(_P)]),
+    "femgpu_destroy": (C.c_int, [_P]),
+    "femgpu_set_material": (C.c_int, [_P, C.POINTER(Material)]),
+    "femgpu_set_elastic_modulus_device": (C.c_int, [_P, _P]),
+    "femgpu_set_traction": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "femgpu_add_constraint": (C.c_int, [_P, C.c_uint32, C.c_double, C.c_double]),
+    "femgpu_add_constraints": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_double]),
+    "femgpu_clear_constraints": (C.c_int, [_P]),
+    "femgpu_assemble": (C.c_int, [_P, C.c_int, C.c_int]),
+    "femgpu_solve_system": (C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "femgpu_solve": (C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "femgpu_default_solve_opts": (None, [C.POINTER(SolveOpts)]),
+    "femgpu_get_result": (C.c_int, [_P, _P, C.c_size_t]),
+    "femgpu_get_result_device": (C.c_int, [_P, _P]),
+    "femgpu_set_result": (C.c_int, [_P, _P, C.c_size_t]),
+    "femgpu_get_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
+    "femgpu_get_matrix": (C.c_int, [_P, _P, _P, _P]),
+    "femgpu_get_sizes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint32)]),
+    "femgpu_get_max_diag": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "femgpu_adjoint": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "femgpu_adjoint_device": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "femgpu_elem_grad": (C.c_int, [_P, _P, _P, _P]),
+    "femgpu_create_from_bsr": (C.c_int, [C.c_uint32, _P, _P, _P, _P, C.POINTER(_P)]),
+    "femgpu_make_rect": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "femgpu_skin_faces": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(C.c_uint32)]),
+    "femgpu_batch_create": (C.c_int, [C.c_uint32, C.POINTER(Mesh), C.POINTER(_P)]),
+    "femgpu_batch_set_material": (C.c_int, [_P, C.c_uint32, C.POINTER(Material)]),
+    "femgpu_batch_set_traction": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "femgpu_batch_add_constraints": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.c_double]),
+    "femgpu_batch_assemble": (C.c_int, [_P]),
+    "femgpu_batch_solve": (C.c_int, [_P, C.POINTER(SolveOpts), _P, C.POINTER(C.c_double)]),
+    "femgpu_batch_get_result": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "femgpu_batch_destroy": (C.c_int, [_P]),
+    "femgpu_status_string": (C.c_char_p, [C.c_int]),
+    "femgpu_last_error": (C.c_char_p, []),
+    "femgpu_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "femgpu_time_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+}
+
+
+def load():
+    """Load libfemgpu.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("lmtpp_amd: %s is not built (run __graft_entry__.build())" % LIB_PATH)
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != OK:
+        lib = load()
+        raise FemGpuError(rc, "%s: %s" % (what, lib.femgpu_last_error().decode()))
+    return rc
+
+
+def declared_symbols():
+    """Names of every function declared in include/femgpu.h."""
+    import re
+    with open(HEADER) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"FEMGPU_API\s+[\w\s\*]+?\b(femgpu_\w+)\s*\(", txt)))

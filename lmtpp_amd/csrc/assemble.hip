@@ -1,0 +1,422 @@
+// Assembly kernels for gfx950: per-element stiffness and load of isotropic linear elasticity
+// (Tetra, Tetra_10, Hexa), scattered colour by colour into the 3x3-block CSR without atomics;
+// skin tractions; penalty constraints; FP32 copy; per-element adjoint gradient.
+//
+// Layout of K in HBM: 9 planes of nnzb doubles (plane q = block entry (q/3, q%3)), plane
+// stride P.ps; a block's position p is its CSR slot.  Plane-major storage makes every
+// SpMV load a contiguous run across the lanes of a row group.
+#include "femgpu_internal.hpp"
+
+namespace femgpu {
+
+// ---------------------------------------------------------------------------------------------
+// Element descriptions.  Quadrature of the runtime Gauss loop (formulation.h:2917-2933) at
+// order 2*degree-2 (formal_lf/formulation.py:30-32,121-123):
+//   Tetra    1 point  w=1/6 at (1/4,1/4,1/4)                  include/mesh/element_Tetra.h:8
+//   Tetra_10 4 points w=1/24, a=0.1381966011250105, b=0.5854101966249685 (element_Tetra.h:12-15)
+//   Hexa     2x2x2    w=1/8 at 0.211324865405 / 0.788675134595 (12 digits, element_Hexa.h:12-19)
+// Shape functions: formulations/element_Tetra.py:27-30, element_Tetra_10.py:27-41,
+// element_Hexa.py:61-68.
+// ---------------------------------------------------------------------------------------------
+template <int K> struct ET;
+template <> struct ET<TETRA> { static constexpr int NPE = 4, NGP = 1, TEAM = 16; };
+template <> struct ET<TETRA10> { static constexpr int NPE = 10, NGP = 4, TEAM = 128; };
+template <> struct ET<HEXA> { static constexpr int NPE = 8, NGP = 8, TEAM = 64; };
+
+template <int K>
+__device__ inline void gauss(int q, double xi[3], double &w) {
+    if constexpr (K == TETRA) {
+        w = 0.16666666666666666667; xi[0] = xi[1] = xi[2] = 0.25;
+    } else if constexpr (K == TETRA10) {
+        const double a = 0.1381966011250105, b = 0.5854101966249685;
+        w = 0.041666666666666666667;
+        xi[0] = q == 1 ? b : a; xi[1] = q == 2 ? b : a; xi[2] = q == 3 ? b : a;
+    } else {
+        const double lo = 0.211324865405, hi = 0.788675134595;
+        // point order (a,a,a),(b,a,a),(b,b,a),(a,b,a), then the same at z = b
+        const int bx = (q & 3) == 1 || (q & 3) == 2, by = (q & 3) >= 2, bz = q >= 4;
+        w = 0.125; xi[0] = bx ? hi : lo; xi[1] = by ? hi : lo; xi[2] = bz ? hi : lo;
+    }
+}
+
+// value and reference gradient of shape function a at xi
+template <int K>
+__device__ inline double shape(int a, const double xi[3], double d[3]) {
+    const double x = xi[0], y = xi[1], z = xi[2];
+    if constexpr (K == TETRA) {
+        d[0] = a == 0 ? -1.0 : a == 1 ? 1.0 : 0.0;
+        d[1] = a == 0 ? -1.0 : a == 2 ? 1.0 : 0.0;
+        d[2] = a == 0 ? -1.0 : a == 3 ? 1.0 : 0.0;
+        return a == 0 ? 1 - x - y - z : a == 1 ? x : a == 2 ? y : z;
+    } else if constexpr (K == TETRA10) {
+        const double L[4] = {1 - x - y - z, x, y, z};
+        // dL_m/dxi_j = -1 for m = 0, delta(m, j+1) otherwise
+        auto dL = [](int m, int j) { return m == 0 ? -1.0 : (m == j + 1 ? 1.0 : 0.0); };
+        if (a < 4) {
+            const double s = 4 * L[a] - 1;
+            for (int j = 0; j < 3; ++j) d[j] = s * dL(a, j);
+            return L[a] * (2 * L[a] - 1);
+        }
+        // nodes 4..9 on edges 01, 12, 02, 03, 13, 23
+        const int i0 = a == 4 ? 0 : a == 5 ? 1 : a == 6 ? 0 : a == 7 ? 0 : a == 8 ? 1 : 2;
+        const int i1 = a == 4 ? 1 : a == 5 ? 2 : a == 6 ? 2 : a == 7 ? 3 : a == 8 ? 3 : 3;
+        for (int j = 0; j < 3; ++j) d[j] = 4 * (dL(i0, j) * L[i1] + L[i0] * dL(i1, j));
+        return 4 * L[i0] * L[i1];
+    } else {
+        const int bx = a == 1 || a == 2 || a == 5 || a == 6, by = a == 2 || a == 3 || a == 6 || a == 7, bz = a >= 4;
+        const double X = bx ? x : 1 - x, Y = by ? y : 1 - y, Z = bz ? z : 1 - z;
+        const double sx = bx ? 1.0 : -1.0, sy = by ? 1.0 : -1.0, sz = bz ? 1.0 : -1.0;
+        d[0] = sx * Y * Z; d[1] = X * sy * Z; d[2] = X * Y * sz;
+        return X * Y * Z;
+    }
+}
+
+// Lame constants equal to the inverse of the isotropic compliance with s1 = 1/E, s2 = -nu/E,
+// s3 = (1+nu)/E on tensor shear (formal_lf/material_help.py:116-128,181-183).
+__device__ inline void lame(double E, double nu, double &lam, double &mu) {
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    mu = E / (2 * (1 + nu));
+}
+
+// Phases A (coordinates to LDS) and B (per Gauss point gradients) shared by the assembly and
+// the adjoint-gradient kernels.  J[j][i] = dx_i/dxi_j (formal_lf/element.py:195-205), detJ
+// signed (formal_lf/formulation.py:553), grad = J^-1 dN.
+template <int K>
+struct ElemLds {
+    double x[ET<K>::NPE * 3];
+    double g[ET<K>::NGP][ET<K>::NPE][3];
+    double wd[ET<K>::NGP];
+    double N[ET<K>::NGP][ET<K>::NPE];
+};
+
+template <int K>
+__device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t e, const uint32_t *__restrict__ conn,
+                                     const double *__restrict__ coords, int *flags) {
+    constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP;
+    if (valid && t < NPE * 3) L.x[t] = coords[3 * (size_t)conn[(size_t)e * NPE + t / 3] + t % 3];
+    __syncthreads();
+    if (valid && t < NGP * NPE) {
+        const int q = t / NPE, a = t % NPE;
+        double xi[3], w;
+        gauss<K>(q, xi, w);
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int b = 0; b < NPE; ++b) {
+            double d[3];
+            shape<K>(b, xi, d);
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) J[j][i] += d[j] * L.x[3 * b + i];
+        }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double id = 1.0 / det;
+        double Ji[3][3];
+        Ji[0][0] = c00 * id; Ji[1][0] = c01 * id; Ji[2][0] = c02 * id;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        double d[3];
+        L.N[q][a] = shape<K>(a, xi, d);
+        for (int j = 0; j < 3; ++j) L.g[q][a][j] = Ji[j][0] * d[0] + Ji[j][1] * d[1] + Ji[j][2] * d[2];
+        if (a == 0) {
+            L.wd[q] = w * det;
+            if (!(det > 0)) atomicOr(flags, 1);
+        }
+    }
+    __syncthreads();
+}
+
+// 3x3 block of element nodes (a,b):
+//   K_ab(i,j) = sum_q w_q detJ_q [lambda g_a,i g_b,j + mu g_a,j g_b,i + mu delta_ij g_a.g_b]
+// (energy trace_sym_col(sigma, eps_test) with tensor shear, include/codegen/__init__.py:23-42,
+// 82-87).  Products are formed so that lane (b,a) computes the bitwise transpose of (a,b).
+template <int K>
+__device__ inline void elem_block(const ElemLds<K> &L, int a, int b, double lam, double mu, double k[9]) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) k[c] = 0;
+    for (int q = 0; q < ET<K>::NGP; ++q) {
+        const double *ga = L.g[q][a], *gb = L.g[q][b];
+        const double wd = L.wd[q];
+        const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double v = lam * (ga[i] * gb[j]) + mu * (ga[j] * gb[i]);
+                if (i == j) v += mu * dot;
+                k[3 * i + j] += wd * v;
+            }
+    }
+}
+
+// One colour of the element loop (formulation.h:626-647).  A team of ET<K>::TEAM lanes owns one
+// element; no two elements of a colour share a node, so the read-modify-write scatter into K and
+// F needs no atomics and its order is fixed.
+template <int K>
+__global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ order, uint32_t count,
+                                                  const uint32_t *__restrict__ conn, const double *__restrict__ coords,
+                                                  const double *__restrict__ E, double nu, double bf0, double bf1,
+                                                  double bf2, const uint32_t *__restrict__ slot, double *__restrict__ Kv,
+                                                  uint64_t ps, double *__restrict__ F, int do_mat, int do_vec,
+                                                  int *flags) {
+    constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP, TEAM = ET<K>::TEAM, TPW = 256 / TEAM;
+    __shared__ ElemLds<K> lds[TPW];
+    const int team = threadIdx.x / TEAM, t = threadIdx.x % TEAM;
+    const uint32_t idx = blockIdx.x * TPW + team;
+    const bool valid = idx < count;
+    const uint32_t e = valid ? order[idx] : 0;
+    ElemLds<K> &L = lds[team];
+    elem_geometry<K>(L, t, valid, e, conn, coords, flags);
+    if (!valid) return;
+    if (do_mat && t < NPE * NPE) {
+        const int a = t / NPE, b = t % NPE;
+        double lam, mu;
+        lame(E[e], nu, lam, mu);
+        double k[9];
+        elem_block<K>(L, a, b, lam, mu, k);
+        const uint64_t p = slot[(size_t)e * NPE * NPE + t];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Kv[c * ps + p] += k[c];
+    }
+    if (do_vec && t < NPE * 3) {
+        // F_a = sum_q w_q detJ_q N_a(q) (density f_vol) (formulation_elasticity_iso.py:52)
+        const int a = t / 3, d = t % 3;
+        const double f = d == 0 ? bf0 : d == 1 ? bf1 : bf2;
+        double s = 0;
+        for (int q = 0; q < NGP; ++q) s += L.wd[q] * L.N[q][a] * f;
+        F[3 * (size_t)conn[(size_t)e * NPE + a] + d] += s;
+    }
+}
+
+// Skin loads, one thread per face, faces coloured like elements.  Gauss tables of the face
+// elements at the generator's key (formal_lf/formulation.py:400-442): Triangle 1 point,
+// Triangle_6 three mid-edge points, Quad 2x2 at (1 +- sqrt(1/3))/2 (element_Quad.py:70-75).
+template <int FK>
+__global__ __launch_bounds__(256) void k_face_load(const uint32_t *__restrict__ order, uint32_t count,
+                                                   const uint32_t *__restrict__ faces, const double *__restrict__ trac,
+                                                   const double *__restrict__ coords, double *__restrict__ F) {
+    constexpr int NPF = FK == TRI3 ? 3 : FK == TRI6 ? 6 : 4;
+    constexpr int NG = FK == TRI3 ? 1 : FK == TRI6 ? 3 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t f = order[i];
+    double x[NPF][3];
+    uint32_t nd[NPF];
+    for (int a = 0; a < NPF; ++a) {
+        nd[a] = faces[(size_t)f * NPF + a];
+        for (int d = 0; d < 3; ++d) x[a][d] = coords[3 * (size_t)nd[a] + d];
+    }
+    const double t0 = trac[3 * (size_t)f], t1 = trac[3 * (size_t)f + 1], t2 = trac[3 * (size_t)f + 2];
+    double acc[NPF] = {};
+    for (int g = 0; g < NG; ++g) {
+        double u, v, w;
+        if constexpr (FK == TRI3) { u = v = 1.0 / 3.0; w = 0.5; }
+        else if constexpr (FK == TRI6) { u = g == 2 ? 0.0 : 0.5; v = g == 0 ? 0.0 : 0.5; w = 1.0 / 6.0; }
+        else {
+            const double lo = (1.0 - sqrt(1.0 / 3.0)) / 2.0, hi = (1.0 + sqrt(1.0 / 3.0)) / 2.0;
+            u = (g & 1) ? lo : hi; v = (g & 2) ? lo : hi; w = 0.25;
+        }
+        double N[NPF], du[NPF], dv[NPF];
+        if constexpr (FK == TRI3) {
+            N[0] = 1 - u - v; N[1] = u; N[2] = v;
+            du[0] = -1; du[1] = 1; du[2] = 0; dv[0] = -1; dv[1] = 0; dv[2] = 1;
+        } else if constexpr (FK == TRI6) {  // formulations/element_Triangle_6.py:38-48
+            const double L0 = 1 - u - v;
+            N[0] = (2 * L0 - 1) * L0; N[1] = (2 * u - 1) * u; N[2] = (2 * v - 1) * v;
+            N[3] = 4 * L0 * u; N[4] = 4 * u * v; N[5] = 4 * v * L0;
+            du[0] = -(4 * L0 - 1); du[1] = 4 * u - 1; du[2] = 0; du[3] = 4 * (L0 - u); du[4] = 4 * v; du[5] = -4 * v;
+            dv[0] = -(4 * L0 - 1); dv[1] = 0; dv[2] = 4 * v - 1; dv[3] = -4 * u; dv[4] = 4 * u; dv[5] = 4 * (L0 - v);
+        } else {  // formulations/element_Quad.py:22-25
+            N[0] = (1 - u) * (1 - v); N[1] = u * (1 - v); N[2] = u * v; N[3] = (1 - u) * v;
+            du[0] = -(1 - v); du[1] = 1 - v; du[2] = v; du[3] = -v;
+            dv[0] = -(1 - u); dv[1] = -u; dv[2] = u; dv[3] = 1 - u;
+        }
+        double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+        for (int a = 0; a < NPF; ++a)
+            for (int d = 0; d < 3; ++d) { a0[d] += du[a] * x[a][d]; a1[d] += dv[a] * x[a][d]; }
+        const double n0 = a0[1] * a1[2] - a0[2] * a1[1], n1 = a0[2] * a1[0] - a0[0] * a1[2], n2 = a0[0] * a1[1] - a0[1] * a1[0];
+        const double ds = w * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        for (int a = 0; a < NPF; ++a) acc[a] += ds * N[a];
+    }
+    for (int a = 0; a < NPF; ++a) {
+        F[3 * (size_t)nd[a]] += acc[a] * t0;
+        F[3 * (size_t)nd[a] + 1] += acc[a] * t1;
+        F[3 * (size_t)nd[a] + 2] += acc[a] * t2;
+    }
+}
+
+__global__ void k_fill(double *p, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template <int K>
+static hipError_t assemble_kind(Problem &P, int do_mat, int do_vec, int *d_flags) {
+    constexpr int TPW = 256 / ET<K>::TEAM;
+    for (uint32_t c = 0; c < P.ecol.n_colors(); ++c) {
+        const uint32_t b = P.ecol.color_ptr[c], n = P.ecol.color_ptr[c + 1] - b;
+        if (!n) continue;
+        hipLaunchKernelGGL(k_assemble<K>, dim3((n + TPW - 1) / TPW), dim3(256), 0, P.stream, P.d_order + b, n, P.d_conn,
+                           P.d_coords, P.d_E, P.nu, P.bf[0], P.bf[1], P.bf[2], P.d_slot, P.d_K, P.ps, P.d_F, do_mat,
+                           do_vec, d_flags);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(Problem &P, int do_mat, int do_vec, int *d_flags) {
+    hipError_t e = hipSuccess;
+    if (do_mat) e = launch_fill(P.d_K, 9 * P.ps, 0.0, P.stream);
+    if (e == hipSuccess && do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
+    if (e != hipSuccess) return e;
+    if (P.kind == TETRA) e = assemble_kind<TETRA>(P, do_mat, do_vec, d_flags);
+    else if (P.kind == TETRA10) e = assemble_kind<TETRA10>(P, do_mat, do_vec, d_flags);
+    else e = assemble_kind<HEXA>(P, do_mat, do_vec, d_flags);
+    if (e != hipSuccess || !do_vec || !P.n_faces) return e;
+    const int fk = face_kind_of(P.kind);
+    for (uint32_t c = 0; c < P.fcol.n_colors(); ++c) {
+        const uint32_t b = P.fcol.color_ptr[c], n = P.fcol.color_ptr[c + 1] - b;
+        if (!n) continue;
+        const dim3 g((n + 255) / 256);
+        if (fk == TRI3) hipLaunchKernelGGL(k_face_load<TRI3>, g, dim3(256), 0, P.stream, P.d_forder + b, n, P.d_faces, P.d_trac, P.d_coords, P.d_F);
+        else if (fk == TRI6) hipLaunchKernelGGL(k_face_load<TRI6>, g, dim3(256), 0, P.stream, P.d_forder + b, n, P.d_faces, P.d_trac, P.d_coords, P.d_F);
+        else hipLaunchKernelGGL(k_face_load<QUAD4>, g, dim3(256), 0, P.stream, P.d_forder + b, n, P.d_faces, P.d_trac, P.d_coords, P.d_F);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Penalty constraints (formulation.h:1028-1070): M = max|diag K| over all scalar diagonal
+// entries, taken after the element loop and before any constraint; then, per constrained DOF in
+// insertion order, K(p,p) += M*P and F(p) += M*P*value.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_max_diag(const uint32_t *__restrict__ diag, uint32_t n_nodes,
+                                                  const double *__restrict__ Kv, uint64_t ps, double *partials,
+                                                  Scalars *sc) {
+    double m = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_nodes; r += gridDim.x * blockDim.x) {
+        const uint64_t p = diag[r];
+        m = fmax(m, fmax(fabs(Kv[0 * ps + p]), fmax(fabs(Kv[4 * ps + p]), fabs(Kv[8 * ps + p]))));
+    }
+    // deterministic two-level max
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+        __threadfence();
+        last = atomicAdd(&sc->counter[CTR_MAXDIAG], 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        const volatile double *vp = partials;
+        for (unsigned i = 0; i < gridDim.x; ++i) t = fmax(t, vp[i]);
+        sc->max_diag = t;
+        sc->counter[CTR_MAXDIAG] = 0;
+    }
+}
+
+__global__ void k_constraints(uint32_t n_cdof, const uint32_t *__restrict__ cdof, const uint32_t *__restrict__ cptr,
+                              const double *__restrict__ cval, const double *__restrict__ cpen,
+                              const uint32_t *__restrict__ diag, double *Kv, uint64_t ps, double *F, const Scalars *sc,
+                              int do_mat, int do_vec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_cdof) return;
+    const double M = sc->max_diag;
+    const uint32_t dof = cdof[i], r = dof / 3, d = dof % 3;
+    double *kp = &Kv[(uint64_t)(4 * d) * ps + diag[r]];
+    for (uint32_t k = cptr[i]; k < cptr[i + 1]; ++k) {
+        const double C = 1.0 * M * cpen[k];
+        if (do_vec) F[dof] += C * cval[k];
+        if (do_mat) *kp += C * 1.0;
+    }
+}
+
+hipError_t launch_constraints(Problem &P, int do_mat, int do_vec) {
+    if (do_mat) hipLaunchKernelGGL(k_max_diag, dim3(256), dim3(256), 0, P.stream, P.d_diag, P.n_nodes, P.d_K, P.ps, P.d_partials, P.d_sc);
+    if (!P.n_cdof) return hipGetLastError();
+    hipLaunchKernelGGL(k_constraints, dim3((P.n_cdof + 255) / 256), dim3(256), 0, P.stream, P.n_cdof, P.d_cdof, P.d_cptr,
+                       P.d_cval, P.d_cpen, P.d_diag, P.d_K, P.ps, P.d_F, P.d_sc, do_mat, do_vec);
+    return hipGetLastError();
+}
+
+__global__ void k_to_fp32(const double *__restrict__ a, float *__restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+
+hipError_t launch_to_fp32(Problem &P) {
+    hipLaunchKernelGGL(k_to_fp32, dim3(8192), dim3(256), 0, P.stream, P.d_K, P.d_K32, 9 * P.ps);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// dJ/dE_e = -lambda_e^T (dK_e/dE_e) u_e; K_e is linear in E_e, so dK_e/dE_e = K_e(E=1).
+// Element-parallel, no scatter; the sum over the team's (a,b) blocks is a fixed-order LDS tree.
+// ---------------------------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(256) void k_elem_grad(uint32_t n_elem, const uint32_t *__restrict__ conn,
+                                                   const double *__restrict__ coords, double nu,
+                                                   const double *__restrict__ u, const double *__restrict__ lam,
+                                                   double *__restrict__ g, int *flags) {
+    constexpr int NPE = ET<K>::NPE, TEAM = ET<K>::TEAM, TPW = 256 / TEAM;
+    __shared__ ElemLds<K> lds[TPW];
+    __shared__ double red[256];
+    const int team = threadIdx.x / TEAM, t = threadIdx.x % TEAM;
+    const uint32_t e = blockIdx.x * TPW + team;
+    const bool valid = e < n_elem;
+    ElemLds<K> &L = lds[team];
+    elem_geometry<K>(L, t, valid, valid ? e : 0, conn, coords, flags);
+    double s = 0;
+    if (valid && t < NPE * NPE) {
+        const int a = t / NPE, b = t % NPE;
+        double l1, m1;
+        lame(1.0, nu, l1, m1);
+        double k[9];
+        elem_block<K>(L, a, b, l1, m1, k);
+        const uint32_t na = conn[(size_t)e * NPE + a], nb = conn[(size_t)e * NPE + b];
+        for (int i = 0; i < 3; ++i) {
+            const double ku = k[3 * i] * u[3 * (size_t)nb] + k[3 * i + 1] * u[3 * (size_t)nb + 1] + k[3 * i + 2] * u[3 * (size_t)nb + 2];
+            s += lam[3 * (size_t)na + i] * ku;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = TEAM / 2; o > 0; o >>= 1) {
+        if (t < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (valid && t == 0) g[e] = -red[threadIdx.x];
+}
+
+hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, double *d_g) {
+    int *flags = reinterpret_cast<int *>(&P.d_sc->pad);
+    if (P.kind == TETRA) hipLaunchKernelGGL(k_elem_grad<TETRA>, dim3((P.n_elem + 15) / 16), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.nu, d_u, d_lam, d_g, flags);
+    else if (P.kind == TETRA10) hipLaunchKernelGGL(k_elem_grad<TETRA10>, dim3((P.n_elem + 1) / 2), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.nu, d_u, d_lam, d_g, flags);
+    else hipLaunchKernelGGL(k_elem_grad<HEXA>, dim3((P.n_elem + 3) / 4), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.nu, d_u, d_lam, d_g, flags);
+    return hipGetLastError();
+}
+
+}  // namespace femgpu

@@ -1,0 +1,46 @@
+// Host-side mesh services of the MI355X path: make_rect lattices, skin faces,
+// the 3x3-block pattern, element colouring and the element->block scatter map.
+// Pure C++ (no HIP); the device side consumes the arrays built here.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace femgpu {
+
+enum ElemKind { TETRA = 0, TETRA10 = 1, HEXA = 2 };
+enum FaceKind { TRI3 = 0, TRI6 = 1, QUAD4 = 2 };
+
+inline int nodes_per_elem(int k) { return k == TETRA ? 4 : k == TETRA10 ? 10 : k == HEXA ? 8 : 0; }
+inline int face_kind_of(int k) { return k == TETRA ? TRI3 : k == TETRA10 ? TRI6 : QUAD4; }
+inline int nodes_per_face(int f) { return f == TRI3 ? 3 : f == TRI6 ? 6 : f == QUAD4 ? 4 : 0; }
+
+// make_rect (mesh/make_rect.h): counts and lattice.
+bool rect_counts(int kind, const uint32_t nb[3], uint32_t &n_nodes, uint32_t &n_elem);
+int make_rect(int kind, const double X0[3], const double X1[3], const uint32_t nb[3], double *coords, uint32_t *conn);
+
+// Faces with exactly one parent element, in (element, local face) order.
+std::vector<uint32_t> skin_faces(int kind, uint32_t n_nodes, uint32_t n_elem, const uint32_t *conn);
+
+// Full symmetric 3x3-block pattern: node adjacency, rows sorted, every row holds its diagonal.
+struct Pattern {
+    uint32_t n_rows = 0;
+    std::vector<uint32_t> row_ptr, col, diag;  // diag[r] = slot of block (r,r)
+};
+Pattern build_pattern(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t *conn);
+
+// Slot of block (conn[e][a], conn[e][b]) for every element and local pair: n_elem * npe * npe.
+std::vector<uint32_t> build_slot_map(const Pattern &pat, uint32_t n_elem, int npe, const uint32_t *conn);
+
+// Greedy first-fit colouring: no two elements of one colour share a node.
+// order = elements sorted by colour (stable), color_ptr = offsets (n_colors + 1).
+struct Coloring {
+    std::vector<uint32_t> order, color_ptr;
+    uint32_t n_colors() const { return color_ptr.empty() ? 0 : (uint32_t)color_ptr.size() - 1; }
+};
+Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t *conn);
+
+// Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
+std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);
+
+}  // namespace femgpu

@@ -1,0 +1,231 @@
+"""Host-side mirror of LMT++'s FormulationAncestor<double> for the elasticity_iso formulation.
+
+Same call surface as the reference (include/formulation/formulation_ancestor.h:49-64,179-237):
+``allocate_matrices()``, ``assemble(assemble_mat, assemble_vec)``, ``solve_system(crit)``,
+``solve(crit)``, ``get_result()``, ``get_sollicitation()``, ``get_mat()``, ``add_constraint``,
+``shift()``, ``update_variables()``.  Every computation runs in libfemgpu.so on the GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Formulation:
+    """One elasticity_iso formulation on one mesh (formulation.h:102-2787).
+
+    ``solve_system(crit)`` keeps the reference meaning of ``crit``: with ``crit > 0`` the solve
+    stops when max_i |r_i| <= crit (containers/MatWithTinyBlocks.h:1115,1136); ``rel_tol``
+    selects the relative L2 stop test instead.  Like the reference, the methods return ``True``
+    on success and ``False`` when the system could not be solved (formulation.h:1394,1530-1531);
+    the reason is kept in ``last_stats`` / ``last_status``.
+    """
+
+    def __init__(self, kind, coords, conn):
+        lib = A.load()
+        self.kind = int(kind)
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+        self.conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(-1, A.NPE[self.kind])
+        m = A.Mesh(self.kind, self.coords.shape[0], _ptr(self.coords), self.conn.shape[0], _ptr(self.conn))
+        h = C.c_void_p()
+        A.check(lib.femgpu_create(C.byref(m), C.byref(h)), "femgpu_create")
+        self._h = h
+        self._lib = lib
+        self.n_nodes = self.coords.shape[0]
+        self.n_elem = self.conn.shape[0]
+        # public knobs of FormulationAncestor (formulation_ancestor.h:244-257)
+        self.default_iterative_criterium = 1e-10
+        self.precision = A.FP64
+        self.precond = A.JACOBI
+        self.max_iter = 100000
+        self.last_stats = None
+        self.last_status = A.OK
+        self.result = np.zeros(3 * self.n_nodes)
+
+    # -- lifecycle -----------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.femgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- inputs --------------------------------------------------------------------------------
+    def set_material(self, E=210000.0, nu=0.3, body_force=(0.0, 0.0, 0.0)):
+        """elastic_modulus (global float or per-element array), poisson_ratio, density*f_vol."""
+        E_arr = np.asarray(E, dtype=np.float64)
+        keep = None
+        if E_arr.ndim == 0:
+            mat = A.Material(float(nu), float(E_arr), None, (C.c_double * 3)(*body_force))
+        else:
+            keep = np.ascontiguousarray(E_arr.reshape(-1))
+            if keep.shape[0] != self.n_elem:
+                raise ValueError("E must have one value per element")
+            mat = A.Material(float(nu), 0.0, _ptr(keep), (C.c_double * 3)(*body_force))
+        A.check(self._lib.femgpu_set_material(self._h, C.byref(mat)), "set_material")
+        del keep
+
+    def set_traction(self, faces, traction):
+        faces = np.ascontiguousarray(faces, dtype=np.uint32)
+        tr = np.ascontiguousarray(np.broadcast_to(np.asarray(traction, dtype=np.float64), (faces.shape[0], 3)))
+        A.check(self._lib.femgpu_set_traction(self._h, faces.shape[0], _ptr(faces), _ptr(tr)), "set_traction")
+
+    def add_constraint(self, dof, value=0.0, penalty=1e5):
+        """Penalty constraint u[dof] = value (formulation_ancestor.h:197, formulation.h:1028-1070)."""
+        A.check(self._lib.femgpu_add_constraint(self._h, int(dof), float(value), float(penalty)), "add_constraint")
+
+    def add_constraints(self, dofs, values=None, penalty=1e5):
+        dofs = np.ascontiguousarray(dofs, dtype=np.uint32)
+        vals = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        A.check(self._lib.femgpu_add_constraints(self._h, dofs.shape[0], _ptr(dofs),
+                                                 None if vals is None else _ptr(vals), float(penalty)),
+                "add_constraints")
+
+    def clear_constraints(self):
+        A.check(self._lib.femgpu_clear_constraints(self._h), "clear_constraints")
+
+    # -- hot path ------------------------------------------------------------------------------
+    def allocate_matrices(self, *_):
+        """Pattern and DOF numbering are built once at construction (formulation.h:338-412)."""
+        return None
+
+    def assemble(self, assemble_mat=True, assemble_vec=True):
+        A.check(self._lib.femgpu_assemble(self._h, int(bool(assemble_mat)), int(bool(assemble_vec))), "assemble")
+
+    def _opts(self, crit, rel_tol):
+        o = A.SolveOpts()
+        self._lib.femgpu_default_solve_opts(C.byref(o))
+        o.precision = self.precision
+        o.precond = self.precond
+        o.max_iter = self.max_iter
+        if rel_tol is not None:
+            o.crit_kind, o.tol = A.REL_L2, float(rel_tol)
+        else:
+            c = crit if crit else self.default_iterative_criterium
+            o.crit_kind, o.tol = A.ABS_MAXNORM, float(c)
+        return o
+
+    def solve_system(self, crit=0.0, rel_tol=None):
+        o = self._opts(crit, rel_tol)
+        st = A.SolveStats()
+        rc = self._lib.femgpu_solve_system(self._h, C.byref(o), C.byref(st))
+        self.last_stats, self.last_status = st, rc
+        if rc in (A.BAD_INPUT, A.HIP_ERROR, A.NOT_READY):
+            A.check(rc, "solve_system")
+        return rc == A.OK
+
+    def solve(self, crit=0.0, rel_tol=None):
+        self.assemble(True, True)
+        ok = self.solve_system(crit, rel_tol)
+        if ok:
+            self.update_variables()
+        return ok
+
+    # -- results -------------------------------------------------------------------------------
+    def get_result(self):
+        u = np.zeros(3 * self.n_nodes)
+        A.check(self._lib.femgpu_get_result(self._h, _ptr(u), u.size), "get_result")
+        return u
+
+    def set_result(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        A.check(self._lib.femgpu_set_result(self._h, _ptr(u), u.size), "set_result")
+
+    def shift(self):
+        """vectors[0] (the previous solution) becomes the initial guess; it already is on the GPU."""
+        return None
+
+    def update_variables(self):
+        """node.dep[d] = vectors[0][3 i + d] (formulation.h:2571-2576)."""
+        self.result = self.get_result()
+        return self.result.reshape(-1, 3)
+
+    def get_sollicitation(self):
+        F = np.zeros(3 * self.n_nodes)
+        A.check(self._lib.femgpu_get_sollicitation(self._h, _ptr(F), F.size), "get_sollicitation")
+        return F
+
+    def sizes(self):
+        nn, ne, nz, nc = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint32()
+        A.check(self._lib.femgpu_get_sizes(self._h, C.byref(nn), C.byref(ne), C.byref(nz), C.byref(nc)), "sizes")
+        return dict(n_nodes=nn.value, n_elem=ne.value, nnzb=nz.value, n_colors=nc.value)
+
+    def get_mat(self):
+        """Assembled K as full 3x3-block CSR: (row_ptr, col, vals[nnzb,3,3])."""
+        nnzb = self.sizes()["nnzb"]
+        rp = np.zeros(self.n_nodes + 1, dtype=np.uint32)
+        col = np.zeros(nnzb, dtype=np.uint32)
+        vals = np.zeros((nnzb, 3, 3))
+        A.check(self._lib.femgpu_get_matrix(self._h, _ptr(rp), _ptr(col), _ptr(vals)), "get_mat")
+        return rp, col, vals
+
+    def max_diag(self):
+        v = C.c_double()
+        A.check(self._lib.femgpu_get_max_diag(self._h, C.byref(v)), "max_diag")
+        return v.value
+
+    def assemble_ms(self):
+        v = C.c_double()
+        A.check(self._lib.femgpu_get_assemble_ms(self._h, C.byref(v)), "assemble_ms")
+        return v.value
+
+    def time_spmv(self, precision=A.FP64, reps=20):
+        v = C.c_double()
+        A.check(self._lib.femgpu_time_spmv(self._h, int(precision), int(reps), C.byref(v)), "time_spmv")
+        return v.value
+
+    # -- differentiation -----------------------------------------------------------------------
+    def adjoint(self, dJdu, rel_tol=1e-12):
+        dJdu = np.ascontiguousarray(dJdu, dtype=np.float64)
+        g = np.zeros(self.n_elem)
+        o = self._opts(0.0, rel_tol)
+        st = A.SolveStats()
+        A.check(self._lib.femgpu_adjoint(self._h, _ptr(dJdu), _ptr(g), C.byref(o), C.byref(st)), "adjoint")
+        self.last_stats = st
+        return g
+
+    def elem_grad(self, u, lam):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        g = np.zeros(self.n_elem)
+        A.check(self._lib.femgpu_elem_grad(self._h, _ptr(u), _ptr(lam), _ptr(g)), "elem_grad")
+        return g
+
+
+class BlockSystem(Formulation):
+    """MatWithTinyBlocks<double,Sym<3>> built from a caller's full 3x3-block CSR and solved by the
+    same PCG (containers/MatWithTinyBlocks.h:224-269,1101-1151)."""
+
+    def __init__(self, row_ptr, col, vals, rhs):  # noqa: D401  (no mesh)
+        lib = A.load()
+        rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        cl = np.ascontiguousarray(col, dtype=np.uint32)
+        vv = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1, 9)
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        h = C.c_void_p()
+        A.check(lib.femgpu_create_from_bsr(rp.shape[0] - 1, _ptr(rp), _ptr(cl), _ptr(vv), _ptr(b), C.byref(h)),
+                "create_from_bsr")
+        self._h, self._lib = h, lib
+        self.kind = None
+        self.n_nodes = rp.shape[0] - 1
+        self.n_elem = 0
+        self.default_iterative_criterium = 1e-10
+        self.precision, self.precond, self.max_iter = A.FP64, A.JACOBI, 100000
+        self.last_stats, self.last_status = None, A.OK
+        self.result = np.zeros(3 * self.n_nodes)
+
+    def assemble(self, assemble_mat=True, assemble_vec=True):
+        return None

@@ -1,0 +1,77 @@
+"""Writes the golden fixtures of tests/golden/ from the reference builds in oracle/_ref.
+
+oracle/_ref is compiled from the LMT++ sources (oracle/Makefile, target `ref`); this script is
+run where those sources exist.  Fixtures are data only (inputs and the reference's outputs):
+  ref_mesh_<kind>_<nx>_<ny>_<nz>.npz  coords, conn   -- make_rect (mesh/make_rect.h)
+  ref_tinyblock_pcg_tetra11.npz        K, F, crit, x, iters, factor
+        -- MatWithTinyBlocks chol_incomp + solve_using_incomplete_chol_factorize
+           (containers/MatWithTinyBlocks.h:426-523,1101-1151) on the config-0 system
+  ref_cg_kat.json                       known-answer systems of
+        tests/include/containers/conjugate_gradient_test.cpp:15-71
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import oracle as O  # noqa: E402
+
+MESHES = [("tetra", O.TETRA, (4, 4, 4)), ("tetra", O.TETRA, (4, 3, 5)), ("hexa", O.HEXA, (3, 4, 5)),
+          ("tetra10", O.TETRA10, (3, 3, 3)), ("tetra10", O.TETRA10, (4, 3, 2))]
+
+
+def config0_system():
+    c, e = O.make_rect(O.TETRA, (11, 11, 11))
+    faces = O.skin_faces(O.TETRA, len(c), e)
+    fx = faces[np.all(c[faces][:, :, 0] > 1 - 1e-6, axis=1)]
+    rp, col, vals, F, _ = O.assemble(O.TETRA, c, e, 210000.0, 0.3, faces=fx, face_trac=(0, 0, -1))
+    clamp = np.where(c[:, 0] < 1e-6)[0]
+    dofs = (3 * clamp[:, None] + np.arange(3)).ravel()
+    O.penalty(rp, col, vals, F, dofs, None, 1e5)
+    return rp, col, vals, F
+
+
+def main():
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "liboracle.so"])
+    tmp = tempfile.mkdtemp()
+    for name, kind, nb in MESHES:
+        c, e = O.ref_mesh(name, nb, tmp)
+        np.savez_compressed(os.path.join(HERE, "ref_mesh_%s_%d_%d_%d.npz" % ((name,) + nb)), coords=c, conn=e)
+    rp, col, vals, F = config0_system()
+    crit = 1e-10 * np.abs(F).max()
+    x, iters, L, present = O.ref_tinyblock_pcg(rp, col, vals, F, np.zeros_like(F), crit, tmp)
+    np.savez_compressed(os.path.join(HERE, "ref_tinyblock_pcg_tetra11.npz"), row_ptr=rp, col=col, vals=vals, F=F,
+                        crit=crit, x=x, iters=iters, factor=L, present=present)
+    kat = {
+        "source": "tests/include/containers/conjugate_gradient_test.cpp:15-71",
+        "sym3": {"lower": [[1], [2, 5], [-3, -7, 13]], "b": [3, 4, 5], "x": [11, 2, 4], "rel_tol": 1e-12},
+        "dense10": {
+            "rows": [[348, 295, 221, 165, 221, 311, 311, 279, 216, 200],
+                     [295, 333, 113, 152, 174, 271, 244, 250, 169, 179],
+                     [221, 113, 277, 125, 166, 215, 272, 159, 207, 134],
+                     [165, 152, 125, 152, 155, 135, 206, 107, 143, 143],
+                     [221, 174, 166, 155, 270, 250, 280, 170, 219, 151],
+                     [311, 271, 215, 135, 250, 412, 389, 287, 247, 167],
+                     [311, 244, 272, 206, 280, 389, 474, 259, 259, 219],
+                     [279, 250, 159, 107, 170, 287, 259, 274, 140, 159],
+                     [216, 169, 207, 143, 219, 247, 259, 140, 297, 141],
+                     [200, 179, 134, 143, 151, 167, 219, 159, 141, 165]],
+            "b": [0, 8, 1, 4, 6, 6, 0, 9, 0, 6],
+            "x": [-1.4176810081375044408, -3.0067729851774937789, 3.5389214608316368121, 7.5278767154067853482,
+                  -.95330870992906846260, 16.004952790217101221, -11.900092892559983095, -7.5660154416662888416,
+                  -6.0846051557343736576, 8.5770626957989029109],
+            "rel_tol": 1e-12},
+    }
+    with open(os.path.join(HERE, "ref_cg_kat.json"), "w") as f:
+        json.dump(kat, f, indent=1)
+    print("golden fixtures written; reference PCG iterations:", iters)
+
+
+if __name__ == "__main__":
+    main()

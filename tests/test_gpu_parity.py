@@ -1,0 +1,244 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle and the reference fixtures.
+
+Tolerances: assembled blocks agree to 1e-13 of the matrix scale (the reference's generated
+kernels and summation order are not reproducible bit for bit); patterns and DOF numbering are
+exact; FP64 solutions agree with the oracle's dense solve to 1e-10 relative; MIXED solutions to
+1e-8 relative.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lmtpp_amd as L
+from conftest import GOLDEN
+from problems import E0, NU0, build, jitter_interior, lognormal_E, tetra10_jittered
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_system(O, kind, c, e, E, nu, bf, faces, tr, dofs, P=1e5):
+    rp, col, vals, F, rc = O.assemble(kind, c, e, E, nu, bf=bf, faces=faces, face_trac=tr)
+    M = O.penalty(rp, col, vals, F, dofs, None, P)
+    return rp, col, vals, F, rc, M
+
+
+def dense(rp, col, vals):
+    n = 3 * (len(rp) - 1)
+    A = np.zeros((n, n))
+    for r in range(len(rp) - 1):
+        for p in range(rp[r], rp[r + 1]):
+            A[3 * r:3 * r + 3, 3 * col[p]:3 * col[p] + 3] = vals[p]
+    return A
+
+
+def small_case(kind, seed):
+    if kind == L.TETRA10:
+        c, e = tetra10_jittered(3, seed=seed)
+    else:
+        c, e = L.make_rect(kind, (5, 4, 4))
+        c = jitter_interior(c, 0.25, 0.15, seed)
+    E = lognormal_E(len(e), 0.5, seed)
+    return c, e, E
+
+
+@pytest.mark.parametrize("kind", [L.TETRA, L.TETRA10, L.HEXA])
+def test_assembly_matches_oracle(oracle, kind):
+    say("assembly", kind)
+    c, e, E = small_case(kind, 11 + kind)
+    bf = (0.3, -0.2, -9.81 * 7.8e-9 * 1e6)
+    f, faces, tr, dofs = build(kind, c, e, E, bf=bf)
+    f.assemble()
+    rp, col, vals = f.get_mat()
+    F = f.get_sollicitation()
+    orp, ocol, ovals, oF, rc, M = oracle_system(oracle, kind, c, e, E, NU0, bf, faces, tr, dofs)
+    assert rc == 0
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)          # pattern: exact
+    scale = np.abs(ovals).max()
+    assert np.abs(vals - ovals).max() <= 1e-13 * scale
+    blk = np.abs(ovals).max(axis=(1, 2))
+    big = blk > 1e-3 * scale
+    assert (np.abs(vals - ovals).max(axis=(1, 2))[big] <= 1e-14 * 8 * blk[big]).all()
+    assert np.abs(F - oF).max() <= 1e-13 * np.abs(oF).max()
+    assert f.max_diag() == pytest.approx(M, rel=1e-14)
+    # exact symmetry of the stored full matrix
+    A = dense(rp, col, vals)
+    assert np.array_equal(A, A.T)
+
+
+def test_config0_solution_matches_reference_and_oracle(oracle):
+    say("config0 solve")
+    g = np.load(os.path.join(GOLDEN, "ref_tinyblock_pcg_tetra11.npz"))
+    c, e = L.make_rect(L.TETRA, (11, 11, 11))
+    f, faces, tr, dofs = build(L.TETRA, c, e, E0)
+    f.assemble()
+    rp, col, vals = f.get_mat()
+    F = f.get_sollicitation()
+    assert np.array_equal(rp, g["row_ptr"]) and np.array_equal(col, g["col"])
+    assert np.abs(vals - g["vals"]).max() <= 1e-13 * np.abs(g["vals"]).max()
+    assert np.abs(F - g["F"]).max() <= 1e-15
+    xs = np.linalg.solve(dense(g["row_ptr"], g["col"], g["vals"]), g["F"])
+    for prec in (L.JACOBI, L.BLOCK_JACOBI):
+        f.precond = prec
+        f.precision = L.FP64
+        f.set_result(np.zeros_like(F))
+        assert f.solve_system(rel_tol=1e-13), f.last_stats.status
+        u = f.get_result()
+        assert f.last_stats.rel_res <= 1e-13
+        assert np.linalg.norm(u - xs) <= 1e-10 * np.linalg.norm(xs)
+        # the reference's own block IC(0)-PCG result, to its stop test (max|r| <= 1e-10 max|F|)
+        assert np.linalg.norm(u - g["x"]) <= 1e-9 * np.linalg.norm(g["x"])
+    f.precision = L.MIXED
+    f.precond = L.JACOBI
+    f.set_result(np.zeros_like(F))
+    assert f.solve_system(rel_tol=1e-12)
+    u = f.get_result()
+    assert f.last_stats.refreshes >= 1
+    assert np.linalg.norm(u - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_abs_maxnorm_stop_matches_reference_semantics(oracle):
+    say("abs maxnorm")
+    c, e = L.make_rect(L.TETRA, (6, 5, 5))
+    f, *_ = build(L.TETRA, c, e, E0)
+    f.assemble()
+    F = f.get_sollicitation()
+    crit = 1e-9 * np.abs(F).max()
+    assert f.solve_system(crit=crit)
+    assert f.last_stats.max_abs_res <= crit
+    rp, col, vals = f.get_mat()
+    r = F - oracle.spmv(rp, col, vals, f.get_result())
+    assert np.abs(r).max() <= 1.001 * crit
+
+
+@pytest.mark.parametrize("kind", [L.TETRA10, L.HEXA])
+def test_solve_matches_oracle_pcg(oracle, kind):
+    say("solve parity", kind)
+    c, e, E = small_case(kind, 3)
+    f, faces, tr, dofs = build(kind, c, e, E, bf=(0.0, 0.0, -9.81 * 7.8e-9))
+    assert f.solve(rel_tol=1e-12)
+    u = f.get_result()
+    orp, ocol, ovals, oF, _, _ = oracle_system(oracle, kind, c, e, E, NU0, (0.0, 0.0, -9.81 * 7.8e-9), faces, tr, dofs)
+    xs = np.linalg.solve(dense(orp, ocol, ovals), oF)
+    assert np.linalg.norm(u - xs) <= 1e-10 * np.linalg.norm(xs)
+    ux = f.update_variables()
+    assert ux.shape == (len(c), 3)
+
+
+def test_block_system_known_answers():
+    say("bsr kat")
+    with open(os.path.join(GOLDEN, "ref_cg_kat.json")) as fh:
+        k = json.load(fh)
+    A = np.zeros((3, 3))
+    for i, row in enumerate(k["sym3"]["lower"]):
+        for j, v in enumerate(row):
+            A[i, j] = A[j, i] = v
+    s = L.BlockSystem([0, 1], [0], A.reshape(1, 3, 3), np.array(k["sym3"]["b"], float))
+    s.precond = L.BLOCK_JACOBI
+    assert s.solve_system(rel_tol=1e-15)
+    assert np.allclose(s.get_result(), k["sym3"]["x"], rtol=1e-12, atol=0)
+    # 10x10 dense system padded with an identity block row to 12 = 4 block rows
+    A10 = np.array(k["dense10"]["rows"], float)
+    A12 = np.eye(12)
+    A12[:10, :10] = A10
+    b12 = np.zeros(12)
+    b12[:10] = k["dense10"]["b"]
+    rp = np.arange(0, 17, 4)
+    col = np.tile(np.arange(4), 4)
+    vals = np.array([A12[3 * r:3 * r + 3, 3 * c:3 * c + 3] for r in range(4) for c in range(4)])
+    s = L.BlockSystem(rp, col, vals, b12)
+    s.precond = L.BLOCK_JACOBI
+    assert s.solve_system(rel_tol=1e-15)
+    x = s.get_result()[:10]
+    xr = np.array(k["dense10"]["x"])
+    assert (np.abs(x - xr) / (1e-20 + np.abs(x) + np.abs(xr))).max() < 1e-10
+
+
+def test_adjoint_gradient_matches_oracle_and_fd(oracle):
+    say("adjoint")
+    kind = L.HEXA
+    c, e = L.make_rect(kind, (6, 6, 6))
+    E = lognormal_E(len(e), 0.3, 5)
+    f, faces, tr, dofs = build(kind, c, e, E)
+    assert f.solve(rel_tol=1e-14)
+    u = f.get_result()
+    rng = np.random.default_rng(1)
+    top = np.where(np.abs(c[:, 2] - 1.0) < 1e-9)[0]
+    obs = rng.choice(top, size=max(3, len(top) // 10), replace=False)
+    odofs = (3 * obs[:, None] + np.arange(3)).ravel()
+    u_meas = u[odofs] * (1 + 0.05 * rng.standard_normal(len(odofs)))
+
+    def J_of(uu):
+        return 0.5 * np.sum((uu[odofs] - u_meas) ** 2)
+
+    dJdu = np.zeros_like(u)
+    dJdu[odofs] = u[odofs] - u_meas
+    g = f.adjoint(dJdu, rel_tol=1e-14)
+    # oracle adjoint: dense solve with the oracle's K (penalty scale held fixed)
+    orp, ocol, ovals, oF, _, _ = oracle_system(oracle, kind, c, e, E, NU0, (0, 0, 0), faces, tr, dofs)
+    Ad = dense(orp, ocol, ovals)
+    lam = np.linalg.solve(Ad, dJdu)
+    ug = np.linalg.solve(Ad, oF)
+    go = oracle.elem_grad_E(kind, e, c, E, NU0, ug, lam)
+    assert np.abs(g - go).max() <= 1e-9 * np.abs(go).max()
+    # element-gradient kernel alone, same u and lambda
+    assert np.abs(f.elem_grad(ug, lam) - go).max() <= 1e-12 * np.abs(go).max()
+    # central finite differences through the GPU solve (step 1e-4 E_e)
+    for el in rng.choice(len(e), size=6, replace=False):
+        Jp = []
+        for s in (+1, -1):
+            E2 = E.copy()
+            E2[el] *= 1 + s * 1e-4
+            f.set_material(E2, NU0)
+            f.set_result(u)
+            assert f.solve(rel_tol=1e-14)
+            Jp.append(J_of(f.get_result()))
+        fd = (Jp[0] - Jp[1]) / (2e-4 * E[el])
+        assert abs(fd - g[el]) <= 1e-5 * abs(g).max()
+
+
+def test_error_paths():
+    say("errors")
+    c, e = L.make_rect(L.TETRA, (3, 3, 3))
+    f = L.Formulation(L.TETRA, c, e)
+    with pytest.raises(L.FemGpuError) as ei:
+        f.assemble()
+    assert ei.value.status == L._capi.NOT_READY
+    with pytest.raises(L.FemGpuError) as ei:
+        f.set_material(np.full(len(e), np.nan), 0.3)
+    assert ei.value.status == L._capi.NAN
+    bad = e.copy()
+    bad[0, [1, 2]] = bad[0, [2, 1]]   # inverted element
+    g = L.Formulation(L.TETRA, c, bad)
+    g.set_material(1.0, 0.3)
+    with pytest.raises(L.FemGpuError) as ei:
+        g.assemble()
+    assert ei.value.status == L._capi.NEG_JACOBIAN
+    # no Dirichlet condition: singular K, iteration cap reported, nothing hangs
+    h = L.Formulation(L.TETRA, c, e)
+    h.set_material(1.0, 0.3, (0.0, 0.0, 1.0))
+    h.max_iter = 50
+    h.assemble()
+    assert not h.solve_system(rel_tol=1e-30)
+    assert h.last_status in (L._capi.MAX_ITER, L._capi.NOT_SPD)
+    with pytest.raises(L.FemGpuError):
+        h.add_constraint(10 ** 6, 0.0, 1e5)
+
+
+def test_bitwise_reproducible():
+    say("determinism")
+    c, e, E = small_case(L.HEXA, 9)
+    outs = []
+    for _ in range(2):
+        f, *_ = build(L.HEXA, c, e, E)
+        assert f.solve(rel_tol=1e-12)
+        outs.append((f.get_mat()[2], f.get_result()))
+        f.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])

@@ -190,6 +190,8 @@ FEMGPU_API int femgpu_batch_assemble(femgpu_batch *b);
 FEMGPU_API int femgpu_batch_solve(femgpu_batch *b, const femgpu_solve_opts *opts, femgpu_solve_stats *per_problem,
                                   double *total_ms);
 FEMGPU_API int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, size_t n);
+/* Device time (ms) of the last batched assembly. */
+FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
 FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);
 
 /* ---- diagnostics -------------------------------------------------------------------------- */
@@ -201,6 +203,14 @@ FEMGPU_API int femgpu_get_assemble_ms(femgpu_problem *p, double *ms);
 /* Average device time (ms) of one block-SpMV launch (precision FP64 or the FP32 copy), measured
  * with HIP events around reps back-to-back launches on the problem's stream. */
 FEMGPU_API int femgpu_time_spmv(femgpu_problem *p, int precision, int reps, double *avg_ms);
+
+/* ---- tuning (no reference counterpart: the reference has no device code) ------------------- */
+
+/* Times the block-SpMV variants (SELL slots per loop trip 1, 2, 3) on this matrix and keeps the
+ * fastest; *unroll receives the pick. */
+FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
+/* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */
+FEMGPU_API int femgpu_set_graph_replay(femgpu_problem *p, int enable);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,128 @@
+// femgpu_formulation.hpp -- C++ host facade over the femgpu C ABI with the call surface of
+// LMT++'s FormulationAncestor<double> (include/formulation/formulation_ancestor.h) for the
+// elasticity_iso formulation:
+//
+//     allocate_matrices()            formulation_ancestor.h:51   (pattern built at construction)
+//     shift()                        formulation_ancestor.h:57   (result kept as initial guess)
+//     assemble(mat, vec)             formulation_ancestor.h:63
+//     solve_system(crit) -> bool     formulation_ancestor.h:64
+//     solve(crit) -> bool            formulation_ancestor.h:49
+//     update_variables()             formulation_ancestor.h:179
+//     add_constraint(dof, penalty)   formulation_ancestor.h:197  (penalty form)
+//     get_sollicitation()            formulation_ancestor.h:236
+//     get_result()                   formulation_ancestor.h:237
+//     default_iterative_criterium    formulation_ancestor.h:244-257
+//
+// Like the reference, solve()/solve_system() return false when the system could not be solved
+// (formulation.h:1394,1530-1531); the femgpu status of the last call is in last_status().
+// Bad arguments and HIP failures throw femgpu::Error.
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "femgpu.h"
+
+namespace femgpu {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string &what) : std::runtime_error(what), status(s) {}
+};
+
+inline void check(int rc, const char *what) {
+    if (rc != FEMGPU_OK) throw Error(rc, std::string(what) + ": " + femgpu_last_error());
+}
+
+class FormulationGpu {
+public:
+    // Mesh in the reference layout: node coordinates (3 per node) and element connectivity in
+    // the reference local node order (make_rect output or any mesh reader).
+    FormulationGpu(femgpu_elem_kind kind, const std::vector<double> &coords, const std::vector<uint32_t> &conn)
+        : n_nodes_(uint32_t(coords.size() / 3)) {
+        const int npe = kind == FEMGPU_TETRA ? 4 : kind == FEMGPU_TETRA10 ? 10 : 8;
+        n_elem_ = uint32_t(conn.size() / npe);
+        femgpu_mesh m{kind, n_nodes_, coords.data(), n_elem_, conn.data()};
+        check(femgpu_create(&m, &h_), "femgpu_create");
+        femgpu_default_solve_opts(&opts_);
+    }
+    FormulationGpu(const FormulationGpu &) = delete;
+    FormulationGpu &operator=(const FormulationGpu &) = delete;
+    ~FormulationGpu() { femgpu_destroy(h_); }
+
+    // public knobs
+    double default_iterative_criterium = 1e-10;
+    femgpu_solve_opts &options() { return opts_; }
+
+    // material: elastic_modulus (global or per element), poisson_ratio, density * f_vol
+    void set_material(double E, double nu, const double body_force[3] = nullptr) {
+        femgpu_material mat{nu, E, nullptr, {0, 0, 0}};
+        if (body_force) for (int d = 0; d < 3; ++d) mat.body_force[d] = body_force[d];
+        check(femgpu_set_material(h_, &mat), "set_material");
+    }
+    void set_material(const std::vector<double> &E_elem, double nu, const double body_force[3] = nullptr) {
+        femgpu_material mat{nu, 0.0, E_elem.data(), {0, 0, 0}};
+        if (body_force) for (int d = 0; d < 3; ++d) mat.body_force[d] = body_force[d];
+        check(femgpu_set_material(h_, &mat), "set_material");
+    }
+    void set_traction(const std::vector<uint32_t> &faces, const std::vector<double> &trac) {
+        check(femgpu_set_traction(h_, uint32_t(trac.size() / 3), faces.data(), trac.data()), "set_traction");
+    }
+    void add_constraint(uint32_t dof, double value, double penalty) {
+        check(femgpu_add_constraint(h_, dof, value, penalty), "add_constraint");
+    }
+
+    void allocate_matrices(bool = true, bool = true) {}
+    void shift() {}
+    void assemble(bool assemble_mat = true, bool assemble_vec = true) {
+        check(femgpu_assemble(h_, assemble_mat, assemble_vec), "assemble");
+    }
+    // crit > 0: stop when max_i |r_i| <= crit (MatWithTinyBlocks.h:1115,1136); crit == 0 uses
+    // default_iterative_criterium.  rel_tol > 0 selects ||r|| <= rel_tol ||b|| instead.
+    bool solve_system(double crit = 0, double rel_tol = 0) {
+        femgpu_solve_opts o = opts_;
+        if (rel_tol > 0) { o.crit_kind = FEMGPU_REL_L2; o.tol = rel_tol; }
+        else { o.crit_kind = FEMGPU_ABS_MAXNORM; o.tol = crit > 0 ? crit : default_iterative_criterium; }
+        last_status_ = femgpu_solve_system(h_, &o, &stats_);
+        if (last_status_ == FEMGPU_BAD_INPUT || last_status_ == FEMGPU_HIP_ERROR || last_status_ == FEMGPU_NOT_READY)
+            check(last_status_, "solve_system");
+        return last_status_ == FEMGPU_OK;
+    }
+    bool solve(double crit = 0, double rel_tol = 0) {
+        assemble(true, true);
+        const bool ok = solve_system(crit, rel_tol);
+        if (ok) update_variables();
+        return ok;
+    }
+    // node.dep[d] = vectors[0][3 i + d] (formulation.h:2571-2576)
+    const std::vector<double> &update_variables() {
+        dep_.resize(3 * size_t(n_nodes_));
+        check(femgpu_get_result(h_, dep_.data(), dep_.size()), "get_result");
+        return dep_;
+    }
+    std::vector<double> get_result() const {
+        std::vector<double> u(3 * size_t(n_nodes_));
+        check(femgpu_get_result(h_, u.data(), u.size()), "get_result");
+        return u;
+    }
+    std::vector<double> get_sollicitation() const {
+        std::vector<double> F(3 * size_t(n_nodes_));
+        check(femgpu_get_sollicitation(h_, F.data(), F.size()), "get_sollicitation");
+        return F;
+    }
+    int last_status() const { return last_status_; }
+    const femgpu_solve_stats &last_stats() const { return stats_; }
+    femgpu_problem *handle() const { return h_; }
+    uint32_t nb_nodes() const { return n_nodes_; }
+    uint32_t nb_elems() const { return n_elem_; }
+
+private:
+    femgpu_problem *h_ = nullptr;
+    uint32_t n_nodes_ = 0, n_elem_ = 0;
+    femgpu_solve_opts opts_{};
+    femgpu_solve_stats stats_{};
+    int last_status_ = FEMGPU_OK;
+    std::vector<double> dep_;
+};
+
+}  // namespace femgpu

@@ -88,6 +88,9 @@ _SIGS = {
     "femgpu_last_error": (C.c_char_p, []),
     "femgpu_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_time_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "femgpu_autotune": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "femgpu_set_graph_replay": (C.c_int, [_P, C.c_int]),
+    "femgpu_batch_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 

@@ -1,0 +1,94 @@
+"""Batched uneven problems: many independent elasticity systems assembled and solved by one call
+(femgpu_batch_*).  Each sub-problem keeps its own material, tractions, constraints, CG scalars
+and stop test; chunks of iterations are replayed from a captured hipGraph."""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Batch:
+    def __init__(self, kind, meshes):
+        """meshes: list of (coords[n,3], conn[m,npe]) in the reference layout."""
+        lib = A.load()
+        self._lib = lib
+        self.kind = int(kind)
+        self._keep = []
+        arr = (A.Mesh * len(meshes))()
+        self.sizes = []
+        for i, (c, e) in enumerate(meshes):
+            c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3)
+            e = np.ascontiguousarray(e, dtype=np.uint32).reshape(-1, A.NPE[self.kind])
+            self._keep += [c, e]
+            arr[i] = A.Mesh(self.kind, c.shape[0], _ptr(c), e.shape[0], _ptr(e))
+            self.sizes.append((c.shape[0], e.shape[0]))
+        h = C.c_void_p()
+        A.check(lib.femgpu_batch_create(len(meshes), arr, C.byref(h)), "batch_create")
+        self._h = h
+        self.n = len(meshes)
+        self.last_stats = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.femgpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_material(self, i, E, nu, body_force=(0.0, 0.0, 0.0)):
+        E_arr = np.asarray(E, dtype=np.float64)
+        keep = None
+        if E_arr.ndim == 0:
+            mat = A.Material(float(nu), float(E_arr), None, (C.c_double * 3)(*body_force))
+        else:
+            keep = np.ascontiguousarray(E_arr.reshape(-1))
+            mat = A.Material(float(nu), 0.0, _ptr(keep), (C.c_double * 3)(*body_force))
+        A.check(self._lib.femgpu_batch_set_material(self._h, i, C.byref(mat)), "batch_set_material")
+        del keep
+
+    def set_traction(self, i, faces, traction):
+        faces = np.ascontiguousarray(faces, dtype=np.uint32)
+        tr = np.ascontiguousarray(np.broadcast_to(np.asarray(traction, dtype=np.float64), (faces.shape[0], 3)))
+        A.check(self._lib.femgpu_batch_set_traction(self._h, i, faces.shape[0], _ptr(faces), _ptr(tr)), "batch_traction")
+
+    def add_constraints(self, i, dofs, values=None, penalty=1e5):
+        dofs = np.ascontiguousarray(dofs, dtype=np.uint32)
+        vals = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        A.check(self._lib.femgpu_batch_add_constraints(self._h, i, dofs.shape[0], _ptr(dofs),
+                                                       None if vals is None else _ptr(vals), float(penalty)),
+                "batch_add_constraints")
+
+    def assemble(self):
+        A.check(self._lib.femgpu_batch_assemble(self._h), "batch_assemble")
+
+    def assemble_ms(self):
+        v = C.c_double()
+        A.check(self._lib.femgpu_batch_get_assemble_ms(self._h, C.byref(v)), "batch_assemble_ms")
+        return v.value
+
+    def solve(self, rel_tol=1e-10, precision=A.MIXED, precond=A.JACOBI, max_iter=100000, chunk=32):
+        o = A.SolveOpts()
+        self._lib.femgpu_default_solve_opts(C.byref(o))
+        o.precision, o.precond, o.crit_kind, o.tol, o.max_iter, o.chunk = (
+            int(precision), int(precond), A.REL_L2, float(rel_tol), int(max_iter), int(chunk))
+        st = (A.SolveStats * self.n)()
+        ms = C.c_double()
+        rc = self._lib.femgpu_batch_solve(self._h, C.byref(o), st, C.byref(ms))
+        self.last_stats = list(st)
+        if rc in (A.BAD_INPUT, A.HIP_ERROR, A.NOT_READY):
+            A.check(rc, "batch_solve")
+        return rc == A.OK, ms.value
+
+    def get_result(self, i):
+        u = np.zeros(3 * self.sizes[i][0])
+        A.check(self._lib.femgpu_batch_get_result(self._h, i, _ptr(u), u.size), "batch_get_result")
+        return u

@@ -5,7 +5,7 @@
 // Layout of K in HBM: 9 planes of nnzb doubles (plane q = block entry (q/3, q%3)), plane
 // stride P.ps; a block's position p is its CSR slot.  Plane-major storage makes every
 // SpMV load a contiguous run across the lanes of a row group.
-#include "femgpu_internal.hpp"
+#include "device_common.hpp"
 
 namespace femgpu {
 
@@ -159,10 +159,10 @@ __device__ inline void elem_block(const ElemLds<K> &L, int a, int b, double lam,
 template <int K>
 __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ order, uint32_t count,
                                                   const uint32_t *__restrict__ conn, const double *__restrict__ coords,
-                                                  const double *__restrict__ E, double nu, double bf0, double bf1,
-                                                  double bf2, const uint32_t *__restrict__ slot, double *__restrict__ Kv,
-                                                  uint64_t ps, double *__restrict__ F, int do_mat, int do_vec,
-                                                  int *flags) {
+                                                  const double *__restrict__ E, const uint32_t *__restrict__ elem_sub,
+                                                  const SubMaterial *__restrict__ mat, const uint32_t *__restrict__ slot,
+                                                  double *__restrict__ Kv, uint64_t ps, double *__restrict__ F, int do_mat,
+                                                  int do_vec, int *flags) {
     constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP, TEAM = ET<K>::TEAM, TPW = 256 / TEAM;
     __shared__ ElemLds<K> lds[TPW];
     const int team = threadIdx.x / TEAM, t = threadIdx.x % TEAM;
@@ -172,20 +172,32 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
     ElemLds<K> &L = lds[team];
     elem_geometry<K>(L, t, valid, e, conn, coords, flags);
     if (!valid) return;
-    if (do_mat && t < NPE * NPE) {
+    const SubMaterial m = mat[elem_sub[e]];
+    // Only the local upper triangle is evaluated (formal_lf/calculate_matrix.py:69-97); block (a,b),
+    // a < b, is added at (a,b) and, transposed, at (b,a), so the stored matrix is exactly symmetric.
+    if (do_mat && t < NPE * NPE && t / NPE <= t % NPE) {
         const int a = t / NPE, b = t % NPE;
         double lam, mu;
-        lame(E[e], nu, lam, mu);
+        lame(E[e], m.nu, lam, mu);
         double k[9];
         elem_block<K>(L, a, b, lam, mu, k);
         const uint64_t p = slot[(size_t)e * NPE * NPE + t];
+        if (a == b) {
+            k[3] = k[1]; k[6] = k[2]; k[7] = k[5];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) Kv[c * ps + p] += k[c];
+            for (int c = 0; c < 9; ++c) Kv[c * ps + p] += k[c];
+        } else {
+            const uint64_t pt = slot[(size_t)e * NPE * NPE + b * NPE + a];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Kv[c * ps + p] += k[c];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Kv[c * ps + pt] += k[3 * (c % 3) + c / 3];
+        }
     }
     if (do_vec && t < NPE * 3) {
         // F_a = sum_q w_q detJ_q N_a(q) (density f_vol) (formulation_elasticity_iso.py:52)
         const int a = t / 3, d = t % 3;
-        const double f = d == 0 ? bf0 : d == 1 ? bf1 : bf2;
+        const double f = d == 0 ? m.bf0 : d == 1 ? m.bf1 : m.bf2;
         double s = 0;
         for (int q = 0; q < NGP; ++q) s += L.wd[q] * L.N[q][a] * f;
         F[3 * (size_t)conn[(size_t)e * NPE + a] + d] += s;
@@ -270,26 +282,26 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
 }
 
 template <int K>
-static hipError_t assemble_kind(Problem &P, int do_mat, int do_vec, int *d_flags) {
+static hipError_t assemble_kind(Problem &P, int do_mat, int do_vec) {
     constexpr int TPW = 256 / ET<K>::TEAM;
     for (uint32_t c = 0; c < P.ecol.n_colors(); ++c) {
         const uint32_t b = P.ecol.color_ptr[c], n = P.ecol.color_ptr[c + 1] - b;
         if (!n) continue;
         hipLaunchKernelGGL(k_assemble<K>, dim3((n + TPW - 1) / TPW), dim3(256), 0, P.stream, P.d_order + b, n, P.d_conn,
-                           P.d_coords, P.d_E, P.nu, P.bf[0], P.bf[1], P.bf[2], P.d_slot, P.d_K, P.ps, P.d_F, do_mat,
-                           do_vec, d_flags);
+                           P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.d_slot, P.d_K, P.ps, P.d_F, do_mat, do_vec,
+                           P.d_flags);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_assemble(Problem &P, int do_mat, int do_vec, int *d_flags) {
+hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
     hipError_t e = hipSuccess;
     if (do_mat) e = launch_fill(P.d_K, 9 * P.ps, 0.0, P.stream);
     if (e == hipSuccess && do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
     if (e != hipSuccess) return e;
-    if (P.kind == TETRA) e = assemble_kind<TETRA>(P, do_mat, do_vec, d_flags);
-    else if (P.kind == TETRA10) e = assemble_kind<TETRA10>(P, do_mat, do_vec, d_flags);
-    else e = assemble_kind<HEXA>(P, do_mat, do_vec, d_flags);
+    if (P.kind == TETRA) e = assemble_kind<TETRA>(P, do_mat, do_vec);
+    else if (P.kind == TETRA10) e = assemble_kind<TETRA10>(P, do_mat, do_vec);
+    else e = assemble_kind<HEXA>(P, do_mat, do_vec);
     if (e != hipSuccess || !do_vec || !P.n_faces) return e;
     const int fk = face_kind_of(P.kind);
     for (uint32_t c = 0; c < P.fcol.n_colors(); ++c) {
@@ -308,44 +320,35 @@ hipError_t launch_assemble(Problem &P, int do_mat, int do_vec, int *d_flags) {
 // entries, taken after the element loop and before any constraint; then, per constrained DOF in
 // insertion order, K(p,p) += M*P and F(p) += M*P*value.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_max_diag(const uint32_t *__restrict__ diag, uint32_t n_nodes,
-                                                  const double *__restrict__ Kv, uint64_t ps, double *partials,
-                                                  Scalars *sc) {
-    double m = 0;
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_nodes; r += gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(256) void k_max_diag(const uint32_t *__restrict__ diag, const double *__restrict__ Kv,
+                                                  uint64_t ps, WgTables wt, ProbScalars *S, double *partials) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB) return;
+    double m[1] = {0.0};
+    const uint32_t r = wt.row0[L] + threadIdx.x;
+    if (r < wt.row1[L]) {
         const uint64_t p = diag[r];
-        m = fmax(m, fmax(fabs(Kv[0 * ps + p]), fmax(fabs(Kv[4 * ps + p]), fabs(Kv[8 * ps + p]))));
+        m[0] = fmax(fabs(Kv[p]), fmax(fabs(Kv[4 * ps + p]), fabs(Kv[8 * ps + p])));
     }
-    // deterministic two-level max
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-    __shared__ double sm[4];
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
-        __threadfence();
-        last = atomicAdd(&sc->counter[CTR_MAXDIAG], 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        double t = 0;
-        const volatile double *vp = partials;
-        for (unsigned i = 0; i < gridDim.x; ++i) t = fmax(t, vp[i]);
-        sc->max_diag = t;
-        sc->counter[CTR_MAXDIAG] = 0;
-    }
+    wg_partial<1, 1u>(m, partials, gridDim.x, L);
+}
+
+__global__ __launch_bounds__(256) void k_fin_max_diag(ProbScalars *S, const double *__restrict__ partials, unsigned stride,
+                                                      const uint32_t *__restrict__ sub_wg0,
+                                                      const uint32_t *__restrict__ sub_nwg) {
+    double o[1];
+    wg_combine<1, 1u>(partials, stride, sub_wg0[blockIdx.x], sub_nwg[blockIdx.x], o);
+    if (threadIdx.x == 0) S[blockIdx.x].max_diag = o[0];
 }
 
 __global__ void k_constraints(uint32_t n_cdof, const uint32_t *__restrict__ cdof, const uint32_t *__restrict__ cptr,
-                              const double *__restrict__ cval, const double *__restrict__ cpen,
-                              const uint32_t *__restrict__ diag, double *Kv, uint64_t ps, double *F, const Scalars *sc,
-                              int do_mat, int do_vec) {
+                              const uint32_t *__restrict__ csub, const double *__restrict__ cval,
+                              const double *__restrict__ cpen, const uint32_t *__restrict__ diag, double *Kv, uint64_t ps,
+                              double *F, const ProbScalars *S, int do_mat, int do_vec) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_cdof) return;
-    const double M = sc->max_diag;
+    const double M = S[csub[i]].max_diag;
     const uint32_t dof = cdof[i], r = dof / 3, d = dof % 3;
     double *kp = &Kv[(uint64_t)(4 * d) * ps + diag[r]];
     for (uint32_t k = cptr[i]; k < cptr[i + 1]; ++k) {
@@ -356,10 +359,15 @@ __global__ void k_constraints(uint32_t n_cdof, const uint32_t *__restrict__ cdof
 }
 
 hipError_t launch_constraints(Problem &P, int do_mat, int do_vec) {
-    if (do_mat) hipLaunchKernelGGL(k_max_diag, dim3(256), dim3(256), 0, P.stream, P.d_diag, P.n_nodes, P.d_K, P.ps, P.d_partials, P.d_sc);
+    const WgTables wt{P.d_wg_row0, P.d_wg_row1, P.d_wg_sub, P.d_sub_wg0, P.d_sub_nwg};
+    if (do_mat) {
+        hipLaunchKernelGGL(k_max_diag, dim3(P.n_wg), dim3(256), 0, P.stream, P.d_diag, P.d_K, P.ps, wt, P.d_ps, P.d_partials);
+        hipLaunchKernelGGL(k_fin_max_diag, dim3(P.n_sub), dim3(256), 0, P.stream, P.d_ps, P.d_partials, P.n_wg, P.d_sub_wg0,
+                           P.d_sub_nwg);
+    }
     if (!P.n_cdof) return hipGetLastError();
     hipLaunchKernelGGL(k_constraints, dim3((P.n_cdof + 255) / 256), dim3(256), 0, P.stream, P.n_cdof, P.d_cdof, P.d_cptr,
-                       P.d_cval, P.d_cpen, P.d_diag, P.d_K, P.ps, P.d_F, P.d_sc, do_mat, do_vec);
+                       P.d_csub, P.d_cval, P.d_cpen, P.d_diag, P.d_K, P.ps, P.d_F, P.d_ps, do_mat, do_vec);
     return hipGetLastError();
 }
 
@@ -378,9 +386,10 @@ hipError_t launch_to_fp32(Problem &P) {
 // ---------------------------------------------------------------------------------------------
 template <int K>
 __global__ __launch_bounds__(256) void k_elem_grad(uint32_t n_elem, const uint32_t *__restrict__ conn,
-                                                   const double *__restrict__ coords, double nu,
-                                                   const double *__restrict__ u, const double *__restrict__ lam,
-                                                   double *__restrict__ g, int *flags) {
+                                                   const double *__restrict__ coords,
+                                                   const uint32_t *__restrict__ elem_sub,
+                                                   const SubMaterial *__restrict__ mat, const double *__restrict__ u,
+                                                   const double *__restrict__ lam, double *__restrict__ g, int *flags) {
     constexpr int NPE = ET<K>::NPE, TEAM = ET<K>::TEAM, TPW = 256 / TEAM;
     __shared__ ElemLds<K> lds[TPW];
     __shared__ double red[256];
@@ -393,7 +402,7 @@ __global__ __launch_bounds__(256) void k_elem_grad(uint32_t n_elem, const uint32
     if (valid && t < NPE * NPE) {
         const int a = t / NPE, b = t % NPE;
         double l1, m1;
-        lame(1.0, nu, l1, m1);
+        lame(1.0, mat[elem_sub[e]].nu, l1, m1);
         double k[9];
         elem_block<K>(L, a, b, l1, m1, k);
         const uint32_t na = conn[(size_t)e * NPE + a], nb = conn[(size_t)e * NPE + b];
@@ -412,10 +421,10 @@ __global__ __launch_bounds__(256) void k_elem_grad(uint32_t n_elem, const uint32
 }
 
 hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, double *d_g) {
-    int *flags = reinterpret_cast<int *>(&P.d_sc->pad);
-    if (P.kind == TETRA) hipLaunchKernelGGL(k_elem_grad<TETRA>, dim3((P.n_elem + 15) / 16), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.nu, d_u, d_lam, d_g, flags);
-    else if (P.kind == TETRA10) hipLaunchKernelGGL(k_elem_grad<TETRA10>, dim3((P.n_elem + 1) / 2), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.nu, d_u, d_lam, d_g, flags);
-    else hipLaunchKernelGGL(k_elem_grad<HEXA>, dim3((P.n_elem + 3) / 4), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.nu, d_u, d_lam, d_g, flags);
+    int *flags = P.d_flags;
+    if (P.kind == TETRA) hipLaunchKernelGGL(k_elem_grad<TETRA>, dim3((P.n_elem + 15) / 16), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_elem_sub, P.d_mat, d_u, d_lam, d_g, flags);
+    else if (P.kind == TETRA10) hipLaunchKernelGGL(k_elem_grad<TETRA10>, dim3((P.n_elem + 1) / 2), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_elem_sub, P.d_mat, d_u, d_lam, d_g, flags);
+    else hipLaunchKernelGGL(k_elem_grad<HEXA>, dim3((P.n_elem + 3) / 4), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_elem_sub, P.d_mat, d_u, d_lam, d_g, flags);
     return hipGetLastError();
 }
 

@@ -1,9 +1,9 @@
 // C ABI of the MI355X assemble + solve path (declarations and reference citations in
-// include/femgpu.h).
+// include/femgpu.h).  Single problems and batches share one implementation: a Problem with
+// n_sub sub-problems (n_sub = 1 for femgpu_create).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <map>
 #include <new>
 
 #include "../../include/femgpu.h"
@@ -20,6 +20,12 @@ using namespace femgpu;
 
 struct femgpu_problem {
     Problem P;
+    std::vector<std::vector<uint32_t>> sub_faces;  // per sub-problem, local node indices
+    std::vector<std::vector<double>> sub_trac;
+    bool faces_dirty = false;
+};
+struct femgpu_batch {
+    femgpu_problem H;
 };
 
 namespace {
@@ -29,9 +35,9 @@ int fail(int status, const std::string &msg) {
     return status;
 }
 
-#define HIPCHK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
+#define HIPCHK(expr)                                                                                              \
+    do {                                                                                                          \
+        hipError_t e_ = (expr);                                                                                   \
         if (e_ != hipSuccess) return fail(FEMGPU_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
@@ -52,29 +58,67 @@ void dfree(T *&p) {
 }
 
 void free_problem(Problem &P) {
+    destroy_graph(P);
     dfree(P.d_coords); dfree(P.d_E); dfree(P.d_conn); dfree(P.d_rp); dfree(P.d_col); dfree(P.d_diag);
-    dfree(P.d_slot); dfree(P.d_order); dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
-    dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac); dfree(P.d_cdof); dfree(P.d_cptr); dfree(P.d_cval);
-    dfree(P.d_cpen); dfree(P.d_r); dfree(P.d_p); dfree(P.d_q); dfree(P.d_Minv); dfree(P.d_r32); dfree(P.d_p32);
-    dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32); dfree(P.d_partials); dfree(P.d_sc); dfree(P.d_wgrows);
-    dfree(P.d_aux);
-    if (P.h_sc) (void)hipHostFree(P.h_sc);
-    P.h_sc = nullptr;
+    dfree(P.d_slot); dfree(P.d_order); dfree(P.d_sptr); dfree(P.d_elem_sub); dfree(P.d_mat);
+    dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
+    dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
+    dfree(P.d_cdof); dfree(P.d_cptr); dfree(P.d_csub); dfree(P.d_cval); dfree(P.d_cpen);
+    dfree(P.d_r); dfree(P.d_p); dfree(P.d_q); dfree(P.d_Minv); dfree(P.d_aux);
+    dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32);
+    dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
+    dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
+    if (P.h_ps) (void)hipHostFree(P.h_ps);
+    if (P.h_flags) (void)hipHostFree(P.h_flags);
+    P.h_ps = nullptr;
+    P.h_flags = nullptr;
     for (auto &e : P.ev)
         if (e) (void)hipEventDestroy(e);
     if (P.stream) (void)hipStreamDestroy(P.stream);
     P.stream = nullptr;
 }
 
+// Workgroup tables: every sub-problem's rows split into workgroups of up to 256 rows; the grid is
+// rounded up to a multiple of 8 with idle workgroups (sub = NO_SUB).
+void build_wg_tables(Problem &P, std::vector<uint32_t> &row0, std::vector<uint32_t> &row1, std::vector<uint32_t> &sub,
+                     std::vector<uint32_t> &swg0, std::vector<uint32_t> &snwg) {
+    swg0.assign(P.n_sub, 0);
+    snwg.assign(P.n_sub, 0);
+    for (uint32_t s = 0; s < P.n_sub; ++s) {
+        swg0[s] = (uint32_t)row0.size();
+        for (uint32_t r = P.node_off[s]; r < P.node_off[s + 1]; r += ROWS_PER_WG) {
+            row0.push_back(r);
+            row1.push_back(std::min(P.node_off[s + 1], r + ROWS_PER_WG));
+            sub.push_back(s);
+        }
+        snwg[s] = (uint32_t)row0.size() - swg0[s];
+    }
+    while (row0.size() % 8 || row0.empty()) { row0.push_back(0); row1.push_back(0); sub.push_back(NO_SUB); }
+    P.n_wg = (uint32_t)row0.size();
+}
+
 // Device buffers common to mesh problems and plain block systems.
 int alloc_system(Problem &P) {
     const size_t n = 3 * (size_t)P.n_nodes;
-    P.ps = (P.nnzb + 63) / 64 * 64;
+    P.ps = (P.pat.n_sell + 63) / 64 * 64;
     HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
     for (auto &e : P.ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(upload(&P.d_rp, P.pat.row_ptr.data(), P.pat.row_ptr.size()));
-    HIPCHK(upload(&P.d_col, P.pat.col.data(), P.pat.col.size()));
+    {
+        std::vector<uint32_t> cs = sell_columns(P.pat);
+        HIPCHK(upload(&P.d_col, cs.data(), cs.size()));
+    }
+    HIPCHK(upload(&P.d_sptr, P.pat.sell_ptr.data(), P.pat.sell_ptr.size()));
     HIPCHK(upload(&P.d_diag, P.pat.diag.data(), P.pat.diag.size()));
+    {
+        std::vector<uint32_t> row0, row1, sub, swg0, snwg;
+        build_wg_tables(P, row0, row1, sub, swg0, snwg);
+        HIPCHK(upload(&P.d_wg_row0, row0.data(), row0.size()));
+        HIPCHK(upload(&P.d_wg_row1, row1.data(), row1.size()));
+        HIPCHK(upload(&P.d_wg_sub, sub.data(), sub.size()));
+        HIPCHK(upload(&P.d_sub_wg0, swg0.data(), swg0.size()));
+        HIPCHK(upload(&P.d_sub_nwg, snwg.data(), snwg.size()));
+    }
     HIPCHK(dalloc(&P.d_K, 9 * P.ps));
     HIPCHK(dalloc(&P.d_F, n));
     HIPCHK(dalloc(&P.d_u, n));
@@ -87,11 +131,15 @@ int alloc_system(Problem &P) {
     HIPCHK(dalloc(&P.d_p32, n));
     HIPCHK(dalloc(&P.d_q32, n));
     HIPCHK(dalloc(&P.d_x32, n));
-    HIPCHK(dalloc(&P.d_partials, 8 * 4096));
-    HIPCHK(dalloc(&P.d_sc, 1));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&P.h_sc), sizeof(Scalars), hipHostMallocDefault));
-    std::memset(P.h_sc, 0, sizeof(Scalars));
-    HIPCHK(hipMemcpy(P.d_sc, P.h_sc, sizeof(Scalars), hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&P.d_partials, 3 * (size_t)P.n_wg));
+    HIPCHK(dalloc(&P.d_ps, P.n_sub));
+    HIPCHK(dalloc(&P.d_flags, 4));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&P.h_ps), sizeof(ProbScalars) * P.n_sub, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&P.h_flags), sizeof(int) * 4, hipHostMallocDefault));
+    std::memset(P.h_ps, 0, sizeof(ProbScalars) * P.n_sub);
+    HIPCHK(hipMemcpy(P.d_ps, P.h_ps, sizeof(ProbScalars) * P.n_sub, hipMemcpyHostToDevice));
+    P.max_diag.assign(P.n_sub, 0.0);
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t *>(P.d_flags), 4, 0u, P.stream));
     HIPCHK(launch_fill(P.d_u, n, 0.0, P.stream));
     HIPCHK(launch_fill(P.d_p, n, 0.0, P.stream));
     HIPCHK(launch_fill(P.d_K, 9 * P.ps, 0.0, P.stream));
@@ -100,20 +148,155 @@ int alloc_system(Problem &P) {
     return FEMGPU_OK;
 }
 
+// n meshes of one element kind concatenated into one device system.
+int build_problem(femgpu_problem &H, uint32_t n_sub, const femgpu_mesh *meshes) {
+    Problem &P = H.P;
+    if (!n_sub || !meshes) return fail(FEMGPU_BAD_INPUT, "create: no mesh");
+    const int kind = meshes[0].elem_kind;
+    if (kind < 0 || kind > 2) return fail(FEMGPU_BAD_INPUT, "create: unknown element kind");
+    const int npe = nodes_per_elem(kind);
+    P.kind = kind;
+    P.npe = npe;
+    P.n_sub = n_sub;
+    P.node_off.assign(n_sub + 1, 0);
+    P.elem_off.assign(n_sub + 1, 0);
+    for (uint32_t s = 0; s < n_sub; ++s) {
+        const femgpu_mesh &m = meshes[s];
+        if (m.elem_kind != kind || !m.coords || !m.conn || !m.n_nodes || !m.n_elem)
+            return fail(FEMGPU_BAD_INPUT, "create: bad mesh " + std::to_string(s));
+        for (size_t k = 0; k < (size_t)m.n_elem * npe; ++k)
+            if (m.conn[k] >= m.n_nodes) return fail(FEMGPU_BAD_INPUT, "create: node index out of range");
+        for (size_t k = 0; k < 3 * (size_t)m.n_nodes; ++k)
+            if (!std::isfinite(m.coords[k])) return fail(FEMGPU_NAN, "create: non-finite coordinate");
+        if ((uint64_t)P.node_off[s] + m.n_nodes > 0x7fffffffull || (uint64_t)P.elem_off[s] + m.n_elem > 0x7fffffffull)
+            return fail(FEMGPU_BAD_INPUT, "create: mesh too large");
+        P.node_off[s + 1] = P.node_off[s] + m.n_nodes;
+        P.elem_off[s + 1] = P.elem_off[s] + m.n_elem;
+    }
+    P.n_nodes = P.node_off[n_sub];
+    P.n_elem = P.elem_off[n_sub];
+    std::vector<double> coords(3 * (size_t)P.n_nodes);
+    std::vector<uint32_t> conn((size_t)P.n_elem * npe), esub(P.n_elem);
+    for (uint32_t s = 0; s < n_sub; ++s) {
+        const femgpu_mesh &m = meshes[s];
+        std::memcpy(coords.data() + 3 * (size_t)P.node_off[s], m.coords, 24 * (size_t)m.n_nodes);
+        uint32_t *c = conn.data() + (size_t)P.elem_off[s] * npe;
+        for (size_t k = 0; k < (size_t)m.n_elem * npe; ++k) c[k] = m.conn[k] + P.node_off[s];
+        std::fill(esub.begin() + P.elem_off[s], esub.begin() + P.elem_off[s + 1], s);
+    }
+    (void)hipGetDevice(&P.device);
+    P.pat = build_pattern(P.n_nodes, P.n_elem, npe, conn.data());
+    P.nnzb = P.pat.col.size();
+    P.ecol = color_elements(P.n_nodes, P.n_elem, npe, conn.data());
+    int rc = alloc_system(P);
+    if (rc) return rc;
+    std::vector<uint32_t> slot = build_slot_map(P.pat, P.n_elem, npe, conn.data());
+    HIPCHK(upload(&P.d_slot, slot.data(), slot.size()));
+    HIPCHK(upload(&P.d_coords, coords.data(), coords.size()));
+    HIPCHK(upload(&P.d_conn, conn.data(), conn.size()));
+    HIPCHK(upload(&P.d_order, P.ecol.order.data(), P.ecol.order.size()));
+    HIPCHK(upload(&P.d_elem_sub, esub.data(), esub.size()));
+    HIPCHK(dalloc(&P.d_E, P.n_elem));
+    P.mat_h.assign(n_sub, SubMaterial{0.3, 0, 0, 0});
+    P.material_set.assign(n_sub, 0);
+    HIPCHK(upload(&P.d_mat, P.mat_h.data(), P.mat_h.size()));
+    H.sub_faces.assign(n_sub, {});
+    H.sub_trac.assign(n_sub, {});
+    return FEMGPU_OK;
+}
+
+int set_material(femgpu_problem &H, uint32_t s, const femgpu_material *m) {
+    Problem &P = H.P;
+    if (!m || P.from_bsr || s >= P.n_sub) return fail(FEMGPU_BAD_INPUT, "set_material: bad input");
+    if (!(m->nu > -1.0 && m->nu < 0.5)) return fail(FEMGPU_BAD_INPUT, "set_material: nu out of (-1, 0.5)");
+    const uint32_t ne = P.elem_off[s + 1] - P.elem_off[s];
+    for (int d = 0; d < 3; ++d)
+        if (!std::isfinite(m->body_force[d])) return fail(FEMGPU_NAN, "set_material: non-finite body force");
+    if (m->E_elem) {
+        for (uint32_t e = 0; e < ne; ++e)
+            if (!std::isfinite(m->E_elem[e])) return fail(FEMGPU_NAN, "set_material: non-finite E");
+        HIPCHK(hipMemcpy(P.d_E + P.elem_off[s], m->E_elem, (size_t)ne * 8, hipMemcpyHostToDevice));
+    } else {
+        if (!std::isfinite(m->E)) return fail(FEMGPU_NAN, "set_material: non-finite E");
+        HIPCHK(launch_fill(P.d_E + P.elem_off[s], ne, m->E, P.stream));
+        HIPCHK(hipStreamSynchronize(P.stream));
+    }
+    P.mat_h[s] = SubMaterial{m->nu, m->body_force[0], m->body_force[1], m->body_force[2]};
+    HIPCHK(hipMemcpy(P.d_mat, P.mat_h.data(), P.mat_h.size() * sizeof(SubMaterial), hipMemcpyHostToDevice));
+    P.material_set[s] = 1;
+    return FEMGPU_OK;
+}
+
+int set_traction(femgpu_problem &H, uint32_t s, uint32_t n_faces, const uint32_t *faces, const double *trac) {
+    Problem &P = H.P;
+    if (P.from_bsr || s >= P.n_sub || (n_faces && (!faces || !trac))) return fail(FEMGPU_BAD_INPUT, "set_traction: bad input");
+    const int npf = nodes_per_face(face_kind_of(P.kind));
+    const uint32_t nn = P.node_off[s + 1] - P.node_off[s];
+    for (size_t k = 0; k < (size_t)n_faces * npf; ++k)
+        if (faces[k] >= nn) return fail(FEMGPU_BAD_INPUT, "set_traction: node index out of range");
+    for (size_t k = 0; k < 3 * (size_t)n_faces; ++k)
+        if (!std::isfinite(trac[k])) return fail(FEMGPU_NAN, "set_traction: non-finite traction");
+    H.sub_faces[s].assign(faces, faces + (size_t)n_faces * npf);
+    H.sub_trac[s].assign(trac, trac + 3 * (size_t)n_faces);
+    H.faces_dirty = true;
+    return FEMGPU_OK;
+}
+
+int upload_faces(femgpu_problem &H) {
+    Problem &P = H.P;
+    if (!H.faces_dirty) return FEMGPU_OK;
+    const int npf = nodes_per_face(face_kind_of(P.kind));
+    std::vector<uint32_t> faces;
+    std::vector<double> trac;
+    for (uint32_t s = 0; s < P.n_sub; ++s) {
+        for (uint32_t v : H.sub_faces[s]) faces.push_back(v + P.node_off[s]);
+        trac.insert(trac.end(), H.sub_trac[s].begin(), H.sub_trac[s].end());
+    }
+    dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
+    P.n_faces = (uint32_t)(trac.size() / 3);
+    P.fcol = Coloring{};
+    if (P.n_faces) {
+        P.fcol = color_elements(P.n_nodes, P.n_faces, npf, faces.data());
+        HIPCHK(upload(&P.d_faces, faces.data(), faces.size()));
+        HIPCHK(upload(&P.d_forder, P.fcol.order.data(), P.fcol.order.size()));
+        HIPCHK(upload(&P.d_trac, trac.data(), trac.size()));
+    }
+    H.faces_dirty = false;
+    return FEMGPU_OK;
+}
+
+int add_constraints(femgpu_problem &H, uint32_t s, uint32_t n, const uint32_t *dofs, const double *values,
+                    double penalty) {
+    Problem &P = H.P;
+    if (s >= P.n_sub || (n && !dofs)) return fail(FEMGPU_BAD_INPUT, "add_constraints: bad input");
+    if (!(penalty > 0) || !std::isfinite(penalty))
+        return fail(FEMGPU_BAD_INPUT, "add_constraints: only the penalty form (penalty > 0) keeps K SPD");
+    const uint32_t nd = 3 * (P.node_off[s + 1] - P.node_off[s]);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (dofs[k] >= nd) return fail(FEMGPU_BAD_INPUT, "add_constraints: dof out of range");
+        if (values && !std::isfinite(values[k])) return fail(FEMGPU_NAN, "add_constraints: non-finite value");
+    }
+    for (uint32_t k = 0; k < n; ++k) P.cons.push_back({dofs[k] + 3 * P.node_off[s], values ? values[k] : 0.0, penalty});
+    P.cons_dirty = true;
+    return FEMGPU_OK;
+}
+
 int upload_constraints(Problem &P) {
     if (!P.cons_dirty) return FEMGPU_OK;
-    dfree(P.d_cdof); dfree(P.d_cptr); dfree(P.d_cval); dfree(P.d_cpen);
+    dfree(P.d_cdof); dfree(P.d_cptr); dfree(P.d_csub); dfree(P.d_cval); dfree(P.d_cpen);
     // group by DOF, keeping insertion order inside a group
     std::vector<size_t> idx(P.cons.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
     std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return P.cons[a].dof < P.cons[b].dof; });
-    std::vector<uint32_t> dof, ptr(1, 0);
+    std::vector<uint32_t> dof, ptr(1, 0), sub;
     std::vector<double> val, pen;
     for (size_t k = 0; k < idx.size(); ++k) {
         const auto &c = P.cons[idx[k]];
         if (dof.empty() || dof.back() != c.dof) {
             if (!dof.empty()) ptr.push_back((uint32_t)val.size());
             dof.push_back(c.dof);
+            const uint32_t node = c.dof / 3;
+            sub.push_back(uint32_t(std::upper_bound(P.node_off.begin(), P.node_off.end(), node) - P.node_off.begin()) - 1);
         }
         val.push_back(c.val);
         pen.push_back(c.pen);
@@ -122,9 +305,40 @@ int upload_constraints(Problem &P) {
     P.n_cdof = (uint32_t)dof.size();
     HIPCHK(upload(&P.d_cdof, dof.data(), dof.size()));
     HIPCHK(upload(&P.d_cptr, ptr.data(), ptr.size()));
+    HIPCHK(upload(&P.d_csub, sub.data(), sub.size()));
     HIPCHK(upload(&P.d_cval, val.data(), val.size()));
     HIPCHK(upload(&P.d_cpen, pen.data(), pen.size()));
     P.cons_dirty = false;
+    return FEMGPU_OK;
+}
+
+int assemble(femgpu_problem &H, int do_mat, int do_vec) {
+    Problem &P = H.P;
+    if (P.from_bsr) return fail(FEMGPU_BAD_INPUT, "assemble: problem built from a matrix");
+    for (uint32_t s = 0; s < P.n_sub; ++s)
+        if (!P.material_set[s]) return fail(FEMGPU_NOT_READY, "assemble: material not set");
+    int rc = upload_constraints(P);
+    if (!rc) rc = upload_faces(H);
+    if (rc) return rc;
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t *>(P.d_flags), 4, 0u, P.stream));
+    HIPCHK(hipEventRecord(P.ev[0], P.stream));
+    HIPCHK(launch_assemble(P, do_mat, do_vec));
+    HIPCHK(launch_constraints(P, do_mat, do_vec));
+    HIPCHK(hipEventRecord(P.ev[1], P.stream));
+    HIPCHK(hipMemcpyAsync(P.h_flags, P.d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, P.stream));
+    HIPCHK(hipMemcpyAsync(P.h_ps, P.d_ps, sizeof(ProbScalars) * P.n_sub, hipMemcpyDeviceToHost, P.stream));
+    HIPCHK(hipStreamSynchronize(P.stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev[0], P.ev[1]));
+    P.last_asm_ms = ms;
+    if (do_mat) {
+        P.assembled_mat = true;
+        P.k32_valid = false;
+        P.minv_kind = -1;
+        for (uint32_t s = 0; s < P.n_sub; ++s) P.max_diag[s] = P.h_ps[s].max_diag;
+    }
+    if (do_vec) P.assembled_vec = true;
+    if (P.h_flags[0]) return fail(FEMGPU_NEG_JACOBIAN, "assemble: element with det J <= 0");
     return FEMGPU_OK;
 }
 
@@ -151,6 +365,28 @@ int check_opts(const femgpu_solve_opts *o) {
         o->crit_kind > 1 || !(o->tol >= 0) || o->max_iter < 0 || o->chunk < 1 || !(o->reliable_delta > 0) ||
         !(o->reliable_delta < 1))
         return fail(FEMGPU_BAD_INPUT, "invalid solve options");
+    return FEMGPU_OK;
+}
+
+int solve_system(femgpu_problem &H, const femgpu_solve_opts *o, femgpu_solve_stats *per, double *d_rhs, double *d_x) {
+    int rc = check_opts(o);
+    if (rc) return rc;
+    Problem &P = H.P;
+    if (!P.assembled_mat || !P.assembled_vec) return fail(FEMGPU_NOT_READY, "solve_system: assemble first");
+    std::vector<SolveOut> out(P.n_sub);
+    rc = solve_pcg(P, d_rhs ? d_rhs : P.d_F, d_x ? d_x : P.d_u, to_internal(o), out.data());
+    if (per)
+        for (uint32_t s = 0; s < P.n_sub; ++s) to_stats(out[s], &per[s]);
+    if (rc) return fail(rc, std::string("solve_system: ") + femgpu_status_string(rc));
+    return FEMGPU_OK;
+}
+
+int copy_sub(femgpu_problem &H, uint32_t s, const double *src, double *dst_host, size_t n) {
+    Problem &P = H.P;
+    if (s >= P.n_sub || !dst_host || n != 3 * (size_t)(P.node_off[s + 1] - P.node_off[s]))
+        return fail(FEMGPU_BAD_INPUT, "get: size");
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipMemcpy(dst_host, src + 3 * (size_t)P.node_off[s], n * 8, hipMemcpyDeviceToHost));
     return FEMGPU_OK;
 }
 
@@ -215,37 +451,13 @@ int femgpu_skin_faces(int kind, uint32_t n_nodes, uint32_t n_elem, const uint32_
 }
 
 int femgpu_create(const femgpu_mesh *m, femgpu_problem **out) {
-    if (!m || !out || !m->coords || !m->conn || m->elem_kind < 0 || m->elem_kind > 2 || !m->n_nodes || !m->n_elem)
-        return fail(FEMGPU_BAD_INPUT, "create: bad mesh");
+    if (!m || !out) return fail(FEMGPU_BAD_INPUT, "create: null");
     *out = nullptr;
-    const int npe = nodes_per_elem(m->elem_kind);
-    for (size_t s = 0; s < (size_t)m->n_elem * npe; ++s)
-        if (m->conn[s] >= m->n_nodes) return fail(FEMGPU_BAD_INPUT, "create: node index out of range");
     auto *h = new (std::nothrow) femgpu_problem();
     if (!h) return fail(FEMGPU_BAD_INPUT, "create: out of host memory");
-    Problem &P = h->P;
-    P.kind = m->elem_kind;
-    P.npe = npe;
-    P.n_nodes = m->n_nodes;
-    P.n_elem = m->n_elem;
-    (void)hipGetDevice(&P.device);
-    P.coords_h.assign(m->coords, m->coords + 3 * (size_t)m->n_nodes);
-    P.conn_h.assign(m->conn, m->conn + (size_t)m->n_elem * npe);
-    P.pat = build_pattern(P.n_nodes, P.n_elem, npe, m->conn);
-    P.nnzb = P.pat.col.size();
-    P.ecol = color_elements(P.n_nodes, P.n_elem, npe, m->conn);
-    int rc = alloc_system(P);
-    if (rc == FEMGPU_OK) {
-        std::vector<uint32_t> slot = build_slot_map(P.pat, P.n_elem, npe, m->conn);
-        hipError_t e = upload(&P.d_slot, slot.data(), slot.size());
-        if (e == hipSuccess) e = upload(&P.d_coords, m->coords, 3 * (size_t)P.n_nodes);
-        if (e == hipSuccess) e = upload(&P.d_conn, m->conn, (size_t)P.n_elem * npe);
-        if (e == hipSuccess) e = upload(&P.d_order, P.ecol.order.data(), P.ecol.order.size());
-        if (e == hipSuccess) e = dalloc(&P.d_E, P.n_elem);
-        if (e != hipSuccess) rc = fail(FEMGPU_HIP_ERROR, std::string("create: ") + hipGetErrorString(e));
-    }
-    if (rc != FEMGPU_OK) {
-        free_problem(P);
+    const int rc = build_problem(*h, 1, m);
+    if (rc) {
+        free_problem(h->P);
         delete h;
         return rc;
     }
@@ -261,31 +473,40 @@ int femgpu_create_from_bsr(uint32_t n_brows, const uint32_t *row_ptr, const uint
     Pattern pat;
     pat.n_rows = n_brows;
     pat.row_ptr.assign(row_ptr, row_ptr + n_brows + 1);
-    pat.col.assign(col, col + row_ptr[n_brows]);
-    pat.diag.assign(n_brows, 0xffffffffu);
-    for (uint32_t r = 0; r < n_brows; ++r) {
+    for (uint32_t r = 0; r < n_brows; ++r)
         if (row_ptr[r + 1] < row_ptr[r]) return fail(FEMGPU_BAD_INPUT, "create_from_bsr: row_ptr not monotone");
+    pat.col.assign(col, col + row_ptr[n_brows]);
+    for (uint32_t r = 0; r < n_brows; ++r) {
+        bool has_diag = false;
         for (uint32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
             if (col[p] >= n_brows || (p > row_ptr[r] && col[p] <= col[p - 1]))
                 return fail(FEMGPU_BAD_INPUT, "create_from_bsr: columns out of range or unsorted");
-            if (col[p] == r) pat.diag[r] = p;
+            has_diag |= col[p] == r;
         }
-        if (pat.diag[r] == 0xffffffffu) return fail(FEMGPU_BAD_INPUT, "create_from_bsr: missing diagonal block");
+        if (!has_diag) return fail(FEMGPU_BAD_INPUT, "create_from_bsr: missing diagonal block");
     }
+    for (size_t k = 0; k < 9 * (size_t)row_ptr[n_brows]; ++k)
+        if (!std::isfinite(vals[k])) return fail(FEMGPU_NAN, "create_from_bsr: non-finite value");
+    finish_pattern(pat);
     auto *h = new (std::nothrow) femgpu_problem();
     if (!h) return fail(FEMGPU_BAD_INPUT, "create_from_bsr: out of host memory");
     Problem &P = h->P;
     P.from_bsr = true;
+    P.n_sub = 1;
     P.n_nodes = n_brows;
+    P.node_off = {0, n_brows};
+    P.elem_off = {0, 0};
     P.pat = std::move(pat);
     P.nnzb = P.pat.col.size();
     (void)hipGetDevice(&P.device);
     int rc = alloc_system(P);
     if (rc == FEMGPU_OK) {
-        // block row-major input -> nine planes
         std::vector<double> planes(9 * P.ps, 0.0);
-        for (size_t p = 0; p < P.nnzb; ++p)
-            for (int c = 0; c < 9; ++c) planes[c * P.ps + p] = vals[9 * p + c];
+        for (uint32_t r = 0; r < n_brows; ++r)
+            for (uint32_t p = P.pat.row_ptr[r]; p < P.pat.row_ptr[r + 1]; ++p) {
+                const uint64_t q = P.pat.pos_of_slot(r, p);
+                for (int c = 0; c < 9; ++c) planes[c * P.ps + q] = vals[9 * (size_t)p + c];
+            }
         hipError_t e = hipMemcpy(P.d_K, planes.data(), planes.size() * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess && rhs) e = hipMemcpy(P.d_F, rhs, 3 * (size_t)n_brows * 8, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = fail(FEMGPU_HIP_ERROR, hipGetErrorString(e));
@@ -308,22 +529,8 @@ int femgpu_destroy(femgpu_problem *h) {
 }
 
 int femgpu_set_material(femgpu_problem *h, const femgpu_material *m) {
-    if (!h || !m || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "set_material: bad input");
-    Problem &P = h->P;
-    if (!(m->nu > -1.0 && m->nu < 0.5)) return fail(FEMGPU_BAD_INPUT, "set_material: nu out of (-1, 0.5)");
-    P.nu = m->nu;
-    for (int d = 0; d < 3; ++d) P.bf[d] = m->body_force[d];
-    if (m->E_elem) {
-        for (uint32_t e = 0; e < P.n_elem; ++e)
-            if (!std::isfinite(m->E_elem[e])) return fail(FEMGPU_NAN, "set_material: non-finite E");
-        HIPCHK(hipMemcpy(P.d_E, m->E_elem, (size_t)P.n_elem * 8, hipMemcpyHostToDevice));
-    } else {
-        if (!std::isfinite(m->E)) return fail(FEMGPU_NAN, "set_material: non-finite E");
-        HIPCHK(launch_fill(P.d_E, P.n_elem, m->E, P.stream));
-        HIPCHK(hipStreamSynchronize(P.stream));
-    }
-    P.material_set = true;
-    return FEMGPU_OK;
+    if (!h) return fail(FEMGPU_BAD_INPUT, "set_material: null");
+    return set_material(*h, 0, m);
 }
 
 int femgpu_set_elastic_modulus_device(femgpu_problem *h, const double *d_E) {
@@ -331,24 +538,13 @@ int femgpu_set_elastic_modulus_device(femgpu_problem *h, const double *d_E) {
     Problem &P = h->P;
     HIPCHK(hipMemcpyAsync(P.d_E, d_E, (size_t)P.n_elem * 8, hipMemcpyDeviceToDevice, P.stream));
     HIPCHK(hipStreamSynchronize(P.stream));
+    std::fill(P.material_set.begin(), P.material_set.end(), 1);
     return FEMGPU_OK;
 }
 
 int femgpu_set_traction(femgpu_problem *h, uint32_t n_faces, const uint32_t *faces, const double *trac) {
-    if (!h || h->P.from_bsr || (n_faces && (!faces || !trac))) return fail(FEMGPU_BAD_INPUT, "set_traction: bad input");
-    Problem &P = h->P;
-    const int npf = nodes_per_face(face_kind_of(P.kind));
-    for (size_t s = 0; s < (size_t)n_faces * npf; ++s)
-        if (faces[s] >= P.n_nodes) return fail(FEMGPU_BAD_INPUT, "set_traction: node index out of range");
-    dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
-    P.n_faces = n_faces;
-    P.fcol = Coloring{};
-    if (!n_faces) return FEMGPU_OK;
-    P.fcol = color_elements(P.n_nodes, n_faces, npf, faces);
-    HIPCHK(upload(&P.d_faces, faces, (size_t)n_faces * npf));
-    HIPCHK(upload(&P.d_forder, P.fcol.order.data(), P.fcol.order.size()));
-    HIPCHK(upload(&P.d_trac, trac, 3 * (size_t)n_faces));
-    return FEMGPU_OK;
+    if (!h) return fail(FEMGPU_BAD_INPUT, "set_traction: null");
+    return set_traction(*h, 0, n_faces, faces, trac);
 }
 
 int femgpu_add_constraint(femgpu_problem *h, uint32_t dof, double value, double penalty) {
@@ -356,15 +552,8 @@ int femgpu_add_constraint(femgpu_problem *h, uint32_t dof, double value, double 
 }
 
 int femgpu_add_constraints(femgpu_problem *h, uint32_t n, const uint32_t *dofs, const double *values, double penalty) {
-    if (!h || (n && !dofs)) return fail(FEMGPU_BAD_INPUT, "add_constraints: bad input");
-    Problem &P = h->P;
-    if (!(penalty > 0)) return fail(FEMGPU_BAD_INPUT, "add_constraints: only the penalty form (penalty > 0) keeps K SPD");
-    for (uint32_t k = 0; k < n; ++k) {
-        if (dofs[k] >= 3 * P.n_nodes) return fail(FEMGPU_BAD_INPUT, "add_constraints: dof out of range");
-        P.cons.push_back({dofs[k], values ? values[k] : 0.0, penalty});
-    }
-    P.cons_dirty = true;
-    return FEMGPU_OK;
+    if (!h) return fail(FEMGPU_BAD_INPUT, "add_constraints: null");
+    return add_constraints(*h, 0, n, dofs, values, penalty);
 }
 
 int femgpu_clear_constraints(femgpu_problem *h) {
@@ -376,39 +565,15 @@ int femgpu_clear_constraints(femgpu_problem *h) {
 
 int femgpu_assemble(femgpu_problem *h, int do_mat, int do_vec) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "assemble: null");
-    Problem &P = h->P;
-    if (P.from_bsr) return fail(FEMGPU_BAD_INPUT, "assemble: problem built from a matrix");
-    if (!P.material_set) return fail(FEMGPU_NOT_READY, "assemble: material not set");
-    int rc = upload_constraints(P);
-    if (rc) return rc;
-    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t *>(&P.d_sc->pad), 1, 0u, P.stream));
-    HIPCHK(hipEventRecord(P.ev[0], P.stream));
-    HIPCHK(launch_assemble(P, do_mat, do_vec, &P.d_sc->pad));
-    HIPCHK(launch_constraints(P, do_mat, do_vec));
-    HIPCHK(hipEventRecord(P.ev[1], P.stream));
-    HIPCHK(hipMemcpyAsync(P.h_sc, P.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, P.stream));
-    HIPCHK(hipStreamSynchronize(P.stream));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, P.ev[0], P.ev[1]));
-    P.last_asm_ms = ms;
-    if (do_mat) { P.assembled_mat = true; P.k32_valid = false; P.minv_kind = -1; P.max_diag = P.h_sc->max_diag; }
-    if (do_vec) P.assembled_vec = true;
-    if (P.h_sc->pad) return fail(FEMGPU_NEG_JACOBIAN, "assemble: element with det J <= 0");
-    return FEMGPU_OK;
+    return assemble(*h, do_mat, do_vec);
 }
 
 int femgpu_solve_system(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_stats *st) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "solve_system: null");
-    int rc = check_opts(o);
-    if (rc) return rc;
-    Problem &P = h->P;
-    if (!P.assembled_mat || !P.assembled_vec) return fail(FEMGPU_NOT_READY, "solve_system: assemble first");
-    SolveOut s;
-    rc = solve_pcg(P, P.d_F, P.d_u, to_internal(o), s);
-    to_stats(s, st);
+    femgpu_solve_stats tmp{};
+    const int rc = solve_system(*h, o, st ? st : &tmp, nullptr, nullptr);
     if (st) st->status = rc;
-    if (rc) return fail(rc, std::string("solve_system: ") + femgpu_status_string(rc));
-    return FEMGPU_OK;
+    return rc;
 }
 
 int femgpu_solve(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_stats *st) {
@@ -417,10 +582,8 @@ int femgpu_solve(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_sta
 }
 
 int femgpu_get_result(femgpu_problem *h, double *u, size_t n) {
-    if (!h || !u || n != 3 * (size_t)h->P.n_nodes) return fail(FEMGPU_BAD_INPUT, "get_result: size");
-    HIPCHK(hipStreamSynchronize(h->P.stream));
-    HIPCHK(hipMemcpy(u, h->P.d_u, n * 8, hipMemcpyDeviceToHost));
-    return FEMGPU_OK;
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_result: null");
+    return copy_sub(*h, 0, h->P.d_u, u, n);
 }
 
 int femgpu_get_result_device(femgpu_problem *h, double *d_u) {
@@ -438,10 +601,8 @@ int femgpu_set_result(femgpu_problem *h, const double *u, size_t n) {
 }
 
 int femgpu_get_sollicitation(femgpu_problem *h, double *F, size_t n) {
-    if (!h || !F || n != 3 * (size_t)h->P.n_nodes) return fail(FEMGPU_BAD_INPUT, "get_sollicitation: size");
-    HIPCHK(hipStreamSynchronize(h->P.stream));
-    HIPCHK(hipMemcpy(F, h->P.d_F, n * 8, hipMemcpyDeviceToHost));
-    return FEMGPU_OK;
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_sollicitation: null");
+    return copy_sub(*h, 0, h->P.d_F, F, n);
 }
 
 int femgpu_get_matrix(femgpu_problem *h, uint32_t *row_ptr, uint32_t *col, double *vals) {
@@ -453,8 +614,11 @@ int femgpu_get_matrix(femgpu_problem *h, uint32_t *row_ptr, uint32_t *col, doubl
         std::vector<double> planes(9 * P.ps);
         HIPCHK(hipStreamSynchronize(P.stream));
         HIPCHK(hipMemcpy(planes.data(), P.d_K, planes.size() * 8, hipMemcpyDeviceToHost));
-        for (size_t p = 0; p < P.nnzb; ++p)
-            for (int c = 0; c < 9; ++c) vals[9 * p + c] = planes[c * P.ps + p];
+        for (uint32_t r = 0; r < P.n_nodes; ++r)
+            for (uint32_t p = P.pat.row_ptr[r]; p < P.pat.row_ptr[r + 1]; ++p) {
+                const uint64_t q = P.pat.pos_of_slot(r, p);
+                for (int c = 0; c < 9; ++c) vals[9 * (size_t)p + c] = planes[c * P.ps + q];
+            }
     }
     return FEMGPU_OK;
 }
@@ -470,7 +634,7 @@ int femgpu_get_sizes(femgpu_problem *h, uint32_t *n_nodes, uint32_t *n_elem, uin
 
 int femgpu_get_max_diag(femgpu_problem *h, double *md) {
     if (!h || !md) return fail(FEMGPU_BAD_INPUT, "get_max_diag: null");
-    *md = h->P.max_diag;
+    *md = h->P.max_diag.empty() ? 0.0 : h->P.max_diag[0];
     return FEMGPU_OK;
 }
 
@@ -493,8 +657,6 @@ int femgpu_elem_grad(femgpu_problem *h, const double *u, const double *lam, doub
 int femgpu_adjoint_device(femgpu_problem *h, const double *d_dJdu, double *d_dJdE, const femgpu_solve_opts *o,
                           femgpu_solve_stats *st) {
     if (!h || !d_dJdu || !d_dJdE || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "adjoint: bad input");
-    int rc = check_opts(o);
-    if (rc) return rc;
     Problem &P = h->P;
     if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "adjoint: assemble first");
     const size_t n = 3 * (size_t)P.n_nodes;
@@ -504,17 +666,16 @@ int femgpu_adjoint_device(femgpu_problem *h, const double *d_dJdu, double *d_dJd
     hipError_t e = hipMemcpyAsync(d_rhs, d_dJdu, n * 8, hipMemcpyDeviceToDevice, P.stream);
     if (e == hipSuccess) e = launch_fill(P.d_aux, n, 0.0, P.stream);
     if (e != hipSuccess) { (void)hipFree(d_rhs); return fail(FEMGPU_HIP_ERROR, hipGetErrorString(e)); }
-    SolveOut s;
-    rc = solve_pcg(P, d_rhs, P.d_aux, to_internal(o), s);
-    to_stats(s, st);
+    femgpu_solve_stats tmp{};
+    int rc = solve_system(*h, o, st ? st : &tmp, d_rhs, P.d_aux);
     if (rc == FEMGPU_OK) {
         e = launch_elem_grad(P, P.d_u, P.d_aux, d_dJdE);
         if (e == hipSuccess) e = hipStreamSynchronize(P.stream);
         if (e != hipSuccess) rc = fail(FEMGPU_HIP_ERROR, hipGetErrorString(e));
     }
     (void)hipFree(d_rhs);
-    if (rc) return fail(rc, std::string("adjoint: ") + femgpu_status_string(rc));
-    return FEMGPU_OK;
+    if (st) st->status = rc;
+    return rc;
 }
 
 int femgpu_adjoint(femgpu_problem *h, const double *dJdu, double *dJdE, const femgpu_solve_opts *o,
@@ -552,6 +713,86 @@ int femgpu_time_spmv(femgpu_problem *h, int precision, int reps, double *avg_ms)
     const double t = time_spmv(P, precision, reps);
     if (t < 0) return fail(FEMGPU_HIP_ERROR, "time_spmv failed");
     *avg_ms = t;
+    return FEMGPU_OK;
+}
+
+int femgpu_autotune(femgpu_problem *h, int *unroll) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "autotune: null");
+    const int u = autotune_spmv(h->P);
+    if (unroll) *unroll = u;
+    return FEMGPU_OK;
+}
+
+int femgpu_set_graph_replay(femgpu_problem *h, int enable) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "set_graph_replay: null");
+    h->P.use_graph = enable != 0;
+    if (!enable) destroy_graph(h->P);
+    return FEMGPU_OK;
+}
+
+// ---- batched uneven problems -------------------------------------------------------------------
+
+int femgpu_batch_create(uint32_t n, const femgpu_mesh *meshes, femgpu_batch **out) {
+    if (!out || !meshes || !n) return fail(FEMGPU_BAD_INPUT, "batch_create: bad input");
+    *out = nullptr;
+    auto *b = new (std::nothrow) femgpu_batch();
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_create: out of host memory");
+    const int rc = build_problem(b->H, n, meshes);
+    if (rc) {
+        free_problem(b->H.P);
+        delete b;
+        return rc;
+    }
+    b->H.P.use_graph = true;
+    *out = b;
+    return FEMGPU_OK;
+}
+
+int femgpu_batch_set_material(femgpu_batch *b, uint32_t i, const femgpu_material *mat) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_set_material: null");
+    return set_material(b->H, i, mat);
+}
+
+int femgpu_batch_set_traction(femgpu_batch *b, uint32_t i, uint32_t n_faces, const uint32_t *faces, const double *trac) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_set_traction: null");
+    return set_traction(b->H, i, n_faces, faces, trac);
+}
+
+int femgpu_batch_add_constraints(femgpu_batch *b, uint32_t i, uint32_t n, const uint32_t *dofs, const double *values,
+                                 double penalty) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_add_constraints: null");
+    return add_constraints(b->H, i, n, dofs, values, penalty);
+}
+
+int femgpu_batch_assemble(femgpu_batch *b) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_assemble: null");
+    return assemble(b->H, 1, 1);
+}
+
+int femgpu_batch_solve(femgpu_batch *b, const femgpu_solve_opts *opts, femgpu_solve_stats *per, double *total_ms) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_solve: null");
+    std::vector<femgpu_solve_stats> st(b->H.P.n_sub);
+    const int rc = solve_system(b->H, opts, st.data(), nullptr, nullptr);
+    if (per) std::copy(st.begin(), st.end(), per);
+    if (total_ms) *total_ms = st.empty() ? 0.0 : st[0].solve_ms;
+    return rc;
+}
+
+int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, size_t n) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_get_result: null");
+    return copy_sub(b->H, i, b->H.P.d_u, u, n);
+}
+
+int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms) {
+    if (!b || !ms) return fail(FEMGPU_BAD_INPUT, "batch_get_assemble_ms: null");
+    *ms = b->H.P.last_asm_ms;
+    return FEMGPU_OK;
+}
+
+int femgpu_batch_destroy(femgpu_batch *b) {
+    if (!b) return FEMGPU_OK;
+    free_problem(b->H.P);
+    delete b;
     return FEMGPU_OK;
 }
 

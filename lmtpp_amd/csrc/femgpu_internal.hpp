@@ -1,4 +1,11 @@
-// Internal structures shared by the C ABI (capi.hip) and the kernels (assemble.hip, solve.hip).
+// Internal structures shared by the C ABI (capi.hip, batch.hip) and the kernels (assemble.hip,
+// solve.hip).
+//
+// A Problem holds n_sub >= 1 independent elasticity systems in one device system: their nodes,
+// elements, faces and constraints are concatenated (node indices offset), the block pattern is
+// block diagonal across sub-problems, and every workgroup of the row-parallel kernels covers
+// rows of exactly one sub-problem.  Each sub-problem keeps its own CG scalars, stop test and
+// last-workgroup counters, so one launch advances every unfinished system.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -10,78 +17,95 @@
 
 namespace femgpu {
 
-// Device-resident scalars of one CG solve.  Written by the last workgroup of each reducing
-// kernel, read by the next kernel and, between chunks of iterations, by the host.
-struct Scalars {
+// Device-resident scalars of one sub-problem's CG.  Written by the last workgroup of that
+// sub-problem in each reducing kernel, read by the next kernel and, between chunks of
+// iterations, by the host.
+struct ProbScalars {
     double rho, rho_prev, pq, alpha, beta, rr, rmax;
     double bb, stop2, crit;       // stop test: rr <= stop2 (REL_L2) or rmax <= crit (ABS_MAXNORM)
-    double rr_ref, delta2;        // mixed: refresh when rr <= delta2 * rr_ref
-    double max_diag;              // assembly: max |diag K|
-    int iter, max_iter, done, pause, status, crit_kind, mixed, pad;
+    double rr_ref, delta2;        // mixed: pause for an FP64 refresh when rr <= delta2 * rr_ref
+    double max_diag;              // assembly: max |diag K| of this sub-problem
+    int iter, max_iter, done, pause, status, crit_kind, mixed, refreshes;
     unsigned counter[8];          // last-workgroup counters, one per reducing kernel
 };
 
-enum { CTR_SPMV = 0, CTR_UPD = 1, CTR_RES = 2, CTR_REFRESH = 3, CTR_MAXDIAG = 4, CTR_INIT = 5 };
+enum { CTR_SPMV = 0, CTR_UPD = 1, CTR_RES = 2, CTR_REFRESH = 3, CTR_MAXDIAG = 4 };
+// pause states: 0 running, 1 FP64 refresh requested, 2 refreshed (direction update pending)
+enum { PAUSE_NONE = 0, PAUSE_REQ = 1, PAUSE_DONE = 2 };
+// sub-problem selectors of the refresh kernels
+enum { SEL_ACTIVE = 0, SEL_PAUSE_REQ = 1, SEL_PAUSE_DONE = 2, SEL_ALL = 3 };
 
-constexpr int WG = 256;   // threads per workgroup for all vector / SpMV kernels
+struct SubMaterial { double nu, bf0, bf1, bf2; };
+
+constexpr uint32_t ROWS_PER_WG = 256;
+constexpr uint32_t NO_SUB = 0xffffffffu;
 
 struct SpmvConfig {
-    int group = 32;       // lanes per block row
-    int nwg = 2048;       // workgroups (multiple of 8: XCD-contiguous row slabs)
+    int unroll = 2;       // SELL slots per loop trip of the block SpMV (1, 2 or 3)
+    bool tuned = false;
 };
 
 struct Problem {
     int kind = 0, npe = 0, device = 0;
-    uint32_t n_nodes = 0, n_elem = 0;
-    uint64_t nnzb = 0, ps = 0;  // stored blocks, plane stride (elements)
+    uint32_t n_sub = 1;
+    uint32_t n_nodes = 0, n_elem = 0, n_faces = 0;          // totals over sub-problems
+    std::vector<uint32_t> node_off, elem_off;                // n_sub + 1 offsets
+    uint64_t nnzb = 0, ps = 0;  // stored blocks, plane stride (SELL positions, padded to 64)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
 
     Pattern pat;
     Coloring ecol, fcol;
-    std::vector<double> coords_h;
-    std::vector<uint32_t> conn_h;
+
+    // workgroup tables of the row-parallel kernels (indexed by logical workgroup)
+    uint32_t n_wg = 0;                                       // grid size, multiple of 8
+    uint32_t *d_wg_row0 = nullptr, *d_wg_row1 = nullptr, *d_wg_sub = nullptr;
+    uint32_t *d_sub_wg0 = nullptr, *d_sub_nwg = nullptr;     // per sub-problem workgroup range
 
     // device: mesh, pattern, scatter map
     double *d_coords = nullptr, *d_E = nullptr;
-    uint32_t *d_conn = nullptr, *d_rp = nullptr, *d_col = nullptr, *d_diag = nullptr, *d_slot = nullptr, *d_order = nullptr;
+    uint32_t *d_conn = nullptr, *d_rp = nullptr, *d_col = nullptr, *d_diag = nullptr, *d_slot = nullptr,
+             *d_order = nullptr, *d_sptr = nullptr, *d_elem_sub = nullptr;
+    SubMaterial *d_mat = nullptr;
+    std::vector<SubMaterial> mat_h;
+    std::vector<uint8_t> material_set;
     // device: system
     double *d_K = nullptr, *d_F = nullptr, *d_u = nullptr;
     float *d_K32 = nullptr;
     bool k32_valid = false, assembled_mat = false, assembled_vec = false;
     // skin loads
-    uint32_t n_faces = 0;
     uint32_t *d_faces = nullptr, *d_forder = nullptr;
     double *d_trac = nullptr;
+    std::vector<uint32_t> faces_h;
+    std::vector<double> trac_h;
     // constraints, grouped by dof in insertion order
     struct Cons { uint32_t dof; double val, pen; };
     std::vector<Cons> cons;
     bool cons_dirty = true;
     uint32_t n_cdof = 0;
-    uint32_t *d_cdof = nullptr, *d_cptr = nullptr;
+    uint32_t *d_cdof = nullptr, *d_cptr = nullptr, *d_csub = nullptr;
     double *d_cval = nullptr, *d_cpen = nullptr;
-    // material
-    double nu = 0.3, bf[3] = {0, 0, 0};
-    bool material_set = false;
     // solver work vectors
-    double *d_r = nullptr, *d_p = nullptr, *d_q = nullptr, *d_Minv = nullptr, *d_b = nullptr, *d_x = nullptr;
+    double *d_r = nullptr, *d_p = nullptr, *d_q = nullptr, *d_Minv = nullptr;
     double *d_aux = nullptr;  // adjoint state / scratch vector
     float *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_x32 = nullptr, *d_Minv32 = nullptr;
     double *d_partials = nullptr;
-    Scalars *d_sc = nullptr, *h_sc = nullptr;
+    ProbScalars *d_ps = nullptr, *h_ps = nullptr;
+    int *d_flags = nullptr, *h_flags = nullptr;  // [0]: element with det J <= 0
     int minv_kind = -1;  // preconditioner currently stored in d_Minv
     SpmvConfig spmv;
-    uint32_t *d_wgrows = nullptr;
-    int wgrows_nwg = 0;
+    std::vector<double> max_diag;  // per sub-problem, of the last matrix assembly
     double last_asm_ms = 0;
-    double max_diag = 0;  // max|diag K| of the last matrix assembly, before constraints
     bool from_bsr = false;
+    bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
 };
 
 // ---- launchers (assemble.hip) ----
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
-hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec, int *d_flags);
+hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_constraints(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_to_fp32(Problem &P);
 hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, double *d_g);
@@ -89,10 +113,11 @@ hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, 
 // ---- solver (solve.hip) ----
 struct SolveOpts { int precision, precond, crit_kind; double tol; int max_iter; double delta; int chunk; };
 struct SolveOut { int status, iters, refreshes; double rel_res, max_abs_res, ms; };
-int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut &out);
-hipError_t ensure_wgrows(Problem &P);
+// Solves every sub-problem; out[s] per sub-problem; returns the first non-zero status.
+int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out);
 double time_spmv(Problem &P, int precision, int reps);
-hipError_t spmv_apply(Problem &P, const double *d_x, double *d_y);
+int autotune_spmv(Problem &P);
+void destroy_graph(Problem &P);
 
 std::string &last_error();
 

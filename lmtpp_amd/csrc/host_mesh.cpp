@@ -250,15 +250,39 @@ Pattern build_pattern(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t
     P.row_ptr[0] = 0;
     for (uint32_t n = 0; n < n_nodes; ++n) P.row_ptr[n + 1] = P.row_ptr[n] + (uint32_t)rows[n].size();
     P.col.resize(P.row_ptr[n_nodes]);
-    P.diag.resize(n_nodes);
     parallel_for(n_nodes, [&](size_t b, size_t e) {
         for (size_t n = b; n < e; ++n) {
             std::copy(rows[n].begin(), rows[n].end(), P.col.begin() + P.row_ptr[n]);
-            P.diag[n] = P.row_ptr[n] + uint32_t(std::lower_bound(rows[n].begin(), rows[n].end(), (uint32_t)n) - rows[n].begin());
             std::vector<uint32_t>().swap(rows[n]);
         }
     });
+    finish_pattern(P);
     return P;
+}
+
+void finish_pattern(Pattern &P) {
+    const uint32_t n = P.n_rows, ns = (n + 63) / 64;
+    P.sell_ptr.assign((size_t)ns + 1, 0);
+    for (uint32_t s = 0; s < ns; ++s) {
+        uint32_t w = 0;
+        for (uint32_t r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, P.row_ptr[r + 1] - P.row_ptr[r]);
+        P.sell_ptr[s + 1] = P.sell_ptr[s] + 64 * w;
+    }
+    P.n_sell = P.sell_ptr[ns];
+    P.diag.assign(n, 0);
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t *b = P.col.data() + P.row_ptr[r], *e = P.col.data() + P.row_ptr[r + 1];
+        P.diag[r] = (uint32_t)P.pos(r, uint32_t(std::lower_bound(b, e, r) - b));
+    }
+}
+
+std::vector<uint32_t> sell_columns(const Pattern &P) {
+    std::vector<uint32_t> c(std::max<uint64_t>(P.n_sell, 1));
+    for (uint32_t r = 0; r < P.n_rows; ++r) {
+        const uint32_t s = r >> 6, w = (P.sell_ptr[s + 1] - P.sell_ptr[s]) / 64, len = P.row_ptr[r + 1] - P.row_ptr[r];
+        for (uint32_t k = 0; k < w; ++k) c[P.pos(r, k)] = k < len ? P.col[P.row_ptr[r] + k] : r;
+    }
+    return c;
 }
 
 std::vector<uint32_t> build_slot_map(const Pattern &pat, uint32_t n_elem, int npe, const uint32_t *conn) {
@@ -269,7 +293,7 @@ std::vector<uint32_t> build_slot_map(const Pattern &pat, uint32_t n_elem, int np
             for (int a = 0; a < npe; ++a) {
                 const uint32_t *r0 = pat.col.data() + pat.row_ptr[v[a]], *r1 = pat.col.data() + pat.row_ptr[v[a] + 1];
                 for (int c = 0; c < npe; ++c)
-                    slot[(el * npe + a) * npe + c] = uint32_t(std::lower_bound(r0, r1, v[c]) - pat.col.data());
+                    slot[(el * npe + a) * npe + c] = (uint32_t)pat.pos(v[a], uint32_t(std::lower_bound(r0, r1, v[c]) - r0));
             }
         }
     });

@@ -23,13 +23,26 @@ int make_rect(int kind, const double X0[3], const double X1[3], const uint32_t n
 std::vector<uint32_t> skin_faces(int kind, uint32_t n_nodes, uint32_t n_elem, const uint32_t *conn);
 
 // Full symmetric 3x3-block pattern: node adjacency, rows sorted, every row holds its diagonal.
+// Device storage of the blocks is sliced ELL (SELL-64): block rows are grouped in slices of 64
+// consecutive rows, slice s is as wide as its longest row, and block k of row r sits at
+// position pos(r, k) = sell_ptr[r/64] + 64 k + r%64, so the 64 rows of a slice read their k-th
+// blocks from 64 consecutive positions.
 struct Pattern {
     uint32_t n_rows = 0;
-    std::vector<uint32_t> row_ptr, col, diag;  // diag[r] = slot of block (r,r)
+    std::vector<uint32_t> row_ptr, col;   // CSR view (the API and the reference's row order)
+    std::vector<uint32_t> sell_ptr;       // n_slices + 1 slice starts, in block positions
+    std::vector<uint32_t> diag;           // diag[r] = SELL position of block (r,r)
+    uint64_t n_sell = 0;                  // positions including padding
+    uint64_t pos(uint32_t r, uint32_t k) const { return sell_ptr[r >> 6] + 64ull * k + (r & 63u); }
+    uint64_t pos_of_slot(uint32_t r, uint32_t slot) const { return pos(r, slot - row_ptr[r]); }
 };
 Pattern build_pattern(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t *conn);
+// Fills sell_ptr, n_sell and diag from row_ptr / col.
+void finish_pattern(Pattern &P);
+// Column of every SELL position (padding positions hold their own row, with zero values).
+std::vector<uint32_t> sell_columns(const Pattern &P);
 
-// Slot of block (conn[e][a], conn[e][b]) for every element and local pair: n_elem * npe * npe.
+// SELL position of block (conn[e][a], conn[e][b]) for every element and local pair: n_elem * npe * npe.
 std::vector<uint32_t> build_slot_map(const Pattern &pat, uint32_t n_elem, int npe, const uint32_t *conn);
 
 // Greedy first-fit colouring: no two elements of one colour share a node.

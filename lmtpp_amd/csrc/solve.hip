@@ -1,114 +1,55 @@
-// Jacobi-preconditioned CG on the 3x3-block CSR, FP64 or FP32-inner with FP64 residual
-// refreshes.  Block SpMV fused with the p.q reduction; vector updates fused with r.z, r.r and
-// max|r|; every reduction is a fixed-order two-level sum (waves, then workgroup partials summed
-// by the last workgroup), so results are bitwise reproducible run to run.
+// Jacobi-preconditioned CG on the 3x3-block SELL-64 matrix, FP64 or FP32-inner with FP64 residual
+// refreshes, for one or many independent sub-problems at once.
 //
 // The iteration is the PCG of containers/MatWithTinyBlocks.h:1101-1151 /
 // containers/conjugate_gradient.h:27-68 with the block IC(0) preconditioner replaced by
 // (block-)Jacobi:  q = K d; alpha = rho / d.q; x += alpha d; r -= alpha q; z = M^-1 r;
 // rho' = r.z; beta = rho'/rho; d = z + beta d.
+//
+// Every row-parallel kernel uses the same workgroup tables: logical workgroup L covers rows
+// [wg_row0[L], wg_row1[L]) of sub-problem wg_sub[L].  Reductions are segmented by sub-problem:
+// a wave-shuffle tree, a fixed 4-wave sum, per-workgroup partials, and the last workgroup of the
+// sub-problem (agent-scope atomic counter + fences) sums that sub-problem's partials in index
+// order -- bitwise reproducible.  Workgroups of finished or paused sub-problems return at once.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
-#include "femgpu_internal.hpp"
+#include "device_common.hpp"
 
 namespace femgpu {
 
 enum { MODE_PLAIN = 0, MODE_PQ = 1, MODE_RES = 2 };
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
-
-// Two-level reduction of NV values (bit k of MAXMASK set = max, else sum).  Returns true in the
-// last workgroup to arrive; there, out[] is valid in every thread.
-template <int NV, unsigned MAXMASK>
-__device__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *counter, double (&out)[NV]) {
-    __shared__ double sm[NV][4];
-    __shared__ bool last;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double t = (MAXMASK >> k & 1) ? wave_max(v[k]) : wave_sum(v[k]);
-        if (lane == 0) sm[k][wave] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const double t = (MAXMASK >> k & 1) ? fmax(fmax(sm[k][0], sm[k][1]), fmax(sm[k][2], sm[k][3]))
-                                                : ((sm[k][0] + sm[k][1]) + sm[k][2]) + sm[k][3];
-            partials[(size_t)k * gridDim.x + blockIdx.x] = t;
-        }
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    const volatile double *vp = partials;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double a = 0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-            const double t = vp[(size_t)k * gridDim.x + i];
-            a = (MAXMASK >> k & 1) ? fmax(a, t) : a + t;
-        }
-        v[k] = a;
-    }
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double t = (MAXMASK >> k & 1) ? wave_max(v[k]) : wave_sum(v[k]);
-        if (lane == 0) sm[k][wave] = t;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-        out[k] = (MAXMASK >> k & 1) ? fmax(fmax(sm[k][0], sm[k][1]), fmax(sm[k][2], sm[k][3]))
-                                    : ((sm[k][0] + sm[k][1]) + sm[k][2]) + sm[k][3];
-    if (threadIdx.x == 0) *counter = 0;
-    return true;
-}
-
-__device__ inline bool finite(double v) { return v == v && fabs(v) <= 1.7976931348623157e308; }
-
-// XCD-aware workgroup index: hardware hands workgroup i to XCD i % 8, so logical slab
-// (i % 8) * (nwg / 8) + i / 8 gives each XCD one contiguous range of rows and keeps its
-// x-vector neighbourhood in that XCD's L2.
-__device__ inline unsigned xcd_slab(unsigned nwg) { return (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3); }
-
 // ---------------------------------------------------------------------------------------------
-// Block SpMV, G lanes per block row, lane k of a row handles blocks k, k+G, ...; the nine value
-// planes are read as contiguous runs.  MODE_PQ: y = K x and p.q -> alpha.  MODE_RES:
-// y = b - K x (FP64) with ||y||^2 and max|y|.
+// Block SpMV on SELL-64 storage: one lane per block row; the 64 lanes of a wave are 64
+// consecutive rows, so every value-plane and column load of slot k is one contiguous run.
+// Rows shorter than their slice stop early (masked lanes fetch nothing).
+// MODE_PQ: y = K x and the partial of p.q.  MODE_RES: y = b - K x (FP64) with partials of ||y||^2
+// and max|y|.  The matching k_finalize turns the partials into the sub-problem's scalars.
 // ---------------------------------------------------------------------------------------------
-template <int G, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
-                                              const T *__restrict__ Kv, uint64_t ps, const T *__restrict__ x,
-                                              const double *__restrict__ b, T *__restrict__ y,
-                                              const uint32_t *__restrict__ wgrows, Scalars *sc, double *partials,
-                                              int check) {
-    if (MODE == MODE_PQ && (sc->done | sc->pause)) return;
-    if (MODE == MODE_RES && check && sc->done) return;
-    constexpr int RPT = 256 / G;
-    const unsigned w = xcd_slab(gridDim.x);
-    const uint32_t r0 = wgrows[w], r1 = wgrows[w + 1];
-    const int lg = threadIdx.x % G, grp = threadIdx.x / G;
+template <typename T, int MODE, int UNROLL>
+__global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, const uint32_t *__restrict__ sptr,
+                                              const uint32_t *__restrict__ colS, const T *__restrict__ Kv, uint64_t ps,
+                                              const T *__restrict__ x, const double *__restrict__ b,
+                                              T *__restrict__ y, WgTables wt, const ProbScalars *S, double *partials,
+                                              int sel) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB) return;
+    if (MODE != MODE_PLAIN && !selected(S[s], MODE == MODE_PQ ? (int)SEL_ACTIVE : sel)) return;
+    const uint32_t r = wt.row0[L] + threadIdx.x;
     double acc[2] = {0.0, 0.0};
-    for (uint32_t base = r0; base < r1; base += RPT) {
-        const uint32_t r = base + grp;
+    if (r < wt.row1[L]) {
+        const uint32_t len = rp[r + 1] - rp[r];
+        const uint64_t base = sptr[r >> 6] + (r & 63u);
         T s0 = 0, s1 = 0, s2 = 0;
-        if (r < r1) {
-            const uint32_t pe = rp[r + 1];
-            for (uint32_t p = rp[r] + lg; p < pe; p += G) {
-                const uint32_t c = col[p];
+        uint32_t k = 0;
+        for (; k + UNROLL <= len; k += UNROLL) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const uint64_t p = base + 64ull * (k + u);
+                const uint32_t c = colS[p];
                 const T a0 = Kv[p], a1 = Kv[ps + p], a2 = Kv[2 * ps + p];
                 const T a3 = Kv[3 * ps + p], a4 = Kv[4 * ps + p], a5 = Kv[5 * ps + p];
                 const T a6 = Kv[6 * ps + p], a7 = Kv[7 * ps + p], a8 = Kv[8 * ps + p];
@@ -118,49 +59,35 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, c
                 s2 += a6 * x0 + a7 * x1 + a8 * x2;
             }
         }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            s0 += __shfl_xor(s0, o);
-            s1 += __shfl_xor(s1, o);
-            s2 += __shfl_xor(s2, o);
+        for (; k < len; ++k) {
+            const uint64_t p = base + 64ull * k;
+            const uint32_t c = colS[p];
+            const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+            s0 += Kv[p] * x0 + Kv[ps + p] * x1 + Kv[2 * ps + p] * x2;
+            s1 += Kv[3 * ps + p] * x0 + Kv[4 * ps + p] * x1 + Kv[5 * ps + p] * x2;
+            s2 += Kv[6 * ps + p] * x0 + Kv[7 * ps + p] * x1 + Kv[8 * ps + p] * x2;
         }
-        if (r < r1 && lg < 3) {
-            const T s = lg == 0 ? s0 : lg == 1 ? s1 : s2;
-            const size_t i = 3 * (size_t)r + lg;
+        const T sv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = 3 * (size_t)r + d;
             if (MODE == MODE_RES) {
-                const double v = b[i] - (double)s;
+                const double v = b[i] - (double)sv[d];
                 y[i] = (T)v;
                 acc[0] += v * v;
                 acc[1] = fmax(acc[1], fabs(v));
             } else {
-                y[i] = s;
-                if (MODE == MODE_PQ) acc[0] += (double)s * (double)x[i];
+                y[i] = sv[d];
+                if (MODE == MODE_PQ) acc[0] += (double)sv[d] * (double)x[i];
             }
         }
     }
     if (MODE == MODE_PLAIN) return;
-    double out[2];
     if (MODE == MODE_PQ) {
-        double v1[1] = {acc[0]}, o1[1];
-        if (!grid_reduce<1, 0u>(v1, partials, &sc->counter[CTR_SPMV], o1)) return;
-        out[0] = o1[0];
-        if (threadIdx.x == 0) {
-            sc->pq = out[0];
-            if (!finite(out[0])) { sc->status = 6; sc->done = 1; }
-            else if (!(out[0] > 0)) { sc->status = 3; sc->done = 1; }
-            else sc->alpha = sc->rho / out[0];
-        }
+        double v1[1] = {acc[0]};
+        wg_partial<1, 0u>(v1, partials, gridDim.x, L);
     } else {
-        if (!grid_reduce<2, 2u>(acc, partials, &sc->counter[CTR_RES], out)) return;
-        if (threadIdx.x == 0) {
-            sc->rr = out[0];
-            sc->rmax = out[1];
-            if (check) {
-                sc->rr_ref = out[0];
-                if (!finite(out[0])) { sc->status = 6; sc->done = 1; }
-                else if (sc->crit_kind == 0 ? out[0] <= sc->stop2 : out[1] <= sc->crit) sc->done = 1;
-            }
-        }
+        wg_partial<2, 2u>(acc, partials, gridDim.x, L);
     }
 }
 
@@ -177,20 +104,23 @@ __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3
     }
 }
 
-// x += alpha p; r -= alpha q; z = M^-1 r; reductions r.z, r.r, max|r|; the last workgroup
-// advances the scalars and applies the stop test.
+// x += alpha p; r -= alpha q; z = M^-1 r; partials of r.z, r.r, max|r| (k_finalize FIN_UPD
+// advances the scalars and applies the stop test).
 template <int PRE, typename T>
-__global__ __launch_bounds__(256) void k_update(uint32_t n_nodes, T *__restrict__ x, T *__restrict__ r,
-                                                const T *__restrict__ p, const T *__restrict__ q,
-                                                const T *__restrict__ Minv, Scalars *sc, double *partials) {
-    if (sc->done | sc->pause) return;
-    const T alpha = (T)sc->alpha;
+__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+                                                const T *__restrict__ q, const T *__restrict__ Minv, WgTables wt,
+                                                const ProbScalars *S, double *partials) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], SEL_ACTIVE)) return;
+    const T alpha = (T)S[s].alpha;
     double acc[3] = {0, 0, 0};
-    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < n_nodes; n += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
         T rv[3], zv[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const size_t i = 3 * n + d;
+            const size_t i = 3 * (size_t)n + d;
             x[i] += alpha * p[i];
             rv[d] = r[i] - alpha * q[i];
             r[i] = rv[d];
@@ -203,71 +133,136 @@ __global__ __launch_bounds__(256) void k_update(uint32_t n_nodes, T *__restrict_
             acc[2] = fmax(acc[2], fabs((double)rv[d]));
         }
     }
-    double out[3];
-    if (!grid_reduce<3, 4u>(acc, partials, &sc->counter[CTR_UPD], out)) return;
-    if (threadIdx.x == 0) {
-        sc->iter += 1;
-        sc->rho_prev = sc->rho;
-        sc->rho = out[0];
-        sc->beta = out[0] / sc->rho_prev;
-        sc->rr = out[1];
-        sc->rmax = out[2];
-        if (!finite(out[0]) || !finite(out[1])) { sc->status = 6; sc->done = 1; return; }
-        const bool conv = sc->crit_kind == 0 ? out[1] <= sc->stop2 : out[2] <= sc->crit;
-        if (sc->mixed) {
-            if (conv || out[1] <= sc->delta2 * sc->rr_ref) sc->pause = 1;
-        } else if (conv) {
-            sc->done = 1;
-        }
-        if (!sc->done && sc->iter >= sc->max_iter) { sc->status = 4; sc->done = 1; }
-    }
+    wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
 
-// p = M^-1 r + beta p
+// p = M^-1 r + beta p  (sel: SEL_ACTIVE in the iteration, SEL_PAUSE_DONE after a refresh)
 template <int PRE, typename T>
-__global__ __launch_bounds__(256) void k_update_p(uint32_t n_nodes, const T *__restrict__ r, T *__restrict__ p,
-                                                  const T *__restrict__ Minv, const Scalars *sc) {
-    if (sc->done | sc->pause) return;
-    const T beta = (T)sc->beta;
-    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < n_nodes; n += (size_t)gridDim.x * blockDim.x) {
-        T rv[3] = {r[3 * n], r[3 * n + 1], r[3 * n + 2]}, zv[3];
-        apply_m<PRE, T>(Minv, n, rv, zv);
+__global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ r, T *__restrict__ p,
+                                                  const T *__restrict__ Minv, WgTables wt, const ProbScalars *S,
+                                                  int sel) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], sel)) return;
+    const T beta = (T)S[s].beta;
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n >= wt.row1[L]) return;
+    T rv[3] = {r[3 * (size_t)n], r[3 * (size_t)n + 1], r[3 * (size_t)n + 2]}, zv[3];
+    apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) p[3 * n + d] = zv[d] + beta * p[3 * n + d];
-    }
+    for (int d = 0; d < 3; ++d) p[3 * (size_t)n + d] = zv[d] + beta * p[3 * (size_t)n + d];
 }
 
-// Start (or restart after an FP64 residual refresh) of the search direction from an FP64
-// residual r64:  r = (T) r64; z = M^-1 r; first: p = z, rho = r.z;  refresh: rho = r.z,
-// beta = rho / rho_prev and the pause is lifted (p is then updated by k_update_p).
+// Start (first = 1, every unfinished sub-problem) or restart after an FP64 residual refresh
+// (first = 0, sub-problems with a pending refresh) of the recurrence from the FP64 residual r64:
+// r = (T) r64; z = M^-1 r; first: p = z, rho = r.z;  refresh: rho = r.z, beta = rho / rho_prev,
+// pause -> PAUSE_DONE (the direction update follows).
 template <int PRE, typename T>
-__global__ __launch_bounds__(256) void k_restart(uint32_t n_nodes, const double *r64, T *r,
-                                                 T *__restrict__ p, const T *__restrict__ Minv, Scalars *sc,
+__global__ __launch_bounds__(256) void k_restart(const double *r64, T *r, T *__restrict__ p,
+                                                 const T *__restrict__ Minv, WgTables wt, const ProbScalars *S,
                                                  double *partials, int first) {
-    if (sc->done) return;
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], first ? (int)SEL_ALL : (int)SEL_PAUSE_REQ)) return;
     double acc[1] = {0};
-    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < n_nodes; n += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
         T rv[3], zv[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { rv[d] = (T)r64[3 * n + d]; r[3 * n + d] = rv[d]; }
+        for (int d = 0; d < 3; ++d) { rv[d] = (T)r64[3 * (size_t)n + d]; r[3 * (size_t)n + d] = rv[d]; }
         apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (first) p[3 * n + d] = zv[d];
+            if (first) p[3 * (size_t)n + d] = zv[d];
             acc[0] += (double)rv[d] * (double)zv[d];
         }
     }
-    double out[1];
-    if (!grid_reduce<1, 0u>(acc, partials, &sc->counter[CTR_REFRESH], out)) return;
-    if (threadIdx.x == 0) {
-        if (first) {
-            sc->rho = out[0];
-        } else {
-            sc->rho = out[0];
-            sc->beta = out[0] / sc->rho_prev;
+    wg_partial<1, 0u>(acc, partials, gridDim.x, L);
+}
+
+// Finalize of a row kernel's partials, one workgroup per sub-problem (grid = n_sub); selects the
+// same sub-problems as the row kernel it follows (their flags are unchanged in between).
+enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5 };
+
+__global__ __launch_bounds__(256) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
+                                                  unsigned stride, const uint32_t *__restrict__ sub_wg0,
+                                                  const uint32_t *__restrict__ sub_nwg, int sel) {
+    const uint32_t s = blockIdx.x;
+    ProbScalars &P = S[s];
+    const int esel = what == FIN_PQ || what == FIN_UPD ? (int)SEL_ACTIVE
+                     : what == FIN_RESTART_FIRST       ? (int)SEL_ALL
+                     : what == FIN_RESTART             ? (int)SEL_PAUSE_REQ
+                                                       : sel;
+    if (!selected(P, esel)) return;
+    const unsigned w0 = sub_wg0[s], nw = sub_nwg[s];
+    if (what == FIN_PQ) {
+        double o[1];
+        wg_combine<1, 0u>(partials, stride, w0, nw, o);
+        if (threadIdx.x) return;
+        P.pq = o[0];
+        if (!finite(o[0])) { P.status = 6; P.done = 1; }
+        else if (!(o[0] > 0)) { P.status = 3; P.done = 1; }
+        else P.alpha = P.rho / o[0];
+    } else if (what == FIN_UPD) {
+        double o[3];
+        wg_combine<3, 4u>(partials, stride, w0, nw, o);
+        if (threadIdx.x) return;
+        P.iter += 1;
+        P.rho_prev = P.rho;
+        P.rho = o[0];
+        P.beta = o[0] / P.rho_prev;
+        P.rr = o[1];
+        P.rmax = o[2];
+        if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
+        const bool conv = P.crit_kind == 0 ? o[1] <= P.stop2 : o[2] <= P.crit;
+        if (P.mixed) {
+            if (conv || o[1] <= P.delta2 * P.rr_ref) P.pause = PAUSE_REQ;
+        } else if (conv) {
+            P.done = 1;
         }
-        if (!finite(out[0])) { sc->status = 6; sc->done = 1; }
-        sc->pause = 0;
+        if (!P.done && P.iter >= P.max_iter) { P.status = 4; P.done = 1; }
+    } else if (what == FIN_RES || what == FIN_RES_CHECK) {
+        double o[2];
+        wg_combine<2, 2u>(partials, stride, w0, nw, o);
+        if (threadIdx.x) return;
+        P.rr = o[0];
+        P.rmax = o[1];
+        if (what == FIN_RES_CHECK) {
+            P.rr_ref = o[0];
+            if (!finite(o[0])) { P.status = 6; P.done = 1; }
+            else if (P.crit_kind == 0 ? o[0] <= P.stop2 : o[1] <= P.crit) P.done = 1;
+        }
+    } else {
+        double o[1];
+        wg_combine<1, 0u>(partials, stride, w0, nw, o);
+        if (threadIdx.x) return;
+        P.rho = o[0];
+        if (what == FIN_RESTART) {
+            P.beta = o[0] / P.rho_prev;
+            P.pause = PAUSE_DONE;
+            P.refreshes += 1;
+        }
+        if (!finite(o[0])) { P.status = 6; P.done = 1; }
+    }
+}
+
+__global__ void k_clear_pause(ProbScalars *S, uint32_t n_sub) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_sub && S[s].pause == PAUSE_DONE) S[s].pause = PAUSE_NONE;
+}
+
+// x64 += x32; x32 = 0 on the selected sub-problems
+__global__ __launch_bounds__(256) void k_accumulate(double *__restrict__ x, float *__restrict__ x32, WgTables wt,
+                                                    const ProbScalars *S, int sel) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], sel)) return;
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n >= wt.row1[L]) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[3 * (size_t)n + d] += (double)x32[3 * (size_t)n + d];
+        x32[3 * (size_t)n + d] = 0.0f;
     }
 }
 
@@ -275,44 +270,39 @@ __global__ __launch_bounds__(256) void k_zero_f32(size_t n, float *__restrict__ 
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = 0.0f;
 }
 
-// x64 += x32; x32 = 0
-__global__ __launch_bounds__(256) void k_accumulate(size_t n, double *__restrict__ x, float *__restrict__ x32) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        x[i] += (double)x32[i];
-        x32[i] = 0.0f;
-    }
-}
-
-// Jacobi factors from the diagonal blocks; a non-positive pivot marks the matrix NOT_SPD.
+// Jacobi factors from the diagonal blocks; a non-positive pivot marks the sub-problem NOT_SPD.
 template <int PRE>
-__global__ __launch_bounds__(256) void k_build_minv(uint32_t n_nodes, const uint32_t *__restrict__ diag,
-                                                    const double *__restrict__ Kv, uint64_t ps, double *__restrict__ M,
-                                                    float *__restrict__ M32, Scalars *sc) {
-    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < n_nodes; n += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t p = diag[n];
-        double a[9];
+__global__ __launch_bounds__(256) void k_build_minv(const uint32_t *__restrict__ diag, const double *__restrict__ Kv,
+                                                    uint64_t ps, double *__restrict__ M, float *__restrict__ M32,
+                                                    WgTables wt, ProbScalars *S) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB) return;
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n >= wt.row1[L]) return;
+    const uint64_t p = diag[n];
+    double a[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) a[c] = Kv[c * ps + p];
-        if (PRE == 0) {
-            for (int d = 0; d < 3; ++d) {
-                const double v = a[4 * d];
-                if (!(v > 0)) sc->status = 3;
-                M[3 * n + d] = 1.0 / v;
-                if (M32) M32[3 * n + d] = (float)(1.0 / v);
-            }
-        } else {
-            const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
-            const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
-            if (!(det > 0) || !(a[0] > 0)) sc->status = 3;
-            const double id = 1.0 / det;
-            double m[9];
-            m[0] = c00 * id; m[3] = c01 * id; m[6] = c02 * id;
-            m[1] = (a[2] * a[7] - a[1] * a[8]) * id; m[4] = (a[0] * a[8] - a[2] * a[6]) * id; m[7] = (a[1] * a[6] - a[0] * a[7]) * id;
-            m[2] = (a[1] * a[5] - a[2] * a[4]) * id; m[5] = (a[2] * a[3] - a[0] * a[5]) * id; m[8] = (a[0] * a[4] - a[1] * a[3]) * id;
-            for (int c = 0; c < 9; ++c) {
-                M[9 * n + c] = m[c];
-                if (M32) M32[9 * n + c] = (float)m[c];
-            }
+    for (int c = 0; c < 9; ++c) a[c] = Kv[c * ps + p];
+    if (PRE == 0) {
+        for (int d = 0; d < 3; ++d) {
+            const double v = a[4 * d];
+            if (!(v > 0)) S[s].status = 3;
+            M[3 * (size_t)n + d] = 1.0 / v;
+            if (M32) M32[3 * (size_t)n + d] = (float)(1.0 / v);
+        }
+    } else {
+        const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+        const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+        if (!(det > 0) || !(a[0] > 0)) S[s].status = 3;
+        const double id = 1.0 / det;
+        double m[9];
+        m[0] = c00 * id; m[3] = c01 * id; m[6] = c02 * id;
+        m[1] = (a[2] * a[7] - a[1] * a[8]) * id; m[4] = (a[0] * a[8] - a[2] * a[6]) * id; m[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+        m[2] = (a[1] * a[5] - a[2] * a[4]) * id; m[5] = (a[2] * a[3] - a[0] * a[5]) * id; m[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+        for (int c = 0; c < 9; ++c) {
+            M[9 * (size_t)n + c] = m[c];
+            if (M32) M32[9 * (size_t)n + c] = (float)m[c];
         }
     }
 }
@@ -321,156 +311,224 @@ __global__ __launch_bounds__(256) void k_build_minv(uint32_t n_nodes, const uint
 // host side
 // ---------------------------------------------------------------------------------------------
 
-hipError_t ensure_wgrows(Problem &P) {
-    if (P.d_wgrows && P.wgrows_nwg == P.spmv.nwg) return hipSuccess;
-    if (P.d_wgrows) { (void)hipFree(P.d_wgrows); P.d_wgrows = nullptr; }
-    std::vector<uint32_t> cut = balance_rows(P.pat.row_ptr, (uint32_t)P.spmv.nwg);
-    hipError_t e = hipMalloc(&P.d_wgrows, cut.size() * 4);
-    if (e != hipSuccess) return e;
-    e = hipMemcpy(P.d_wgrows, cut.data(), cut.size() * 4, hipMemcpyHostToDevice);
-    P.wgrows_nwg = P.spmv.nwg;
-    return e;
+static void finalize(Problem &P, int what, int sel = 0) {
+    hipLaunchKernelGGL(k_finalize, dim3(P.n_sub), dim3(256), 0, P.stream, what, P.d_ps, P.d_partials, P.n_wg, P.d_sub_wg0,
+                       P.d_sub_nwg, sel);
+}
+
+static WgTables tables(const Problem &P) {
+    return WgTables{P.d_wg_row0, P.d_wg_row1, P.d_wg_sub, P.d_sub_wg0, P.d_sub_nwg};
 }
 
 template <typename T, int MODE>
-static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int check) {
-    const dim3 g(P.spmv.nwg), blk(256);
-#define FEMGPU_SPMV_CASE(GG) \
-    hipLaunchKernelGGL((k_spmv<GG, T, MODE>), g, blk, 0, P.stream, P.d_rp, P.d_col, K, P.ps, x, b, y, P.d_wgrows, P.d_sc, P.d_partials, check)
-    switch (P.spmv.group) {
-        case 8: FEMGPU_SPMV_CASE(8); break;
-        case 16: FEMGPU_SPMV_CASE(16); break;
-        case 64: FEMGPU_SPMV_CASE(64); break;
-        default: FEMGPU_SPMV_CASE(32); break;
+static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int sel) {
+    const dim3 g(P.n_wg), blk(256);
+    const WgTables wt = tables(P);
+#define FEMGPU_SPMV_CASE(U) \
+    hipLaunchKernelGGL((k_spmv<T, MODE, U>), g, blk, 0, P.stream, P.d_rp, P.d_sptr, P.d_col, K, P.ps, x, b, y, wt, P.d_ps, P.d_partials, sel)
+    switch (P.spmv.unroll) {
+        case 1: FEMGPU_SPMV_CASE(1); break;
+        case 3: FEMGPU_SPMV_CASE(3); break;
+        default: FEMGPU_SPMV_CASE(2); break;
     }
 #undef FEMGPU_SPMV_CASE
 }
 
-static unsigned vec_grid(uint32_t n_nodes) {
-    return (unsigned)std::max<size_t>(8, std::min<size_t>(2048, (n_nodes + 255) / 256));
-}
-
-hipError_t spmv_apply(Problem &P, const double *d_x, double *d_y) {
-    hipError_t e = ensure_wgrows(P);
-    if (e != hipSuccess) return e;
-    launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, d_x, nullptr, d_y, 0);
-    return hipGetLastError();
-}
-
 double time_spmv(Problem &P, int precision, int reps) {
-    if (ensure_wgrows(P) != hipSuccess) return -1;
     if (precision == 1 && !P.k32_valid) return -1;
-    hipEventRecord(P.ev[2], P.stream);
+    (void)hipEventRecord(P.ev[2], P.stream);
     for (int i = 0; i < reps; ++i) {
         if (precision == 1) launch_spmv_t<float, MODE_PLAIN>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, 0);
         else launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, P.d_p, nullptr, P.d_q, 0);
     }
-    hipEventRecord(P.ev[3], P.stream);
-    hipEventSynchronize(P.ev[3]);
+    (void)hipEventRecord(P.ev[3], P.stream);
+    if (hipEventSynchronize(P.ev[3]) != hipSuccess) return -1;
     float ms = 0;
-    hipEventElapsedTime(&ms, P.ev[2], P.ev[3]);
+    (void)hipEventElapsedTime(&ms, P.ev[2], P.ev[3]);
     return (double)ms / reps;
 }
 
+// Picks the SpMV unroll depth by timing each variant on this matrix (FP64, 10 launches each).
+int autotune_spmv(Problem &P) {
+    double best = 1e30;
+    int pick = P.spmv.unroll;
+    for (int u : {1, 2, 3}) {
+        P.spmv.unroll = u;
+        time_spmv(P, 0, 2);
+        const double t = time_spmv(P, 0, 10);
+        if (t > 0 && t < best) { best = t; pick = u; }
+    }
+    P.spmv.unroll = pick;
+    P.spmv.tuned = true;
+    destroy_graph(P);
+    return pick;
+}
+
+void destroy_graph(Problem &P) {
+    if (P.graph_exec) (void)hipGraphExecDestroy(P.graph_exec);
+    P.graph_exec = nullptr;
+    P.graph_key = 0;
+}
+
 static int read_scalars(Problem &P) {
-    if (hipMemcpyAsync(P.h_sc, P.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, P.stream) != hipSuccess) return 5;
+    if (hipMemcpyAsync(P.h_ps, P.d_ps, sizeof(ProbScalars) * P.n_sub, hipMemcpyDeviceToHost, P.stream) != hipSuccess)
+        return 5;
     if (hipStreamSynchronize(P.stream) != hipSuccess) return 5;
     return 0;
 }
 
-template <int PRE>
-static void launch_iteration(Problem &P, bool fp32, unsigned vg) {
-    if (fp32) {
-        launch_spmv_t<float, MODE_PQ>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, 0);
-        hipLaunchKernelGGL((k_update<PRE, float>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_x32, P.d_r32, P.d_p32, P.d_q32, P.d_Minv32, P.d_sc, P.d_partials);
-        hipLaunchKernelGGL((k_update_p<PRE, float>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_r32, P.d_p32, P.d_Minv32, P.d_sc);
-    } else {
-        launch_spmv_t<double, MODE_PQ>(P, P.d_K, P.d_p, nullptr, P.d_q, 0);
-        hipLaunchKernelGGL((k_update<PRE, double>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_x, P.d_r, P.d_p, P.d_q, P.d_Minv, P.d_sc, P.d_partials);
-        hipLaunchKernelGGL((k_update_p<PRE, double>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_r, P.d_p, P.d_Minv, P.d_sc);
-    }
+static int write_scalars(Problem &P) {
+    if (hipMemcpyAsync(P.d_ps, P.h_ps, sizeof(ProbScalars) * P.n_sub, hipMemcpyHostToDevice, P.stream) != hipSuccess)
+        return 5;
+    return 0;
 }
 
 template <int PRE>
-static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut &out) {
+static void launch_iteration(Problem &P, bool fp32, double *d_x) {
+    const WgTables wt = tables(P);
+    const dim3 g(P.n_wg), blk(256);
+    if (fp32) {
+        launch_spmv_t<float, MODE_PQ>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, SEL_ACTIVE);
+        finalize(P, FIN_PQ);
+        hipLaunchKernelGGL((k_update<PRE, float>), g, blk, 0, P.stream, P.d_x32, P.d_r32, P.d_p32, P.d_q32, P.d_Minv32, wt, P.d_ps, P.d_partials);
+        finalize(P, FIN_UPD);
+        hipLaunchKernelGGL((k_update_p<PRE, float>), g, blk, 0, P.stream, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, (int)SEL_ACTIVE);
+    } else {
+        launch_spmv_t<double, MODE_PQ>(P, P.d_K, P.d_p, nullptr, P.d_q, SEL_ACTIVE);
+        finalize(P, FIN_PQ);
+        hipLaunchKernelGGL((k_update<PRE, double>), g, blk, 0, P.stream, d_x, P.d_r, P.d_p, P.d_q, P.d_Minv, wt, P.d_ps, P.d_partials);
+        finalize(P, FIN_UPD);
+        hipLaunchKernelGGL((k_update_p<PRE, double>), g, blk, 0, P.stream, P.d_r, P.d_p, P.d_Minv, wt, P.d_ps, (int)SEL_ACTIVE);
+    }
+}
+
+// One chunk of iterations: replayed from a captured hipGraph when enabled (the kernels read all
+// state from device memory, so the same graph serves every chunk of every solve on d_x).
+template <int PRE>
+static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
+    if (!P.use_graph) {
+        for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, fp32, d_x);
+        return hipGetLastError() == hipSuccess ? 0 : 5;
+    }
+    const uint64_t key = (reinterpret_cast<uint64_t>(d_x) << 8) ^ ((uint64_t)chunk << 4) ^ ((uint64_t)P.spmv.unroll << 2) ^
+                         ((uint64_t)PRE << 1) ^ (uint64_t)fp32;
+    if (!P.graph_exec || P.graph_key != key) {
+        destroy_graph(P);
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 5;
+        for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, fp32, d_x);
+        if (hipStreamEndCapture(P.stream, &g) != hipSuccess) return 5;
+        const hipError_t e = hipGraphInstantiate(&P.graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) { P.graph_exec = nullptr; return 5; }
+        P.graph_key = key;
+    }
+    return hipGraphLaunch(P.graph_exec, P.stream) == hipSuccess ? 0 : 5;
+}
+
+template <int PRE>
+static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out) {
     const bool mixed = o.precision == 1;
-    const unsigned vg = vec_grid(P.n_nodes);
     const size_t n = 3 * (size_t)P.n_nodes;
-    P.d_b = const_cast<double *>(d_rhs);
-    P.d_x = d_x;
-    // b.b on the host side of the stop test: one pass
-    Scalars s0 = {};
-    s0.crit_kind = o.crit_kind;
-    s0.crit = o.tol;
-    s0.max_iter = o.max_iter;
-    s0.mixed = mixed ? 1 : 0;
-    s0.delta2 = o.delta * o.delta;
-    if (hipMemcpyAsync(P.d_sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice, P.stream) != hipSuccess) return 5;
-    // ||b||^2 with the residual kernel on x = 0 (r = b)
-    launch_fill(P.d_q, n, 0.0, P.stream);
-    launch_spmv_t<double, MODE_RES>(P, P.d_K, P.d_q, d_rhs, P.d_r, 0);
+    const WgTables wt = tables(P);
+    const dim3 g(P.n_wg), blk(256);
+    std::memset(P.h_ps, 0, sizeof(ProbScalars) * P.n_sub);
+    for (uint32_t s = 0; s < P.n_sub; ++s) {
+        ProbScalars &S = P.h_ps[s];
+        S.crit_kind = o.crit_kind;
+        S.crit = o.tol;
+        S.max_iter = o.max_iter;
+        S.mixed = mixed ? 1 : 0;
+        S.delta2 = o.delta * o.delta;
+        S.max_diag = s < P.max_diag.size() ? P.max_diag[s] : 0.0;
+    }
+    if (write_scalars(P)) return 5;
+    // ||b||^2 per sub-problem: the residual kernel with x = 0
+    if (launch_fill(P.d_q, n, 0.0, P.stream) != hipSuccess) return 5;
+    launch_spmv_t<double, MODE_RES>(P, P.d_K, P.d_q, d_rhs, P.d_r, SEL_EVERY);
+    finalize(P, FIN_RES, SEL_EVERY);
     if (read_scalars(P)) return 5;
-    const double bb = P.h_sc->rr;
-    const double stop2 = o.tol * o.tol * (bb > 0 ? bb : 1.0);
-    P.h_sc->bb = bb;
-    P.h_sc->stop2 = stop2;
-    P.h_sc->rr = 0;
-    P.h_sc->rmax = 0;
-    if (hipMemcpyAsync(P.d_sc, P.h_sc, sizeof(Scalars), hipMemcpyHostToDevice, P.stream) != hipSuccess) return 5;
-    // preconditioner
+    std::vector<double> bb(P.n_sub);
+    for (uint32_t s = 0; s < P.n_sub; ++s) {
+        ProbScalars &S = P.h_ps[s];
+        bb[s] = S.rr;
+        S.bb = S.rr;
+        S.stop2 = o.tol * o.tol * (S.rr > 0 ? S.rr : 1.0);
+        S.rr = S.rmax = 0;
+    }
+    if (write_scalars(P)) return 5;
     if (P.minv_kind != PRE || (mixed && !P.d_Minv32)) {
-        if (mixed && !P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * P.n_nodes) != hipSuccess) return 5;
-        hipLaunchKernelGGL(k_build_minv<PRE>, dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_diag, P.d_K, P.ps, P.d_Minv,
-                           P.d_Minv32, P.d_sc);
+        if (mixed && !P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
+            return 5;
+        hipLaunchKernelGGL(k_build_minv<PRE>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
         P.minv_kind = PRE;
     }
     if (read_scalars(P)) return 5;
-    if (P.h_sc->status) { out.status = P.h_sc->status; return out.status; }
+    for (uint32_t s = 0; s < P.n_sub; ++s)
+        if (P.h_ps[s].status) {
+            P.minv_kind = -1;
+            for (uint32_t t = 0; t < P.n_sub; ++t) out[t] = SolveOut{P.h_ps[t].status, 0, 0, 0, 0, 0};
+            return P.h_ps[s].status;
+        }
 
-    hipEventRecord(P.ev[0], P.stream);
-    // r = b - K x; stop if already converged
-    launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, 1);
+    (void)hipEventRecord(P.ev[0], P.stream);
+    launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, SEL_ALL);
+    finalize(P, FIN_RES_CHECK, SEL_ALL);
     if (mixed) {
-        hipLaunchKernelGGL(k_zero_f32, dim3(vg), dim3(256), 0, P.stream, n, P.d_x32);
-        hipLaunchKernelGGL((k_restart<PRE, float>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, P.d_sc, P.d_partials, 1);
+        hipLaunchKernelGGL(k_zero_f32, dim3(std::max<size_t>(1, std::min<size_t>(8192, (n + 255) / 256))), blk, 0, P.stream, n, P.d_x32);
+        hipLaunchKernelGGL((k_restart<PRE, float>), g, blk, 0, P.stream, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, P.d_partials, 1);
     } else {
-        hipLaunchKernelGGL((k_restart<PRE, double>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_r, P.d_r, P.d_p, P.d_Minv, P.d_sc, P.d_partials, 1);
+        hipLaunchKernelGGL((k_restart<PRE, double>), g, blk, 0, P.stream, P.d_r, P.d_r, P.d_p, P.d_Minv, wt, P.d_ps, P.d_partials, 1);
     }
-    int chunk = 2, refreshes = 0;
-    for (;;) {
-        for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, mixed, vg);
-        if (hipGetLastError() != hipSuccess) return 5;
+    finalize(P, FIN_RESTART_FIRST);
+    int chunk = 2;
+    const long max_chunks = (long)o.max_iter / 2 + 1000;
+    for (long c = 0;; ++c) {
+        if (c > max_chunks) {  // cannot happen while every sub-problem honours max_iter
+            for (uint32_t s = 0; s < P.n_sub; ++s) if (!P.h_ps[s].done) P.h_ps[s].status = 4;
+            break;
+        }
+        if (run_chunk<PRE>(P, mixed, d_x, P.use_graph ? std::max(2, o.chunk) : chunk)) return 5;
         if (read_scalars(P)) return 5;
         chunk = std::min(chunk * 2, std::max(2, o.chunk));
-        if (P.h_sc->done) break;
-        if (mixed && P.h_sc->pause) {
-            // FP64 refresh: x += x32; r = b - K x (FP64); restart the FP32 recurrence from it
-            hipLaunchKernelGGL(k_accumulate, dim3(vg), dim3(256), 0, P.stream, n, d_x, P.d_x32);
-            launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, 1);
-            hipLaunchKernelGGL((k_restart<PRE, float>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, P.d_sc, P.d_partials, 0);
-            hipLaunchKernelGGL((k_update_p<PRE, float>), dim3(vg), dim3(256), 0, P.stream, P.n_nodes, P.d_r32, P.d_p32, P.d_Minv32, P.d_sc);
-            ++refreshes;
+        bool all_done = true, refresh = false;
+        for (uint32_t s = 0; s < P.n_sub; ++s) {
+            all_done &= P.h_ps[s].done != 0;
+            refresh |= !P.h_ps[s].done && P.h_ps[s].pause == PAUSE_REQ;
+        }
+        if (all_done) break;
+        if (mixed && refresh) {
+            // FP64 refresh of the paused sub-problems: x += x32; r = b - K x (FP64, stop test on
+            // the true residual); restart the FP32 recurrence from it, keeping the direction.
+            hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_PAUSE_REQ);
+            launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, SEL_PAUSE_REQ);
+            finalize(P, FIN_RES_CHECK, SEL_PAUSE_REQ);
+            hipLaunchKernelGGL((k_restart<PRE, float>), g, blk, 0, P.stream, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, P.d_partials, 0);
+            finalize(P, FIN_RESTART);
+            hipLaunchKernelGGL((k_update_p<PRE, float>), g, blk, 0, P.stream, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, (int)SEL_PAUSE_DONE);
+            hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
         }
     }
-    if (mixed) hipLaunchKernelGGL(k_accumulate, dim3(vg), dim3(256), 0, P.stream, n, d_x, P.d_x32);
-    hipEventRecord(P.ev[1], P.stream);
-    // final true residual for the report
-    launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_q, 0);
+    if (mixed) hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_EVERY);
+    (void)hipEventRecord(P.ev[1], P.stream);
+    // final true residual of every sub-problem, for the report
+    launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_q, SEL_EVERY);
+    finalize(P, FIN_RES, SEL_EVERY);
+    if (hipGetLastError() != hipSuccess) return 5;
     if (read_scalars(P)) return 5;
     float ms = 0;
-    hipEventElapsedTime(&ms, P.ev[0], P.ev[1]);
-    out.ms = ms;
-    out.iters = P.h_sc->iter;
-    out.refreshes = refreshes;
-    out.rel_res = std::sqrt(P.h_sc->rr / (bb > 0 ? bb : 1.0));
-    out.max_abs_res = P.h_sc->rmax;
-    out.status = P.h_sc->status;
-    return out.status;
+    (void)hipEventElapsedTime(&ms, P.ev[0], P.ev[1]);
+    int first_bad = 0;
+    for (uint32_t s = 0; s < P.n_sub; ++s) {
+        const ProbScalars &S = P.h_ps[s];
+        out[s] = SolveOut{S.status, S.iter, S.refreshes, std::sqrt(S.rr / (bb[s] > 0 ? bb[s] : 1.0)), S.rmax, (double)ms};
+        if (S.status && !first_bad) first_bad = S.status;
+    }
+    return first_bad;
 }
 
-int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut &out) {
-    out = SolveOut{};
-    if (ensure_wgrows(P) != hipSuccess) return 5;
+int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out) {
+    for (uint32_t s = 0; s < P.n_sub; ++s) out[s] = SolveOut{};
     if (o.precision == 1 && !P.k32_valid) {
         if (!P.d_K32 && hipMalloc(&P.d_K32, sizeof(float) * 9 * P.ps) != hipSuccess) return 5;
         if (launch_to_fp32(P) != hipSuccess) return 5;

@@ -187,6 +187,15 @@ class Formulation:
         A.check(self._lib.femgpu_time_spmv(self._h, int(precision), int(reps), C.byref(v)), "time_spmv")
         return v.value
 
+    def autotune(self):
+        """Pick the fastest block-SpMV variant for this matrix (returns the SELL unroll depth)."""
+        u = C.c_int()
+        A.check(self._lib.femgpu_autotune(self._h, C.byref(u)), "autotune")
+        return u.value
+
+    def set_graph_replay(self, enable=True):
+        A.check(self._lib.femgpu_set_graph_replay(self._h, int(bool(enable))), "set_graph_replay")
+
     # -- differentiation -----------------------------------------------------------------------
     def adjoint(self, dJdu, rel_tol=1e-12):
         dJdu = np.ascontiguousarray(dJdu, dtype=np.float64)

@@ -190,6 +190,9 @@ FEMGPU_API int femgpu_batch_assemble(femgpu_batch *b);
 FEMGPU_API int femgpu_batch_solve(femgpu_batch *b, const femgpu_solve_opts *opts, femgpu_solve_stats *per_problem,
                                   double *total_ms);
 FEMGPU_API int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, size_t n);
+/* Zero initial guess for every problem of the batch (each solve otherwise starts from the previous
+ * result, as shift() does for one formulation). */
+FEMGPU_API int femgpu_batch_clear_result(femgpu_batch *b);
 /* Device time (ms) of the last batched assembly. */
 FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
 FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);
@@ -203,6 +206,10 @@ FEMGPU_API int femgpu_get_assemble_ms(femgpu_problem *p, double *ms);
 /* Average device time (ms) of one block-SpMV launch (precision FP64 or the FP32 copy), measured
  * with HIP events around reps back-to-back launches on the problem's stream. */
 FEMGPU_API int femgpu_time_spmv(femgpu_problem *p, int precision, int reps, double *avg_ms);
+
+/* Average device time (ms) of a plain copy of n doubles (n even; 16 B per lane) over reps
+ * launches: the measured-bandwidth calibration (moves 16 n bytes per launch). */
+FEMGPU_API int femgpu_stream_copy(size_t n, int reps, double *avg_ms);
 
 /* ---- tuning (no reference counterpart: the reference has no device code) ------------------- */
 

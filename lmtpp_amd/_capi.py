@@ -89,8 +89,10 @@ _SIGS = {
     "femgpu_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_time_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "femgpu_autotune": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "femgpu_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "femgpu_set_graph_replay": (C.c_int, [_P, C.c_int]),
     "femgpu_batch_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "femgpu_batch_clear_result": (C.c_int, [_P]),
 }
 
 
@@ -114,6 +116,14 @@ def check(rc, what=""):
         lib = load()
         raise FemGpuError(rc, "%s: %s" % (what, lib.femgpu_last_error().decode()))
     return rc
+
+
+def stream_copy_gbs(n_doubles=1 << 27, reps=20):
+    """Measured copy bandwidth (GB/s, read + write bytes) of the library's plain copy kernel."""
+    lib = load()
+    ms = C.c_double()
+    check(lib.femgpu_stream_copy(n_doubles, reps, C.byref(ms)), "stream_copy")
+    return 16.0 * n_doubles / (ms.value * 1e-3) / 1e9, ms.value
 
 
 def declared_symbols():

@@ -88,6 +88,10 @@ class Batch:
             A.check(rc, "batch_solve")
         return rc == A.OK, ms.value
 
+    def clear_result(self):
+        """Zero initial guess for every problem (solves otherwise start from the previous result)."""
+        A.check(self._lib.femgpu_batch_clear_result(self._h), "batch_clear_result")
+
     def get_result(self, i):
         u = np.zeros(3 * self.sizes[i][0])
         A.check(self._lib.femgpu_batch_get_result(self._h, i, _ptr(u), u.size), "batch_get_result")

@@ -268,6 +268,27 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Plain copy, four 16-byte loads per lane issued before the four stores: the calibration kernel
+// for measured bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters (it moves exactly
+// 2 * 8 * n bytes).
+__global__ __launch_bounds__(256) void k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n2) {
+    const size_t base = blockIdx.x * (size_t)1024 + threadIdx.x;
+    double2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n2) v[k] = src[base + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n2) dst[base + 256 * k] = v[k];
+}
+
+hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t s) {
+    const size_t n2 = n / 2;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n2 + 1023) / 1024)), dim3(256), 0, s, reinterpret_cast<double2 *>(dst),
+                       reinterpret_cast<const double2 *>(src), n2);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s) {
     if (!n) return hipSuccess;
     const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);

@@ -716,6 +716,33 @@ int femgpu_time_spmv(femgpu_problem *h, int precision, int reps, double *avg_ms)
     return FEMGPU_OK;
 }
 
+int femgpu_stream_copy(size_t n, int reps, double *avg_ms) {
+    if (!avg_ms || reps < 1 || n < 2 || (n & 1)) return fail(FEMGPU_BAD_INPUT, "stream_copy: bad input");
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FEMGPU_OK;
+    if (dalloc(&a, n) != hipSuccess || dalloc(&b, n) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(FEMGPU_HIP_ERROR, "stream_copy: setup");
+    if (!rc && (launch_fill(a, n, 1.0, s) != hipSuccess || launch_copy(b, a, n, s) != hipSuccess)) rc = fail(FEMGPU_HIP_ERROR, "stream_copy: warmup");
+    if (!rc) {
+        (void)hipEventRecord(e0, s);
+        for (int i = 0; i < reps; ++i) (void)launch_copy(b, a, n, s);
+        (void)hipEventRecord(e1, s);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(FEMGPU_HIP_ERROR, "stream_copy: timing");
+        *avg_ms = (double)ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    dfree(a);
+    dfree(b);
+    return rc;
+}
+
 int femgpu_autotune(femgpu_problem *h, int *unroll) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "autotune: null");
     const int u = autotune_spmv(h->P);
@@ -781,6 +808,13 @@ int femgpu_batch_solve(femgpu_batch *b, const femgpu_solve_opts *opts, femgpu_so
 int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, size_t n) {
     if (!b) return fail(FEMGPU_BAD_INPUT, "batch_get_result: null");
     return copy_sub(b->H, i, b->H.P.d_u, u, n);
+}
+
+int femgpu_batch_clear_result(femgpu_batch *b) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_clear_result: null");
+    HIPCHK(launch_fill(b->H.P.d_u, 3 * (size_t)b->H.P.n_nodes, 0.0, b->H.P.stream));
+    HIPCHK(hipStreamSynchronize(b->H.P.stream));
+    return FEMGPU_OK;
 }
 
 int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms) {

@@ -63,12 +63,18 @@ __device__ void wg_combine(const double *partials, unsigned stride, unsigned w0,
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        double a = 0;
-        for (unsigned i = threadIdx.x; i < nw; i += blockDim.x) {
-            const double t = partials[(size_t)k * stride + w0 + i];
-            a = (MAXMASK >> k & 1) ? fmax(a, t) : a + t;
+        // four independent chains per thread (i, i+B, i+2B, i+3B), joined in a fixed order
+        const double *src = partials + (size_t)k * stride + w0;
+        const unsigned B = blockDim.x;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        unsigned i = threadIdx.x;
+        for (; i + 3 * B < nw; i += 4 * B) {
+            const double t0 = src[i], t1 = src[i + B], t2 = src[i + 2 * B], t3 = src[i + 3 * B];
+            if (MAXMASK >> k & 1) { a0 = fmax(a0, t0); a1 = fmax(a1, t1); a2 = fmax(a2, t2); a3 = fmax(a3, t3); }
+            else { a0 += t0; a1 += t1; a2 += t2; a3 += t3; }
         }
-        v[k] = a;
+        for (; i < nw; i += B) a0 = (MAXMASK >> k & 1) ? fmax(a0, src[i]) : a0 + src[i];
+        v[k] = (MAXMASK >> k & 1) ? fmax(fmax(a0, a1), fmax(a2, a3)) : (a0 + a1) + (a2 + a3);
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {

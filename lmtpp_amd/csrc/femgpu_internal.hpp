@@ -104,6 +104,7 @@ struct Problem {
 
 // ---- launchers (assemble.hip) ----
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
+hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t s);  // n even
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_constraints(Problem &P, int assemble_mat, int assemble_vec);

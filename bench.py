@@ -300,6 +300,8 @@ def main():
 
     # -- calibration (untimed) ---------------------------------------------------------------
     f.assemble()
+    unroll = f.autotune()
+    log("autotuned SpMV: %d SELL slots per loop trip" % unroll)
     f.precision, f.precond = L.FP64, L.JACOBI
     f.max_iter = 40000
     f.set_result(np.zeros(ndof))
@@ -397,7 +399,7 @@ def main():
         "iters_fp64": int(st64.iters),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": load_traffic("spmv_fp32"),
-                     "kernel": "k_spmv<32,float> (FP32 block SpMV)", "avg_launch_ms": round(t32, 4),
+                     "kernel": "k_spmv<float,0,%d> (FP32 SELL-64 block SpMV)" % unroll, "avg_launch_ms": round(t32, 4),
                      "algorithmic_bytes": b32},
         "roofline_fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                "avg_launch_ms": round(t64, 4), "algorithmic_bytes": b64,

@@ -6,77 +6,23 @@
 // stride P.ps; a block's position p is its CSR slot.  Plane-major storage makes every
 // SpMV load a contiguous run across the lanes of a row group.
 #include "device_common.hpp"
+#include "generated/elasticity_iso.inc"
 
 namespace femgpu {
 
 // ---------------------------------------------------------------------------------------------
-// Element descriptions.  Quadrature of the runtime Gauss loop (formulation.h:2917-2933) at
-// order 2*degree-2 (formal_lf/formulation.py:30-32,121-123):
-//   Tetra    1 point  w=1/6 at (1/4,1/4,1/4)                  include/mesh/element_Tetra.h:8
-//   Tetra_10 4 points w=1/24, a=0.1381966011250105, b=0.5854101966249685 (element_Tetra.h:12-15)
-//   Hexa     2x2x2    w=1/8 at 0.211324865405 / 0.788675134595 (12 digits, element_Hexa.h:12-19)
-// Shape functions: formulations/element_Tetra.py:27-30, element_Tetra_10.py:27-41,
-// element_Hexa.py:61-68.
+// Element descriptions, Hooke law, per-Gauss-point blocks and loads come from the generator
+// (lmtpp_amd/formal_hip -> generated/elasticity_iso.inc): Gauss tables of the runtime loop
+// (formulation.h:2917-2933) at order 2*degree-2, shape functions of formulations/element_*.py,
+// and the derivatives of the elasticity_iso weak form.
 // ---------------------------------------------------------------------------------------------
 template <int K> struct ET;
 template <> struct ET<TETRA> { static constexpr int NPE = 4, NGP = 1, TEAM = 16; };
 template <> struct ET<TETRA10> { static constexpr int NPE = 10, NGP = 4, TEAM = 128; };
 template <> struct ET<HEXA> { static constexpr int NPE = 8, NGP = 8, TEAM = 64; };
-
-template <int K>
-__device__ inline void gauss(int q, double xi[3], double &w) {
-    if constexpr (K == TETRA) {
-        w = 0.16666666666666666667; xi[0] = xi[1] = xi[2] = 0.25;
-    } else if constexpr (K == TETRA10) {
-        const double a = 0.1381966011250105, b = 0.5854101966249685;
-        w = 0.041666666666666666667;
-        xi[0] = q == 1 ? b : a; xi[1] = q == 2 ? b : a; xi[2] = q == 3 ? b : a;
-    } else {
-        const double lo = 0.211324865405, hi = 0.788675134595;
-        // point order (a,a,a),(b,a,a),(b,b,a),(a,b,a), then the same at z = b
-        const int bx = (q & 3) == 1 || (q & 3) == 2, by = (q & 3) >= 2, bz = q >= 4;
-        w = 0.125; xi[0] = bx ? hi : lo; xi[1] = by ? hi : lo; xi[2] = bz ? hi : lo;
-    }
-}
-
-// value and reference gradient of shape function a at xi
-template <int K>
-__device__ inline double shape(int a, const double xi[3], double d[3]) {
-    const double x = xi[0], y = xi[1], z = xi[2];
-    if constexpr (K == TETRA) {
-        d[0] = a == 0 ? -1.0 : a == 1 ? 1.0 : 0.0;
-        d[1] = a == 0 ? -1.0 : a == 2 ? 1.0 : 0.0;
-        d[2] = a == 0 ? -1.0 : a == 3 ? 1.0 : 0.0;
-        return a == 0 ? 1 - x - y - z : a == 1 ? x : a == 2 ? y : z;
-    } else if constexpr (K == TETRA10) {
-        const double L[4] = {1 - x - y - z, x, y, z};
-        // dL_m/dxi_j = -1 for m = 0, delta(m, j+1) otherwise
-        auto dL = [](int m, int j) { return m == 0 ? -1.0 : (m == j + 1 ? 1.0 : 0.0); };
-        if (a < 4) {
-            const double s = 4 * L[a] - 1;
-            for (int j = 0; j < 3; ++j) d[j] = s * dL(a, j);
-            return L[a] * (2 * L[a] - 1);
-        }
-        // nodes 4..9 on edges 01, 12, 02, 03, 13, 23
-        const int i0 = a == 4 ? 0 : a == 5 ? 1 : a == 6 ? 0 : a == 7 ? 0 : a == 8 ? 1 : 2;
-        const int i1 = a == 4 ? 1 : a == 5 ? 2 : a == 6 ? 2 : a == 7 ? 3 : a == 8 ? 3 : 3;
-        for (int j = 0; j < 3; ++j) d[j] = 4 * (dL(i0, j) * L[i1] + L[i0] * dL(i1, j));
-        return 4 * L[i0] * L[i1];
-    } else {
-        const int bx = a == 1 || a == 2 || a == 5 || a == 6, by = a == 2 || a == 3 || a == 6 || a == 7, bz = a >= 4;
-        const double X = bx ? x : 1 - x, Y = by ? y : 1 - y, Z = bz ? z : 1 - z;
-        const double sx = bx ? 1.0 : -1.0, sy = by ? 1.0 : -1.0, sz = bz ? 1.0 : -1.0;
-        d[0] = sx * Y * Z; d[1] = X * sy * Z; d[2] = X * Y * sz;
-        return X * Y * Z;
-    }
-}
-
-// Lame constants equal to the inverse of the isotropic compliance with s1 = 1/E, s2 = -nu/E,
-// s3 = (1+nu)/E on tensor shear (formal_lf/material_help.py:116-128,181-183).
-__device__ inline void lame(double E, double nu, double &lam, double &mu) {
-    lam = E * nu / ((1 + nu) * (1 - 2 * nu));
-    mu = E / (2 * (1 + nu));
-}
+static_assert(gen::vol<TETRA>::NPE == 4 && gen::vol<TETRA>::NGP == 1, "generated Tetra");
+static_assert(gen::vol<TETRA10>::NPE == 10 && gen::vol<TETRA10>::NGP == 4, "generated Tetra_10");
+static_assert(gen::vol<HEXA>::NPE == 8 && gen::vol<HEXA>::NGP == 8, "generated Hexa");
 
 // Phases A (coordinates to LDS) and B (per Gauss point gradients) shared by the assembly and
 // the adjoint-gradient kernels.  J[j][i] = dx_i/dxi_j (formal_lf/element.py:195-205), detJ
@@ -98,11 +44,11 @@ __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t 
     if (valid && t < NGP * NPE) {
         const int q = t / NPE, a = t % NPE;
         double xi[3], w;
-        gauss<K>(q, xi, w);
+        gen::vol<K>::gauss(q, xi, w);
         double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         for (int b = 0; b < NPE; ++b) {
             double d[3];
-            shape<K>(b, xi, d);
+            gen::vol<K>::shape(b, xi, d);
             for (int j = 0; j < 3; ++j)
                 for (int i = 0; i < 3; ++i) J[j][i] += d[j] * L.x[3 * b + i];
         }
@@ -120,7 +66,7 @@ __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t 
         Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
         Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
         double d[3];
-        L.N[q][a] = shape<K>(a, xi, d);
+        L.N[q][a] = gen::vol<K>::shape(a, xi, d);
         for (int j = 0; j < 3; ++j) L.g[q][a][j] = Ji[j][0] * d[0] + Ji[j][1] * d[1] + Ji[j][2] * d[2];
         if (a == 0) {
             L.wd[q] = w * det;
@@ -130,27 +76,13 @@ __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t 
     __syncthreads();
 }
 
-// 3x3 block of element nodes (a,b):
-//   K_ab(i,j) = sum_q w_q detJ_q [lambda g_a,i g_b,j + mu g_a,j g_b,i + mu delta_ij g_a.g_b]
-// (energy trace_sym_col(sigma, eps_test) with tensor shear, include/codegen/__init__.py:23-42,
-// 82-87).  Products are formed so that lane (b,a) computes the bitwise transpose of (a,b).
+// 3x3 block of element nodes (a,b): sum over Gauss points of the generated per-point block
+//   w_q detJ_q [lambda g_a,i g_b,j + mu g_a,j g_b,i + mu delta_ij g_a.g_b]   (H = lambda+2mu, lambda, 2mu)
 template <int K>
-__device__ inline void elem_block(const ElemLds<K> &L, int a, int b, double lam, double mu, double k[9]) {
+__device__ inline void elem_block(const ElemLds<K> &L, int a, int b, const double H[3], double k[9]) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) k[c] = 0;
-    for (int q = 0; q < ET<K>::NGP; ++q) {
-        const double *ga = L.g[q][a], *gb = L.g[q][b];
-        const double wd = L.wd[q];
-        const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                double v = lam * (ga[i] * gb[j]) + mu * (ga[j] * gb[i]);
-                if (i == j) v += mu * dot;
-                k[3 * i + j] += wd * v;
-            }
-    }
+    for (int q = 0; q < ET<K>::NGP; ++q) gen::block(L.g[q][a], L.g[q][b], H, L.wd[q], k);
 }
 
 // One colour of the element loop (formulation.h:626-647).  A team of ET<K>::TEAM lanes owns one
@@ -177,10 +109,10 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
     // a < b, is added at (a,b) and, transposed, at (b,a), so the stored matrix is exactly symmetric.
     if (do_mat && t < NPE * NPE && t / NPE <= t % NPE) {
         const int a = t / NPE, b = t % NPE;
-        double lam, mu;
-        lame(E[e], m.nu, lam, mu);
+        double H[3];
+        gen::hooke(E[e], m.nu, H);
         double k[9];
-        elem_block<K>(L, a, b, lam, mu, k);
+        elem_block<K>(L, a, b, H, k);
         const uint64_t p = slot[(size_t)e * NPE * NPE + t];
         if (a == b) {
             k[3] = k[1]; k[6] = k[2]; k[7] = k[5];
@@ -197,10 +129,9 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
     if (do_vec && t < NPE * 3) {
         // F_a = sum_q w_q detJ_q N_a(q) (density f_vol) (formulation_elasticity_iso.py:52)
         const int a = t / 3, d = t % 3;
-        const double f = d == 0 ? m.bf0 : d == 1 ? m.bf1 : m.bf2;
-        double s = 0;
-        for (int q = 0; q < NGP; ++q) s += L.wd[q] * L.N[q][a] * f;
-        F[3 * (size_t)conn[(size_t)e * NPE + a] + d] += s;
+        double f3[3] = {0, 0, 0};
+        for (int q = 0; q < NGP; ++q) gen::load(L.N[q][a], m.bf0, m.bf1, m.bf2, L.wd[q], f3);
+        F[3 * (size_t)conn[(size_t)e * NPE + a] + d] += f3[d];
     }
 }
 
@@ -211,8 +142,8 @@ template <int FK>
 __global__ __launch_bounds__(256) void k_face_load(const uint32_t *__restrict__ order, uint32_t count,
                                                    const uint32_t *__restrict__ faces, const double *__restrict__ trac,
                                                    const double *__restrict__ coords, double *__restrict__ F) {
-    constexpr int NPF = FK == TRI3 ? 3 : FK == TRI6 ? 6 : 4;
-    constexpr int NG = FK == TRI3 ? 1 : FK == TRI6 ? 3 : 4;
+    using FE = gen::face<FK>;
+    constexpr int NPF = FE::NPE;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const uint32_t f = order[i];
@@ -224,34 +155,21 @@ __global__ __launch_bounds__(256) void k_face_load(const uint32_t *__restrict__ 
     }
     const double t0 = trac[3 * (size_t)f], t1 = trac[3 * (size_t)f + 1], t2 = trac[3 * (size_t)f + 2];
     double acc[NPF] = {};
-    for (int g = 0; g < NG; ++g) {
-        double u, v, w;
-        if constexpr (FK == TRI3) { u = v = 1.0 / 3.0; w = 0.5; }
-        else if constexpr (FK == TRI6) { u = g == 2 ? 0.0 : 0.5; v = g == 0 ? 0.0 : 0.5; w = 1.0 / 6.0; }
-        else {
-            const double lo = (1.0 - sqrt(1.0 / 3.0)) / 2.0, hi = (1.0 + sqrt(1.0 / 3.0)) / 2.0;
-            u = (g & 1) ? lo : hi; v = (g & 2) ? lo : hi; w = 0.25;
-        }
+    for (int g = 0; g < FE::NGP; ++g) {
+        double uv[2], w;
+        FE::gauss(g, uv, w);
         double N[NPF], du[NPF], dv[NPF];
-        if constexpr (FK == TRI3) {
-            N[0] = 1 - u - v; N[1] = u; N[2] = v;
-            du[0] = -1; du[1] = 1; du[2] = 0; dv[0] = -1; dv[1] = 0; dv[2] = 1;
-        } else if constexpr (FK == TRI6) {  // formulations/element_Triangle_6.py:38-48
-            const double L0 = 1 - u - v;
-            N[0] = (2 * L0 - 1) * L0; N[1] = (2 * u - 1) * u; N[2] = (2 * v - 1) * v;
-            N[3] = 4 * L0 * u; N[4] = 4 * u * v; N[5] = 4 * v * L0;
-            du[0] = -(4 * L0 - 1); du[1] = 4 * u - 1; du[2] = 0; du[3] = 4 * (L0 - u); du[4] = 4 * v; du[5] = -4 * v;
-            dv[0] = -(4 * L0 - 1); dv[1] = 0; dv[2] = 4 * v - 1; dv[3] = -4 * u; dv[4] = 4 * u; dv[5] = 4 * (L0 - v);
-        } else {  // formulations/element_Quad.py:22-25
-            N[0] = (1 - u) * (1 - v); N[1] = u * (1 - v); N[2] = u * v; N[3] = (1 - u) * v;
-            du[0] = -(1 - v); du[1] = 1 - v; du[2] = v; du[3] = -v;
-            dv[0] = -(1 - u); dv[1] = -u; dv[2] = u; dv[3] = 1 - u;
+        for (int a = 0; a < NPF; ++a) {
+            double d[2];
+            N[a] = FE::shape(a, uv, d);
+            du[a] = d[0];
+            dv[a] = d[1];
         }
         double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
         for (int a = 0; a < NPF; ++a)
             for (int d = 0; d < 3; ++d) { a0[d] += du[a] * x[a][d]; a1[d] += dv[a] * x[a][d]; }
         const double n0 = a0[1] * a1[2] - a0[2] * a1[1], n1 = a0[2] * a1[0] - a0[0] * a1[2], n2 = a0[0] * a1[1] - a0[1] * a1[0];
-        const double ds = w * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        const double ds = w * sqrt(n0 * n0 + n1 * n1 + n2 * n2);   // |dS| of the skin element
         for (int a = 0; a < NPF; ++a) acc[a] += ds * N[a];
     }
     for (int a = 0; a < NPF; ++a) {
@@ -422,10 +340,10 @@ __global__ __launch_bounds__(256) void k_elem_grad(uint32_t n_elem, const uint32
     double s = 0;
     if (valid && t < NPE * NPE) {
         const int a = t / NPE, b = t % NPE;
-        double l1, m1;
-        lame(1.0, mat[elem_sub[e]].nu, l1, m1);
+        double H[3];
+        gen::hooke(1.0, mat[elem_sub[e]].nu, H);
         double k[9];
-        elem_block<K>(L, a, b, l1, m1, k);
+        elem_block<K>(L, a, b, H, k);
         const uint32_t na = conn[(size_t)e * NPE + a], nb = conn[(size_t)e * NPE + b];
         for (int i = 0; i < 3; ++i) {
             const double ku = k[3 * i] * u[3 * (size_t)nb] + k[3 * i + 1] * u[3 * (size_t)nb + 1] + k[3 * i + 2] * u[3 * (size_t)nb + 2];

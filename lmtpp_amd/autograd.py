@@ -6,6 +6,9 @@ the element kernel dJ/dE_e = -lambda_e^T (dK_e/dE_e) u_e (femgpu_adjoint_device)
 only.  The reference identifies parameters with forward sensitivities, one solve per parameter
 (FormulationFit.h:19-85); the adjoint needs one solve for all of them.  The penalty scale
 max|diag K| is held fixed in the adjoint.
+
+Import order: torch (this module imports it) must be loaded before libfemgpu.so, so that the
+library binds to torch's HIP runtime; a process holds one HIP runtime.
 """
 import ctypes as C
 

@@ -43,7 +43,16 @@ typedef enum { FEMGPU_TETRA = 0, FEMGPU_TETRA10 = 1, FEMGPU_HEXA = 2 } femgpu_el
 typedef enum { FEMGPU_TRI3 = 0, FEMGPU_TRI6 = 1, FEMGPU_QUAD4 = 2 } femgpu_face_kind;
 
 typedef enum { FEMGPU_FP64 = 0, FEMGPU_MIXED = 1 } femgpu_precision;
-typedef enum { FEMGPU_JACOBI = 0, FEMGPU_BLOCK_JACOBI = 1 } femgpu_precond;
+/* FEMGPU_BLOCK_IC0: the reference's block incomplete Cholesky (chol_incomp,
+ * containers/MatWithTinyBlocks.h:426-523) in natural node order -- the reference's factor,
+ * level-scheduled; FEMGPU_BLOCK_IC0_COLOURED: the same factorisation in a multicolour node
+ * order (a different, GPU-friendly preconditioner with a few wide levels). */
+typedef enum {
+    FEMGPU_JACOBI = 0,
+    FEMGPU_BLOCK_JACOBI = 1,
+    FEMGPU_BLOCK_IC0 = 2,
+    FEMGPU_BLOCK_IC0_COLOURED = 3
+} femgpu_precond;
 /* stop tests: relative L2 residual, or the reference's max_i |r_i| <= crit
  * (containers/MatWithTinyBlocks.h:1115,1136) */
 typedef enum { FEMGPU_REL_L2 = 0, FEMGPU_ABS_MAXNORM = 1 } femgpu_crit_kind;
@@ -140,6 +149,14 @@ FEMGPU_API int femgpu_get_sollicitation(femgpu_problem *p, double *F, size_t n);
 FEMGPU_API int femgpu_get_matrix(femgpu_problem *p, uint32_t *row_ptr, uint32_t *col, double *vals);
 FEMGPU_API int femgpu_get_sizes(femgpu_problem *p, uint32_t *n_nodes, uint32_t *n_elem, uint64_t *nnzb,
                                 uint32_t *n_colors);
+/* Block IC(0) factor of the assembled K (chol_incomp, containers/MatWithTinyBlocks.h:426-523;
+ * factorised on the GPU if not current) for precond FEMGPU_BLOCK_IC0 or _COLOURED, in
+ * elimination numbering: order[t] = node eliminated t-th (n_nodes), row_ptr (n_nodes + 1) and
+ * col (nnzb) = the full pattern renumbered by t, rows sorted; L (9 nnzb, block row-major) holds
+ * L_tc for c < t, the lower-triangular diagonal factor for c = t and zeros for c > t.  Any
+ * output may be NULL; *n_levels receives the number of level sets. */
+FEMGPU_API int femgpu_get_ic0_factor(femgpu_problem *p, int precond, uint32_t *order, uint32_t *row_ptr,
+                                     uint32_t *col, double *L, uint32_t *n_levels);
 /* max|diag K| taken before the constraints (formulation.h:1029-1030). */
 FEMGPU_API int femgpu_get_max_diag(femgpu_problem *p, double *max_diag);
 
@@ -174,6 +191,13 @@ FEMGPU_API int femgpu_make_rect(int elem_kind, const double X0[3], const double 
  * call with faces == NULL to get the count. */
 FEMGPU_API int femgpu_skin_faces(int elem_kind, uint32_t n_nodes, uint32_t n_elem, const uint32_t *conn,
                                  uint32_t *faces, uint32_t *n_faces);
+
+/* Elimination order and level sets of the block IC(0) for a full 3x3-block CSR pattern (both
+ * triangles, rows sorted, diagonal present): order (n_brows), level_ptr (n_levels + 1; may be
+ * NULL: *n_levels then only receives the count; else *n_levels gives its capacity in levels).
+ * Rows of one level share no block.  No reference counterpart: chol_incomp runs row by row. */
+FEMGPU_API int femgpu_ic0_schedule(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, int precond,
+                                   uint32_t *order, uint32_t *level_ptr, uint32_t *n_levels);
 
 /* ---- batched uneven meshes ---------------------------------------------------------------- */
 

@@ -14,7 +14,7 @@ OK, BAD_INPUT, NEG_JACOBIAN, NOT_SPD, MAX_ITER, HIP_ERROR, NAN, NOT_READY = rang
 TETRA, TETRA10, HEXA = 0, 1, 2
 TRI3, TRI6, QUAD4 = 0, 1, 2
 FP64, MIXED = 0, 1
-JACOBI, BLOCK_JACOBI = 0, 1
+JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED = 0, 1, 2, 3
 REL_L2, ABS_MAXNORM = 0, 1
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
@@ -93,6 +93,8 @@ _SIGS = {
     "femgpu_set_graph_replay": (C.c_int, [_P, C.c_int]),
     "femgpu_batch_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_batch_clear_result": (C.c_int, [_P]),
+    "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
+    "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
 }
 
 
@@ -124,6 +126,24 @@ def stream_copy_gbs(n_doubles=1 << 27, reps=20):
     ms = C.c_double()
     check(lib.femgpu_stream_copy(n_doubles, reps, C.byref(ms)), "stream_copy")
     return 16.0 * n_doubles / (ms.value * 1e-3) / 1e9, ms.value
+
+
+def ic0_schedule(row_ptr, col, precond=BLOCK_IC0_COLOURED):
+    """Elimination order and level sets of the block IC(0) for a full 3x3-block CSR pattern
+    (host only).  Returns (order, level_ptr)."""
+    import numpy as np
+    lib = load()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    cl = np.ascontiguousarray(col, dtype=np.uint32)
+    n = len(rp) - 1
+    order = np.zeros(n, np.uint32)
+    nl = C.c_uint32(0)
+    check(lib.femgpu_ic0_schedule(n, rp.ctypes.data, cl.ctypes.data, precond, order.ctypes.data, None, C.byref(nl)),
+          "ic0_schedule")
+    lv = np.zeros(nl.value + 1, np.uint32)
+    check(lib.femgpu_ic0_schedule(n, rp.ctypes.data, cl.ctypes.data, precond, order.ctypes.data, lv.ctypes.data,
+                                  C.byref(nl)), "ic0_schedule")
+    return order, lv
 
 
 def declared_symbols():

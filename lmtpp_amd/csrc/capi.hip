@@ -59,6 +59,7 @@ void dfree(T *&p) {
 
 void free_problem(Problem &P) {
     destroy_graph(P);
+    ic0_free(P.ic0);
     dfree(P.d_coords); dfree(P.d_E); dfree(P.d_conn); dfree(P.d_rp); dfree(P.d_col); dfree(P.d_diag);
     dfree(P.d_slot); dfree(P.d_order); dfree(P.d_sptr); dfree(P.d_elem_sub); dfree(P.d_mat);
     dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
@@ -335,6 +336,7 @@ int assemble(femgpu_problem &H, int do_mat, int do_vec) {
         P.assembled_mat = true;
         P.k32_valid = false;
         P.minv_kind = -1;
+        P.ic0.valid = P.ic0.valid32 = false;
         for (uint32_t s = 0; s < P.n_sub; ++s) P.max_diag[s] = P.h_ps[s].max_diag;
     }
     if (do_vec) P.assembled_vec = true;
@@ -361,7 +363,7 @@ void to_stats(const SolveOut &s, femgpu_solve_stats *st) {
 
 int check_opts(const femgpu_solve_opts *o) {
     if (!o) return FEMGPU_OK;
-    if (o->precision < 0 || o->precision > 1 || o->precond < 0 || o->precond > 1 || o->crit_kind < 0 ||
+    if (o->precision < 0 || o->precision > 1 || o->precond < 0 || o->precond > 3 || o->crit_kind < 0 ||
         o->crit_kind > 1 || !(o->tol >= 0) || o->max_iter < 0 || o->chunk < 1 || !(o->reliable_delta > 0) ||
         !(o->reliable_delta < 1))
         return fail(FEMGPU_BAD_INPUT, "invalid solve options");
@@ -620,6 +622,52 @@ int femgpu_get_matrix(femgpu_problem *h, uint32_t *row_ptr, uint32_t *col, doubl
                 for (int c = 0; c < 9; ++c) vals[9 * (size_t)p + c] = planes[c * P.ps + q];
             }
     }
+    return FEMGPU_OK;
+}
+
+int femgpu_get_ic0_factor(femgpu_problem *h, int precond, uint32_t *order, uint32_t *row_ptr, uint32_t *col, double *L,
+                          uint32_t *n_levels) {
+    if (!h || (precond != FEMGPU_BLOCK_IC0 && precond != FEMGPU_BLOCK_IC0_COLOURED))
+        return fail(FEMGPU_BAD_INPUT, "get_ic0_factor: bad input");
+    Problem &P = h->P;
+    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "get_ic0_factor: assemble first");
+    const int rc = ic0_ensure(P, precond == FEMGPU_BLOCK_IC0 ? IC0_NATURAL : IC0_COLOURED, false);
+    if (rc == 3) return fail(FEMGPU_NOT_SPD, "get_ic0_factor: non-positive pivot");
+    if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR,
+                        rc == 1 ? "get_ic0_factor: pattern not symmetric" : "get_ic0_factor: factorisation failed");
+    if (n_levels) *n_levels = P.ic0.sched.n_levels();
+    if (ic0_download(P, order, row_ptr, col, L)) return fail(FEMGPU_HIP_ERROR, "get_ic0_factor: download");
+    return FEMGPU_OK;
+}
+
+int femgpu_ic0_schedule(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, int precond, uint32_t *order,
+                        uint32_t *level_ptr, uint32_t *n_levels) {
+    if (!n_brows || !row_ptr || !col || !order || !n_levels ||
+        (precond != FEMGPU_BLOCK_IC0 && precond != FEMGPU_BLOCK_IC0_COLOURED))
+        return fail(FEMGPU_BAD_INPUT, "ic0_schedule: bad input");
+    Pattern pat;
+    pat.n_rows = n_brows;
+    pat.row_ptr.assign(row_ptr, row_ptr + n_brows + 1);
+    pat.col.assign(col, col + row_ptr[n_brows]);
+    for (uint32_t r = 0; r < n_brows; ++r) {
+        bool has_diag = false;
+        for (uint32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+            if (col[p] >= n_brows || (p > row_ptr[r] && col[p] <= col[p - 1]))
+                return fail(FEMGPU_BAD_INPUT, "ic0_schedule: columns out of range or unsorted");
+            has_diag |= col[p] == r;
+        }
+        if (!has_diag) return fail(FEMGPU_BAD_INPUT, "ic0_schedule: missing diagonal block");
+    }
+    finish_pattern(pat);
+    Ic0Schedule S;
+    if (!build_ic0_schedule(pat, precond == FEMGPU_BLOCK_IC0 ? IC0_NATURAL : IC0_COLOURED, S))
+        return fail(FEMGPU_BAD_INPUT, "ic0_schedule: pattern not symmetric");
+    std::memcpy(order, S.order.data(), 4 * (size_t)n_brows);
+    if (level_ptr) {
+        if (*n_levels < S.n_levels()) return fail(FEMGPU_BAD_INPUT, "ic0_schedule: level_ptr too small");
+        std::memcpy(level_ptr, S.lv_ptr.data(), 4 * S.lv_ptr.size());
+    }
+    *n_levels = S.n_levels();
     return FEMGPU_OK;
 }
 

@@ -1,11 +1,11 @@
-// Internal structures shared by the C ABI (capi.hip, batch.hip) and the kernels (assemble.hip,
-// solve.hip).
+// Internal structures shared by the C ABI (capi.hip) and the kernels (assemble.hip, solve.hip,
+// ic0.hip).
 //
 // A Problem holds n_sub >= 1 independent elasticity systems in one device system: their nodes,
 // elements, faces and constraints are concatenated (node indices offset), the block pattern is
 // block diagonal across sub-problems, and every workgroup of the row-parallel kernels covers
-// rows of exactly one sub-problem.  Each sub-problem keeps its own CG scalars, stop test and
-// last-workgroup counters, so one launch advances every unfinished system.
+// rows of exactly one sub-problem.  Each sub-problem keeps its own CG scalars and stop test, so
+// one launch advances every unfinished system.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,19 +17,17 @@
 
 namespace femgpu {
 
-// Device-resident scalars of one sub-problem's CG.  Written by the last workgroup of that
-// sub-problem in each reducing kernel, read by the next kernel and, between chunks of
-// iterations, by the host.
+// Device-resident scalars of one sub-problem's CG.  Written by the finalize kernel that combines
+// the per-workgroup partials of a reducing kernel, read by the next kernels and, between chunks
+// of iterations, by the host.
 struct ProbScalars {
     double rho, rho_prev, pq, alpha, beta, rr, rmax;
     double bb, stop2, crit;       // stop test: rr <= stop2 (REL_L2) or rmax <= crit (ABS_MAXNORM)
     double rr_ref, delta2;        // mixed: pause for an FP64 refresh when rr <= delta2 * rr_ref
     double max_diag;              // assembly: max |diag K| of this sub-problem
     int iter, max_iter, done, pause, status, crit_kind, mixed, refreshes;
-    unsigned counter[8];          // last-workgroup counters, one per reducing kernel
 };
 
-enum { CTR_SPMV = 0, CTR_UPD = 1, CTR_RES = 2, CTR_REFRESH = 3, CTR_MAXDIAG = 4 };
 // pause states: 0 running, 1 FP64 refresh requested, 2 refreshed (direction update pending)
 enum { PAUSE_NONE = 0, PAUSE_REQ = 1, PAUSE_DONE = 2 };
 // sub-problem selectors of the refresh kernels
@@ -43,6 +41,22 @@ constexpr uint32_t NO_SUB = 0xffffffffu;
 struct SpmvConfig {
     int unroll = 2;       // SELL slots per loop trip of the block SpMV (1, 2 or 3)
     bool tuned = false;
+};
+
+// Device side of the block IC(0) preconditioner (schedule in host_mesh.hpp).
+struct Ic0Args {
+    const uint32_t *nat, *sub, *len, *nlow, *sptr, *colN, *colT, *kpos, *tpos;
+    uint64_t ps;  // plane stride of the factor values
+};
+struct Ic0 {
+    Ic0Schedule sched;
+    uint32_t *d_nat = nullptr, *d_sub = nullptr, *d_len = nullptr, *d_nlow = nullptr, *d_sptr = nullptr,
+             *d_colN = nullptr, *d_colT = nullptr, *d_kpos = nullptr, *d_tpos = nullptr;
+    double *d_L = nullptr, *d_z = nullptr;   // factor (9 planes), preconditioned residual
+    float *d_L32 = nullptr, *d_z32 = nullptr;
+    uint64_t ps = 0;
+    bool valid = false, valid32 = false;     // factor of the current K (FP64, FP32 copy)
+    Ic0Args args() const { return Ic0Args{d_nat, d_sub, d_len, d_nlow, d_sptr, d_colN, d_colT, d_kpos, d_tpos, ps}; }
 };
 
 struct Problem {
@@ -91,7 +105,7 @@ struct Problem {
     float *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_x32 = nullptr, *d_Minv32 = nullptr;
     double *d_partials = nullptr;
     ProbScalars *d_ps = nullptr, *h_ps = nullptr;
-    int *d_flags = nullptr, *h_flags = nullptr;  // [0]: element with det J <= 0
+    int *d_flags = nullptr, *h_flags = nullptr;  // [0]: element with det J <= 0; [1]: IC(0) pivot <= 0
     int minv_kind = -1;  // preconditioner currently stored in d_Minv
     SpmvConfig spmv;
     std::vector<double> max_diag;  // per sub-problem, of the last matrix assembly
@@ -100,6 +114,7 @@ struct Problem {
     bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;
+    Ic0 ic0;
 };
 
 // ---- launchers (assemble.hip) ----
@@ -119,6 +134,20 @@ int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, 
 double time_spmv(Problem &P, int precision, int reps);
 int autotune_spmv(Problem &P);
 void destroy_graph(Problem &P);
+
+// ---- block IC(0) (ic0.hip) ----
+// Schedule for `ordering` (built and uploaded when it changes) and the numeric factor of the
+// current K (FP64, plus the FP32 copy when fp32).  Returns 0, 3 (non-positive pivot: the
+// sub-problem's status is set to 3) or 5.
+int ic0_ensure(Problem &P, int ordering, bool fp32);
+// z = (L L^T)^-1 r on the sub-problems selected by sel: forward sweep level by level, backward
+// sweep in reverse level order (T = double or float).
+void ic0_apply(Problem &P, const double *r, double *z, int sel);
+void ic0_apply(Problem &P, const float *r, float *z, int sel);
+// Factor in elimination numbering: order[t] = node, full-pattern CSR over t (rows sorted), values
+// of the lower blocks and of the lower-triangular diagonal blocks, zeros in the upper blocks.
+int ic0_download(Problem &P, uint32_t *order, uint32_t *row_ptr, uint32_t *col, double *L);
+void ic0_free(Ic0 &I);
 
 std::string &last_error();
 

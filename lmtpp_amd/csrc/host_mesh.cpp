@@ -344,4 +344,123 @@ std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_
     return cut;
 }
 
+bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S) {
+    const uint32_t n = pat.n_rows;
+    const uint32_t *rp = pat.row_ptr.data(), *col = pat.col.data();
+    S = Ic0Schedule();
+    S.ordering = ordering;
+    S.n_rows = n;
+    // elimination order
+    std::vector<uint32_t> elim0(n), order0(n);
+    if (ordering == IC0_COLOURED) {
+        std::vector<uint32_t> colour(n, 0xffffffffu);
+        std::vector<uint64_t> seen;
+        uint32_t nc = 0;
+        for (uint32_t r = 0; r < n; ++r) {
+            seen.assign(nc / 64 + 1, 0);
+            for (uint32_t p = rp[r]; p < rp[r + 1]; ++p) {
+                const uint32_t c = colour[col[p]];
+                if (c != 0xffffffffu) seen[c / 64] |= 1ull << (c % 64);
+            }
+            uint32_t c = 0;
+            while (c / 64 < seen.size() && (seen[c / 64] >> (c % 64) & 1)) ++c;
+            colour[r] = c;
+            nc = std::max(nc, c + 1);
+        }
+        std::vector<uint32_t> cnt(nc + 1, 0);
+        for (uint32_t r = 0; r < n; ++r) cnt[colour[r] + 1]++;
+        for (uint32_t c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+        for (uint32_t r = 0; r < n; ++r) order0[cnt[colour[r]]++] = r;
+    } else {
+        for (uint32_t r = 0; r < n; ++r) order0[r] = r;
+    }
+    for (uint32_t i = 0; i < n; ++i) elim0[order0[i]] = i;
+    // levels in elimination order
+    std::vector<uint32_t> level(n, 0);
+    uint32_t nl = n ? 1 : 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t r = order0[i];
+        uint32_t l = 0;
+        for (uint32_t p = rp[r]; p < rp[r + 1]; ++p)
+            if (elim0[col[p]] < i) l = std::max(l, level[col[p]] + 1);
+        level[r] = l;
+        nl = std::max(nl, l + 1);
+    }
+    // rows t by (level, elimination position)
+    S.lv_ptr.assign(nl + 1, 0);
+    for (uint32_t r = 0; r < n; ++r) S.lv_ptr[level[r] + 1]++;
+    for (uint32_t l = 0; l < nl; ++l) S.lv_ptr[l + 1] += S.lv_ptr[l];
+    S.order.resize(n);
+    {
+        std::vector<uint32_t> fill(S.lv_ptr.begin(), S.lv_ptr.end() - 1);
+        for (uint32_t i = 0; i < n; ++i) S.order[fill[level[order0[i]]]++] = order0[i];
+    }
+    std::vector<uint32_t> tof(n);
+    for (uint32_t t = 0; t < n; ++t) tof[S.order[t]] = t;
+    // rows: neighbours sorted by t
+    S.len.resize(n);
+    S.nlow.resize(n);
+    const uint32_t ns = (n + 63) / 64;
+    S.sptr.assign((size_t)ns + 1, 0);
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t r = S.order[t];
+        S.len[t] = rp[r + 1] - rp[r];
+        uint32_t lo = 0;
+        for (uint32_t p = rp[r]; p < rp[r + 1]; ++p) lo += tof[col[p]] < t;
+        S.nlow[t] = lo;
+    }
+    for (uint32_t s = 0; s < ns; ++s) {
+        uint32_t w = 0;
+        for (uint32_t t = 64 * s; t < std::min(n, 64 * s + 64); ++t) w = std::max(w, S.len[t]);
+        S.sptr[s + 1] = S.sptr[s] + 64 * w;
+    }
+    S.n_sell = S.sptr[ns];
+    const size_t np = std::max<uint64_t>(S.n_sell, 1);
+    S.colN.assign(np, 0);
+    S.colT.assign(np, 0);
+    S.kpos.assign(np, 0);
+    S.tpos.assign(np, 0);
+    parallel_for(n, [&](size_t b, size_t e) {
+        std::vector<std::pair<uint32_t, uint32_t>> nb;  // (t of neighbour, CSR slot in row r)
+        for (size_t t = b; t < e; ++t) {
+            const uint32_t r = S.order[t], s = (uint32_t)t >> 6, w = (S.sptr[s + 1] - S.sptr[s]) / 64;
+            nb.clear();
+            for (uint32_t p = rp[r]; p < rp[r + 1]; ++p) nb.emplace_back(tof[col[p]], p - rp[r]);
+            std::sort(nb.begin(), nb.end());
+            for (uint32_t k = 0; k < w; ++k) {
+                const uint64_t q = S.pos((uint32_t)t, k);
+                if (k < nb.size()) {
+                    S.colT[q] = nb[k].first;
+                    S.colN[q] = S.order[nb[k].first];
+                    S.kpos[q] = (uint32_t)pat.pos(r, nb[k].second);
+                } else {  // padding: own row, never read
+                    S.colT[q] = (uint32_t)t;
+                    S.colN[q] = r;
+                    S.kpos[q] = pat.diag[r];
+                }
+            }
+        }
+    });
+    // transposed slot of every lower block: row c holds t among its upper neighbours (the
+    // pattern must be symmetric)
+    std::vector<uint8_t> bad(n, 0);
+    parallel_for(n, [&](size_t b, size_t e) {
+        for (size_t t = b; t < e; ++t)
+            for (uint32_t k = 0; k < S.nlow[t]; ++k) {
+                const uint64_t q = S.pos((uint32_t)t, k);
+                const uint32_t c = S.colT[q];
+                uint32_t lo = S.nlow[c] + 1, hi = S.len[c];
+                while (lo < hi) {
+                    const uint32_t m = (lo + hi) / 2;
+                    if (S.colT[S.pos(c, m)] < t) lo = m + 1; else hi = m;
+                }
+                if (lo == S.len[c] || S.colT[S.pos(c, lo)] != t) { bad[t] = 1; continue; }
+                S.tpos[q] = (uint32_t)S.pos(c, lo);
+            }
+    });
+    for (uint32_t t = 0; t < n; ++t)
+        if (bad[t]) return false;
+    return true;
+}
+
 }  // namespace femgpu

@@ -56,4 +56,32 @@ Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32
 // Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);
 
+// Block IC(0) schedule (containers/MatWithTinyBlocks.h:426-523 factorises in natural row order).
+// An elimination order is chosen (IC0_NATURAL: node order, the reference's factor;
+// IC0_COLOURED: first-fit colouring of the node graph, nodes sorted by colour), each node gets
+// the level 1 + max level of its neighbours eliminated before it, and rows are renumbered
+// t = 0..n-1 by (level, elimination position).  Rows of one level share no block, so a level is
+// factorised and swept in parallel; reordering rows within a level leaves the factor unchanged.
+// Factor storage is SELL-64 over t: row t holds its neighbours sorted by t -- nlow[t] strictly
+// lower blocks, the diagonal block, then the upper blocks (the upper slot (t, c) receives L_ct,
+// read transposed by the backward sweep).
+enum { IC0_NATURAL = 0, IC0_COLOURED = 1 };
+struct Ic0Schedule {
+    int ordering = -1;
+    uint32_t n_rows = 0;
+    std::vector<uint32_t> order;     // t -> node
+    std::vector<uint32_t> lv_ptr;    // level l covers rows t in [lv_ptr[l], lv_ptr[l+1])
+    std::vector<uint32_t> sptr;      // slice starts (positions), n_slices + 1
+    std::vector<uint32_t> len, nlow; // per row: stored blocks, strictly lower blocks
+    uint64_t n_sell = 0;
+    std::vector<uint32_t> colN;      // per position: neighbour node
+    std::vector<uint32_t> colT;      // per position: neighbour row t
+    std::vector<uint32_t> kpos;      // per position: SELL position of the block in K
+    std::vector<uint32_t> tpos;      // lower positions: position of the transposed (upper) slot
+    uint64_t pos(uint32_t t, uint32_t k) const { return sptr[t >> 6] + 64ull * k + (t & 63u); }
+    uint32_t n_levels() const { return lv_ptr.empty() ? 0 : (uint32_t)lv_ptr.size() - 1; }
+};
+// Returns false when the pattern is not symmetric.
+bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S);
+
 }  // namespace femgpu

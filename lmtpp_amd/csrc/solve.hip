@@ -1,16 +1,18 @@
-// Jacobi-preconditioned CG on the 3x3-block SELL-64 matrix, FP64 or FP32-inner with FP64 residual
+// Preconditioned CG on the 3x3-block SELL-64 matrix, FP64 or FP32-inner with FP64 residual
 // refreshes, for one or many independent sub-problems at once.
 //
 // The iteration is the PCG of containers/MatWithTinyBlocks.h:1101-1151 /
-// containers/conjugate_gradient.h:27-68 with the block IC(0) preconditioner replaced by
-// (block-)Jacobi:  q = K d; alpha = rho / d.q; x += alpha d; r -= alpha q; z = M^-1 r;
-// rho' = r.z; beta = rho'/rho; d = z + beta d.
+// containers/conjugate_gradient.h:27-68:  q = K d; alpha = rho / d.q; x += alpha d;
+// r -= alpha q; z = M^-1 r; rho' = r.z; beta = rho'/rho; d = z + beta d, with M = scalar or 3x3
+// block Jacobi (fused into the vector updates) or the block IC(0) factor of ic0.hip (level
+// sweeps between the update and the r.z reduction).
 //
 // Every row-parallel kernel uses the same workgroup tables: logical workgroup L covers rows
-// [wg_row0[L], wg_row1[L]) of sub-problem wg_sub[L].  Reductions are segmented by sub-problem:
-// a wave-shuffle tree, a fixed 4-wave sum, per-workgroup partials, and the last workgroup of the
-// sub-problem (agent-scope atomic counter + fences) sums that sub-problem's partials in index
-// order -- bitwise reproducible.  Workgroups of finished or paused sub-problems return at once.
+// [wg_row0[L], wg_row1[L]) of sub-problem wg_sub[L].  Reductions are segmented by sub-problem and
+// deterministic: a wave-shuffle tree and a fixed 4-wave sum give one partial per workgroup, and
+// k_finalize (one workgroup per sub-problem, launched after the row kernel) combines that
+// sub-problem's partials in a fixed order and advances its scalars -- bitwise reproducible, no
+// fences or atomics.  Workgroups of finished or paused sub-problems return at once.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -91,10 +93,14 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, c
     }
 }
 
-// z = M^-1 r for one node; PRE 0: scalar Jacobi (3 values per node), 1: 3x3 block Jacobi (9).
+// z = M^-1 r for one node; PRE 0: scalar Jacobi (3 values per node), 1: 3x3 block Jacobi (9),
+// 2: block IC(0), whose z the level sweeps have already written to the vector passed as Minv.
+enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2 };
 template <int PRE, typename T>
 __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3], T z[3]) {
-    if (PRE == 0) {
+    if (PRE == PRE_IC0) {
+        z[0] = Minv[3 * n]; z[1] = Minv[3 * n + 1]; z[2] = Minv[3 * n + 2];
+    } else if (PRE == 0) {
         z[0] = Minv[3 * n] * r[0]; z[1] = Minv[3 * n + 1] * r[1]; z[2] = Minv[3 * n + 2] * r[2];
     } else {
         const T *m = Minv + 9 * n;
@@ -105,7 +111,8 @@ __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3
 }
 
 // x += alpha p; r -= alpha q; z = M^-1 r; partials of r.z, r.r, max|r| (k_finalize FIN_UPD
-// advances the scalars and applies the stop test).
+// advances the scalars and applies the stop test).  IC(0): z needs the sweeps, so r.z is left
+// to k_dot_rz (partial 0 stays zero, FIN_UPD_IC).
 template <int PRE, typename T>
 __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
                                                 const T *__restrict__ q, const T *__restrict__ Minv, WgTables wt,
@@ -125,10 +132,10 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
             rv[d] = r[i] - alpha * q[i];
             r[i] = rv[d];
         }
-        apply_m<PRE, T>(Minv, n, rv, zv);
+        if (PRE != PRE_IC0) apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            acc[0] += (double)rv[d] * (double)zv[d];
+            if (PRE != PRE_IC0) acc[0] += (double)rv[d] * (double)zv[d];
             acc[1] += (double)rv[d] * (double)rv[d];
             acc[2] = fmax(acc[2], fabs((double)rv[d]));
         }
@@ -136,11 +143,12 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
     wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
 
-// p = M^-1 r + beta p  (sel: SEL_ACTIVE in the iteration, SEL_PAUSE_DONE after a refresh)
+// p = M^-1 r + beta p, or p = M^-1 r when first (sel: SEL_ACTIVE in the iteration,
+// SEL_PAUSE_DONE after a refresh, SEL_ALL for the first direction of IC(0))
 template <int PRE, typename T>
 __global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ r, T *__restrict__ p,
                                                   const T *__restrict__ Minv, WgTables wt, const ProbScalars *S,
-                                                  int sel) {
+                                                  int sel, int first) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB || !selected(S[s], sel)) return;
@@ -150,13 +158,14 @@ __global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ r, T *__
     T rv[3] = {r[3 * (size_t)n], r[3 * (size_t)n + 1], r[3 * (size_t)n + 2]}, zv[3];
     apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) p[3 * (size_t)n + d] = zv[d] + beta * p[3 * (size_t)n + d];
+    for (int d = 0; d < 3; ++d) p[3 * (size_t)n + d] = first ? zv[d] : zv[d] + beta * p[3 * (size_t)n + d];
 }
 
 // Start (first = 1, every unfinished sub-problem) or restart after an FP64 residual refresh
 // (first = 0, sub-problems with a pending refresh) of the recurrence from the FP64 residual r64:
 // r = (T) r64; z = M^-1 r; first: p = z, rho = r.z;  refresh: rho = r.z, beta = rho / rho_prev,
-// pause -> PAUSE_DONE (the direction update follows).
+// pause -> PAUSE_DONE (the direction update follows).  IC(0): only r = (T) r64 here; the sweeps,
+// k_dot_rz and the direction update follow.
 template <int PRE, typename T>
 __global__ __launch_bounds__(256) void k_restart(const double *r64, T *r, T *__restrict__ p,
                                                  const T *__restrict__ Minv, WgTables wt, const ProbScalars *S,
@@ -170,26 +179,47 @@ __global__ __launch_bounds__(256) void k_restart(const double *r64, T *r, T *__r
         T rv[3], zv[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) { rv[d] = (T)r64[3 * (size_t)n + d]; r[3 * (size_t)n + d] = rv[d]; }
-        apply_m<PRE, T>(Minv, n, rv, zv);
+        if (PRE != PRE_IC0) {
+            apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (first) p[3 * (size_t)n + d] = zv[d];
-            acc[0] += (double)rv[d] * (double)zv[d];
+            for (int d = 0; d < 3; ++d) {
+                if (first) p[3 * (size_t)n + d] = zv[d];
+                acc[0] += (double)rv[d] * (double)zv[d];
+            }
         }
+    }
+    if (PRE != PRE_IC0) wg_partial<1, 0u>(acc, partials, gridDim.x, L);
+}
+
+// Partials of r.z (IC(0): after the sweeps); finalized by FIN_RHO, FIN_RESTART_FIRST or
+// FIN_RESTART.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_rz(const T *__restrict__ r, const T *__restrict__ z, WgTables wt,
+                                                const ProbScalars *S, double *partials, int sel) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], sel)) return;
+    double acc[1] = {0};
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[0] += (double)r[3 * (size_t)n + d] * (double)z[3 * (size_t)n + d];
     }
     wg_partial<1, 0u>(acc, partials, gridDim.x, L);
 }
 
 // Finalize of a row kernel's partials, one workgroup per sub-problem (grid = n_sub); selects the
 // same sub-problems as the row kernel it follows (their flags are unchanged in between).
-enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5 };
+// FIN_UPD_IC: as FIN_UPD without r.z (rho_prev = rho; FIN_RHO then sets rho and beta).
+enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5,
+       FIN_UPD_IC = 6, FIN_RHO = 7 };
 
 __global__ __launch_bounds__(256) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
                                                   unsigned stride, const uint32_t *__restrict__ sub_wg0,
                                                   const uint32_t *__restrict__ sub_nwg, int sel) {
     const uint32_t s = blockIdx.x;
     ProbScalars &P = S[s];
-    const int esel = what == FIN_PQ || what == FIN_UPD ? (int)SEL_ACTIVE
+    const int esel = what == FIN_PQ || what == FIN_UPD || what == FIN_UPD_IC || what == FIN_RHO ? (int)SEL_ACTIVE
                      : what == FIN_RESTART_FIRST       ? (int)SEL_ALL
                      : what == FIN_RESTART             ? (int)SEL_PAUSE_REQ
                                                        : sel;
@@ -203,14 +233,16 @@ __global__ __launch_bounds__(256) void k_finalize(int what, ProbScalars *S, cons
         if (!finite(o[0])) { P.status = 6; P.done = 1; }
         else if (!(o[0] > 0)) { P.status = 3; P.done = 1; }
         else P.alpha = P.rho / o[0];
-    } else if (what == FIN_UPD) {
+    } else if (what == FIN_UPD || what == FIN_UPD_IC) {
         double o[3];
         wg_combine<3, 4u>(partials, stride, w0, nw, o);
         if (threadIdx.x) return;
         P.iter += 1;
         P.rho_prev = P.rho;
-        P.rho = o[0];
-        P.beta = o[0] / P.rho_prev;
+        if (what == FIN_UPD) {
+            P.rho = o[0];
+            P.beta = o[0] / P.rho_prev;
+        }
         P.rr = o[1];
         P.rmax = o[2];
         if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
@@ -237,6 +269,7 @@ __global__ __launch_bounds__(256) void k_finalize(int what, ProbScalars *S, cons
         wg_combine<1, 0u>(partials, stride, w0, nw, o);
         if (threadIdx.x) return;
         P.rho = o[0];
+        if (what == FIN_RHO) P.beta = o[0] / P.rho_prev;
         if (what == FIN_RESTART) {
             P.beta = o[0] / P.rho_prev;
             P.pause = PAUSE_DONE;
@@ -383,22 +416,55 @@ static int write_scalars(Problem &P) {
     return 0;
 }
 
-template <int PRE>
-static void launch_iteration(Problem &P, bool fp32, double *d_x) {
+// One PCG iteration on precision T (K, vectors and the Jacobi factors or IC(0) buffers of T).
+template <int PRE, typename T>
+static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, const T *Minv, T *z) {
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
-    if (fp32) {
-        launch_spmv_t<float, MODE_PQ>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, SEL_ACTIVE);
-        finalize(P, FIN_PQ);
-        hipLaunchKernelGGL((k_update<PRE, float>), g, blk, 0, P.stream, P.d_x32, P.d_r32, P.d_p32, P.d_q32, P.d_Minv32, wt, P.d_ps, P.d_partials);
-        finalize(P, FIN_UPD);
-        hipLaunchKernelGGL((k_update_p<PRE, float>), g, blk, 0, P.stream, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, (int)SEL_ACTIVE);
+    launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE);
+    finalize(P, FIN_PQ);
+    hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, q, Minv, wt, P.d_ps, P.d_partials);
+    if (PRE == PRE_IC0) {
+        finalize(P, FIN_UPD_IC);
+        ic0_apply(P, r, z, SEL_ACTIVE);
+        hipLaunchKernelGGL(k_dot_rz<T>, g, blk, 0, P.stream, r, z, wt, P.d_ps, P.d_partials, (int)SEL_ACTIVE);
+        finalize(P, FIN_RHO);
+        hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, z, wt, P.d_ps, (int)SEL_ACTIVE, 0);
     } else {
-        launch_spmv_t<double, MODE_PQ>(P, P.d_K, P.d_p, nullptr, P.d_q, SEL_ACTIVE);
-        finalize(P, FIN_PQ);
-        hipLaunchKernelGGL((k_update<PRE, double>), g, blk, 0, P.stream, d_x, P.d_r, P.d_p, P.d_q, P.d_Minv, wt, P.d_ps, P.d_partials);
         finalize(P, FIN_UPD);
-        hipLaunchKernelGGL((k_update_p<PRE, double>), g, blk, 0, P.stream, P.d_r, P.d_p, P.d_Minv, wt, P.d_ps, (int)SEL_ACTIVE);
+        hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, Minv, wt, P.d_ps, (int)SEL_ACTIVE, 0);
+    }
+}
+
+template <int PRE>
+static void launch_iteration(Problem &P, bool fp32, double *d_x) {
+    if (fp32)
+        launch_iteration_t<PRE, float>(P, P.d_K32, P.d_x32, P.d_r32, P.d_p32, P.d_q32, P.d_Minv32, P.ic0.d_z32);
+    else
+        launch_iteration_t<PRE, double>(P, P.d_K, d_x, P.d_r, P.d_p, P.d_q, P.d_Minv, P.ic0.d_z);
+}
+
+// (Re)start of the recurrence on precision T from the FP64 residual r64: first = 1 for every
+// unfinished sub-problem (p = z, rho = r.z), else for those with a pending refresh (rho = r.z,
+// beta = rho / rho_prev, p = z + beta p).
+template <int PRE, typename T>
+static void launch_restart(Problem &P, const double *r64, T *r, T *p, const T *Minv, T *z, int first) {
+    const WgTables wt = tables(P);
+    const dim3 g(P.n_wg), blk(256);
+    const int sel = first ? (int)SEL_ALL : (int)SEL_PAUSE_REQ;
+    if (PRE == PRE_IC0) {
+        if ((const void *)r64 != (const void *)r)
+            hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, wt, P.d_ps, P.d_partials, first);
+        ic0_apply(P, r, z, sel);
+        hipLaunchKernelGGL(k_dot_rz<T>, g, blk, 0, P.stream, r, z, wt, P.d_ps, P.d_partials, sel);
+        finalize(P, first ? FIN_RESTART_FIRST : FIN_RESTART);
+        hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, z, wt, P.d_ps,
+                           first ? (int)SEL_ALL : (int)SEL_PAUSE_DONE, first);
+    } else {
+        hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, wt, P.d_ps, P.d_partials, first);
+        finalize(P, first ? FIN_RESTART_FIRST : FIN_RESTART);
+        if (!first)
+            hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, Minv, wt, P.d_ps, (int)SEL_PAUSE_DONE, 0);
     }
 }
 
@@ -406,11 +472,12 @@ static void launch_iteration(Problem &P, bool fp32, double *d_x) {
 // state from device memory, so the same graph serves every chunk of every solve on d_x).
 template <int PRE>
 static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
-    if (!P.use_graph) {
+    // level-scheduled IC(0) in natural order has hundreds of sweep launches per iteration: no graph
+    if (!P.use_graph || (PRE == PRE_IC0 && P.ic0.sched.n_levels() > 64)) {
         for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, fp32, d_x);
         return hipGetLastError() == hipSuccess ? 0 : 5;
     }
-    const uint64_t key = (reinterpret_cast<uint64_t>(d_x) << 8) ^ ((uint64_t)chunk << 4) ^ ((uint64_t)P.spmv.unroll << 2) ^
+    const uint64_t key = (reinterpret_cast<uint64_t>(d_x) << 8) ^ ((uint64_t)chunk << 5) ^ ((uint64_t)P.spmv.unroll << 3) ^
                          ((uint64_t)PRE << 1) ^ (uint64_t)fp32;
     if (!P.graph_exec || P.graph_key != key) {
         destroy_graph(P);
@@ -427,7 +494,7 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
 }
 
 template <int PRE>
-static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out) {
+static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out, int ordering) {
     const bool mixed = o.precision == 1;
     const size_t n = 3 * (size_t)P.n_nodes;
     const WgTables wt = tables(P);
@@ -457,7 +524,10 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         S.rr = S.rmax = 0;
     }
     if (write_scalars(P)) return 5;
-    if (P.minv_kind != PRE || (mixed && !P.d_Minv32)) {
+    if (PRE == PRE_IC0) {
+        const int rc = ic0_ensure(P, ordering, mixed);
+        if (rc != 0 && rc != 3) return rc == 1 ? 1 : 5;  // 3: statuses are read below
+    } else if (P.minv_kind != PRE || (mixed && !P.d_Minv32)) {
         if (mixed && !P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
             return 5;
         hipLaunchKernelGGL(k_build_minv<PRE>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
@@ -476,11 +546,10 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     finalize(P, FIN_RES_CHECK, SEL_ALL);
     if (mixed) {
         hipLaunchKernelGGL(k_zero_f32, dim3(std::max<size_t>(1, std::min<size_t>(8192, (n + 255) / 256))), blk, 0, P.stream, n, P.d_x32);
-        hipLaunchKernelGGL((k_restart<PRE, float>), g, blk, 0, P.stream, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, P.d_partials, 1);
+        launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, P.ic0.d_z32, 1);
     } else {
-        hipLaunchKernelGGL((k_restart<PRE, double>), g, blk, 0, P.stream, P.d_r, P.d_r, P.d_p, P.d_Minv, wt, P.d_ps, P.d_partials, 1);
+        launch_restart<PRE, double>(P, P.d_r, P.d_r, P.d_p, P.d_Minv, P.ic0.d_z, 1);
     }
-    finalize(P, FIN_RESTART_FIRST);
     int chunk = 2;
     const long max_chunks = (long)o.max_iter / 2 + 1000;
     for (long c = 0;; ++c) {
@@ -503,9 +572,7 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_PAUSE_REQ);
             launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, SEL_PAUSE_REQ);
             finalize(P, FIN_RES_CHECK, SEL_PAUSE_REQ);
-            hipLaunchKernelGGL((k_restart<PRE, float>), g, blk, 0, P.stream, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, P.d_partials, 0);
-            finalize(P, FIN_RESTART);
-            hipLaunchKernelGGL((k_update_p<PRE, float>), g, blk, 0, P.stream, P.d_r32, P.d_p32, P.d_Minv32, wt, P.d_ps, (int)SEL_PAUSE_DONE);
+            launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, P.ic0.d_z32, 0);
             hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
         }
     }
@@ -534,7 +601,12 @@ int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, 
         if (launch_to_fp32(P) != hipSuccess) return 5;
         P.k32_valid = true;
     }
-    return o.precond == 1 ? solve_impl<1>(P, d_rhs, d_x, o, out) : solve_impl<0>(P, d_rhs, d_x, o, out);
+    switch (o.precond) {
+        case 1: return solve_impl<PRE_BLOCK_JACOBI>(P, d_rhs, d_x, o, out, 0);
+        case 2: return solve_impl<PRE_IC0>(P, d_rhs, d_x, o, out, IC0_NATURAL);
+        case 3: return solve_impl<PRE_IC0>(P, d_rhs, d_x, o, out, IC0_COLOURED);
+        default: return solve_impl<PRE_JACOBI>(P, d_rhs, d_x, o, out, 0);
+    }
 }
 
 }  // namespace femgpu

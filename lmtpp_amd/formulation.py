@@ -172,6 +172,21 @@ class Formulation:
         A.check(self._lib.femgpu_get_matrix(self._h, _ptr(rp), _ptr(col), _ptr(vals)), "get_mat")
         return rp, col, vals
 
+    def get_ic0_factor(self, precond=A.BLOCK_IC0):
+        """Block IC(0) factor of the assembled K (MatWithTinyBlocks::chol_incomp,
+        containers/MatWithTinyBlocks.h:426-523), factorised on the GPU, in elimination numbering:
+        (order, row_ptr, col, L[nnzb,3,3], n_levels); L holds L_tc for c < t, the lower-triangular
+        diagonal factor for c = t and zeros above."""
+        nnzb = self.sizes()["nnzb"]
+        order = np.zeros(self.n_nodes, dtype=np.uint32)
+        rp = np.zeros(self.n_nodes + 1, dtype=np.uint32)
+        col = np.zeros(nnzb, dtype=np.uint32)
+        L = np.zeros((nnzb, 3, 3))
+        nl = C.c_uint32()
+        A.check(self._lib.femgpu_get_ic0_factor(self._h, int(precond), _ptr(order), _ptr(rp), _ptr(col), _ptr(L),
+                                                C.byref(nl)), "get_ic0_factor")
+        return order, rp, col, L, nl.value
+
     def max_diag(self):
         v = C.c_double()
         A.check(self._lib.femgpu_get_max_diag(self._h, C.byref(v)), "max_diag")

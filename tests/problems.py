@@ -66,3 +66,23 @@ def config0():
 def config1(n=100, seed=0):
     c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
     return L.HEXA, c, e, lognormal_E(len(e), 0.5, seed)
+
+
+def permute_bsr(row_ptr, col, vals, order):
+    """Full 3x3-block CSR renumbered by an elimination order (order[t] = node): row t holds the
+    blocks of row order[t], columns renumbered and sorted."""
+    n = len(row_ptr) - 1
+    order = np.asarray(order, dtype=np.int64)
+    tof = np.empty(n, dtype=np.int64)
+    tof[order] = np.arange(n)
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    cols, vs = [], []
+    for t in range(n):
+        r = order[t]
+        sl = slice(int(row_ptr[r]), int(row_ptr[r + 1]))
+        c = tof[col[sl]]
+        k = np.argsort(c, kind="stable")
+        cols.append(c[k])
+        vs.append(vals[sl][k])
+        rp[t + 1] = rp[t] + len(k)
+    return rp, np.concatenate(cols).astype(np.uint32), np.ascontiguousarray(np.concatenate(vs))

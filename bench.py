@@ -1,11 +1,13 @@
 """Benchmark of the MI355X assemble + PCG path on Hexa8 100^3 elasticity (3,090,903 DOF).
 
 One step = assembly of K and F on the GPU from the device-resident mesh and material, then the
-mixed-precision (FP32 inner / FP64 refreshed) Jacobi-PCG solve from a zero initial guess to the
-relative residual that was calibrated, before timing, to give <= 1e-8 relative L2 error against
-an FP64 solve converged to 1e-13.  ``value`` is DOF / step seconds (MDOF/s) of the whole step;
-the FP64-only solve, the assembly rate, the SpMV roofline and a CPU baseline (the oracle on the
-host cores, bounded sample) are reported beside it.
+mixed-precision (FP32 inner / FP64 refreshed) block-Jacobi PCG solve from a zero initial guess to
+the relative residual that was calibrated, before timing, to give <= 1e-8 relative L2 error
+against an FP64 solve converged to 1e-13.  ``value`` is DOF / step seconds (MDOF/s) of the whole
+step; the FP64-only solve with the same preconditioner, the assembly rate, the SpMV roofline, the
+other preconditioners (scalar Jacobi, block IC(0) in the reference's natural order and in
+multicolour order), the after-solve recovery and a CPU baseline (the oracle on the host cores,
+bounded sample) are reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100]
 """
@@ -47,7 +49,7 @@ def load_traffic(name):
 
 def cpu_baseline(fp64_iters, ndof, n_elem, sample_n=64):
     """Oracle (plain C restatement, one host core) on a bounded sample of the same workload: full
-    assembly of a Hexa8 sample_n^3 cube and 100 Jacobi-PCG iterations on it, scaled per element
+    assembly of a Hexa8 sample_n^3 cube and 100 block-Jacobi PCG iterations on it, scaled per element
     and per DOF-iteration to the full cube and to the FP64 iteration count the GPU needed."""
     from oracle import oracle as O
     import lmtpp_amd as L
@@ -63,13 +65,13 @@ def cpu_baseline(fp64_iters, ndof, n_elem, sample_n=64):
     t_asm = time.perf_counter() - t0
     its = 100
     t0 = time.perf_counter()
-    O.pcg_jacobi(rp, col, vals, F, rel_tol=0.0, max_iter=its)
+    O.pcg_jacobi(rp, col, vals, F, rel_tol=0.0, max_iter=its, block=True)
     t_it = (time.perf_counter() - t0) / its
     s_ndof = 3 * len(c)
     t_total = t_asm * n_elem / len(e) + t_it * (ndof / s_ndof) * fp64_iters
     return {"value": round(ndof / t_total / 1e6, 5), "unit": "MDOF/s", "cores": 1, "kind": "port",
             "sample": ("oracle/fem_oracle.c (gcc -O2, 1 thread) on Hexa8 %d^3 (%d elements, %d DOF): assembly %.2f s and "
-                       "%.4f s per Jacobi-PCG iteration (mean of %d); scaled to %d elements and %d DOF x %d FP64 iterations "
+                       "%.4f s per block-Jacobi PCG iteration (mean of %d); scaled to %d elements and %d DOF x %d FP64 iterations "
                        "(the count the GPU needed for 1e-8) = %.1f s" % (sample_n, len(e), s_ndof, t_asm, t_it, its, n_elem,
                                                                        ndof, fp64_iters, t_total)),
             "est_total_s": round(t_total, 2)}
@@ -119,12 +121,22 @@ def config0():
     F = f.get_sollicitation()
     f.precision = L.FP64
     f.set_result(np.zeros_like(F))
-    ok = f.solve_system(crit=1e-10 * np.abs(F).max())   # the reference's stop test
+    crit = 1e-10 * np.abs(F).max()
+    ok = f.solve_system(crit=crit)   # the reference's stop test
     st = f.last_stats
+    # the reference's own preconditioner: block IC(0) in natural order (its factor, on the GPU)
+    f.precond = L.BLOCK_IC0
+    f.set_result(np.zeros_like(F))
+    ok_ic = f.solve_system(crit=crit)
+    st_ic = f.last_stats
+    setup_ic = f.precond_setup_ms()
     return {"workload": "Tetra4 11^3 points (6000 elements, 3993 DOF), E=210000, nu=0.3, clamp x=0, traction x=1; "
-                        "FP64 Jacobi-PCG with the reference stop test max|r| <= 1e-10 max|F|",
+                        "FP64 PCG with the reference stop test max|r| <= 1e-10 max|F|",
             "ok": bool(ok), "iters": st.iters, "solve_ms": round(st.solve_ms, 3), "assembly_ms": round(f.assemble_ms(), 4),
-            "max_abs_res": st.max_abs_res, "reference_ic0_pcg_iters": 47}
+            "max_abs_res": st.max_abs_res,
+            "block_ic0_natural": {"ok": bool(ok_ic), "iters": st_ic.iters, "solve_ms": round(st_ic.solve_ms, 3),
+                                  "factor_ms": round(setup_ic, 3), "max_abs_res": st_ic.max_abs_res},
+            "reference_ic0_pcg_iters": 47}
 
 
 def config2():
@@ -139,16 +151,23 @@ def config2():
     for _ in range(3):
         f.assemble()
         asm.append(f.assemble_ms())
-    u_ref, tol, mtol, (st64, err64), (stm, errm), _ = calibrated_solves(f, ndof, "cfg2")
+    f.precond = L.JACOBI
+    u_ref, tol, mtol, (st64, err64), (stm, errm), solve = calibrated_solves(f, ndof, "cfg2")
+    # block Jacobi at the same tolerances (fewer FP64 iterations, more FP32 ones on this mesh)
+    f.precond = L.BLOCK_JACOBI
+    _, bst64, berr64 = solve(L.FP64, tol)
+    _, bstm, berrm = solve(L.MIXED, mtol)
     a = float(np.median(asm))
     return {"workload": "Tetra_10 40^3 cells x 6 (384,000 elements, %d DOF), interior corner nodes jittered +-0.2h, "
-                        "E=210000, nu=0.3, body force (0,0,-9.81*7.8e-9) + traction" % ndof,
+                        "E=210000, nu=0.3, body force (0,0,-9.81*7.8e-9) + traction; Jacobi PCG (block Jacobi beside)" % ndof,
             "elements": sz["n_elem"], "dof": ndof, "blocks": sz["nnzb"], "assembly_ms": round(a, 3),
             "assembly_melem_s": round(sz["n_elem"] / (a * 1e-3) / 1e6, 2),
             "fp64_solve_ms": round(st64.solve_ms, 2), "fp64_iters": st64.iters, "rel_err_fp64": err64,
             "mixed_solve_ms": round(stm.solve_ms, 2), "mixed_iters": stm.iters, "rel_err_mixed": errm,
             "mixed_vs_fp64_speedup": round(st64.solve_ms / stm.solve_ms, 3),
-            "solve_mdof_s_mixed": round(ndof / (stm.solve_ms * 1e-3) / 1e6, 3)}
+            "solve_mdof_s_mixed": round(ndof / (stm.solve_ms * 1e-3) / 1e6, 3),
+            "block_jacobi": {"fp64_solve_ms": round(bst64.solve_ms, 2), "fp64_iters": bst64.iters, "rel_err_fp64": berr64,
+                             "mixed_solve_ms": round(bstm.solve_ms, 2), "mixed_iters": bstm.iters, "rel_err_mixed": berrm}}
 
 
 def box_filter27(z, n, passes=3):
@@ -255,13 +274,13 @@ def config4(n_models=256):
         b.assemble()
         b.clear_result()
         t0 = time.perf_counter()
-        ok, ms = b.solve(rel_tol=1e-10, precision=L.MIXED, max_iter=40000)
+        ok, ms = b.solve(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI, max_iter=40000)
         times.append((time.perf_counter() - t0, ms))
     errs = [np.linalg.norm(b.get_result(i) - refs[i]) / np.linalg.norm(refs[i]) for i in range(n_models)]
     ms = float(np.median([t[1] for t in times]))
     iters = [st.iters for st in b.last_stats]
     return {"workload": "256 Hexa8 cubes, side ~ U{8..32} cells (seed 4), lognormal E (sigma 0.5), nu ~ U[0.2,0.4], "
-                        "traction ~ N(0,1) on x=1, clamp x=0; one batch: per-problem MIXED Jacobi-PCG to rel. res. 1e-10, "
+                        "traction ~ N(0,1) on x=1, clamp x=0; one batch: per-problem MIXED block-Jacobi PCG to rel. res. 1e-10, "
                         "iteration chunks replayed from a captured hipGraph",
             "models": n_models, "total_dof": ndof, "batch_setup_s": round(setup, 2),
             "assembly_ms": round(float(np.median(t_asm)), 3), "solve_ms": round(ms, 2),
@@ -302,7 +321,7 @@ def main():
     f.assemble()
     unroll = f.autotune()
     log("autotuned SpMV: %d SELL slots per loop trip" % unroll)
-    f.precision, f.precond = L.FP64, L.JACOBI
+    f.precision, f.precond = L.FP64, L.BLOCK_JACOBI
     f.max_iter = 40000
     f.set_result(np.zeros(ndof))
     assert f.solve_system(rel_tol=1e-13), f.last_stats.status
@@ -312,8 +331,8 @@ def main():
                                                                      f.last_stats.solve_ms))
     nref = np.linalg.norm(u_ref)
 
-    def solve(prec, tol):
-        f.precision = prec
+    def solve(prec, tol, precond=L.BLOCK_JACOBI):
+        f.precision, f.precond = prec, precond
         f.set_result(np.zeros(ndof))
         ok = f.solve_system(rel_tol=tol)
         st = f.last_stats
@@ -340,7 +359,7 @@ def main():
     assert errm <= 1e-8
 
     # -- timed steps ---------------------------------------------------------------------------
-    f.precision = L.MIXED
+    f.precision, f.precond = L.MIXED, L.BLOCK_JACOBI
 
     def step():
         f.assemble()
@@ -368,6 +387,31 @@ def main():
         ok, st, err = solve(L.FP64, tol)
         fp64_ms.append(st.solve_ms)
 
+    # -- the other preconditioners at the same tolerances ------------------------------------------
+    precond = {}
+    for name, pc, precs in (("jacobi", L.JACOBI, (L.FP64, L.MIXED)), ("block_jacobi", L.BLOCK_JACOBI, (L.FP64, L.MIXED)),
+                            ("block_ic0_coloured", L.BLOCK_IC0_COLOURED, (L.FP64, L.MIXED)),
+                            ("block_ic0_natural", L.BLOCK_IC0, (L.FP64,))):
+        row = {}
+        for prec in precs:
+            ok, st, err = solve(prec, tol if prec == L.FP64 else mtol, pc)
+            key = "fp64" if prec == L.FP64 else "mixed"
+            row[key] = {"ok": bool(ok), "iters": int(st.iters), "solve_ms": round(st.solve_ms, 2),
+                        "setup_ms": round(f.precond_setup_ms(), 3), "rel_err": float(err)}
+            log("%s %s: %d iters %.1f ms (setup %.1f ms) err %.1e" % (name, key, st.iters, st.solve_ms,
+                                                                      f.precond_setup_ms(), err))
+        precond[name] = row
+
+    # -- after-solve recovery (strain, stress, trace per element) ------------------------------------
+    f.precision, f.precond = L.FP64, L.BLOCK_JACOBI
+    f.set_result(u_ref)
+    as_ms = []
+    for _ in range(5):
+        f.call_after_solve()
+        as_ms.append(f.after_solve_ms())
+    as_bytes = sz["n_nodes"] * 48 + sz["n_elem"] * (4 * 8 + 8 + 13 * 8)
+    a_ms = float(np.median(as_ms))
+
     # -- roofline of the dominant kernel (FP32 block SpMV), HIP events on the problem's stream --
     reps = 50
     t32 = f.time_spmv(L.MIXED, reps)
@@ -383,7 +427,7 @@ def main():
         "vs_baseline": None, "dtype": "f32-inner/f64-refined",
         "data": "synthetic (make_rect Hexa lattice, lognormal E from seed 0)",
         "config": {"workload": "Hexa8 %d^3 elasticity, E_e=210000*exp(0.5 z_e), nu=0.3, clamp x=0 (penalty 1e5), "
-                               "traction (0,0,-1) on x=1; step = GPU assembly + mixed Jacobi-PCG to 1e-8 rel. err"
+                               "traction (0,0,-1) on x=1; step = GPU assembly + mixed block-Jacobi PCG to 1e-8 rel. err"
                                % n,
                    "elements": sz["n_elem"], "nodes": sz["n_nodes"], "dof": ndof, "blocks": sz["nnzb"],
                    "parallelism": "single GPU"},
@@ -404,6 +448,10 @@ def main():
         "roofline_fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                "avg_launch_ms": round(t64, 4), "algorithmic_bytes": b64,
                                "traffic": load_traffic("spmv_fp64")},
+        "preconditioners": precond,
+        "after_solve": {"kernel": "k_after_solve<2>", "ms": round(a_ms, 4),
+                        "melem_s": round(sz["n_elem"] / (a_ms * 1e-3) / 1e6, 1), "algorithmic_bytes": as_bytes,
+                        "achieved_gbs": round(as_bytes / (a_ms * 1e-3) / 1e9, 1)},
     }
     from lmtpp_amd import _capi as A
     gbs, _ = A.stream_copy_gbs(1 << 27, 20)

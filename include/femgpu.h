@@ -160,6 +160,19 @@ FEMGPU_API int femgpu_get_ic0_factor(femgpu_problem *p, int precond, uint32_t *o
 /* max|diag K| taken before the constraints (formulation.h:1029-1030). */
 FEMGPU_API int femgpu_get_max_diag(femgpu_problem *p, double *max_diag);
 
+/* ---- after solve ------------------------------------------------------------------------- */
+
+/* call_after_solve() (formulation.h:2026-2050) running the generated after_solve of
+ * apply_on_elements_after_solve (formulations/formulation_elasticity_iso.py:64-85; the Hexa
+ * instance is generated in std_formulations/definition_probleme_elasticity_dep_static.h:721-789)
+ * on the current result: per element, at xi = (1/2, 1/2, 1/2), eps = grad_sym_col(u) (Voigt xx,
+ * yy, zz, xy, xz, yz, tensor shear), sigma = H(E_e, nu) eps and tr_epsilon.  eps and sigma hold
+ * 6 values per element (element-major), tr_eps one; any output may be NULL. */
+FEMGPU_API int femgpu_after_solve(femgpu_problem *p, double *eps, double *sigma, double *tr_eps);
+FEMGPU_API int femgpu_after_solve_device(femgpu_problem *p, double *d_eps, double *d_sigma, double *d_tr_eps);
+/* Device time (ms) of the last after_solve kernel. */
+FEMGPU_API int femgpu_get_after_solve_ms(femgpu_problem *p, double *ms);
+
 /* ---- differentiation ---------------------------------------------------------------------- */
 
 /* Adjoint of the assembled system: solves K lambda = dJ/du and returns
@@ -217,6 +230,8 @@ FEMGPU_API int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, s
 /* Zero initial guess for every problem of the batch (each solve otherwise starts from the previous
  * result, as shift() does for one formulation). */
 FEMGPU_API int femgpu_batch_clear_result(femgpu_batch *b);
+/* after_solve of problem i of the batch (outputs sized for that problem's elements). */
+FEMGPU_API int femgpu_batch_after_solve(femgpu_batch *b, uint32_t i, double *eps, double *sigma, double *tr_eps);
 /* Device time (ms) of the last batched assembly. */
 FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
 FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);
@@ -227,6 +242,9 @@ FEMGPU_API const char *femgpu_status_string(int status);
 FEMGPU_API const char *femgpu_last_error(void);
 /* Device time (ms) of the last assembly (element + skin + constraint kernels). */
 FEMGPU_API int femgpu_get_assemble_ms(femgpu_problem *p, double *ms);
+/* Device time (ms) of the preconditioner set-up of the last solve (Jacobi factors, or the block
+ * IC(0) factorisation and its FP32 copy); 0 when the previous set-up was reused. */
+FEMGPU_API int femgpu_get_precond_setup_ms(femgpu_problem *p, double *ms);
 /* Average device time (ms) of one block-SpMV launch (precision FP64 or the FP32 copy), measured
  * with HIP events around reps back-to-back launches on the problem's stream. */
 FEMGPU_API int femgpu_time_spmv(femgpu_problem *p, int precision, int reps, double *avg_ms);

@@ -93,6 +93,11 @@ _SIGS = {
     "femgpu_set_graph_replay": (C.c_int, [_P, C.c_int]),
     "femgpu_batch_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_batch_clear_result": (C.c_int, [_P]),
+    "femgpu_after_solve": (C.c_int, [_P, _P, _P, _P]),
+    "femgpu_after_solve_device": (C.c_int, [_P, _P, _P, _P]),
+    "femgpu_get_after_solve_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "femgpu_batch_after_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "femgpu_get_precond_setup_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
 }

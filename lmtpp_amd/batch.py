@@ -92,6 +92,13 @@ class Batch:
         """Zero initial guess for every problem (solves otherwise start from the previous result)."""
         A.check(self._lib.femgpu_batch_clear_result(self._h), "batch_clear_result")
 
+    def call_after_solve(self, i):
+        """Per-element (epsilon[n,6], sigma[n,6], tr_epsilon[n]) of problem i (Formulation.call_after_solve)."""
+        ne = self.sizes[i][1]
+        eps, sig, tr = np.zeros((ne, 6)), np.zeros((ne, 6)), np.zeros(ne)
+        A.check(self._lib.femgpu_batch_after_solve(self._h, i, _ptr(eps), _ptr(sig), _ptr(tr)), "batch_after_solve")
+        return eps, sig, tr
+
     def get_result(self, i):
         u = np.zeros(3 * self.sizes[i][0])
         A.check(self._lib.femgpu_batch_get_result(self._h, i, _ptr(u), u.size), "batch_get_result")

@@ -35,6 +35,23 @@ struct ElemLds {
     double N[ET<K>::NGP][ET<K>::NPE];
 };
 
+// det J and J^-1 by cofactors (exmatrix.cpp:98-125), J[j][i] = dx_i/dxi_j.
+__device__ inline double jac_inverse(const double (&J)[3][3], double (&Ji)[3][3]) {
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double id = 1.0 / det;
+    Ji[0][0] = c00 * id; Ji[1][0] = c01 * id; Ji[2][0] = c02 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return det;
+}
+
 template <int K>
 __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t e, const uint32_t *__restrict__ conn,
                                      const double *__restrict__ coords, int *flags) {
@@ -52,19 +69,8 @@ __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t 
             for (int j = 0; j < 3; ++j)
                 for (int i = 0; i < 3; ++i) J[j][i] += d[j] * L.x[3 * b + i];
         }
-        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-        const double id = 1.0 / det;
         double Ji[3][3];
-        Ji[0][0] = c00 * id; Ji[1][0] = c01 * id; Ji[2][0] = c02 * id;
-        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        const double det = jac_inverse(J, Ji);
         double d[3];
         L.N[q][a] = gen::vol<K>::shape(a, xi, d);
         for (int j = 0; j < 3; ++j) L.g[q][a][j] = Ji[j][0] * d[0] + Ji[j][1] * d[1] + Ji[j][2] * d[2];
@@ -364,6 +370,76 @@ hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, 
     if (P.kind == TETRA) hipLaunchKernelGGL(k_elem_grad<TETRA>, dim3((P.n_elem + 15) / 16), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_elem_sub, P.d_mat, d_u, d_lam, d_g, flags);
     else if (P.kind == TETRA10) hipLaunchKernelGGL(k_elem_grad<TETRA10>, dim3((P.n_elem + 1) / 2), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_elem_sub, P.d_mat, d_u, d_lam, d_g, flags);
     else hipLaunchKernelGGL(k_elem_grad<HEXA>, dim3((P.n_elem + 3) / 4), dim3(256), 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_elem_sub, P.d_mat, d_u, d_lam, d_g, flags);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// After-solve recovery (call_after_solve, formulation.h:2026-2050, calling the generated
+// after_solve of apply_on_elements_after_solve, formulations/formulation_elasticity_iso.py:64-85):
+// eps = grad_sym_col(u), tr_epsilon = eps_xx + eps_yy + eps_zz and sigma = H(E_e, nu) eps, at
+// xi = (1/2, 1/2, 1/2) for every element kind (each var_inter is set to 1/2; for Tetra and
+// Tetra_10 that point lies outside the reference simplex, as in the reference).  One lane per
+// element; element-major outputs eps[6e + i], sig[6e + i], tr[e] (elem.epsilon[0][i],
+// elem.sigma[0][i], elem.tr_epsilon).
+// ---------------------------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(256) void k_after_solve(uint32_t n_elem, const uint32_t *__restrict__ conn,
+                                                     const double *__restrict__ coords, const double *__restrict__ E,
+                                                     const uint32_t *__restrict__ elem_sub,
+                                                     const SubMaterial *__restrict__ mat, const double *__restrict__ u,
+                                                     double *__restrict__ eps, double *__restrict__ sig,
+                                                     double *__restrict__ tr) {
+    constexpr int NPE = ET<K>::NPE;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    const double xi[3] = {0.5, 0.5, 0.5};
+    uint32_t nd[NPE];
+    double dN[NPE][3];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        nd[a] = conn[(size_t)e * NPE + a];
+        gen::vol<K>::shape(a, xi, dN[a]);
+        const double x0 = coords[3 * (size_t)nd[a]], x1 = coords[3 * (size_t)nd[a] + 1], x2 = coords[3 * (size_t)nd[a] + 2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            J[j][0] += dN[a][j] * x0;
+            J[j][1] += dN[a][j] * x1;
+            J[j][2] += dN[a][j] * x2;
+        }
+    }
+    double Ji[3][3];
+    jac_inverse(J, Ji);
+    double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        double g[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN[a][0] + Ji[j][1] * dN[a][1] + Ji[j][2] * dN[a][2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double ui = u[3 * (size_t)nd[a] + i];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) G[3 * i + j] += ui * g[j];
+        }
+    }
+    double ev[6], sv[6], H[3];
+    gen::strain(G, ev);
+    gen::hooke(E[e], mat[elem_sub[e]].nu, H);
+    gen::stress(H, ev, sv);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        if (eps) eps[6 * (size_t)e + i] = ev[i];
+        if (sig) sig[6 * (size_t)e + i] = sv[i];
+    }
+    if (tr) tr[e] = ev[0] + ev[1] + ev[2];
+}
+
+hipError_t launch_after_solve(Problem &P, const double *d_u, double *d_eps, double *d_sig, double *d_tr) {
+    const dim3 g((P.n_elem + 255) / 256), b(256);
+    if (P.kind == TETRA) hipLaunchKernelGGL(k_after_solve<TETRA>, g, b, 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, d_u, d_eps, d_sig, d_tr);
+    else if (P.kind == TETRA10) hipLaunchKernelGGL(k_after_solve<TETRA10>, g, b, 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, d_u, d_eps, d_sig, d_tr);
+    else hipLaunchKernelGGL(k_after_solve<HEXA>, g, b, 0, P.stream, P.n_elem, P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, d_u, d_eps, d_sig, d_tr);
     return hipGetLastError();
 }
 

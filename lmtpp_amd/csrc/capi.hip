@@ -686,6 +686,55 @@ int femgpu_get_max_diag(femgpu_problem *h, double *md) {
     return FEMGPU_OK;
 }
 
+int femgpu_after_solve_device(femgpu_problem *h, double *d_eps, double *d_sigma, double *d_tr) {
+    if (!h || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "after_solve: bad input");
+    Problem &P = h->P;
+    for (uint32_t s = 0; s < P.n_sub; ++s)
+        if (!P.material_set[s]) return fail(FEMGPU_NOT_READY, "after_solve: material not set");
+    HIPCHK(hipEventRecord(P.ev[2], P.stream));
+    HIPCHK(launch_after_solve(P, P.d_u, d_eps, d_sigma, d_tr));
+    HIPCHK(hipEventRecord(P.ev[3], P.stream));
+    HIPCHK(hipEventSynchronize(P.ev[3]));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev[2], P.ev[3]));
+    P.last_after_solve_ms = ms;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_after_solve_ms(femgpu_problem *h, double *ms) {
+    if (!h || !ms) return fail(FEMGPU_BAD_INPUT, "get_after_solve_ms: null");
+    *ms = h->P.last_after_solve_ms;
+    return FEMGPU_OK;
+}
+
+// Recovery over every element, then the rows of elements [e0, e0 + ne) copied to the host.
+static int after_solve_host(femgpu_problem *h, uint32_t e0, uint32_t ne, double *eps, double *sig, double *tr) {
+    Problem &P = h->P;
+    double *d = nullptr;
+    HIPCHK(dalloc(&d, 13 * (size_t)P.n_elem));
+    double *d_eps = d, *d_sig = d + 6 * (size_t)P.n_elem, *d_tr = d + 12 * (size_t)P.n_elem;
+    int rc = femgpu_after_solve_device(h, d_eps, d_sig, d_tr);
+    hipError_t e = hipSuccess;
+    if (!rc && eps) e = hipMemcpy(eps, d_eps + 6 * (size_t)e0, 48 * (size_t)ne, hipMemcpyDeviceToHost);
+    if (!rc && sig && e == hipSuccess) e = hipMemcpy(sig, d_sig + 6 * (size_t)e0, 48 * (size_t)ne, hipMemcpyDeviceToHost);
+    if (!rc && tr && e == hipSuccess) e = hipMemcpy(tr, d_tr + e0, 8 * (size_t)ne, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(FEMGPU_HIP_ERROR, hipGetErrorString(e));
+    return FEMGPU_OK;
+}
+
+int femgpu_after_solve(femgpu_problem *h, double *eps, double *sigma, double *tr) {
+    if (!h || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "after_solve: bad input");
+    return after_solve_host(h, 0, h->P.n_elem, eps, sigma, tr);
+}
+
+int femgpu_batch_after_solve(femgpu_batch *b, uint32_t i, double *eps, double *sigma, double *tr) {
+    if (!b || i >= b->H.P.n_sub) return fail(FEMGPU_BAD_INPUT, "batch_after_solve: bad input");
+    const Problem &P = b->H.P;
+    return after_solve_host(&b->H, P.elem_off[i], P.elem_off[i + 1] - P.elem_off[i], eps, sigma, tr);
+}
+
 int femgpu_elem_grad(femgpu_problem *h, const double *u, const double *lam, double *g) {
     if (!h || !u || !lam || !g || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "elem_grad: bad input");
     Problem &P = h->P;
@@ -747,6 +796,12 @@ int femgpu_adjoint(femgpu_problem *h, const double *dJdu, double *dJdE, const fe
 int femgpu_get_assemble_ms(femgpu_problem *h, double *ms) {
     if (!h || !ms) return fail(FEMGPU_BAD_INPUT, "get_assemble_ms: null");
     *ms = h->P.last_asm_ms;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_precond_setup_ms(femgpu_problem *h, double *ms) {
+    if (!h || !ms) return fail(FEMGPU_BAD_INPUT, "get_precond_setup_ms: null");
+    *ms = h->P.last_precond_ms;
     return FEMGPU_OK;
 }
 

@@ -110,6 +110,8 @@ struct Problem {
     SpmvConfig spmv;
     std::vector<double> max_diag;  // per sub-problem, of the last matrix assembly
     double last_asm_ms = 0;
+    double last_precond_ms = 0;
+    double last_after_solve_ms = 0;    // device time of the last preconditioner set-up (0: reused)
     bool from_bsr = false;
     bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
@@ -125,6 +127,8 @@ hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_constraints(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_to_fp32(Problem &P);
 hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, double *d_g);
+// eps (6 per element), sigma (6 per element), tr_epsilon (1 per element); any may be null
+hipError_t launch_after_solve(Problem &P, const double *d_u, double *d_eps, double *d_sig, double *d_tr);
 
 // ---- solver (solve.hip) ----
 struct SolveOpts { int precision, precond, crit_kind; double tol; int max_iter; double delta; int chunk; };

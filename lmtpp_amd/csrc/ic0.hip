@@ -232,6 +232,9 @@ int ic0_ensure(Problem &P, int ordering, bool fp32) {
     int rc = ic0_prepare(P, ordering);
     if (rc) return rc;
     Ic0 &I = P.ic0;
+    P.last_precond_ms = 0;
+    if (I.valid && (!fp32 || I.valid32)) return 0;
+    (void)hipEventRecord(P.ev[2], P.stream);
     if (!I.valid) {
         I.valid32 = false;
         if (hipMemsetAsync(P.d_flags + 1, 0, sizeof(int), P.stream) != hipSuccess) return 5;
@@ -256,6 +259,11 @@ int ic0_ensure(Problem &P, int ordering, bool fp32) {
         if (hipGetLastError() != hipSuccess) return 5;
         I.valid32 = true;
     }
+    (void)hipEventRecord(P.ev[3], P.stream);
+    if (hipEventSynchronize(P.ev[3]) != hipSuccess) return 5;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, P.ev[2], P.ev[3]);
+    P.last_precond_ms = ms;
     return 0;
 }
 

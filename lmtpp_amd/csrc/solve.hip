@@ -524,16 +524,26 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         S.rr = S.rmax = 0;
     }
     if (write_scalars(P)) return 5;
+    bool built_minv = false;
     if (PRE == PRE_IC0) {
         const int rc = ic0_ensure(P, ordering, mixed);
         if (rc != 0 && rc != 3) return rc == 1 ? 1 : 5;  // 3: statuses are read below
     } else if (P.minv_kind != PRE || (mixed && !P.d_Minv32)) {
         if (mixed && !P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
             return 5;
+        (void)hipEventRecord(P.ev[2], P.stream);
         hipLaunchKernelGGL(k_build_minv<PRE>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
+        (void)hipEventRecord(P.ev[3], P.stream);
         P.minv_kind = PRE;
+        built_minv = true;
+    } else {
+        P.last_precond_ms = 0;
     }
     if (read_scalars(P)) return 5;
+    if (built_minv) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, P.ev[2], P.ev[3]) == hipSuccess) P.last_precond_ms = ms;
+    }
     for (uint32_t s = 0; s < P.n_sub; ++s)
         if (P.h_ps[s].status) {
             P.minv_kind = -1;

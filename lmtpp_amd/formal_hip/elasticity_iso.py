@@ -67,6 +67,24 @@ def block_entries():
     return [sp.expand(sp.diff(res, VA[i], UB[j])) for i in range(3) for j in range(3)]
 
 
+GU = sp.symbols("G0:9")       # G[3i+j] = d u_i / d x_j at the recovery point
+EPS = sp.symbols("e0:6")
+
+
+def strain_entries():
+    """grad_sym_col of the displacement from its physical gradient G (formal_lf/element.py:270-296):
+    Voigt [xx, yy, zz, xy, xz, yz], tensor shear (apply_on_elements_after_solve,
+    formulations/formulation_elasticity_iso.py:64-85)."""
+    G = sp.Matrix(3, 3, lambda i, j: GU[3 * i + j])
+    return _col((G + G.T) / 2)
+
+
+def stress_entries():
+    """sigma = H eps with the isotropic H (entries H0, H1, H2) and Voigt eps."""
+    Hs = sp.Matrix(6, 6, lambda i, j: (H0 if i == j else H1) if (i < 3 and j < 3) else (H2 if i == j else 0))
+    return [sp.expand(v) for v in Hs * sp.Matrix(EPS)]
+
+
 def load_entries():
     """-d res / d v_{a,i} at u = 0 for the volume term density * dot(acc - f_vol, v) (acc = 0,
     static); v = N_a v_a at the Gauss point, density * f_vol passed as BF."""

@@ -197,6 +197,12 @@ class Formulation:
         A.check(self._lib.femgpu_get_assemble_ms(self._h, C.byref(v)), "assemble_ms")
         return v.value
 
+    def precond_setup_ms(self):
+        """Device time of the last solve's preconditioner set-up (0 when reused)."""
+        v = C.c_double()
+        A.check(self._lib.femgpu_get_precond_setup_ms(self._h, C.byref(v)), "precond_setup_ms")
+        return v.value
+
     def time_spmv(self, precision=A.FP64, reps=20):
         v = C.c_double()
         A.check(self._lib.femgpu_time_spmv(self._h, int(precision), int(reps), C.byref(v)), "time_spmv")
@@ -210,6 +216,23 @@ class Formulation:
 
     def set_graph_replay(self, enable=True):
         A.check(self._lib.femgpu_set_graph_replay(self._h, int(bool(enable))), "set_graph_replay")
+
+    def call_after_solve(self):
+        """call_after_solve() (formulation.h:2026-2050): per-element strain, stress and trace at
+        xi = (1/2, 1/2, 1/2) from the current result, as the generated after_solve of
+        formulations/formulation_elasticity_iso.py:64-85.  Returns (epsilon[n_elem, 6],
+        sigma[n_elem, 6], tr_epsilon[n_elem]), Voigt xx, yy, zz, xy, xz, yz with tensor shear."""
+        eps = np.zeros((self.n_elem, 6))
+        sig = np.zeros((self.n_elem, 6))
+        tr = np.zeros(self.n_elem)
+        A.check(self._lib.femgpu_after_solve(self._h, _ptr(eps), _ptr(sig), _ptr(tr)), "after_solve")
+        return eps, sig, tr
+
+    def after_solve_ms(self):
+        """Device time of the last after-solve kernel."""
+        v = C.c_double()
+        A.check(self._lib.femgpu_get_after_solve_ms(self._h, C.byref(v)), "after_solve_ms")
+        return v.value
 
     # -- differentiation -----------------------------------------------------------------------
     def adjoint(self, dJdu, rel_tol=1e-12):

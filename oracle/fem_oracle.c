@@ -879,6 +879,50 @@ int or_pcg_ic0(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, c
 }
 
 /* ------------------------------------------------------------------------- */
+/* After-solve recovery                                                       */
+/* ------------------------------------------------------------------------- */
+
+/* apply_on_elements_after_solve (formulations/formulation_elasticity_iso.py:64-85), called per
+ * element by call_after_solve (include/formulation/formulation.h:2026-2050): every var_inter is
+ * substituted by 1/2, so the gradient is taken at xi = (1/2,1/2,1/2) whatever the element (for the
+ * tetrahedra that point lies outside the reference simplex).  G = sum_a u_a (J^-1 dN_a)^T,
+ * eps = grad_sym_col(u) = [G00, G11, G22, (G01+G10)/2, (G02+G20)/2, (G12+G21)/2], and sigma = H eps
+ * written with the Lame constants (lambda tr(eps) + 2 mu eps_ii, 2 mu eps_ij). */
+int or_after_solve(int kind, uint32_t n_elem, const uint32_t *conn, const double *coords, const double *E_e,
+                   double nu, const double *u, double *eps, double *sig, double *tr)
+{
+    const int npe = or_nodes_per_elem(kind);
+    const double xi[3] = {0.5, 0.5, 0.5};
+    double N[10], dN[10][3];
+    shape(kind, xi, N, dN);
+    for (uint32_t e = 0; e < n_elem; ++e) {
+        const uint32_t *en = conn + (size_t)e * npe;
+        double xe[30], J[3][3], Ji[3][3], G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < npe; ++a)
+            for (int d = 0; d < 3; ++d) xe[3 * a + d] = coords[3 * (size_t)en[a] + d];
+        const double det = jacobian(npe, xe, dN, J, Ji);
+        if (!(det != 0)) return 2;
+        for (int a = 0; a < npe; ++a) {
+            double g[3];
+            for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN[a][0] + Ji[j][1] * dN[a][1] + Ji[j][2] * dN[a][2];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) G[i][j] += u[3 * (size_t)en[a] + i] * g[j];
+        }
+        double ev[6] = {G[0][0], G[1][1], G[2][2], 0.5 * (G[0][1] + G[1][0]), 0.5 * (G[0][2] + G[2][0]),
+                        0.5 * (G[1][2] + G[2][1])};
+        double lam, mu;
+        lame(E_e[e], nu, &lam, &mu);
+        const double t = ev[0] + ev[1] + ev[2];
+        for (int i = 0; i < 6; ++i) {
+            if (eps) eps[6 * (size_t)e + i] = ev[i];
+            if (sig) sig[6 * (size_t)e + i] = i < 3 ? lam * t + 2 * mu * ev[i] : 2 * mu * ev[i];
+        }
+        if (tr) tr[e] = t;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Adjoint gradient                                                           */
 /* ------------------------------------------------------------------------- */
 

@@ -30,6 +30,14 @@
  *   or_pcg_jacobi       pinned by the reference CG known-answer tests
  *                       (3x3 -> (11,2,4); 10x10 Maple solution) and by a dense
  *                       LAPACK solve of the assembled system.
+ *   or_after_solve      pinned: Hexa against the reference's generated
+ *                       after_solve (std_formulations/definition_probleme_
+ *                       elasticity_dep_static.h:721-789, the same DSL expression)
+ *                       compiled from source (oracle/_ref, golden fixture);
+ *                       all kinds by closed forms (affine fields give the exact
+ *                       constant strain, sigma = lambda tr I + 2 mu eps) and the
+ *                       analytic gradient of a quadratic field on a straight
+ *                       Tetra_10 at the point xi = (1/2,1/2,1/2).
  *   or_elem_grad_E      pinned by central finite differences of the oracle's
  *                       own assembled objective (tests), no reference
  *                       counterpart exists (the reference has no adjoint).
@@ -105,6 +113,13 @@ int or_block_ic0(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col,
 /* PCG with the block IC(0) factor and the reference's stop test max|r_i| <= crit. */
 int or_pcg_ic0(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, const double *vals,
                const double *L, const double *b, double *x, double crit, int max_iter, int *iters);
+
+/* After-solve recovery (apply_on_elements_after_solve,
+ * formulations/formulation_elasticity_iso.py:64-85): per element at xi = (1/2,1/2,1/2),
+ * eps = grad_sym_col(u) (Voigt xx,yy,zz,xy,xz,yz, tensor shear), sigma = H eps, tr_eps.
+ * eps, sig: 6 per element; tr: one per element. */
+int or_after_solve(int kind, uint32_t n_elem, const uint32_t *conn, const double *coords, const double *E_e,
+                   double nu, const double *u, double *eps, double *sig, double *tr);
 
 /* Per-element gradient of J w.r.t. E_e given u and the adjoint lambda:
  * g_e = -lambda_e^T (dK_e/dE_e) u_e  (K_e linear in E_e). */

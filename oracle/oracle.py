@@ -188,11 +188,22 @@ def elem_grad_E(kind, conn, coords, E_e, nu, u, lam):
     return g
 
 
+def after_solve(kind, conn, coords, E_e, nu, u):
+    """(eps[n,6], sigma[n,6], tr[n]) at xi = (1/2,1/2,1/2) per element."""
+    conn, coords = _u32(conn), _f64(coords)
+    n = conn.shape[0]
+    E_e = _f64(np.broadcast_to(np.asarray(E_e, dtype=np.float64), (n,)))
+    eps, sig, tr = np.zeros((n, 6)), np.zeros((n, 6)), np.zeros(n)
+    rc = lib().or_after_solve(kind, n, _p(conn), _p(coords), _p(E_e), C.c_double(nu), _p(_f64(u)), _p(eps), _p(sig),
+                              _p(tr))
+    assert rc == 0, rc
+    return eps, sig, tr
+
+
 # --- reference builds (oracle/_ref), available only where the LMT++ sources are -----------------
 
 def ref_available():
-    return os.path.exists(os.path.join(REF_DIR, "ref_mesh_dump")) and os.path.exists(
-        os.path.join(REF_DIR, "ref_tinyblock_pcg"))
+    return all(os.path.exists(os.path.join(REF_DIR, b)) for b in ("ref_mesh_dump", "ref_tinyblock_pcg", "ref_after_solve"))
 
 
 def ref_mesh(kind_name, nb_points, tmpdir):
@@ -204,6 +215,25 @@ def ref_mesh(kind_name, nb_points, tmpdir):
         coords = np.frombuffer(f.read(8 * 3 * int(nn)), dtype=np.float64).reshape(-1, 3).copy()
         conn = np.frombuffer(f.read(4 * int(ne) * int(npe)), dtype=np.uint32).reshape(-1, int(npe)).copy()
     return coords, conn
+
+
+def ref_after_solve_hexa(nb_points, coords, u, E, nu, tmpdir):
+    """The reference's generated Hexa after_solve on make_rect(Hexa, nb_points) with the node
+    positions replaced by coords: (eps[n,6], sigma[n,6], tr[n]) in element order."""
+    inp = os.path.join(tmpdir, "as_in.bin")
+    out = os.path.join(tmpdir, "as_out.bin")
+    coords, u = _f64(coords), _f64(u)
+    with open(inp, "wb") as f:
+        f.write(np.array([coords.shape[0]], dtype=np.uint32).tobytes())
+        f.write(coords.tobytes())
+        f.write(u.tobytes())
+        f.write(np.array([E, nu], dtype=np.float64).tobytes())
+    subprocess.check_call([os.path.join(REF_DIR, "ref_after_solve")] + [str(v) for v in nb_points] + [inp, out],
+                          stdout=subprocess.DEVNULL)
+    with open(out, "rb") as f:
+        ne = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
+        d = np.frombuffer(f.read(8 * 13 * ne), dtype=np.float64).reshape(ne, 13)
+    return d[:, :6].copy(), d[:, 6:12].copy(), d[:, 12].copy()
 
 
 def ref_tinyblock_pcg(row_ptr, col, vals, b, x0, crit, tmpdir):

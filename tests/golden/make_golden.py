@@ -8,6 +8,11 @@ run where those sources exist.  Fixtures are data only (inputs and the reference
            (containers/MatWithTinyBlocks.h:426-523,1101-1151) on the config-0 system
   ref_cg_kat.json                       known-answer systems of
         tests/include/containers/conjugate_gradient_test.cpp:15-71
+  ref_after_solve_hexa_4_3_5.npz       coords, conn, u, E, nu -> eps, sigma, tr_epsilon
+        -- the generated Hexa after_solve of std_formulations/
+           definition_probleme_elasticity_dep_static.h:721-789 on a jittered make_rect mesh
+
+    python tests/golden/make_golden.py [after_solve]   (no argument: every fixture)
 """
 import json
 import os
@@ -37,9 +42,27 @@ def config0_system():
     return rp, col, vals, F
 
 
+def after_solve_fixture(tmp):
+    nb = (4, 3, 5)
+    c, e = O.ref_mesh("hexa", nb, tmp)
+    rng = np.random.default_rng(2024)
+    inside = np.all((c > 1e-9) & (c < 1 - 1e-9), axis=1)
+    c = c.copy()
+    c[inside] += rng.uniform(-0.06, 0.06, size=(int(inside.sum()), 3))
+    u = 1e-3 * rng.standard_normal(3 * len(c))
+    E, nu = 273000.0, 0.27
+    eps, sig, tr = O.ref_after_solve_hexa(nb, c, u, E, nu, tmp)
+    np.savez_compressed(os.path.join(HERE, "ref_after_solve_hexa_4_3_5.npz"), coords=c, conn=e, u=u, E=E, nu=nu,
+                        eps=eps, sigma=sig, tr=tr)
+
+
 def main():
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "liboracle.so"])
     tmp = tempfile.mkdtemp()
+    if sys.argv[1:] == ["after_solve"]:
+        after_solve_fixture(tmp)
+        return
+    after_solve_fixture(tmp)
     for name, kind, nb in MESHES:
         c, e = O.ref_mesh(name, nb, tmp)
         np.savez_compressed(os.path.join(HERE, "ref_mesh_%s_%d_%d_%d.npz" % ((name,) + nb)), coords=c, conn=e)

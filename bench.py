@@ -297,6 +297,9 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="cells per side (Hexa8 n^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-configs", action="store_true", help="skip configurations 0, 2, 3, 4")
+    ap.add_argument("--no-precond", action="store_true", help="skip the preconditioner comparison")
+    ap.add_argument("--no-natural-ic0", action="store_true",
+                    help="skip the natural-order block IC(0) solve (hundreds of level launches per iteration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the path shards as described in DESIGN.md "
@@ -389,9 +392,9 @@ def main():
 
     # -- the other preconditioners at the same tolerances ------------------------------------------
     precond = {}
-    for name, pc, precs in (("jacobi", L.JACOBI, (L.FP64, L.MIXED)), ("block_jacobi", L.BLOCK_JACOBI, (L.FP64, L.MIXED)),
+    for name, pc, precs in () if args.no_precond else (("jacobi", L.JACOBI, (L.FP64, L.MIXED)), ("block_jacobi", L.BLOCK_JACOBI, (L.FP64, L.MIXED)),
                             ("block_ic0_coloured", L.BLOCK_IC0_COLOURED, (L.FP64, L.MIXED)),
-                            ("block_ic0_natural", L.BLOCK_IC0, (L.FP64,))):
+                            ("block_ic0_natural", L.BLOCK_IC0, () if args.no_natural_ic0 else (L.FP64,))):
         row = {}
         for prec in precs:
             ok, st, err = solve(prec, tol if prec == L.FP64 else mtol, pc)

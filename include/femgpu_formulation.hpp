@@ -11,7 +11,12 @@
 //     add_constraint(dof, penalty)   formulation_ancestor.h:197  (penalty form)
 //     get_sollicitation()            formulation_ancestor.h:236
 //     get_result()                   formulation_ancestor.h:237
+//     call_after_solve()             formulation.h:2026-2050 (elem.epsilon / sigma / tr_epsilon)
 //     default_iterative_criterium    formulation_ancestor.h:244-257
+//
+// The preconditioner is options().precond: FEMGPU_JACOBI, FEMGPU_BLOCK_JACOBI, FEMGPU_BLOCK_IC0
+// (the reference's block incomplete Cholesky, MatWithTinyBlocks.h:426-709) or
+// FEMGPU_BLOCK_IC0_COLOURED.
 //
 // Like the reference, solve()/solve_system() return false when the system could not be solved
 // (formulation.h:1394,1530-1531); the femgpu status of the last call is in last_status().
@@ -109,6 +114,17 @@ public:
         std::vector<double> F(3 * size_t(n_nodes_));
         check(femgpu_get_sollicitation(h_, F.data(), F.size()), "get_sollicitation");
         return F;
+    }
+    // Per-element strain and stress at xi = (1/2,1/2,1/2) from the current result: 6 values per
+    // element each (Voigt xx, yy, zz, xy, xz, yz, tensor shear) and the trace.
+    struct AfterSolve { std::vector<double> epsilon, sigma, tr_epsilon; };
+    AfterSolve call_after_solve() const {
+        AfterSolve r;
+        r.epsilon.resize(6 * size_t(n_elem_));
+        r.sigma.resize(6 * size_t(n_elem_));
+        r.tr_epsilon.resize(n_elem_);
+        check(femgpu_after_solve(h_, r.epsilon.data(), r.sigma.data(), r.tr_epsilon.data()), "after_solve");
+        return r;
     }
     int last_status() const { return last_status_; }
     const femgpu_solve_stats &last_stats() const { return stats_; }

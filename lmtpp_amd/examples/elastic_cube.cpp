@@ -1,9 +1,10 @@
 // Elastic cube through the C++ facade: the counterpart of the reference's example
 // (include/mesh/exemple_probleme_elastique.txt:59-213) on the GPU path.
-//   elastic_cube <tetra|tetra10|hexa> <points per side> <out.bin>
+//   elastic_cube <tetra|tetra10|hexa> <points per side> <out.bin> [jacobi|ic0]
 // Clamps x = 0 by penalty (1e5, as the reference example), applies the traction (0,0,-1) on
-// x = 1, E = 210000, nu = 0.3, solves with the reference stop test max|r| <= 1e-10 max|F| and
-// writes the displacements (3 doubles per node) to out.bin.
+// x = 1, E = 210000, nu = 0.3, solves with the reference stop test max|r| <= 1e-10 max|F| (Jacobi
+// PCG, or the reference's block IC(0) PCG with "ic0"), writes the displacements (3 doubles per
+// node) to out.bin and prints the largest von Mises stress of call_after_solve().
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -13,8 +14,8 @@
 #include "femgpu_formulation.hpp"
 
 int main(int argc, char **argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s <tetra|tetra10|hexa> <points per side> <out.bin>\n", argv[0]);
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: %s <tetra|tetra10|hexa> <points per side> <out.bin> [jacobi|ic0]\n", argv[0]);
         return 2;
     }
     const femgpu_elem_kind kind = !std::strcmp(argv[1], "hexa") ? FEMGPU_HEXA
@@ -35,6 +36,7 @@ int main(int argc, char **argv) {
     femgpu::check(femgpu_skin_faces(kind, nn, ne, conn.data(), skin.data(), &nf), "skin_faces");
 
     femgpu::FormulationGpu f(kind, coords, conn);
+    if (argc == 5 && !std::strcmp(argv[4], "ic0")) f.options().precond = FEMGPU_BLOCK_IC0;
     f.set_material(210000.0, 0.3);
     std::vector<uint32_t> faces;
     std::vector<double> trac;
@@ -62,7 +64,15 @@ int main(int argc, char **argv) {
     if (!o) return 1;
     std::fwrite(u.data(), 8, u.size(), o);
     std::fclose(o);
-    std::printf("%u nodes %u elems: %d iterations, max|r| %.3e, tip uz %.9e\n", nn, ne, f.last_stats().iters,
-                f.last_stats().max_abs_res, u[3 * (size_t)(nn - 1) + 2]);
+    const femgpu::FormulationGpu::AfterSolve as = f.call_after_solve();
+    double vm = 0;
+    for (uint32_t e = 0; e < ne; ++e) {
+        const double *s = &as.sigma[6 * (size_t)e];
+        const double j2 = ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) + (s[2] - s[0]) * (s[2] - s[0])) / 6 +
+                          s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
+        vm = std::fmax(vm, std::sqrt(3 * j2));
+    }
+    std::printf("%u nodes %u elems: %d iterations, max|r| %.3e, tip uz %.9e, max von Mises %.6e\n", nn, ne,
+                f.last_stats().iters, f.last_stats().max_abs_res, u[3 * (size_t)(nn - 1) + 2], vm);
     return 0;
 }

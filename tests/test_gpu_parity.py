@@ -256,6 +256,12 @@ def test_cpp_facade_example(tmp_path):
     u = np.fromfile(out)
     g = np.load(os.path.join(GOLDEN, "ref_tinyblock_pcg_tetra11.npz"))
     assert np.linalg.norm(u - g["x"]) <= 1e-8 * np.linalg.norm(g["x"])
+    # with the reference's own preconditioner: its iteration count and its solution
+    r = subprocess.run([exe, "tetra", "11", out, "ic0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert " %d iterations" % int(g["iters"]) in r.stdout, r.stdout
+    u = np.fromfile(out)
+    assert np.linalg.norm(u - g["x"]) <= 1e-10 * np.linalg.norm(g["x"])
 
 
 def _batch_inputs(seed=4, n=3):

@@ -322,8 +322,11 @@ def main():
 
     # -- calibration (untimed) ---------------------------------------------------------------
     f.assemble()
-    unroll = f.autotune()
-    log("autotuned SpMV: %d SELL slots per loop trip" % unroll)
+    spmv_cfg = f.autotune()
+    unroll = spmv_cfg["fp32"][1]
+    fmt_name = {0: "full", 1: "sym"}
+    log("autotuned SpMV: FP64 %s/%d, FP32 %s/%d (format / SELL slots per loop trip)" % (
+        fmt_name[spmv_cfg["fp64"][0]], spmv_cfg["fp64"][1], fmt_name[spmv_cfg["fp32"][0]], spmv_cfg["fp32"][1]))
     f.precision, f.precond = L.FP64, L.BLOCK_JACOBI
     f.max_iter = 40000
     f.set_result(np.zeros(ndof))

@@ -255,9 +255,19 @@ FEMGPU_API int femgpu_stream_copy(size_t n, int reps, double *avg_ms);
 
 /* ---- tuning (no reference counterpart: the reference has no device code) ------------------- */
 
-/* Times the block-SpMV variants (SELL slots per loop trip 1, 2, 3) on this matrix and keeps the
- * fastest; *unroll receives the pick. */
+/* Block-SpMV storage formats: FEMGPU_SPMV_FULL stores both triangles (each row all its blocks);
+ * FEMGPU_SPMV_SYM stores each row's upper blocks and reads the lower ones transposed from their
+ * owner row, as MatWithTinyBlocks::partial_mul uses each stored block twice
+ * (containers/MatWithTinyBlocks.h:303-378). */
+typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1 } femgpu_spmv_format;
+/* Times every block-SpMV variant (format x SELL slots per loop trip 1, 2, 3) on this assembled
+ * matrix, for FP64 and for the FP32 copy, and keeps the fastest of each; *unroll receives the
+ * FP64 pick (may be NULL). */
 FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
+/* Current (or forced) variant per precision: index 0 FP64, 1 FP32; format femgpu_spmv_format,
+ * unroll 1..3. */
+FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2]);
+FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2]);
 /* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */
 FEMGPU_API int femgpu_set_graph_replay(femgpu_problem *p, int enable);
 

@@ -16,6 +16,7 @@ TRI3, TRI6, QUAD4 = 0, 1, 2
 FP64, MIXED = 0, 1
 JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED = 0, 1, 2, 3
 REL_L2, ABS_MAXNORM = 0, 1
+SPMV_FULL, SPMV_SYM = 0, 1
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
 NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}
@@ -97,6 +98,8 @@ _SIGS = {
     "femgpu_after_solve_device": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_after_solve_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_batch_after_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "femgpu_get_spmv_config": (C.c_int, [_P, _P, _P]),
+    "femgpu_set_spmv_config": (C.c_int, [_P, _P, _P]),
     "femgpu_get_precond_setup_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),

@@ -60,6 +60,7 @@ void dfree(T *&p) {
 void free_problem(Problem &P) {
     destroy_graph(P);
     ic0_free(P.ic0);
+    sym_free(P.sym);
     dfree(P.d_coords); dfree(P.d_E); dfree(P.d_conn); dfree(P.d_rp); dfree(P.d_col); dfree(P.d_diag);
     dfree(P.d_slot); dfree(P.d_order); dfree(P.d_sptr); dfree(P.d_elem_sub); dfree(P.d_mat);
     dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
@@ -337,6 +338,7 @@ int assemble(femgpu_problem &H, int do_mat, int do_vec) {
         P.k32_valid = false;
         P.minv_kind = -1;
         P.ic0.valid = P.ic0.valid32 = false;
+        P.sym.valid[0] = P.sym.valid[1] = false;
         for (uint32_t s = 0; s < P.n_sub; ++s) P.max_diag[s] = P.h_ps[s].max_diag;
     }
     if (do_vec) P.assembled_vec = true;
@@ -848,8 +850,32 @@ int femgpu_stream_copy(size_t n, int reps, double *avg_ms) {
 
 int femgpu_autotune(femgpu_problem *h, int *unroll) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "autotune: null");
-    const int u = autotune_spmv(h->P);
-    if (unroll) *unroll = u;
+    Problem &P = h->P;
+    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "autotune: assemble first");
+    if (autotune_spmv(P)) return fail(FEMGPU_HIP_ERROR, "autotune failed");
+    if (unroll) *unroll = P.spmv.unroll[0];
+    return FEMGPU_OK;
+}
+
+int femgpu_get_spmv_config(femgpu_problem *h, int format[2], int unroll[2]) {
+    if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "get_spmv_config: null");
+    for (int i = 0; i < 2; ++i) { format[i] = h->P.spmv.format[i]; unroll[i] = h->P.spmv.unroll[i]; }
+    return FEMGPU_OK;
+}
+
+int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unroll[2]) {
+    if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "set_spmv_config: null");
+    for (int i = 0; i < 2; ++i)
+        if (format[i] < 0 || format[i] > 1 || unroll[i] < 1 || unroll[i] > 3)
+            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1, unroll 1..3");
+    Problem &P = h->P;
+    if ((format[0] == FMT_SYM || format[1] == FMT_SYM) && !P.sym.built) {
+        const int rc = sym_ensure(P, format[0] == FMT_SYM ? 0 : 1);
+        if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "set_spmv_config: symmetric storage");
+        P.sym.valid[0] = P.sym.valid[1] = false;  // values are (re)made from K before the next product
+    }
+    for (int i = 0; i < 2; ++i) { P.spmv.format[i] = format[i]; P.spmv.unroll[i] = unroll[i]; }
+    destroy_graph(P);
     return FEMGPU_OK;
 }
 

@@ -38,9 +38,23 @@ struct SubMaterial { double nu, bf0, bf1, bf2; };
 constexpr uint32_t ROWS_PER_WG = 256;
 constexpr uint32_t NO_SUB = 0xffffffffu;
 
+// Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format and SELL slots
+// per loop trip (1, 2 or 3).
+enum { FMT_FULL = 0, FMT_SYM = 1 };
 struct SpmvConfig {
-    int unroll = 2;       // SELL slots per loop trip of the block SpMV (1, 2 or 3)
+    int unroll[2] = {2, 2};
+    int format[2] = {FMT_FULL, FMT_FULL};
     bool tuned = false;
+};
+
+// Device copy of the symmetric SELL storage (host_mesh.hpp SymSell) and its values.
+struct SymDev {
+    uint32_t *d_sptrU = nullptr, *d_lenU = nullptr, *d_colU = nullptr, *d_srcU = nullptr;
+    uint32_t *d_sptrL = nullptr, *d_lenL = nullptr, *d_colL = nullptr, *d_tposL = nullptr;
+    uint64_t psU = 0, nL = 0;
+    double *d_K = nullptr;
+    float *d_K32 = nullptr;
+    bool built = false, valid[2] = {false, false};
 };
 
 // Device side of the block IC(0) preconditioner (schedule in host_mesh.hpp).
@@ -117,6 +131,7 @@ struct Problem {
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;
     Ic0 ic0;
+    SymDev sym;
 };
 
 // ---- launchers (assemble.hip) ----
@@ -136,7 +151,11 @@ struct SolveOut { int status, iters, refreshes; double rel_res, max_abs_res, ms;
 // Solves every sub-problem; out[s] per sub-problem; returns the first non-zero status.
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out);
 double time_spmv(Problem &P, int precision, int reps);
+// Picks format and unroll per precision by timing every variant on this matrix; returns 0 or 5.
 int autotune_spmv(Problem &P);
+// Symmetric storage and its values for precision (0 FP64, 1 FP32) from the current K.
+int sym_ensure(Problem &P, int precision);
+void sym_free(SymDev &S);
 void destroy_graph(Problem &P);
 
 // ---- block IC(0) (ic0.hip) ----

@@ -344,6 +344,68 @@ std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_
     return cut;
 }
 
+bool build_sym_sell(const Pattern &pat, SymSell &S) {
+    const uint32_t n = pat.n_rows, ns = (n + 63) / 64;
+    const uint32_t *rp = pat.row_ptr.data(), *col = pat.col.data();
+    S = SymSell();
+    S.lenU.resize(n);
+    S.lenL.resize(n);
+    std::vector<uint32_t> dslot(n);  // CSR slot of the diagonal
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t *b = col + rp[r], *e = col + rp[r + 1];
+        const uint32_t d = uint32_t(std::lower_bound(b, e, r) - b);
+        if (b + d == e || b[d] != r) return false;
+        dslot[r] = d;
+        S.lenL[r] = d;
+        S.lenU[r] = rp[r + 1] - rp[r] - d;
+    }
+    S.sptrU.assign((size_t)ns + 1, 0);
+    S.sptrL.assign((size_t)ns + 1, 0);
+    for (uint32_t s = 0; s < ns; ++s) {
+        uint32_t wu = 0, wl = 0;
+        for (uint32_t r = 64 * s; r < std::min(n, 64 * s + 64); ++r) {
+            wu = std::max(wu, S.lenU[r]);
+            wl = std::max(wl, S.lenL[r]);
+        }
+        if (uint64_t(S.sptrU[s]) + 64ull * wu >= (1ull << 32) || uint64_t(S.sptrL[s]) + 64ull * wl >= (1ull << 32)) return false;
+        S.sptrU[s + 1] = S.sptrU[s] + 64 * wu;
+        S.sptrL[s + 1] = S.sptrL[s] + 64 * wl;
+    }
+    S.nU = S.sptrU[ns];
+    S.nL = S.sptrL[ns];
+    S.colU.assign(std::max<uint64_t>(S.nU, 1), 0);
+    S.srcU.assign(std::max<uint64_t>(S.nU, 1), 0);
+    S.colL.assign(std::max<uint64_t>(S.nL, 1), 0);
+    S.tposL.assign(std::max<uint64_t>(S.nL, 1), 0);
+    std::vector<uint8_t> bad(n, 0);
+    parallel_for(n, [&](size_t b, size_t e) {
+        for (size_t ri = b; ri < e; ++ri) {
+            const uint32_t r = (uint32_t)ri, s = r >> 6;
+            const uint32_t wu = (S.sptrU[s + 1] - S.sptrU[s]) / 64, wl = (S.sptrL[s + 1] - S.sptrL[s]) / 64;
+            for (uint32_t k = 0; k < wu; ++k) {  // padding: own row, zero values (src = the diagonal)
+                const uint64_t q = S.posU(r, k);
+                const bool real = k < S.lenU[r];
+                S.colU[q] = real ? col[rp[r] + dslot[r] + k] : r;
+                S.srcU[q] = real ? (uint32_t)pat.pos(r, dslot[r] + k) : 0xffffffffu;
+            }
+            for (uint32_t k = 0; k < wl; ++k) {
+                const uint64_t q = S.posL(r, k);
+                if (k >= S.lenL[r]) { S.colL[q] = r; S.tposL[q] = 0xffffffffu; continue; }
+                const uint32_t c = col[rp[r] + k];
+                // row c holds r among its upper columns
+                const uint32_t *cb = col + rp[c] + dslot[c], *ce = col + rp[c + 1];
+                const uint32_t *f = std::lower_bound(cb, ce, r);
+                if (f == ce || *f != r) { bad[r] = 1; continue; }
+                S.colL[q] = c;
+                S.tposL[q] = (uint32_t)S.posU(c, uint32_t(f - cb));
+            }
+        }
+    });
+    for (uint32_t r = 0; r < n; ++r)
+        if (bad[r]) return false;
+    return true;
+}
+
 bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S) {
     const uint32_t n = pat.n_rows;
     const uint32_t *rp = pat.row_ptr.data(), *col = pat.col.data();

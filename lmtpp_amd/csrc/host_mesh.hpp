@@ -56,6 +56,22 @@ Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32
 // Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);
 
+// Symmetric SELL-64 storage of K for the block SpMV: each row keeps its upper blocks (c >= r,
+// diagonal first) with values; its lower blocks (c < r) are read transposed from row c's upper
+// storage through a position index, the way the reference's MatWithTinyBlocks::partial_mul uses
+// each stored (lower) block for both rows (containers/MatWithTinyBlocks.h:303-378).  Halves the
+// value bytes streamed per product; the transposed reads of recently processed rows hit L2 or
+// the Infinity Cache.
+struct SymSell {
+    std::vector<uint32_t> sptrU, lenU, colU, srcU;   // upper part; srcU = position of the block in K
+    std::vector<uint32_t> sptrL, lenL, colL, tposL;  // lower part: column, position in the upper part
+    uint64_t nU = 0, nL = 0;
+    uint64_t posU(uint32_t r, uint32_t k) const { return sptrU[r >> 6] + 64ull * k + (r & 63u); }
+    uint64_t posL(uint32_t r, uint32_t k) const { return sptrL[r >> 6] + 64ull * k + (r & 63u); }
+};
+// Returns false when the pattern is not symmetric or a position overflows 32 bits.
+bool build_sym_sell(const Pattern &pat, SymSell &S);
+
 // Block IC(0) schedule (containers/MatWithTinyBlocks.h:426-523 factorises in natural row order).
 // An elimination order is chosen (IC0_NATURAL: node order, the reference's factor;
 // IC0_COLOURED: first-fit colouring of the node graph, nodes sorted by colour), each node gets

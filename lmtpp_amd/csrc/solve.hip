@@ -27,15 +27,84 @@ enum { MODE_PLAIN = 0, MODE_PQ = 1, MODE_RES = 2 };
 // Block SpMV on SELL-64 storage: one lane per block row; the 64 lanes of a wave are 64
 // consecutive rows, so every value-plane and column load of slot k is one contiguous run.
 // Rows shorter than their slice stop early (masked lanes fetch nothing).
+// FMT_FULL: every block of the row is stored in the row (row lengths from row_ptr).
+// FMT_SYM: the row stores its upper blocks (c >= r); each lower block K_rc = K_cr^T is read from
+// row c's upper storage through tposL, as MatWithTinyBlocks::partial_mul uses every stored block
+// for both rows (containers/MatWithTinyBlocks.h:303-378).
 // MODE_PQ: y = K x and the partial of p.q.  MODE_RES: y = b - K x (FP64) with partials of ||y||^2
 // and max|y|.  The matching k_finalize turns the partials into the sub-problem's scalars.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int MODE, int UNROLL>
-__global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, const uint32_t *__restrict__ sptr,
-                                              const uint32_t *__restrict__ colS, const T *__restrict__ Kv, uint64_t ps,
-                                              const T *__restrict__ x, const double *__restrict__ b,
-                                              T *__restrict__ y, WgTables wt, const ProbScalars *S, double *partials,
-                                              int sel) {
+struct SpmvArgs {
+    const uint32_t *len;    // FMT_FULL: row_ptr; FMT_SYM: upper lengths
+    const uint32_t *sptr, *col;
+    uint64_t ps;
+    const uint32_t *sptrL, *lenL, *colL, *tposL;  // FMT_SYM lower part
+};
+
+template <typename T, int UNROLL>
+__device__ inline void spmv_rows(const uint32_t *__restrict__ colS, const T *__restrict__ Kv, uint64_t ps, uint64_t base,
+                                 uint32_t len, const T *__restrict__ x, T &s0, T &s1, T &s2) {
+    uint32_t k = 0;
+    for (; k + UNROLL <= len; k += UNROLL) {
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const uint64_t p = base + 64ull * (k + u);
+            const uint32_t c = colS[p];
+            const T a0 = Kv[p], a1 = Kv[ps + p], a2 = Kv[2 * ps + p];
+            const T a3 = Kv[3 * ps + p], a4 = Kv[4 * ps + p], a5 = Kv[5 * ps + p];
+            const T a6 = Kv[6 * ps + p], a7 = Kv[7 * ps + p], a8 = Kv[8 * ps + p];
+            const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+            s0 += a0 * x0 + a1 * x1 + a2 * x2;
+            s1 += a3 * x0 + a4 * x1 + a5 * x2;
+            s2 += a6 * x0 + a7 * x1 + a8 * x2;
+        }
+    }
+    for (; k < len; ++k) {
+        const uint64_t p = base + 64ull * k;
+        const uint32_t c = colS[p];
+        const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+        s0 += Kv[p] * x0 + Kv[ps + p] * x1 + Kv[2 * ps + p] * x2;
+        s1 += Kv[3 * ps + p] * x0 + Kv[4 * ps + p] * x1 + Kv[5 * ps + p] * x2;
+        s2 += Kv[6 * ps + p] * x0 + Kv[7 * ps + p] * x1 + Kv[8 * ps + p] * x2;
+    }
+}
+
+// lower blocks of FMT_SYM: K_rc x_c = K_cr^T x_c with K_cr at position tposL in the upper part
+template <typename T, int UNROLL>
+__device__ inline void spmv_rows_t(const uint32_t *__restrict__ colL, const uint32_t *__restrict__ tposL,
+                                   const T *__restrict__ Kv, uint64_t ps, uint64_t base, uint32_t len,
+                                   const T *__restrict__ x, T &s0, T &s1, T &s2) {
+    uint32_t k = 0;
+    for (; k + UNROLL <= len; k += UNROLL) {
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const uint64_t pl = base + 64ull * (k + u);
+            const uint32_t c = colL[pl];
+            const uint64_t p = tposL[pl];
+            const T a0 = Kv[p], a1 = Kv[ps + p], a2 = Kv[2 * ps + p];
+            const T a3 = Kv[3 * ps + p], a4 = Kv[4 * ps + p], a5 = Kv[5 * ps + p];
+            const T a6 = Kv[6 * ps + p], a7 = Kv[7 * ps + p], a8 = Kv[8 * ps + p];
+            const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+            s0 += a0 * x0 + a3 * x1 + a6 * x2;
+            s1 += a1 * x0 + a4 * x1 + a7 * x2;
+            s2 += a2 * x0 + a5 * x1 + a8 * x2;
+        }
+    }
+    for (; k < len; ++k) {
+        const uint64_t pl = base + 64ull * k;
+        const uint32_t c = colL[pl];
+        const uint64_t p = tposL[pl];
+        const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+        s0 += Kv[p] * x0 + Kv[3 * ps + p] * x1 + Kv[6 * ps + p] * x2;
+        s1 += Kv[ps + p] * x0 + Kv[4 * ps + p] * x1 + Kv[7 * ps + p] * x2;
+        s2 += Kv[2 * ps + p] * x0 + Kv[5 * ps + p] * x1 + Kv[8 * ps + p] * x2;
+    }
+}
+
+template <typename T, int MODE, int UNROLL, int FMT>
+__global__ __launch_bounds__(256) void k_spmv(SpmvArgs A, const T *__restrict__ Kv, const T *__restrict__ x,
+                                              const double *b, T *__restrict__ y, WgTables wt, const ProbScalars *S,
+                                              double *partials, int sel) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB) return;
@@ -43,31 +112,13 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, c
     const uint32_t r = wt.row0[L] + threadIdx.x;
     double acc[2] = {0.0, 0.0};
     if (r < wt.row1[L]) {
-        const uint32_t len = rp[r + 1] - rp[r];
-        const uint64_t base = sptr[r >> 6] + (r & 63u);
         T s0 = 0, s1 = 0, s2 = 0;
-        uint32_t k = 0;
-        for (; k + UNROLL <= len; k += UNROLL) {
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const uint64_t p = base + 64ull * (k + u);
-                const uint32_t c = colS[p];
-                const T a0 = Kv[p], a1 = Kv[ps + p], a2 = Kv[2 * ps + p];
-                const T a3 = Kv[3 * ps + p], a4 = Kv[4 * ps + p], a5 = Kv[5 * ps + p];
-                const T a6 = Kv[6 * ps + p], a7 = Kv[7 * ps + p], a8 = Kv[8 * ps + p];
-                const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
-                s0 += a0 * x0 + a1 * x1 + a2 * x2;
-                s1 += a3 * x0 + a4 * x1 + a5 * x2;
-                s2 += a6 * x0 + a7 * x1 + a8 * x2;
-            }
-        }
-        for (; k < len; ++k) {
-            const uint64_t p = base + 64ull * k;
-            const uint32_t c = colS[p];
-            const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
-            s0 += Kv[p] * x0 + Kv[ps + p] * x1 + Kv[2 * ps + p] * x2;
-            s1 += Kv[3 * ps + p] * x0 + Kv[4 * ps + p] * x1 + Kv[5 * ps + p] * x2;
-            s2 += Kv[6 * ps + p] * x0 + Kv[7 * ps + p] * x1 + Kv[8 * ps + p] * x2;
+        const uint64_t off = r & 63u;
+        if (FMT == FMT_FULL) {
+            spmv_rows<T, UNROLL>(A.col, Kv, A.ps, A.sptr[r >> 6] + off, A.len[r + 1] - A.len[r], x, s0, s1, s2);
+        } else {
+            spmv_rows<T, UNROLL>(A.col, Kv, A.ps, A.sptr[r >> 6] + off, A.len[r], x, s0, s1, s2);
+            spmv_rows_t<T, UNROLL>(A.colL, A.tposL, Kv, A.ps, A.sptrL[r >> 6] + off, A.lenL[r], x, s0, s1, s2);
         }
         const T sv[3] = {s0, s1, s2};
 #pragma unroll
@@ -90,6 +141,18 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp, c
         wg_partial<1, 0u>(v1, partials, gridDim.x, L);
     } else {
         wg_partial<2, 2u>(acc, partials, gridDim.x, L);
+    }
+}
+
+// Values of the symmetric storage from the full K: plane q of upper position p is K[q][src[p]]
+// (zero at padding positions).
+template <typename T>
+__global__ void k_to_sym(const double *__restrict__ K, uint64_t ps, const uint32_t *__restrict__ src, T *__restrict__ Ku,
+                         uint64_t psU) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < psU; p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = src[p];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) Ku[j * psU + p] = q == 0xffffffffu ? (T)0 : (T)K[j * ps + q];
     }
 }
 
@@ -357,9 +420,25 @@ template <typename T, int MODE>
 static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int sel) {
     const dim3 g(P.n_wg), blk(256);
     const WgTables wt = tables(P);
+    const int pr = sizeof(T) == 4 ? 1 : 0;
+    if (P.spmv.format[pr] == FMT_SYM) {
+        const SymDev &Y = P.sym;
+        const SpmvArgs A{Y.d_lenU, Y.d_sptrU, Y.d_colU, Y.psU, Y.d_sptrL, Y.d_lenL, Y.d_colL, Y.d_tposL};
+        const T *Ks = pr ? reinterpret_cast<const T *>(Y.d_K32) : reinterpret_cast<const T *>(Y.d_K);
 #define FEMGPU_SPMV_CASE(U) \
-    hipLaunchKernelGGL((k_spmv<T, MODE, U>), g, blk, 0, P.stream, P.d_rp, P.d_sptr, P.d_col, K, P.ps, x, b, y, wt, P.d_ps, P.d_partials, sel)
-    switch (P.spmv.unroll) {
+    hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_SYM>), g, blk, 0, P.stream, A, Ks, x, b, y, wt, P.d_ps, P.d_partials, sel)
+        switch (P.spmv.unroll[pr]) {
+            case 1: FEMGPU_SPMV_CASE(1); break;
+            case 3: FEMGPU_SPMV_CASE(3); break;
+            default: FEMGPU_SPMV_CASE(2); break;
+        }
+#undef FEMGPU_SPMV_CASE
+        return;
+    }
+    const SpmvArgs A{P.d_rp, P.d_sptr, P.d_col, P.ps, nullptr, nullptr, nullptr, nullptr};
+#define FEMGPU_SPMV_CASE(U) \
+    hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_FULL>), g, blk, 0, P.stream, A, K, x, b, y, wt, P.d_ps, P.d_partials, sel)
+    switch (P.spmv.unroll[pr]) {
         case 1: FEMGPU_SPMV_CASE(1); break;
         case 3: FEMGPU_SPMV_CASE(3); break;
         default: FEMGPU_SPMV_CASE(2); break;
@@ -367,8 +446,71 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
 #undef FEMGPU_SPMV_CASE
 }
 
+namespace {
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <class T>
+hipError_t upload(T **p, const std::vector<T> &h, size_t n_alloc) {
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(1, n_alloc) * sizeof(T));
+    if (e == hipSuccess && !h.empty()) e = hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+}  // namespace
+
+void sym_free(SymDev &S) {
+    dfree(S.d_sptrU); dfree(S.d_lenU); dfree(S.d_colU); dfree(S.d_srcU);
+    dfree(S.d_sptrL); dfree(S.d_lenL); dfree(S.d_colL); dfree(S.d_tposL);
+    dfree(S.d_K); dfree(S.d_K32);
+    S = SymDev();
+}
+
+int sym_ensure(Problem &P, int precision) {
+    SymDev &Y = P.sym;
+    if (!Y.built) {
+        SymSell H;
+        if (!build_sym_sell(P.pat, H)) return 1;
+        Y.psU = (H.nU + 63) / 64 * 64;
+        Y.nL = H.nL;
+        std::vector<uint32_t> src = H.srcU;
+        src.resize(std::max<uint64_t>(Y.psU, 1), 0xffffffffu);
+        if (upload(&Y.d_sptrU, H.sptrU, H.sptrU.size()) || upload(&Y.d_lenU, H.lenU, H.lenU.size()) ||
+            upload(&Y.d_colU, H.colU, H.colU.size()) || upload(&Y.d_srcU, src, src.size()) ||
+            upload(&Y.d_sptrL, H.sptrL, H.sptrL.size()) || upload(&Y.d_lenL, H.lenL, H.lenL.size()) ||
+            upload(&Y.d_colL, H.colL, H.colL.size()) || upload(&Y.d_tposL, H.tposL, H.tposL.size()))
+            return 5;
+        Y.built = true;
+        destroy_graph(P);
+    }
+    if (Y.valid[precision]) return 0;
+    const dim3 g(8192), blk(256);
+    if (precision == 1) {
+        if (!Y.d_K32 && hipMalloc(&Y.d_K32, sizeof(float) * 9 * std::max<uint64_t>(Y.psU, 1)) != hipSuccess) return 5;
+        hipLaunchKernelGGL(k_to_sym<float>, g, blk, 0, P.stream, P.d_K, P.ps, Y.d_srcU, Y.d_K32, Y.psU);
+    } else {
+        if (!Y.d_K && hipMalloc(&Y.d_K, sizeof(double) * 9 * std::max<uint64_t>(Y.psU, 1)) != hipSuccess) return 5;
+        hipLaunchKernelGGL(k_to_sym<double>, g, blk, 0, P.stream, P.d_K, P.ps, Y.d_srcU, Y.d_K, Y.psU);
+    }
+    if (hipGetLastError() != hipSuccess) return 5;
+    Y.valid[precision] = true;
+    return 0;
+}
+
+// Values the SpMV of this precision reads (FP32 full copy or symmetric storage).
+static int spmv_ensure(Problem &P, int precision) {
+    if (P.spmv.format[precision] == FMT_SYM) return sym_ensure(P, precision);
+    if (precision == 1 && !P.k32_valid) {
+        if (!P.d_K32 && hipMalloc(&P.d_K32, sizeof(float) * 9 * P.ps) != hipSuccess) return 5;
+        if (launch_to_fp32(P) != hipSuccess) return 5;
+        P.k32_valid = true;
+    }
+    return 0;
+}
+
 double time_spmv(Problem &P, int precision, int reps) {
-    if (precision == 1 && !P.k32_valid) return -1;
+    if (spmv_ensure(P, precision)) return -1;
     (void)hipEventRecord(P.ev[2], P.stream);
     for (int i = 0; i < reps; ++i) {
         if (precision == 1) launch_spmv_t<float, MODE_PLAIN>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, 0);
@@ -381,20 +523,34 @@ double time_spmv(Problem &P, int precision, int reps) {
     return (double)ms / reps;
 }
 
-// Picks the SpMV unroll depth by timing each variant on this matrix (FP64, 10 launches each).
+// Times every SpMV variant (storage format x SELL slots per loop trip) on this matrix, for each
+// precision (10 launches after 2 warm-up launches), and keeps the fastest.  The symmetric storage
+// holds a second copy of the values, so it is kept only when it beats the full storage by more
+// than 3%; storage no precision picked is released.
 int autotune_spmv(Problem &P) {
-    double best = 1e30;
-    int pick = P.spmv.unroll;
-    for (int u : {1, 2, 3}) {
-        P.spmv.unroll = u;
-        time_spmv(P, 0, 2);
-        const double t = time_spmv(P, 0, 10);
-        if (t > 0 && t < best) { best = t; pick = u; }
+    for (int pr = 0; pr < 2; ++pr) {
+        double best[2] = {1e30, 1e30};
+        int bu[2] = {P.spmv.unroll[pr], P.spmv.unroll[pr]};
+        for (int f : {FMT_FULL, FMT_SYM})
+            for (int u : {1, 2, 3}) {
+                P.spmv.format[pr] = f;
+                P.spmv.unroll[pr] = u;
+                if (time_spmv(P, pr, 2) < 0) continue;
+                const double t = time_spmv(P, pr, 10);
+                if (t > 0 && t < best[f]) { best[f] = t; bu[f] = u; }
+            }
+        const int f = best[FMT_SYM] < 0.97 * best[FMT_FULL] ? FMT_SYM : FMT_FULL;
+        P.spmv.format[pr] = f;
+        P.spmv.unroll[pr] = bu[f];
     }
-    P.spmv.unroll = pick;
+    if (P.spmv.format[0] != FMT_SYM && P.spmv.format[1] != FMT_SYM) sym_free(P.sym);
+    else {
+        if (P.spmv.format[0] != FMT_SYM) { dfree(P.sym.d_K); P.sym.valid[0] = false; }
+        if (P.spmv.format[1] != FMT_SYM) { dfree(P.sym.d_K32); P.sym.valid[1] = false; }
+    }
     P.spmv.tuned = true;
     destroy_graph(P);
-    return pick;
+    return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 void destroy_graph(Problem &P) {
@@ -477,8 +633,10 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
         for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, fp32, d_x);
         return hipGetLastError() == hipSuccess ? 0 : 5;
     }
-    const uint64_t key = (reinterpret_cast<uint64_t>(d_x) << 8) ^ ((uint64_t)chunk << 5) ^ ((uint64_t)P.spmv.unroll << 3) ^
-                         ((uint64_t)PRE << 1) ^ (uint64_t)fp32;
+    const int pr = fp32 ? 1 : 0;
+    const uint64_t key = (reinterpret_cast<uint64_t>(d_x) << 12) ^ ((uint64_t)chunk << 6) ^
+                         ((uint64_t)P.spmv.unroll[pr] << 4) ^ ((uint64_t)P.spmv.format[pr] << 3) ^ ((uint64_t)PRE << 1) ^
+                         (uint64_t)fp32;
     if (!P.graph_exec || P.graph_key != key) {
         destroy_graph(P);
         hipGraph_t g = nullptr;
@@ -606,11 +764,9 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
 
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out) {
     for (uint32_t s = 0; s < P.n_sub; ++s) out[s] = SolveOut{};
-    if (o.precision == 1 && !P.k32_valid) {
-        if (!P.d_K32 && hipMalloc(&P.d_K32, sizeof(float) * 9 * P.ps) != hipSuccess) return 5;
-        if (launch_to_fp32(P) != hipSuccess) return 5;
-        P.k32_valid = true;
-    }
+    int rc = spmv_ensure(P, 0);
+    if (!rc && o.precision == 1) rc = spmv_ensure(P, 1);
+    if (rc) return rc;
     switch (o.precond) {
         case 1: return solve_impl<PRE_BLOCK_JACOBI>(P, d_rhs, d_x, o, out, 0);
         case 2: return solve_impl<PRE_IC0>(P, d_rhs, d_x, o, out, IC0_NATURAL);

@@ -209,10 +209,22 @@ class Formulation:
         return v.value
 
     def autotune(self):
-        """Pick the fastest block-SpMV variant for this matrix (returns the SELL unroll depth)."""
-        u = C.c_int()
-        A.check(self._lib.femgpu_autotune(self._h, C.byref(u)), "autotune")
-        return u.value
+        """Pick the fastest block-SpMV variant (storage format, SELL slots per loop trip) for this
+        matrix in FP64 and FP32; returns spmv_config()."""
+        A.check(self._lib.femgpu_autotune(self._h, None), "autotune")
+        return self.spmv_config()
+
+    def spmv_config(self):
+        """{"fp64": (format, unroll), "fp32": (format, unroll)}, format SPMV_FULL or SPMV_SYM."""
+        f = (C.c_int * 2)()
+        u = (C.c_int * 2)()
+        A.check(self._lib.femgpu_get_spmv_config(self._h, f, u), "get_spmv_config")
+        return {"fp64": (f[0], u[0]), "fp32": (f[1], u[1])}
+
+    def set_spmv_config(self, fp64=(A.SPMV_FULL, 2), fp32=(A.SPMV_FULL, 2)):
+        f = (C.c_int * 2)(int(fp64[0]), int(fp32[0]))
+        u = (C.c_int * 2)(int(fp64[1]), int(fp32[1]))
+        A.check(self._lib.femgpu_set_spmv_config(self._h, f, u), "set_spmv_config")
 
     def set_graph_replay(self, enable=True):
         A.check(self._lib.femgpu_set_graph_replay(self._h, int(bool(enable))), "set_graph_replay")

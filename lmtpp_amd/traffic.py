@@ -5,7 +5,8 @@ Workload (run it under rocprofv3 --pmc, one counter per run):
     python -m lmtpp_amd.traffic run [--n 100]
 
 launches the plain copy kernel (k_copy, 2^26 doubles: exactly 512 MiB read and 512 MiB written),
-then 5 FP64 and 5 FP32 block SpMV launches on Hexa8 n^3.  Then
+then 5 FP64 and 5 FP32 block SpMV launches on Hexa8 n^3 for each storage format (full, and
+symmetric: upper blocks stored, lower ones read transposed).  Then
 
     python -m lmtpp_amd.traffic parse <fetch_dir> <write_dir> > profiles/spmv_traffic.json
 
@@ -17,6 +18,7 @@ import csv
 import glob
 import json
 import os
+import re
 import sys
 
 COPY_DOUBLES = 1 << 26
@@ -38,10 +40,26 @@ def run(n):
     f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
     f.assemble()
     f.set_result(np.zeros(3 * len(c)))
-    t64 = f.time_spmv(L.FP64, 5)
-    t32 = f.time_spmv(L.MIXED, 5)
     sz = f.sizes()
-    print(json.dumps({"nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64, "spmv_fp32_ms": t32}), flush=True)
+    for fmt in (A.SPMV_FULL, A.SPMV_SYM):
+        f.set_spmv_config(fp64=(fmt, 2), fp32=(fmt, 2))
+        t64 = f.time_spmv(L.FP64, 5)
+        t32 = f.time_spmv(L.MIXED, 5)
+        print(json.dumps({"format": fmt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
+                          "spmv_fp32_ms": t32}), flush=True)
+
+
+def spmv_bytes(nnzb, n_nodes, fp32, fmt):
+    """Algorithmic bytes of one block SpMV: every stored value and index read once, x read once,
+    y written once.  Symmetric format: upper blocks (incl. diagonal) with values and columns,
+    lower blocks as column + position (their values are the upper blocks of other rows)."""
+    v = 36 if fp32 else 72
+    ndof = 3 * n_nodes
+    vec = ndof * (8 if fp32 else 16)
+    if fmt == 0:
+        return nnzb * (v + 4) + n_nodes * 4 + vec
+    nu, nl = (nnzb + n_nodes) // 2, (nnzb - n_nodes) // 2
+    return nu * (v + 4) + nl * 8 + n_nodes * 8 + vec
 
 
 def _counters(d):
@@ -50,11 +68,15 @@ def _counters(d):
         with open(p) as f:
             rows += list(csv.DictReader(f))
     per = {}
+    pat = re.compile(r"k_spmv<(double|float), 0, (\d), (\d)>")
     for r in rows:
         name = r.get("Kernel_Name", "")
-        key = "copy" if "k_copy" in name else ("spmv_fp64" if "k_spmv<double" in name else
-                                                 "spmv_fp32" if "k_spmv<float" in name else None)
-        if key is None:
+        m = pat.search(name)
+        if "k_copy" in name:
+            key = "copy"
+        elif m:
+            key = "spmv_%s%s" % ("fp64" if m.group(1) == "double" else "fp32", "" if m.group(3) == "0" else "_sym")
+        else:
             continue
         per.setdefault(key, []).append(float(r["Counter_Value"]))
     return per
@@ -65,10 +87,11 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
     known_kib = 8.0 * COPY_DOUBLES / 1024.0
     f_scale = known_kib / (sum(fe["copy"]) / len(fe["copy"]))
     w_scale = known_kib / (sum(wr["copy"]) / len(wr["copy"]))
-    ndof = 3 * n_nodes
     out = {"copy_scale": {"FETCH_SIZE": f_scale, "WRITE_SIZE": w_scale},
            "note": "bytes per launch = mean counter (KiB) x 1024 x scale; scale = known copy KiB / counted KiB"}
-    for key, alg in (("spmv_fp64", nnzb * 76 + n_nodes * 4 + ndof * 16), ("spmv_fp32", nnzb * 40 + n_nodes * 4 + ndof * 8)):
+    for key, fp32, fmt in (("spmv_fp64", False, 0), ("spmv_fp32", True, 0), ("spmv_fp64_sym", False, 1),
+                           ("spmv_fp32_sym", True, 1)):
+        alg = spmv_bytes(nnzb, n_nodes, fp32, fmt)
         if key not in fe or key not in wr:
             continue
         fb = sum(fe[key]) / len(fe[key]) * 1024.0 * f_scale

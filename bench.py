@@ -31,11 +31,9 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(nnzb, n_nodes, fp32):
-    ndof = 3 * n_nodes
-    if fp32:
-        return nnzb * (36 + 4) + n_nodes * 4 + ndof * 8
-    return nnzb * (72 + 4) + n_nodes * 4 + ndof * 16
+def spmv_bytes(nnzb, n_nodes, fp32, fmt=0):
+    from lmtpp_amd.traffic import spmv_bytes as sb
+    return sb(nnzb, n_nodes, fp32, fmt)
 
 
 def load_traffic(name):
@@ -422,8 +420,9 @@ def main():
     reps = 50
     t32 = f.time_spmv(L.MIXED, reps)
     t64 = f.time_spmv(L.FP64, reps)
-    b32 = spmv_bytes(sz["nnzb"], sz["n_nodes"], True)
-    b64 = spmv_bytes(sz["nnzb"], sz["n_nodes"], False)
+    fmt32, fmt64 = spmv_cfg["fp32"][0], spmv_cfg["fp64"][0]
+    b32 = spmv_bytes(sz["nnzb"], sz["n_nodes"], True, fmt32)
+    b64 = spmv_bytes(sz["nnzb"], sz["n_nodes"], False, fmt64)
     ach32 = b32 / (t32 * 1e-3) / 1e9
     ach64 = b64 / (t64 * 1e-3) / 1e9
 
@@ -448,12 +447,14 @@ def main():
         "rel_res_tol": mtol, "iters_mixed": int(stm.iters), "refreshes_mixed": int(stm.refreshes),
         "iters_fp64": int(st64.iters),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": load_traffic("spmv_fp32"),
-                     "kernel": "k_spmv<float,0,%d> (FP32 SELL-64 block SpMV)" % unroll, "avg_launch_ms": round(t32, 4),
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": load_traffic("spmv_fp32" + ("_sym" if fmt32 else "")),
+                     "kernel": "k_spmv<float,0,%d,%d> (FP32 SELL-64 block SpMV, %s storage)" % (
+                         unroll, fmt32, fmt_name[fmt32]), "avg_launch_ms": round(t32, 4),
                      "algorithmic_bytes": b32},
         "roofline_fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                "avg_launch_ms": round(t64, 4), "algorithmic_bytes": b64,
-                               "traffic": load_traffic("spmv_fp64")},
+                               "traffic": load_traffic("spmv_fp64" + ("_sym" if fmt64 else "")),
+                               "kernel": "k_spmv<double,0,%d,%d>" % (spmv_cfg["fp64"][1], fmt64)},
         "preconditioners": precond,
         "after_solve": {"kernel": "k_after_solve<2>", "ms": round(a_ms, 4),
                         "melem_s": round(sz["n_elem"] / (a_ms * 1e-3) / 1e6, 1), "algorithmic_bytes": as_bytes,

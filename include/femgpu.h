@@ -262,12 +262,19 @@ FEMGPU_API int femgpu_stream_copy(size_t n, int reps, double *avg_ms);
 typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1 } femgpu_spmv_format;
 /* Times every block-SpMV variant (format x SELL slots per loop trip 1, 2, 3) on this assembled
  * matrix, for FP64 and for the FP32 copy, and keeps the fastest of each; *unroll receives the
- * FP64 pick (may be NULL). */
+ * FP64 pick (may be NULL).  Then times the element-loop variants (femgpu_set_assembly_rows) and
+ * keeps the fastest, re-assembling K and F with it. */
 FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
 /* Current (or forced) variant per precision: index 0 FP64, 1 FP32; format femgpu_spmv_format,
  * unroll 1..3. */
 FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2]);
 FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2]);
+/* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
+ * into K (read-modify-write).  rows = R (a power of two, at most *max_rows): each workgroup owns R
+ * consecutive block rows, accumulates their blocks from the incident elements colour by colour in
+ * LDS and writes the rows once.  The default is the largest R. */
+FEMGPU_API int femgpu_set_assembly_rows(femgpu_problem *p, uint32_t rows);
+FEMGPU_API int femgpu_get_assembly_rows(femgpu_problem *p, uint32_t *rows, uint32_t *max_rows);
 /* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */
 FEMGPU_API int femgpu_set_graph_replay(femgpu_problem *p, int enable);
 

@@ -52,11 +52,21 @@ __device__ inline double jac_inverse(const double (&J)[3][3], double (&Ji)[3][3]
     return det;
 }
 
+// Phase B alone, coordinates already in L.x (written before the barrier this function starts with).
+template <int K>
+__device__ inline void elem_gradients(ElemLds<K> &L, int t, bool valid, int *flags);
+
 template <int K>
 __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t e, const uint32_t *__restrict__ conn,
                                      const double *__restrict__ coords, int *flags) {
-    constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP;
+    constexpr int NPE = ET<K>::NPE;
     if (valid && t < NPE * 3) L.x[t] = coords[3 * (size_t)conn[(size_t)e * NPE + t / 3] + t % 3];
+    elem_gradients<K>(L, t, valid, flags);
+}
+
+template <int K>
+__device__ inline void elem_gradients(ElemLds<K> &L, int t, bool valid, int *flags) {
+    constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP;
     __syncthreads();
     if (valid && t < NGP * NPE) {
         const int q = t / NPE, a = t % NPE;
@@ -139,6 +149,108 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
         for (int q = 0; q < NGP; ++q) gen::load(L.N[q][a], m.bf0, m.bf1, m.bf2, L.wd[q], f3);
         F[3 * (size_t)conn[(size_t)e * NPE + a] + d] += f3[d];
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row-group element loop: workgroup G owns block rows [G R, G R + R) (R divides 64, so the group
+// lies in one SELL slice).  It walks the elements incident to its rows colour by colour, the same
+// order as the colour-pass kernel above, computes each element's geometry in a team of lanes and
+// adds the blocks (and loads) of its own rows into an LDS accumulator; rows of one colour's
+// elements are distinct, so no two lanes add to one entry at once and the summation order of
+// every entry is the colour order.  The finished rows are then streamed out once, each (slot,
+// plane) a contiguous run of R values: K and F are written, not read-modified-written, and need
+// no clearing.  Elements touching several groups redo their geometry in each (about 4x on a Hexa
+// lattice with R = 64); every block is evaluated once, by the group owning its row.
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t ASM_SLOTS = 1792;   // R x (slice width) block slots accumulated in LDS
+constexpr int ASM_THREADS = 512;
+
+template <int K>
+__global__ __launch_bounds__(ASM_THREADS) void k_assemble_rows(
+    uint32_t R, uint32_t n_rows, uint32_t nc, const uint32_t *__restrict__ gptr, const uint32_t *__restrict__ gel,
+    const uint32_t *__restrict__ conn, const double *__restrict__ coords, const double *__restrict__ E,
+    const uint32_t *__restrict__ elem_sub, const SubMaterial *__restrict__ mat, const uint32_t *__restrict__ slot,
+    const uint32_t *__restrict__ sptr, double *__restrict__ Kv, uint64_t ps, double *__restrict__ F, int do_mat,
+    int do_vec, int *flags) {
+    constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP, TEAM = ET<K>::TEAM, TPW = ASM_THREADS / TEAM;
+    __shared__ double acc[ASM_SLOTS * 9];   // acc[(k * 9 + q) * R + i]: row r0 + i, slot k, plane q
+    __shared__ double facc[64 * 3];
+    __shared__ ElemLds<K> lds[TPW];
+    const uint32_t G = blockIdx.x, r0 = G * R, r1 = min(r0 + R, n_rows);
+    const uint32_t s = r0 >> 6, W = (sptr[s + 1] - sptr[s]) >> 6, base = sptr[s];
+    for (uint32_t i = threadIdx.x; i < W * 9 * R; i += blockDim.x) acc[i] = 0.0;
+    for (uint32_t i = threadIdx.x; i < R * 3; i += blockDim.x) facc[i] = 0.0;
+    const int team = threadIdx.x / TEAM, t = threadIdx.x % TEAM;
+    ElemLds<K> &L = lds[team];
+    // Batches of TPW elements never straddle colours.  The coordinates of the next batch are
+    // loaded into registers while the current one computes (the gather latency is otherwise
+    // exposed: one workgroup per CU).
+    const uint32_t *gp = gptr + (size_t)G * (nc + 1);
+    uint32_t c = 0, bb = gp[0];
+    while (c < nc && bb >= gp[c + 1]) bb = gp[++c];
+    uint32_t e_nx = 0;
+    bool v_nx = false;
+    double x_nx = 0.0;
+    auto fetch = [&](uint32_t cc, uint32_t b) {
+        const uint32_t idx = b + team;
+        v_nx = cc < nc && idx < gp[cc + 1];
+        e_nx = v_nx ? gel[idx] : 0;
+        if (v_nx && t < NPE * 3) x_nx = coords[3 * (size_t)conn[(size_t)e_nx * NPE + t / 3] + t % 3];
+    };
+    fetch(c, bb);
+    while (c < nc) {
+        const bool valid = v_nx;
+        const uint32_t e = e_nx;
+        if (valid && t < NPE * 3) L.x[t] = x_nx;
+        // advance to the next batch (next colour when this one is exhausted) and prefetch it
+        bb += TPW;
+        while (c < nc && bb >= gp[c + 1]) bb = gp[++c];
+        fetch(c, bb);
+        elem_gradients<K>(L, t, valid, flags);   // its barriers order the batches
+        if (!valid) continue;
+        const SubMaterial m = mat[elem_sub[e]];
+        if (do_mat && t < NPE * NPE) {
+            const int a = t / NPE, b = t % NPE;
+            const uint32_t row = conn[(size_t)e * NPE + a];
+            if (row >= r0 && row < r1) {
+                double H[3];
+                gen::hooke(E[e], m.nu, H);
+                double k[9];
+                elem_block<K>(L, a < b ? a : b, a < b ? b : a, H, k);
+                const uint32_t kk = (slot[(size_t)e * NPE * NPE + t] - base - (row & 63u)) >> 6;
+                double *A = acc + (size_t)kk * 9 * R + (row - r0);
+                if (a == b) {
+                    k[3] = k[1]; k[6] = k[2]; k[7] = k[5];
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) A[q * R] += k[q];
+                } else if (a < b) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) A[q * R] += k[q];
+                } else {  // block (a, b) = block (b, a)^T
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) A[q * R] += k[3 * (q % 3) + q / 3];
+                }
+            }
+        }
+        if (do_vec && t < NPE * 3) {
+            const int a = t / 3, d = t % 3;
+            const uint32_t row = conn[(size_t)e * NPE + a];
+            if (row >= r0 && row < r1) {
+                double f3[3] = {0, 0, 0};
+                for (int q = 0; q < NGP; ++q) gen::load(L.N[q][a], m.bf0, m.bf1, m.bf2, L.wd[q], f3);
+                facc[(row - r0) * 3 + d] += f3[d];
+            }
+        }
+    }
+    __syncthreads();
+    if (do_mat)
+        for (uint32_t i = threadIdx.x; i < W * 9 * R; i += blockDim.x) {
+            const uint32_t li = i % R, kq = i / R, q = kq % 9, kk = kq / 9;
+            if (r0 + li < r1) Kv[q * ps + base + 64ull * kk + ((r0 + li) & 63u)] = acc[i];
+        }
+    if (do_vec)
+        for (uint32_t i = threadIdx.x; i < R * 3; i += blockDim.x)
+            if (r0 + i / 3 < r1) F[3 * (size_t)r0 + i] = facc[i];
 }
 
 // Skin loads, one thread per face, faces coloured like elements.  Gauss tables of the face
@@ -239,14 +351,36 @@ static hipError_t assemble_kind(Problem &P, int do_mat, int do_vec) {
     return hipGetLastError();
 }
 
+template <int K>
+static void assemble_rows_kind(Problem &P, int do_mat, int do_vec) {
+    hipLaunchKernelGGL(k_assemble_rows<K>, dim3(P.rg.n_groups), dim3(ASM_THREADS), 0, P.stream, P.rg.R, P.n_nodes,
+                       P.rg.n_colors, P.d_gptr, P.d_gel, P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.d_slot,
+                       P.d_sptr, P.d_K, P.ps, P.d_F, do_mat, do_vec, P.d_flags);
+}
+
+uint32_t max_assembly_rows(const Problem &P) {
+    uint32_t w = 0;
+    for (uint32_t r = 0; r < P.n_nodes; ++r) w = std::max(w, P.pat.row_ptr[r + 1] - P.pat.row_ptr[r]);
+    uint32_t R = 64;
+    while (R > 0 && R * w > ASM_SLOTS) R /= 2;
+    return R;
+}
+
 hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
     hipError_t e = hipSuccess;
-    if (do_mat) e = launch_fill(P.d_K, 9 * P.ps, 0.0, P.stream);
-    if (e == hipSuccess && do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
-    if (e != hipSuccess) return e;
-    if (P.kind == TETRA) e = assemble_kind<TETRA>(P, do_mat, do_vec);
-    else if (P.kind == TETRA10) e = assemble_kind<TETRA10>(P, do_mat, do_vec);
-    else e = assemble_kind<HEXA>(P, do_mat, do_vec);
+    if (P.asm_rows && P.rg.R == P.asm_rows) {
+        if (P.kind == TETRA) assemble_rows_kind<TETRA>(P, do_mat, do_vec);
+        else if (P.kind == TETRA10) assemble_rows_kind<TETRA10>(P, do_mat, do_vec);
+        else assemble_rows_kind<HEXA>(P, do_mat, do_vec);
+        e = hipGetLastError();
+    } else {
+        if (do_mat) e = launch_fill(P.d_K, 9 * P.ps, 0.0, P.stream);
+        if (e == hipSuccess && do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
+        if (e != hipSuccess) return e;
+        if (P.kind == TETRA) e = assemble_kind<TETRA>(P, do_mat, do_vec);
+        else if (P.kind == TETRA10) e = assemble_kind<TETRA10>(P, do_mat, do_vec);
+        else e = assemble_kind<HEXA>(P, do_mat, do_vec);
+    }
     if (e != hipSuccess || !do_vec || !P.n_faces) return e;
     const int fk = face_kind_of(P.kind);
     for (uint32_t c = 0; c < P.fcol.n_colors(); ++c) {

@@ -70,6 +70,7 @@ void free_problem(Problem &P) {
     dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32);
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
+    dfree(P.d_gptr); dfree(P.d_gel);
     if (P.h_ps) (void)hipHostFree(P.h_ps);
     if (P.h_flags) (void)hipHostFree(P.h_flags);
     P.h_ps = nullptr;
@@ -97,6 +98,26 @@ void build_wg_tables(Problem &P, std::vector<uint32_t> &row0, std::vector<uint32
     }
     while (row0.size() % 8 || row0.empty()) { row0.push_back(0); row1.push_back(0); sub.push_back(NO_SUB); }
     P.n_wg = (uint32_t)row0.size();
+}
+
+// Element-loop variant: 0 = colour passes, R = row groups of R rows (a power of two <= 64 whose
+// LDS accumulator fits, see max_assembly_rows).
+int set_assembly_rows(Problem &P, uint32_t R) {
+    if (R == 0) {
+        P.asm_rows = 0;
+        return FEMGPU_OK;
+    }
+    if (R > 64 || (R & (R - 1)) || R > max_assembly_rows(P))
+        return fail(FEMGPU_BAD_INPUT, "assembly rows: a power of two, at most " + std::to_string(max_assembly_rows(P)));
+    if (P.rg.R != R) {
+        P.rg = build_row_groups(P.n_nodes, R, P.n_elem, P.npe, P.conn_h.data(), P.ecol);
+        dfree(P.d_gptr);
+        dfree(P.d_gel);
+        HIPCHK(upload(&P.d_gptr, P.rg.ptr.data(), P.rg.ptr.size()));
+        HIPCHK(upload(&P.d_gel, P.rg.elems.data(), P.rg.elems.size()));
+    }
+    P.asm_rows = R;
+    return FEMGPU_OK;
 }
 
 // Device buffers common to mesh problems and plain block systems.
@@ -204,7 +225,8 @@ int build_problem(femgpu_problem &H, uint32_t n_sub, const femgpu_mesh *meshes) 
     HIPCHK(upload(&P.d_mat, P.mat_h.data(), P.mat_h.size()));
     H.sub_faces.assign(n_sub, {});
     H.sub_trac.assign(n_sub, {});
-    return FEMGPU_OK;
+    P.conn_h = std::move(conn);
+    return set_assembly_rows(P, max_assembly_rows(P));
 }
 
 int set_material(femgpu_problem &H, uint32_t s, const femgpu_material *m) {
@@ -854,6 +876,41 @@ int femgpu_autotune(femgpu_problem *h, int *unroll) {
     if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "autotune: assemble first");
     if (autotune_spmv(P)) return fail(FEMGPU_HIP_ERROR, "autotune failed");
     if (unroll) *unroll = P.spmv.unroll[0];
+    if (P.from_bsr) return FEMGPU_OK;
+    // element loop: colour passes or row groups of Rmax, Rmax/2, Rmax/4 rows (min of 3 timings)
+    const uint32_t Rmax = max_assembly_rows(P);
+    uint32_t best_R = P.asm_rows;
+    double best = 1e30;
+    for (uint32_t R : {0u, Rmax, Rmax / 2, Rmax / 4}) {
+        if (R && R < 8) continue;
+        int rc = set_assembly_rows(P, R);
+        if (rc) return rc;
+        double t = 1e30;
+        for (int rep = 0; rep < 4; ++rep) {
+            HIPCHK(hipEventRecord(P.ev[2], P.stream));
+            HIPCHK(launch_assemble(P, 1, 1));
+            HIPCHK(hipEventRecord(P.ev[3], P.stream));
+            HIPCHK(hipEventSynchronize(P.ev[3]));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, P.ev[2], P.ev[3]));
+            if (rep) t = std::min(t, (double)ms);
+        }
+        if (t < best) { best = t; best_R = R; }
+    }
+    int rc = set_assembly_rows(P, best_R);
+    if (!rc) rc = assemble(*h, 1, P.assembled_vec ? 1 : 0);   // K and F as the chosen variant makes them
+    return rc;
+}
+
+int femgpu_set_assembly_rows(femgpu_problem *h, uint32_t rows) {
+    if (!h || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "set_assembly_rows: bad input");
+    return set_assembly_rows(h->P, rows);
+}
+
+int femgpu_get_assembly_rows(femgpu_problem *h, uint32_t *rows, uint32_t *max_rows) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_assembly_rows: null");
+    if (rows) *rows = h->P.asm_rows;
+    if (max_rows) *max_rows = h->P.from_bsr ? 0 : max_assembly_rows(h->P);
     return FEMGPU_OK;
 }
 

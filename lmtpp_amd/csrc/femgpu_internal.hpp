@@ -132,6 +132,12 @@ struct Problem {
     uint64_t graph_key = 0;
     Ic0 ic0;
     SymDev sym;
+    // element loop variant: 0 = colour passes over all elements (read-modify-write of K),
+    // R > 0 = row groups of R rows accumulated in LDS (work lists in rg / d_gptr, d_gel)
+    uint32_t asm_rows = 0;
+    std::vector<uint32_t> conn_h;  // host copy of the (offset) connectivity, for the work lists
+    RowGroups rg;
+    uint32_t *d_gptr = nullptr, *d_gel = nullptr;
 };
 
 // ---- launchers (assemble.hip) ----
@@ -139,6 +145,8 @@ hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t s);  // n even
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
+// Largest row-group size (power of two <= 64) whose accumulator fits the LDS budget; 0 if none.
+uint32_t max_assembly_rows(const Problem &P);
 hipError_t launch_constraints(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_to_fp32(Problem &P);
 hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, double *d_g);

@@ -330,6 +330,51 @@ Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32
     return C;
 }
 
+RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe, const uint32_t *conn,
+                           const Coloring &col) {
+    RowGroups G;
+    G.R = R;
+    G.n_groups = (n_rows + R - 1) / R;
+    G.n_colors = col.n_colors();
+    const uint32_t nc = G.n_colors;
+    std::vector<uint32_t> colour(n_elem), rank(n_elem);
+    for (uint32_t c = 0; c < nc; ++c)
+        for (uint32_t i = col.color_ptr[c]; i < col.color_ptr[c + 1]; ++i) {
+            colour[col.order[i]] = c;
+            rank[col.order[i]] = i;  // position in (colour, element) order
+        }
+    // (group, element) pairs, deduplicated per element
+    std::vector<uint32_t> cnt((size_t)G.n_groups * nc + 1, 0);
+    std::vector<uint32_t> gl;
+    gl.reserve(4 * (size_t)n_elem);
+    std::vector<uint64_t> pairs;
+    pairs.reserve(4 * (size_t)n_elem);
+    for (uint32_t e = 0; e < n_elem; ++e) {
+        uint32_t gs[10];
+        int ng = 0;
+        for (int a = 0; a < npe; ++a) {
+            const uint32_t g = conn[(size_t)e * npe + a] / R;
+            bool seen = false;
+            for (int k = 0; k < ng; ++k) seen |= gs[k] == g;
+            if (!seen) gs[ng++] = g;
+        }
+        for (int k = 0; k < ng; ++k) pairs.push_back(((uint64_t)gs[k] << 32) | rank[e]);
+    }
+    std::sort(pairs.begin(), pairs.end());
+    G.elems.resize(pairs.size());
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        const uint32_t g = uint32_t(pairs[i] >> 32), e = col.order[uint32_t(pairs[i])];
+        G.elems[i] = e;
+        cnt[(size_t)g * nc + colour[e] + 1]++;
+    }
+    for (size_t i = 0; i + 1 < cnt.size(); ++i) cnt[i + 1] += cnt[i];
+    // ptr[G*(nc+1)+c]: start of colour c in group G (colour nc = end of the group)
+    G.ptr.assign((size_t)G.n_groups * (nc + 1), 0);
+    for (uint32_t g = 0; g < G.n_groups; ++g)
+        for (uint32_t c = 0; c <= nc; ++c) G.ptr[(size_t)g * (nc + 1) + c] = cnt[(size_t)g * nc + c];
+    return G;
+}
+
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts) {
     const uint32_t n = (uint32_t)row_ptr.size() - 1;
     std::vector<uint32_t> cut(n_parts + 1);

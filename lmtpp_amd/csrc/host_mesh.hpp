@@ -53,6 +53,16 @@ struct Coloring {
 };
 Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t *conn);
 
+// Row-group assembly work lists: rows are cut into groups of R consecutive rows; group G lists
+// the elements incident to its rows, sorted by (colour, element), colour c of group G spanning
+// elems[ptr[G*(nc+1)+c] .. ptr[G*(nc+1)+c+1]).
+struct RowGroups {
+    uint32_t R = 0, n_groups = 0, n_colors = 0;
+    std::vector<uint32_t> ptr, elems;
+};
+RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe, const uint32_t *conn,
+                           const Coloring &col);
+
 // Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);
 

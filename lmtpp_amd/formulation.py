@@ -221,6 +221,16 @@ class Formulation:
         A.check(self._lib.femgpu_get_spmv_config(self._h, f, u), "get_spmv_config")
         return {"fp64": (f[0], u[0]), "fp32": (f[1], u[1])}
 
+    def set_assembly_rows(self, rows):
+        """Element-loop variant: 0 = colour passes, R = row groups of R rows (see femgpu.h)."""
+        A.check(self._lib.femgpu_set_assembly_rows(self._h, int(rows)), "set_assembly_rows")
+
+    def assembly_rows(self):
+        """(current variant, largest admissible R)."""
+        r, m = C.c_uint32(), C.c_uint32()
+        A.check(self._lib.femgpu_get_assembly_rows(self._h, C.byref(r), C.byref(m)), "get_assembly_rows")
+        return r.value, m.value
+
     def set_spmv_config(self, fp64=(A.SPMV_FULL, 2), fp32=(A.SPMV_FULL, 2)):
         f = (C.c_int * 2)(int(fp64[0]), int(fp32[0]))
         u = (C.c_int * 2)(int(fp64[1]), int(fp32[1]))

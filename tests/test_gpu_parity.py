@@ -361,3 +361,32 @@ def test_symmetric_spmv_format_matches_oracle(oracle, kind):
     for (fmt, u, prec), n in its.items():
         assert abs(n - its[(0, 1, prec)]) <= max(3, 0.05 * n)
     assert f.spmv_config() == {"fp64": (1, 3), "fp32": (1, 3)}
+
+
+@pytest.mark.parametrize("kind", [L.TETRA, L.TETRA10, L.HEXA])
+def test_assembly_variants_agree(oracle, kind):
+    """Colour-pass and row-group element loops (every admissible group size) give the oracle's K
+    and F and agree with one another to rounding (same per-entry summation order)."""
+    say("assembly variants", kind)
+    c, e, E = small_case(kind, 40 + kind)
+    bf = (0.1, 0.2, -0.3)
+    f, faces, tr, dofs = build(kind, c, e, E, bf=bf)
+    orp, ocol, ovals, oF, rc, M = oracle_system(oracle, kind, c, e, E, NU0, bf, faces, tr, dofs)
+    _, rmax = f.assembly_rows()
+    assert rmax >= 8
+    out = {}
+    for R in (0, rmax, rmax // 2, 8):
+        f.set_assembly_rows(R)
+        f.assemble()
+        rp, col, vals = f.get_mat()
+        F = f.get_sollicitation()
+        scale = np.abs(ovals).max()
+        assert np.abs(vals - ovals).max() <= 1e-13 * scale, R
+        assert np.abs(F - oF).max() <= 1e-13 * np.abs(oF).max(), R
+        out[R] = (vals, F)
+    for R, (v, F) in out.items():
+        assert np.abs(v - out[0][0]).max() <= 1e-15 * np.abs(out[0][0]).max(), R
+        assert np.abs(F - out[0][1]).max() <= 1e-15 * np.abs(out[0][1]).max(), R
+    assert f.assembly_rows()[0] == 8
+    with pytest.raises(L.FemGpuError):
+        f.set_assembly_rows(3)

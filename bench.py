@@ -145,6 +145,7 @@ def config2():
     sz = f.sizes()
     ndof = 3 * sz["n_nodes"]
     f.assemble()
+    f.autotune()   # SpMV variant and element-loop variant for this mesh
     asm = []
     for _ in range(3):
         f.assemble()
@@ -323,8 +324,9 @@ def main():
     spmv_cfg = f.autotune()
     unroll = spmv_cfg["fp32"][1]
     fmt_name = {0: "full", 1: "sym"}
-    log("autotuned SpMV: FP64 %s/%d, FP32 %s/%d (format / SELL slots per loop trip)" % (
-        fmt_name[spmv_cfg["fp64"][0]], spmv_cfg["fp64"][1], fmt_name[spmv_cfg["fp32"][0]], spmv_cfg["fp32"][1]))
+    log("autotuned SpMV: FP64 %s/%d/nt%d, FP32 %s/%d/nt%d (format / SELL slots per loop trip / non-temporal)" % (
+        fmt_name[spmv_cfg["fp64"][0]], spmv_cfg["fp64"][1], spmv_cfg["fp64"][2], fmt_name[spmv_cfg["fp32"][0]],
+        spmv_cfg["fp32"][1], spmv_cfg["fp32"][2]))
     f.precision, f.precond = L.FP64, L.BLOCK_JACOBI
     f.max_iter = 40000
     f.set_result(np.zeros(ndof))
@@ -447,14 +449,17 @@ def main():
         "rel_res_tol": mtol, "iters_mixed": int(stm.iters), "refreshes_mixed": int(stm.refreshes),
         "iters_fp64": int(st64.iters),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": load_traffic("spmv_fp32" + ("_sym" if fmt32 else "")),
-                     "kernel": "k_spmv<float,0,%d,%d> (FP32 SELL-64 block SpMV, %s storage)" % (
-                         unroll, fmt32, fmt_name[fmt32]), "avg_launch_ms": round(t32, 4),
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic("spmv_fp32" + ("_sym" if fmt32 else ("" if spmv_cfg["fp32"][2] else "_plain"))),
+                     "kernel": "k_spmv<float,0,%d,%d,%s> (FP32 SELL-64 block SpMV, %s storage)" % (
+                         unroll, fmt32, "true" if spmv_cfg["fp32"][2] else "false", fmt_name[fmt32]),
+                     "avg_launch_ms": round(t32, 4),
                      "algorithmic_bytes": b32},
         "roofline_fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                "avg_launch_ms": round(t64, 4), "algorithmic_bytes": b64,
-                               "traffic": load_traffic("spmv_fp64" + ("_sym" if fmt64 else "")),
-                               "kernel": "k_spmv<double,0,%d,%d>" % (spmv_cfg["fp64"][1], fmt64)},
+                               "traffic": load_traffic("spmv_fp64" + ("_sym" if fmt64 else ("" if spmv_cfg["fp64"][2] else "_plain"))),
+                               "kernel": "k_spmv<double,0,%d,%d,%s>" % (spmv_cfg["fp64"][1], fmt64,
+                                                                        "true" if spmv_cfg["fp64"][2] else "false")},
         "preconditioners": precond,
         "after_solve": {"kernel": "k_after_solve<2>", "ms": round(a_ms, 4),
                         "melem_s": round(sz["n_elem"] / (a_ms * 1e-3) / 1e6, 1), "algorithmic_bytes": as_bytes,
@@ -462,8 +467,11 @@ def main():
     }
     from lmtpp_amd import _capi as A
     gbs, _ = A.stream_copy_gbs(1 << 27, 20)
+    rgbs, _ = A.stream_read_gbs(1 << 28, 20)
     out["stream_copy_gbs"] = round(gbs, 1)
+    out["stream_read_gbs"] = round(rgbs, 1)
     out["roofline"]["frac_of_measured_copy"] = round(ach32 / gbs, 4)
+    out["roofline"]["frac_of_measured_read"] = round(ach32 / rgbs, 4)
     f.close()
     if not args.no_configs:
         out["configs"] = {}

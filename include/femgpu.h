@@ -252,6 +252,9 @@ FEMGPU_API int femgpu_time_spmv(femgpu_problem *p, int precision, int reps, doub
 /* Average device time (ms) of a plain copy of n doubles (n even; 16 B per lane) over reps
  * launches: the measured-bandwidth calibration (moves 16 n bytes per launch). */
 FEMGPU_API int femgpu_stream_copy(size_t n, int reps, double *avg_ms);
+/* The same for a streaming read of n doubles (n even; moves 8 n bytes per launch): the ceiling
+ * for read-dominated kernels such as the block SpMV. */
+FEMGPU_API int femgpu_stream_read(size_t n, int reps, double *avg_ms);
 
 /* ---- tuning (no reference counterpart: the reference has no device code) ------------------- */
 
@@ -260,15 +263,17 @@ FEMGPU_API int femgpu_stream_copy(size_t n, int reps, double *avg_ms);
  * owner row, as MatWithTinyBlocks::partial_mul uses each stored block twice
  * (containers/MatWithTinyBlocks.h:303-378). */
 typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1 } femgpu_spmv_format;
-/* Times every block-SpMV variant (format x SELL slots per loop trip 1, 2, 3) on this assembled
+/* Times every block-SpMV variant (format x load kind x SELL slots per loop trip 1, 2, 3) on this assembled
  * matrix, for FP64 and for the FP32 copy, and keeps the fastest of each; *unroll receives the
  * FP64 pick (may be NULL).  Then times the element-loop variants (femgpu_set_assembly_rows) and
  * keeps the fastest, re-assembling K and F with it. */
 FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
 /* Current (or forced) variant per precision: index 0 FP64, 1 FP32; format femgpu_spmv_format,
- * unroll 1..3. */
-FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2]);
-FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2]);
+ * unroll 1..3, nontemporal 0/1 (non-temporal loads of the matrix, full format only; may be
+ * NULL = 0). */
+FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2], int nontemporal[2]);
+FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2],
+                                      const int nontemporal[2]);
 /* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
  * into K (read-modify-write).  rows = R (a power of two, at most *max_rows): each workgroup owns R
  * consecutive block rows, accumulates their blocks from the incident elements colour by colour in

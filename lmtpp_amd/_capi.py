@@ -91,6 +91,7 @@ _SIGS = {
     "femgpu_time_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "femgpu_autotune": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "femgpu_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "femgpu_stream_read": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "femgpu_set_graph_replay": (C.c_int, [_P, C.c_int]),
     "femgpu_batch_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_batch_clear_result": (C.c_int, [_P]),
@@ -98,10 +99,10 @@ _SIGS = {
     "femgpu_after_solve_device": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_after_solve_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_batch_after_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
-    "femgpu_get_spmv_config": (C.c_int, [_P, _P, _P]),
+    "femgpu_get_spmv_config": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_set_assembly_rows": (C.c_int, [_P, C.c_uint32]),
     "femgpu_get_assembly_rows": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
-    "femgpu_set_spmv_config": (C.c_int, [_P, _P, _P]),
+    "femgpu_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_precond_setup_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
@@ -154,6 +155,14 @@ def ic0_schedule(row_ptr, col, precond=BLOCK_IC0_COLOURED):
     check(lib.femgpu_ic0_schedule(n, rp.ctypes.data, cl.ctypes.data, precond, order.ctypes.data, lv.ctypes.data,
                                   C.byref(nl)), "ic0_schedule")
     return order, lv
+
+
+def stream_read_gbs(n_doubles=1 << 27, reps=20):
+    """Measured streaming-read bandwidth (GB/s) of the library's read kernel."""
+    lib = load()
+    ms = C.c_double()
+    check(lib.femgpu_stream_read(n_doubles, reps, C.byref(ms)), "stream_read")
+    return 8.0 * n_doubles / (ms.value * 1e-3) / 1e9, ms.value
 
 
 def declared_symbols():

@@ -318,6 +318,27 @@ __global__ __launch_bounds__(256) void k_copy(double2 *__restrict__ dst, const d
         if (base + 256 * k < n2) dst[base + 256 * k] = v[k];
 }
 
+// Streaming read (sum of the input, one partial per workgroup), four 16-byte loads per lane in
+// flight: the read-bandwidth calibration for read-dominated kernels (reads exactly 8 n bytes).
+__global__ __launch_bounds__(256) void k_read(const double2 *__restrict__ src, size_t n2, double *__restrict__ out) {
+    const size_t base = blockIdx.x * (size_t)1024 + threadIdx.x;
+    double2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = base + 256 * k < n2 ? src[base + 256 * k] : double2{0.0, 0.0};
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s += v[k].x + v[k].y;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) out[blockIdx.x * 4 + (threadIdx.x >> 6)] = s;
+}
+
+hipError_t launch_read(const double *src, size_t n, double *out, hipStream_t s) {
+    const size_t n2 = n / 2;
+    hipLaunchKernelGGL(k_read, dim3((unsigned)((n2 + 1023) / 1024)), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(src), n2, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t s) {
     const size_t n2 = n / 2;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n2 + 1023) / 1024)), dim3(256), 0, s, reinterpret_cast<double2 *>(dst),

@@ -843,23 +843,27 @@ int femgpu_time_spmv(femgpu_problem *h, int precision, int reps, double *avg_ms)
     return FEMGPU_OK;
 }
 
-int femgpu_stream_copy(size_t n, int reps, double *avg_ms) {
-    if (!avg_ms || reps < 1 || n < 2 || (n & 1)) return fail(FEMGPU_BAD_INPUT, "stream_copy: bad input");
+// Calibration kernels on a fresh buffer of n doubles: plain copy (16 n bytes moved per launch) or
+// streaming read (8 n bytes).
+static int stream_bench(size_t n, int reps, double *avg_ms, bool read_only) {
+    if (!avg_ms || reps < 1 || n < 2 || (n & 1)) return fail(FEMGPU_BAD_INPUT, "stream bench: bad input");
     double *a = nullptr, *b = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = FEMGPU_OK;
-    if (dalloc(&a, n) != hipSuccess || dalloc(&b, n) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+    const size_t nb = read_only ? 4 * (n / 2048 + 1) : n;
+    auto run = [&]() { return read_only ? launch_read(a, n, b, s) : launch_copy(b, a, n, s); };
+    if (dalloc(&a, n) != hipSuccess || dalloc(&b, nb) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-        rc = fail(FEMGPU_HIP_ERROR, "stream_copy: setup");
-    if (!rc && (launch_fill(a, n, 1.0, s) != hipSuccess || launch_copy(b, a, n, s) != hipSuccess)) rc = fail(FEMGPU_HIP_ERROR, "stream_copy: warmup");
+        rc = fail(FEMGPU_HIP_ERROR, "stream bench: setup");
+    if (!rc && (launch_fill(a, n, 1.0, s) != hipSuccess || run() != hipSuccess)) rc = fail(FEMGPU_HIP_ERROR, "stream bench: warmup");
     if (!rc) {
         (void)hipEventRecord(e0, s);
-        for (int i = 0; i < reps; ++i) (void)launch_copy(b, a, n, s);
+        for (int i = 0; i < reps; ++i) (void)run();
         (void)hipEventRecord(e1, s);
         float ms = 0;
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-            rc = fail(FEMGPU_HIP_ERROR, "stream_copy: timing");
+            rc = fail(FEMGPU_HIP_ERROR, "stream bench: timing");
         *avg_ms = (double)ms / reps;
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -869,6 +873,9 @@ int femgpu_stream_copy(size_t n, int reps, double *avg_ms) {
     dfree(b);
     return rc;
 }
+
+int femgpu_stream_copy(size_t n, int reps, double *avg_ms) { return stream_bench(n, reps, avg_ms, false); }
+int femgpu_stream_read(size_t n, int reps, double *avg_ms) { return stream_bench(n, reps, avg_ms, true); }
 
 int femgpu_autotune(femgpu_problem *h, int *unroll) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "autotune: null");
@@ -914,24 +921,33 @@ int femgpu_get_assembly_rows(femgpu_problem *h, uint32_t *rows, uint32_t *max_ro
     return FEMGPU_OK;
 }
 
-int femgpu_get_spmv_config(femgpu_problem *h, int format[2], int unroll[2]) {
+int femgpu_get_spmv_config(femgpu_problem *h, int format[2], int unroll[2], int nontemporal[2]) {
     if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "get_spmv_config: null");
-    for (int i = 0; i < 2; ++i) { format[i] = h->P.spmv.format[i]; unroll[i] = h->P.spmv.unroll[i]; }
+    for (int i = 0; i < 2; ++i) {
+        format[i] = h->P.spmv.format[i];
+        unroll[i] = h->P.spmv.unroll[i];
+        if (nontemporal) nontemporal[i] = h->P.spmv.nt[i];
+    }
     return FEMGPU_OK;
 }
 
-int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unroll[2]) {
+int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unroll[2], const int nontemporal[2]) {
     if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "set_spmv_config: null");
     for (int i = 0; i < 2; ++i)
-        if (format[i] < 0 || format[i] > 1 || unroll[i] < 1 || unroll[i] > 3)
-            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1, unroll 1..3");
+        if (format[i] < 0 || format[i] > 1 || unroll[i] < 1 || unroll[i] > 3 ||
+            (nontemporal && nontemporal[i] && format[i] != FMT_FULL))
+            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1, unroll 1..3, non-temporal loads with format 0");
     Problem &P = h->P;
     if ((format[0] == FMT_SYM || format[1] == FMT_SYM) && !P.sym.built) {
         const int rc = sym_ensure(P, format[0] == FMT_SYM ? 0 : 1);
         if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "set_spmv_config: symmetric storage");
         P.sym.valid[0] = P.sym.valid[1] = false;  // values are (re)made from K before the next product
     }
-    for (int i = 0; i < 2; ++i) { P.spmv.format[i] = format[i]; P.spmv.unroll[i] = unroll[i]; }
+    for (int i = 0; i < 2; ++i) {
+        P.spmv.format[i] = format[i];
+        P.spmv.unroll[i] = unroll[i];
+        P.spmv.nt[i] = nontemporal && nontemporal[i];
+    }
     destroy_graph(P);
     return FEMGPU_OK;
 }

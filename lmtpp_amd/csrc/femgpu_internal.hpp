@@ -44,6 +44,7 @@ enum { FMT_FULL = 0, FMT_SYM = 1 };
 struct SpmvConfig {
     int unroll[2] = {2, 2};
     int format[2] = {FMT_FULL, FMT_FULL};
+    bool nt[2] = {true, true};     // non-temporal loads of the matrix (full format)
     bool tuned = false;
 };
 
@@ -143,6 +144,8 @@ struct Problem {
 // ---- launchers (assemble.hip) ----
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t s);  // n even
+// out: 4 doubles per 1024 input pairs (one per wave)
+hipError_t launch_read(const double *src, size_t n, double *out, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
 // Largest row-group size (power of two <= 64) whose accumulator fits the LDS budget; 0 if none.

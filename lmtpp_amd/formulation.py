@@ -215,11 +215,12 @@ class Formulation:
         return self.spmv_config()
 
     def spmv_config(self):
-        """{"fp64": (format, unroll), "fp32": (format, unroll)}, format SPMV_FULL or SPMV_SYM."""
+        """{"fp64": (format, unroll, nontemporal), "fp32": (...)}, format SPMV_FULL or SPMV_SYM."""
         f = (C.c_int * 2)()
         u = (C.c_int * 2)()
-        A.check(self._lib.femgpu_get_spmv_config(self._h, f, u), "get_spmv_config")
-        return {"fp64": (f[0], u[0]), "fp32": (f[1], u[1])}
+        nt = (C.c_int * 2)()
+        A.check(self._lib.femgpu_get_spmv_config(self._h, f, u, nt), "get_spmv_config")
+        return {"fp64": (f[0], u[0], nt[0]), "fp32": (f[1], u[1], nt[1])}
 
     def set_assembly_rows(self, rows):
         """Element-loop variant: 0 = colour passes, R = row groups of R rows (see femgpu.h)."""
@@ -231,10 +232,13 @@ class Formulation:
         A.check(self._lib.femgpu_get_assembly_rows(self._h, C.byref(r), C.byref(m)), "get_assembly_rows")
         return r.value, m.value
 
-    def set_spmv_config(self, fp64=(A.SPMV_FULL, 2), fp32=(A.SPMV_FULL, 2)):
+    def set_spmv_config(self, fp64=(A.SPMV_FULL, 2, 0), fp32=(A.SPMV_FULL, 2, 0)):
+        """Force the SpMV variant per precision: (format, unroll, nontemporal)."""
+        fp64, fp32 = tuple(fp64) + (0,) * (3 - len(fp64)), tuple(fp32) + (0,) * (3 - len(fp32))
         f = (C.c_int * 2)(int(fp64[0]), int(fp32[0]))
         u = (C.c_int * 2)(int(fp64[1]), int(fp32[1]))
-        A.check(self._lib.femgpu_set_spmv_config(self._h, f, u), "set_spmv_config")
+        nt = (C.c_int * 2)(int(fp64[2]), int(fp32[2]))
+        A.check(self._lib.femgpu_set_spmv_config(self._h, f, u, nt), "set_spmv_config")
 
     def set_graph_replay(self, enable=True):
         A.check(self._lib.femgpu_set_graph_replay(self._h, int(bool(enable))), "set_graph_replay")

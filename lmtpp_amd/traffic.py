@@ -5,8 +5,9 @@ Workload (run it under rocprofv3 --pmc, one counter per run):
     python -m lmtpp_amd.traffic run [--n 100]
 
 launches the plain copy kernel (k_copy, 2^26 doubles: exactly 512 MiB read and 512 MiB written),
-then 5 FP64 and 5 FP32 block SpMV launches on Hexa8 n^3 for each storage format (full, and
-symmetric: upper blocks stored, lower ones read transposed).  Then
+then 5 FP64 and 5 FP32 block SpMV launches on Hexa8 n^3 for each variant: full storage with
+non-temporal matrix loads (the default), full storage with plain loads, and symmetric storage
+(upper blocks stored, lower ones read transposed).  Then
 
     python -m lmtpp_amd.traffic parse <fetch_dir> <write_dir> > profiles/spmv_traffic.json
 
@@ -41,11 +42,11 @@ def run(n):
     f.assemble()
     f.set_result(np.zeros(3 * len(c)))
     sz = f.sizes()
-    for fmt in (A.SPMV_FULL, A.SPMV_SYM):
-        f.set_spmv_config(fp64=(fmt, 2), fp32=(fmt, 2))
+    for fmt, nt in ((A.SPMV_FULL, 1), (A.SPMV_FULL, 0), (A.SPMV_SYM, 0)):
+        f.set_spmv_config(fp64=(fmt, 2, nt), fp32=(fmt, 2, nt))
         t64 = f.time_spmv(L.FP64, 5)
         t32 = f.time_spmv(L.MIXED, 5)
-        print(json.dumps({"format": fmt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
+        print(json.dumps({"format": fmt, "nontemporal": nt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
                           "spmv_fp32_ms": t32}), flush=True)
 
 
@@ -68,14 +69,15 @@ def _counters(d):
         with open(p) as f:
             rows += list(csv.DictReader(f))
     per = {}
-    pat = re.compile(r"k_spmv<(double|float), 0, (\d), (\d)>")
+    pat = re.compile(r"k_spmv<(double|float), 0, (\d), (\d), (true|false)>")
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
         if "k_copy" in name:
             key = "copy"
         elif m:
-            key = "spmv_%s%s" % ("fp64" if m.group(1) == "double" else "fp32", "" if m.group(3) == "0" else "_sym")
+            kind = "_sym" if m.group(3) == "1" else ("" if m.group(4) == "true" else "_plain")
+            key = "spmv_%s%s" % ("fp64" if m.group(1) == "double" else "fp32", kind)
         else:
             continue
         per.setdefault(key, []).append(float(r["Counter_Value"]))
@@ -89,8 +91,8 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
     w_scale = known_kib / (sum(wr["copy"]) / len(wr["copy"]))
     out = {"copy_scale": {"FETCH_SIZE": f_scale, "WRITE_SIZE": w_scale},
            "note": "bytes per launch = mean counter (KiB) x 1024 x scale; scale = known copy KiB / counted KiB"}
-    for key, fp32, fmt in (("spmv_fp64", False, 0), ("spmv_fp32", True, 0), ("spmv_fp64_sym", False, 1),
-                           ("spmv_fp32_sym", True, 1)):
+    for key, fp32, fmt in (("spmv_fp64", False, 0), ("spmv_fp32", True, 0), ("spmv_fp64_plain", False, 0),
+                           ("spmv_fp32_plain", True, 0), ("spmv_fp64_sym", False, 1), ("spmv_fp32_sym", True, 1)):
         alg = spmv_bytes(nnzb, n_nodes, fp32, fmt)
         if key not in fe or key not in wr:
             continue

@@ -328,7 +328,7 @@ def test_autotune_keeps_results():
     assert f.solve(rel_tol=1e-11)
     u0 = f.get_result()
     cfg = f.autotune()
-    assert all(fm in (0, 1) and u in (1, 2, 3) for fm, u in cfg.values())
+    assert all(fm in (0, 1) and u in (1, 2, 3) and nt in (0, 1) for fm, u, nt in cfg.values())
     f.set_result(np.zeros_like(u0))
     assert f.solve(rel_tol=1e-11)
     assert np.linalg.norm(f.get_result() - u0) <= 1e-9 * np.linalg.norm(u0)
@@ -347,20 +347,20 @@ def test_symmetric_spmv_format_matches_oracle(oracle, kind):
     x_ref, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-13)
     assert rc == 0
     its = {}
-    for fmt in (L._capi.SPMV_FULL, L._capi.SPMV_SYM):
+    for fmt, nt in ((L._capi.SPMV_FULL, 0), (L._capi.SPMV_FULL, 1), (L._capi.SPMV_SYM, 0)):
         for u in (1, 3):
-            f.set_spmv_config(fp64=(fmt, u), fp32=(fmt, u))
+            f.set_spmv_config(fp64=(fmt, u, nt), fp32=(fmt, u, nt))
             for prec, tol in ((L.FP64, 1e-10), (L.MIXED, 1e-8)):
                 f.precision = prec
                 f.set_result(np.zeros_like(F))
                 assert f.solve_system(rel_tol=1e-11), (fmt, u, prec, f.last_stats.status)
                 assert f.last_stats.rel_res <= 1e-11
                 assert np.linalg.norm(f.get_result() - x_ref) <= tol * np.linalg.norm(x_ref)
-                its[(fmt, u, prec)] = f.last_stats.iters
+                its[(fmt, nt, u, prec)] = f.last_stats.iters
     # same Krylov process up to rounding: iteration counts within a few percent
-    for (fmt, u, prec), n in its.items():
-        assert abs(n - its[(0, 1, prec)]) <= max(3, 0.05 * n)
-    assert f.spmv_config() == {"fp64": (1, 3), "fp32": (1, 3)}
+    for (fmt, nt, u, prec), n in its.items():
+        assert abs(n - its[(0, 0, 1, prec)]) <= max(3, 0.05 * n)
+    assert f.spmv_config() == {"fp64": (1, 3, 0), "fp32": (1, 3, 0)}
 
 
 @pytest.mark.parametrize("kind", [L.TETRA, L.TETRA10, L.HEXA])

@@ -274,6 +274,11 @@ FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
 FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2], int nontemporal[2]);
 FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2],
                                       const int nontemporal[2]);
+/* The same for a batch.  Non-temporal loads suit matrices streamed once per product; when most
+ * sub-problems have converged, the few still iterating may fit the 256 MB Infinity Cache, where
+ * plain loads keep them resident across iterations. */
+FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2],
+                                            const int nontemporal[2]);
 /* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
  * into K (read-modify-write).  rows = R (a power of two, at most *max_rows): each workgroup owns R
  * consecutive block rows, accumulates their blocks from the incident elements colour by colour in

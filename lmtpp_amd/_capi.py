@@ -103,6 +103,7 @@ _SIGS = {
     "femgpu_set_assembly_rows": (C.c_int, [_P, C.c_uint32]),
     "femgpu_get_assembly_rows": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "femgpu_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),
+    "femgpu_batch_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_precond_setup_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),

@@ -88,6 +88,13 @@ class Batch:
             A.check(rc, "batch_solve")
         return rc == A.OK, ms.value
 
+    def set_spmv_config(self, fp64=(A.SPMV_FULL, 2, 1), fp32=(A.SPMV_FULL, 2, 1)):
+        """Block-SpMV variant per precision: (format, unroll, nontemporal), as Formulation."""
+        f = (C.c_int * 2)(int(fp64[0]), int(fp32[0]))
+        u = (C.c_int * 2)(int(fp64[1]), int(fp32[1]))
+        nt = (C.c_int * 2)(int(fp64[2]), int(fp32[2]))
+        A.check(self._lib.femgpu_batch_set_spmv_config(self._h, f, u, nt), "batch_set_spmv_config")
+
     def clear_result(self):
         """Zero initial guess for every problem (solves otherwise start from the previous result)."""
         A.check(self._lib.femgpu_batch_clear_result(self._h), "batch_clear_result")

@@ -909,6 +909,11 @@ int femgpu_autotune(femgpu_problem *h, int *unroll) {
     return rc;
 }
 
+int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2], const int nontemporal[2]) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_set_spmv_config: null");
+    return femgpu_set_spmv_config(&b->H, format, unroll, nontemporal);
+}
+
 int femgpu_set_assembly_rows(femgpu_problem *h, uint32_t rows) {
     if (!h || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "set_assembly_rows: bad input");
     return set_assembly_rows(h->P, rows);

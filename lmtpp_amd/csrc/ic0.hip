@@ -114,6 +114,10 @@ __global__ __launch_bounds__(256) void k_ic0_factor(uint32_t t0, uint32_t t1, Ic
     for (int m = 0; m < 9; ++m) L[m * ps + pd] = D[m];
 }
 
+// The sweeps stream the factor once each: its values and columns are read non-temporally.
+template <typename V>
+__device__ inline V nt(const V *p) { return __builtin_nontemporal_load(p); }
+
 // Forward sweep of one level: z_t = L_tt^-1 (r_t - sum_{c<t} L_tc z_c).
 template <typename T>
 __global__ __launch_bounds__(256) void k_ic0_fwd(uint32_t t0, uint32_t t1, Ic0Args a, const T *__restrict__ L,
@@ -125,11 +129,11 @@ __global__ __launch_bounds__(256) void k_ic0_fwd(uint32_t t0, uint32_t t1, Ic0Ar
     T s0 = r[3 * (size_t)n], s1 = r[3 * (size_t)n + 1], s2 = r[3 * (size_t)n + 2];
     for (uint32_t k = 0; k < nl; ++k) {
         const uint64_t p = base + 64ull * k;
-        const uint32_t c = a.colN[p];
+        const uint32_t c = nt(a.colN + p);
         const T x0 = z[3 * (size_t)c], x1 = z[3 * (size_t)c + 1], x2 = z[3 * (size_t)c + 2];
-        s0 -= L[p] * x0 + L[ps + p] * x1 + L[2 * ps + p] * x2;
-        s1 -= L[3 * ps + p] * x0 + L[4 * ps + p] * x1 + L[5 * ps + p] * x2;
-        s2 -= L[6 * ps + p] * x0 + L[7 * ps + p] * x1 + L[8 * ps + p] * x2;
+        s0 -= nt(L + p) * x0 + nt(L + ps + p) * x1 + nt(L + 2 * ps + p) * x2;
+        s1 -= nt(L + 3 * ps + p) * x0 + nt(L + 4 * ps + p) * x1 + nt(L + 5 * ps + p) * x2;
+        s2 -= nt(L + 6 * ps + p) * x0 + nt(L + 7 * ps + p) * x1 + nt(L + 8 * ps + p) * x2;
     }
     const uint64_t pd = base + 64ull * nl;
     const T z0 = s0 / L[pd];
@@ -152,11 +156,11 @@ __global__ __launch_bounds__(256) void k_ic0_bwd(uint32_t t0, uint32_t t1, Ic0Ar
     T s0 = z[3 * (size_t)n], s1 = z[3 * (size_t)n + 1], s2 = z[3 * (size_t)n + 2];
     for (uint32_t k = nl + 1; k < len; ++k) {
         const uint64_t p = base + 64ull * k;
-        const uint32_t c = a.colN[p];
+        const uint32_t c = nt(a.colN + p);
         const T y0 = z[3 * (size_t)c], y1 = z[3 * (size_t)c + 1], y2 = z[3 * (size_t)c + 2];
-        s0 -= L[p] * y0 + L[3 * ps + p] * y1 + L[6 * ps + p] * y2;
-        s1 -= L[ps + p] * y0 + L[4 * ps + p] * y1 + L[7 * ps + p] * y2;
-        s2 -= L[2 * ps + p] * y0 + L[5 * ps + p] * y1 + L[8 * ps + p] * y2;
+        s0 -= nt(L + p) * y0 + nt(L + 3 * ps + p) * y1 + nt(L + 6 * ps + p) * y2;
+        s1 -= nt(L + ps + p) * y0 + nt(L + 4 * ps + p) * y1 + nt(L + 7 * ps + p) * y2;
+        s2 -= nt(L + 2 * ps + p) * y0 + nt(L + 5 * ps + p) * y1 + nt(L + 8 * ps + p) * y2;
     }
     const uint64_t pd = base + 64ull * nl;
     const T z2 = s2 / L[8 * ps + pd];

@@ -31,9 +31,9 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(nnzb, n_nodes, fp32, fmt=0):
+def spmv_bytes(nnzb, n_nodes, fp32, fmt=0, col16=0):
     from lmtpp_amd.traffic import spmv_bytes as sb
-    return sb(nnzb, n_nodes, fp32, fmt)
+    return sb(nnzb, n_nodes, fp32, fmt, col16)
 
 
 def load_traffic(name):
@@ -268,10 +268,10 @@ def config4(n_models=256):
     for _ in range(2):
         b.assemble()
         t_asm.append(b.assemble_ms())
-    # matrix loads: non-temporal (streamed once per product) or plain (the few problems still
-    # iterating late fit the Infinity Cache); both timed, the faster kept
+    # matrix loads: non-temporal (streamed once per product; with 16-bit column offsets) or plain
+    # (the few problems still iterating late fit the Infinity Cache); all timed, the fastest kept
     by_nt = {}
-    for nt in (1, 0):
+    for nt in (2, 1, 0):
         b.set_spmv_config(fp64=(0, 2, nt), fp32=(0, 2, nt))
         times = []
         for _ in range(2):
@@ -300,8 +300,9 @@ def config4(n_models=256):
             "assembly_ms": round(float(np.median(t_asm)), 3), "solve_ms": round(ms, 2),
             "aggregate_mdof_s": round(ndof / (ms * 1e-3) / 1e6, 3), "max_rel_err_vs_fp64": float(max(errs)),
             "median_rel_err": float(np.median(errs)), "iters_min": int(min(iters)), "iters_max": int(max(iters)),
-            "all_ok": bool(ok), "nontemporal_loads": int(best_nt),
-            "solve_ms_by_load": {"nontemporal": round(by_nt[1][0], 2), "plain": round(by_nt[0][0], 2)}}
+            "all_ok": bool(ok), "spmv_loads": int(best_nt),
+            "solve_ms_by_loads": {"nontemporal_col16": round(by_nt[2][0], 2), "nontemporal": round(by_nt[1][0], 2),
+                                  "plain": round(by_nt[0][0], 2)}}
 
 
 def main():
@@ -340,7 +341,7 @@ def main():
     spmv_cfg = f.autotune()
     unroll = spmv_cfg["fp32"][1]
     fmt_name = {0: "full", 1: "sym"}
-    log("autotuned SpMV: FP64 %s/%d/nt%d, FP32 %s/%d/nt%d (format / SELL slots per loop trip / non-temporal)" % (
+    log("autotuned SpMV: FP64 %s/%d/%d, FP32 %s/%d/%d (format / SELL slots per loop trip / loads)" % (
         fmt_name[spmv_cfg["fp64"][0]], spmv_cfg["fp64"][1], spmv_cfg["fp64"][2], fmt_name[spmv_cfg["fp32"][0]],
         spmv_cfg["fp32"][1], spmv_cfg["fp32"][2]))
     f.precision, f.precond = L.FP64, L.BLOCK_JACOBI
@@ -439,8 +440,9 @@ def main():
     t32 = f.time_spmv(L.MIXED, reps)
     t64 = f.time_spmv(L.FP64, reps)
     fmt32, fmt64 = spmv_cfg["fp32"][0], spmv_cfg["fp64"][0]
-    b32 = spmv_bytes(sz["nnzb"], sz["n_nodes"], True, fmt32)
-    b64 = spmv_bytes(sz["nnzb"], sz["n_nodes"], False, fmt64)
+    b32 = spmv_bytes(sz["nnzb"], sz["n_nodes"], True, fmt32, spmv_cfg["fp32"][2] == 2)
+    b64 = spmv_bytes(sz["nnzb"], sz["n_nodes"], False, fmt64, spmv_cfg["fp64"][2] == 2)
+    load_key = {0: "_plain", 1: "", 2: "_col16"}
     ach32 = b32 / (t32 * 1e-3) / 1e9
     ach64 = b64 / (t64 * 1e-3) / 1e9
 
@@ -466,16 +468,15 @@ def main():
         "iters_fp64": int(st64.iters),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach32 / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic("spmv_fp32" + ("_sym" if fmt32 else ("" if spmv_cfg["fp32"][2] else "_plain"))),
-                     "kernel": "k_spmv<float,0,%d,%d,%s> (FP32 SELL-64 block SpMV, %s storage)" % (
-                         unroll, fmt32, "true" if spmv_cfg["fp32"][2] else "false", fmt_name[fmt32]),
+                     "traffic": load_traffic("spmv_fp32" + ("_sym" if fmt32 else load_key[spmv_cfg["fp32"][2]])),
+                     "kernel": "k_spmv<float,0,%d,%d,%d> (FP32 SELL-64 block SpMV, %s storage, loads %d)" % (
+                         unroll, fmt32, spmv_cfg["fp32"][2], fmt_name[fmt32], spmv_cfg["fp32"][2]),
                      "avg_launch_ms": round(t32, 4),
                      "algorithmic_bytes": b32},
         "roofline_fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                "avg_launch_ms": round(t64, 4), "algorithmic_bytes": b64,
-                               "traffic": load_traffic("spmv_fp64" + ("_sym" if fmt64 else ("" if spmv_cfg["fp64"][2] else "_plain"))),
-                               "kernel": "k_spmv<double,0,%d,%d,%s>" % (spmv_cfg["fp64"][1], fmt64,
-                                                                        "true" if spmv_cfg["fp64"][2] else "false")},
+                               "traffic": load_traffic("spmv_fp64" + ("_sym" if fmt64 else load_key[spmv_cfg["fp64"][2]])),
+                               "kernel": "k_spmv<double,0,%d,%d,%d>" % (spmv_cfg["fp64"][1], fmt64, spmv_cfg["fp64"][2])},
         "preconditioners": precond,
         "after_solve": {"kernel": "k_after_solve<2>", "ms": round(a_ms, 4),
                         "melem_s": round(sz["n_elem"] / (a_ms * 1e-3) / 1e6, 1), "algorithmic_bytes": as_bytes,

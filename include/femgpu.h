@@ -268,17 +268,20 @@ typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1 } femgpu_spmv_format;
  * FP64 pick (may be NULL).  Then times the element-loop variants (femgpu_set_assembly_rows) and
  * keeps the fastest, re-assembling K and F with it. */
 FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
+/* How the full format's matrix is read: plain loads; non-temporal loads (streamed once, L2 left to
+ * the x gathers); non-temporal loads with 16-bit column offsets c - r (available when every
+ * offset fits, e.g. lattice numberings). */
+typedef enum { FEMGPU_LOADS_PLAIN = 0, FEMGPU_LOADS_NT = 1, FEMGPU_LOADS_NT_COL16 = 2 } femgpu_spmv_loads;
 /* Current (or forced) variant per precision: index 0 FP64, 1 FP32; format femgpu_spmv_format,
- * unroll 1..3, nontemporal 0/1 (non-temporal loads of the matrix, full format only; may be
- * NULL = 0). */
-FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2], int nontemporal[2]);
+ * unroll 1..3, loads femgpu_spmv_loads (full format only; NULL = plain). */
+FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2], int loads[2]);
 FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2],
-                                      const int nontemporal[2]);
+                                      const int loads[2]);
 /* The same for a batch.  Non-temporal loads suit matrices streamed once per product; when most
  * sub-problems have converged, the few still iterating may fit the 256 MB Infinity Cache, where
  * plain loads keep them resident across iterations. */
 FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2],
-                                            const int nontemporal[2]);
+                                            const int loads[2]);
 /* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
  * into K (read-modify-write).  rows = R (a power of two, at most *max_rows): each workgroup owns R
  * consecutive block rows, accumulates their blocks from the incident elements colour by colour in

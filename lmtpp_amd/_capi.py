@@ -17,6 +17,7 @@ FP64, MIXED = 0, 1
 JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED = 0, 1, 2, 3
 REL_L2, ABS_MAXNORM = 0, 1
 SPMV_FULL, SPMV_SYM = 0, 1
+LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
 NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}

@@ -89,7 +89,7 @@ class Batch:
         return rc == A.OK, ms.value
 
     def set_spmv_config(self, fp64=(A.SPMV_FULL, 2, 1), fp32=(A.SPMV_FULL, 2, 1)):
-        """Block-SpMV variant per precision: (format, unroll, nontemporal), as Formulation."""
+        """Block-SpMV variant per precision: (format, unroll, loads), as Formulation."""
         f = (C.c_int * 2)(int(fp64[0]), int(fp32[0]))
         u = (C.c_int * 2)(int(fp64[1]), int(fp32[1]))
         nt = (C.c_int * 2)(int(fp64[2]), int(fp32[2]))

@@ -61,7 +61,7 @@ void free_problem(Problem &P) {
     destroy_graph(P);
     ic0_free(P.ic0);
     sym_free(P.sym);
-    dfree(P.d_coords); dfree(P.d_E); dfree(P.d_conn); dfree(P.d_rp); dfree(P.d_col); dfree(P.d_diag);
+    dfree(P.d_coords); dfree(P.d_E); dfree(P.d_conn); dfree(P.d_rp); dfree(P.d_col); dfree(P.d_diag); dfree(P.d_col16);
     dfree(P.d_slot); dfree(P.d_order); dfree(P.d_sptr); dfree(P.d_elem_sub); dfree(P.d_mat);
     dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
     dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
@@ -130,6 +130,19 @@ int alloc_system(Problem &P) {
     {
         std::vector<uint32_t> cs = sell_columns(P.pat);
         HIPCHK(upload(&P.d_col, cs.data(), cs.size()));
+        // 16-bit column offsets c - r (row of position p from the slice walk)
+        std::vector<int16_t> c16(cs.size(), 0);
+        bool fits = true;
+        for (uint32_t r = 0; r < P.pat.n_rows && fits; ++r) {
+            const uint32_t s = r >> 6, w = (P.pat.sell_ptr[s + 1] - P.pat.sell_ptr[s]) / 64;
+            for (uint32_t k = 0; k < w; ++k) {
+                const int64_t d = (int64_t)cs[P.pat.pos(r, k)] - (int64_t)r;
+                if (d < -32768 || d > 32767) { fits = false; break; }
+                c16[P.pat.pos(r, k)] = (int16_t)d;
+            }
+        }
+        if (fits) HIPCHK(upload(&P.d_col16, c16.data(), c16.size()));
+        else P.spmv.loads[0] = P.spmv.loads[1] = LOAD_NT;
     }
     HIPCHK(upload(&P.d_sptr, P.pat.sell_ptr.data(), P.pat.sell_ptr.size()));
     HIPCHK(upload(&P.d_diag, P.pat.diag.data(), P.pat.diag.size()));
@@ -909,9 +922,9 @@ int femgpu_autotune(femgpu_problem *h, int *unroll) {
     return rc;
 }
 
-int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2], const int nontemporal[2]) {
+int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2], const int loads[2]) {
     if (!b) return fail(FEMGPU_BAD_INPUT, "batch_set_spmv_config: null");
-    return femgpu_set_spmv_config(&b->H, format, unroll, nontemporal);
+    return femgpu_set_spmv_config(&b->H, format, unroll, loads);
 }
 
 int femgpu_set_assembly_rows(femgpu_problem *h, uint32_t rows) {
@@ -926,22 +939,25 @@ int femgpu_get_assembly_rows(femgpu_problem *h, uint32_t *rows, uint32_t *max_ro
     return FEMGPU_OK;
 }
 
-int femgpu_get_spmv_config(femgpu_problem *h, int format[2], int unroll[2], int nontemporal[2]) {
+int femgpu_get_spmv_config(femgpu_problem *h, int format[2], int unroll[2], int loads[2]) {
     if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "get_spmv_config: null");
     for (int i = 0; i < 2; ++i) {
         format[i] = h->P.spmv.format[i];
         unroll[i] = h->P.spmv.unroll[i];
-        if (nontemporal) nontemporal[i] = h->P.spmv.nt[i];
+        if (loads) loads[i] = h->P.spmv.loads[i];
     }
     return FEMGPU_OK;
 }
 
-int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unroll[2], const int nontemporal[2]) {
+int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unroll[2], const int loads[2]) {
     if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "set_spmv_config: null");
-    for (int i = 0; i < 2; ++i)
-        if (format[i] < 0 || format[i] > 1 || unroll[i] < 1 || unroll[i] > 3 ||
-            (nontemporal && nontemporal[i] && format[i] != FMT_FULL))
-            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1, unroll 1..3, non-temporal loads with format 0");
+    for (int i = 0; i < 2; ++i) {
+        const int ld = loads ? loads[i] : LOAD_PLAIN;
+        if (format[i] < 0 || format[i] > 1 || unroll[i] < 1 || unroll[i] > 3 || ld < 0 || ld > 2 ||
+            (ld != LOAD_PLAIN && format[i] != FMT_FULL) || (ld == LOAD_NT16 && !h->P.d_col16))
+            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1, unroll 1..3, loads 0..2 (1, 2 with format 0; "
+                                          "2 needs every column offset in 16 bits)");
+    }
     Problem &P = h->P;
     if ((format[0] == FMT_SYM || format[1] == FMT_SYM) && !P.sym.built) {
         const int rc = sym_ensure(P, format[0] == FMT_SYM ? 0 : 1);
@@ -951,7 +967,7 @@ int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unr
     for (int i = 0; i < 2; ++i) {
         P.spmv.format[i] = format[i];
         P.spmv.unroll[i] = unroll[i];
-        P.spmv.nt[i] = nontemporal && nontemporal[i];
+        P.spmv.loads[i] = loads ? loads[i] : LOAD_PLAIN;
     }
     destroy_graph(P);
     return FEMGPU_OK;

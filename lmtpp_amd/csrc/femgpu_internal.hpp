@@ -38,13 +38,15 @@ struct SubMaterial { double nu, bf0, bf1, bf2; };
 constexpr uint32_t ROWS_PER_WG = 256;
 constexpr uint32_t NO_SUB = 0xffffffffu;
 
-// Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format and SELL slots
-// per loop trip (1, 2 or 3).
+// Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format, SELL slots per loop
+// trip (1, 2 or 3) and how the full format's matrix is loaded: plain loads, non-temporal loads,
+// or non-temporal loads with 16-bit column offsets (c - r; only when every offset fits).
 enum { FMT_FULL = 0, FMT_SYM = 1 };
+enum { LOAD_PLAIN = 0, LOAD_NT = 1, LOAD_NT16 = 2 };
 struct SpmvConfig {
     int unroll[2] = {2, 2};
     int format[2] = {FMT_FULL, FMT_FULL};
-    bool nt[2] = {true, true};     // non-temporal loads of the matrix (full format)
+    int loads[2] = {LOAD_NT, LOAD_NT};
     bool tuned = false;
 };
 
@@ -93,6 +95,7 @@ struct Problem {
 
     // device: mesh, pattern, scatter map
     double *d_coords = nullptr, *d_E = nullptr;
+    int16_t *d_col16 = nullptr;  // SELL column offsets c - r, when all fit in 16 bits (else null)
     uint32_t *d_conn = nullptr, *d_rp = nullptr, *d_col = nullptr, *d_diag = nullptr, *d_slot = nullptr,
              *d_order = nullptr, *d_sptr = nullptr, *d_elem_sub = nullptr;
     SubMaterial *d_mat = nullptr;
@@ -131,6 +134,7 @@ struct Problem {
     bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;
+    const double *graph_x = nullptr;   // solution vector the captured chunk updates
     Ic0 ic0;
     SymDev sym;
     // element loop variant: 0 = colour passes over all elements (read-modify-write of K),

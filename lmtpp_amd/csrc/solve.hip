@@ -39,25 +39,27 @@ struct SpmvArgs {
     const uint32_t *sptr, *col;
     uint64_t ps;
     const uint32_t *sptrL, *lenL, *colL, *tposL;  // FMT_SYM lower part
+    const int16_t *col16;   // LOAD_NT16: column offsets c - r
 };
 
 // NT: the matrix values and columns are read with non-temporal loads (streamed once; L2 is left
-// to the x gathers).
+// to the x gathers).  C16: columns come as 16-bit offsets c - r.
 template <bool NT, typename V>
 __device__ inline V ld(const V *p) {
     if (NT) return __builtin_nontemporal_load(p);
     return *p;
 }
 
-template <typename T, int UNROLL, bool NT>
-__device__ inline void spmv_rows(const uint32_t *__restrict__ colS, const T *__restrict__ Kv, uint64_t ps, uint64_t base,
-                                 uint32_t len, const T *__restrict__ x, T &s0, T &s1, T &s2) {
+template <typename T, int UNROLL, bool NT, bool C16>
+__device__ inline void spmv_rows(const uint32_t *__restrict__ colS, const int16_t *__restrict__ col16, uint32_t r,
+                                 const T *__restrict__ Kv, uint64_t ps, uint64_t base, uint32_t len,
+                                 const T *__restrict__ x, T &s0, T &s1, T &s2) {
     uint32_t k = 0;
     for (; k + UNROLL <= len; k += UNROLL) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const uint64_t p = base + 64ull * (k + u);
-            const uint32_t c = ld<NT>(colS + p);
+            const uint32_t c = C16 ? r + (int)ld<NT>(col16 + p) : ld<NT>(colS + p);
             const T a0 = ld<NT>(Kv + p), a1 = ld<NT>(Kv + ps + p), a2 = ld<NT>(Kv + 2 * ps + p);
             const T a3 = ld<NT>(Kv + 3 * ps + p), a4 = ld<NT>(Kv + 4 * ps + p), a5 = ld<NT>(Kv + 5 * ps + p);
             const T a6 = ld<NT>(Kv + 6 * ps + p), a7 = ld<NT>(Kv + 7 * ps + p), a8 = ld<NT>(Kv + 8 * ps + p);
@@ -69,7 +71,7 @@ __device__ inline void spmv_rows(const uint32_t *__restrict__ colS, const T *__r
     }
     for (; k < len; ++k) {
         const uint64_t p = base + 64ull * k;
-        const uint32_t c = ld<NT>(colS + p);
+        const uint32_t c = C16 ? r + (int)ld<NT>(col16 + p) : ld<NT>(colS + p);
         const T x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
         s0 += ld<NT>(Kv + p) * x0 + ld<NT>(Kv + ps + p) * x1 + ld<NT>(Kv + 2 * ps + p) * x2;
         s1 += ld<NT>(Kv + 3 * ps + p) * x0 + ld<NT>(Kv + 4 * ps + p) * x1 + ld<NT>(Kv + 5 * ps + p) * x2;
@@ -109,7 +111,7 @@ __device__ inline void spmv_rows_t(const uint32_t *__restrict__ colL, const uint
     }
 }
 
-template <typename T, int MODE, int UNROLL, int FMT, bool NT>
+template <typename T, int MODE, int UNROLL, int FMT, int LD>
 __global__ __launch_bounds__(256) void k_spmv(SpmvArgs A, const T *__restrict__ Kv, const T *__restrict__ x,
                                               const double *b, T *__restrict__ y, WgTables wt, const ProbScalars *S,
                                               double *partials, int sel) {
@@ -123,9 +125,10 @@ __global__ __launch_bounds__(256) void k_spmv(SpmvArgs A, const T *__restrict__ 
         T s0 = 0, s1 = 0, s2 = 0;
         const uint64_t off = r & 63u;
         if (FMT == FMT_FULL) {
-            spmv_rows<T, UNROLL, NT>(A.col, Kv, A.ps, A.sptr[r >> 6] + off, A.len[r + 1] - A.len[r], x, s0, s1, s2);
+            spmv_rows<T, UNROLL, LD != LOAD_PLAIN, LD == LOAD_NT16>(A.col, A.col16, r, Kv, A.ps, A.sptr[r >> 6] + off,
+                                                                    A.len[r + 1] - A.len[r], x, s0, s1, s2);
         } else {
-            spmv_rows<T, UNROLL, NT>(A.col, Kv, A.ps, A.sptr[r >> 6] + off, A.len[r], x, s0, s1, s2);
+            spmv_rows<T, UNROLL, false, false>(A.col, nullptr, r, Kv, A.ps, A.sptr[r >> 6] + off, A.len[r], x, s0, s1, s2);
             spmv_rows_t<T, UNROLL>(A.colL, A.tposL, Kv, A.ps, A.sptrL[r >> 6] + off, A.lenL[r], x, s0, s1, s2);
         }
         const T sv[3] = {s0, s1, s2};
@@ -431,10 +434,10 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     const int pr = sizeof(T) == 4 ? 1 : 0;
     if (P.spmv.format[pr] == FMT_SYM) {
         const SymDev &Y = P.sym;
-        const SpmvArgs A{Y.d_lenU, Y.d_sptrU, Y.d_colU, Y.psU, Y.d_sptrL, Y.d_lenL, Y.d_colL, Y.d_tposL};
+        const SpmvArgs A{Y.d_lenU, Y.d_sptrU, Y.d_colU, Y.psU, Y.d_sptrL, Y.d_lenL, Y.d_colL, Y.d_tposL, nullptr};
         const T *Ks = pr ? reinterpret_cast<const T *>(Y.d_K32) : reinterpret_cast<const T *>(Y.d_K);
 #define FEMGPU_SPMV_CASE(U) \
-    hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_SYM, false>), g, blk, 0, P.stream, A, Ks, x, b, y, wt, P.d_ps, P.d_partials, sel)
+    hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_SYM, LOAD_PLAIN>), g, blk, 0, P.stream, A, Ks, x, b, y, wt, P.d_ps, P.d_partials, sel)
         switch (P.spmv.unroll[pr]) {
             case 1: FEMGPU_SPMV_CASE(1); break;
             case 3: FEMGPU_SPMV_CASE(3); break;
@@ -443,15 +446,21 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
 #undef FEMGPU_SPMV_CASE
         return;
     }
-    const SpmvArgs A{P.d_rp, P.d_sptr, P.d_col, P.ps, nullptr, nullptr, nullptr, nullptr};
-#define FEMGPU_SPMV_CASE(U, NT) \
-    hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_FULL, NT>), g, blk, 0, P.stream, A, K, x, b, y, wt, P.d_ps, P.d_partials, sel)
-    const bool nt = P.spmv.nt[pr];
-    switch (P.spmv.unroll[pr]) {
-        case 1: if (nt) FEMGPU_SPMV_CASE(1, true); else FEMGPU_SPMV_CASE(1, false); break;
-        case 3: if (nt) FEMGPU_SPMV_CASE(3, true); else FEMGPU_SPMV_CASE(3, false); break;
-        default: if (nt) FEMGPU_SPMV_CASE(2, true); else FEMGPU_SPMV_CASE(2, false); break;
+    const SpmvArgs A{P.d_rp, P.d_sptr, P.d_col, P.ps, nullptr, nullptr, nullptr, nullptr, P.d_col16};
+#define FEMGPU_SPMV_CASE(U, LD) \
+    hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_FULL, LD>), g, blk, 0, P.stream, A, K, x, b, y, wt, P.d_ps, P.d_partials, sel)
+#define FEMGPU_SPMV_LOADS(U)                                                   \
+    switch (P.spmv.loads[pr]) {                                                \
+        case LOAD_PLAIN: FEMGPU_SPMV_CASE(U, LOAD_PLAIN); break;               \
+        case LOAD_NT16: FEMGPU_SPMV_CASE(U, LOAD_NT16); break;                 \
+        default: FEMGPU_SPMV_CASE(U, LOAD_NT); break;                          \
     }
+    switch (P.spmv.unroll[pr]) {
+        case 1: FEMGPU_SPMV_LOADS(1); break;
+        case 3: FEMGPU_SPMV_LOADS(3); break;
+        default: FEMGPU_SPMV_LOADS(2); break;
+    }
+#undef FEMGPU_SPMV_LOADS
 #undef FEMGPU_SPMV_CASE
 }
 
@@ -540,25 +549,25 @@ int autotune_spmv(Problem &P) {
     for (int pr = 0; pr < 2; ++pr) {
         double best[2] = {1e30, 1e30};
         int bu[2] = {P.spmv.unroll[pr], P.spmv.unroll[pr]};
-        bool bnt = false;
+        int bld = LOAD_NT;
         for (int f : {FMT_FULL, FMT_SYM})
-            for (int nt = 0; nt < (f == FMT_FULL ? 2 : 1); ++nt)
+            for (int ld = LOAD_PLAIN; ld <= (f == FMT_FULL ? (P.d_col16 ? LOAD_NT16 : LOAD_NT) : LOAD_PLAIN); ++ld)
                 for (int u : {1, 2, 3}) {
                     P.spmv.format[pr] = f;
                     P.spmv.unroll[pr] = u;
-                    P.spmv.nt[pr] = nt != 0;
+                    P.spmv.loads[pr] = ld;
                     if (time_spmv(P, pr, 2) < 0) continue;
                     const double t = time_spmv(P, pr, 10);
                     if (t > 0 && t < best[f]) {
                         best[f] = t;
                         bu[f] = u;
-                        if (f == FMT_FULL) bnt = nt != 0;
+                        if (f == FMT_FULL) bld = ld;
                     }
                 }
         const int f = best[FMT_SYM] < 0.97 * best[FMT_FULL] ? FMT_SYM : FMT_FULL;
         P.spmv.format[pr] = f;
         P.spmv.unroll[pr] = bu[f];
-        P.spmv.nt[pr] = f == FMT_FULL && bnt;
+        P.spmv.loads[pr] = f == FMT_FULL ? bld : LOAD_PLAIN;
     }
     if (P.spmv.format[0] != FMT_SYM && P.spmv.format[1] != FMT_SYM) sym_free(P.sym);
     else {
@@ -574,6 +583,7 @@ void destroy_graph(Problem &P) {
     if (P.graph_exec) (void)hipGraphExecDestroy(P.graph_exec);
     P.graph_exec = nullptr;
     P.graph_key = 0;
+    P.graph_x = nullptr;
 }
 
 static int read_scalars(Problem &P) {
@@ -651,10 +661,10 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
         return hipGetLastError() == hipSuccess ? 0 : 5;
     }
     const int pr = fp32 ? 1 : 0;
-    const uint64_t key = (reinterpret_cast<uint64_t>(d_x) << 13) ^ ((uint64_t)chunk << 7) ^ ((uint64_t)P.spmv.nt[pr] << 6) ^
-                         ((uint64_t)P.spmv.unroll[pr] << 4) ^ ((uint64_t)P.spmv.format[pr] << 3) ^ ((uint64_t)PRE << 1) ^
-                         (uint64_t)fp32;
-    if (!P.graph_exec || P.graph_key != key) {
+    // every field in its own bits; the solution pointer is compared separately
+    const uint64_t key = (uint64_t)fp32 | ((uint64_t)PRE << 1) | ((uint64_t)P.spmv.format[pr] << 3) |
+                         ((uint64_t)P.spmv.unroll[pr] << 4) | ((uint64_t)P.spmv.loads[pr] << 6) | ((uint64_t)chunk << 8);
+    if (!P.graph_exec || P.graph_key != key || P.graph_x != d_x) {
         destroy_graph(P);
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 5;
@@ -664,6 +674,7 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) { P.graph_exec = nullptr; return 5; }
         P.graph_key = key;
+        P.graph_x = d_x;
     }
     return hipGraphLaunch(P.graph_exec, P.stream) == hipSuccess ? 0 : 5;
 }

@@ -215,7 +215,8 @@ class Formulation:
         return self.spmv_config()
 
     def spmv_config(self):
-        """{"fp64": (format, unroll, nontemporal), "fp32": (...)}, format SPMV_FULL or SPMV_SYM."""
+        """{"fp64": (format, unroll, loads), "fp32": (...)}: format SPMV_FULL or SPMV_SYM, loads
+        LOADS_PLAIN, LOADS_NT or LOADS_NT_COL16."""
         f = (C.c_int * 2)()
         u = (C.c_int * 2)()
         nt = (C.c_int * 2)()
@@ -233,7 +234,7 @@ class Formulation:
         return r.value, m.value
 
     def set_spmv_config(self, fp64=(A.SPMV_FULL, 2, 0), fp32=(A.SPMV_FULL, 2, 0)):
-        """Force the SpMV variant per precision: (format, unroll, nontemporal)."""
+        """Force the SpMV variant per precision: (format, unroll, loads)."""
         fp64, fp32 = tuple(fp64) + (0,) * (3 - len(fp64)), tuple(fp32) + (0,) * (3 - len(fp32))
         f = (C.c_int * 2)(int(fp64[0]), int(fp32[0]))
         u = (C.c_int * 2)(int(fp64[1]), int(fp32[1]))

@@ -42,23 +42,25 @@ def run(n):
     f.assemble()
     f.set_result(np.zeros(3 * len(c)))
     sz = f.sizes()
-    for fmt, nt in ((A.SPMV_FULL, 1), (A.SPMV_FULL, 0), (A.SPMV_SYM, 0)):
+    for fmt, nt in ((A.SPMV_FULL, A.LOADS_NT), (A.SPMV_FULL, A.LOADS_NT_COL16), (A.SPMV_FULL, A.LOADS_PLAIN),
+                    (A.SPMV_SYM, A.LOADS_PLAIN)):
         f.set_spmv_config(fp64=(fmt, 2, nt), fp32=(fmt, 2, nt))
         t64 = f.time_spmv(L.FP64, 5)
         t32 = f.time_spmv(L.MIXED, 5)
-        print(json.dumps({"format": fmt, "nontemporal": nt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
+        print(json.dumps({"format": fmt, "loads": nt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
                           "spmv_fp32_ms": t32}), flush=True)
 
 
-def spmv_bytes(nnzb, n_nodes, fp32, fmt):
+def spmv_bytes(nnzb, n_nodes, fp32, fmt, col16=0):
     """Algorithmic bytes of one block SpMV: every stored value and index read once, x read once,
-    y written once.  Symmetric format: upper blocks (incl. diagonal) with values and columns,
-    lower blocks as column + position (their values are the upper blocks of other rows)."""
+    y written once.  Full format: 4-byte columns, or 2-byte column offsets (col16).  Symmetric
+    format: upper blocks (incl. diagonal) with values and columns, lower blocks as column +
+    position (their values are the upper blocks of other rows)."""
     v = 36 if fp32 else 72
     ndof = 3 * n_nodes
     vec = ndof * (8 if fp32 else 16)
     if fmt == 0:
-        return nnzb * (v + 4) + n_nodes * 4 + vec
+        return nnzb * (v + (2 if col16 else 4)) + n_nodes * 4 + vec
     nu, nl = (nnzb + n_nodes) // 2, (nnzb - n_nodes) // 2
     return nu * (v + 4) + nl * 8 + n_nodes * 8 + vec
 
@@ -69,14 +71,14 @@ def _counters(d):
         with open(p) as f:
             rows += list(csv.DictReader(f))
     per = {}
-    pat = re.compile(r"k_spmv<(double|float), 0, (\d), (\d), (true|false)>")
+    pat = re.compile(r"k_spmv<(double|float), 0, (\d), (\d), (\d)>")
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
         if "k_copy" in name:
             key = "copy"
         elif m:
-            kind = "_sym" if m.group(3) == "1" else ("" if m.group(4) == "true" else "_plain")
+            kind = "_sym" if m.group(3) == "1" else {"0": "_plain", "1": "", "2": "_col16"}[m.group(4)]
             key = "spmv_%s%s" % ("fp64" if m.group(1) == "double" else "fp32", kind)
         else:
             continue
@@ -91,9 +93,10 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
     w_scale = known_kib / (sum(wr["copy"]) / len(wr["copy"]))
     out = {"copy_scale": {"FETCH_SIZE": f_scale, "WRITE_SIZE": w_scale},
            "note": "bytes per launch = mean counter (KiB) x 1024 x scale; scale = known copy KiB / counted KiB"}
-    for key, fp32, fmt in (("spmv_fp64", False, 0), ("spmv_fp32", True, 0), ("spmv_fp64_plain", False, 0),
-                           ("spmv_fp32_plain", True, 0), ("spmv_fp64_sym", False, 1), ("spmv_fp32_sym", True, 1)):
-        alg = spmv_bytes(nnzb, n_nodes, fp32, fmt)
+    for key, fp32, fmt, c16 in (("spmv_fp64", False, 0, 0), ("spmv_fp32", True, 0, 0), ("spmv_fp64_col16", False, 0, 1),
+                                ("spmv_fp32_col16", True, 0, 1), ("spmv_fp64_plain", False, 0, 0),
+                                ("spmv_fp32_plain", True, 0, 0), ("spmv_fp64_sym", False, 1, 0), ("spmv_fp32_sym", True, 1, 0)):
+        alg = spmv_bytes(nnzb, n_nodes, fp32, fmt, c16)
         if key not in fe or key not in wr:
             continue
         fb = sum(fe[key]) / len(fe[key]) * 1024.0 * f_scale

@@ -328,7 +328,7 @@ def test_autotune_keeps_results():
     assert f.solve(rel_tol=1e-11)
     u0 = f.get_result()
     cfg = f.autotune()
-    assert all(fm in (0, 1) and u in (1, 2, 3) and nt in (0, 1) for fm, u, nt in cfg.values())
+    assert all(fm in (0, 1) and u in (1, 2, 3) and ld in (0, 1, 2) for fm, u, ld in cfg.values())
     f.set_result(np.zeros_like(u0))
     assert f.solve(rel_tol=1e-11)
     assert np.linalg.norm(f.get_result() - u0) <= 1e-9 * np.linalg.norm(u0)
@@ -347,7 +347,7 @@ def test_symmetric_spmv_format_matches_oracle(oracle, kind):
     x_ref, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-13)
     assert rc == 0
     its = {}
-    for fmt, nt in ((L._capi.SPMV_FULL, 0), (L._capi.SPMV_FULL, 1), (L._capi.SPMV_SYM, 0)):
+    for fmt, nt in ((L._capi.SPMV_FULL, 0), (L._capi.SPMV_FULL, 1), (L._capi.SPMV_FULL, 2), (L._capi.SPMV_SYM, 0)):
         for u in (1, 3):
             f.set_spmv_config(fp64=(fmt, u, nt), fp32=(fmt, u, nt))
             for prec, tol in ((L.FP64, 1e-10), (L.MIXED, 1e-8)):

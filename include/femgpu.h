@@ -291,6 +291,28 @@ FEMGPU_API int femgpu_get_assembly_rows(femgpu_problem *p, uint32_t *rows, uint3
 /* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */
 FEMGPU_API int femgpu_set_graph_replay(femgpu_problem *p, int enable);
 
+/* Storage order of the block rows in the SELL-64 slices (64 rows per slice, each slice as wide as
+ * its longest row).  FEMGPU_ROW_ORDER_BY_LENGTH stores the rows of every run of 1024 block rows
+ * (four workgroups of one sub-problem) by decreasing length, so a slice holds rows of similar
+ * length and pads less (meshes mixing node kinds, such as Tetra_10 vertex and edge nodes);
+ * FEMGPU_ROW_ORDER_AUTO does so when it saves more than 3% of the positions.  The order is
+ * internal: every vector and the CSR view keep the reference's row order. */
+typedef enum {
+    FEMGPU_ROW_ORDER_AUTO = 0,
+    FEMGPU_ROW_ORDER_NATURAL = 1,
+    FEMGPU_ROW_ORDER_BY_LENGTH = 2
+} femgpu_row_order;
+/* Process-wide order for problems and batches created afterwards (default AUTO). */
+FEMGPU_API int femgpu_set_row_order(int order);
+/* Order a problem uses (NATURAL or BY_LENGTH) and its SELL positions (blocks + padding). */
+FEMGPU_API int femgpu_get_row_order(femgpu_problem *p, int *order, uint64_t *n_positions);
+/* Host only: the storage order the library would choose for a block-row length profile
+ * (row_ptr, n_brows + 1 entries; one sub-problem).  perm[t] = block row stored at t (may be
+ * NULL); *n_natural and *n_positions receive the SELL positions of the natural and the chosen
+ * order (either may be NULL). */
+FEMGPU_API int femgpu_sell_order(uint32_t n_brows, const uint32_t *row_ptr, int order, uint32_t *perm,
+                                 uint64_t *n_natural, uint64_t *n_positions);
+
 #ifdef __cplusplus
 }
 #endif

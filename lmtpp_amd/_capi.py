@@ -18,6 +18,7 @@ JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED = 0, 1, 2, 3
 REL_L2, ABS_MAXNORM = 0, 1
 SPMV_FULL, SPMV_SYM = 0, 1
 LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
+ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
 NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}
@@ -108,6 +109,9 @@ _SIGS = {
     "femgpu_get_precond_setup_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
+    "femgpu_set_row_order": (C.c_int, [C.c_int]),
+    "femgpu_get_row_order": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "femgpu_sell_order": (C.c_int, [C.c_uint32, _P, C.c_int, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
 
 
@@ -157,6 +161,34 @@ def ic0_schedule(row_ptr, col, precond=BLOCK_IC0_COLOURED):
     check(lib.femgpu_ic0_schedule(n, rp.ctypes.data, cl.ctypes.data, precond, order.ctypes.data, lv.ctypes.data,
                                   C.byref(nl)), "ic0_schedule")
     return order, lv
+
+
+def sell_order(row_ptr, order=ROW_ORDER_BY_LENGTH):
+    """Storage order the library picks for a block-row length profile (host only).
+    Returns (perm, n_natural, n_positions): perm[t] = block row stored at t."""
+    import numpy as np
+    lib = load()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    n = len(rp) - 1
+    perm = np.zeros(n, np.uint32)
+    nat, npos = C.c_uint64(), C.c_uint64()
+    check(lib.femgpu_sell_order(n, rp.ctypes.data, order, perm.ctypes.data, C.byref(nat), C.byref(npos)), "sell_order")
+    return perm, nat.value, npos.value
+
+
+class row_order:
+    """Context manager: problems created inside use the given block-row storage order."""
+
+    def __init__(self, order):
+        self.order = order
+
+    def __enter__(self):
+        check(load().femgpu_set_row_order(self.order), "set_row_order")
+        return self
+
+    def __exit__(self, *exc):
+        check(load().femgpu_set_row_order(ROW_ORDER_AUTO), "set_row_order")
+        return False
 
 
 def stream_read_gbs(n_doubles=1 << 27, reps=20):

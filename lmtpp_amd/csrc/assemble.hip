@@ -152,8 +152,8 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
 }
 
 // ---------------------------------------------------------------------------------------------
-// Row-group element loop: workgroup G owns block rows [G R, G R + R) (R divides 64, so the group
-// lies in one SELL slice).  It walks the elements incident to its rows colour by colour, the same
+// Row-group element loop: workgroup G owns storage rows [G R, G R + R) (R divides 64, so the group
+// lies in one SELL slice; storage row t holds block row perm[t], perm null for the identity).  It walks the elements incident to its rows colour by colour, the same
 // order as the colour-pass kernel above, computes each element's geometry in a team of lanes and
 // adds the blocks (and loads) of its own rows into an LDS accumulator; rows of one colour's
 // elements are distinct, so no two lanes add to one entry at once and the summation order of
@@ -170,10 +170,10 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble_rows(
     uint32_t R, uint32_t n_rows, uint32_t nc, const uint32_t *__restrict__ gptr, const uint32_t *__restrict__ gel,
     const uint32_t *__restrict__ conn, const double *__restrict__ coords, const double *__restrict__ E,
     const uint32_t *__restrict__ elem_sub, const SubMaterial *__restrict__ mat, const uint32_t *__restrict__ slot,
-    const uint32_t *__restrict__ sptr, double *__restrict__ Kv, uint64_t ps, double *__restrict__ F, int do_mat,
-    int do_vec, int *flags) {
+    const uint32_t *__restrict__ sptr, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ iperm,
+    double *__restrict__ Kv, uint64_t ps, double *__restrict__ F, int do_mat, int do_vec, int *flags) {
     constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP, TEAM = ET<K>::TEAM, TPW = ASM_THREADS / TEAM;
-    __shared__ double acc[ASM_SLOTS * 9];   // acc[(k * 9 + q) * R + i]: row r0 + i, slot k, plane q
+    __shared__ double acc[ASM_SLOTS * 9];   // acc[(k * 9 + q) * R + i]: storage row r0 + i, slot k, plane q
     __shared__ double facc[64 * 3];
     __shared__ ElemLds<K> lds[TPW];
     const uint32_t G = blockIdx.x, r0 = G * R, r1 = min(r0 + R, n_rows);
@@ -211,14 +211,14 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble_rows(
         const SubMaterial m = mat[elem_sub[e]];
         if (do_mat && t < NPE * NPE) {
             const int a = t / NPE, b = t % NPE;
-            const uint32_t row = conn[(size_t)e * NPE + a];
-            if (row >= r0 && row < r1) {
+            const uint32_t row = conn[(size_t)e * NPE + a], tr = iperm ? iperm[row] : row;
+            if (tr >= r0 && tr < r1) {
                 double H[3];
                 gen::hooke(E[e], m.nu, H);
                 double k[9];
                 elem_block<K>(L, a < b ? a : b, a < b ? b : a, H, k);
-                const uint32_t kk = (slot[(size_t)e * NPE * NPE + t] - base - (row & 63u)) >> 6;
-                double *A = acc + (size_t)kk * 9 * R + (row - r0);
+                const uint32_t kk = (slot[(size_t)e * NPE * NPE + t] - base - (tr & 63u)) >> 6;
+                double *A = acc + (size_t)kk * 9 * R + (tr - r0);
                 if (a == b) {
                     k[3] = k[1]; k[6] = k[2]; k[7] = k[5];
 #pragma unroll
@@ -234,11 +234,11 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble_rows(
         }
         if (do_vec && t < NPE * 3) {
             const int a = t / 3, d = t % 3;
-            const uint32_t row = conn[(size_t)e * NPE + a];
-            if (row >= r0 && row < r1) {
+            const uint32_t row = conn[(size_t)e * NPE + a], tr = iperm ? iperm[row] : row;
+            if (tr >= r0 && tr < r1) {
                 double f3[3] = {0, 0, 0};
                 for (int q = 0; q < NGP; ++q) gen::load(L.N[q][a], m.bf0, m.bf1, m.bf2, L.wd[q], f3);
-                facc[(row - r0) * 3 + d] += f3[d];
+                facc[(tr - r0) * 3 + d] += f3[d];
             }
         }
     }
@@ -249,8 +249,10 @@ __global__ __launch_bounds__(ASM_THREADS) void k_assemble_rows(
             if (r0 + li < r1) Kv[q * ps + base + 64ull * kk + ((r0 + li) & 63u)] = acc[i];
         }
     if (do_vec)
-        for (uint32_t i = threadIdx.x; i < R * 3; i += blockDim.x)
-            if (r0 + i / 3 < r1) F[3 * (size_t)r0 + i] = facc[i];
+        for (uint32_t i = threadIdx.x; i < R * 3; i += blockDim.x) {
+            const uint32_t tr = r0 + i / 3;
+            if (tr < r1) F[3 * (size_t)(perm ? perm[tr] : tr) + i % 3] = facc[i];
+        }
 }
 
 // Skin loads, one thread per face, faces coloured like elements.  Gauss tables of the face
@@ -376,7 +378,7 @@ template <int K>
 static void assemble_rows_kind(Problem &P, int do_mat, int do_vec) {
     hipLaunchKernelGGL(k_assemble_rows<K>, dim3(P.rg.n_groups), dim3(ASM_THREADS), 0, P.stream, P.rg.R, P.n_nodes,
                        P.rg.n_colors, P.d_gptr, P.d_gel, P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.d_slot,
-                       P.d_sptr, P.d_K, P.ps, P.d_F, do_mat, do_vec, P.d_flags);
+                       P.d_sptr, P.d_perm, P.d_iperm, P.d_K, P.ps, P.d_F, do_mat, do_vec, P.d_flags);
 }
 
 uint32_t max_assembly_rows(const Problem &P) {

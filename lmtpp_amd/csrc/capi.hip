@@ -2,6 +2,7 @@
 // include/femgpu.h).  Single problems and batches share one implementation: a Problem with
 // n_sub sub-problems (n_sub = 1 for femgpu_create).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -70,7 +71,7 @@ void free_problem(Problem &P) {
     dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32);
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
-    dfree(P.d_gptr); dfree(P.d_gel);
+    dfree(P.d_gptr); dfree(P.d_gel); dfree(P.d_perm); dfree(P.d_iperm);
     if (P.h_ps) (void)hipHostFree(P.h_ps);
     if (P.h_flags) (void)hipHostFree(P.h_flags);
     P.h_ps = nullptr;
@@ -110,7 +111,8 @@ int set_assembly_rows(Problem &P, uint32_t R) {
     if (R > 64 || (R & (R - 1)) || R > max_assembly_rows(P))
         return fail(FEMGPU_BAD_INPUT, "assembly rows: a power of two, at most " + std::to_string(max_assembly_rows(P)));
     if (P.rg.R != R) {
-        P.rg = build_row_groups(P.n_nodes, R, P.n_elem, P.npe, P.conn_h.data(), P.ecol);
+        P.rg = build_row_groups(P.n_nodes, R, P.n_elem, P.npe, P.conn_h.data(), P.ecol,
+                                P.pat.iperm.empty() ? nullptr : P.pat.iperm.data());
         dfree(P.d_gptr);
         dfree(P.d_gel);
         HIPCHK(upload(&P.d_gptr, P.rg.ptr.data(), P.rg.ptr.size()));
@@ -120,21 +122,33 @@ int set_assembly_rows(Problem &P, uint32_t R) {
     return FEMGPU_OK;
 }
 
+namespace {
+std::atomic<int> g_row_order{ROW_ORDER_AUTO};
+}
+int row_order_default() { return g_row_order.load(); }
+
 // Device buffers common to mesh problems and plain block systems.
 int alloc_system(Problem &P) {
     const size_t n = 3 * (size_t)P.n_nodes;
-    P.ps = (P.pat.n_sell + 63) / 64 * 64;
     HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
     for (auto &e : P.ev) HIPCHK(hipEventCreate(&e));
+    const int order = row_order_default();
+    if (order != ROW_ORDER_NATURAL)
+        sort_rows_by_length(P.pat, P.node_off, SORT_WINDOW_WGS * ROWS_PER_WG, order == ROW_ORDER_BY_LENGTH ? -1.0 : 0.03);
+    P.ps = (P.pat.n_sell + 63) / 64 * 64;
+    if (!P.pat.perm.empty()) {
+        HIPCHK(upload(&P.d_perm, P.pat.perm.data(), P.pat.perm.size()));
+        HIPCHK(upload(&P.d_iperm, P.pat.iperm.data(), P.pat.iperm.size()));
+    }
     HIPCHK(upload(&P.d_rp, P.pat.row_ptr.data(), P.pat.row_ptr.size()));
     {
         std::vector<uint32_t> cs = sell_columns(P.pat);
         HIPCHK(upload(&P.d_col, cs.data(), cs.size()));
-        // 16-bit column offsets c - r (row of position p from the slice walk)
+        // 16-bit column offsets c - r, r the block row held by the position's storage row
         std::vector<int16_t> c16(cs.size(), 0);
         bool fits = true;
-        for (uint32_t r = 0; r < P.pat.n_rows && fits; ++r) {
-            const uint32_t s = r >> 6, w = (P.pat.sell_ptr[s + 1] - P.pat.sell_ptr[s]) / 64;
+        for (uint32_t t = 0; t < P.pat.n_rows && fits; ++t) {
+            const uint32_t r = P.pat.row_of(t), s = t >> 6, w = (P.pat.sell_ptr[s + 1] - P.pat.sell_ptr[s]) / 64;
             for (uint32_t k = 0; k < w; ++k) {
                 const int64_t d = (int64_t)cs[P.pat.pos(r, k)] - (int64_t)r;
                 if (d < -32768 || d > 32767) { fits = false; break; }
@@ -970,6 +984,37 @@ int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unr
         P.spmv.loads[i] = loads ? loads[i] : LOAD_PLAIN;
     }
     destroy_graph(P);
+    return FEMGPU_OK;
+}
+
+int femgpu_set_row_order(int order) {
+    if (order < ROW_ORDER_AUTO || order > ROW_ORDER_BY_LENGTH) return fail(FEMGPU_BAD_INPUT, "set_row_order: 0, 1 or 2");
+    g_row_order.store(order);
+    return FEMGPU_OK;
+}
+
+int femgpu_get_row_order(femgpu_problem *h, int *order, uint64_t *n_positions) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_row_order: null");
+    if (order) *order = h->P.pat.perm.empty() ? ROW_ORDER_NATURAL : ROW_ORDER_BY_LENGTH;
+    if (n_positions) *n_positions = h->P.pat.n_sell;
+    return FEMGPU_OK;
+}
+
+int femgpu_sell_order(uint32_t n_brows, const uint32_t *row_ptr, int order, uint32_t *perm, uint64_t *n_natural,
+                      uint64_t *n_positions) {
+    if (!n_brows || !row_ptr || order < ROW_ORDER_AUTO || order > ROW_ORDER_BY_LENGTH)
+        return fail(FEMGPU_BAD_INPUT, "sell_order: bad input");
+    for (uint32_t r = 0; r < n_brows; ++r)
+        if (row_ptr[r + 1] < row_ptr[r]) return fail(FEMGPU_BAD_INPUT, "sell_order: row_ptr decreasing");
+    Pattern pat;
+    pat.n_rows = n_brows;
+    pat.row_ptr.assign(row_ptr, row_ptr + n_brows + 1);
+    if (order != ROW_ORDER_NATURAL)
+        pat.perm = length_order(pat, {0, n_brows}, SORT_WINDOW_WGS * ROWS_PER_WG, order == ROW_ORDER_BY_LENGTH ? -1.0 : 0.03);
+    if (perm)
+        for (uint32_t t = 0; t < n_brows; ++t) perm[t] = pat.row_of(t);
+    if (n_natural) *n_natural = sell_size(pat, {});
+    if (n_positions) *n_positions = sell_size(pat, pat.perm);
     return FEMGPU_OK;
 }
 

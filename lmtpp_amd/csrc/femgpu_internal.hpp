@@ -37,6 +37,12 @@ struct SubMaterial { double nu, bf0, bf1, bf2; };
 
 constexpr uint32_t ROWS_PER_WG = 256;
 constexpr uint32_t NO_SUB = 0xffffffffu;
+// Storage order of the block rows, chosen when a problem is created: AUTO sorts rows by length
+// within windows of SORT_WINDOW_WGS workgroups when that saves more than 3% of the SELL
+// positions; NATURAL keeps t = r; BY_LENGTH always sorts.
+enum { ROW_ORDER_AUTO = 0, ROW_ORDER_NATURAL = 1, ROW_ORDER_BY_LENGTH = 2 };
+constexpr uint32_t SORT_WINDOW_WGS = 4;
+int row_order_default();
 
 // Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format, SELL slots per loop
 // trip (1, 2 or 3) and how the full format's matrix is loaded: plain loads, non-temporal loads,
@@ -96,6 +102,8 @@ struct Problem {
     // device: mesh, pattern, scatter map
     double *d_coords = nullptr, *d_E = nullptr;
     int16_t *d_col16 = nullptr;  // SELL column offsets c - r, when all fit in 16 bits (else null)
+    // storage order of the block rows (null: identity): storage row t holds block row d_perm[t]
+    uint32_t *d_perm = nullptr, *d_iperm = nullptr;
     uint32_t *d_conn = nullptr, *d_rp = nullptr, *d_col = nullptr, *d_diag = nullptr, *d_slot = nullptr,
              *d_order = nullptr, *d_sptr = nullptr, *d_elem_sub = nullptr;
     SubMaterial *d_mat = nullptr;

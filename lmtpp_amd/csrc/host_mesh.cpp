@@ -260,12 +260,23 @@ Pattern build_pattern(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t
     return P;
 }
 
+uint64_t sell_size(const Pattern &P, const std::vector<uint32_t> &perm) {
+    const uint32_t n = P.n_rows;
+    uint64_t tot = 0;
+    for (uint32_t t0 = 0; t0 < n; t0 += 64) {
+        uint32_t w = 0;
+        for (uint32_t t = t0; t < std::min(n, t0 + 64); ++t) w = std::max(w, P.len(perm.empty() ? t : perm[t]));
+        tot += 64ull * w;
+    }
+    return tot;
+}
+
 void finish_pattern(Pattern &P) {
     const uint32_t n = P.n_rows, ns = (n + 63) / 64;
     P.sell_ptr.assign((size_t)ns + 1, 0);
     for (uint32_t s = 0; s < ns; ++s) {
         uint32_t w = 0;
-        for (uint32_t r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, P.row_ptr[r + 1] - P.row_ptr[r]);
+        for (uint32_t t = 64 * s; t < std::min(n, 64 * s + 64); ++t) w = std::max(w, P.len(P.row_of(t)));
         P.sell_ptr[s + 1] = P.sell_ptr[s] + 64 * w;
     }
     P.n_sell = P.sell_ptr[ns];
@@ -276,10 +287,54 @@ void finish_pattern(Pattern &P) {
     }
 }
 
+std::vector<uint32_t> length_order(const Pattern &P, const std::vector<uint32_t> &part, uint32_t window,
+                                   double min_gain) {
+    const uint32_t n = P.n_rows;
+    if (part.size() < 2 || part.front() != 0 || part.back() != n || window == 0) return {};
+    std::vector<uint32_t> perm(n);
+    for (uint32_t t = 0; t < n; ++t) perm[t] = t;
+    bool changed = false;
+    auto slices = [&](uint32_t a, uint32_t b, bool sorted) {  // positions of rows a..b as their own slices
+        uint64_t tot = 0;
+        for (uint32_t t0 = a; t0 < b; t0 += 64) {
+            uint32_t w = 0;
+            for (uint32_t t = t0; t < std::min(b, t0 + 64); ++t) w = std::max(w, P.len(sorted ? perm[t] : t));
+            tot += 64ull * w;
+        }
+        return tot;
+    };
+    for (size_t s = 0; s + 1 < part.size(); ++s) {
+        const uint32_t a = part[s], b = part[s + 1];
+        if (a > b) return {};
+        for (uint32_t w0 = a; w0 < b; w0 += window)
+            std::stable_sort(perm.begin() + w0, perm.begin() + std::min(b, w0 + window),
+                             [&](uint32_t x, uint32_t y) { return P.len(x) > P.len(y); });
+        bool same = true;
+        for (uint32_t t = a; t < b && same; ++t) same = perm[t] == t;
+        const bool keep = !same && (min_gain < 0 || (double)slices(a, b, true) < (1.0 - min_gain) * (double)slices(a, b, false));
+        if (!keep)
+            for (uint32_t t = a; t < b; ++t) perm[t] = t;
+        changed |= keep;
+    }
+    if (!changed) return {};
+    return perm;
+}
+
+bool sort_rows_by_length(Pattern &P, const std::vector<uint32_t> &part, uint32_t window, double min_gain) {
+    std::vector<uint32_t> perm = length_order(P, part, window, min_gain);
+    if (perm.empty()) return false;
+    const uint32_t n = P.n_rows;
+    P.iperm.assign(n, 0);
+    for (uint32_t t = 0; t < n; ++t) P.iperm[perm[t]] = t;
+    P.perm = std::move(perm);
+    finish_pattern(P);
+    return true;
+}
+
 std::vector<uint32_t> sell_columns(const Pattern &P) {
     std::vector<uint32_t> c(std::max<uint64_t>(P.n_sell, 1));
-    for (uint32_t r = 0; r < P.n_rows; ++r) {
-        const uint32_t s = r >> 6, w = (P.sell_ptr[s + 1] - P.sell_ptr[s]) / 64, len = P.row_ptr[r + 1] - P.row_ptr[r];
+    for (uint32_t t = 0; t < P.n_rows; ++t) {
+        const uint32_t r = P.row_of(t), s = t >> 6, w = (P.sell_ptr[s + 1] - P.sell_ptr[s]) / 64, len = P.len(r);
         for (uint32_t k = 0; k < w; ++k) c[P.pos(r, k)] = k < len ? P.col[P.row_ptr[r] + k] : r;
     }
     return c;
@@ -331,7 +386,7 @@ Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32
 }
 
 RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe, const uint32_t *conn,
-                           const Coloring &col) {
+                           const Coloring &col, const uint32_t *iperm) {
     RowGroups G;
     G.R = R;
     G.n_groups = (n_rows + R - 1) / R;
@@ -353,7 +408,7 @@ RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe
         uint32_t gs[10];
         int ng = 0;
         for (int a = 0; a < npe; ++a) {
-            const uint32_t g = conn[(size_t)e * npe + a] / R;
+            const uint32_t v = conn[(size_t)e * npe + a], g = (iperm ? iperm[v] : v) / R;
             bool seen = false;
             for (int k = 0; k < ng; ++k) seen |= gs[k] == g;
             if (!seen) gs[ng++] = g;

@@ -23,22 +23,42 @@ int make_rect(int kind, const double X0[3], const double X1[3], const uint32_t n
 std::vector<uint32_t> skin_faces(int kind, uint32_t n_nodes, uint32_t n_elem, const uint32_t *conn);
 
 // Full symmetric 3x3-block pattern: node adjacency, rows sorted, every row holds its diagonal.
-// Device storage of the blocks is sliced ELL (SELL-64): block rows are grouped in slices of 64
-// consecutive rows, slice s is as wide as its longest row, and block k of row r sits at
-// position pos(r, k) = sell_ptr[r/64] + 64 k + r%64, so the 64 rows of a slice read their k-th
-// blocks from 64 consecutive positions.
+// Device storage of the blocks is sliced ELL (SELL-64): storage rows t are grouped in slices of
+// 64 consecutive t, slice s is as wide as its longest row, and block k of the row stored at t
+// sits at position sell_ptr[t/64] + 64 k + t%64, so the 64 rows of a slice read their k-th blocks
+// from 64 consecutive positions.  Storage row t holds block row perm[t]; perm and its inverse
+// iperm are empty for the identity (t = r) and set by sort_rows_by_length.
 struct Pattern {
     uint32_t n_rows = 0;
     std::vector<uint32_t> row_ptr, col;   // CSR view (the API and the reference's row order)
     std::vector<uint32_t> sell_ptr;       // n_slices + 1 slice starts, in block positions
     std::vector<uint32_t> diag;           // diag[r] = SELL position of block (r,r)
+    std::vector<uint32_t> perm, iperm;    // storage row -> block row, block row -> storage row
     uint64_t n_sell = 0;                  // positions including padding
-    uint64_t pos(uint32_t r, uint32_t k) const { return sell_ptr[r >> 6] + 64ull * k + (r & 63u); }
+    uint32_t t_of(uint32_t r) const { return iperm.empty() ? r : iperm[r]; }
+    uint32_t row_of(uint32_t t) const { return perm.empty() ? t : perm[t]; }
+    uint32_t len(uint32_t r) const { return row_ptr[r + 1] - row_ptr[r]; }
+    uint64_t pos(uint32_t r, uint32_t k) const {
+        const uint32_t t = t_of(r);
+        return sell_ptr[t >> 6] + 64ull * k + (t & 63u);
+    }
     uint64_t pos_of_slot(uint32_t r, uint32_t slot) const { return pos(r, slot - row_ptr[r]); }
 };
 Pattern build_pattern(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t *conn);
-// Fills sell_ptr, n_sell and diag from row_ptr / col.
+// Fills sell_ptr, n_sell and diag from row_ptr / col and the storage order.
 void finish_pattern(Pattern &P);
+// Positions the slices would take with rows stored in the order perm (empty: identity).
+uint64_t sell_size(const Pattern &P, const std::vector<uint32_t> &perm);
+// SELL-C-sigma: the rows of every part [part[i], part[i+1]) (parts cover 0..n_rows in increasing
+// order: the sub-problems) are cut into windows of `window` rows from the part's start, and the
+// rows of a window are stored by decreasing length (ties by row), so a slice holds rows of
+// similar length and pads less.  A part keeps the sorted order when it saves more than min_gain of
+// the positions its rows take as slices of their own (min_gain < 0: always), so a mesh gets the
+// same order alone or in a batch.  Returns whether the storage order changed.
+bool sort_rows_by_length(Pattern &P, const std::vector<uint32_t> &part, uint32_t window, double min_gain);
+// The order sort_rows_by_length would set (needs only n_rows and row_ptr); empty for the identity.
+std::vector<uint32_t> length_order(const Pattern &P, const std::vector<uint32_t> &part, uint32_t window,
+                                   double min_gain);
 // Column of every SELL position (padding positions hold their own row, with zero values).
 std::vector<uint32_t> sell_columns(const Pattern &P);
 
@@ -53,15 +73,16 @@ struct Coloring {
 };
 Coloring color_elements(uint32_t n_nodes, uint32_t n_elem, int npe, const uint32_t *conn);
 
-// Row-group assembly work lists: rows are cut into groups of R consecutive rows; group G lists
-// the elements incident to its rows, sorted by (colour, element), colour c of group G spanning
-// elems[ptr[G*(nc+1)+c] .. ptr[G*(nc+1)+c+1]).
+// Row-group assembly work lists: storage rows are cut into groups of R consecutive rows; group G
+// lists the elements incident to its block rows, sorted by (colour, element), colour c of group G
+// spanning elems[ptr[G*(nc+1)+c] .. ptr[G*(nc+1)+c+1]).  iperm: storage row of each block row
+// (nullptr: identity).
 struct RowGroups {
     uint32_t R = 0, n_groups = 0, n_colors = 0;
     std::vector<uint32_t> ptr, elems;
 };
 RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe, const uint32_t *conn,
-                           const Coloring &col);
+                           const Coloring &col, const uint32_t *iperm = nullptr);
 
 // Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);

@@ -24,9 +24,11 @@ namespace femgpu {
 enum { MODE_PLAIN = 0, MODE_PQ = 1, MODE_RES = 2 };
 
 // ---------------------------------------------------------------------------------------------
-// Block SpMV on SELL-64 storage: one lane per block row; the 64 lanes of a wave are 64
-// consecutive rows, so every value-plane and column load of slot k is one contiguous run.
-// Rows shorter than their slice stop early (masked lanes fetch nothing).
+// Block SpMV on SELL-64 storage: one lane per storage row; the 64 lanes of a wave are 64
+// consecutive storage rows, so every value-plane and column load of slot k is one contiguous run.
+// Rows shorter than their slice stop early (masked lanes fetch nothing).  FMT_FULL storage row t
+// holds block row perm[t] (rows sorted by length within windows of whole workgroups, so a
+// workgroup's storage rows are block rows of its own sub-problem); FMT_SYM keeps t = r.
 // FMT_FULL: every block of the row is stored in the row (row lengths from row_ptr).
 // FMT_SYM: the row stores its upper blocks (c >= r); each lower block K_rc = K_cr^T is read from
 // row c's upper storage through tposL, as MatWithTinyBlocks::partial_mul uses every stored block
@@ -40,6 +42,7 @@ struct SpmvArgs {
     uint64_t ps;
     const uint32_t *sptrL, *lenL, *colL, *tposL;  // FMT_SYM lower part
     const int16_t *col16;   // LOAD_NT16: column offsets c - r
+    const uint32_t *perm;   // FMT_FULL: block row held by storage row t (null: t)
 };
 
 // NT: the matrix values and columns are read with non-temporal loads (streamed once; L2 is left
@@ -119,13 +122,14 @@ __global__ __launch_bounds__(256) void k_spmv(SpmvArgs A, const T *__restrict__ 
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB) return;
     if (MODE != MODE_PLAIN && !selected(S[s], MODE == MODE_PQ ? (int)SEL_ACTIVE : sel)) return;
-    const uint32_t r = wt.row0[L] + threadIdx.x;
+    const uint32_t t = wt.row0[L] + threadIdx.x;
     double acc[2] = {0.0, 0.0};
-    if (r < wt.row1[L]) {
+    if (t < wt.row1[L]) {
         T s0 = 0, s1 = 0, s2 = 0;
-        const uint64_t off = r & 63u;
+        const uint64_t off = t & 63u;
+        const uint32_t r = FMT == FMT_FULL && A.perm ? A.perm[t] : t;
         if (FMT == FMT_FULL) {
-            spmv_rows<T, UNROLL, LD != LOAD_PLAIN, LD == LOAD_NT16>(A.col, A.col16, r, Kv, A.ps, A.sptr[r >> 6] + off,
+            spmv_rows<T, UNROLL, LD != LOAD_PLAIN, LD == LOAD_NT16>(A.col, A.col16, r, Kv, A.ps, A.sptr[t >> 6] + off,
                                                                     A.len[r + 1] - A.len[r], x, s0, s1, s2);
         } else {
             spmv_rows<T, UNROLL, false, false>(A.col, nullptr, r, Kv, A.ps, A.sptr[r >> 6] + off, A.len[r], x, s0, s1, s2);
@@ -434,7 +438,7 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     const int pr = sizeof(T) == 4 ? 1 : 0;
     if (P.spmv.format[pr] == FMT_SYM) {
         const SymDev &Y = P.sym;
-        const SpmvArgs A{Y.d_lenU, Y.d_sptrU, Y.d_colU, Y.psU, Y.d_sptrL, Y.d_lenL, Y.d_colL, Y.d_tposL, nullptr};
+        const SpmvArgs A{Y.d_lenU, Y.d_sptrU, Y.d_colU, Y.psU, Y.d_sptrL, Y.d_lenL, Y.d_colL, Y.d_tposL, nullptr, nullptr};
         const T *Ks = pr ? reinterpret_cast<const T *>(Y.d_K32) : reinterpret_cast<const T *>(Y.d_K);
 #define FEMGPU_SPMV_CASE(U) \
     hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_SYM, LOAD_PLAIN>), g, blk, 0, P.stream, A, Ks, x, b, y, wt, P.d_ps, P.d_partials, sel)
@@ -446,7 +450,7 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
 #undef FEMGPU_SPMV_CASE
         return;
     }
-    const SpmvArgs A{P.d_rp, P.d_sptr, P.d_col, P.ps, nullptr, nullptr, nullptr, nullptr, P.d_col16};
+    const SpmvArgs A{P.d_rp, P.d_sptr, P.d_col, P.ps, nullptr, nullptr, nullptr, nullptr, P.d_col16, P.d_perm};
 #define FEMGPU_SPMV_CASE(U, LD) \
     hipLaunchKernelGGL((k_spmv<T, MODE, U, FMT_FULL, LD>), g, blk, 0, P.stream, A, K, x, b, y, wt, P.d_ps, P.d_partials, sel)
 #define FEMGPU_SPMV_LOADS(U)                                                   \

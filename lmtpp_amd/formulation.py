@@ -233,6 +233,13 @@ class Formulation:
         A.check(self._lib.femgpu_get_assembly_rows(self._h, C.byref(r), C.byref(m)), "get_assembly_rows")
         return r.value, m.value
 
+    def row_order(self):
+        """(storage order of the block rows: ROW_ORDER_NATURAL or ROW_ORDER_BY_LENGTH, SELL
+        positions including padding)."""
+        o, n = C.c_int(), C.c_uint64()
+        A.check(self._lib.femgpu_get_row_order(self._h, C.byref(o), C.byref(n)), "get_row_order")
+        return o.value, n.value
+
     def set_spmv_config(self, fp64=(A.SPMV_FULL, 2, 0), fp32=(A.SPMV_FULL, 2, 0)):
         """Force the SpMV variant per precision: (format, unroll, loads)."""
         fp64, fp32 = tuple(fp64) + (0,) * (3 - len(fp64)), tuple(fp32) + (0,) * (3 - len(fp32))

@@ -246,7 +246,8 @@ FEMGPU_API int femgpu_get_assemble_ms(femgpu_problem *p, double *ms);
  * IC(0) factorisation and its FP32 copy); 0 when the previous set-up was reused. */
 FEMGPU_API int femgpu_get_precond_setup_ms(femgpu_problem *p, double *ms);
 /* Average device time (ms) of one block-SpMV launch (precision FP64 or the FP32 copy), measured
- * with HIP events around reps back-to-back launches on the problem's stream. */
+ * with HIP events around reps back-to-back launches on the problem's stream, after one untimed
+ * launch. */
 FEMGPU_API int femgpu_time_spmv(femgpu_problem *p, int precision, int reps, double *avg_ms);
 
 /* Average device time (ms) of a plain copy of n doubles (n even; 16 B per lane) over reps

@@ -531,8 +531,12 @@ static int spmv_ensure(Problem &P, int precision) {
     return 0;
 }
 
+// Average device time of one product over reps launches, after one untimed launch (the first
+// launch of a kernel in a process also loads its code object).
 double time_spmv(Problem &P, int precision, int reps) {
     if (spmv_ensure(P, precision)) return -1;
+    if (precision == 1) launch_spmv_t<float, MODE_PLAIN>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, 0);
+    else launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, P.d_p, nullptr, P.d_q, 0);
     (void)hipEventRecord(P.ev[2], P.stream);
     for (int i = 0; i < reps; ++i) {
         if (precision == 1) launch_spmv_t<float, MODE_PLAIN>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, 0);

@@ -314,6 +314,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-configs", action="store_true", help="skip configurations 0, 2, 3, 4")
     ap.add_argument("--no-precond", action="store_true", help="skip the preconditioner comparison")
+    ap.add_argument("--no-stepping", action="store_true", help="skip the time-stepping measurement")
     ap.add_argument("--no-natural-ic0", action="store_true",
                     help="skip the natural-order block IC(0) solve (hundreds of level launches per iteration)")
     args = ap.parse_args()
@@ -435,6 +436,28 @@ def main():
     as_bytes = sz["n_nodes"] * 48 + sz["n_elem"] * (4 * 8 + 8 + 13 * 8)
     a_ms = float(np.median(as_ms))
 
+    # -- time steps (ProblemAncestor::iteration): traction ramp, constant K, warm starts ----------
+    ts_rows = []
+    if not args.no_stepping:
+        f.precision, f.precond = L.MIXED, L.BLOCK_JACOBI
+        f.assume_constant_matrix = True
+        f.set_initial_time_step(1.0)
+        f.set_result(np.zeros(ndof))
+        prob = L.Problem([f], rel_tol=mtol)
+        for k in range(1, 6):
+            f.set_traction(faces, k * tr)
+            t1 = time.perf_counter()
+            prob.iteration()
+            wall_k = (time.perf_counter() - t1) * 1e3
+            st = f.last_stats
+            ts_rows.append({"step": k, "time": prob.time, "iters": int(st.iters), "solve_ms": round(st.solve_ms, 2),
+                            "assembly_ms": round(f.assemble_ms(), 3), "wall_ms": round(wall_k, 2),
+                            "rel_err": float(np.linalg.norm(f.get_result() - k * u_ref) / (k * nref))})
+            log("time step %d: %d iters, solve %.1f ms, assembly %.2f ms, wall %.1f ms, err %.1e" % (
+                k, st.iters, st.solve_ms, f.assemble_ms(), wall_k, ts_rows[-1]["rel_err"]))
+        prob.close()
+        f.assume_constant_matrix = False
+
     # -- roofline of the dominant kernel (FP32 block SpMV), HIP events on the problem's stream --
     reps = 50
     t32 = f.time_spmv(L.MIXED, reps)
@@ -478,6 +501,10 @@ def main():
                                "traffic": load_traffic("spmv_fp64" + ("_sym" if fmt64 else load_key[spmv_cfg["fp64"][2]])),
                                "kernel": "k_spmv<double,0,%d,%d,%d>" % (spmv_cfg["fp64"][1], fmt64, spmv_cfg["fp64"][2])},
         "preconditioners": precond,
+        "time_stepping": {"workload": "5 steps of ProblemAncestor::iteration, traction x step, assume_constant_matrix "
+                                      "(K, its FP32 copy and the block-Jacobi factors built at step 1; steps 2-5 "
+                                      "assemble F only), MIXED PCG warm-started from the previous step",
+                          "steps": ts_rows} if ts_rows else None,
         "after_solve": {"kernel": "k_after_solve<2>", "ms": round(a_ms, 4),
                         "melem_s": round(sz["n_elem"] / (a_ms * 1e-3) / 1e6, 1), "algorithmic_bytes": as_bytes,
                         "achieved_gbs": round(as_bytes / (a_ms * 1e-3) / 1e9, 1)},

@@ -101,7 +101,7 @@ typedef struct {
  * DOF numbering, the 3x3-block pattern (ResizeMat::GetNZ, formulation.h:200-230), the element
  * colouring and the scatter map, and uploads everything to the current HIP device. */
 FEMGPU_API int femgpu_create(const femgpu_mesh *mesh, femgpu_problem **out);
-/* ProblemAncestor deletes its formulations (problem_ancestor.h:298-301). */
+/* ProblemAncestor deletes its formulations (problem_ancestor.h:35-38). */
 FEMGPU_API int femgpu_destroy(femgpu_problem *p);
 
 /* ---- inputs ------------------------------------------------------------------------------- */
@@ -131,7 +131,11 @@ FEMGPU_API int femgpu_assemble(femgpu_problem *p, int assemble_mat, int assemble
  * :1487-1492, containers/MatWithTinyBlocks.h:1101-1151).  The initial guess is the current
  * result vector (vectors[0] after shift(), formulation.h:424-433). */
 FEMGPU_API int femgpu_solve_system(femgpu_problem *p, const femgpu_solve_opts *opts, femgpu_solve_stats *stats);
-/* solve(crit) (formulation_ancestor.h:49; formulation.h:1517-1568): assemble + solve_system. */
+/* solve(crit) (formulation_ancestor.h:49; formulation.h:1517-1568): shift(), then assemble +
+ * solve_system, repeated while a non-linear criterium is set (femgpu_set_nonlinear) and the max
+ * norm of the change since the previous iterate (zero at first) exceeds it; FEMGPU_MAX_ITER after
+ * max_non_linear_iteration + 1 solves (the reference throws SolveException).  With no criterium
+ * (the default) one assemble + solve_system. */
 FEMGPU_API int femgpu_solve(femgpu_problem *p, const femgpu_solve_opts *opts, femgpu_solve_stats *stats);
 FEMGPU_API void femgpu_default_solve_opts(femgpu_solve_opts *opts);
 
@@ -211,6 +215,57 @@ FEMGPU_API int femgpu_skin_faces(int elem_kind, uint32_t n_nodes, uint32_t n_ele
  * Rows of one level share no block.  No reference counterpart: chol_incomp runs row by row. */
 FEMGPU_API int femgpu_ic0_schedule(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, int precond,
                                    uint32_t *order, uint32_t *level_ptr, uint32_t *n_levels);
+
+/* ---- time steps, non-linear loop, preconditioner reuse ------------------------------------- */
+
+/* shift(nb) (formulation.h:424-433): time += time step, and the solution history moves one step
+ * (vectors[1] = vectors[0]); vectors[0] stays, the initial guess of the next solve. */
+FEMGPU_API int femgpu_shift(femgpu_problem *p, int nb);
+/* set_initial_time_step(ts) (formulation.h:2624): time step ts, time = -2 ts.  The time step
+ * starts at 1e40 and time at 0 (formulation.h:128-129). */
+FEMGPU_API int femgpu_set_initial_time_step(femgpu_problem *p, double ts);
+/* time_steps[0] = ts (the public member the reference's callers set between steps). */
+FEMGPU_API int femgpu_set_next_time_step(femgpu_problem *p, double ts);
+/* get_next_time_step / get_time / set_time (formulation.h:2625-2627). */
+FEMGPU_API int femgpu_get_next_time_step(femgpu_problem *p, double *ts);
+FEMGPU_API int femgpu_get_time(femgpu_problem *p, double *time);
+FEMGPU_API int femgpu_set_time(femgpu_problem *p, double time);
+/* vectors[1]: the solution before the last shift (node-major, 3 n_nodes doubles). */
+FEMGPU_API int femgpu_get_previous_result(femgpu_problem *p, double *u, size_t n);
+/* assume_constant_matrix (formulation_ancestor.h:31,244): ProblemAncestor::iteration then
+ * assembles K once, builds the preconditioner once (get_precond) and afterwards assembles only the
+ * right-hand side. */
+FEMGPU_API int femgpu_set_assume_constant_matrix(femgpu_problem *p, int flag);
+/* non_linear_iterative_criterium and max_non_linear_iteration (formulation_ancestor.h:33-35;
+ * defaults 0 and 50). */
+FEMGPU_API int femgpu_set_nonlinear(femgpu_problem *p, double crit, int max_iter);
+/* solve_system calls made by the last femgpu_solve or stepper iteration. */
+FEMGPU_API int femgpu_get_nonlinear_solves(femgpu_problem *p, int *n);
+/* default_iterative_criterium and the other solve options ProblemAncestor::iteration uses for
+ * this formulation (formulation_ancestor.h:32; femgpu_default_solve_opts until set). */
+FEMGPU_API int femgpu_set_default_solve_opts(femgpu_problem *p, const femgpu_solve_opts *opts);
+/* get_precond() (formulation.h:1655,1785-1795): builds opts->precond (FP64 and FP32 copies) for
+ * the current K and keeps it, through later assemblies, for solve_system_using_precond.  The
+ * reference factorises incomplete Cholesky there: FEMGPU_BLOCK_IC0 gives the same factor. */
+FEMGPU_API int femgpu_get_precond(femgpu_problem *p, const femgpu_solve_opts *opts);
+/* solve_system_using_precond(crit) (formulation.h:1656,1796-1805): PCG with get_precond's
+ * preconditioner (opts->precond must be the same kind). */
+FEMGPU_API int femgpu_solve_system_using_precond(femgpu_problem *p, const femgpu_solve_opts *opts,
+                                                 femgpu_solve_stats *stats);
+
+/* ProblemAncestor (problem_ancestor.h:21-140) over n formulations, which stay owned by the
+ * caller.  iteration() (problem_ancestor.h:41-129): the first call shifts and assembles every
+ * formulation and builds the preconditioner of those assuming a constant matrix; later calls
+ * update only the formulations whose time equals the problem's time, shifting them and
+ * reassembling F (and K unless constant).  Constant-matrix formulations then solve with the kept
+ * preconditioner; the others run the non-linear loop (2-norm of the change, reassembling between
+ * solves).  The problem's time becomes the smallest formulation time.  stats and updated receive
+ * one entry per formulation (either may be NULL). */
+typedef struct femgpu_stepper femgpu_stepper;
+FEMGPU_API int femgpu_stepper_create(uint32_t n, femgpu_problem *const *forms, femgpu_stepper **out);
+FEMGPU_API int femgpu_stepper_destroy(femgpu_stepper *s);
+FEMGPU_API int femgpu_stepper_iteration(femgpu_stepper *s, femgpu_solve_stats *stats, int *updated);
+FEMGPU_API int femgpu_stepper_get_time(femgpu_stepper *s, double *time);
 
 /* ---- batched uneven meshes ---------------------------------------------------------------- */
 

@@ -3,7 +3,7 @@
 // elasticity_iso formulation:
 //
 //     allocate_matrices()            formulation_ancestor.h:51   (pattern built at construction)
-//     shift()                        formulation_ancestor.h:57   (result kept as initial guess)
+//     shift(nb)                      formulation_ancestor.h:57; formulation.h:424-433
 //     assemble(mat, vec)             formulation_ancestor.h:63
 //     solve_system(crit) -> bool     formulation_ancestor.h:64
 //     solve(crit) -> bool            formulation_ancestor.h:49
@@ -13,6 +13,13 @@
 //     get_result()                   formulation_ancestor.h:237
 //     call_after_solve()             formulation.h:2026-2050 (elem.epsilon / sigma / tr_epsilon)
 //     default_iterative_criterium    formulation_ancestor.h:244-257
+//     assume_constant_matrix, non_linear_iterative_criterium, max_non_linear_iteration
+//                                    formulation_ancestor.h:31-35,244
+//     get_precond(), solve_system_using_precond(crit)   formulation.h:1655-1656
+//     set_initial_time_step / get_next_time_step / get_time / set_time   formulation.h:2624-2627
+//
+// and ProblemGpu, the ProblemAncestor (problem_ancestor.h:21-140) time-stepping driver:
+//     iteration(), call_after_solve(), time
 //
 // The preconditioner is options().precond: FEMGPU_JACOBI, FEMGPU_BLOCK_JACOBI, FEMGPU_BLOCK_IC0
 // (the reference's block incomplete Cholesky, MatWithTinyBlocks.h:426-709) or
@@ -24,6 +31,7 @@
 #pragma once
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "femgpu.h"
@@ -58,6 +66,14 @@ public:
     // public knobs
     double default_iterative_criterium = 1e-10;
     femgpu_solve_opts &options() { return opts_; }
+    void set_assume_constant_matrix(bool v) { check(femgpu_set_assume_constant_matrix(h_, v), "assume_constant_matrix"); }
+    // non_linear_iterative_criterium (0: linear, one solve) and max_non_linear_iteration
+    void set_non_linear(double crit, int max_iter = 50) { check(femgpu_set_nonlinear(h_, crit, max_iter), "set_nonlinear"); }
+    int nonlinear_solves() const {
+        int n = 0;
+        check(femgpu_get_nonlinear_solves(h_, &n), "nonlinear_solves");
+        return n;
+    }
 
     // material: elastic_modulus (global or per element), poisson_ratio, density * f_vol
     void set_material(double E, double nu, const double body_force[3] = nullptr) {
@@ -78,26 +94,63 @@ public:
     }
 
     void allocate_matrices(bool = true, bool = true) {}
-    void shift() {}
+    // time += time step; the previous solution moves to vectors[1]; vectors[0] stays the next
+    // initial guess
+    void shift(int nb = 1) { check(femgpu_shift(h_, nb), "shift"); }
+    void set_initial_time_step(double ts) { check(femgpu_set_initial_time_step(h_, ts), "set_initial_time_step"); }
+    double get_next_time_step() const {
+        double v = 0;
+        check(femgpu_get_next_time_step(h_, &v), "get_next_time_step");
+        return v;
+    }
+    double get_time() const {
+        double v = 0;
+        check(femgpu_get_time(h_, &v), "get_time");
+        return v;
+    }
+    void set_time(double t) { check(femgpu_set_time(h_, t), "set_time"); }
     void assemble(bool assemble_mat = true, bool assemble_vec = true) {
         check(femgpu_assemble(h_, assemble_mat, assemble_vec), "assemble");
     }
     // crit > 0: stop when max_i |r_i| <= crit (MatWithTinyBlocks.h:1115,1136); crit == 0 uses
     // default_iterative_criterium.  rel_tol > 0 selects ||r|| <= rel_tol ||b|| instead.
     bool solve_system(double crit = 0, double rel_tol = 0) {
-        femgpu_solve_opts o = opts_;
-        if (rel_tol > 0) { o.crit_kind = FEMGPU_REL_L2; o.tol = rel_tol; }
-        else { o.crit_kind = FEMGPU_ABS_MAXNORM; o.tol = crit > 0 ? crit : default_iterative_criterium; }
+        const femgpu_solve_opts o = solve_opts(crit, rel_tol);
         last_status_ = femgpu_solve_system(h_, &o, &stats_);
         if (last_status_ == FEMGPU_BAD_INPUT || last_status_ == FEMGPU_HIP_ERROR || last_status_ == FEMGPU_NOT_READY)
             check(last_status_, "solve_system");
         return last_status_ == FEMGPU_OK;
     }
+    // shift, then assemble + solve_system (repeated by the non-linear loop); throws when the
+    // loop reaches max_non_linear_iteration, as the reference throws SolveException
     bool solve(double crit = 0, double rel_tol = 0) {
-        assemble(true, true);
-        const bool ok = solve_system(crit, rel_tol);
-        if (ok) update_variables();
-        return ok;
+        const femgpu_solve_opts o = solve_opts(crit, rel_tol);
+        last_status_ = femgpu_solve(h_, &o, &stats_);
+        if (last_status_ == FEMGPU_BAD_INPUT || last_status_ == FEMGPU_HIP_ERROR || last_status_ == FEMGPU_NOT_READY ||
+            last_status_ == FEMGPU_NEG_JACOBIAN || (last_status_ == FEMGPU_MAX_ITER && stats_.status == FEMGPU_OK))
+            check(last_status_, "solve");
+        if (last_status_ == FEMGPU_OK) update_variables();
+        return last_status_ == FEMGPU_OK;
+    }
+    // preconditioner of the current K (options().precond), kept through later assemblies
+    void get_precond(double crit = 0, double rel_tol = 0) {
+        const femgpu_solve_opts o = solve_opts(crit, rel_tol);
+        check(femgpu_get_precond(h_, &o), "get_precond");
+    }
+    bool solve_system_using_precond(double crit = 0, double rel_tol = 0) {
+        const femgpu_solve_opts o = solve_opts(crit, rel_tol);
+        last_status_ = femgpu_solve_system_using_precond(h_, &o, &stats_);
+        if (last_status_ == FEMGPU_BAD_INPUT || last_status_ == FEMGPU_HIP_ERROR || last_status_ == FEMGPU_NOT_READY)
+            check(last_status_, "solve_system_using_precond");
+        return last_status_ == FEMGPU_OK;
+    }
+    // options of a solve: crit > 0 max-norm test, else default_iterative_criterium; rel_tol > 0
+    // the relative L2 test
+    femgpu_solve_opts solve_opts(double crit = 0, double rel_tol = 0) const {
+        femgpu_solve_opts o = opts_;
+        if (rel_tol > 0) { o.crit_kind = FEMGPU_REL_L2; o.tol = rel_tol; }
+        else { o.crit_kind = FEMGPU_ABS_MAXNORM; o.tol = crit > 0 ? crit : default_iterative_criterium; }
+        return o;
     }
     // node.dep[d] = vectors[0][3 i + d] (formulation.h:2571-2576)
     const std::vector<double> &update_variables() {
@@ -128,6 +181,7 @@ public:
     }
     int last_status() const { return last_status_; }
     const femgpu_solve_stats &last_stats() const { return stats_; }
+    void set_last(int status, const femgpu_solve_stats &st) { last_status_ = status; stats_ = st; }
     femgpu_problem *handle() const { return h_; }
     uint32_t nb_nodes() const { return n_nodes_; }
     uint32_t nb_elems() const { return n_elem_; }
@@ -139,6 +193,53 @@ private:
     femgpu_solve_stats stats_{};
     int last_status_ = FEMGPU_OK;
     std::vector<double> dep_;
+};
+
+// ProblemAncestor (problem_ancestor.h:21-140) over formulations the caller owns; each solve uses
+// that formulation's solve_opts() (rel_tol > 0 selects the relative L2 test for all).
+class ProblemGpu {
+public:
+    explicit ProblemGpu(std::vector<FormulationGpu *> forms, double rel_tol = 0) : forms_(std::move(forms)), rel_tol_(rel_tol) {
+        std::vector<femgpu_problem *> h;
+        for (FormulationGpu *f : forms_) h.push_back(f->handle());
+        check(femgpu_stepper_create(uint32_t(h.size()), h.data(), &s_), "stepper_create");
+    }
+    ProblemGpu(const ProblemGpu &) = delete;
+    ProblemGpu &operator=(const ProblemGpu &) = delete;
+    ~ProblemGpu() { femgpu_stepper_destroy(s_); }
+
+    // problem_ancestor.h:41-129; returns which formulations were updated
+    std::vector<int> iteration() {
+        const size_t n = forms_.size();
+        for (FormulationGpu *f : forms_) {
+            const femgpu_solve_opts o = f->solve_opts(0, rel_tol_);
+            check(femgpu_set_default_solve_opts(f->handle(), &o), "set_default_solve_opts");
+        }
+        std::vector<femgpu_solve_stats> st(n);
+        std::vector<int> upd(n, 0);
+        const int rc = femgpu_stepper_iteration(s_, st.data(), upd.data());
+        for (size_t i = 0; i < n; ++i)
+            if (upd[i]) forms_[i]->set_last(st[i].status, st[i]);
+        check(rc, "iteration");
+        for (size_t i = 0; i < n; ++i)
+            if (upd[i]) forms_[i]->update_variables();
+        return upd;
+    }
+    double time() const {
+        double t = 0;
+        check(femgpu_stepper_get_time(s_, &t), "stepper_get_time");
+        return t;
+    }
+    std::vector<FormulationGpu::AfterSolve> call_after_solve() const {
+        std::vector<FormulationGpu::AfterSolve> r;
+        for (FormulationGpu *f : forms_) r.push_back(f->call_after_solve());
+        return r;
+    }
+
+private:
+    std::vector<FormulationGpu *> forms_;
+    double rel_tol_ = 0;
+    femgpu_stepper *s_ = nullptr;
 };
 
 }  // namespace femgpu

@@ -11,8 +11,9 @@ from ._capi import (ABS_MAXNORM, BLOCK_IC0, BLOCK_IC0_COLOURED, BLOCK_JACOBI, FP
 from .batch import Batch  # noqa: F401
 from .formulation import BlockSystem, Formulation  # noqa: F401
 from .mesh import clamp_dofs, faces_on_plane, make_rect, nodes_on_plane, skin_faces  # noqa: F401
+from .problem import Problem  # noqa: F401
 
-__all__ = ["Formulation", "Batch", "BlockSystem", "make_rect", "skin_faces", "faces_on_plane", "nodes_on_plane",
+__all__ = ["Formulation", "Batch", "BlockSystem", "Problem", "make_rect", "skin_faces", "faces_on_plane", "nodes_on_plane",
            "clamp_dofs", "FemGpuError", "load", "TETRA", "TETRA10", "HEXA", "FP64", "MIXED", "JACOBI",
            "BLOCK_JACOBI", "BLOCK_IC0", "BLOCK_IC0_COLOURED", "REL_L2", "ABS_MAXNORM", "ROW_ORDER_AUTO",
            "ROW_ORDER_NATURAL", "ROW_ORDER_BY_LENGTH", "row_order"]

@@ -110,6 +110,23 @@ _SIGS = {
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_set_row_order": (C.c_int, [C.c_int]),
+    "femgpu_shift": (C.c_int, [_P, C.c_int]),
+    "femgpu_set_initial_time_step": (C.c_int, [_P, C.c_double]),
+    "femgpu_set_next_time_step": (C.c_int, [_P, C.c_double]),
+    "femgpu_get_next_time_step": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "femgpu_get_time": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "femgpu_set_time": (C.c_int, [_P, C.c_double]),
+    "femgpu_get_previous_result": (C.c_int, [_P, _P, C.c_size_t]),
+    "femgpu_set_assume_constant_matrix": (C.c_int, [_P, C.c_int]),
+    "femgpu_set_nonlinear": (C.c_int, [_P, C.c_double, C.c_int]),
+    "femgpu_get_nonlinear_solves": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "femgpu_set_default_solve_opts": (C.c_int, [_P, C.POINTER(SolveOpts)]),
+    "femgpu_get_precond": (C.c_int, [_P, C.POINTER(SolveOpts)]),
+    "femgpu_solve_system_using_precond": (C.c_int, [_P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "femgpu_stepper_create": (C.c_int, [C.c_uint32, _P, C.POINTER(_P)]),
+    "femgpu_stepper_destroy": (C.c_int, [_P]),
+    "femgpu_stepper_iteration": (C.c_int, [_P, _P, _P]),
+    "femgpu_stepper_get_time": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_row_order": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "femgpu_sell_order": (C.c_int, [C.c_uint32, _P, C.c_int, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }

@@ -391,7 +391,12 @@ uint32_t max_assembly_rows(const Problem &P) {
 
 hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
     hipError_t e = hipSuccess;
-    if (P.asm_rows && P.rg.R == P.asm_rows) {
+    bool body_force = false;
+    for (const SubMaterial &m : P.mat_h) body_force |= m.bf0 != 0.0 || m.bf1 != 0.0 || m.bf2 != 0.0;
+    if (!do_mat && !body_force) {
+        // right-hand side only, no body force: every element contribution is exactly zero
+        if (do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
+    } else if (P.asm_rows && P.rg.R == P.asm_rows) {
         if (P.kind == TETRA) assemble_rows_kind<TETRA>(P, do_mat, do_vec);
         else if (P.kind == TETRA10) assemble_rows_kind<TETRA10>(P, do_mat, do_vec);
         else assemble_rows_kind<HEXA>(P, do_mat, do_vec);

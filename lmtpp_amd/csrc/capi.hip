@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <new>
 
 #include "../../include/femgpu.h"
@@ -24,6 +25,27 @@ struct femgpu_problem {
     std::vector<std::vector<uint32_t>> sub_faces;  // per sub-problem, local node indices
     std::vector<std::vector<double>> sub_trac;
     bool faces_dirty = false;
+    // time stepping (formulation.h:124-130, 424-433, 2624-2627)
+    double time = 0.0, time_step = 1e40;
+    double *d_prev = nullptr;     // vectors[1]: the solution before the last shift
+    bool has_prev = false;
+    // FormulationAncestor knobs (formulation_ancestor.h:31-35, 244)
+    bool assume_constant_matrix = false;
+    double nl_crit = 0.0;
+    int max_nl = 50;
+    int last_nl_solves = 0;       // solve_system calls of the last solve() / iteration()
+    double *d_old = nullptr;      // previous iterate of the non-linear loop
+    femgpu_solve_opts dopts{};    // default_iterative_criterium and the rest of the solve options
+    // get_precond(): preconditioner kept for solve_system_using_precond, even across assemblies
+    bool precond_pinned = false, precond_stale = false;
+    int pinned_kind = -1;
+    femgpu_problem() { femgpu_default_solve_opts(&dopts); }
+};
+// ProblemAncestor (problem_ancestor.h:21-140) over formulations the caller owns.
+struct femgpu_stepper {
+    std::vector<femgpu_problem *> forms;
+    double time = 0.0;
+    bool initialised = false;
 };
 struct femgpu_batch {
     femgpu_problem H;
@@ -67,7 +89,7 @@ void free_problem(Problem &P) {
     dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
     dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
     dfree(P.d_cdof); dfree(P.d_cptr); dfree(P.d_csub); dfree(P.d_cval); dfree(P.d_cpen);
-    dfree(P.d_r); dfree(P.d_p); dfree(P.d_q); dfree(P.d_Minv); dfree(P.d_aux);
+    dfree(P.d_r); dfree(P.d_p); dfree(P.d_q); dfree(P.d_Minv); dfree(P.d_aux); dfree(P.d_red);
     dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32);
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
@@ -385,8 +407,12 @@ int assemble(femgpu_problem &H, int do_mat, int do_vec) {
     if (do_mat) {
         P.assembled_mat = true;
         P.k32_valid = false;
-        P.minv_kind = -1;
-        P.ic0.valid = P.ic0.valid32 = false;
+        if (H.precond_pinned) {
+            H.precond_stale = true;  // get_precond's factors stay for solve_system_using_precond
+        } else {
+            P.minv_kind = -1;
+            P.ic0.valid = P.ic0.valid32 = false;
+        }
         P.sym.valid[0] = P.sym.valid[1] = false;
         for (uint32_t s = 0; s < P.n_sub; ++s) P.max_diag[s] = P.h_ps[s].max_diag;
     }
@@ -441,6 +467,42 @@ int copy_sub(femgpu_problem &H, uint32_t s, const double *src, double *dst_host,
     HIPCHK(hipStreamSynchronize(P.stream));
     HIPCHK(hipMemcpy(dst_host, src + 3 * (size_t)P.node_off[s], n * 8, hipMemcpyDeviceToHost));
     return FEMGPU_OK;
+}
+
+// Formulation::solve's loop (formulation.h:1517-1568) and the non-constant branch of
+// ProblemAncestor::iteration (problem_ancestor.h:85-106): solve_system, then stop when no
+// non-linear criterium is set or when the change since the previous iterate (zero at first) is at
+// most |criterium| -- max norm for solve(), 2-norm for iteration() -- else reassemble and solve
+// again, at most max_non_linear_iteration + 1 times.  solve() assembles before every solve;
+// iteration() has assembled already and reassembles after.
+int nonlinear_solve(femgpu_problem &H, const femgpu_solve_opts *o, femgpu_solve_stats *st, bool norm2,
+                    bool assemble_first) {
+    Problem &P = H.P;
+    const size_t n = 3 * (size_t)P.n_nodes;
+    H.last_nl_solves = 0;
+    bool have_old = false;
+    for (int nb = 0;; ) {
+        int rc = FEMGPU_OK;
+        if (assemble_first) rc = assemble(H, 1, 1);
+        if (!rc) rc = femgpu_solve_system(&H, o, st);
+        if (rc) return rc;
+        ++H.last_nl_solves;
+        if (H.nl_crit == 0.0) return FEMGPU_OK;
+        if (!have_old) {
+            if (!H.d_old) HIPCHK(dalloc(&H.d_old, n));
+            HIPCHK(launch_fill(H.d_old, n, 0.0, P.stream));
+            have_old = true;
+        }
+        double d2 = 0, dinf = 0;
+        if (diff_norm(P, H.d_old, P.d_u, n, &d2, &dinf)) return fail(FEMGPU_HIP_ERROR, "solve: difference norm");
+        if ((norm2 ? d2 : dinf) <= std::fabs(H.nl_crit)) return FEMGPU_OK;
+        if (nb++ >= H.max_nl) return fail(FEMGPU_MAX_ITER, "solve: max_non_linear_iteration reached");
+        HIPCHK(hipMemcpyAsync(H.d_old, P.d_u, n * 8, hipMemcpyDeviceToDevice, P.stream));
+        if (!assemble_first) {
+            rc = assemble(H, H.assume_constant_matrix ? 0 : 1, 1);
+            if (rc) return rc;
+        }
+    }
 }
 
 }  // namespace
@@ -577,6 +639,8 @@ int femgpu_create_from_bsr(uint32_t n_brows, const uint32_t *row_ptr, const uint
 int femgpu_destroy(femgpu_problem *h) {
     if (!h) return FEMGPU_OK;
     free_problem(h->P);
+    dfree(h->d_prev);
+    dfree(h->d_old);
     delete h;
     return FEMGPU_OK;
 }
@@ -623,6 +687,11 @@ int femgpu_assemble(femgpu_problem *h, int do_mat, int do_vec) {
 
 int femgpu_solve_system(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_stats *st) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "solve_system: null");
+    if (h->precond_stale) {  // K changed since get_precond: solve_system builds its own
+        h->P.minv_kind = -1;
+        h->P.ic0.valid = h->P.ic0.valid32 = false;
+        h->precond_pinned = h->precond_stale = false;
+    }
     femgpu_solve_stats tmp{};
     const int rc = solve_system(*h, o, st ? st : &tmp, nullptr, nullptr);
     if (st) st->status = rc;
@@ -630,8 +699,10 @@ int femgpu_solve_system(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_so
 }
 
 int femgpu_solve(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_stats *st) {
-    int rc = femgpu_assemble(h, 1, 1);
-    return rc ? rc : femgpu_solve_system(h, o, st);
+    if (!h) return fail(FEMGPU_BAD_INPUT, "solve: null");
+    int rc = femgpu_shift(h, 1);
+    if (rc) return rc;
+    return nonlinear_solve(*h, o, st, false, true);
 }
 
 int femgpu_get_result(femgpu_problem *h, double *u, size_t n) {
@@ -1015,6 +1086,180 @@ int femgpu_sell_order(uint32_t n_brows, const uint32_t *row_ptr, int order, uint
         for (uint32_t t = 0; t < n_brows; ++t) perm[t] = pat.row_of(t);
     if (n_natural) *n_natural = sell_size(pat, {});
     if (n_positions) *n_positions = sell_size(pat, pat.perm);
+    return FEMGPU_OK;
+}
+
+int femgpu_shift(femgpu_problem *h, int nb) {
+    if (!h || nb < 0) return fail(FEMGPU_BAD_INPUT, "shift: bad input");
+    Problem &P = h->P;
+    const size_t n = 3 * (size_t)P.n_nodes;
+    while (nb--) {
+        h->time += h->time_step;
+        if (!h->d_prev) HIPCHK(dalloc(&h->d_prev, n));
+        HIPCHK(hipMemcpyAsync(h->d_prev, P.d_u, n * 8, hipMemcpyDeviceToDevice, P.stream));
+        h->has_prev = true;
+    }
+    HIPCHK(hipStreamSynchronize(P.stream));
+    return FEMGPU_OK;
+}
+
+int femgpu_set_initial_time_step(femgpu_problem *h, double ts) {
+    if (!h || !std::isfinite(ts)) return fail(FEMGPU_BAD_INPUT, "set_initial_time_step: bad input");
+    h->time_step = ts;
+    h->time = -2.0 * ts;
+    return FEMGPU_OK;
+}
+
+int femgpu_set_next_time_step(femgpu_problem *h, double ts) {
+    if (!h || !std::isfinite(ts)) return fail(FEMGPU_BAD_INPUT, "set_next_time_step: bad input");
+    h->time_step = ts;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_next_time_step(femgpu_problem *h, double *ts) {
+    if (!h || !ts) return fail(FEMGPU_BAD_INPUT, "get_next_time_step: null");
+    *ts = h->time_step;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_time(femgpu_problem *h, double *t) {
+    if (!h || !t) return fail(FEMGPU_BAD_INPUT, "get_time: null");
+    *t = h->time;
+    return FEMGPU_OK;
+}
+
+int femgpu_set_time(femgpu_problem *h, double t) {
+    if (!h || !std::isfinite(t)) return fail(FEMGPU_BAD_INPUT, "set_time: bad input");
+    h->time = t;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_previous_result(femgpu_problem *h, double *u, size_t n) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_previous_result: null");
+    if (!h->has_prev) return fail(FEMGPU_NOT_READY, "get_previous_result: no shift yet");
+    return copy_sub(*h, 0, h->d_prev, u, n);
+}
+
+int femgpu_set_assume_constant_matrix(femgpu_problem *h, int flag) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "set_assume_constant_matrix: null");
+    h->assume_constant_matrix = flag != 0;
+    return FEMGPU_OK;
+}
+
+int femgpu_set_nonlinear(femgpu_problem *h, double crit, int max_iter) {
+    if (!h || !std::isfinite(crit) || max_iter < 0) return fail(FEMGPU_BAD_INPUT, "set_nonlinear: bad input");
+    h->nl_crit = crit;
+    h->max_nl = max_iter;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_nonlinear_solves(femgpu_problem *h, int *n) {
+    if (!h || !n) return fail(FEMGPU_BAD_INPUT, "get_nonlinear_solves: null");
+    *n = h->last_nl_solves;
+    return FEMGPU_OK;
+}
+
+int femgpu_set_default_solve_opts(femgpu_problem *h, const femgpu_solve_opts *o) {
+    if (!h || !o) return fail(FEMGPU_BAD_INPUT, "set_default_solve_opts: null");
+    const int rc = check_opts(o);
+    if (rc) return rc;
+    h->dopts = *o;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_precond(femgpu_problem *h, const femgpu_solve_opts *o) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_precond: null");
+    int rc = check_opts(o);
+    if (rc) return rc;
+    Problem &P = h->P;
+    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "get_precond: assemble first");
+    const int kind = o ? o->precond : h->dopts.precond;
+    P.minv_kind = -1;
+    P.ic0.valid = P.ic0.valid32 = false;
+    h->precond_pinned = h->precond_stale = false;
+    rc = precond_build(P, kind);
+    if (rc == 3) return fail(FEMGPU_NOT_SPD, "get_precond: non-positive pivot");
+    if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "get_precond failed");
+    h->precond_pinned = true;
+    h->pinned_kind = kind;
+    return FEMGPU_OK;
+}
+
+int femgpu_solve_system_using_precond(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_stats *st) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "solve_system_using_precond: null");
+    if (!h->precond_pinned) return fail(FEMGPU_NOT_READY, "solve_system_using_precond: call get_precond first");
+    if ((o ? o->precond : h->dopts.precond) != h->pinned_kind)
+        return fail(FEMGPU_BAD_INPUT, "solve_system_using_precond: preconditioner differs from get_precond's");
+    femgpu_solve_stats tmp{};
+    const int rc = solve_system(*h, o ? o : &h->dopts, st ? st : &tmp, nullptr, nullptr);
+    if (st) st->status = rc;
+    return rc;
+}
+
+int femgpu_stepper_create(uint32_t n, femgpu_problem *const *forms, femgpu_stepper **out) {
+    if (!out || (n && !forms)) return fail(FEMGPU_BAD_INPUT, "stepper_create: null");
+    for (uint32_t i = 0; i < n; ++i)
+        if (!forms[i] || forms[i]->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "stepper_create: formulation " + std::to_string(i));
+    auto *s = new (std::nothrow) femgpu_stepper();
+    if (!s) return fail(FEMGPU_BAD_INPUT, "stepper_create: out of host memory");
+    s->forms.assign(forms, forms + n);
+    *out = s;
+    return FEMGPU_OK;
+}
+
+int femgpu_stepper_destroy(femgpu_stepper *s) {
+    delete s;
+    return FEMGPU_OK;
+}
+
+int femgpu_stepper_get_time(femgpu_stepper *s, double *t) {
+    if (!s || !t) return fail(FEMGPU_BAD_INPUT, "stepper_get_time: null");
+    *t = s->time;
+    return FEMGPU_OK;
+}
+
+int femgpu_stepper_iteration(femgpu_stepper *s, femgpu_solve_stats *stats, int *updated) {
+    if (!s) return fail(FEMGPU_BAD_INPUT, "stepper_iteration: null");
+    const size_t nf = s->forms.size();
+    std::vector<int> upd(nf, 1);
+    int rc = FEMGPU_OK;
+    if (!s->initialised) {
+        for (size_t i = 0; i < nf && !rc; ++i) {
+            femgpu_problem *f = s->forms[i];
+            rc = femgpu_shift(f, 1);
+            if (!rc) rc = femgpu_assemble(f, 1, 1);
+            if (!rc && f->assume_constant_matrix) rc = femgpu_get_precond(f, &f->dopts);
+        }
+        if (rc) return rc;
+        s->initialised = true;
+    } else {
+        for (size_t i = 0; i < nf; ++i) upd[i] = s->forms[i]->time == s->time;
+        for (size_t i = 0; i < nf && !rc; ++i) {
+            if (!upd[i]) continue;
+            femgpu_problem *f = s->forms[i];
+            rc = femgpu_shift(f, 1);
+            if (!rc) rc = femgpu_assemble(f, f->assume_constant_matrix ? 0 : 1, 1);
+        }
+        if (rc) return rc;
+    }
+    for (size_t i = 0; i < nf; ++i) {
+        femgpu_problem *f = s->forms[i];
+        femgpu_solve_stats st{};
+        if (upd[i]) {
+            if (f->assume_constant_matrix) {
+                rc = femgpu_solve_system_using_precond(f, &f->dopts, &st);
+                f->last_nl_solves = 1;
+            } else {
+                rc = nonlinear_solve(*f, &f->dopts, &st, true, false);
+            }
+        }
+        if (stats) stats[i] = st;
+        if (updated) updated[i] = upd[i];
+        if (rc) return rc;
+    }
+    double t = std::numeric_limits<double>::max();
+    for (femgpu_problem *f : s->forms) t = std::min(t, f->time);
+    s->time = t;
     return FEMGPU_OK;
 }
 

@@ -128,6 +128,7 @@ struct Problem {
     // solver work vectors
     double *d_r = nullptr, *d_p = nullptr, *d_q = nullptr, *d_Minv = nullptr;
     double *d_aux = nullptr;  // adjoint state / scratch vector
+    double *d_red = nullptr;  // diff_norm partials
     float *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_x32 = nullptr, *d_Minv32 = nullptr;
     double *d_partials = nullptr;
     ProbScalars *d_ps = nullptr, *h_ps = nullptr;
@@ -174,6 +175,11 @@ struct SolveOut { int status, iters, refreshes; double rel_res, max_abs_res, ms;
 // Solves every sub-problem; out[s] per sub-problem; returns the first non-zero status.
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out);
 double time_spmv(Problem &P, int precision, int reps);
+// Preconditioner precond (femgpu_precond) of the current K with its FP32 copy, as get_precond
+// does before the solves that reuse it; returns 0, 1 (bad input), 3 (non-positive pivot) or 5.
+int precond_build(Problem &P, int precond);
+// ||a - b||_2 and max |a - b| over n device values (deterministic); returns 0 or 5.
+int diff_norm(Problem &P, const double *a, const double *b, size_t n, double *norm2, double *norm_inf);
 // Picks format and unroll per precision by timing every variant on this matrix; returns 0 or 5.
 int autotune_spmv(Problem &P);
 // Symmetric storage and its values for precision (0 FP64, 1 FP32) from the current K.

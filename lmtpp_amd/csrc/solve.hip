@@ -798,6 +798,79 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     return first_bad;
 }
 
+int precond_build(Problem &P, int precond) {
+    if (precond == 2 || precond == 3) return ic0_ensure(P, precond == 2 ? IC0_NATURAL : IC0_COLOURED, true);
+    if (precond != 0 && precond != 1) return 1;
+    if (!P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
+        return 5;
+    std::memset(P.h_ps, 0, sizeof(ProbScalars) * P.n_sub);
+    if (write_scalars(P)) return 5;
+    const WgTables wt = tables(P);
+    const dim3 g(P.n_wg), blk(256);
+    (void)hipEventRecord(P.ev[2], P.stream);
+    if (precond == 1)
+        hipLaunchKernelGGL(k_build_minv<PRE_BLOCK_JACOBI>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
+    else
+        hipLaunchKernelGGL(k_build_minv<PRE_JACOBI>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
+    (void)hipEventRecord(P.ev[3], P.stream);
+    if (hipGetLastError() != hipSuccess || read_scalars(P)) return 5;
+    for (uint32_t s = 0; s < P.n_sub; ++s)
+        if (P.h_ps[s].status) {
+            P.minv_kind = -1;
+            return P.h_ps[s].status;
+        }
+    P.minv_kind = precond == 1 ? PRE_BLOCK_JACOBI : PRE_JACOBI;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, P.ev[2], P.ev[3]) == hipSuccess) P.last_precond_ms = ms;
+    return 0;
+}
+
+// sum (a - b)^2 and max |a - b| over n values: one partial pair per workgroup (fixed grid, fixed
+// tree order), combined on the host in workgroup order -- deterministic.
+constexpr int DIFF_WGS = 512;
+__global__ __launch_bounds__(256) void k_diff_norm(const double *__restrict__ a, const double *__restrict__ b, size_t n,
+                                                   double *__restrict__ out) {
+    __shared__ double s2[256], sm[256];
+    double acc = 0, mx = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double d = a[i] - b[i];
+        acc += d * d;
+        mx = fmax(mx, fabs(d));
+    }
+    s2[threadIdx.x] = acc;
+    sm[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = s2[0];
+        out[2 * blockIdx.x + 1] = sm[0];
+    }
+}
+
+int diff_norm(Problem &P, const double *a, const double *b, size_t n, double *norm2, double *norm_inf) {
+    if (!P.d_red && hipMalloc(&P.d_red, sizeof(double) * 2 * DIFF_WGS) != hipSuccess) return 5;
+    hipLaunchKernelGGL(k_diff_norm, dim3(DIFF_WGS), dim3(256), 0, P.stream, a, b, n, P.d_red);
+    double h[2 * DIFF_WGS];
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h, P.d_red, sizeof(h), hipMemcpyDeviceToHost, P.stream) != hipSuccess ||
+        hipStreamSynchronize(P.stream) != hipSuccess)
+        return 5;
+    double s = 0, m = 0;
+    for (int w = 0; w < DIFF_WGS; ++w) {
+        s += h[2 * w];
+        m = std::max(m, h[2 * w + 1]);
+    }
+    if (norm2) *norm2 = std::sqrt(s);
+    if (norm_inf) *norm_inf = m;
+    return 0;
+}
+
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out) {
     for (uint32_t s = 0; s < P.n_sub; ++s) out[s] = SolveOut{};
     int rc = spmv_ensure(P, 0);

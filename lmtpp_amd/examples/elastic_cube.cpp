@@ -1,10 +1,12 @@
 // Elastic cube through the C++ facade: the counterpart of the reference's example
 // (include/mesh/exemple_probleme_elastique.txt:59-213) on the GPU path.
-//   elastic_cube <tetra|tetra10|hexa> <points per side> <out.bin> [jacobi|ic0]
+//   elastic_cube <tetra|tetra10|hexa> <points per side> <out.bin> [jacobi|ic0|ramp]
 // Clamps x = 0 by penalty (1e5, as the reference example), applies the traction (0,0,-1) on
 // x = 1, E = 210000, nu = 0.3, solves with the reference stop test max|r| <= 1e-10 max|F| (Jacobi
 // PCG, or the reference's block IC(0) PCG with "ic0"), writes the displacements (3 doubles per
-// node) to out.bin and prints the largest von Mises stress of call_after_solve().
+// node) to out.bin and prints the largest von Mises stress of call_after_solve().  "ramp" then
+// runs three time steps of ProblemGpu with the traction scaled by the step number, the matrix
+// assumed constant (K and its block IC(0) factor built once) and each step warm-started.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -15,7 +17,7 @@
 
 int main(int argc, char **argv) {
     if (argc != 4 && argc != 5) {
-        std::fprintf(stderr, "usage: %s <tetra|tetra10|hexa> <points per side> <out.bin> [jacobi|ic0]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <tetra|tetra10|hexa> <points per side> <out.bin> [jacobi|ic0|ramp]\n", argv[0]);
         return 2;
     }
     const femgpu_elem_kind kind = !std::strcmp(argv[1], "hexa") ? FEMGPU_HEXA
@@ -36,7 +38,8 @@ int main(int argc, char **argv) {
     femgpu::check(femgpu_skin_faces(kind, nn, ne, conn.data(), skin.data(), &nf), "skin_faces");
 
     femgpu::FormulationGpu f(kind, coords, conn);
-    if (argc == 5 && !std::strcmp(argv[4], "ic0")) f.options().precond = FEMGPU_BLOCK_IC0;
+    const bool ramp = argc == 5 && !std::strcmp(argv[4], "ramp");
+    if (argc == 5 && (ramp || !std::strcmp(argv[4], "ic0"))) f.options().precond = FEMGPU_BLOCK_IC0;
     f.set_material(210000.0, 0.3);
     std::vector<uint32_t> faces;
     std::vector<double> trac;
@@ -74,5 +77,19 @@ int main(int argc, char **argv) {
     }
     std::printf("%u nodes %u elems: %d iterations, max|r| %.3e, tip uz %.9e, max von Mises %.6e\n", nn, ne,
                 f.last_stats().iters, f.last_stats().max_abs_res, u[3 * (size_t)(nn - 1) + 2], vm);
+    if (!ramp) return 0;
+    f.set_assume_constant_matrix(true);
+    f.set_initial_time_step(1.0);
+    f.default_iterative_criterium = 1e-12 * fmax;
+    femgpu::ProblemGpu problem({&f});
+    for (int k = 1; k <= 3; ++k) {
+        std::vector<double> tk(trac);
+        for (double &v : tk) v *= k;
+        f.set_traction(faces, tk);
+        problem.iteration();
+        const std::vector<double> &uk = f.update_variables();
+        std::printf("step %d time %g: %d iterations, tip uz %.9e\n", k, problem.time(), f.last_stats().iters,
+                    uk[3 * (size_t)(nn - 1) + 2]);
+    }
     return 0;
 }

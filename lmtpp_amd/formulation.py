@@ -46,6 +46,7 @@ class Formulation:
         self.last_stats = None
         self.last_status = A.OK
         self.result = np.zeros(3 * self.n_nodes)
+        self._const, self._nl_crit, self._max_nl = False, 0.0, 50   # the library's defaults
 
     # -- lifecycle -----------------------------------------------------------------------------
     def close(self):
@@ -128,11 +129,108 @@ class Formulation:
         return rc == A.OK
 
     def solve(self, crit=0.0, rel_tol=None):
-        self.assemble(True, True)
-        ok = self.solve_system(crit, rel_tol)
-        if ok:
+        """solve(crit) (formulation.h:1517-1568): shift, then assemble + solve_system, repeated
+        while non_linear_iterative_criterium is set and the iterate still moves; True on success.
+        A loop that reaches max_non_linear_iteration raises (the reference throws)."""
+        o = self._opts(crit, rel_tol)
+        st = A.SolveStats()
+        rc = self._lib.femgpu_solve(self._h, C.byref(o), C.byref(st))
+        self.last_stats, self.last_status = st, rc
+        if rc in (A.BAD_INPUT, A.HIP_ERROR, A.NOT_READY, A.NEG_JACOBIAN):
+            A.check(rc, "solve")
+        if rc == A.MAX_ITER and st.status == A.OK:   # the last solve converged: the non-linear loop ran out
+            A.check(rc, "solve")
+        if rc == A.OK:
             self.update_variables()
-        return ok
+        return rc == A.OK
+
+    # -- time steps, non-linear loop, preconditioner reuse --------------------------------------
+    @property
+    def assume_constant_matrix(self):
+        """formulation_ancestor.h:244: Problem.iteration assembles K and its preconditioner once."""
+        return self._const
+
+    @assume_constant_matrix.setter
+    def assume_constant_matrix(self, v):
+        A.check(self._lib.femgpu_set_assume_constant_matrix(self._h, int(bool(v))), "assume_constant_matrix")
+        self._const = bool(v)
+
+    @property
+    def non_linear_iterative_criterium(self):
+        return self._nl_crit
+
+    @non_linear_iterative_criterium.setter
+    def non_linear_iterative_criterium(self, v):
+        A.check(self._lib.femgpu_set_nonlinear(self._h, float(v), int(self._max_nl)), "set_nonlinear")
+        self._nl_crit = float(v)
+
+    @property
+    def max_non_linear_iteration(self):
+        return self._max_nl
+
+    @max_non_linear_iteration.setter
+    def max_non_linear_iteration(self, v):
+        A.check(self._lib.femgpu_set_nonlinear(self._h, float(self._nl_crit), int(v)), "set_nonlinear")
+        self._max_nl = int(v)
+
+    def nonlinear_solves(self):
+        """solve_system calls made by the last solve() or Problem.iteration()."""
+        n = C.c_int()
+        A.check(self._lib.femgpu_get_nonlinear_solves(self._h, C.byref(n)), "get_nonlinear_solves")
+        return n.value
+
+    def shift(self, nb=1):
+        """shift(nb) (formulation.h:424-433): time += time step; the previous solution moves to
+        vectors[1]; vectors[0] stays the next initial guess."""
+        A.check(self._lib.femgpu_shift(self._h, int(nb)), "shift")
+
+    def set_initial_time_step(self, ts):
+        A.check(self._lib.femgpu_set_initial_time_step(self._h, float(ts)), "set_initial_time_step")
+
+    def set_next_time_step(self, ts):
+        A.check(self._lib.femgpu_set_next_time_step(self._h, float(ts)), "set_next_time_step")
+
+    def get_next_time_step(self):
+        v = C.c_double()
+        A.check(self._lib.femgpu_get_next_time_step(self._h, C.byref(v)), "get_next_time_step")
+        return v.value
+
+    def get_time(self):
+        v = C.c_double()
+        A.check(self._lib.femgpu_get_time(self._h, C.byref(v)), "get_time")
+        return v.value
+
+    def set_time(self, t):
+        A.check(self._lib.femgpu_set_time(self._h, float(t)), "set_time")
+
+    def get_previous_result(self):
+        """vectors[1]: the solution before the last shift."""
+        u = np.zeros(3 * self.n_nodes)
+        A.check(self._lib.femgpu_get_previous_result(self._h, _ptr(u), u.size), "get_previous_result")
+        return u
+
+    def get_precond(self, crit=0.0, rel_tol=None):
+        """get_precond() (formulation.h:1655): build self.precond for the current K and keep it for
+        solve_system_using_precond, through later assemblies."""
+        o = self._opts(crit, rel_tol)
+        A.check(self._lib.femgpu_get_precond(self._h, C.byref(o)), "get_precond")
+
+    def solve_system_using_precond(self, crit=0.0, rel_tol=None):
+        """solve_system_using_precond(crit) (formulation.h:1656): PCG with get_precond's
+        preconditioner."""
+        o = self._opts(crit, rel_tol)
+        st = A.SolveStats()
+        rc = self._lib.femgpu_solve_system_using_precond(self._h, C.byref(o), C.byref(st))
+        self.last_stats, self.last_status = st, rc
+        if rc in (A.BAD_INPUT, A.HIP_ERROR, A.NOT_READY):
+            A.check(rc, "solve_system_using_precond")
+        return rc == A.OK
+
+    def _push_defaults(self, rel_tol=None):
+        """Hand default_iterative_criterium, precision, precond and max_iter to the library for
+        Problem.iteration."""
+        o = self._opts(0.0, rel_tol)
+        A.check(self._lib.femgpu_set_default_solve_opts(self._h, C.byref(o)), "set_default_solve_opts")
 
     # -- results -------------------------------------------------------------------------------
     def get_result(self):
@@ -143,10 +241,6 @@ class Formulation:
     def set_result(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         A.check(self._lib.femgpu_set_result(self._h, _ptr(u), u.size), "set_result")
-
-    def shift(self):
-        """vectors[0] (the previous solution) becomes the initial guess; it already is on the GPU."""
-        return None
 
     def update_variables(self):
         """node.dep[d] = vectors[0][3 i + d] (formulation.h:2571-2576)."""

@@ -262,6 +262,16 @@ def test_cpp_facade_example(tmp_path):
     assert " %d iterations" % int(g["iters"]) in r.stdout, r.stdout
     u = np.fromfile(out)
     assert np.linalg.norm(u - g["x"]) <= 1e-10 * np.linalg.norm(g["x"])
+    # ProblemGpu time steps: traction x k, constant K and factor, warm starts
+    r = subprocess.run([exe, "tetra", "11", out, "ramp"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    steps = [ln for ln in r.stdout.splitlines() if ln.startswith("step ")]
+    assert len(steps) == 3, r.stdout
+    tip = [float(ln.split("tip uz ")[1]) for ln in steps]
+    times = [float(ln.split("time ")[1].split(":")[0]) for ln in steps]
+    assert times == [-1.0, 0.0, 1.0]
+    for k in (2, 3):
+        assert abs(tip[k - 1] - k * tip[0]) <= 1e-9 * abs(k * tip[0])
 
 
 def _batch_inputs(seed=4, n=3):

@@ -216,6 +216,29 @@ FEMGPU_API int femgpu_skin_faces(int elem_kind, uint32_t n_nodes, uint32_t n_ele
 FEMGPU_API int femgpu_ic0_schedule(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, int precond,
                                    uint32_t *order, uint32_t *level_ptr, uint32_t *n_levels);
 
+/* ---- result files ----------------------------------------------------------------------- */
+
+/* One named field of a result file: n_comp doubles per node or element (one item for a global
+ * field). */
+typedef struct {
+    const char *name;
+    uint32_t n_comp;
+    const double *data;
+} femgpu_vtu_field;
+/* write_mesh_vtk(filename, m) (mesh/write_mesh_vtk.h:827-831 -> write_mesh_vtk<true>,
+ * :434-656): a VTK XML unstructured grid with the reference writer's layout and bytes -- point
+ * fields in the given order (2-component fields padded to 3), cell fields in name order, the
+ * element group in ASCII (group NULL: 0), global fields repeated for every cell, the points, the
+ * cells (VTK types 10 / 24 / 12) in ASCII, and every float array base64-encoded in one appended
+ * block.  Host only. */
+FEMGPU_API int femgpu_write_vtu(const char *path, int elem_kind, uint32_t n_nodes, const double *coords,
+                                uint32_t n_elem, const uint32_t *conn, uint32_t n_point,
+                                const femgpu_vtu_field *point, uint32_t n_cell, const femgpu_vtu_field *cell,
+                                const uint32_t *group, uint32_t n_global, const femgpu_vtu_field *global);
+/* The problem's mesh and current result in that layout: dep; elastic_modulus, epsilon, sigma,
+ * tr_epsilon (call_after_solve of the current result); poisson_ratio. */
+FEMGPU_API int femgpu_write_result_vtu(femgpu_problem *p, const char *path);
+
 /* ---- time steps, non-linear loop, preconditioner reuse ------------------------------------- */
 
 /* shift(nb) (formulation.h:424-433): time += time step, and the solution history moves one step
@@ -287,6 +310,8 @@ FEMGPU_API int femgpu_batch_get_result(femgpu_batch *b, uint32_t i, double *u, s
 FEMGPU_API int femgpu_batch_clear_result(femgpu_batch *b);
 /* after_solve of problem i of the batch (outputs sized for that problem's elements). */
 FEMGPU_API int femgpu_batch_after_solve(femgpu_batch *b, uint32_t i, double *eps, double *sigma, double *tr_eps);
+/* femgpu_write_result_vtu for problem i of a batch (its own mesh, group = i). */
+FEMGPU_API int femgpu_batch_write_result_vtu(femgpu_batch *b, uint32_t i, const char *path);
 /* Device time (ms) of the last batched assembly. */
 FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
 FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);

@@ -43,6 +43,10 @@ class SolveStats(C.Structure):
                 ("max_abs_res", C.c_double), ("solve_ms", C.c_double)]
 
 
+class VtuField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("n_comp", C.c_uint32), ("data", C.c_void_p)]
+
+
 class FemGpuError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s (status %d)" % (msg, status))
@@ -110,6 +114,10 @@ _SIGS = {
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_set_row_order": (C.c_int, [C.c_int]),
+    "femgpu_write_vtu": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P,
+                                   _P, C.c_uint32, _P]),
+    "femgpu_write_result_vtu": (C.c_int, [_P, C.c_char_p]),
+    "femgpu_batch_write_result_vtu": (C.c_int, [_P, C.c_uint32, C.c_char_p]),
     "femgpu_shift": (C.c_int, [_P, C.c_int]),
     "femgpu_set_initial_time_step": (C.c_int, [_P, C.c_double]),
     "femgpu_set_next_time_step": (C.c_int, [_P, C.c_double]),
@@ -191,6 +199,35 @@ def sell_order(row_ptr, order=ROW_ORDER_BY_LENGTH):
     nat, npos = C.c_uint64(), C.c_uint64()
     check(lib.femgpu_sell_order(n, rp.ctypes.data, order, perm.ctypes.data, C.byref(nat), C.byref(npos)), "sell_order")
     return perm, nat.value, npos.value
+
+
+def write_vtu(path, kind, coords, conn, point=(), cell=(), group=None, glob=()):
+    """Result file in the reference writer's layout (femgpu_write_vtu, host only).  point / cell /
+    glob: sequences of (name, array) with one row of components per node / element, or one row
+    for a global field."""
+    import numpy as np
+    lib = load()
+    keep = []
+
+    def fields(items, rows):
+        arr = (VtuField * max(len(items), 1))()
+        for i, (name, a) in enumerate(items):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            a = a.reshape(rows, -1) if rows else a.reshape(1, -1)
+            keep.append(a)
+            arr[i] = VtuField(name.encode(), a.shape[1], a.ctypes.data)
+        return arr
+
+    coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(-1, NPE[int(kind)])
+    pf, cf, gf = fields(point, len(coords)), fields(cell, len(conn)), fields(glob, 0)
+    g = None
+    if group is not None:
+        g = np.ascontiguousarray(group, dtype=np.uint32)
+        keep.append(g)
+    check(lib.femgpu_write_vtu(str(path).encode(), int(kind), len(coords), coords.ctypes.data, len(conn),
+                               conn.ctypes.data, len(point), pf, len(cell), cf, None if g is None else g.ctypes.data,
+                               len(glob), gf), "write_vtu")
 
 
 class row_order:

@@ -106,6 +106,10 @@ class Batch:
         A.check(self._lib.femgpu_batch_after_solve(self._h, i, _ptr(eps), _ptr(sig), _ptr(tr)), "batch_after_solve")
         return eps, sig, tr
 
+    def write_vtu(self, i, path):
+        """Problem i's mesh and result as a VTK unstructured grid (Formulation.write_vtu)."""
+        A.check(self._lib.femgpu_batch_write_result_vtu(self._h, int(i), str(path).encode()), "batch_write_result_vtu")
+
     def get_result(self, i):
         u = np.zeros(3 * self.sizes[i][0])
         A.check(self._lib.femgpu_batch_get_result(self._h, i, _ptr(u), u.size), "batch_get_result")

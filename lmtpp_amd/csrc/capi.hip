@@ -10,6 +10,7 @@
 
 #include "../../include/femgpu.h"
 #include "femgpu_internal.hpp"
+#include "io_vtu.hpp"
 
 namespace femgpu {
 std::string &last_error() {
@@ -855,6 +856,66 @@ int femgpu_batch_after_solve(femgpu_batch *b, uint32_t i, double *eps, double *s
     if (!b || i >= b->H.P.n_sub) return fail(FEMGPU_BAD_INPUT, "batch_after_solve: bad input");
     const Problem &P = b->H.P;
     return after_solve_host(&b->H, P.elem_off[i], P.elem_off[i + 1] - P.elem_off[i], eps, sigma, tr);
+}
+
+int femgpu_write_vtu(const char *path, int elem_kind, uint32_t n_nodes, const double *coords, uint32_t n_elem,
+                     const uint32_t *conn, uint32_t n_point, const femgpu_vtu_field *point, uint32_t n_cell,
+                     const femgpu_vtu_field *cell, const uint32_t *group, uint32_t n_global, const femgpu_vtu_field *global) {
+    int npe = 0;
+    if (!path || !vtk_cell_type(elem_kind, &npe) || (n_nodes && !coords) || (n_elem && !conn) ||
+        (n_point && !point) || (n_cell && !cell) || (n_global && !global))
+        return fail(FEMGPU_BAD_INPUT, "write_vtu: bad input");
+    for (size_t k = 0; k < (size_t)n_elem * npe; ++k)
+        if (conn[k] >= n_nodes) return fail(FEMGPU_BAD_INPUT, "write_vtu: node index out of range");
+    auto fields = [](uint32_t n, const femgpu_vtu_field *f, size_t items, std::vector<VtuField> &out) {
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!f[i].name || (f[i].n_comp && items && !f[i].data)) return false;
+            out.push_back(VtuField{f[i].name, f[i].n_comp, f[i].data});
+        }
+        return true;
+    };
+    std::vector<VtuField> pf, cf, gf;
+    if (!fields(n_point, point, n_nodes, pf) || !fields(n_cell, cell, n_elem, cf) || !fields(n_global, global, 1, gf))
+        return fail(FEMGPU_BAD_INPUT, "write_vtu: field without name or data");
+    if (!write_vtu(path, elem_kind, n_nodes, coords, n_elem, conn, pf, cf, group, gf))
+        return fail(FEMGPU_BAD_INPUT, std::string("write_vtu: cannot write ") + path);
+    return FEMGPU_OK;
+}
+
+namespace {
+// Sub-problem s of a mesh problem as a result file: dep; elastic_modulus, epsilon, sigma,
+// tr_epsilon (the after-solve recovery of the current result); group = s; poisson_ratio.
+int write_result(femgpu_problem &H, uint32_t s, const char *path) {
+    Problem &P = H.P;
+    if (!path || P.from_bsr || s >= P.n_sub) return fail(FEMGPU_BAD_INPUT, "write_result_vtu: bad input");
+    const uint32_t n0 = P.node_off[s], nn = P.node_off[s + 1] - n0, e0 = P.elem_off[s], ne = P.elem_off[s + 1] - e0;
+    std::vector<double> coords(3 * (size_t)nn), u(3 * (size_t)nn), E(ne), eps(6 * (size_t)ne), sig(6 * (size_t)ne), tr(ne);
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipMemcpy(coords.data(), P.d_coords + 3 * (size_t)n0, coords.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(u.data(), P.d_u + 3 * (size_t)n0, u.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(E.data(), P.d_E + e0, E.size() * 8, hipMemcpyDeviceToHost));
+    int rc = after_solve_host(&H, e0, ne, eps.data(), sig.data(), tr.data());
+    if (rc) return rc;
+    std::vector<uint32_t> conn(P.conn_h.begin() + (size_t)e0 * P.npe, P.conn_h.begin() + (size_t)(e0 + ne) * P.npe);
+    for (uint32_t &v : conn) v -= n0;
+    const std::vector<uint32_t> group(ne, s);
+    const double nu = P.mat_h[s].nu;
+    const femgpu_vtu_field pf[] = {{"dep", 3, u.data()}};
+    const femgpu_vtu_field cf[] = {{"elastic_modulus", 1, E.data()}, {"epsilon", 6, eps.data()},
+                                   {"sigma", 6, sig.data()}, {"tr_epsilon", 1, tr.data()}};
+    const femgpu_vtu_field gf[] = {{"poisson_ratio", 1, &nu}};
+    return femgpu_write_vtu(path, P.kind, nn, coords.data(), ne, conn.data(), 1, pf, 4, cf, group.data(), 1, gf);
+}
+}  // namespace
+
+int femgpu_write_result_vtu(femgpu_problem *h, const char *path) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "write_result_vtu: null");
+    return write_result(*h, 0, path);
+}
+
+int femgpu_batch_write_result_vtu(femgpu_batch *b, uint32_t i, const char *path) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_write_result_vtu: null");
+    return write_result(b->H, i, path);
 }
 
 int femgpu_elem_grad(femgpu_problem *h, const double *u, const double *lam, double *g) {

@@ -356,6 +356,12 @@ class Formulation:
         A.check(self._lib.femgpu_after_solve(self._h, _ptr(eps), _ptr(sig), _ptr(tr)), "after_solve")
         return eps, sig, tr
 
+    def write_vtu(self, path):
+        """Mesh and current result as a VTK unstructured grid in the reference writer's layout
+        (write_mesh_vtk, mesh/write_mesh_vtk.h:827-831): dep; elastic_modulus, epsilon, sigma,
+        tr_epsilon; poisson_ratio."""
+        A.check(self._lib.femgpu_write_result_vtu(self._h, str(path).encode()), "write_result_vtu")
+
     def after_solve_ms(self):
         """Device time of the last after-solve kernel."""
         v = C.c_double()

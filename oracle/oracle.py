@@ -203,7 +203,8 @@ def after_solve(kind, conn, coords, E_e, nu, u):
 # --- reference builds (oracle/_ref), available only where the LMT++ sources are -----------------
 
 def ref_available():
-    return all(os.path.exists(os.path.join(REF_DIR, b)) for b in ("ref_mesh_dump", "ref_tinyblock_pcg", "ref_after_solve"))
+    return all(os.path.exists(os.path.join(REF_DIR, b))
+               for b in ("ref_mesh_dump", "ref_tinyblock_pcg", "ref_after_solve", "ref_write_vtu"))
 
 
 def ref_mesh(kind_name, nb_points, tmpdir):
@@ -234,6 +235,29 @@ def ref_after_solve_hexa(nb_points, coords, u, E, nu, tmpdir):
         ne = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
         d = np.frombuffer(f.read(8 * 13 * ne), dtype=np.float64).reshape(ne, 13)
     return d[:, :6].copy(), d[:, 6:12].copy(), d[:, 12].copy()
+
+
+def ref_write_vtu_hexa(nb_points, coords, u, E, nu, tmpdir):
+    """The reference's write_mesh_vtk (binary, appended) for make_rect(Hexa, nb_points) of the
+    elasticity_dep_static mesh with node positions coords, displacements u, E, nu and the strain
+    and stress of its generated after_solve: (file bytes, eps[n,6], sigma[n,6], tr[n])."""
+    inp = os.path.join(tmpdir, "vtu_in.bin")
+    out = os.path.join(tmpdir, "vtu_out.vtu")
+    flds = os.path.join(tmpdir, "vtu_fields.bin")
+    coords, u = _f64(coords), _f64(u)
+    with open(inp, "wb") as f:
+        f.write(np.array([coords.shape[0]], dtype=np.uint32).tobytes())
+        f.write(coords.tobytes())
+        f.write(u.tobytes())
+        f.write(np.array([E, nu], dtype=np.float64).tobytes())
+    subprocess.check_call([os.path.join(REF_DIR, "ref_write_vtu")] + [str(v) for v in nb_points] + [inp, out, flds],
+                          stdout=subprocess.DEVNULL)
+    with open(out, "rb") as f:
+        data = f.read()
+    with open(flds, "rb") as f:
+        ne = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
+        d = np.frombuffer(f.read(8 * 13 * ne), dtype=np.float64).reshape(ne, 13)
+    return data, d[:, :6].copy(), d[:, 6:12].copy(), d[:, 12].copy()
 
 
 def ref_tinyblock_pcg(row_ptr, col, vals, b, x0, crit, tmpdir):

@@ -11,8 +11,10 @@ run where those sources exist.  Fixtures are data only (inputs and the reference
   ref_after_solve_hexa_4_3_5.npz       coords, conn, u, E, nu -> eps, sigma, tr_epsilon
         -- the generated Hexa after_solve of std_formulations/
            definition_probleme_elasticity_dep_static.h:721-789 on a jittered make_rect mesh
+  ref_write_vtu_hexa_3_4_2.npz         coords, conn, u, E, nu, eps, sigma, tr -> vtu (file bytes)
+        -- write_mesh_vtk(filename, m) (mesh/write_mesh_vtk.h:434-656,827-831) on that mesh
 
-    python tests/golden/make_golden.py [after_solve]   (no argument: every fixture)
+    python tests/golden/make_golden.py [after_solve|vtu]   (no argument: every fixture)
 """
 import json
 import os
@@ -56,13 +58,31 @@ def after_solve_fixture(tmp):
                         eps=eps, sigma=sig, tr=tr)
 
 
+def vtu_fixture(tmp):
+    nb = (3, 4, 2)
+    c, e = O.ref_mesh("hexa", nb, tmp)
+    rng = np.random.default_rng(77)
+    inside = np.all((c > 1e-9) & (c < 1 - 1e-9), axis=1)
+    c = c.copy()
+    c[inside] += rng.uniform(-0.05, 0.05, size=(int(inside.sum()), 3))
+    u = 1e-3 * rng.standard_normal(3 * len(c))
+    E, nu = 190000.0, 0.31
+    data, eps, sig, tr = O.ref_write_vtu_hexa(nb, c, u, E, nu, tmp)
+    np.savez_compressed(os.path.join(HERE, "ref_write_vtu_hexa_3_4_2.npz"), coords=c, conn=e, u=u, E=E, nu=nu,
+                        eps=eps, sigma=sig, tr=tr, vtu=np.frombuffer(data, dtype=np.uint8))
+
+
 def main():
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "liboracle.so"])
     tmp = tempfile.mkdtemp()
     if sys.argv[1:] == ["after_solve"]:
         after_solve_fixture(tmp)
         return
+    if sys.argv[1:] == ["vtu"]:
+        vtu_fixture(tmp)
+        return
     after_solve_fixture(tmp)
+    vtu_fixture(tmp)
     for name, kind, nb in MESHES:
         c, e = O.ref_mesh(name, nb, tmp)
         np.savez_compressed(os.path.join(HERE, "ref_mesh_%s_%d_%d_%d.npz" % ((name,) + nb)), coords=c, conn=e)

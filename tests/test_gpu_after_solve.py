@@ -73,3 +73,37 @@ def test_batch_after_solve_matches_single():
         single = f.call_after_solve()
         for x, y in zip(b.call_after_solve(i), single):
             assert np.array_equal(x, y)
+
+
+def test_result_file_holds_result_and_recovery(tmp_path):
+    """femgpu_write_result_vtu: dep, E and the after-solve fields of the problem, decoded back."""
+    import base64
+    import struct
+    import xml.etree.ElementTree as ET
+    c, e = L.make_rect(L.TETRA10, (4, 3, 3))
+    E = lognormal_E(len(e), 0.3, 9)
+    f, *_ = build(L.TETRA10, c, e, E)
+    assert f.solve(rel_tol=1e-12)
+    path = tmp_path / "res.vtu"
+    f.write_vtu(path)
+    root = ET.fromstring(path.read_text())
+    block = root.find("AppendedData").text.strip()[1:]
+
+    def arr(d, rows):
+        off = int(d.get("offset"))
+        n = struct.unpack("<I", base64.b64decode(block[off:off + 8])[:4])[0]
+        raw = base64.b64decode(block[off:off + 4 * ((4 + n + 2) // 3)])
+        return np.frombuffer(raw[4:], dtype=np.float64).reshape(rows, -1)
+
+    piece = root.find("UnstructuredGrid/Piece")
+    pd = {d.get("Name"): d for d in piece.findall("PointData/DataArray")}
+    cd = {d.get("Name"): d for d in piece.findall("CellData/DataArray")}
+    assert np.array_equal(arr(pd["dep"], len(c)).ravel(), f.get_result())
+    eps, sig, tr = f.call_after_solve()
+    assert np.array_equal(arr(cd["epsilon"], len(e)), eps)
+    assert np.array_equal(arr(cd["sigma"], len(e)), sig)
+    assert np.array_equal(arr(cd["tr_epsilon"], len(e)).ravel(), tr)
+    assert np.array_equal(arr(cd["elastic_modulus"], len(e)).ravel(), E)
+    assert (arr(cd["poisson_ratio"], len(e)) == 0.3).all()
+    assert np.array_equal(arr(piece.find("Points/DataArray"), len(c)), c)
+    f.close()

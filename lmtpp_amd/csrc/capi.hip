@@ -91,7 +91,7 @@ void free_problem(Problem &P) {
     dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
     dfree(P.d_cdof); dfree(P.d_cptr); dfree(P.d_csub); dfree(P.d_cval); dfree(P.d_cpen);
     dfree(P.d_r); dfree(P.d_p); dfree(P.d_q); dfree(P.d_Minv); dfree(P.d_aux); dfree(P.d_red);
-    dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32);
+    dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32); dfree(P.d_z); dfree(P.d_z32);
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
     dfree(P.d_gptr); dfree(P.d_gel); dfree(P.d_perm); dfree(P.d_iperm);
@@ -204,6 +204,8 @@ int alloc_system(Problem &P) {
     HIPCHK(dalloc(&P.d_p32, n));
     HIPCHK(dalloc(&P.d_q32, n));
     HIPCHK(dalloc(&P.d_x32, n));
+    HIPCHK(dalloc(&P.d_z, n));
+    HIPCHK(dalloc(&P.d_z32, n));
     HIPCHK(dalloc(&P.d_partials, 3 * (size_t)P.n_wg));
     HIPCHK(dalloc(&P.d_ps, P.n_sub));
     HIPCHK(dalloc(&P.d_flags, 4));

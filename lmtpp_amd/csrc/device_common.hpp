@@ -54,38 +54,48 @@ __device__ void wg_partial(double (&v)[NV], double *partials, unsigned stride, u
     }
 }
 
-// One workgroup combines the partials of workgroups [w0, w0 + nw) in a fixed order (strided
-// per thread, wave tree, 4-wave combination).  out[] is valid in every thread.
+// One workgroup (up to 1024 lanes) combines the partials of workgroups [w0, w0 + nw) in a fixed
+// order: each round issues the loads of all NV values for four strided elements per lane at once
+// (one memory round trip per round: a finalize after a row kernel of 1024 x 4 workgroups needs
+// one), then a wave tree and a fixed combination of the waves.  out[] is valid in every thread.
 template <int NV, unsigned MAXMASK>
 __device__ void wg_combine(const double *partials, unsigned stride, unsigned w0, unsigned nw, double (&out)[NV]) {
-    __shared__ double sm[NV][4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double v[NV];
+    __shared__ double sm[NV][16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n_waves = (blockDim.x + 63) >> 6;
+    const unsigned B = blockDim.x;
+    double a[NV][4];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        // four independent chains per thread (i, i+B, i+2B, i+3B), joined in a fixed order
-        const double *src = partials + (size_t)k * stride + w0;
-        const unsigned B = blockDim.x;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        unsigned i = threadIdx.x;
-        for (; i + 3 * B < nw; i += 4 * B) {
-            const double t0 = src[i], t1 = src[i + B], t2 = src[i + 2 * B], t3 = src[i + 3 * B];
-            if (MAXMASK >> k & 1) { a0 = fmax(a0, t0); a1 = fmax(a1, t1); a2 = fmax(a2, t2); a3 = fmax(a3, t3); }
-            else { a0 += t0; a1 += t1; a2 += t2; a3 += t3; }
-        }
-        for (; i < nw; i += B) a0 = (MAXMASK >> k & 1) ? fmax(a0, src[i]) : a0 + src[i];
-        v[k] = (MAXMASK >> k & 1) ? fmax(fmax(a0, a1), fmax(a2, a3)) : (a0 + a1) + (a2 + a3);
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[k][j] = 0.0;
+    for (unsigned i0 = threadIdx.x; i0 < nw; i0 += 4 * B) {
+        double t[NV][4];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const unsigned i = i0 + j * B;
+                t[k][j] = i < nw ? partials[(size_t)k * stride + w0 + i] : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a[k][j] = (MAXMASK >> k & 1) ? fmax(a[k][j], t[k][j]) : a[k][j] + t[k][j];
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const double t = (MAXMASK >> k & 1) ? wave_max(v[k]) : wave_sum(v[k]);
+        const double v = (MAXMASK >> k & 1) ? fmax(fmax(a[k][0], a[k][1]), fmax(a[k][2], a[k][3]))
+                                            : (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
+        const double t = (MAXMASK >> k & 1) ? wave_max(v) : wave_sum(v);
         if (lane == 0) sm[k][wave] = t;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-        out[k] = (MAXMASK >> k & 1) ? fmax(fmax(sm[k][0], sm[k][1]), fmax(sm[k][2], sm[k][3]))
-                                    : ((sm[k][0] + sm[k][1]) + sm[k][2]) + sm[k][3];
+    for (int k = 0; k < NV; ++k) {
+        double o = sm[k][0];
+        for (int w = 1; w < n_waves; ++w) o = (MAXMASK >> k & 1) ? fmax(o, sm[k][w]) : o + sm[k][w];
+        out[k] = o;
+    }
 }
 
 __device__ inline bool finite(double v) { return v == v && fabs(v) <= 1.7976931348623157e308; }

@@ -130,6 +130,8 @@ struct Problem {
     double *d_aux = nullptr;  // adjoint state / scratch vector
     double *d_red = nullptr;  // diff_norm partials
     float *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_x32 = nullptr, *d_Minv32 = nullptr;
+    double *d_z = nullptr;    // z = M^-1 r of the Jacobi preconditioners (written by the update)
+    float *d_z32 = nullptr;
     double *d_partials = nullptr;
     ProbScalars *d_ps = nullptr, *h_ps = nullptr;
     int *d_flags = nullptr, *h_flags = nullptr;  // [0]: element with det J <= 0; [1]: IC(0) pivot <= 0

@@ -171,30 +171,29 @@ __global__ void k_to_sym(const double *__restrict__ K, uint64_t ps, const uint32
     }
 }
 
-// z = M^-1 r for one node; PRE 0: scalar Jacobi (3 values per node), 1: 3x3 block Jacobi (9),
-// 2: block IC(0), whose z the level sweeps have already written to the vector passed as Minv.
+// z = M^-1 r for one node; PRE 0: scalar Jacobi (3 values per node), 1: 3x3 block Jacobi, the
+// inverse of a symmetric diagonal block stored as its 6 upper values (m00 m01 m02 m11 m12 m22),
+// 2: block IC(0), whose z the level sweeps write (not applied here).
 enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2 };
 template <int PRE, typename T>
 __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3], T z[3]) {
-    if (PRE == PRE_IC0) {
-        z[0] = Minv[3 * n]; z[1] = Minv[3 * n + 1]; z[2] = Minv[3 * n + 2];
-    } else if (PRE == 0) {
+    if (PRE == 0) {
         z[0] = Minv[3 * n] * r[0]; z[1] = Minv[3 * n + 1] * r[1]; z[2] = Minv[3 * n + 2] * r[2];
-    } else {
-        const T *m = Minv + 9 * n;
+    } else if (PRE == 1) {
+        const T *m = Minv + 6 * n;
         z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
-        z[1] = m[3] * r[0] + m[4] * r[1] + m[5] * r[2];
-        z[2] = m[6] * r[0] + m[7] * r[1] + m[8] * r[2];
+        z[1] = m[1] * r[0] + m[3] * r[1] + m[4] * r[2];
+        z[2] = m[2] * r[0] + m[4] * r[1] + m[5] * r[2];
     }
 }
 
-// x += alpha p; r -= alpha q; z = M^-1 r; partials of r.z, r.r, max|r| (k_finalize FIN_UPD
-// advances the scalars and applies the stop test).  IC(0): z needs the sweeps, so r.z is left
-// to k_dot_rz (partial 0 stays zero, FIN_UPD_IC).
+// x += alpha p; r -= alpha q; z = M^-1 r (stored for k_update_p); partials of r.z, r.r, max|r|
+// (k_finalize FIN_UPD advances the scalars and applies the stop test).  IC(0): z needs the
+// sweeps, so r.z is left to k_dot_rz (partial 0 stays zero, FIN_UPD_IC).
 template <int PRE, typename T>
 __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
-                                                const T *__restrict__ q, const T *__restrict__ Minv, WgTables wt,
-                                                const ProbScalars *S, double *partials) {
+                                                const T *__restrict__ q, const T *__restrict__ Minv, T *__restrict__ z,
+                                                WgTables wt, const ProbScalars *S, double *partials) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB || !selected(S[s], SEL_ACTIVE)) return;
@@ -213,7 +212,10 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
         if (PRE != PRE_IC0) apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (PRE != PRE_IC0) acc[0] += (double)rv[d] * (double)zv[d];
+            if (PRE != PRE_IC0) {
+                z[3 * (size_t)n + d] = zv[d];
+                acc[0] += (double)rv[d] * (double)zv[d];
+            }
             acc[1] += (double)rv[d] * (double)rv[d];
             acc[2] = fmax(acc[2], fabs((double)rv[d]));
         }
@@ -221,22 +223,23 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
     wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
 
-// p = M^-1 r + beta p, or p = M^-1 r when first (sel: SEL_ACTIVE in the iteration,
-// SEL_PAUSE_DONE after a refresh, SEL_ALL for the first direction of IC(0))
-template <int PRE, typename T>
-__global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ r, T *__restrict__ p,
-                                                  const T *__restrict__ Minv, WgTables wt, const ProbScalars *S,
-                                                  int sel, int first) {
+// p = z + beta p, or p = z when first, with z = M^-1 r as the update, the restart or the IC(0)
+// sweeps stored it (sel: SEL_ACTIVE in the iteration, SEL_PAUSE_DONE after a refresh, SEL_ALL
+// for the first direction of IC(0))
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ z, T *__restrict__ p, WgTables wt,
+                                                  const ProbScalars *S, int sel, int first) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB || !selected(S[s], sel)) return;
     const T beta = (T)S[s].beta;
     const uint32_t n = wt.row0[L] + threadIdx.x;
     if (n >= wt.row1[L]) return;
-    T rv[3] = {r[3 * (size_t)n], r[3 * (size_t)n + 1], r[3 * (size_t)n + 2]}, zv[3];
-    apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) p[3 * (size_t)n + d] = first ? zv[d] : zv[d] + beta * p[3 * (size_t)n + d];
+    for (int d = 0; d < 3; ++d) {
+        const size_t i = 3 * (size_t)n + d;
+        p[i] = first ? z[i] : z[i] + beta * p[i];
+    }
 }
 
 // Start (first = 1, every unfinished sub-problem) or restart after an FP64 residual refresh
@@ -246,8 +249,8 @@ __global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ r, T *__
 // k_dot_rz and the direction update follow.
 template <int PRE, typename T>
 __global__ __launch_bounds__(256) void k_restart(const double *r64, T *r, T *__restrict__ p,
-                                                 const T *__restrict__ Minv, WgTables wt, const ProbScalars *S,
-                                                 double *partials, int first) {
+                                                 const T *__restrict__ Minv, T *__restrict__ z, WgTables wt,
+                                                 const ProbScalars *S, double *partials, int first) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB || !selected(S[s], first ? (int)SEL_ALL : (int)SEL_PAUSE_REQ)) return;
@@ -261,6 +264,7 @@ __global__ __launch_bounds__(256) void k_restart(const double *r64, T *r, T *__r
             apply_m<PRE, T>(Minv, n, rv, zv);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
+                z[3 * (size_t)n + d] = zv[d];
                 if (first) p[3 * (size_t)n + d] = zv[d];
                 acc[0] += (double)rv[d] * (double)zv[d];
             }
@@ -292,7 +296,7 @@ __global__ __launch_bounds__(256) void k_dot_rz(const T *__restrict__ r, const T
 enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5,
        FIN_UPD_IC = 6, FIN_RHO = 7 };
 
-__global__ __launch_bounds__(256) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
+__global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
                                                   unsigned stride, const uint32_t *__restrict__ sub_wg0,
                                                   const uint32_t *__restrict__ sub_nwg, int sel) {
     const uint32_t s = blockIdx.x;
@@ -411,9 +415,10 @@ __global__ __launch_bounds__(256) void k_build_minv(const uint32_t *__restrict__
         m[0] = c00 * id; m[3] = c01 * id; m[6] = c02 * id;
         m[1] = (a[2] * a[7] - a[1] * a[8]) * id; m[4] = (a[0] * a[8] - a[2] * a[6]) * id; m[7] = (a[1] * a[6] - a[0] * a[7]) * id;
         m[2] = (a[1] * a[5] - a[2] * a[4]) * id; m[5] = (a[2] * a[3] - a[0] * a[5]) * id; m[8] = (a[0] * a[4] - a[1] * a[3]) * id;
-        for (int c = 0; c < 9; ++c) {
-            M[9 * (size_t)n + c] = m[c];
-            if (M32) M32[9 * (size_t)n + c] = (float)m[c];
+        const int up[6] = {0, 1, 2, 4, 5, 8};   // m is exactly symmetric (K's diagonal block is)
+        for (int c = 0; c < 6; ++c) {
+            M[6 * (size_t)n + c] = m[up[c]];
+            if (M32) M32[6 * (size_t)n + c] = (float)m[up[c]];
         }
     }
 }
@@ -423,7 +428,7 @@ __global__ __launch_bounds__(256) void k_build_minv(const uint32_t *__restrict__
 // ---------------------------------------------------------------------------------------------
 
 static void finalize(Problem &P, int what, int sel = 0) {
-    hipLaunchKernelGGL(k_finalize, dim3(P.n_sub), dim3(256), 0, P.stream, what, P.d_ps, P.d_partials, P.n_wg, P.d_sub_wg0,
+    hipLaunchKernelGGL(k_finalize, dim3(P.n_sub), dim3(1024), 0, P.stream, what, P.d_ps, P.d_partials, P.n_wg, P.d_sub_wg0,
                        P.d_sub_nwg, sel);
 }
 
@@ -607,6 +612,13 @@ static int write_scalars(Problem &P) {
     return 0;
 }
 
+// z = M^-1 r of precision T: the IC(0) sweep output, or the vector the Jacobi updates store.
+template <int PRE, typename T>
+static T *zbuf(Problem &P) {
+    if (sizeof(T) == 4) return reinterpret_cast<T *>(PRE == PRE_IC0 ? P.ic0.d_z32 : P.d_z32);
+    return reinterpret_cast<T *>(PRE == PRE_IC0 ? P.ic0.d_z : P.d_z);
+}
+
 // One PCG iteration on precision T (K, vectors and the Jacobi factors or IC(0) buffers of T).
 template <int PRE, typename T>
 static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, const T *Minv, T *z) {
@@ -614,25 +626,24 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const dim3 g(P.n_wg), blk(256);
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE);
     finalize(P, FIN_PQ);
-    hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, q, Minv, wt, P.d_ps, P.d_partials);
+    hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, q, Minv, z, wt, P.d_ps, P.d_partials);
     if (PRE == PRE_IC0) {
         finalize(P, FIN_UPD_IC);
         ic0_apply(P, r, z, SEL_ACTIVE);
         hipLaunchKernelGGL(k_dot_rz<T>, g, blk, 0, P.stream, r, z, wt, P.d_ps, P.d_partials, (int)SEL_ACTIVE);
         finalize(P, FIN_RHO);
-        hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, z, wt, P.d_ps, (int)SEL_ACTIVE, 0);
     } else {
         finalize(P, FIN_UPD);
-        hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, Minv, wt, P.d_ps, (int)SEL_ACTIVE, 0);
     }
+    hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_ACTIVE, 0);
 }
 
 template <int PRE>
 static void launch_iteration(Problem &P, bool fp32, double *d_x) {
     if (fp32)
-        launch_iteration_t<PRE, float>(P, P.d_K32, P.d_x32, P.d_r32, P.d_p32, P.d_q32, P.d_Minv32, P.ic0.d_z32);
+        launch_iteration_t<PRE, float>(P, P.d_K32, P.d_x32, P.d_r32, P.d_p32, P.d_q32, P.d_Minv32, zbuf<PRE, float>(P));
     else
-        launch_iteration_t<PRE, double>(P, P.d_K, d_x, P.d_r, P.d_p, P.d_q, P.d_Minv, P.ic0.d_z);
+        launch_iteration_t<PRE, double>(P, P.d_K, d_x, P.d_r, P.d_p, P.d_q, P.d_Minv, zbuf<PRE, double>(P));
 }
 
 // (Re)start of the recurrence on precision T from the FP64 residual r64: first = 1 for every
@@ -645,17 +656,17 @@ static void launch_restart(Problem &P, const double *r64, T *r, T *p, const T *M
     const int sel = first ? (int)SEL_ALL : (int)SEL_PAUSE_REQ;
     if (PRE == PRE_IC0) {
         if ((const void *)r64 != (const void *)r)
-            hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, wt, P.d_ps, P.d_partials, first);
+            hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps, P.d_partials, first);
         ic0_apply(P, r, z, sel);
         hipLaunchKernelGGL(k_dot_rz<T>, g, blk, 0, P.stream, r, z, wt, P.d_ps, P.d_partials, sel);
         finalize(P, first ? FIN_RESTART_FIRST : FIN_RESTART);
-        hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, z, wt, P.d_ps,
-                           first ? (int)SEL_ALL : (int)SEL_PAUSE_DONE, first);
+        hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, first ? (int)SEL_ALL : (int)SEL_PAUSE_DONE,
+                           first);
     } else {
-        hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, wt, P.d_ps, P.d_partials, first);
+        hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps, P.d_partials, first);
         finalize(P, first ? FIN_RESTART_FIRST : FIN_RESTART);
         if (!first)
-            hipLaunchKernelGGL((k_update_p<PRE, T>), g, blk, 0, P.stream, r, p, Minv, wt, P.d_ps, (int)SEL_PAUSE_DONE, 0);
+            hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_PAUSE_DONE, 0);
     }
 }
 
@@ -750,9 +761,9 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     finalize(P, FIN_RES_CHECK, SEL_ALL);
     if (mixed) {
         hipLaunchKernelGGL(k_zero_f32, dim3(std::max<size_t>(1, std::min<size_t>(8192, (n + 255) / 256))), blk, 0, P.stream, n, P.d_x32);
-        launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, P.ic0.d_z32, 1);
+        launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, zbuf<PRE, float>(P), 1);
     } else {
-        launch_restart<PRE, double>(P, P.d_r, P.d_r, P.d_p, P.d_Minv, P.ic0.d_z, 1);
+        launch_restart<PRE, double>(P, P.d_r, P.d_r, P.d_p, P.d_Minv, zbuf<PRE, double>(P), 1);
     }
     int chunk = 2;
     const long max_chunks = (long)o.max_iter / 2 + 1000;
@@ -776,7 +787,7 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_PAUSE_REQ);
             launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, SEL_PAUSE_REQ);
             finalize(P, FIN_RES_CHECK, SEL_PAUSE_REQ);
-            launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, P.ic0.d_z32, 0);
+            launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, zbuf<PRE, float>(P), 0);
             hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
         }
     }

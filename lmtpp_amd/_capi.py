@@ -19,6 +19,7 @@ REL_L2, ABS_MAXNORM = 0, 1
 SPMV_FULL, SPMV_SYM = 0, 1
 LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
 ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
+NODE_ORDER_NATURAL, NODE_ORDER_RCM = 0, 1
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
 NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}
@@ -114,6 +115,9 @@ _SIGS = {
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_set_row_order": (C.c_int, [C.c_int]),
+    "femgpu_node_order": (C.c_int, [C.c_uint32, _P, _P, _P]),
+    "femgpu_set_node_order": (C.c_int, [C.c_int]),
+    "femgpu_get_node_order": (C.c_int, [_P, C.POINTER(C.c_int), _P]),
     "femgpu_write_vtu": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P,
                                    _P, C.c_uint32, _P]),
     "femgpu_write_result_vtu": (C.c_int, [_P, C.c_char_p]),
@@ -230,6 +234,17 @@ def write_vtu(path, kind, coords, conn, point=(), cell=(), group=None, glob=()):
                                len(glob), gf), "write_vtu")
 
 
+def node_order(row_ptr, col):
+    """Cuthill-McKee visiting order of a node graph (femgpu_node_order, host only)."""
+    import numpy as np
+    lib = load()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    cl = np.ascontiguousarray(col, dtype=np.uint32)
+    o = np.zeros(len(rp) - 1, np.uint32)
+    check(lib.femgpu_node_order(len(rp) - 1, rp.ctypes.data, cl.ctypes.data, o.ctypes.data), "node_order")
+    return o
+
+
 class row_order:
     """Context manager: problems created inside use the given block-row storage order."""
 
@@ -242,6 +257,21 @@ class row_order:
 
     def __exit__(self, *exc):
         check(load().femgpu_set_row_order(ROW_ORDER_AUTO), "set_row_order")
+        return False
+
+
+class node_order_ctx:
+    """Context manager: problems created inside store their nodes in the given order."""
+
+    def __init__(self, order):
+        self.order = order
+
+    def __enter__(self):
+        check(load().femgpu_set_node_order(self.order), "set_node_order")
+        return self
+
+    def __exit__(self, *exc):
+        check(load().femgpu_set_node_order(NODE_ORDER_NATURAL), "set_node_order")
         return False
 
 

@@ -389,6 +389,21 @@ uint32_t max_assembly_rows(const Problem &P) {
     return R;
 }
 
+__global__ __launch_bounds__(256) void k_permute_nodes(double *__restrict__ dst, const double *__restrict__ src,
+                                                       const uint32_t *__restrict__ map, uint32_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = map[i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[3 * m + d];
+    }
+}
+
+hipError_t launch_permute_nodes(double *dst, const double *src, const uint32_t *map, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_permute_nodes, dim3(std::min<uint32_t>(8192, (n + 255) / 256)), dim3(256), 0, s, dst, src, map, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
     hipError_t e = hipSuccess;
     bool body_force = false;

@@ -94,7 +94,7 @@ void free_problem(Problem &P) {
     dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32); dfree(P.d_z); dfree(P.d_z32);
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
-    dfree(P.d_gptr); dfree(P.d_gel); dfree(P.d_perm); dfree(P.d_iperm);
+    dfree(P.d_gptr); dfree(P.d_gel); dfree(P.d_perm); dfree(P.d_iperm); dfree(P.d_nperm); dfree(P.d_ninv);
     if (P.h_ps) (void)hipHostFree(P.h_ps);
     if (P.h_flags) (void)hipHostFree(P.h_flags);
     P.h_ps = nullptr;
@@ -147,8 +147,10 @@ int set_assembly_rows(Problem &P, uint32_t R) {
 
 namespace {
 std::atomic<int> g_row_order{ROW_ORDER_AUTO};
+std::atomic<int> g_node_order{NODE_ORDER_NATURAL};
 }
 int row_order_default() { return g_row_order.load(); }
+int node_order_default() { return g_node_order.load(); }
 
 // Device buffers common to mesh problems and plain block systems.
 int alloc_system(Problem &P) {
@@ -260,6 +262,28 @@ int build_problem(femgpu_problem &H, uint32_t n_sub, const femgpu_mesh *meshes) 
         std::fill(esub.begin() + P.elem_off[s], esub.begin() + P.elem_off[s + 1], s);
     }
     (void)hipGetDevice(&P.device);
+    if (node_order_default() == NODE_ORDER_CM) {
+        // symrcm-style renumbering (formulation.h:319-323), per sub-problem
+        const Pattern api = build_pattern(P.n_nodes, P.n_elem, npe, conn.data());
+        std::vector<uint32_t> perm;
+        perm.reserve(P.n_nodes);
+        for (uint32_t s = 0; s < n_sub; ++s) {
+            const std::vector<uint32_t> o = cm_order(api, P.node_off[s], P.node_off[s + 1]);
+            perm.insert(perm.end(), o.begin(), o.end());
+        }
+        bool identity = true;
+        for (uint32_t i = 0; i < P.n_nodes && identity; ++i) identity = perm[i] == i;
+        if (!identity) {
+            P.ninv.assign(P.n_nodes, 0);
+            for (uint32_t i = 0; i < P.n_nodes; ++i) P.ninv[perm[i]] = i;
+            P.nperm = std::move(perm);
+            for (uint32_t &v : conn) v = P.ninv[v];
+            std::vector<double> c2(coords.size());
+            for (uint32_t i = 0; i < P.n_nodes; ++i)
+                for (int d = 0; d < 3; ++d) c2[3 * (size_t)i + d] = coords[3 * (size_t)P.nperm[i] + d];
+            coords.swap(c2);
+        }
+    }
     P.pat = build_pattern(P.n_nodes, P.n_elem, npe, conn.data());
     P.nnzb = P.pat.col.size();
     P.ecol = color_elements(P.n_nodes, P.n_elem, npe, conn.data());
@@ -267,6 +291,10 @@ int build_problem(femgpu_problem &H, uint32_t n_sub, const femgpu_mesh *meshes) 
     if (rc) return rc;
     std::vector<uint32_t> slot = build_slot_map(P.pat, P.n_elem, npe, conn.data());
     HIPCHK(upload(&P.d_slot, slot.data(), slot.size()));
+    if (!P.nperm.empty()) {
+        HIPCHK(upload(&P.d_nperm, P.nperm.data(), P.nperm.size()));
+        HIPCHK(upload(&P.d_ninv, P.ninv.data(), P.ninv.size()));
+    }
     HIPCHK(upload(&P.d_coords, coords.data(), coords.size()));
     HIPCHK(upload(&P.d_conn, conn.data(), conn.size()));
     HIPCHK(upload(&P.d_order, P.ecol.order.data(), P.ecol.order.size()));
@@ -325,7 +353,7 @@ int upload_faces(femgpu_problem &H) {
     std::vector<uint32_t> faces;
     std::vector<double> trac;
     for (uint32_t s = 0; s < P.n_sub; ++s) {
-        for (uint32_t v : H.sub_faces[s]) faces.push_back(v + P.node_off[s]);
+        for (uint32_t v : H.sub_faces[s]) faces.push_back(P.ninv.empty() ? v + P.node_off[s] : P.ninv[v + P.node_off[s]]);
         trac.insert(trac.end(), H.sub_trac[s].begin(), H.sub_trac[s].end());
     }
     dfree(P.d_faces); dfree(P.d_forder); dfree(P.d_trac);
@@ -352,7 +380,10 @@ int add_constraints(femgpu_problem &H, uint32_t s, uint32_t n, const uint32_t *d
         if (dofs[k] >= nd) return fail(FEMGPU_BAD_INPUT, "add_constraints: dof out of range");
         if (values && !std::isfinite(values[k])) return fail(FEMGPU_NAN, "add_constraints: non-finite value");
     }
-    for (uint32_t k = 0; k < n; ++k) P.cons.push_back({dofs[k] + 3 * P.node_off[s], values ? values[k] : 0.0, penalty});
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t g = dofs[k] + 3 * P.node_off[s];
+        P.cons.push_back({P.ninv.empty() ? g : 3 * P.ninv[g / 3] + g % 3, values ? values[k] : 0.0, penalty});
+    }
     P.cons_dirty = true;
     return FEMGPU_OK;
 }
@@ -463,12 +494,22 @@ int solve_system(femgpu_problem &H, const femgpu_solve_opts *o, femgpu_solve_sta
     return FEMGPU_OK;
 }
 
+// Node-major values of sub-problem s from the internal node order to the caller's, in place.
+void to_api_order(const Problem &P, uint32_t s, double *v) {
+    if (P.ninv.empty()) return;
+    const uint32_t n0 = P.node_off[s], nn = P.node_off[s + 1] - n0;
+    const std::vector<double> t(v, v + 3 * (size_t)nn);
+    for (uint32_t a = 0; a < nn; ++a)
+        for (int d = 0; d < 3; ++d) v[3 * (size_t)a + d] = t[3 * (size_t)(P.ninv[n0 + a] - n0) + d];
+}
+
 int copy_sub(femgpu_problem &H, uint32_t s, const double *src, double *dst_host, size_t n) {
     Problem &P = H.P;
     if (s >= P.n_sub || !dst_host || n != 3 * (size_t)(P.node_off[s + 1] - P.node_off[s]))
         return fail(FEMGPU_BAD_INPUT, "get: size");
     HIPCHK(hipStreamSynchronize(P.stream));
     HIPCHK(hipMemcpy(dst_host, src + 3 * (size_t)P.node_off[s], n * 8, hipMemcpyDeviceToHost));
+    to_api_order(P, s, dst_host);
     return FEMGPU_OK;
 }
 
@@ -715,15 +756,24 @@ int femgpu_get_result(femgpu_problem *h, double *u, size_t n) {
 
 int femgpu_get_result_device(femgpu_problem *h, double *d_u) {
     if (!h || !d_u) return fail(FEMGPU_BAD_INPUT, "get_result_device: null");
-    HIPCHK(hipMemcpyAsync(d_u, h->P.d_u, 3 * (size_t)h->P.n_nodes * 8, hipMemcpyDeviceToDevice, h->P.stream));
+    if (h->P.d_ninv) HIPCHK(launch_permute_nodes(d_u, h->P.d_u, h->P.d_ninv, h->P.n_nodes, h->P.stream));
+    else HIPCHK(hipMemcpyAsync(d_u, h->P.d_u, 3 * (size_t)h->P.n_nodes * 8, hipMemcpyDeviceToDevice, h->P.stream));
     HIPCHK(hipStreamSynchronize(h->P.stream));
     return FEMGPU_OK;
 }
 
 int femgpu_set_result(femgpu_problem *h, const double *u, size_t n) {
     if (!h || !u || n != 3 * (size_t)h->P.n_nodes) return fail(FEMGPU_BAD_INPUT, "set_result: size");
-    HIPCHK(hipStreamSynchronize(h->P.stream));
-    HIPCHK(hipMemcpy(h->P.d_u, u, n * 8, hipMemcpyHostToDevice));
+    const Problem &P = h->P;
+    std::vector<double> t;
+    if (!P.nperm.empty()) {
+        t.resize(n);
+        for (uint32_t i = 0; i < P.n_nodes; ++i)
+            for (int d = 0; d < 3; ++d) t[3 * (size_t)i + d] = u[3 * (size_t)P.nperm[i] + d];
+        u = t.data();
+    }
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipMemcpy(P.d_u, u, n * 8, hipMemcpyHostToDevice));
     return FEMGPU_OK;
 }
 
@@ -735,17 +785,30 @@ int femgpu_get_sollicitation(femgpu_problem *h, double *F, size_t n) {
 int femgpu_get_matrix(femgpu_problem *h, uint32_t *row_ptr, uint32_t *col, double *vals) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "get_matrix: null");
     Problem &P = h->P;
-    if (row_ptr) std::memcpy(row_ptr, P.pat.row_ptr.data(), P.pat.row_ptr.size() * 4);
-    if (col) std::memcpy(col, P.pat.col.data(), P.pat.col.size() * 4);
+    const Pattern &pat = P.pat;
+    std::vector<double> planes;
     if (vals) {
-        std::vector<double> planes(9 * P.ps);
+        planes.resize(9 * P.ps);
         HIPCHK(hipStreamSynchronize(P.stream));
         HIPCHK(hipMemcpy(planes.data(), P.d_K, planes.size() * 8, hipMemcpyDeviceToHost));
-        for (uint32_t r = 0; r < P.n_nodes; ++r)
-            for (uint32_t p = P.pat.row_ptr[r]; p < P.pat.row_ptr[r + 1]; ++p) {
-                const uint64_t q = P.pat.pos_of_slot(r, p);
-                for (int c = 0; c < 9; ++c) vals[9 * (size_t)p + c] = planes[c * P.ps + q];
-            }
+    }
+    // caller's row a = internal row ninv[a]; its blocks re-sorted by the caller's column
+    std::vector<std::pair<uint32_t, uint64_t>> row;
+    uint32_t p_out = 0;
+    if (row_ptr) row_ptr[0] = 0;
+    for (uint32_t a = 0; a < P.n_nodes; ++a) {
+        const uint32_t r = P.ninv.empty() ? a : P.ninv[a];
+        row.clear();
+        for (uint32_t p = pat.row_ptr[r]; p < pat.row_ptr[r + 1]; ++p)
+            row.push_back({P.nperm.empty() ? pat.col[p] : P.nperm[pat.col[p]], pat.pos_of_slot(r, p)});
+        if (!P.nperm.empty()) std::sort(row.begin(), row.end());
+        for (const auto &e : row) {
+            if (col) col[p_out] = e.first;
+            if (vals)
+                for (int c = 0; c < 9; ++c) vals[9 * (size_t)p_out + c] = planes[c * P.ps + e.second];
+            ++p_out;
+        }
+        if (row_ptr) row_ptr[a + 1] = p_out;
     }
     return FEMGPU_OK;
 }
@@ -762,6 +825,8 @@ int femgpu_get_ic0_factor(femgpu_problem *h, int precond, uint32_t *order, uint3
                         rc == 1 ? "get_ic0_factor: pattern not symmetric" : "get_ic0_factor: factorisation failed");
     if (n_levels) *n_levels = P.ic0.sched.n_levels();
     if (ic0_download(P, order, row_ptr, col, L)) return fail(FEMGPU_HIP_ERROR, "get_ic0_factor: download");
+    if (order && !P.nperm.empty())
+        for (uint32_t t = 0; t < P.n_nodes; ++t) order[t] = P.nperm[order[t]];
     return FEMGPU_OK;
 }
 
@@ -895,11 +960,13 @@ int write_result(femgpu_problem &H, uint32_t s, const char *path) {
     HIPCHK(hipStreamSynchronize(P.stream));
     HIPCHK(hipMemcpy(coords.data(), P.d_coords + 3 * (size_t)n0, coords.size() * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(u.data(), P.d_u + 3 * (size_t)n0, u.size() * 8, hipMemcpyDeviceToHost));
+    to_api_order(P, s, coords.data());
+    to_api_order(P, s, u.data());
     HIPCHK(hipMemcpy(E.data(), P.d_E + e0, E.size() * 8, hipMemcpyDeviceToHost));
     int rc = after_solve_host(&H, e0, ne, eps.data(), sig.data(), tr.data());
     if (rc) return rc;
     std::vector<uint32_t> conn(P.conn_h.begin() + (size_t)e0 * P.npe, P.conn_h.begin() + (size_t)(e0 + ne) * P.npe);
-    for (uint32_t &v : conn) v -= n0;
+    for (uint32_t &v : conn) v = (P.nperm.empty() ? v : P.nperm[v]) - n0;
     const std::vector<uint32_t> group(ne, s);
     const double nu = P.mat_h[s].nu;
     const femgpu_vtu_field pf[] = {{"dep", 3, u.data()}};
@@ -924,6 +991,18 @@ int femgpu_elem_grad(femgpu_problem *h, const double *u, const double *lam, doub
     if (!h || !u || !lam || !g || h->P.from_bsr) return fail(FEMGPU_BAD_INPUT, "elem_grad: bad input");
     Problem &P = h->P;
     const size_t n = 3 * (size_t)P.n_nodes;
+    std::vector<double> ui, li;
+    if (!P.nperm.empty()) {  // to the internal node order
+        ui.resize(n);
+        li.resize(n);
+        for (uint32_t i = 0; i < P.n_nodes; ++i)
+            for (int d = 0; d < 3; ++d) {
+                ui[3 * (size_t)i + d] = u[3 * (size_t)P.nperm[i] + d];
+                li[3 * (size_t)i + d] = lam[3 * (size_t)P.nperm[i] + d];
+            }
+        u = ui.data();
+        lam = li.data();
+    }
     double *d_g = nullptr;
     HIPCHK(dalloc(&d_g, P.n_elem));
     hipError_t e = hipMemcpy(P.d_aux, u, n * 8, hipMemcpyHostToDevice);
@@ -945,7 +1024,8 @@ int femgpu_adjoint_device(femgpu_problem *h, const double *d_dJdu, double *d_dJd
     // lambda in d_aux: K lambda = dJ/du (K symmetric), zero initial guess
     double *d_rhs = nullptr;
     HIPCHK(dalloc(&d_rhs, n));
-    hipError_t e = hipMemcpyAsync(d_rhs, d_dJdu, n * 8, hipMemcpyDeviceToDevice, P.stream);
+    hipError_t e = P.d_nperm ? launch_permute_nodes(d_rhs, d_dJdu, P.d_nperm, P.n_nodes, P.stream)
+                             : hipMemcpyAsync(d_rhs, d_dJdu, n * 8, hipMemcpyDeviceToDevice, P.stream);
     if (e == hipSuccess) e = launch_fill(P.d_aux, n, 0.0, P.stream);
     if (e != hipSuccess) { (void)hipFree(d_rhs); return fail(FEMGPU_HIP_ERROR, hipGetErrorString(e)); }
     femgpu_solve_stats tmp{};
@@ -1131,6 +1211,40 @@ int femgpu_get_row_order(femgpu_problem *h, int *order, uint64_t *n_positions) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "get_row_order: null");
     if (order) *order = h->P.pat.perm.empty() ? ROW_ORDER_NATURAL : ROW_ORDER_BY_LENGTH;
     if (n_positions) *n_positions = h->P.pat.n_sell;
+    return FEMGPU_OK;
+}
+
+int femgpu_set_node_order(int order) {
+    if (order != NODE_ORDER_NATURAL && order != NODE_ORDER_CM) return fail(FEMGPU_BAD_INPUT, "set_node_order: 0 or 1");
+    g_node_order.store(order);
+    return FEMGPU_OK;
+}
+
+int femgpu_get_node_order(femgpu_problem *h, int *order, uint32_t *perm) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_node_order: null");
+    const Problem &P = h->P;
+    if (order) *order = P.nperm.empty() ? NODE_ORDER_NATURAL : NODE_ORDER_CM;
+    if (perm)
+        for (uint32_t i = 0; i < P.n_nodes; ++i) perm[i] = P.nperm.empty() ? i : P.nperm[i];
+    return FEMGPU_OK;
+}
+
+int femgpu_node_order(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, uint32_t *order) {
+    if (!n_brows || !row_ptr || !col || !order) return fail(FEMGPU_BAD_INPUT, "node_order: bad input");
+    Pattern pat;
+    pat.n_rows = n_brows;
+    pat.row_ptr.assign(row_ptr, row_ptr + n_brows + 1);
+    for (uint32_t r = 0; r < n_brows; ++r) {
+        bool has_diag = false;
+        for (uint32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+            if (col[p] >= n_brows) return fail(FEMGPU_BAD_INPUT, "node_order: column out of range");
+            has_diag |= col[p] == r;
+        }
+        if (!has_diag) return fail(FEMGPU_BAD_INPUT, "node_order: every row must hold its node");
+    }
+    pat.col.assign(col, col + row_ptr[n_brows]);
+    const std::vector<uint32_t> o = cm_order(pat, 0, n_brows);
+    std::memcpy(order, o.data(), 4 * (size_t)n_brows);
     return FEMGPU_OK;
 }
 

@@ -43,6 +43,10 @@ constexpr uint32_t NO_SUB = 0xffffffffu;
 enum { ROW_ORDER_AUTO = 0, ROW_ORDER_NATURAL = 1, ROW_ORDER_BY_LENGTH = 2 };
 constexpr uint32_t SORT_WINDOW_WGS = 4;
 int row_order_default();
+// Node order chosen when a mesh problem is created: NATURAL keeps the caller's numbering, CM
+// stores nodes in the Cuthill-McKee order of symrcm (cm_order) for locality.
+enum { NODE_ORDER_NATURAL = 0, NODE_ORDER_CM = 1 };
+int node_order_default();
 
 // Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format, SELL slots per loop
 // trip (1, 2 or 3) and how the full format's matrix is loaded: plain loads, non-temporal loads,
@@ -104,6 +108,11 @@ struct Problem {
     int16_t *d_col16 = nullptr;  // SELL column offsets c - r, when all fit in 16 bits (else null)
     // storage order of the block rows (null: identity): storage row t holds block row d_perm[t]
     uint32_t *d_perm = nullptr, *d_iperm = nullptr;
+    // node order (empty: the caller's): internal node i is the caller's node nperm[i], ninv the
+    // inverse; every vector, the pattern and the device mesh use the internal numbering, the API
+    // the caller's (block-diagonal over sub-problems)
+    std::vector<uint32_t> nperm, ninv;
+    uint32_t *d_nperm = nullptr, *d_ninv = nullptr;
     uint32_t *d_conn = nullptr, *d_rp = nullptr, *d_col = nullptr, *d_diag = nullptr, *d_slot = nullptr,
              *d_order = nullptr, *d_sptr = nullptr, *d_elem_sub = nullptr;
     SubMaterial *d_mat = nullptr;
@@ -162,6 +171,8 @@ hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t s);
 // out: 4 doubles per 1024 input pairs (one per wave)
 hipError_t launch_read(const double *src, size_t n, double *out, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+// dst[3 i + d] = src[3 map[i] + d] for i < n (node-major vectors)
+hipError_t launch_permute_nodes(double *dst, const double *src, const uint32_t *map, uint32_t n, hipStream_t s);
 hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
 // Largest row-group size (power of two <= 64) whose accumulator fits the LDS budget; 0 if none.
 uint32_t max_assembly_rows(const Problem &P);

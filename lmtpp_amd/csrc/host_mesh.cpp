@@ -430,6 +430,38 @@ RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe
     return G;
 }
 
+std::vector<uint32_t> cm_order(const Pattern &pat, uint32_t r0, uint32_t r1) {
+    std::vector<uint32_t> order;
+    if (r1 <= r0) return order;
+    order.reserve(r1 - r0);
+    auto degree = [&](uint32_t v) { return pat.len(v) - 1; };   // the pattern row holds v itself
+    std::vector<uint8_t> seen(r1 - r0, 0);
+    std::vector<uint32_t> nb;
+    for (;;) {
+        uint32_t start = r1;
+        for (uint32_t v = r0; v < r1; ++v)
+            if (!seen[v - r0] && (start == r1 || degree(v) < degree(start))) start = v;
+        if (start == r1) break;
+        size_t head = order.size();
+        order.push_back(start);
+        seen[start - r0] = 1;
+        while (head < order.size()) {
+            const uint32_t v = order[head++];
+            nb.clear();
+            for (uint32_t p = pat.row_ptr[v]; p < pat.row_ptr[v + 1]; ++p) {
+                const uint32_t c = pat.col[p];
+                if (c != v && c >= r0 && c < r1 && !seen[c - r0]) {
+                    seen[c - r0] = 1;
+                    nb.push_back(c);
+                }
+            }
+            std::stable_sort(nb.begin(), nb.end(), [&](uint32_t a, uint32_t b) { return degree(a) < degree(b); });
+            order.insert(order.end(), nb.begin(), nb.end());
+        }
+    }
+    return order;
+}
+
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts) {
     const uint32_t n = (uint32_t)row_ptr.size() - 1;
     std::vector<uint32_t> cut(n_parts + 1);
@@ -506,7 +538,7 @@ bool build_sym_sell(const Pattern &pat, SymSell &S) {
     return true;
 }
 
-bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S) {
+bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S, const std::vector<uint32_t> *base) {
     const uint32_t n = pat.n_rows;
     const uint32_t *rp = pat.row_ptr.data(), *col = pat.col.data();
     S = Ic0Schedule();
@@ -518,7 +550,8 @@ bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S) {
         std::vector<uint32_t> colour(n, 0xffffffffu);
         std::vector<uint64_t> seen;
         uint32_t nc = 0;
-        for (uint32_t r = 0; r < n; ++r) {
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t r = base ? (*base)[k] : k;
             seen.assign(nc / 64 + 1, 0);
             for (uint32_t p = rp[r]; p < rp[r + 1]; ++p) {
                 const uint32_t c = colour[col[p]];
@@ -532,9 +565,12 @@ bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S) {
         std::vector<uint32_t> cnt(nc + 1, 0);
         for (uint32_t r = 0; r < n; ++r) cnt[colour[r] + 1]++;
         for (uint32_t c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
-        for (uint32_t r = 0; r < n; ++r) order0[cnt[colour[r]]++] = r;
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t r = base ? (*base)[k] : k;
+            order0[cnt[colour[r]]++] = r;
+        }
     } else {
-        for (uint32_t r = 0; r < n; ++r) order0[r] = r;
+        for (uint32_t k = 0; k < n; ++k) order0[k] = base ? (*base)[k] : k;
     }
     for (uint32_t i = 0; i < n; ++i) elim0[order0[i]] = i;
     // levels in elimination order

@@ -84,6 +84,13 @@ struct RowGroups {
 RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe, const uint32_t *conn,
                            const Coloring &col, const uint32_t *iperm = nullptr);
 
+// Cuthill-McKee visiting order of the nodes of [r0, r1) (rows with no coupling outside the range)
+// as symrcm (util/symrcm.cpp:38-101) builds it: breadth-first from the node of lowest degree (the
+// first one), each node's unvisited neighbours appended in order of increasing degree, ties in
+// node order; a component the search did not reach restarts from its lowest-degree node (the
+// reference numbers only the first component).  Degrees count neighbours other than the node.
+std::vector<uint32_t> cm_order(const Pattern &pat, uint32_t r0, uint32_t r1);
+
 // Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);
 
@@ -128,7 +135,9 @@ struct Ic0Schedule {
     uint64_t pos(uint32_t t, uint32_t k) const { return sptr[t >> 6] + 64ull * k + (t & 63u); }
     uint32_t n_levels() const { return lv_ptr.empty() ? 0 : (uint32_t)lv_ptr.size() - 1; }
 };
-// Returns false when the pattern is not symmetric.
-bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S);
+// base: the nodes in the order the elimination starts from (natural order, and the visiting
+// order of the greedy colouring); null for 0..n-1.  Returns false when the pattern is not
+// symmetric.
+bool build_ic0_schedule(const Pattern &pat, int ordering, Ic0Schedule &S, const std::vector<uint32_t> *base = nullptr);
 
 }  // namespace femgpu

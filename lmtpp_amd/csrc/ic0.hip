@@ -212,7 +212,9 @@ static int ic0_prepare(Problem &P, int ordering) {
     if (I.sched.ordering == ordering && I.d_L) return 0;
     destroy_graph(P);  // captured sweeps refer to the old buffers
     ic0_free(I);
-    if (!build_ic0_schedule(P.pat, ordering, I.sched) || I.sched.n_sell >= (1ull << 32)) {
+    // natural order = the caller's node order (internal node of each API node)
+    if (!build_ic0_schedule(P.pat, ordering, I.sched, P.ninv.empty() ? nullptr : &P.ninv) ||
+        I.sched.n_sell >= (1ull << 32)) {
         I.sched = Ic0Schedule();
         return 1;
     }

@@ -327,6 +327,14 @@ class Formulation:
         A.check(self._lib.femgpu_get_assembly_rows(self._h, C.byref(r), C.byref(m)), "get_assembly_rows")
         return r.value, m.value
 
+    def node_order(self):
+        """(NODE_ORDER_NATURAL or NODE_ORDER_RCM, perm): perm[i] = the caller's node stored at
+        internal position i."""
+        o = C.c_int()
+        perm = np.zeros(self.n_nodes, dtype=np.uint32)
+        A.check(self._lib.femgpu_get_node_order(self._h, C.byref(o), _ptr(perm)), "get_node_order")
+        return o.value, perm
+
     def row_order(self):
         """(storage order of the block rows: ROW_ORDER_NATURAL or ROW_ORDER_BY_LENGTH, SELL
         positions including padding)."""

@@ -204,7 +204,7 @@ def after_solve(kind, conn, coords, E_e, nu, u):
 
 def ref_available():
     return all(os.path.exists(os.path.join(REF_DIR, b))
-               for b in ("ref_mesh_dump", "ref_tinyblock_pcg", "ref_after_solve", "ref_write_vtu"))
+               for b in ("ref_mesh_dump", "ref_tinyblock_pcg", "ref_after_solve", "ref_write_vtu", "ref_symrcm"))
 
 
 def ref_mesh(kind_name, nb_points, tmpdir):
@@ -258,6 +258,15 @@ def ref_write_vtu_hexa(nb_points, coords, u, E, nu, tmpdir):
         ne = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
         d = np.frombuffer(f.read(8 * 13 * ne), dtype=np.float64).reshape(ne, 13)
     return data, d[:, :6].copy(), d[:, 6:12].copy(), d[:, 12].copy()
+
+
+def ref_symrcm(kind_name, nb_points, tmpdir):
+    """The reference's symrcm(m) order on make_rect(kind, nb_points)."""
+    out = os.path.join(tmpdir, "symrcm.bin")
+    subprocess.check_call([os.path.join(REF_DIR, "ref_symrcm"), kind_name] + [str(v) for v in nb_points] + [out],
+                          stdout=subprocess.DEVNULL)
+    d = np.fromfile(out, dtype=np.uint32)
+    return d[2:2 + d[1]].copy()
 
 
 def ref_tinyblock_pcg(row_ptr, col, vals, b, x0, crit, tmpdir):

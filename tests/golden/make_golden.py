@@ -13,8 +13,10 @@ run where those sources exist.  Fixtures are data only (inputs and the reference
            definition_probleme_elasticity_dep_static.h:721-789 on a jittered make_rect mesh
   ref_write_vtu_hexa_3_4_2.npz         coords, conn, u, E, nu, eps, sigma, tr -> vtu (file bytes)
         -- write_mesh_vtk(filename, m) (mesh/write_mesh_vtk.h:434-656,827-831) on that mesh
+  ref_symrcm.npz                       tetra_7_6_5, hexa_7_6_5: symrcm(m) (util/symrcm.cpp:38-101)
+        node orders of make_rect meshes
 
-    python tests/golden/make_golden.py [after_solve|vtu]   (no argument: every fixture)
+    python tests/golden/make_golden.py [after_solve|vtu|symrcm]   (no argument: every fixture)
 """
 import json
 import os
@@ -72,9 +74,19 @@ def vtu_fixture(tmp):
                         eps=eps, sigma=sig, tr=tr, vtu=np.frombuffer(data, dtype=np.uint8))
 
 
+def symrcm_fixture(tmp):
+    out = {}
+    for name, nb in (("tetra", (7, 6, 5)), ("hexa", (7, 6, 5))):
+        out["%s_%d_%d_%d" % ((name,) + nb)] = O.ref_symrcm(name, nb, tmp)
+    np.savez_compressed(os.path.join(HERE, "ref_symrcm.npz"), **out)
+
+
 def main():
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "liboracle.so"])
     tmp = tempfile.mkdtemp()
+    if sys.argv[1:] == ["symrcm"]:
+        symrcm_fixture(tmp)
+        return
     if sys.argv[1:] == ["after_solve"]:
         after_solve_fixture(tmp)
         return
@@ -83,6 +95,7 @@ def main():
         return
     after_solve_fixture(tmp)
     vtu_fixture(tmp)
+    symrcm_fixture(tmp)
     for name, kind, nb in MESHES:
         c, e = O.ref_mesh(name, nb, tmp)
         np.savez_compressed(os.path.join(HERE, "ref_mesh_%s_%d_%d_%d.npz" % ((name,) + nb)), coords=c, conn=e)

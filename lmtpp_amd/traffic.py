@@ -14,6 +14,10 @@ non-temporal matrix loads (the default), full storage with plain loads, and symm
 reads the per-dispatch counters (KiB), scales each counter by the copy kernel's known bytes
 (factor = known KiB / counted KiB), and reports bytes per SpMV launch next to the algorithmic
 bytes.
+
+    python -m lmtpp_amd.traffic assembly [--n 100]
+
+is the workload of the element-loop counters (both assembly variants, three times each).
 """
 import csv
 import glob
@@ -49,6 +53,25 @@ def run(n):
         t32 = f.time_spmv(L.MIXED, 5)
         print(json.dumps({"format": fmt, "loads": nt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
                           "spmv_fp32_ms": t32}), flush=True)
+
+
+def assembly(n):
+    """Workload of the element-loop counters: Hexa8 n^3 assembled three times with row groups of
+    64 rows, then three times with colour passes."""
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, here)
+    sys.path.insert(0, os.path.join(here, "tests"))
+    import lmtpp_amd as L
+    from problems import build, lognormal_E
+
+    c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
+    f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
+    for R in (64, 0):
+        f.set_assembly_rows(R)
+        for _ in range(3):
+            f.assemble()
+        print("rows %d: assembly %.2f ms" % (R, f.assemble_ms()), flush=True)
+    f.close()
 
 
 def spmv_bytes(nnzb, n_nodes, fp32, fmt, col16=0):
@@ -107,9 +130,9 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) >= 2 and sys.argv[1] == "run":
+    if len(sys.argv) >= 2 and sys.argv[1] in ("run", "assembly"):
         n = int(sys.argv[sys.argv.index("--n") + 1]) if "--n" in sys.argv else 100
-        run(n)
+        (run if sys.argv[1] == "run" else assembly)(n)
     elif len(sys.argv) >= 4 and sys.argv[1] == "parse":
         nnzb = int(sys.argv[4]) if len(sys.argv) > 4 else 27270901
         nn = int(sys.argv[5]) if len(sys.argv) > 5 else 1030301

@@ -364,9 +364,13 @@ FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], co
 FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2],
                                             const int loads[2]);
 /* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
- * into K (read-modify-write).  rows = R (a power of two, at most *max_rows): each workgroup owns R
- * consecutive block rows, accumulates their blocks from the incident elements colour by colour in
- * LDS and writes the rows once.  The default is the largest R. */
+ * into K (read-modify-write).  rows = FEMGPU_ASSEMBLY_STAGED: the same passes into a
+ * block-contiguous copy of K (each block's 9 values adjacent; an extra K-sized buffer), then one
+ * streaming pass into the SpMV's layout.  rows = R (a power of two, at most *max_rows): each
+ * workgroup owns R consecutive block rows, accumulates their blocks from the incident elements
+ * colour by colour in LDS and writes the rows once.  Every variant sums each entry in the same
+ * (colour) order.  The default is FEMGPU_ASSEMBLY_STAGED; femgpu_autotune times them all. */
+#define FEMGPU_ASSEMBLY_STAGED 0xffffffffu
 FEMGPU_API int femgpu_set_assembly_rows(femgpu_problem *p, uint32_t rows);
 FEMGPU_API int femgpu_get_assembly_rows(femgpu_problem *p, uint32_t *rows, uint32_t *max_rows);
 /* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */

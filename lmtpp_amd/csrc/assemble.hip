@@ -103,8 +103,10 @@ __device__ inline void elem_block(const ElemLds<K> &L, int a, int b, const doubl
 
 // One colour of the element loop (formulation.h:626-647).  A team of ET<K>::TEAM lanes owns one
 // element; no two elements of a colour share a node, so the read-modify-write scatter into K and
-// F needs no atomics and its order is fixed.
-template <int K>
+// F needs no atomics and its order is fixed.  BLOCKS: K is a block-contiguous staging copy (the
+// 9 values of position p at 9 p .. 9 p + 8, one or two cache lines per block instead of nine
+// plane-strided ones), turned into planes afterwards by k_blocks_to_planes.
+template <int K, bool BLOCKS>
 __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ order, uint32_t count,
                                                   const uint32_t *__restrict__ conn, const double *__restrict__ coords,
                                                   const double *__restrict__ E, const uint32_t *__restrict__ elem_sub,
@@ -130,16 +132,17 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
         double k[9];
         elem_block<K>(L, a, b, H, k);
         const uint64_t p = slot[(size_t)e * NPE * NPE + t];
+        const uint64_t sp = BLOCKS ? 1 : ps, bp = BLOCKS ? 9 * p : p;
         if (a == b) {
             k[3] = k[1]; k[6] = k[2]; k[7] = k[5];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) Kv[c * ps + p] += k[c];
+            for (int c = 0; c < 9; ++c) Kv[c * sp + bp] += k[c];
         } else {
-            const uint64_t pt = slot[(size_t)e * NPE * NPE + b * NPE + a];
+            const uint64_t pt = slot[(size_t)e * NPE * NPE + b * NPE + a], bt = BLOCKS ? 9 * pt : pt;
 #pragma unroll
-            for (int c = 0; c < 9; ++c) Kv[c * ps + p] += k[c];
+            for (int c = 0; c < 9; ++c) Kv[c * sp + bp] += k[c];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) Kv[c * ps + pt] += k[3 * (c % 3) + c / 3];
+            for (int c = 0; c < 9; ++c) Kv[c * sp + bt] += k[3 * (c % 3) + c / 3];
         }
     }
     if (do_vec && t < NPE * 3) {
@@ -361,16 +364,31 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int K>
+// Block-contiguous staging -> the 9 value planes: one lane per position.
+__global__ __launch_bounds__(256) void k_blocks_to_planes(const double *__restrict__ Kb, double *__restrict__ Kv,
+                                                          uint64_t ps) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < ps; p += (uint64_t)gridDim.x * blockDim.x) {
+        double v[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) v[c] = Kb[9 * p + c];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Kv[c * ps + p] = v[c];
+    }
+}
+
+template <int K, bool BLOCKS>
 static hipError_t assemble_kind(Problem &P, int do_mat, int do_vec) {
     constexpr int TPW = 256 / ET<K>::TEAM;
+    double *Kt = BLOCKS ? P.d_Kb : P.d_K;
     for (uint32_t c = 0; c < P.ecol.n_colors(); ++c) {
         const uint32_t b = P.ecol.color_ptr[c], n = P.ecol.color_ptr[c + 1] - b;
         if (!n) continue;
-        hipLaunchKernelGGL(k_assemble<K>, dim3((n + TPW - 1) / TPW), dim3(256), 0, P.stream, P.d_order + b, n, P.d_conn,
-                           P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.d_slot, P.d_K, P.ps, P.d_F, do_mat, do_vec,
-                           P.d_flags);
+        hipLaunchKernelGGL((k_assemble<K, BLOCKS>), dim3((n + TPW - 1) / TPW), dim3(256), 0, P.stream, P.d_order + b, n,
+                           P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.d_slot, Kt, P.ps, P.d_F, do_mat,
+                           do_vec, P.d_flags);
     }
+    if (BLOCKS && do_mat)
+        hipLaunchKernelGGL(k_blocks_to_planes, dim3(8192), dim3(256), 0, P.stream, P.d_Kb, P.d_K, P.ps);
     return hipGetLastError();
 }
 
@@ -417,12 +435,19 @@ hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
         else assemble_rows_kind<HEXA>(P, do_mat, do_vec);
         e = hipGetLastError();
     } else {
-        if (do_mat) e = launch_fill(P.d_K, 9 * P.ps, 0.0, P.stream);
+        const bool blocks = P.asm_rows == ASM_STAGED && P.d_Kb;
+        if (do_mat) e = launch_fill(blocks ? P.d_Kb : P.d_K, 9 * P.ps, 0.0, P.stream);
         if (e == hipSuccess && do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
         if (e != hipSuccess) return e;
-        if (P.kind == TETRA) e = assemble_kind<TETRA>(P, do_mat, do_vec);
-        else if (P.kind == TETRA10) e = assemble_kind<TETRA10>(P, do_mat, do_vec);
-        else e = assemble_kind<HEXA>(P, do_mat, do_vec);
+        if (blocks) {
+            if (P.kind == TETRA) e = assemble_kind<TETRA, true>(P, do_mat, do_vec);
+            else if (P.kind == TETRA10) e = assemble_kind<TETRA10, true>(P, do_mat, do_vec);
+            else e = assemble_kind<HEXA, true>(P, do_mat, do_vec);
+        } else {
+            if (P.kind == TETRA) e = assemble_kind<TETRA, false>(P, do_mat, do_vec);
+            else if (P.kind == TETRA10) e = assemble_kind<TETRA10, false>(P, do_mat, do_vec);
+            else e = assemble_kind<HEXA, false>(P, do_mat, do_vec);
+        }
     }
     if (e != hipSuccess || !do_vec || !P.n_faces) return e;
     const int fk = face_kind_of(P.kind);

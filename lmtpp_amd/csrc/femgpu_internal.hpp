@@ -36,6 +36,7 @@ enum { SEL_ACTIVE = 0, SEL_PAUSE_REQ = 1, SEL_PAUSE_DONE = 2, SEL_ALL = 3 };
 struct SubMaterial { double nu, bf0, bf1, bf2; };
 
 constexpr uint32_t ROWS_PER_WG = 256;
+constexpr uint32_t ASM_STAGED = 0xffffffffu;
 constexpr uint32_t NO_SUB = 0xffffffffu;
 // Storage order of the block rows, chosen when a problem is created: AUTO sorts rows by length
 // within windows of SORT_WINDOW_WGS workgroups when that saves more than 3% of the SELL
@@ -158,8 +159,10 @@ struct Problem {
     Ic0 ic0;
     SymDev sym;
     // element loop variant: 0 = colour passes over all elements (read-modify-write of K),
-    // R > 0 = row groups of R rows accumulated in LDS (work lists in rg / d_gptr, d_gel)
+    // R > 0 = row groups of R rows accumulated in LDS (work lists in rg / d_gptr, d_gel),
+    // ASM_STAGED = colour passes into the block-contiguous staging copy d_Kb, then the planes
     uint32_t asm_rows = 0;
+    double *d_Kb = nullptr;
     std::vector<uint32_t> conn_h;  // host copy of the (offset) connectivity, for the work lists
     RowGroups rg;
     uint32_t *d_gptr = nullptr, *d_gel = nullptr;

@@ -318,7 +318,8 @@ class Formulation:
         return {"fp64": (f[0], u[0], nt[0]), "fp32": (f[1], u[1], nt[1])}
 
     def set_assembly_rows(self, rows):
-        """Element-loop variant: 0 = colour passes, R = row groups of R rows (see femgpu.h)."""
+        """Element-loop variant: 0 = colour passes, ASSEMBLY_STAGED = colour passes through a
+        block-contiguous copy of K, R = row groups of R rows (see femgpu.h)."""
         A.check(self._lib.femgpu_set_assembly_rows(self._h, int(rows)), "set_assembly_rows")
 
     def assembly_rows(self):

@@ -385,7 +385,7 @@ def test_assembly_variants_agree(oracle, kind):
     _, rmax = f.assembly_rows()
     assert rmax >= 8
     out = {}
-    for R in (0, rmax, rmax // 2, 8):
+    for R in (0, L._capi.ASSEMBLY_STAGED, rmax, rmax // 2, 8):
         f.set_assembly_rows(R)
         f.assemble()
         rp, col, vals = f.get_mat()
@@ -394,6 +394,8 @@ def test_assembly_variants_agree(oracle, kind):
         assert np.abs(vals - ovals).max() <= 1e-13 * scale, R
         assert np.abs(F - oF).max() <= 1e-13 * np.abs(oF).max(), R
         out[R] = (vals, F)
+    staged = out[L._capi.ASSEMBLY_STAGED]
+    assert np.array_equal(staged[0], out[0][0]) and np.array_equal(staged[1], out[0][1])   # same adds, same order
     for R, (v, F) in out.items():
         assert np.abs(v - out[0][0]).max() <= 1e-15 * np.abs(out[0][0]).max(), R
         assert np.abs(F - out[0][1]).max() <= 1e-15 * np.abs(out[0][1]).max(), R

@@ -342,8 +342,25 @@ FEMGPU_API int femgpu_stream_read(size_t n, int reps, double *avg_ms);
 /* Block-SpMV storage formats: FEMGPU_SPMV_FULL stores both triangles (each row all its blocks);
  * FEMGPU_SPMV_SYM stores each row's upper blocks and reads the lower ones transposed from their
  * owner row, as MatWithTinyBlocks::partial_mul uses each stored block twice
- * (containers/MatWithTinyBlocks.h:303-378). */
-typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1 } femgpu_spmv_format;
+ * (containers/MatWithTinyBlocks.h:303-378); FEMGPU_SPMV_MATRIX_FREE (Hexa problems with one
+ * sub-problem) evaluates K x element by element from the mesh with the element loop's quadrature
+ * (formulation.h:626-647), the material and penalties of the last matrix assembly, colour by colour
+ * without atomics; residual products of that precision keep the full storage with the configured
+ * unroll and loads. */
+typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1, FEMGPU_SPMV_MATRIX_FREE = 2 } femgpu_spmv_format;
+/* y = K x (n = 3 n_nodes, caller's node order) with the current product variant of precision
+ * FEMGPU_FP64 or the FP32 one (x rounded to float, y widened): the operator the solver iterates
+ * with, as Mat * Vec computes the stored matrix's product (containers/MatWithTinyBlocks.h:303-378). */
+FEMGPU_API int femgpu_apply(femgpu_problem *p, int precision, const double *x, double *y, size_t n);
+/* The matrix-free product's layout: info[0] element blocks (256 element slots each, sorted along a
+ * Morton curve), [1] sub-colours, [2] block-local node entries, [3] nodes shared by several blocks,
+ * [4] their partial slots, [5] nodes of no element. */
+FEMGPU_API int femgpu_matrix_free_info(femgpu_problem *p, uint64_t info[6]);
+/* Average device times (ms) of the matrix-free element-block kernel alone and of the whole product
+ * (element blocks, shared nodes, penalties) of precision FEMGPU_FP64 or the FP32 one, over reps
+ * launches each on the problem's stream (needs an assembled matrix). */
+FEMGPU_API int femgpu_time_matrix_free(femgpu_problem *p, int precision, int reps, double *block_ms,
+                                       double *product_ms);
 /* Times every block-SpMV variant (format x load kind x SELL slots per loop trip 1, 2, 3) on this assembled
  * matrix, for FP64 and for the FP32 copy, and keeps the fastest of each; *unroll receives the
  * FP64 pick (may be NULL).  Then times the element-loop variants (femgpu_set_assembly_rows) and
@@ -354,7 +371,8 @@ FEMGPU_API int femgpu_autotune(femgpu_problem *p, int *unroll);
  * offset fits, e.g. lattice numberings). */
 typedef enum { FEMGPU_LOADS_PLAIN = 0, FEMGPU_LOADS_NT = 1, FEMGPU_LOADS_NT_COL16 = 2 } femgpu_spmv_loads;
 /* Current (or forced) variant per precision: index 0 FP64, 1 FP32; format femgpu_spmv_format,
- * unroll 1..3, loads femgpu_spmv_loads (full format only; NULL = plain). */
+ * unroll 1..3, loads femgpu_spmv_loads (not with the symmetric format; NULL = plain; with the
+ * matrix-free format, unroll and loads are those of the residual products). */
 FEMGPU_API int femgpu_get_spmv_config(femgpu_problem *p, int format[2], int unroll[2], int loads[2]);
 FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], const int unroll[2],
                                       const int loads[2]);

@@ -16,7 +16,7 @@ TRI3, TRI6, QUAD4 = 0, 1, 2
 FP64, MIXED = 0, 1
 JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED = 0, 1, 2, 3
 REL_L2, ABS_MAXNORM = 0, 1
-SPMV_FULL, SPMV_SYM = 0, 1
+SPMV_FULL, SPMV_SYM, SPMV_MATRIX_FREE = 0, 1, 2
 LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
 ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
 NODE_ORDER_NATURAL, NODE_ORDER_RCM = 0, 1
@@ -97,6 +97,9 @@ _SIGS = {
     "femgpu_last_error": (C.c_char_p, []),
     "femgpu_get_assemble_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_time_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "femgpu_apply": (C.c_int, [_P, C.c_int, _P, _P, C.c_size_t]),
+    "femgpu_matrix_free_info": (C.c_int, [_P, _P]),
+    "femgpu_time_matrix_free": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "femgpu_autotune": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "femgpu_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "femgpu_stream_read": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),

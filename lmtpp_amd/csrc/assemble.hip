@@ -35,23 +35,6 @@ struct ElemLds {
     double N[ET<K>::NGP][ET<K>::NPE];
 };
 
-// det J and J^-1 by cofactors (exmatrix.cpp:98-125), J[j][i] = dx_i/dxi_j.
-__device__ inline double jac_inverse(const double (&J)[3][3], double (&Ji)[3][3]) {
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const double id = 1.0 / det;
-    Ji[0][0] = c00 * id; Ji[1][0] = c01 * id; Ji[2][0] = c02 * id;
-    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-    return det;
-}
-
 // Phase B alone, coordinates already in L.x (written before the barrier this function starts with).
 template <int K>
 __device__ inline void elem_gradients(ElemLds<K> &L, int t, bool valid, int *flags);

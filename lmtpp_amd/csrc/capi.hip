@@ -85,6 +85,7 @@ void free_problem(Problem &P) {
     destroy_graph(P);
     ic0_free(P.ic0);
     sym_free(P.sym);
+    mf_free(P.mf);
     dfree(P.d_coords); dfree(P.d_E); dfree(P.d_conn); dfree(P.d_rp); dfree(P.d_col); dfree(P.d_diag); dfree(P.d_col16);
     dfree(P.d_slot); dfree(P.d_order); dfree(P.d_sptr); dfree(P.d_elem_sub); dfree(P.d_mat);
     dfree(P.d_K); dfree(P.d_F); dfree(P.d_u); dfree(P.d_K32);
@@ -440,6 +441,7 @@ int assemble(femgpu_problem &H, int do_mat, int do_vec) {
     HIPCHK(launch_assemble(P, do_mat, do_vec));
     HIPCHK(launch_constraints(P, do_mat, do_vec));
     HIPCHK(hipEventRecord(P.ev[1], P.stream));
+    if (do_mat) HIPCHK(mf_snapshot(P));
     HIPCHK(hipMemcpyAsync(P.h_flags, P.d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, P.stream));
     HIPCHK(hipMemcpyAsync(P.h_ps, P.d_ps, sizeof(ProbScalars) * P.n_sub, hipMemcpyDeviceToHost, P.stream));
     HIPCHK(hipStreamSynchronize(P.stream));
@@ -1092,6 +1094,53 @@ int femgpu_time_spmv(femgpu_problem *h, int precision, int reps, double *avg_ms)
     return FEMGPU_OK;
 }
 
+int femgpu_apply(femgpu_problem *h, int precision, const double *x, double *y, size_t n) {
+    if (!h || !x || !y || precision < 0 || precision > 1 || n != 3 * (size_t)h->P.n_nodes)
+        return fail(FEMGPU_BAD_INPUT, "apply: bad input");
+    Problem &P = h->P;
+    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "apply: matrix not assembled");
+    std::vector<double> t;
+    if (!P.nperm.empty()) {
+        t.resize(n);
+        for (uint32_t i = 0; i < P.n_nodes; ++i)
+            for (int d = 0; d < 3; ++d) t[3 * (size_t)i + d] = x[3 * (size_t)P.nperm[i] + d];
+        x = t.data();
+    }
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipMemcpy(P.d_aux, x, n * 8, hipMemcpyHostToDevice));
+    const int rc = apply_spmv(P, precision, P.d_aux, P.d_q);
+    if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "apply: product variant");
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipMemcpy(y, P.d_q, n * 8, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < P.n_sub; ++s) to_api_order(P, s, y + 3 * (size_t)P.node_off[s]);
+    return FEMGPU_OK;
+}
+
+int femgpu_time_matrix_free(femgpu_problem *h, int precision, int reps, double *block_ms, double *product_ms) {
+    if (!h || reps < 1 || precision < 0 || precision > 1) return fail(FEMGPU_BAD_INPUT, "time_matrix_free: bad input");
+    const int rc = mf_time(h->P, precision, reps, block_ms, product_ms);
+    if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "time_matrix_free: not a Hexa problem with one "
+                                                                        "sub-problem and an assembled matrix");
+    return FEMGPU_OK;
+}
+
+int femgpu_matrix_free_info(femgpu_problem *h, uint64_t info[6]) {
+    if (!h || !info) return fail(FEMGPU_BAD_INPUT, "matrix_free_info: null");
+    const int rc = mf_build(h->P);
+    if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "matrix_free_info: not a Hexa problem with one "
+                                                                        "sub-problem");
+    const MatFree &M = h->P.mf;
+    uint32_t n_local = 0;
+    HIPCHK(hipMemcpy(&n_local, M.d_lptr + M.n_blocks, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    info[0] = M.n_blocks;
+    info[1] = M.n_colors;
+    info[2] = n_local;
+    info[3] = M.n_bnodes;
+    info[4] = M.n_halo;
+    info[5] = M.n_iso;
+    return FEMGPU_OK;
+}
+
 // Calibration kernels on a fresh buffer of n doubles: plain copy (16 n bytes moved per launch) or
 // streaming read (8 n bytes).
 static int stream_bench(size_t n, int reps, double *avg_ms, bool read_only) {
@@ -1190,10 +1239,12 @@ int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unr
     if (!h || !format || !unroll) return fail(FEMGPU_BAD_INPUT, "set_spmv_config: null");
     for (int i = 0; i < 2; ++i) {
         const int ld = loads ? loads[i] : LOAD_PLAIN;
-        if (format[i] < 0 || format[i] > 1 || unroll[i] < 1 || unroll[i] > 3 || ld < 0 || ld > 2 ||
-            (ld != LOAD_PLAIN && format[i] != FMT_FULL) || (ld == LOAD_NT16 && !h->P.d_col16))
-            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1, unroll 1..3, loads 0..2 (1, 2 with format 0; "
-                                          "2 needs every column offset in 16 bits)");
+        if (format[i] < 0 || format[i] > 2 || unroll[i] < 1 || unroll[i] > 3 || ld < 0 || ld > 2 ||
+            (ld != LOAD_PLAIN && format[i] == FMT_SYM) || (ld == LOAD_NT16 && !h->P.d_col16) ||
+            (format[i] == FMT_MF && !mf_eligible(h->P)))
+            return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1/2, unroll 1..3, loads 0..2 (1, 2 not with "
+                                          "format 1; 2 needs every column offset in 16 bits; format 2 needs a Hexa "
+                                          "problem with one sub-problem)");
     }
     Problem &P = h->P;
     if ((format[0] == FMT_SYM || format[1] == FMT_SYM) && !P.sym.built) {

@@ -100,6 +100,25 @@ __device__ void wg_combine(const double *partials, unsigned stride, unsigned w0,
 
 __device__ inline bool finite(double v) { return v == v && fabs(v) <= 1.7976931348623157e308; }
 
+// det J and J^-1 by cofactors (exmatrix.cpp:98-125), J[j][i] = dx_i/dxi_j, so that
+// Ji[j][i] = dxi_i/dx_j and the physical gradient is grad_j = sum_i Ji[j][i] d/dxi_i.
+template <typename T>
+__device__ inline T jac_inverse(const T (&J)[3][3], T (&Ji)[3][3]) {
+    const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const T id = T(1) / det;
+    Ji[0][0] = c00 * id; Ji[1][0] = c01 * id; Ji[2][0] = c02 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return det;
+}
+
 struct WgTables {
     const uint32_t *row0, *row1, *sub, *sub_wg0, *sub_nwg;
 };

@@ -52,7 +52,10 @@ int node_order_default();
 // Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format, SELL slots per loop
 // trip (1, 2 or 3) and how the full format's matrix is loaded: plain loads, non-temporal loads,
 // or non-temporal loads with 16-bit column offsets (c - r; only when every offset fits).
-enum { FMT_FULL = 0, FMT_SYM = 1 };
+// FMT_MF: the product is evaluated element by element from the mesh (Hexa, one sub-problem; see
+// MatFree); the residual products of that precision keep the full storage with its unroll and
+// loads.
+enum { FMT_FULL = 0, FMT_SYM = 1, FMT_MF = 2 };
 enum { LOAD_PLAIN = 0, LOAD_NT = 1, LOAD_NT16 = 2 };
 struct SpmvConfig {
     int unroll[2] = {2, 2};
@@ -69,6 +72,37 @@ struct SymDev {
     double *d_K = nullptr;
     float *d_K32 = nullptr;
     bool built = false, valid[2] = {false, false};
+};
+
+// Matrix-free product y = K x of a Hexa problem with one sub-problem: K_e x_e evaluated per element
+// with the element loop's quadrature.  Elements are sorted along a Morton curve of their centres
+// and cut into blocks of MF_BLOCK; a workgroup sums its block's nodal forces in LDS (sub-colour by
+// sub-colour, so every sum has a fixed order) and stores the nodes all of whose elements are in
+// the block; a node shared by several blocks gets one partial per block, summed in block order by
+// a second pass.  Then the penalty diagonal.  Mesh arrays are built once; the material and
+// penalties are those of the last matrix assembly, so the operator is the assembled K up to
+// rounding even when the material changed later (assume_constant_matrix).
+constexpr uint32_t MF_BLOCK = 256;
+struct MatFree {
+    bool built = false, snap = false;
+    uint32_t n_blocks = 0, n_colors = 0, max_local = 0, n_bnodes = 0, n_halo = 0, n_iso = 0;
+    uint32_t n_slots = 0, n_fix = 0, cap_fix = 0, cap_slots = 0;
+    uint32_t *d_conn = nullptr;      // 8 nodes per block slot (MF_BLOCK slots per block)
+    uint16_t *d_loc = nullptr;       // 8 block-local node indices per slot
+    uint8_t *d_color = nullptr;      // sub-colour per slot (0xff: empty slot)
+    uint32_t *d_elem = nullptr;      // element per slot (0 for empty slots)
+    uint32_t *d_lptr = nullptr;      // block b: local nodes [lptr[b], lptr[b + 1])
+    uint32_t *d_lnode = nullptr;     // global node of each local node
+    uint32_t *d_lslot = nullptr;     // its partial slot when shared with another block, else ~0
+    uint32_t *d_bnode = nullptr, *d_bptr = nullptr;  // shared node k: partials [bptr[k], bptr[k + 1])
+    double *d_halo = nullptr;        // 3 values per partial slot (FP64, or FP32 in its first half)
+    uint32_t *d_iso = nullptr;       // nodes of no element (their products are the penalty alone)
+    double *d_E = nullptr;           // E per element at the last matrix assembly
+    SubMaterial *d_mat = nullptr;    // material at the last matrix assembly
+    uint32_t *d_fdof = nullptr;      // constrained DOFs ...
+    double *d_fpen = nullptr;        // ... and their summed penalty M * P at the last matrix assembly
+    double *d_partials = nullptr;    // one p.q partial per workgroup (element blocks, then penalties)
+    uint32_t *d_wg0 = nullptr, *d_nwg = nullptr;  // {0}, {n_slots}: the finalize's slot range
 };
 
 // Device side of the block IC(0) preconditioner (schedule in host_mesh.hpp).
@@ -158,6 +192,7 @@ struct Problem {
     const double *graph_x = nullptr;   // solution vector the captured chunk updates
     Ic0 ic0;
     SymDev sym;
+    MatFree mf;
     // element loop variant: 0 = colour passes over all elements (read-modify-write of K),
     // R > 0 = row groups of R rows accumulated in LDS (work lists in rg / d_gptr, d_gel),
     // ASM_STAGED = colour passes into the block-contiguous staging copy d_Kb, then the planes
@@ -191,6 +226,9 @@ struct SolveOut { int status, iters, refreshes; double rel_res, max_abs_res, ms;
 // Solves every sub-problem; out[s] per sub-problem; returns the first non-zero status.
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out);
 double time_spmv(Problem &P, int precision, int reps);
+// d_y = K d_x (device, internal order) with the product variant of precision 0 (FP64) or 1 (FP32);
+// returns 0, 1 (the variant cannot serve this problem) or 5.
+int apply_spmv(Problem &P, int precision, const double *d_x, double *d_y);
 // Preconditioner precond (femgpu_precond) of the current K with its FP32 copy, as get_precond
 // does before the solves that reuse it; returns 0, 1 (bad input), 3 (non-positive pivot) or 5.
 int precond_build(Problem &P, int precond);
@@ -202,6 +240,23 @@ int autotune_spmv(Problem &P);
 int sym_ensure(Problem &P, int precision);
 void sym_free(SymDev &S);
 void destroy_graph(Problem &P);
+
+// ---- matrix-free product (matfree.hip) ----
+bool mf_eligible(const Problem &P);
+// Colour-order mesh arrays (once per problem); returns 0, 1 (not a Hexa problem with one
+// sub-problem) or 5.
+int mf_build(Problem &P);
+// Material and penalties of the matrix assembly just launched (stream order); eligible problems only.
+hipError_t mf_snapshot(Problem &P);
+// Builds what the product needs outside any graph capture: returns 0, 1 (not eligible, or no
+// matrix assembled yet) or 5.
+int mf_prepare(Problem &P);
+// y = K x (pq: on the active sub-problem, with the p.q partials in mf.d_partials).
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq);
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq);
+void mf_free(MatFree &M);
+// Device time (ms) of the element-block kernel alone and of the whole product; returns 0, 1 or 5.
+int mf_time(Problem &P, int precision, int reps, double *block_ms, double *product_ms);
 
 // ---- block IC(0) (ic0.hip) ----
 // Schedule for `ordering` (built and uploaded when it changes) and the numeric factor of the

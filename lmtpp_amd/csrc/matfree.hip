@@ -1,0 +1,600 @@
+// Matrix-free block product y = K x for Hexa problems on gfx950.
+//
+// The assembled FP32 product reads about 1 KB of matrix per node (27 blocks of 36 bytes) and runs
+// at the HBM read bound.  The same product evaluated element by element reads 32 bytes of
+// connectivity and one modulus per element, plus the coordinates and x of its nodes, and spends
+// about 1.5k vector instructions instead: K_e x_e is the element loop's quadrature of the
+// elasticity weak form (formulation.h:626-647 with formulations/formulation_elasticity_iso.py:45-60,
+// 2x2x2 Gauss points of formulation.h:2917-2933), applied without forming K_e:
+//   per Gauss point q: G = grad u (3x3), sigma = H eps(G), f_a += w_q detJ_q sigma grad N_a,
+// and, the Hexa shape functions being products phi(xi0) phi(xi1) phi(xi2), every reference
+// derivative at the 8 Gauss points comes from 1D interpolations of lattice differences (sum
+// factorisation), the transposed projection the same way.  With adj = detJ J^-1, the Gauss point
+// term is (w_q / detJ) sigma(Gr adj^T) adj: one division per point.
+//
+// Scatter without atomics: elements sorted along a Morton curve of their centres form blocks of
+// MF_BLOCK, one workgroup each (compact bricks on lattices).  The workgroup adds its elements'
+// nodal forces into LDS sub-colour by sub-colour (no two elements of a sub-colour share a node),
+// then stores the nodes whose elements all lie in the block and, for a node shared with other
+// blocks, its partial in a slot of its own; a node-parallel pass sums each shared node's partials
+// in block order.  Every sum has a fixed order: the product is deterministic.  The penalty
+// diagonal of the constraints follows in a pass over the constrained DOFs.
+#include "device_common.hpp"
+#include "generated/elasticity_iso.inc"
+
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+
+namespace femgpu {
+
+namespace {
+
+// Node of the Hexa at corner (x, y, z) of the reference cube: the generated shape function N_a of
+// formulations/element_Hexa.py is phi_x(xi0) phi_y(xi1) phi_z(xi2) with phi_0(t) = 1 - t,
+// phi_1(t) = t, so N_a = 1 at that corner.
+constexpr int HN[2][2][2] = {{{0, 4}, {3, 7}}, {{1, 5}, {2, 6}}};
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+
+}  // namespace
+
+// 1D abscissae of the element loop's 2x2x2 Gauss rule, from its generated table (every weight 1/8).
+__device__ inline void hexa_gauss(double &g0, double &g1, double &w) {
+    double xi[3];
+    gen::vol<HEXA>::gauss(0, xi, w);
+    g0 = xi[0];
+    gen::vol<HEXA>::gauss(6, xi, w);
+    g1 = xi[0];
+}
+
+// V[s][t] = sum_{p,q} D[p][q] phi_p(g_s) phi_q(g_t): values at the 2x2 Gauss points of a bilinear
+// field given at the 2x2 lattice corners.
+template <typename T>
+__device__ inline void interp2(const T (&D)[2][2], const T (&g)[2], T (&V)[2][2]) {
+    T e[2][2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const T dd = D[1][q] - D[0][q];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) e[s][q] = D[0][q] + dd * g[s];
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const T dd = e[s][1] - e[s][0];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) V[s][t] = e[s][0] + dd * g[t];
+    }
+}
+
+// Transpose of interp2: R[p][q] = sum_{s,t} phi_p(g_s) phi_q(g_t) P[s][t], phi_p(g_s) = w1[p][s].
+template <typename T>
+__device__ inline void project2(const T (&P)[2][2], const T (&w1)[2][2], T (&R)[2][2]) {
+    T r[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) r[s][q] = P[s][0] * w1[q][0] + P[s][1] * w1[q][1];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) R[p][q] = r[0][q] * w1[p][0] + r[1][q] * w1[p][1];
+}
+
+// det J and adj = det J * J^-1, in jac_inverse's convention (J[j][i] = dx_i/dxi_j).
+template <typename T>
+__device__ inline T adjugate(const T (&J)[3][3], T (&A)[3][3]) {
+    A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    return J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+}
+
+// f[a][c] = (K_e u_e) of one Hexa with node coordinates X and displacements u; returns nothing,
+// H as (H11 - H12, H12, H44 / 2).
+template <typename T>
+__device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T h01, T h1, T h2, T (&f)[8][3]) {
+    double g0d, g1d, wq;
+    hexa_gauss(g0d, g1d, wq);
+    const T g[2] = {(T)g0d, (T)g1d};
+    const T w1[2][2] = {{(T)(1.0 - g0d), (T)(1.0 - g1d)}, {(T)g0d, (T)g1d}};
+    // Jd[d][s][t][c] = d x_c / d xi_d and Ud[d][s][t][c] = d u_c / d xi_d at the Gauss points
+    // (s, t) of the two other axes (d = 0: (y, z), 1: (x, z), 2: (x, y)); the coordinate
+    // differences are taken in FP64 before rounding to T.
+    T Jd[3][2][2][3], Ud[3][2][2][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T DX[3][2][2], DU[3][2][2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                DX[0][p][q] = (T)(X[HN[1][p][q]][c] - X[HN[0][p][q]][c]);
+                DX[1][p][q] = (T)(X[HN[p][1][q]][c] - X[HN[p][0][q]][c]);
+                DX[2][p][q] = (T)(X[HN[p][q][1]][c] - X[HN[p][q][0]][c]);
+                DU[0][p][q] = u[HN[1][p][q]][c] - u[HN[0][p][q]][c];
+                DU[1][p][q] = u[HN[p][1][q]][c] - u[HN[p][0][q]][c];
+                DU[2][p][q] = u[HN[p][q][1]][c] - u[HN[p][q][0]][c];
+            }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T V[2][2], W[2][2];
+            interp2(DX[d], g, V);
+            interp2(DU[d], g, W);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    Jd[d][s][t][c] = V[s][t];
+                    Ud[d][s][t][c] = W[s][t];
+                }
+        }
+    }
+    // Gauss point (i, j, k): S = (w / detJ) sigma(Gr adj^T) adj, summed into Pd over the axis each
+    // reference derivative does not depend on.
+    T Pd[3][2][2][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) Pd[d][s][t][c] = 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                T J[3][3], Gr[3][3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    J[0][c] = Jd[0][j][k][c];
+                    J[1][c] = Jd[1][i][k][c];
+                    J[2][c] = Jd[2][i][j][c];
+                    Gr[c][0] = Ud[0][j][k][c];
+                    Gr[c][1] = Ud[1][i][k][c];
+                    Gr[c][2] = Ud[2][i][j][c];
+                }
+                T Aj[3][3];
+                const T kq = (T)wq / adjugate(J, Aj);
+                T G[3][3];  // detJ * d u_c / d x_m
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) G[c][m] = Gr[c][0] * Aj[m][0] + Gr[c][1] * Aj[m][1] + Gr[c][2] * Aj[m][2];
+                const T tr = G[0][0] + G[1][1] + G[2][2];
+                const T ks = kq * h2;
+                T sg[3][3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sg[c][c] = kq * (h01 * G[c][c] + h1 * tr);
+                sg[0][1] = sg[1][0] = ks * (G[0][1] + G[1][0]);
+                sg[0][2] = sg[2][0] = ks * (G[0][2] + G[2][0]);
+                sg[1][2] = sg[2][1] = ks * (G[1][2] + G[2][1]);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    Pd[0][j][k][c] += sg[c][0] * Aj[0][0] + sg[c][1] * Aj[1][0] + sg[c][2] * Aj[2][0];
+                    Pd[1][i][k][c] += sg[c][0] * Aj[0][1] + sg[c][1] * Aj[1][1] + sg[c][2] * Aj[2][1];
+                    Pd[2][i][j][c] += sg[c][0] * Aj[0][2] + sg[c][1] * Aj[1][2] + sg[c][2] * Aj[2][2];
+                }
+            }
+    // f_a = sum_d phi'_{a_d} (projection of Pd[d] onto the two other lattice axes)
+    T Rd[3][2][2][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T Pin[2][2], Rout[2][2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) Pin[s][t] = Pd[d][s][t][c];
+            project2(Pin, w1, Rout);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) Rd[d][s][t][c] = Rout[s][t];
+        }
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+        for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+            for (int az = 0; az < 2; ++az)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    f[HN[ax][ay][az]][c] = (ax ? Rd[0][ay][az][c] : -Rd[0][ay][az][c]) +
+                                           (ay ? Rd[1][ax][az][c] : -Rd[1][ax][az][c]) +
+                                           (az ? Rd[2][ax][ay][c] : -Rd[2][ax][ay][c]);
+}
+
+// One block of MF_BLOCK element slots per workgroup (logical block b = xcd_slab(), so each XCD
+// takes a contiguous run of blocks): y of the block's own nodes, partials of its shared ones.
+// PQ: partial of sum_e x_e . (K_e x_e), whose sum with the penalty pass's is x . K x.
+template <typename T, bool PQ>
+__global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
+                                                  const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
+                                                  uint32_t n_colors, const uint32_t *__restrict__ lptr,
+                                                  const uint32_t *__restrict__ lnode, const uint32_t *__restrict__ lslot,
+                                                  const double *__restrict__ coords, const double *__restrict__ E,
+                                                  const SubMaterial *__restrict__ mat, const T *__restrict__ x,
+                                                  T *__restrict__ y, T *__restrict__ halo, const ProbScalars *S,
+                                                  double *partials) {
+    extern __shared__ unsigned char mf_lds[];
+    T *acc = reinterpret_cast<T *>(mf_lds);
+    if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    const unsigned b = xcd_slab();
+    const uint32_t l0 = lptr[b], nl = lptr[b + 1] - l0;
+    for (uint32_t i = threadIdx.x; i < 3 * nl; i += MF_BLOCK) acc[i] = 0;
+    const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
+    const uint32_t col = color[slot];
+    double pq[1] = {0.0};
+    T f[8][3];
+    uint32_t lo[8];
+    if (col != 0xffu) {
+        const uint4 ca = reinterpret_cast<const uint4 *>(conn)[2 * slot];
+        const uint4 cb = reinterpret_cast<const uint4 *>(conn)[2 * slot + 1];
+        const uint32_t nd[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[slot];
+        lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
+        lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
+        double Hd[3];
+        gen::hooke(E[elem[slot]], mat[0].nu, Hd);
+        double X[8][3];
+        T u[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[a][c] = coords[3 * (size_t)nd[a] + c];
+                u[a][c] = x[3 * (size_t)nd[a] + c];
+            }
+        hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
+        if (PQ)
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pq[0] += (double)u[a][c] * (double)f[a][c];
+    }
+    __syncthreads();
+    for (uint32_t c = 0; c < n_colors; ++c) {
+        if (col == c)
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) acc[3 * lo[a] + d] += f[a][d];
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < nl; i += MF_BLOCK) {
+        const uint32_t s = lslot[l0 + i];
+        T *dst = s == 0xffffffffu ? y + 3 * (size_t)lnode[l0 + i] : halo + 3 * (size_t)s;
+        dst[0] = acc[3 * i];
+        dst[1] = acc[3 * i + 1];
+        dst[2] = acc[3 * i + 2];
+    }
+    if (PQ) wg_partial<1, 0u>(pq, partials, 0u, b);
+}
+
+// Shared nodes: y = sum of the blocks' partials, in block order.
+template <typename T, bool PQ>
+__global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__restrict__ bnode,
+                                                 const uint32_t *__restrict__ bptr, const T *__restrict__ halo,
+                                                 T *__restrict__ y, const ProbScalars *S) {
+    if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    T v0 = 0, v1 = 0, v2 = 0;
+    for (uint32_t k = bptr[i]; k < bptr[i + 1]; ++k) {
+        v0 += halo[3 * (size_t)k];
+        v1 += halo[3 * (size_t)k + 1];
+        v2 += halo[3 * (size_t)k + 2];
+    }
+    T *py = y + 3 * (size_t)bnode[i];
+    py[0] = v0;
+    py[1] = v1;
+    py[2] = v2;
+}
+
+// Nodes of no element: their rows hold the penalty alone, so the element passes leave them and
+// they start from zero.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mf_iso(uint32_t n, const uint32_t *__restrict__ nodes, T *__restrict__ y) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= 3 * n) return;
+    y[3 * (size_t)nodes[i / 3] + i % 3] = 0;
+}
+
+// Penalty diagonal: y[dof] += M P x[dof] per constrained DOF (distinct DOFs, no races).
+template <typename T, bool PQ>
+__global__ __launch_bounds__(256) void k_mf_fix(uint32_t n, const uint32_t *__restrict__ dof,
+                                                const double *__restrict__ pen, const T *__restrict__ x,
+                                                T *__restrict__ y, const ProbScalars *S, double *partials,
+                                                uint32_t slot0) {
+    if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    double acc[1] = {0.0};
+    if (i < n) {
+        const uint32_t d = dof[i];
+        const T xv = x[d];
+        const T a = (T)pen[i] * xv;
+        y[d] += a;
+        acc[0] = (double)a * (double)xv;
+    }
+    if (PQ) wg_partial<1, 0u>(acc, partials, 0u, slot0 + blockIdx.x);
+}
+
+// Summed penalty per constrained DOF as k_constraints adds it to K (M = max |diag K| of the
+// element loop, times each penalty in insertion order).
+__global__ void k_mf_pen(uint32_t n_cdof, const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ csub,
+                         const double *__restrict__ cpen, const ProbScalars *S, double *__restrict__ fpen) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_cdof) return;
+    const double M = S[csub[i]].max_diag;
+    double s = 0.0;
+    for (uint32_t k = cptr[i]; k < cptr[i + 1]; ++k) s += 1.0 * M * cpen[k];
+    fpen[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+
+bool mf_eligible(const Problem &P) {
+    return P.kind == HEXA && P.n_sub == 1 && !P.from_bsr && P.n_elem > 0 && P.conn_h.size() == 8 * (size_t)P.n_elem;
+}
+
+void mf_free(MatFree &M) {
+    dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr); dfree(M.d_lnode);
+    dfree(M.d_lslot); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_iso); dfree(M.d_E);
+    dfree(M.d_mat); dfree(M.d_fdof); dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_wg0); dfree(M.d_nwg);
+    M = MatFree();
+}
+
+namespace {
+
+// 10-bit-per-axis Morton code (bit 3k + d holds bit k of axis d).
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t m = 0;
+    for (int k = 0; k < 10; ++k) m |= ((x >> k & 1u) << (3 * k)) | ((y >> k & 1u) << (3 * k + 1)) | ((z >> k & 1u) << (3 * k + 2));
+    return m;
+}
+
+template <class V>
+bool upload_vec(V **d, const std::vector<V> &h) {
+    if (hipMalloc(reinterpret_cast<void **>(d), std::max<size_t>(1, h.size()) * sizeof(V)) != hipSuccess) return false;
+    return h.empty() || hipMemcpy(*d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+}  // namespace
+
+int mf_build(Problem &P) {
+    MatFree &M = P.mf;
+    if (M.built) return 0;
+    if (!mf_eligible(P)) return 1;
+    const uint32_t ne = P.n_elem, nn = P.n_nodes;
+    const uint32_t *conn = P.conn_h.data();
+    std::vector<double> X(3 * (size_t)nn);
+    if (hipMemcpy(X.data(), P.d_coords, X.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return 5;
+    // element order: Morton code of the element centre on the bounding box, ties by element id
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t n = 0; n < nn; ++n)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], X[3 * (size_t)n + d]);
+            hi[d] = std::max(hi[d], X[3 * (size_t)n + d]);
+        }
+    std::vector<uint64_t> key(ne);
+    for (uint32_t e = 0; e < ne; ++e) {
+        uint32_t q[3];
+        for (int d = 0; d < 3; ++d) {
+            double c = 0;
+            for (int a = 0; a < 8; ++a) c += X[3 * (size_t)conn[8 * (size_t)e + a] + d];
+            const double t = hi[d] > lo[d] ? (c / 8 - lo[d]) / (hi[d] - lo[d]) : 0.0;
+            q[d] = (uint32_t)std::min(1023.0, std::max(0.0, t * 1024.0));
+        }
+        key[e] = (uint64_t)morton3(q[0], q[1], q[2]) << 32 | e;
+    }
+    std::sort(key.begin(), key.end());
+    // blocks of MF_BLOCK consecutive elements, padded to a multiple of 8 blocks (the XCD mapping)
+    const uint32_t nb = ((ne + MF_BLOCK - 1) / MF_BLOCK + 7) / 8 * 8;
+    const size_t ns = (size_t)nb * MF_BLOCK;
+    std::vector<uint32_t> sconn(8 * ns, 0), selem(ns, 0), lptr(nb + 1, 0), lnode, deg(nn, 0), stamp(nn, 0xffffffffu),
+        lid(nn, 0), in_blk;
+    std::vector<uint16_t> sloc(8 * ns, 0);
+    std::vector<uint8_t> scol(ns, 0xffu);
+    for (size_t s = 0; s < 8 * (size_t)ne; ++s) ++deg[conn[s]];
+    uint32_t n_colors = 0, max_local = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        lptr[b] = (uint32_t)lnode.size();
+        std::vector<uint32_t> mask;
+        for (uint32_t k = 0; k < MF_BLOCK; ++k) {
+            const size_t i = (size_t)b * MF_BLOCK + k;
+            if (i >= ne) break;
+            const uint32_t e = (uint32_t)(key[i] & 0xffffffffu);
+            uint32_t used = 0;
+            for (int a = 0; a < 8; ++a) {
+                const uint32_t n = conn[8 * (size_t)e + a];
+                if (stamp[n] != b) {
+                    stamp[n] = b;
+                    lid[n] = (uint32_t)(lnode.size() - lptr[b]);
+                    lnode.push_back(n);
+                    in_blk.push_back(0);
+                    mask.push_back(0);
+                }
+                sconn[8 * i + a] = n;
+                sloc[8 * i + a] = (uint16_t)lid[n];
+                ++in_blk[lptr[b] + lid[n]];
+                used |= mask[lid[n]];
+            }
+            if (used == 0xffffffffu) return 1;  // more than 32 sub-colours: not served
+            const uint32_t c = (uint32_t)__builtin_ctz(~used);
+            for (int a = 0; a < 8; ++a) mask[lid[conn[8 * (size_t)e + a]]] |= 1u << c;
+            scol[i] = (uint8_t)c;
+            selem[i] = e;
+            n_colors = std::max(n_colors, c + 1);
+        }
+        max_local = std::max<uint32_t>(max_local, (uint32_t)lnode.size() - lptr[b]);
+    }
+    lptr[nb] = (uint32_t)lnode.size();
+    if (3ull * max_local * 8 > 64 * 1024) return 1;  // LDS accumulator of the FP64 product
+    // shared nodes: a partial slot per (node, block), grouped by node in block order
+    std::vector<uint32_t> nshared(nn, 0);
+    for (size_t l = 0; l < lnode.size(); ++l)
+        if (in_blk[l] != deg[lnode[l]]) ++nshared[lnode[l]];
+    std::vector<uint32_t> bnode, bptr(1, 0), first(nn, 0xffffffffu), iso;
+    for (uint32_t n = 0; n < nn; ++n) {
+        if (!deg[n]) iso.push_back(n);
+        if (!nshared[n]) continue;
+        first[n] = bptr.back();
+        bnode.push_back(n);
+        bptr.push_back(bptr.back() + nshared[n]);
+    }
+    std::vector<uint32_t> lslot(lnode.size(), 0xffffffffu);
+    for (size_t l = 0; l < lnode.size(); ++l)  // blocks in order, so slots follow block order
+        if (in_blk[l] != deg[lnode[l]]) lslot[l] = first[lnode[l]]++;
+    const std::vector<uint32_t> zero(1, 0);
+    if (!upload_vec(&M.d_conn, sconn) || !upload_vec(&M.d_loc, sloc) || !upload_vec(&M.d_color, scol) ||
+        !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode, lnode) ||
+        !upload_vec(&M.d_lslot, lslot) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
+        !upload_vec(&M.d_iso, iso) || !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
+        hipMalloc(&M.d_halo, sizeof(double) * 3 * std::max<size_t>(1, bptr.back())) != hipSuccess) {
+        mf_free(M);
+        return 5;
+    }
+    M.n_blocks = nb;
+    M.n_colors = n_colors;
+    M.max_local = max_local;
+    M.n_bnodes = (uint32_t)bnode.size();
+    M.n_halo = bptr.back();
+    M.n_iso = (uint32_t)iso.size();
+    M.built = true;
+    return 0;
+}
+
+hipError_t mf_snapshot(Problem &P) {
+    if (!mf_eligible(P)) return hipSuccess;
+    MatFree &M = P.mf;
+    hipError_t e = hipSuccess;
+    if (!M.d_E) e = hipMalloc(&M.d_E, sizeof(double) * P.n_elem);
+    if (e == hipSuccess && !M.d_mat) e = hipMalloc(&M.d_mat, sizeof(SubMaterial));
+    if (e == hipSuccess && P.n_cdof > M.cap_fix) {
+        dfree(M.d_fdof);
+        dfree(M.d_fpen);
+        e = hipMalloc(&M.d_fdof, sizeof(uint32_t) * P.n_cdof);
+        if (e == hipSuccess) e = hipMalloc(&M.d_fpen, sizeof(double) * P.n_cdof);
+        M.cap_fix = e == hipSuccess ? P.n_cdof : 0;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(M.d_E, P.d_E, sizeof(double) * P.n_elem, hipMemcpyDeviceToDevice, P.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(M.d_mat, P.d_mat, sizeof(SubMaterial), hipMemcpyDeviceToDevice, P.stream);
+    if (e == hipSuccess && P.n_cdof) {
+        e = hipMemcpyAsync(M.d_fdof, P.d_cdof, sizeof(uint32_t) * P.n_cdof, hipMemcpyDeviceToDevice, P.stream);
+        hipLaunchKernelGGL(k_mf_pen, dim3((P.n_cdof + 255) / 256), dim3(256), 0, P.stream, P.n_cdof, P.d_cptr, P.d_csub,
+                           P.d_cpen, P.d_ps, M.d_fpen);
+    }
+    if (e != hipSuccess) return e;
+    if (M.n_fix != P.n_cdof) destroy_graph(P);  // the penalty pass's grid is part of captured chunks
+    M.n_fix = P.n_cdof;
+    M.snap = true;
+    return hipGetLastError();
+}
+
+// Partial slots of the passes and the finalize's range; allocations happen here, never while a
+// chunk of iterations is being captured.
+int mf_prepare(Problem &P) {
+    const int rc = mf_build(P);
+    if (rc) return rc;
+    MatFree &M = P.mf;
+    if (!M.snap) return 1;
+    const uint32_t s = M.n_blocks + (M.n_fix + 255) / 256;
+    if (s > M.cap_slots) {
+        dfree(M.d_partials);
+        if (hipMalloc(&M.d_partials, sizeof(double) * std::max<uint32_t>(1, s)) != hipSuccess) {
+            M.cap_slots = 0;
+            return 5;
+        }
+        M.cap_slots = s;
+    }
+    if (s != M.n_slots) {
+        if (hipMemcpy(M.d_nwg, &s, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) return 5;
+        M.n_slots = s;
+    }
+    return 0;
+}
+
+template <typename T>
+static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq) {
+    const MatFree &M = P.mf;
+    T *halo = reinterpret_cast<T *>(M.d_halo);
+    const size_t lds = 3 * (size_t)std::max<uint32_t>(1, M.max_local) * sizeof(T);
+#define FEMGPU_MF_PASSES(PQ)                                                                                          \
+    hipLaunchKernelGGL((k_mf_block<T, PQ>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn, M.d_loc,        \
+                       M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, P.d_coords, M.d_E, M.d_mat, x, \
+                       y, halo, P.d_ps, M.d_partials);                                                                 \
+    if (M.n_bnodes)                                                                                                    \
+        hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,      \
+                           M.d_bnode, M.d_bptr, halo, y, P.d_ps);                                                      \
+    if (M.n_iso)                                                                                                       \
+        hipLaunchKernelGGL(k_mf_iso<T>, dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, M.n_iso, M.d_iso, y); \
+    if (M.n_fix)                                                                                                       \
+        hipLaunchKernelGGL((k_mf_fix<T, PQ>), dim3((M.n_fix + 255) / 256), dim3(256), 0, P.stream, M.n_fix, M.d_fdof,   \
+                           M.d_fpen, x, y, P.d_ps, M.d_partials, M.n_blocks);
+    if (pq) {
+        FEMGPU_MF_PASSES(true)
+    } else {
+        FEMGPU_MF_PASSES(false)
+    }
+#undef FEMGPU_MF_PASSES
+    return hipGetLastError();
+}
+
+// Average device time of the element-block kernel alone and of the whole product over reps launches
+// each (after one untimed product), on the problem's stream.
+int mf_time(Problem &P, int precision, int reps, double *block_ms, double *product_ms) {
+    const int rc = mf_prepare(P);
+    if (rc) return rc;
+    const MatFree &M = P.mf;
+    const size_t lds = 3 * (size_t)std::max<uint32_t>(1, M.max_local) * (precision ? 4 : 8);
+    auto block = [&]() {
+        if (precision)
+            hipLaunchKernelGGL((k_mf_block<float, false>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn,
+                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, P.d_coords, M.d_E,
+                               M.d_mat, P.d_p32, P.d_q32, reinterpret_cast<float *>(M.d_halo), P.d_ps, M.d_partials);
+        else
+            hipLaunchKernelGGL((k_mf_block<double, false>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn,
+                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, P.d_coords, M.d_E,
+                               M.d_mat, P.d_p, P.d_q, M.d_halo, P.d_ps, M.d_partials);
+    };
+    auto product = [&]() {
+        if (precision) (void)mf_launch_t<float>(P, P.d_p32, P.d_q32, false);
+        else (void)mf_launch_t<double>(P, P.d_p, P.d_q, false);
+    };
+    float ms[2] = {0, 0};
+    product();
+    for (int k = 0; k < 2; ++k) {
+        if (hipEventRecord(P.ev[2], P.stream) != hipSuccess) return 5;
+        for (int i = 0; i < reps; ++i) k ? product() : block();
+        if (hipEventRecord(P.ev[3], P.stream) != hipSuccess || hipEventSynchronize(P.ev[3]) != hipSuccess ||
+            hipEventElapsedTime(&ms[k], P.ev[2], P.ev[3]) != hipSuccess)
+            return 5;
+    }
+    if (hipGetLastError() != hipSuccess) return 5;
+    if (block_ms) *block_ms = (double)ms[0] / reps;
+    if (product_ms) *product_ms = (double)ms[1] / reps;
+    return 0;
+}
+
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq) { return mf_launch_t<float>(P, x, y, pq); }
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq) { return mf_launch_t<double>(P, x, y, pq); }
+
+}  // namespace femgpu

@@ -432,6 +432,16 @@ static void finalize(Problem &P, int what, int sel = 0) {
                        P.d_sub_nwg, sel);
 }
 
+// alpha = rho / p.q from the partials of the product just launched (the row-parallel formats
+// write one per row workgroup, the matrix-free passes one per pass workgroup).
+static void finalize_pq(Problem &P, int precision) {
+    if (P.spmv.format[precision] == FMT_MF)
+        hipLaunchKernelGGL(k_finalize, dim3(P.n_sub), dim3(1024), 0, P.stream, (int)FIN_PQ, P.d_ps, P.mf.d_partials,
+                           P.mf.n_slots, P.mf.d_wg0, P.mf.d_nwg, 0);
+    else
+        finalize(P, FIN_PQ);
+}
+
 static WgTables tables(const Problem &P) {
     return WgTables{P.d_wg_row0, P.d_wg_row1, P.d_wg_sub, P.d_sub_wg0, P.d_sub_nwg};
 }
@@ -441,6 +451,11 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     const dim3 g(P.n_wg), blk(256);
     const WgTables wt = tables(P);
     const int pr = sizeof(T) == 4 ? 1 : 0;
+    // matrix-free products; residuals keep the assembled FP64 storage (full format)
+    if (P.spmv.format[pr] == FMT_MF && MODE != MODE_RES) {
+        (void)mf_launch(P, x, y, MODE == MODE_PQ);
+        return;
+    }
     if (P.spmv.format[pr] == FMT_SYM) {
         const SymDev &Y = P.sym;
         const SpmvArgs A{Y.d_lenU, Y.d_sptrU, Y.d_colU, Y.psU, Y.d_sptrL, Y.d_lenL, Y.d_colL, Y.d_tposL, nullptr, nullptr};
@@ -525,9 +540,11 @@ int sym_ensure(Problem &P, int precision) {
     return 0;
 }
 
-// Values the SpMV of this precision reads (FP32 full copy or symmetric storage).
+// Values the SpMV of this precision reads (FP32 full copy, symmetric storage, or the matrix-free
+// arrays).
 static int spmv_ensure(Problem &P, int precision) {
     if (P.spmv.format[precision] == FMT_SYM) return sym_ensure(P, precision);
+    if (P.spmv.format[precision] == FMT_MF) return mf_prepare(P);
     if (precision == 1 && !P.k32_valid) {
         if (!P.d_K32 && hipMalloc(&P.d_K32, sizeof(float) * 9 * P.ps) != hipSuccess) return 5;
         if (launch_to_fp32(P) != hipSuccess) return 5;
@@ -557,7 +574,9 @@ double time_spmv(Problem &P, int precision, int reps) {
 // Times every SpMV variant (storage format x SELL slots per loop trip) on this matrix, for each
 // precision (10 launches after 2 warm-up launches), and keeps the fastest.  The symmetric storage
 // holds a second copy of the values, so it is kept only when it beats the full storage by more
-// than 3%; storage no precision picked is released.
+// than 3%; storage no precision picked is released.  The matrix-free product (Hexa problems with
+// one sub-problem) is kept when it beats the stored formats; the full storage's fastest variant
+// then serves the residual products.
 int autotune_spmv(Problem &P) {
     for (int pr = 0; pr < 2; ++pr) {
         double best[2] = {1e30, 1e30};
@@ -578,9 +597,21 @@ int autotune_spmv(Problem &P) {
                     }
                 }
         const int f = best[FMT_SYM] < 0.97 * best[FMT_FULL] ? FMT_SYM : FMT_FULL;
-        P.spmv.format[pr] = f;
-        P.spmv.unroll[pr] = bu[f];
-        P.spmv.loads[pr] = f == FMT_FULL ? bld : LOAD_PLAIN;
+        double best_mf = 1e30;
+        if (mf_eligible(P)) {
+            P.spmv.format[pr] = FMT_MF;
+            if (time_spmv(P, pr, 2) >= 0) best_mf = time_spmv(P, pr, 10);
+            if (!(best_mf > 0)) best_mf = 1e30;
+        }
+        if (best_mf < best[f]) {
+            P.spmv.format[pr] = FMT_MF;
+            P.spmv.unroll[pr] = bu[FMT_FULL];
+            P.spmv.loads[pr] = bld;
+        } else {
+            P.spmv.format[pr] = f;
+            P.spmv.unroll[pr] = bu[f];
+            P.spmv.loads[pr] = f == FMT_FULL ? bld : LOAD_PLAIN;
+        }
     }
     if (P.spmv.format[0] != FMT_SYM && P.spmv.format[1] != FMT_SYM) sym_free(P.sym);
     else {
@@ -589,6 +620,29 @@ int autotune_spmv(Problem &P) {
     }
     P.spmv.tuned = true;
     destroy_graph(P);
+    return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+__global__ __launch_bounds__(256) void k_vec_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(256) void k_vec_to_f64(const float *__restrict__ a, double *__restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = (double)a[i];
+}
+
+// y = K x with the product variant of this precision (FP32: x rounded to float, y widened back).
+int apply_spmv(Problem &P, int precision, const double *d_x, double *d_y) {
+    const int rc = spmv_ensure(P, precision);
+    if (rc) return rc;
+    const size_t n = 3 * (size_t)P.n_nodes;
+    const dim3 g((unsigned)std::max<size_t>(1, std::min<size_t>(8192, (n + 255) / 256))), blk(256);
+    if (precision == 1) {
+        hipLaunchKernelGGL(k_vec_to_f32, g, blk, 0, P.stream, d_x, P.d_p32, n);
+        launch_spmv_t<float, MODE_PLAIN>(P, P.d_K32, P.d_p32, nullptr, P.d_q32, 0);
+        hipLaunchKernelGGL(k_vec_to_f64, g, blk, 0, P.stream, P.d_q32, d_y, n);
+    } else {
+        launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, d_x, nullptr, d_y, 0);
+    }
     return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
@@ -625,7 +679,7 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE);
-    finalize(P, FIN_PQ);
+    finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
     hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, q, Minv, z, wt, P.d_ps, P.d_partials);
     if (PRE == PRE_IC0) {
         finalize(P, FIN_UPD_IC);
@@ -682,7 +736,7 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
     const int pr = fp32 ? 1 : 0;
     // every field in its own bits; the solution pointer is compared separately
     const uint64_t key = (uint64_t)fp32 | ((uint64_t)PRE << 1) | ((uint64_t)P.spmv.format[pr] << 3) |
-                         ((uint64_t)P.spmv.unroll[pr] << 4) | ((uint64_t)P.spmv.loads[pr] << 6) | ((uint64_t)chunk << 8);
+                         ((uint64_t)P.spmv.unroll[pr] << 5) | ((uint64_t)P.spmv.loads[pr] << 7) | ((uint64_t)chunk << 9);
     if (!P.graph_exec || P.graph_key != key || P.graph_x != d_x) {
         destroy_graph(P);
         hipGraph_t g = nullptr;

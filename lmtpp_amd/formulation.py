@@ -302,6 +302,29 @@ class Formulation:
         A.check(self._lib.femgpu_time_spmv(self._h, int(precision), int(reps), C.byref(v)), "time_spmv")
         return v.value
 
+    def apply(self, x, precision=A.FP64):
+        """K x (caller's node order) with the product variant the solver uses for precision
+        (FP64, or MIXED for the FP32 one: x rounded to float, the result widened)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        y = np.empty_like(x)
+        A.check(self._lib.femgpu_apply(self._h, int(precision), x.ctypes.data, y.ctypes.data, x.size), "apply")
+        return y
+
+    def matrix_free_info(self):
+        """Layout of the matrix-free product (Hexa, one sub-problem): element blocks, sub-colours,
+        block-local node entries, shared nodes, their partial slots, nodes of no element."""
+        info = (C.c_uint64 * 6)()
+        A.check(self._lib.femgpu_matrix_free_info(self._h, info), "matrix_free_info")
+        return dict(zip(("blocks", "colors", "local_entries", "shared_nodes", "partial_slots", "isolated_nodes"),
+                        (int(v) for v in info)))
+
+    def time_matrix_free(self, precision=A.FP64, reps=20):
+        """(element-block kernel ms, whole product ms), averaged over reps launches."""
+        b, p = C.c_double(), C.c_double()
+        A.check(self._lib.femgpu_time_matrix_free(self._h, int(precision), int(reps), C.byref(b), C.byref(p)),
+                "time_matrix_free")
+        return b.value, p.value
+
     def autotune(self):
         """Pick the fastest block-SpMV variant (storage format, SELL slots per loop trip) for this
         matrix in FP64 and FP32; returns spmv_config()."""
@@ -309,7 +332,8 @@ class Formulation:
         return self.spmv_config()
 
     def spmv_config(self):
-        """{"fp64": (format, unroll, loads), "fp32": (...)}: format SPMV_FULL or SPMV_SYM, loads
+        """{"fp64": (format, unroll, loads), "fp32": (...)}: format SPMV_FULL, SPMV_SYM or
+        SPMV_MATRIX_FREE (unroll and loads then those of the residual products), loads
         LOADS_PLAIN, LOADS_NT or LOADS_NT_COL16."""
         f = (C.c_int * 2)()
         u = (C.c_int * 2)()

@@ -7,7 +7,8 @@ Workload (run it under rocprofv3 --pmc, one counter per run):
 launches the plain copy kernel (k_copy, 2^26 doubles: exactly 512 MiB read and 512 MiB written),
 then 5 FP64 and 5 FP32 block SpMV launches on Hexa8 n^3 for each variant: full storage with
 non-temporal matrix loads (the default), full storage with plain loads, and symmetric storage
-(upper blocks stored, lower ones read transposed).  Then
+(upper blocks stored, lower ones read transposed); then the matrix-free product (its element-block
+kernel and its shared-node pass).  Then
 
     python -m lmtpp_amd.traffic parse <fetch_dir> <write_dir> > profiles/spmv_traffic.json
 
@@ -53,6 +54,11 @@ def run(n):
         t32 = f.time_spmv(L.MIXED, 5)
         print(json.dumps({"format": fmt, "loads": nt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
                           "spmv_fp32_ms": t32}), flush=True)
+    f.set_spmv_config(fp64=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT), fp32=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT))
+    b64, p64 = f.time_matrix_free(L.FP64, 5)
+    b32, p32 = f.time_matrix_free(L.MIXED, 5)
+    print(json.dumps({"format": A.SPMV_MATRIX_FREE, "info": f.matrix_free_info(), "block_fp64_ms": b64,
+                      "product_fp64_ms": p64, "block_fp32_ms": b32, "product_fp32_ms": p32}), flush=True)
 
 
 def assembly(n):
@@ -72,6 +78,32 @@ def assembly(n):
             f.assemble()
         print("rows %d: assembly %.2f ms" % (R, f.assemble_ms()), flush=True)
     f.close()
+
+
+# Arithmetic of one element in the matrix-free product as written (k_mf_block, hexa_apply; an FMA
+# counts 2): lattice differences 72, 18 bilinear interpolations x 20 = 360, per Gauss point
+# adjugate and det 32 + w/det 1 + displacement gradient 45 + trace 2 + stress 19 + projection
+# onto the adjugate 54 = 153 (x 8 = 1224), 9 transposed projections x 24 = 216, nodal forces 48,
+# Hooke law 8.
+MF_FLOPS_PER_ELEM = 72 + 360 + 1224 + 216 + 48 + 8
+
+
+def mf_block_bytes(n_elem, n_nodes, info, fp32):
+    """Algorithmic bytes of one element-block launch: per element its 8 nodes, 8 block-local
+    indices, sub-colour, element id and modulus (61 B); per block its local-node range; per node its
+    coordinates and x once; per block-local node entry its global node and slot (8 B); the stores
+    of the nodes a block owns and of the shared nodes' partials."""
+    s = 12 if fp32 else 24
+    owned = n_nodes - info["shared_nodes"] - info["isolated_nodes"]
+    return (n_elem * 61 + info["blocks"] * 4 + n_nodes * (24 + s) + info["local_entries"] * 8 +
+            (owned + info["partial_slots"]) * s)
+
+
+def mf_halo_bytes(info, fp32):
+    """Algorithmic bytes of the shared-node pass: node and range (8 B) per shared node, its
+    partials read once, its y written once."""
+    s = 12 if fp32 else 24
+    return info["shared_nodes"] * (8 + s) + info["partial_slots"] * s
 
 
 def spmv_bytes(nnzb, n_nodes, fp32, fmt, col16=0):
@@ -98,8 +130,11 @@ def _counters(d):
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
+        mf = re.search(r"k_mf_(block|halo)<(double|float), false>", name)
         if "k_copy" in name:
             key = "copy"
+        elif mf:
+            key = "mf_%s_%s" % (mf.group(1), "fp64" if mf.group(2) == "double" else "fp32")
         elif m:
             kind = "_sym" if m.group(3) == "1" else {"0": "_plain", "1": "", "2": "_col16"}[m.group(4)]
             key = "spmv_%s%s" % ("fp64" if m.group(1) == "double" else "fp32", kind)
@@ -126,6 +161,14 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
         wb = sum(wr[key]) / len(wr[key]) * 1024.0 * w_scale
         out[key] = {"fetch_bytes": fb, "write_bytes": wb, "traffic": fb + wb, "algorithmic_bytes": alg,
                     "traffic_over_algorithmic": (fb + wb) / alg, "dispatches": len(fe[key])}
+    # matrix-free kernels: measured bytes (their algorithmic bytes depend on the block layout and
+    # are computed where the layout is known: mf_block_bytes, mf_halo_bytes)
+    for key in ("mf_block_fp64", "mf_block_fp32", "mf_halo_fp64", "mf_halo_fp32"):
+        if key not in fe or key not in wr:
+            continue
+        fb = sum(fe[key]) / len(fe[key]) * 1024.0 * f_scale
+        wb = sum(wr[key]) / len(wr[key]) * 1024.0 * w_scale
+        out[key] = {"fetch_bytes": fb, "write_bytes": wb, "traffic": fb + wb, "dispatches": len(fe[key])}
     return out
 
 

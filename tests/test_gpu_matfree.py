@@ -1,0 +1,155 @@
+"""Matrix-free Hexa product (femgpu_set_spmv_config FEMGPU_SPMV_MATRIX_FREE) through the C ABI.
+
+The product K x is evaluated element by element from the mesh instead of read from the stored
+matrix.  Pinned here against the oracle's independently assembled K (oracle.assemble + penalty,
+whose element kernels are pinned in test_oracle_elem.py):
+* FP64: |K_mf x - K x| <= 1e-12 (|K| |x|) per DOF; the FP32 product to 2e-5 of the same scale
+  (the stored FP32 copy is held to 1e-6 beside it);
+* the penalty diagonal, a node of no element (its row is the penalty alone) and the internal node
+  order are covered;
+* with assume_constant_matrix the product keeps the material of the last matrix assembly;
+* solutions with the product in the loop match the oracle's solve to 1e-10 (FP64) and 1e-8
+  (MIXED), with block Jacobi and block IC(0), and repeat bitwise;
+* problems it cannot serve (Tetra, batches) are refused.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import lmtpp_amd as L
+from lmtpp_amd import _capi as A
+from problems import clamp_and_traction, jitter_interior, lognormal_E
+
+pytestmark = pytest.mark.gpu
+
+MF = (A.SPMV_MATRIX_FREE, 2, A.LOADS_NT)
+FULL = (A.SPMV_FULL, 2, A.LOADS_NT)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _mesh(div, seed, isolated=False, shuffle=False):
+    c, e = L.make_rect(L.HEXA, div)
+    c = jitter_interior(c, 1.0 / max(div), 0.2, seed)
+    if shuffle:
+        perm = np.random.default_rng(seed).permutation(len(c))
+        new_of = np.empty(len(c), np.int64)
+        new_of[perm] = np.arange(len(c))
+        c, e = c[perm].copy(), new_of[e].astype(np.uint32)
+    if isolated:  # a node no element uses, clamped below like the x = 0 face
+        c = np.vstack([c, [[0.5, 0.5, 2.0]]])
+    return c, e, lognormal_E(len(e), 0.4, seed)
+
+
+def _system(oracle, c, e, E, isolated, order=A.NODE_ORDER_NATURAL, nu=0.3):
+    faces, tr, dofs = clamp_and_traction(L.HEXA, c, e, (0.0, 0.4, -1.0))
+    if isolated:
+        dofs = np.concatenate([dofs, 3 * (len(c) - 1) + np.arange(3)]).astype(np.uint32)
+    with A.node_order_ctx(order):
+        f = L.Formulation(L.HEXA, c, e)
+    f.set_material(E, nu, (0.1, -0.2, 0.3))
+    f.set_traction(faces, tr)
+    f.add_constraints(dofs, None, 1e5)
+    f.assemble()
+    rp, col, vals, F, rc = oracle.assemble(L.HEXA, c, e, E, nu, bf=(0.1, -0.2, 0.3), faces=faces, face_trac=tr)
+    assert rc == 0
+    oracle.penalty(rp, col, vals, F, dofs, None, 1e5)
+    return f, (rp, col, vals, F)
+
+
+def _rel_err(y, ref, scale):
+    return float(np.max(np.abs(y - ref) / scale))
+
+
+@pytest.mark.parametrize("case", ["jittered", "isolated_node", "node_order"])
+def test_product_matches_oracle_matrix(oracle, case):
+    say("matrix-free product", case)
+    iso = case == "isolated_node"
+    c, e, E = _mesh((7, 6, 5), 11, isolated=iso, shuffle=case == "node_order")
+    order = A.NODE_ORDER_RCM if case == "node_order" else A.NODE_ORDER_NATURAL
+    f, (rp, col, vals, F) = _system(oracle, c, e, E, iso, order)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(3 * len(c))
+    ref = oracle.spmv(rp, col, vals, x)
+    scale = oracle.spmv(rp, col, np.abs(vals), np.abs(x))
+    f.set_spmv_config(fp64=FULL, fp32=FULL)
+    e_full64 = _rel_err(f.apply(x, A.FP64), ref, scale)
+    e_full32 = _rel_err(f.apply(x, A.MIXED), ref, scale)
+    f.set_spmv_config(fp64=MF, fp32=MF)
+    assert f.spmv_config()["fp64"][0] == A.SPMV_MATRIX_FREE
+    e_mf64 = _rel_err(f.apply(x, A.FP64), ref, scale)
+    e_mf32 = _rel_err(f.apply(x, A.MIXED), ref, scale)
+    say("  errors full64 %.2e full32 %.2e mf64 %.2e mf32 %.2e" % (e_full64, e_full32, e_mf64, e_mf32))
+    assert e_full64 <= 1e-13 and e_full32 <= 1e-6
+    assert e_mf64 <= 1e-12, e_mf64
+    assert e_mf32 <= 2e-5, e_mf32
+    # repeated products are bitwise equal (fixed colour order, no atomics)
+    assert np.array_equal(f.apply(x, A.FP64), f.apply(x, A.FP64))
+    assert np.array_equal(f.apply(x, A.MIXED), f.apply(x, A.MIXED))
+    f.close()
+
+
+def test_product_keeps_the_assembled_material(oracle):
+    say("matrix-free snapshot")
+    c, e, E = _mesh((5, 5, 4), 12)
+    f, (rp, col, vals, F) = _system(oracle, c, e, E, False)
+    f.set_spmv_config(fp64=MF, fp32=MF)
+    x = np.random.default_rng(6).standard_normal(3 * len(c))
+    y0 = f.apply(x, A.FP64)
+    f.set_material(3.0 * E, 0.25)            # not assembled: K, and the product, keep E and nu
+    assert np.array_equal(f.apply(x, A.FP64), y0)
+    f.assemble()
+    _, (rp2, col2, vals2, _) = _system(oracle, c, e, 3.0 * E, False, nu=0.25)
+    ref = oracle.spmv(rp2, col2, vals2, x)
+    scale = oracle.spmv(rp2, col2, np.abs(vals2), np.abs(x))
+    assert _rel_err(f.apply(x, A.FP64), ref, scale) <= 1e-12
+    f.close()
+
+
+@pytest.mark.parametrize("precond", [L.BLOCK_JACOBI, L.BLOCK_IC0])
+def test_solutions_match_oracle(oracle, precond):
+    say("matrix-free solve", precond)
+    c, e, E = _mesh((8, 6, 6), 13)
+    f, (rp, col, vals, F) = _system(oracle, c, e, E, False)
+    x, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-14, block=True)
+    assert rc == 0
+    f.precond = precond
+    iters = {}
+    for cfg in (FULL, MF):
+        f.set_spmv_config(fp64=cfg, fp32=cfg)
+        for prec, tol in ((L.FP64, 1e-10), (L.MIXED, 1e-8)):
+            f.precision = prec
+            f.set_result(np.zeros_like(F))
+            assert f.solve_system(rel_tol=1e-12), (cfg, prec)
+            u = f.get_result()
+            assert np.linalg.norm(u - x) <= tol * np.linalg.norm(x), (cfg, prec)
+            iters[(cfg[0], prec)] = f.last_stats.iters
+            if cfg == MF:
+                f.set_result(np.zeros_like(F))
+                assert f.solve_system(rel_tol=1e-12)
+                assert np.array_equal(f.get_result(), u)          # deterministic
+    say("  iterations", iters)
+    # the same operator up to rounding: iteration counts agree closely (the FP32 element product
+    # is more accurate than the rounded FP32 matrix, so MIXED may need a few fewer)
+    for prec in (L.FP64, L.MIXED):
+        a, b = iters[(A.SPMV_FULL, prec)], iters[(A.SPMV_MATRIX_FREE, prec)]
+        assert abs(a - b) <= max(5, 0.1 * a), (prec, a, b)
+    f.close()
+
+
+def test_refused_where_it_cannot_serve():
+    say("matrix-free refused")
+    c, e = L.make_rect(L.TETRA, (4, 4, 4))
+    f = L.Formulation(L.TETRA, c, e)
+    with pytest.raises(L.FemGpuError):
+        f.set_spmv_config(fp64=MF, fp32=FULL)
+    f.close()
+    meshes = [L.make_rect(L.HEXA, (3, 3, 3)), L.make_rect(L.HEXA, (4, 3, 3))]
+    b = L.Batch(L.HEXA, meshes)
+    with pytest.raises(L.FemGpuError):
+        b.set_spmv_config(fp64=MF, fp32=MF)
+    b.close()

@@ -419,6 +419,7 @@ int upload_constraints(Problem &P) {
     }
     if (!dof.empty()) ptr.push_back((uint32_t)val.size());
     P.n_cdof = (uint32_t)dof.size();
+    P.cdof_h = dof;
     HIPCHK(upload(&P.d_cdof, dof.data(), dof.size()));
     HIPCHK(upload(&P.d_cptr, ptr.data(), ptr.size()));
     HIPCHK(upload(&P.d_csub, sub.data(), sub.size()));

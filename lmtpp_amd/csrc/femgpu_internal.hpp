@@ -77,11 +77,13 @@ struct SymDev {
 // Matrix-free product y = K x of a Hexa problem with one sub-problem: K_e x_e evaluated per element
 // with the element loop's quadrature.  Elements are sorted along a Morton curve of their centres
 // and cut into blocks of MF_BLOCK; a workgroup sums its block's nodal forces in LDS (sub-colour by
-// sub-colour, so every sum has a fixed order) and stores the nodes all of whose elements are in
-// the block; a node shared by several blocks gets one partial per block, summed in block order by
-// a second pass.  Then the penalty diagonal.  Mesh arrays are built once; the material and
-// penalties are those of the last matrix assembly, so the operator is the assembled K up to
-// rounding even when the material changed later (assume_constant_matrix).
+// sub-colour, so every sum has a fixed order), adds the penalty terms of the constrained DOFs it
+// is assigned (each to the first block holding its node), and stores the nodes all of whose
+// elements are in the block; a node shared by several blocks gets one partial per block, summed in
+// block order by a second pass.  Nodes of no element get the penalty alone.  Mesh arrays are
+// built once; the material and penalties are those of the last matrix assembly, so the operator
+// is the assembled K up to rounding even when the material changed later
+// (assume_constant_matrix).
 constexpr uint32_t MF_BLOCK = 256;
 struct MatFree {
     bool built = false, snap = false;
@@ -96,12 +98,18 @@ struct MatFree {
     uint32_t *d_lslot = nullptr;     // its partial slot when shared with another block, else ~0
     uint32_t *d_bnode = nullptr, *d_bptr = nullptr;  // shared node k: partials [bptr[k], bptr[k + 1])
     double *d_halo = nullptr;        // 3 values per partial slot (FP64, or FP32 in its first half)
-    uint32_t *d_iso = nullptr;       // nodes of no element (their products are the penalty alone)
+    std::vector<uint32_t> node_blk, node_loc;  // first block holding each node, its local index (~0: none)
+    uint32_t *d_idof = nullptr;      // the 3 DOFs of each node of no element ...
+    uint32_t *d_isrc = nullptr;      // ... and the index of their penalty in d_fpen (~0: none)
+    // penalty terms per block: block b adds fpen[psrc[k]] x[pdof[k]] to local entry pidx[k],
+    // k in [pptr[b], pptr[b + 1]); built for the DOF list pen_dofs
+    uint32_t *d_pptr = nullptr, *d_pidx = nullptr, *d_pdof = nullptr, *d_psrc = nullptr;
+    std::vector<uint32_t> snap_dofs, pen_dofs;
+    bool pen_built = false;
     double *d_E = nullptr;           // E per element at the last matrix assembly
     SubMaterial *d_mat = nullptr;    // material at the last matrix assembly
-    uint32_t *d_fdof = nullptr;      // constrained DOFs ...
-    double *d_fpen = nullptr;        // ... and their summed penalty M * P at the last matrix assembly
-    double *d_partials = nullptr;    // one p.q partial per workgroup (element blocks, then penalties)
+    double *d_fpen = nullptr;        // summed penalty M * P of each DOF of snap_dofs at that assembly
+    double *d_partials = nullptr;    // one p.q partial per workgroup (element blocks, then isolated nodes)
     uint32_t *d_wg0 = nullptr, *d_nwg = nullptr;  // {0}, {n_slots}: the finalize's slot range
 };
 
@@ -167,6 +175,7 @@ struct Problem {
     std::vector<Cons> cons;
     bool cons_dirty = true;
     uint32_t n_cdof = 0;
+    std::vector<uint32_t> cdof_h;  // the constrained DOFs of d_cdof, in its order
     uint32_t *d_cdof = nullptr, *d_cptr = nullptr, *d_csub = nullptr;
     double *d_cval = nullptr, *d_cpen = nullptr;
     // solver work vectors

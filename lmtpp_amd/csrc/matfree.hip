@@ -101,8 +101,8 @@ __device__ inline T adjugate(const T (&J)[3][3], T (&A)[3][3]) {
     return J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
 }
 
-// f[a][c] = (K_e u_e) of one Hexa with node coordinates X and displacements u; returns nothing,
-// H as (H11 - H12, H12, H44 / 2).
+// f[a][c] = (K_e u_e) of one Hexa with node coordinates X and displacements u, H as
+// (H11 - H12, H12, H44 / 2).
 template <typename T>
 __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T h01, T h1, T h2, T (&f)[8][3]) {
     double g0d, g1d, wq;
@@ -222,12 +222,16 @@ __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T
 
 // One block of MF_BLOCK element slots per workgroup (logical block b = xcd_slab(), so each XCD
 // takes a contiguous run of blocks): y of the block's own nodes, partials of its shared ones.
-// PQ: partial of sum_e x_e . (K_e x_e), whose sum with the penalty pass's is x . K x.
+// PQ: partial of sum_e x_e . (K_e x_e) = sum over the block's local nodes of x . (their summed
+// element forces), taken at the write-out; with the penalty pass's partials it is x . K x.
 template <typename T, bool PQ>
 __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   uint32_t n_colors, const uint32_t *__restrict__ lptr,
                                                   const uint32_t *__restrict__ lnode, const uint32_t *__restrict__ lslot,
+                                                  const uint32_t *__restrict__ pptr, const uint32_t *__restrict__ pidx,
+                                                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ psrc,
+                                                  const double *__restrict__ fpen,
                                                   const double *__restrict__ coords, const double *__restrict__ E,
                                                   const SubMaterial *__restrict__ mat, const T *__restrict__ x,
                                                   T *__restrict__ y, T *__restrict__ halo, const ProbScalars *S,
@@ -262,13 +266,10 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
                 u[a][c] = x[3 * (size_t)nd[a] + c];
             }
         hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
-        if (PQ)
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) pq[0] += (double)u[a][c] * (double)f[a][c];
     }
     __syncthreads();
+    // no two lanes of a sub-colour touch one entry, so the order of every sum is the sub-colour
+    // order; plain read-add-write (LDS float atomics measured 2.6x slower here in FP32)
     for (uint32_t c = 0; c < n_colors; ++c) {
         if (col == c)
 #pragma unroll
@@ -277,12 +278,19 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
                 for (int d = 0; d < 3; ++d) acc[3 * lo[a] + d] += f[a][d];
         __syncthreads();
     }
+    // penalty terms of the constrained DOFs assigned to this block (distinct entries)
+    const uint32_t p1 = pptr[b + 1];
+    for (uint32_t k = pptr[b] + threadIdx.x; k < p1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * x[pdof[k]];
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < nl; i += MF_BLOCK) {
-        const uint32_t s = lslot[l0 + i];
-        T *dst = s == 0xffffffffu ? y + 3 * (size_t)lnode[l0 + i] : halo + 3 * (size_t)s;
-        dst[0] = acc[3 * i];
-        dst[1] = acc[3 * i + 1];
-        dst[2] = acc[3 * i + 2];
+        const uint32_t s = lslot[l0 + i], g = lnode[l0 + i];
+        T *dst = s == 0xffffffffu ? y + 3 * (size_t)g : halo + 3 * (size_t)s;
+        const T a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
+        dst[0] = a0;
+        dst[1] = a1;
+        dst[2] = a2;
+        if (PQ) pq[0] += (double)x[3 * (size_t)g] * (double)a0 + (double)x[3 * (size_t)g + 1] * (double)a1 +
+                         (double)x[3 * (size_t)g + 2] * (double)a2;
     }
     if (PQ) wg_partial<1, 0u>(pq, partials, 0u, b);
 }
@@ -307,29 +315,21 @@ __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__r
     py[2] = v2;
 }
 
-// Nodes of no element: their rows hold the penalty alone, so the element passes leave them and
-// they start from zero.
-template <typename T>
-__global__ __launch_bounds__(256) void k_mf_iso(uint32_t n, const uint32_t *__restrict__ nodes, T *__restrict__ y) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= 3 * n) return;
-    y[3 * (size_t)nodes[i / 3] + i % 3] = 0;
-}
-
-// Penalty diagonal: y[dof] += M P x[dof] per constrained DOF (distinct DOFs, no races).
+// Nodes of no element: their rows hold the penalty alone, y = M P x on their constrained DOFs and
+// 0 on the others.
 template <typename T, bool PQ>
-__global__ __launch_bounds__(256) void k_mf_fix(uint32_t n, const uint32_t *__restrict__ dof,
-                                                const double *__restrict__ pen, const T *__restrict__ x,
-                                                T *__restrict__ y, const ProbScalars *S, double *partials,
-                                                uint32_t slot0) {
+__global__ __launch_bounds__(256) void k_mf_iso(uint32_t n, const uint32_t *__restrict__ dof,
+                                                const uint32_t *__restrict__ src, const double *__restrict__ fpen,
+                                                const T *__restrict__ x, T *__restrict__ y, const ProbScalars *S,
+                                                double *partials, uint32_t slot0) {
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     double acc[1] = {0.0};
     if (i < n) {
-        const uint32_t d = dof[i];
+        const uint32_t d = dof[i], k = src[i];
         const T xv = x[d];
-        const T a = (T)pen[i] * xv;
-        y[d] += a;
+        const T a = k == 0xffffffffu ? (T)0 : (T)fpen[k] * xv;
+        y[d] = a;
         acc[0] = (double)a * (double)xv;
     }
     if (PQ) wg_partial<1, 0u>(acc, partials, 0u, slot0 + blockIdx.x);
@@ -357,17 +357,20 @@ bool mf_eligible(const Problem &P) {
 
 void mf_free(MatFree &M) {
     dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr); dfree(M.d_lnode);
-    dfree(M.d_lslot); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_iso); dfree(M.d_E);
-    dfree(M.d_mat); dfree(M.d_fdof); dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_wg0); dfree(M.d_nwg);
+    dfree(M.d_lslot); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof); dfree(M.d_isrc);
+    dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_pdof); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
+    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_wg0); dfree(M.d_nwg);
     M = MatFree();
 }
 
 namespace {
 
-// 10-bit-per-axis Morton code (bit 3k + d holds bit k of axis d).
-uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-    uint32_t m = 0;
-    for (int k = 0; k < 10; ++k) m |= ((x >> k & 1u) << (3 * k)) | ((y >> k & 1u) << (3 * k + 1)) | ((z >> k & 1u) << (3 * k + 2));
+// 21-bit-per-axis Morton code (bit 3k + d holds bit k of axis d).
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t m = 0;
+    for (int k = 0; k < 21; ++k)
+        m |= ((uint64_t)(x >> k & 1u) << (3 * k)) | ((uint64_t)(y >> k & 1u) << (3 * k + 1)) |
+             ((uint64_t)(z >> k & 1u) << (3 * k + 2));
     return m;
 }
 
@@ -387,23 +390,30 @@ int mf_build(Problem &P) {
     const uint32_t *conn = P.conn_h.data();
     std::vector<double> X(3 * (size_t)nn);
     if (hipMemcpy(X.data(), P.d_coords, X.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return 5;
-    // element order: Morton code of the element centre on the bounding box, ties by element id
+    // element order: Morton code of the element centre on a grid of the typical element size (on a
+    // lattice, one cell per element, so aligned blocks are 8 x 8 x 4 bricks), ties by element id
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (uint32_t n = 0; n < nn; ++n)
         for (int d = 0; d < 3; ++d) {
             lo[d] = std::min(lo[d], X[3 * (size_t)n + d]);
             hi[d] = std::max(hi[d], X[3 * (size_t)n + d]);
         }
-    std::vector<uint64_t> key(ne);
+    double vol = 1.0, ext = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        ext = std::max(ext, hi[d] - lo[d]);
+        vol *= std::max(hi[d] - lo[d], 0.0);
+    }
+    double h = vol > 0 ? std::cbrt(vol / ne) : ext / std::cbrt((double)ne);
+    if (!(h > 0)) h = 1.0;
+    std::vector<std::pair<uint64_t, uint32_t>> key(ne);
     for (uint32_t e = 0; e < ne; ++e) {
         uint32_t q[3];
         for (int d = 0; d < 3; ++d) {
             double c = 0;
             for (int a = 0; a < 8; ++a) c += X[3 * (size_t)conn[8 * (size_t)e + a] + d];
-            const double t = hi[d] > lo[d] ? (c / 8 - lo[d]) / (hi[d] - lo[d]) : 0.0;
-            q[d] = (uint32_t)std::min(1023.0, std::max(0.0, t * 1024.0));
+            q[d] = (uint32_t)std::min(2097151.0, std::max(0.0, std::floor((c / 8 - lo[d]) / h)));
         }
-        key[e] = (uint64_t)morton3(q[0], q[1], q[2]) << 32 | e;
+        key[e] = {morton3(q[0], q[1], q[2]), e};
     }
     std::sort(key.begin(), key.end());
     // blocks of MF_BLOCK consecutive elements, padded to a multiple of 8 blocks (the XCD mapping)
@@ -411,17 +421,34 @@ int mf_build(Problem &P) {
     const size_t ns = (size_t)nb * MF_BLOCK;
     std::vector<uint32_t> sconn(8 * ns, 0), selem(ns, 0), lptr(nb + 1, 0), lnode, deg(nn, 0), stamp(nn, 0xffffffffu),
         lid(nn, 0), in_blk;
+    M.node_blk.assign(nn, 0xffffffffu);
+    M.node_loc.assign(nn, 0);
     std::vector<uint16_t> sloc(8 * ns, 0);
     std::vector<uint8_t> scol(ns, 0xffu);
     for (size_t s = 0; s < 8 * (size_t)ne; ++s) ++deg[conn[s]];
     uint32_t n_colors = 0, max_local = 0;
+    // element centres, for the sweep order of the sub-colouring
+    std::vector<double> cen(3 * (size_t)ne, 0.0);
+    for (uint32_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d) cen[3 * (size_t)e + d] += X[3 * (size_t)conn[8 * (size_t)e + a] + d] / 8;
+    std::vector<uint32_t> blk;
     for (uint32_t b = 0; b < nb; ++b) {
         lptr[b] = (uint32_t)lnode.size();
         std::vector<uint32_t> mask;
-        for (uint32_t k = 0; k < MF_BLOCK; ++k) {
+        // the block's elements swept by (z, y, x) of their centres: greedy first-fit then finds the
+        // 8-colour parity pattern on lattices (the Morton order alone needs more sub-colours)
+        blk.clear();
+        for (uint32_t k = 0; k < MF_BLOCK && (size_t)b * MF_BLOCK + k < ne; ++k)
+            blk.push_back(key[(size_t)b * MF_BLOCK + k].second);
+        std::stable_sort(blk.begin(), blk.end(), [&](uint32_t p, uint32_t q) {
+            for (int d = 2; d >= 0; --d)
+                if (cen[3 * (size_t)p + d] != cen[3 * (size_t)q + d]) return cen[3 * (size_t)p + d] < cen[3 * (size_t)q + d];
+            return p < q;
+        });
+        for (uint32_t k = 0; k < (uint32_t)blk.size(); ++k) {
             const size_t i = (size_t)b * MF_BLOCK + k;
-            if (i >= ne) break;
-            const uint32_t e = (uint32_t)(key[i] & 0xffffffffu);
+            const uint32_t e = blk[k];
             uint32_t used = 0;
             for (int a = 0; a < 8; ++a) {
                 const uint32_t n = conn[8 * (size_t)e + a];
@@ -431,6 +458,10 @@ int mf_build(Problem &P) {
                     lnode.push_back(n);
                     in_blk.push_back(0);
                     mask.push_back(0);
+                    if (M.node_blk[n] == 0xffffffffu) {
+                        M.node_blk[n] = b;
+                        M.node_loc[n] = lid[n];
+                    }
                 }
                 sconn[8 * i + a] = n;
                 sloc[8 * i + a] = (uint16_t)lid[n];
@@ -452,9 +483,10 @@ int mf_build(Problem &P) {
     std::vector<uint32_t> nshared(nn, 0);
     for (size_t l = 0; l < lnode.size(); ++l)
         if (in_blk[l] != deg[lnode[l]]) ++nshared[lnode[l]];
-    std::vector<uint32_t> bnode, bptr(1, 0), first(nn, 0xffffffffu), iso;
+    std::vector<uint32_t> bnode, bptr(1, 0), first(nn, 0xffffffffu), idof;
     for (uint32_t n = 0; n < nn; ++n) {
-        if (!deg[n]) iso.push_back(n);
+        if (!deg[n])
+            for (uint32_t d = 0; d < 3; ++d) idof.push_back(3 * n + d);
         if (!nshared[n]) continue;
         first[n] = bptr.back();
         bnode.push_back(n);
@@ -467,7 +499,8 @@ int mf_build(Problem &P) {
     if (!upload_vec(&M.d_conn, sconn) || !upload_vec(&M.d_loc, sloc) || !upload_vec(&M.d_color, scol) ||
         !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode, lnode) ||
         !upload_vec(&M.d_lslot, lslot) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
-        !upload_vec(&M.d_iso, iso) || !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
+        !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
+        !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
         hipMalloc(&M.d_halo, sizeof(double) * 3 * std::max<size_t>(1, bptr.back())) != hipSuccess) {
         mf_free(M);
         return 5;
@@ -477,7 +510,7 @@ int mf_build(Problem &P) {
     M.max_local = max_local;
     M.n_bnodes = (uint32_t)bnode.size();
     M.n_halo = bptr.back();
-    M.n_iso = (uint32_t)iso.size();
+    M.n_iso = (uint32_t)idof.size() / 3;
     M.built = true;
     return 0;
 }
@@ -489,34 +522,74 @@ hipError_t mf_snapshot(Problem &P) {
     if (!M.d_E) e = hipMalloc(&M.d_E, sizeof(double) * P.n_elem);
     if (e == hipSuccess && !M.d_mat) e = hipMalloc(&M.d_mat, sizeof(SubMaterial));
     if (e == hipSuccess && P.n_cdof > M.cap_fix) {
-        dfree(M.d_fdof);
         dfree(M.d_fpen);
-        e = hipMalloc(&M.d_fdof, sizeof(uint32_t) * P.n_cdof);
-        if (e == hipSuccess) e = hipMalloc(&M.d_fpen, sizeof(double) * P.n_cdof);
+        e = hipMalloc(&M.d_fpen, sizeof(double) * P.n_cdof);
         M.cap_fix = e == hipSuccess ? P.n_cdof : 0;
+        destroy_graph(P);  // captured products read the penalties through this pointer
+        M.pen_built = false;
     }
     if (e == hipSuccess) e = hipMemcpyAsync(M.d_E, P.d_E, sizeof(double) * P.n_elem, hipMemcpyDeviceToDevice, P.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(M.d_mat, P.d_mat, sizeof(SubMaterial), hipMemcpyDeviceToDevice, P.stream);
-    if (e == hipSuccess && P.n_cdof) {
-        e = hipMemcpyAsync(M.d_fdof, P.d_cdof, sizeof(uint32_t) * P.n_cdof, hipMemcpyDeviceToDevice, P.stream);
+    if (e == hipSuccess && P.n_cdof)
         hipLaunchKernelGGL(k_mf_pen, dim3((P.n_cdof + 255) / 256), dim3(256), 0, P.stream, P.n_cdof, P.d_cptr, P.d_csub,
                            P.d_cpen, P.d_ps, M.d_fpen);
-    }
     if (e != hipSuccess) return e;
-    if (M.n_fix != P.n_cdof) destroy_graph(P);  // the penalty pass's grid is part of captured chunks
+    M.snap_dofs = P.cdof_h;
     M.n_fix = P.n_cdof;
     M.snap = true;
     return hipGetLastError();
 }
 
-// Partial slots of the passes and the finalize's range; allocations happen here, never while a
-// chunk of iterations is being captured.
+// Penalty tables for the DOF list of the last snapshot: each constrained DOF goes to the first
+// block holding its node (or to the nodes-of-no-element pass).
+static int mf_penalties(Problem &P) {
+    MatFree &M = P.mf;
+    if (M.pen_built && M.pen_dofs == M.snap_dofs) return 0;
+    std::vector<std::vector<uint32_t>> per(M.n_blocks);
+    std::vector<uint32_t> isrc(3 * (size_t)M.n_iso, 0xffffffffu);
+    std::vector<uint32_t> idof(3 * (size_t)M.n_iso);
+    if (M.n_iso && hipMemcpy(idof.data(), M.d_idof, idof.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return 5;
+    for (uint32_t i = 0; i < (uint32_t)M.snap_dofs.size(); ++i) {
+        const uint32_t dof = M.snap_dofs[i], n = dof / 3;
+        if (M.node_blk[n] == 0xffffffffu) {
+            const size_t k = std::lower_bound(idof.begin(), idof.end(), dof) - idof.begin();
+            if (k < idof.size() && idof[k] == dof) isrc[k] = i;
+        } else {
+            per[M.node_blk[n]].push_back(i);
+        }
+    }
+    std::vector<uint32_t> pptr(1, 0), pidx, pdof, psrc;
+    for (uint32_t b = 0; b < M.n_blocks; ++b) {
+        for (uint32_t i : per[b]) {
+            const uint32_t dof = M.snap_dofs[i], n = dof / 3;
+            pidx.push_back(3 * M.node_loc[n] + dof % 3);
+            pdof.push_back(dof);
+            psrc.push_back(i);
+        }
+        pptr.push_back((uint32_t)pidx.size());
+    }
+    dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_pdof); dfree(M.d_psrc);
+    if (!upload_vec(&M.d_pptr, pptr) || !upload_vec(&M.d_pidx, pidx) || !upload_vec(&M.d_pdof, pdof) ||
+        !upload_vec(&M.d_psrc, psrc) ||
+        (M.n_iso && hipMemcpy(M.d_isrc, isrc.data(), isrc.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
+        return 5;
+    M.pen_dofs = M.snap_dofs;
+    M.pen_built = true;
+    destroy_graph(P);  // captured products hold the old table pointers
+    return 0;
+}
+
+// Partial slots of the passes and the finalize's range, penalty tables; allocations happen here,
+// never while a chunk of iterations is being captured.
 int mf_prepare(Problem &P) {
-    const int rc = mf_build(P);
+    int rc = mf_build(P);
     if (rc) return rc;
     MatFree &M = P.mf;
     if (!M.snap) return 1;
-    const uint32_t s = M.n_blocks + (M.n_fix + 255) / 256;
+    if (M.n_fix && !M.d_fpen) return 5;
+    rc = mf_penalties(P);
+    if (rc) return rc;
+    const uint32_t s = M.n_blocks + (3 * M.n_iso + 255) / 256;
     if (s > M.cap_slots) {
         dfree(M.d_partials);
         if (hipMalloc(&M.d_partials, sizeof(double) * std::max<uint32_t>(1, s)) != hipSuccess) {
@@ -524,6 +597,7 @@ int mf_prepare(Problem &P) {
             return 5;
         }
         M.cap_slots = s;
+        destroy_graph(P);
     }
     if (s != M.n_slots) {
         if (hipMemcpy(M.d_nwg, &s, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) return 5;
@@ -539,16 +613,14 @@ static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq) {
     const size_t lds = 3 * (size_t)std::max<uint32_t>(1, M.max_local) * sizeof(T);
 #define FEMGPU_MF_PASSES(PQ)                                                                                          \
     hipLaunchKernelGGL((k_mf_block<T, PQ>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn, M.d_loc,        \
-                       M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, P.d_coords, M.d_E, M.d_mat, x, \
-                       y, halo, P.d_ps, M.d_partials);                                                                 \
+                       M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx, M.d_pdof,  \
+                       M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, x, y, halo, P.d_ps, M.d_partials);              \
     if (M.n_bnodes)                                                                                                    \
         hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,      \
                            M.d_bnode, M.d_bptr, halo, y, P.d_ps);                                                      \
     if (M.n_iso)                                                                                                       \
-        hipLaunchKernelGGL(k_mf_iso<T>, dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, M.n_iso, M.d_iso, y); \
-    if (M.n_fix)                                                                                                       \
-        hipLaunchKernelGGL((k_mf_fix<T, PQ>), dim3((M.n_fix + 255) / 256), dim3(256), 0, P.stream, M.n_fix, M.d_fdof,   \
-                           M.d_fpen, x, y, P.d_ps, M.d_partials, M.n_blocks);
+        hipLaunchKernelGGL((k_mf_iso<T, PQ>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,     \
+                           M.d_idof, M.d_isrc, M.d_fpen, x, y, P.d_ps, M.d_partials, M.n_blocks);
     if (pq) {
         FEMGPU_MF_PASSES(true)
     } else {
@@ -568,12 +640,14 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     auto block = [&]() {
         if (precision)
             hipLaunchKernelGGL((k_mf_block<float, false>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn,
-                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, P.d_coords, M.d_E,
-                               M.d_mat, P.d_p32, P.d_q32, reinterpret_cast<float *>(M.d_halo), P.d_ps, M.d_partials);
+                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx,
+                               M.d_pdof, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, P.d_p32, P.d_q32,
+                               reinterpret_cast<float *>(M.d_halo), P.d_ps, M.d_partials);
         else
             hipLaunchKernelGGL((k_mf_block<double, false>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn,
-                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, P.d_coords, M.d_E,
-                               M.d_mat, P.d_p, P.d_q, M.d_halo, P.d_ps, M.d_partials);
+                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx,
+                               M.d_pdof, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, P.d_p, P.d_q, M.d_halo, P.d_ps,
+                               M.d_partials);
     };
     auto product = [&]() {
         if (precision) (void)mf_launch_t<float>(P, P.d_p32, P.d_q32, false);

@@ -133,11 +133,12 @@ def test_solutions_match_oracle(oracle, precond):
                 assert f.solve_system(rel_tol=1e-12)
                 assert np.array_equal(f.get_result(), u)          # deterministic
     say("  iterations", iters)
-    # the same operator up to rounding: iteration counts agree closely (the FP32 element product
-    # is more accurate than the rounded FP32 matrix, so MIXED may need a few fewer)
-    for prec in (L.FP64, L.MIXED):
-        a, b = iters[(A.SPMV_FULL, prec)], iters[(A.SPMV_MATRIX_FREE, prec)]
-        assert abs(a - b) <= max(5, 0.1 * a), (prec, a, b)
+    # FP64: the same operator to ~1e-15, so the same iteration count; MIXED: the FP32 element
+    # product is more accurate than the rounded FP32 matrix, so it may need fewer, never many more
+    a, b = iters[(A.SPMV_FULL, L.FP64)], iters[(A.SPMV_MATRIX_FREE, L.FP64)]
+    assert abs(a - b) <= 2, (a, b)
+    a, b = iters[(A.SPMV_FULL, L.MIXED)], iters[(A.SPMV_MATRIX_FREE, L.MIXED)]
+    assert b <= a + max(5, 0.1 * a), (a, b)
     f.close()
 
 

@@ -197,6 +197,8 @@ def config3(steps=50):
     zt = zt / zt.std()
     E_true = 210000.0 * np.exp(0.3 * zt)
     f, *_ = build(L.HEXA, c, e, E_true)
+    f.assemble()
+    f.autotune()                             # product variant for this mesh (matrix-free on Hexa)
     assert f.solve(rel_tol=1e-12)
     u_true = f.get_result()
     top = np.where(np.abs(c[:, 2] - 1.0) < 1e-9)[0]

@@ -119,6 +119,15 @@ __device__ inline T jac_inverse(const T (&J)[3][3], T (&Ji)[3][3]) {
     return det;
 }
 
+// alpha = rho / p.q once p.q of the iteration is known; a non-finite or non-positive p.q ends the
+// sub-problem (status 6 or 3).
+__device__ inline void finish_pq(ProbScalars &P, double pq) {
+    P.pq = pq;
+    if (!finite(pq)) { P.status = 6; P.done = 1; }
+    else if (!(pq > 0)) { P.status = 3; P.done = 1; }
+    else P.alpha = P.rho / pq;
+}
+
 struct WgTables {
     const uint32_t *row0, *row1, *sub, *sub_wg0, *sub_nwg;
 };

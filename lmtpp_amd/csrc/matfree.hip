@@ -101,6 +101,11 @@ __device__ inline T adjugate(const T (&J)[3][3], T (&A)[3][3]) {
     return J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
 }
 
+// w / det: exact division in FP64; in FP32 the hardware reciprocal (1 ulp, against the ~1e-7
+// relative accuracy of the FP32 product) instead of the ten-instruction IEEE division.
+__device__ inline double wq_over(double det, double w) { return w / det; }
+__device__ inline float wq_over(float det, float w) { return w * __builtin_amdgcn_rcpf(det); }
+
 // f[a][c] = (K_e u_e) of one Hexa with node coordinates X and displacements u, H as
 // (H11 - H12, H12, H44 / 2).
 template <typename T>
@@ -169,7 +174,7 @@ __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T
                     Gr[c][2] = Ud[2][i][j][c];
                 }
                 T Aj[3][3];
-                const T kq = (T)wq / adjugate(J, Aj);
+                const T kq = wq_over(adjugate(J, Aj), (T)wq);
                 T G[3][3];  // detJ * d u_c / d x_m
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
@@ -295,12 +300,20 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
     if (PQ) wg_partial<1, 0u>(pq, partials, 0u, b);
 }
 
-// Shared nodes: y = sum of the blocks' partials, in block order.
+// Shared nodes: y = sum of the blocks' partials, in block order.  PQ: the first workgroup also
+// turns the p.q partials of the element blocks and of the nodes-of-no-element pass (complete:
+// written by earlier kernels) into alpha, which the vector update reads after this kernel.
 template <typename T, bool PQ>
 __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__restrict__ bnode,
                                                  const uint32_t *__restrict__ bptr, const T *__restrict__ halo,
-                                                 T *__restrict__ y, const ProbScalars *S) {
+                                                 T *__restrict__ y, ProbScalars *S, const double *__restrict__ partials,
+                                                 uint32_t n_slots) {
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    if (PQ && blockIdx.x == 0) {
+        double o[1];
+        wg_combine<1, 0u>(partials, 0u, 0u, n_slots, o);
+        if (threadIdx.x == 0) finish_pq(S[0], o[0]);
+    }
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     T v0 = 0, v1 = 0, v2 = 0;
@@ -615,12 +628,12 @@ static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq) {
     hipLaunchKernelGGL((k_mf_block<T, PQ>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn, M.d_loc,        \
                        M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx, M.d_pdof,  \
                        M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, x, y, halo, P.d_ps, M.d_partials);              \
-    if (M.n_bnodes)                                                                                                    \
-        hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,      \
-                           M.d_bnode, M.d_bptr, halo, y, P.d_ps);                                                      \
     if (M.n_iso)                                                                                                       \
         hipLaunchKernelGGL((k_mf_iso<T, PQ>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,     \
-                           M.d_idof, M.d_isrc, M.d_fpen, x, y, P.d_ps, M.d_partials, M.n_blocks);
+                           M.d_idof, M.d_isrc, M.d_fpen, x, y, P.d_ps, M.d_partials, M.n_blocks);                     \
+    if (M.n_bnodes)                                                                                                    \
+        hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,      \
+                           M.d_bnode, M.d_bptr, halo, y, P.d_ps, M.d_partials, M.n_slots);
     if (pq) {
         FEMGPU_MF_PASSES(true)
     } else {

@@ -311,10 +311,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         double o[1];
         wg_combine<1, 0u>(partials, stride, w0, nw, o);
         if (threadIdx.x) return;
-        P.pq = o[0];
-        if (!finite(o[0])) { P.status = 6; P.done = 1; }
-        else if (!(o[0] > 0)) { P.status = 3; P.done = 1; }
-        else P.alpha = P.rho / o[0];
+        finish_pq(P, o[0]);
     } else if (what == FIN_UPD || what == FIN_UPD_IC) {
         double o[3];
         wg_combine<3, 4u>(partials, stride, w0, nw, o);
@@ -433,13 +430,14 @@ static void finalize(Problem &P, int what, int sel = 0) {
 }
 
 // alpha = rho / p.q from the partials of the product just launched (the row-parallel formats
-// write one per row workgroup, the matrix-free passes one per pass workgroup).
+// write one per row workgroup, the matrix-free passes one per pass workgroup; when the matrix-free
+// product has a shared-node pass, its first workgroup has done this already).
 static void finalize_pq(Problem &P, int precision) {
-    if (P.spmv.format[precision] == FMT_MF)
+    if (P.spmv.format[precision] != FMT_MF)
+        finalize(P, FIN_PQ);
+    else if (!P.mf.n_bnodes)
         hipLaunchKernelGGL(k_finalize, dim3(P.n_sub), dim3(1024), 0, P.stream, (int)FIN_PQ, P.d_ps, P.mf.d_partials,
                            P.mf.n_slots, P.mf.d_wg0, P.mf.d_nwg, 0);
-    else
-        finalize(P, FIN_PQ);
 }
 
 static WgTables tables(const Problem &P) {

@@ -18,7 +18,12 @@ bytes.
 
     python -m lmtpp_amd.traffic assembly [--n 100]
 
-is the workload of the element-loop counters (both assembly variants, three times each).
+is the workload of the element-loop counters (both assembly variants, three times each), and
+
+    python -m lmtpp_amd.traffic matfree [--n 100]
+
+that of the matrix-free product's counters (its element-block kernel and whole product, FP32 then
+FP64).
 """
 import csv
 import glob
@@ -106,6 +111,27 @@ def mf_halo_bytes(info, fp32):
     return info["shared_nodes"] * (8 + s) + info["partial_slots"] * s
 
 
+def matfree(n):
+    """Workload of the matrix-free counters: Hexa8 n^3, the element-block kernel alone and the whole
+    product, 3 launches each per precision (after one untimed product)."""
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, here)
+    sys.path.insert(0, os.path.join(here, "tests"))
+    import lmtpp_amd as L
+    from lmtpp_amd import _capi as A
+    from problems import build, lognormal_E
+
+    c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
+    f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
+    f.assemble()
+    f.set_spmv_config(fp64=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT), fp32=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT))
+    for prec, name in ((L.MIXED, "fp32"), (L.FP64, "fp64")):
+        b, p = f.time_matrix_free(prec, 3)
+        print("%s: element blocks %.1f us, product %.1f us" % (name, b * 1e3, p * 1e3), flush=True)
+    print(json.dumps(f.matrix_free_info()), flush=True)
+    f.close()
+
+
 def spmv_bytes(nnzb, n_nodes, fp32, fmt, col16=0):
     """Algorithmic bytes of one block SpMV: every stored value and index read once, x read once,
     y written once.  Full format: 4-byte columns, or 2-byte column offsets (col16).  Symmetric
@@ -173,9 +199,9 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) >= 2 and sys.argv[1] in ("run", "assembly"):
+    if len(sys.argv) >= 2 and sys.argv[1] in ("run", "assembly", "matfree"):
         n = int(sys.argv[sys.argv.index("--n") + 1]) if "--n" in sys.argv else 100
-        (run if sys.argv[1] == "run" else assembly)(n)
+        {"run": run, "assembly": assembly, "matfree": matfree}[sys.argv[1]](n)
     elif len(sys.argv) >= 4 and sys.argv[1] == "parse":
         nnzb = int(sys.argv[4]) if len(sys.argv) > 4 else 27270901
         nn = int(sys.argv[5]) if len(sys.argv) > 5 else 1030301

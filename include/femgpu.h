@@ -354,8 +354,10 @@ typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1, FEMGPU_SPMV_MATRIX_FRE
 FEMGPU_API int femgpu_apply(femgpu_problem *p, int precision, const double *x, double *y, size_t n);
 /* The matrix-free product's layout: info[0] element blocks (256 element slots each, sorted along a
  * Morton curve), [1] sub-colours, [2] block-local node entries, [3] nodes shared by several blocks,
- * [4] their partial slots, [5] nodes of no element. */
-FEMGPU_API int femgpu_matrix_free_info(femgpu_problem *p, uint64_t info[6]);
+ * [4] their partial slots, [5] nodes of no element, [6] elements that are axis-aligned boxes (their
+ * constant diagonal Jacobian takes the reduced product; when every element is one, a kernel
+ * without the general path serves). */
+FEMGPU_API int femgpu_matrix_free_info(femgpu_problem *p, uint64_t info[7]);
 /* Average device times (ms) of the matrix-free element-block kernel alone and of the whole product
  * (element blocks, shared nodes, penalties) of precision FEMGPU_FP64 or the FP32 one, over reps
  * launches each on the problem's stream (needs an assembled matrix). */

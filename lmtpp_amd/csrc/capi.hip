@@ -1125,7 +1125,7 @@ int femgpu_time_matrix_free(femgpu_problem *h, int precision, int reps, double *
     return FEMGPU_OK;
 }
 
-int femgpu_matrix_free_info(femgpu_problem *h, uint64_t info[6]) {
+int femgpu_matrix_free_info(femgpu_problem *h, uint64_t info[7]) {
     if (!h || !info) return fail(FEMGPU_BAD_INPUT, "matrix_free_info: null");
     const int rc = mf_build(h->P);
     if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "matrix_free_info: not a Hexa problem with one "
@@ -1139,6 +1139,7 @@ int femgpu_matrix_free_info(femgpu_problem *h, uint64_t info[6]) {
     info[3] = M.n_bnodes;
     info[4] = M.n_halo;
     info[5] = M.n_iso;
+    info[6] = M.n_boxes;
     return FEMGPU_OK;
 }
 

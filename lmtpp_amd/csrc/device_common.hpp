@@ -128,6 +128,12 @@ __device__ inline void finish_pq(ProbScalars &P, double pq) {
     else P.alpha = P.rho / pq;
 }
 
+// Direction update p = z + beta p of one value, with one rounding (every kernel that forms the
+// direction uses this, so the separate update and the one fused into the matrix-free product
+// agree bit for bit).
+__device__ inline float dir_value(float z, float beta, float p) { return __builtin_fmaf(beta, p, z); }
+__device__ inline double dir_value(double z, double beta, double p) { return __builtin_fma(beta, p, z); }
+
 struct WgTables {
     const uint32_t *row0, *row1, *sub, *sub_wg0, *sub_nwg;
 };

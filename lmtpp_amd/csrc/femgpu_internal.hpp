@@ -76,7 +76,8 @@ struct SymDev {
 
 // Matrix-free product y = K x of a Hexa problem with one sub-problem: K_e x_e evaluated per element
 // with the element loop's quadrature.  Elements are sorted along a Morton curve of their centres
-// and cut into blocks of MF_BLOCK; a workgroup sums its block's nodal forces in LDS (sub-colour by
+// and cut into blocks of MF_BLOCK; a workgroup stages the input at its block's local nodes in
+// LDS, reads its elements' corners there by block-local index, sums their nodal forces in LDS (sub-colour by
 // sub-colour, so every sum has a fixed order), adds the penalty terms of the constrained DOFs it
 // is assigned (each to the first block holding its node), and stores the nodes all of whose
 // elements are in the block; a node shared by several blocks gets one partial per block, summed in
@@ -85,13 +86,18 @@ struct SymDev {
 // is the assembled K up to rounding even when the material changed later
 // (assume_constant_matrix).
 constexpr uint32_t MF_BLOCK = 256;
+constexpr uint32_t MF_EMPTY = 0xffu, MF_BOX = 0x40u, MF_COLOR_BITS = 0x3fu;
 struct MatFree {
     bool built = false, snap = false;
+    bool all_boxes = false;          // every element an axis-aligned box (the box-only kernel serves)
+    uint32_t n_boxes = 0;            // elements that are axis-aligned boxes
+    void *d_rec64 = nullptr, *d_rec32 = nullptr;  // per slot of a box: edge lengths, Hooke coefficients
+    bool rec_valid = false;          // the records hold the last snapshot's material
     uint32_t n_blocks = 0, n_colors = 0, max_local = 0, n_bnodes = 0, n_halo = 0, n_iso = 0;
     uint32_t n_slots = 0, n_fix = 0, cap_fix = 0, cap_slots = 0;
     uint32_t *d_conn = nullptr;      // 8 nodes per block slot (MF_BLOCK slots per block)
     uint16_t *d_loc = nullptr;       // 8 block-local node indices per slot
-    uint8_t *d_color = nullptr;      // sub-colour per slot (0xff: empty slot)
+    uint8_t *d_color = nullptr;      // sub-colour per slot | MF_BOX for axis-aligned boxes (MF_EMPTY: empty slot)
     uint32_t *d_elem = nullptr;      // element per slot (0 for empty slots)
     uint32_t *d_lptr = nullptr;      // block b: local nodes [lptr[b], lptr[b + 1])
     uint32_t *d_lnode = nullptr;     // global node of each local node
@@ -101,9 +107,9 @@ struct MatFree {
     std::vector<uint32_t> node_blk, node_loc;  // first block holding each node, its local index (~0: none)
     uint32_t *d_idof = nullptr;      // the 3 DOFs of each node of no element ...
     uint32_t *d_isrc = nullptr;      // ... and the index of their penalty in d_fpen (~0: none)
-    // penalty terms per block: block b adds fpen[psrc[k]] x[pdof[k]] to local entry pidx[k],
-    // k in [pptr[b], pptr[b + 1]); built for the DOF list pen_dofs
-    uint32_t *d_pptr = nullptr, *d_pidx = nullptr, *d_pdof = nullptr, *d_psrc = nullptr;
+    // penalty terms per block: block b adds fpen[psrc[k]] x_l to local entry l = pidx[k] (x_l its
+    // staged input), k in [pptr[b], pptr[b + 1]); built for the DOF list pen_dofs
+    uint32_t *d_pptr = nullptr, *d_pidx = nullptr, *d_psrc = nullptr;
     std::vector<uint32_t> snap_dofs, pen_dofs;
     bool pen_built = false;
     double *d_E = nullptr;           // E per element at the last matrix assembly
@@ -260,9 +266,10 @@ hipError_t mf_snapshot(Problem &P);
 // Builds what the product needs outside any graph capture: returns 0, 1 (not eligible, or no
 // matrix assembled yet) or 5.
 int mf_prepare(Problem &P);
-// y = K x (pq: on the active sub-problem, with the p.q partials in mf.d_partials).
-hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq);
-hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq);
+// y = K x (pq: on the active sub-problem, with the p.q partials in mf.d_partials).  With z
+// (pq only): y = K p for the CG direction p = z + beta x (beta of the scalars), stored into x.
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z = nullptr);
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z = nullptr);
 void mf_free(MatFree &M);
 // Device time (ms) of the element-block kernel alone and of the whole product; returns 0, 1 or 5.
 int mf_time(Problem &P, int precision, int reps, double *block_ms, double *product_ms);

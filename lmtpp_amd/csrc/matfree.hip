@@ -1,24 +1,29 @@
 // Matrix-free block product y = K x for Hexa problems on gfx950.
 //
 // The assembled FP32 product reads about 1 KB of matrix per node (27 blocks of 36 bytes) and runs
-// at the HBM read bound.  The same product evaluated element by element reads 32 bytes of
-// connectivity and one modulus per element, plus the coordinates and x of its nodes, and spends
-// about 1.5k vector instructions instead: K_e x_e is the element loop's quadrature of the
-// elasticity weak form (formulation.h:626-647 with formulations/formulation_elasticity_iso.py:45-60,
-// 2x2x2 Gauss points of formulation.h:2917-2933), applied without forming K_e:
+// at the HBM read bound.  The same product evaluated element by element reads a few tens of bytes
+// per element (block-local corner indices and either its nodes and modulus or, for a box, one
+// record), and spends vector instructions instead: K_e x_e is the element loop's quadrature of
+// the elasticity weak form (formulation.h:626-647 with
+// formulations/formulation_elasticity_iso.py:45-60, 2x2x2 Gauss points of
+// formulation.h:2917-2933), applied without forming K_e:
 //   per Gauss point q: G = grad u (3x3), sigma = H eps(G), f_a += w_q detJ_q sigma grad N_a,
 // and, the Hexa shape functions being products phi(xi0) phi(xi1) phi(xi2), every reference
 // derivative at the 8 Gauss points comes from 1D interpolations of lattice differences (sum
 // factorisation), the transposed projection the same way.  With adj = detJ J^-1, the Gauss point
-// term is (w_q / detJ) sigma(Gr adj^T) adj: one division per point.
+// term is (w_q / detJ) sigma(Gr adj^T) adj: one division per point.  Elements that are
+// axis-aligned boxes (every lattice mesh) have a constant diagonal J: adj, w / detJ and the
+// Hooke coefficients come from a per-element record and each point needs a third of the work.
 //
 // Scatter without atomics: elements sorted along a Morton curve of their centres form blocks of
-// MF_BLOCK, one workgroup each (compact bricks on lattices).  The workgroup adds its elements'
-// nodal forces into LDS sub-colour by sub-colour (no two elements of a sub-colour share a node),
-// then stores the nodes whose elements all lie in the block and, for a node shared with other
-// blocks, its partial in a slot of its own; a node-parallel pass sums each shared node's partials
-// in block order.  Every sum has a fixed order: the product is deterministic.  The penalty
-// diagonal of the constraints follows in a pass over the constrained DOFs.
+// MF_BLOCK, one workgroup each (compact bricks on lattices).  The workgroup stages the input at its
+// block's nodes in LDS, adds its elements' nodal forces into LDS sub-colour by sub-colour (no two
+// elements of a sub-colour share a node), then stores the nodes whose elements all lie in the
+// block and, for a node shared with other blocks, its partial in a slot of its own; a
+// node-parallel pass sums each shared node's partials in block order.  Every sum has a fixed
+// order: the product is deterministic.  The penalty diagonal of the constraints is added per
+// block, and nodes of no element get it in a pass of their own.  In the CG iteration the direction
+// update p = z + beta p is fused into these passes.
 #include "device_common.hpp"
 #include "generated/elasticity_iso.inc"
 
@@ -106,6 +111,66 @@ __device__ inline T adjugate(const T (&J)[3][3], T (&A)[3][3]) {
 __device__ inline double wq_over(double det, double w) { return w / det; }
 __device__ inline float wq_over(float det, float w) { return w * __builtin_amdgcn_rcpf(det); }
 
+// Ud[d][s][t][c] = d u_c / d xi_d at the Gauss points (s, t) of the two other axes (d = 0:
+// (y, z), 1: (x, z), 2: (x, y)) from the corner values u (sum factorisation: 1D interpolations
+// of lattice differences).
+template <typename T>
+__device__ inline void ref_gradients(const T (&u)[8][3], const T (&g)[2], T (&Ud)[3][2][2][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T DU[3][2][2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                DU[0][p][q] = u[HN[1][p][q]][c] - u[HN[0][p][q]][c];
+                DU[1][p][q] = u[HN[p][1][q]][c] - u[HN[p][0][q]][c];
+                DU[2][p][q] = u[HN[p][q][1]][c] - u[HN[p][q][0]][c];
+            }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T W[2][2];
+            interp2(DU[d], g, W);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) Ud[d][s][t][c] = W[s][t];
+        }
+    }
+}
+
+// f_a = sum_d phi'_{a_d} (projection of Pd[d] onto the two other lattice axes)
+template <typename T>
+__device__ inline void project_out(const T (&Pd)[3][2][2][3], const T (&w1)[2][2], T (&f)[8][3]) {
+    T Rd[3][2][2][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T Pin[2][2], Rout[2][2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) Pin[s][t] = Pd[d][s][t][c];
+            project2(Pin, w1, Rout);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) Rd[d][s][t][c] = Rout[s][t];
+        }
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+        for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+            for (int az = 0; az < 2; ++az)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    f[HN[ax][ay][az]][c] = (ax ? Rd[0][ay][az][c] : -Rd[0][ay][az][c]) +
+                                           (ay ? Rd[1][ax][az][c] : -Rd[1][ax][az][c]) +
+                                           (az ? Rd[2][ax][ay][c] : -Rd[2][ax][ay][c]);
+}
+
 // f[a][c] = (K_e u_e) of one Hexa with node coordinates X and displacements u, H as
 // (H11 - H12, H12, H44 / 2).
 template <typename T>
@@ -114,13 +179,12 @@ __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T
     hexa_gauss(g0d, g1d, wq);
     const T g[2] = {(T)g0d, (T)g1d};
     const T w1[2][2] = {{(T)(1.0 - g0d), (T)(1.0 - g1d)}, {(T)g0d, (T)g1d}};
-    // Jd[d][s][t][c] = d x_c / d xi_d and Ud[d][s][t][c] = d u_c / d xi_d at the Gauss points
-    // (s, t) of the two other axes (d = 0: (y, z), 1: (x, z), 2: (x, y)); the coordinate
-    // differences are taken in FP64 before rounding to T.
+    // Jd[d][s][t][c] = d x_c / d xi_d at the Gauss points (s, t) of the two other axes; the
+    // coordinate differences are taken in FP64 before rounding to T.
     T Jd[3][2][2][3], Ud[3][2][2][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        T DX[3][2][2], DU[3][2][2];
+        T DX[3][2][2];
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -128,24 +192,18 @@ __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T
                 DX[0][p][q] = (T)(X[HN[1][p][q]][c] - X[HN[0][p][q]][c]);
                 DX[1][p][q] = (T)(X[HN[p][1][q]][c] - X[HN[p][0][q]][c]);
                 DX[2][p][q] = (T)(X[HN[p][q][1]][c] - X[HN[p][q][0]][c]);
-                DU[0][p][q] = u[HN[1][p][q]][c] - u[HN[0][p][q]][c];
-                DU[1][p][q] = u[HN[p][1][q]][c] - u[HN[p][0][q]][c];
-                DU[2][p][q] = u[HN[p][q][1]][c] - u[HN[p][q][0]][c];
             }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            T V[2][2], W[2][2];
+            T V[2][2];
             interp2(DX[d], g, V);
-            interp2(DU[d], g, W);
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    Jd[d][s][t][c] = V[s][t];
-                    Ud[d][s][t][c] = W[s][t];
-                }
+                for (int t = 0; t < 2; ++t) Jd[d][s][t][c] = V[s][t];
         }
     }
+    ref_gradients(u, g, Ud);
     // Gauss point (i, j, k): S = (w / detJ) sigma(Gr adj^T) adj, summed into Pd over the axis each
     // reference derivative does not depend on.
     T Pd[3][2][2][3];
@@ -195,107 +253,222 @@ __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T
                     Pd[2][i][j][c] += sg[c][0] * Aj[0][2] + sg[c][1] * Aj[1][2] + sg[c][2] * Aj[2][2];
                 }
             }
-    // f_a = sum_d phi'_{a_d} (projection of Pd[d] onto the two other lattice axes)
-    T Rd[3][2][2][3];
+    project_out(Pd, w1, f);
+}
+
+// The same for an axis-aligned box (every edge along xi_d is h_d e_d): J = diag(h) at every
+// Gauss point, so adj = diag(h1 h2, h0 h2, h0 h1) and w / detJ are constant, and the point term
+// (w / detJ) sigma(Gr adj^T) adj needs the diagonal only: 30 instead of about 90 operations per
+// point, and no geometry (the "Cartesian" cell of matrix-free FE codes).
+template <typename T>
+__device__ inline void hexa_apply_box(const T (&h)[3], const T (&u)[8][3], T h01, T h1, T h2, T (&f)[8][3]) {
+    double g0d, g1d, wq;
+    hexa_gauss(g0d, g1d, wq);
+    const T g[2] = {(T)g0d, (T)g1d};
+    const T w1[2][2] = {{(T)(1.0 - g0d), (T)(1.0 - g1d)}, {(T)g0d, (T)g1d}};
+    T Ud[3][2][2][3];
+    ref_gradients(u, g, Ud);
+    const T ad[3] = {h[1] * h[2], h[0] * h[2], h[0] * h[1]};
+    const T kq = wq_over(h[0] * ad[0], (T)wq);
+    const T bd[3] = {kq * ad[0], kq * ad[1], kq * ad[2]};
+    T Pd[3][2][2][3];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            T Pin[2][2], Rout[2][2];
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) Pin[s][t] = Pd[d][s][t][c];
-            project2(Pin, w1, Rout);
+                for (int c = 0; c < 3; ++c) Pd[d][s][t][c] = 0;
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) Rd[d][s][t][c] = Rout[s][t];
-        }
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int ax = 0; ax < 2; ++ax)
+            for (int k = 0; k < 2; ++k) {
+                T G[3][3];  // detJ * d u_c / d x_m
 #pragma unroll
-        for (int ay = 0; ay < 2; ++ay)
+                for (int c = 0; c < 3; ++c) {
+                    G[c][0] = Ud[0][j][k][c] * ad[0];
+                    G[c][1] = Ud[1][i][k][c] * ad[1];
+                    G[c][2] = Ud[2][i][j][c] * ad[2];
+                }
+                const T htr = h1 * (G[0][0] + G[1][1] + G[2][2]);
+                T sg[3][3];  // sigma(G) / (w / detJ)
 #pragma unroll
-            for (int az = 0; az < 2; ++az)
+                for (int c = 0; c < 3; ++c) sg[c][c] = h01 * G[c][c] + htr;
+                sg[0][1] = sg[1][0] = h2 * (G[0][1] + G[1][0]);
+                sg[0][2] = sg[2][0] = h2 * (G[0][2] + G[2][0]);
+                sg[1][2] = sg[2][1] = h2 * (G[1][2] + G[2][1]);
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    f[HN[ax][ay][az]][c] = (ax ? Rd[0][ay][az][c] : -Rd[0][ay][az][c]) +
-                                           (ay ? Rd[1][ax][az][c] : -Rd[1][ax][az][c]) +
-                                           (az ? Rd[2][ax][ay][c] : -Rd[2][ax][ay][c]);
+                for (int c = 0; c < 3; ++c) {
+                    Pd[0][j][k][c] += sg[c][0] * bd[0];
+                    Pd[1][i][k][c] += sg[c][1] * bd[1];
+                    Pd[2][i][j][c] += sg[c][2] * bd[2];
+                }
+            }
+    project_out(Pd, w1, f);
+}
+
+// Per-slot record of a box element for the product in precision T: its edge lengths and the Hooke
+// coefficients (H11 - H12, H12, H44 / 2) of its modulus at the last matrix assembly, rounded to T
+// exactly as the general path rounds them.  24 B in FP32 instead of the 44 B of nodes, element
+// index and modulus plus the corner gathers.
+template <typename T>
+struct BoxRec {
+    T h[3], c[3];
+};
+
+template <typename T>
+__device__ inline void box_coefs(const double (&X0)[3], const double (&X6)[3], double E, double nu, BoxRec<T> &r) {
+    double Hd[3];
+    gen::hooke(E, nu, Hd);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r.h[d] = (T)(X6[d] - X0[d]);
+    r.c[0] = (T)(Hd[0] - Hd[1]);
+    r.c[1] = (T)Hd[1];
+    r.c[2] = (T)(0.5 * Hd[2]);
+}
+
+// The records of every box slot, both precisions (after each snapshot of the material).
+__global__ __launch_bounds__(256) void k_mf_boxrec(size_t n, const uint8_t *__restrict__ color,
+                                                   const uint32_t *__restrict__ conn, const uint32_t *__restrict__ elem,
+                                                   const double *__restrict__ coords, const double *__restrict__ E,
+                                                   const SubMaterial *__restrict__ mat, BoxRec<double> *__restrict__ r64,
+                                                   BoxRec<float> *__restrict__ r32) {
+    const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t cb = color[s];
+    if (cb == MF_EMPTY || !(cb & MF_BOX)) return;
+    double X0[3], X6[3];
+    for (int d = 0; d < 3; ++d) {
+        X0[d] = coords[3 * (size_t)conn[8 * s] + d];
+        X6[d] = coords[3 * (size_t)conn[8 * s + 6] + d];
+    }
+    box_coefs<double>(X0, X6, E[elem[s]], mat[0].nu, r64[s]);
+    box_coefs<float>(X0, X6, E[elem[s]], mat[0].nu, r32[s]);
+}
+
+// Dynamic LDS of k_mf_block for blocks of up to max_local nodes and values of vsize bytes.
+inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
+    return (size_t)std::max<uint32_t>(1, max_local) * 6 * vsize;
 }
 
 // One block of MF_BLOCK element slots per workgroup (logical block b = xcd_slab(), so each XCD
 // takes a contiguous run of blocks): y of the block's own nodes, partials of its shared ones.
+// The workgroup first stages the product's input at its block's local nodes in LDS (one
+// coalesced pass over the node list), and each element lane reads its 8 corners' values there by
+// their block-local indices; the corner coordinates (FP64, read-only) are gathered from memory.
 // PQ: partial of sum_e x_e . (K_e x_e) = sum over the block's local nodes of x . (their summed
 // element forces), taken at the write-out; with the penalty pass's partials it is x . K x.
-template <typename T, bool PQ>
+// DIR: the CG direction update is fused in: the product is applied to p = z + beta x (beta of
+// the scalars), and the block stores p into x at the nodes it owns (no other block reads them;
+// k_mf_halo stores the shared ones after every block has read them).
+// LDS: per local node 3 input values and 3 accumulators.
+template <typename T, bool PQ, bool DIR, bool BOXES>
 __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   uint32_t n_colors, const uint32_t *__restrict__ lptr,
                                                   const uint32_t *__restrict__ lnode, const uint32_t *__restrict__ lslot,
                                                   const uint32_t *__restrict__ pptr, const uint32_t *__restrict__ pidx,
-                                                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ psrc,
-                                                  const double *__restrict__ fpen,
+                                                  const uint32_t *__restrict__ psrc, const double *__restrict__ fpen,
                                                   const double *__restrict__ coords, const double *__restrict__ E,
-                                                  const SubMaterial *__restrict__ mat, const T *__restrict__ x,
+                                                  const SubMaterial *__restrict__ mat,
+                                                  const BoxRec<T> *__restrict__ rec, T *x, const T *__restrict__ z,
                                                   T *__restrict__ y, T *__restrict__ halo, const ProbScalars *S,
                                                   double *partials) {
-    extern __shared__ unsigned char mf_lds[];
-    T *acc = reinterpret_cast<T *>(mf_lds);
+    extern __shared__ double mf_lds[];
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
     const unsigned b = xcd_slab();
     const uint32_t l0 = lptr[b], nl = lptr[b + 1] - l0;
-    for (uint32_t i = threadIdx.x; i < 3 * nl; i += MF_BLOCK) acc[i] = 0;
+    T *Ps = reinterpret_cast<T *>(mf_lds);  // [nl][3] the product's input
+    T *acc = Ps + 3 * nl;                   // [nl][3] summed nodal forces
+    const T beta = DIR ? (T)S[0].beta : (T)0;
+    for (uint32_t i = threadIdx.x; i < nl; i += MF_BLOCK) {
+        const size_t g = 3 * (size_t)lnode[l0 + i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            Ps[3 * i + d] = DIR ? dir_value(z[g + d], beta, x[g + d]) : x[g + d];
+            acc[3 * i + d] = 0;
+        }
+    }
     const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
-    const uint32_t col = color[slot];
+    const uint32_t cbits = color[slot];
+    const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
+    const bool box = BOXES || (cbits != MF_EMPTY && (cbits & MF_BOX));
     double pq[1] = {0.0};
     T f[8][3];
     uint32_t lo[8];
-    if (col != 0xffu) {
-        const uint4 ca = reinterpret_cast<const uint4 *>(conn)[2 * slot];
-        const uint4 cb = reinterpret_cast<const uint4 *>(conn)[2 * slot + 1];
-        const uint32_t nd[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    double X[8][3];  // corners of a general element
+    double Hd[3] = {0, 0, 0};
+    BoxRec<T> br;    // a box: edge lengths and Hooke coefficients
+    if (col != MF_EMPTY) {
         const uint4 lq = reinterpret_cast<const uint4 *>(loc)[slot];
         lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
         lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
-        double Hd[3];
-        gen::hooke(E[elem[slot]], mat[0].nu, Hd);
-        double X[8][3];
+        if (box) {
+            br = rec[slot];
+        } else {
+            const uint4 ca = reinterpret_cast<const uint4 *>(conn)[2 * slot];
+            const uint4 cb = reinterpret_cast<const uint4 *>(conn)[2 * slot + 1];
+            const uint32_t nd[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+            gen::hooke(E[elem[slot]], mat[0].nu, Hd);
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * (size_t)nd[a] + c];
+        }
+    }
+    __syncthreads();
+    if (col != MF_EMPTY) {
         T u[8][3];
 #pragma unroll
         for (int a = 0; a < 8; ++a)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                X[a][c] = coords[3 * (size_t)nd[a] + c];
-                u[a][c] = x[3 * (size_t)nd[a] + c];
-            }
-        hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
+            for (int c = 0; c < 3; ++c) u[a][c] = Ps[3 * lo[a] + c];
+        if (box) hexa_apply_box<T>(br.h, u, br.c[0], br.c[1], br.c[2], f);
+        else hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
     }
     __syncthreads();
     // no two lanes of a sub-colour touch one entry, so the order of every sum is the sub-colour
     // order; plain read-add-write (LDS float atomics measured 2.6x slower here in FP32)
+    // (an element's 8 nodes are distinct, so all 24 entries are read before any is written: one
+    // LDS round trip per sub-colour instead of eight dependent ones)
     for (uint32_t c = 0; c < n_colors; ++c) {
-        if (col == c)
+        if (col == c) {
+            T cur[8][3];
 #pragma unroll
             for (int a = 0; a < 8; ++a)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) acc[3 * lo[a] + d] += f[a][d];
+                for (int d = 0; d < 3; ++d) cur[a][d] = acc[3 * lo[a] + d];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) acc[3 * lo[a] + d] = cur[a][d] + f[a][d];
+        }
         __syncthreads();
     }
-    // penalty terms of the constrained DOFs assigned to this block (distinct entries)
+    // penalty terms of the constrained DOFs assigned to this block (distinct local entries)
     const uint32_t p1 = pptr[b + 1];
-    for (uint32_t k = pptr[b] + threadIdx.x; k < p1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * x[pdof[k]];
+    for (uint32_t k = pptr[b] + threadIdx.x; k < p1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * Ps[pidx[k]];
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nl; i += MF_BLOCK) {
         const uint32_t s = lslot[l0 + i], g = lnode[l0 + i];
-        T *dst = s == 0xffffffffu ? y + 3 * (size_t)g : halo + 3 * (size_t)s;
+        const bool own = s == 0xffffffffu;
+        T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)s;
         const T a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
         dst[0] = a0;
         dst[1] = a1;
         dst[2] = a2;
-        if (PQ) pq[0] += (double)x[3 * (size_t)g] * (double)a0 + (double)x[3 * (size_t)g + 1] * (double)a1 +
-                         (double)x[3 * (size_t)g + 2] * (double)a2;
+        if (PQ || DIR) {
+            const T x0 = Ps[3 * i], x1 = Ps[3 * i + 1], x2 = Ps[3 * i + 2];
+            if (PQ) pq[0] += (double)x0 * (double)a0 + (double)x1 * (double)a1 + (double)x2 * (double)a2;
+            if (DIR && own) {
+                x[3 * (size_t)g] = x0;
+                x[3 * (size_t)g + 1] = x1;
+                x[3 * (size_t)g + 2] = x2;
+            }
+        }
     }
     if (PQ) wg_partial<1, 0u>(pq, partials, 0u, b);
 }
@@ -303,11 +476,12 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
 // Shared nodes: y = sum of the blocks' partials, in block order.  PQ: the first workgroup also
 // turns the p.q partials of the element blocks and of the nodes-of-no-element pass (complete:
 // written by earlier kernels) into alpha, which the vector update reads after this kernel.
-template <typename T, bool PQ>
+// DIR: also stores the updated direction p = z + beta x of the shared nodes into x.
+template <typename T, bool PQ, bool DIR>
 __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__restrict__ bnode,
                                                  const uint32_t *__restrict__ bptr, const T *__restrict__ halo,
                                                  T *__restrict__ y, ProbScalars *S, const double *__restrict__ partials,
-                                                 uint32_t n_slots) {
+                                                 uint32_t n_slots, T *__restrict__ x, const T *__restrict__ z) {
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
     if (PQ && blockIdx.x == 0) {
         double o[1];
@@ -316,6 +490,12 @@ __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__r
     }
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
+    if (DIR) {
+        const T beta = (T)S[0].beta;
+        const size_t g = 3 * (size_t)bnode[i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[g + d] = dir_value(z[g + d], beta, x[g + d]);
+    }
     T v0 = 0, v1 = 0, v2 = 0;
     for (uint32_t k = bptr[i]; k < bptr[i + 1]; ++k) {
         v0 += halo[3 * (size_t)k];
@@ -330,17 +510,19 @@ __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__r
 
 // Nodes of no element: their rows hold the penalty alone, y = M P x on their constrained DOFs and
 // 0 on the others.
-template <typename T, bool PQ>
+// DIR: applied to (and stores) the updated direction p = z + beta x, as k_mf_block.
+template <typename T, bool PQ, bool DIR>
 __global__ __launch_bounds__(256) void k_mf_iso(uint32_t n, const uint32_t *__restrict__ dof,
                                                 const uint32_t *__restrict__ src, const double *__restrict__ fpen,
-                                                const T *__restrict__ x, T *__restrict__ y, const ProbScalars *S,
-                                                double *partials, uint32_t slot0) {
+                                                T *__restrict__ x, const T *__restrict__ z, T *__restrict__ y,
+                                                const ProbScalars *S, double *partials, uint32_t slot0) {
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     double acc[1] = {0.0};
     if (i < n) {
         const uint32_t d = dof[i], k = src[i];
-        const T xv = x[d];
+        const T xv = DIR ? dir_value(z[d], (T)S[0].beta, x[d]) : x[d];
+        if (DIR) x[d] = xv;
         const T a = k == 0xffffffffu ? (T)0 : (T)fpen[k] * xv;
         y[d] = a;
         acc[0] = (double)a * (double)xv;
@@ -371,12 +553,33 @@ bool mf_eligible(const Problem &P) {
 void mf_free(MatFree &M) {
     dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr); dfree(M.d_lnode);
     dfree(M.d_lslot); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof); dfree(M.d_isrc);
-    dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_pdof); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
-    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_wg0); dfree(M.d_nwg);
+    dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
+    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_wg0); dfree(M.d_nwg);
     M = MatFree();
 }
 
 namespace {
+
+// An axis-aligned box: every edge along xi_d is (h_d > 0) e_d, with one h_d per axis, compared
+// exactly (so the box product sees the coordinate differences of the general one).
+bool is_box(const double *X, const uint32_t *nd) {
+    for (int d = 0; d < 3; ++d) {
+        double hd = 0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = 0; q < 2; ++q) {
+                const int a = d == 0 ? HN[0][p][q] : d == 1 ? HN[p][0][q] : HN[p][q][0];
+                const int b = d == 0 ? HN[1][p][q] : d == 1 ? HN[p][1][q] : HN[p][q][1];
+                for (int c = 0; c < 3; ++c) {
+                    const double v = X[3 * (size_t)nd[b] + c] - X[3 * (size_t)nd[a] + c];
+                    if (c != d && v != 0.0) return false;
+                    if (c == d && (p || q) && v != hd) return false;
+                    if (c == d) hd = v;
+                }
+            }
+        if (!(hd > 0)) return false;
+    }
+    return true;
+}
 
 // 21-bit-per-axis Morton code (bit 3k + d holds bit k of axis d).
 uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
@@ -484,14 +687,14 @@ int mf_build(Problem &P) {
             if (used == 0xffffffffu) return 1;  // more than 32 sub-colours: not served
             const uint32_t c = (uint32_t)__builtin_ctz(~used);
             for (int a = 0; a < 8; ++a) mask[lid[conn[8 * (size_t)e + a]]] |= 1u << c;
-            scol[i] = (uint8_t)c;
+            scol[i] = (uint8_t)(c | (is_box(X.data(), conn + 8 * (size_t)e) ? MF_BOX : 0u));
             selem[i] = e;
             n_colors = std::max(n_colors, c + 1);
         }
         max_local = std::max<uint32_t>(max_local, (uint32_t)lnode.size() - lptr[b]);
     }
     lptr[nb] = (uint32_t)lnode.size();
-    if (3ull * max_local * 8 > 64 * 1024) return 1;  // LDS accumulator of the FP64 product
+    if (mf_lds_bytes(max_local, 8) > 64 * 1024) return 1;  // LDS of the FP64 product's blocks
     // shared nodes: a partial slot per (node, block), grouped by node in block order
     std::vector<uint32_t> nshared(nn, 0);
     for (size_t l = 0; l < lnode.size(); ++l)
@@ -514,12 +717,16 @@ int mf_build(Problem &P) {
         !upload_vec(&M.d_lslot, lslot) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
         !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
+        hipMalloc(&M.d_rec64, sizeof(BoxRec<double>) * ns) != hipSuccess ||
+        hipMalloc(&M.d_rec32, sizeof(BoxRec<float>) * ns) != hipSuccess ||
         hipMalloc(&M.d_halo, sizeof(double) * 3 * std::max<size_t>(1, bptr.back())) != hipSuccess) {
         mf_free(M);
         return 5;
     }
     M.n_blocks = nb;
     M.n_colors = n_colors;
+    M.n_boxes = (uint32_t)std::count_if(scol.begin(), scol.end(), [](uint8_t c) { return c != MF_EMPTY && (c & MF_BOX); });
+    M.all_boxes = M.n_boxes == ne;
     M.max_local = max_local;
     M.n_bnodes = (uint32_t)bnode.size();
     M.n_halo = bptr.back();
@@ -550,6 +757,7 @@ hipError_t mf_snapshot(Problem &P) {
     M.snap_dofs = P.cdof_h;
     M.n_fix = P.n_cdof;
     M.snap = true;
+    M.rec_valid = false;
     return hipGetLastError();
 }
 
@@ -571,18 +779,17 @@ static int mf_penalties(Problem &P) {
             per[M.node_blk[n]].push_back(i);
         }
     }
-    std::vector<uint32_t> pptr(1, 0), pidx, pdof, psrc;
+    std::vector<uint32_t> pptr(1, 0), pidx, psrc;
     for (uint32_t b = 0; b < M.n_blocks; ++b) {
         for (uint32_t i : per[b]) {
             const uint32_t dof = M.snap_dofs[i], n = dof / 3;
             pidx.push_back(3 * M.node_loc[n] + dof % 3);
-            pdof.push_back(dof);
             psrc.push_back(i);
         }
         pptr.push_back((uint32_t)pidx.size());
     }
-    dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_pdof); dfree(M.d_psrc);
-    if (!upload_vec(&M.d_pptr, pptr) || !upload_vec(&M.d_pidx, pidx) || !upload_vec(&M.d_pdof, pdof) ||
+    dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc);
+    if (!upload_vec(&M.d_pptr, pptr) || !upload_vec(&M.d_pidx, pidx) ||
         !upload_vec(&M.d_psrc, psrc) ||
         (M.n_iso && hipMemcpy(M.d_isrc, isrc.data(), isrc.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
         return 5;
@@ -602,6 +809,14 @@ int mf_prepare(Problem &P) {
     if (M.n_fix && !M.d_fpen) return 5;
     rc = mf_penalties(P);
     if (rc) return rc;
+    if (!M.rec_valid && M.n_boxes) {  // stream order: after the snapshot's copies
+        const size_t ns = (size_t)M.n_blocks * MF_BLOCK;
+        hipLaunchKernelGGL(k_mf_boxrec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, P.stream, ns, M.d_color, M.d_conn,
+                           M.d_elem, P.d_coords, M.d_E, M.d_mat, reinterpret_cast<BoxRec<double> *>(M.d_rec64),
+                           reinterpret_cast<BoxRec<float> *>(M.d_rec32));
+        if (hipGetLastError() != hipSuccess) return 5;
+    }
+    M.rec_valid = true;
     const uint32_t s = M.n_blocks + (3 * M.n_iso + 255) / 256;
     if (s > M.cap_slots) {
         dfree(M.d_partials);
@@ -619,27 +834,39 @@ int mf_prepare(Problem &P) {
     return 0;
 }
 
-template <typename T>
-static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq) {
+// Element-block kernel of one variant (BOXES: every element of the mesh is an axis-aligned box).
+template <typename T, bool PQ, bool DIR, bool BOXES>
+static void mf_block_launch(Problem &P, T *x, const T *z, T *y) {
     const MatFree &M = P.mf;
-    T *halo = reinterpret_cast<T *>(M.d_halo);
-    const size_t lds = 3 * (size_t)std::max<uint32_t>(1, M.max_local) * sizeof(T);
-#define FEMGPU_MF_PASSES(PQ)                                                                                          \
-    hipLaunchKernelGGL((k_mf_block<T, PQ>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn, M.d_loc,        \
-                       M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx, M.d_pdof,  \
-                       M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, x, y, halo, P.d_ps, M.d_partials);              \
-    if (M.n_iso)                                                                                                       \
-        hipLaunchKernelGGL((k_mf_iso<T, PQ>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,     \
-                           M.d_idof, M.d_isrc, M.d_fpen, x, y, P.d_ps, M.d_partials, M.n_blocks);                     \
-    if (M.n_bnodes)                                                                                                    \
-        hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,      \
-                           M.d_bnode, M.d_bptr, halo, y, P.d_ps, M.d_partials, M.n_slots);
-    if (pq) {
-        FEMGPU_MF_PASSES(true)
-    } else {
-        FEMGPU_MF_PASSES(false)
-    }
-#undef FEMGPU_MF_PASSES
+    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES>), dim3(M.n_blocks), dim3(MF_BLOCK), mf_lds_bytes(M.max_local, sizeof(T)),
+                       P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot,
+                       M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat,
+                       reinterpret_cast<const BoxRec<T> *>(sizeof(T) == 4 ? (void *)M.d_rec32 : (void *)M.d_rec64), x, z, y,
+                       reinterpret_cast<T *>(M.d_halo), P.d_ps, M.d_partials);
+}
+
+template <typename T, bool PQ, bool DIR>
+static void mf_passes(Problem &P, T *x, const T *z, T *y) {
+    const MatFree &M = P.mf;
+    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, z, y);
+    else mf_block_launch<T, PQ, DIR, false>(P, x, z, y);
+    if (M.n_iso)
+        hipLaunchKernelGGL((k_mf_iso<T, PQ, DIR>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,
+                           M.d_idof, M.d_isrc, M.d_fpen, x, z, y, P.d_ps, M.d_partials, M.n_blocks);
+    if (M.n_bnodes)
+        hipLaunchKernelGGL((k_mf_halo<T, PQ, DIR>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,
+                           M.d_bnode, M.d_bptr, reinterpret_cast<const T *>(M.d_halo), y, P.d_ps, M.d_partials,
+                           M.n_slots, x, z);
+}
+
+// y = K x; with z (PQ only), y = K p for the updated direction p = z + beta x, which is stored
+// into x (the direction update of the CG step, fused into the product's passes).
+template <typename T>
+static hipError_t mf_launch_t(Problem &P, const T *x_in, T *y, bool pq, const T *z) {
+    T *x = const_cast<T *>(x_in);  // written only when z is given
+    if (pq && z) mf_passes<T, true, true>(P, x, z, y);
+    else if (pq) mf_passes<T, true, false>(P, x, nullptr, y);
+    else mf_passes<T, false, false>(P, x, nullptr, y);
     return hipGetLastError();
 }
 
@@ -649,22 +876,18 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     const int rc = mf_prepare(P);
     if (rc) return rc;
     const MatFree &M = P.mf;
-    const size_t lds = 3 * (size_t)std::max<uint32_t>(1, M.max_local) * (precision ? 4 : 8);
     auto block = [&]() {
-        if (precision)
-            hipLaunchKernelGGL((k_mf_block<float, false>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn,
-                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx,
-                               M.d_pdof, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, P.d_p32, P.d_q32,
-                               reinterpret_cast<float *>(M.d_halo), P.d_ps, M.d_partials);
-        else
-            hipLaunchKernelGGL((k_mf_block<double, false>), dim3(M.n_blocks), dim3(MF_BLOCK), lds, P.stream, M.d_conn,
-                               M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot, M.d_pptr, M.d_pidx,
-                               M.d_pdof, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat, P.d_p, P.d_q, M.d_halo, P.d_ps,
-                               M.d_partials);
+        if (precision) {
+            if (M.all_boxes) mf_block_launch<float, false, false, true>(P, P.d_p32, nullptr, P.d_q32);
+            else mf_block_launch<float, false, false, false>(P, P.d_p32, nullptr, P.d_q32);
+        } else {
+            if (M.all_boxes) mf_block_launch<double, false, false, true>(P, P.d_p, nullptr, P.d_q);
+            else mf_block_launch<double, false, false, false>(P, P.d_p, nullptr, P.d_q);
+        }
     };
     auto product = [&]() {
-        if (precision) (void)mf_launch_t<float>(P, P.d_p32, P.d_q32, false);
-        else (void)mf_launch_t<double>(P, P.d_p, P.d_q, false);
+        if (precision) (void)mf_launch_t<float>(P, P.d_p32, P.d_q32, false, nullptr);
+        else (void)mf_launch_t<double>(P, P.d_p, P.d_q, false, nullptr);
     };
     float ms[2] = {0, 0};
     product();
@@ -681,7 +904,11 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     return 0;
 }
 
-hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq) { return mf_launch_t<float>(P, x, y, pq); }
-hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq) { return mf_launch_t<double>(P, x, y, pq); }
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z) {
+    return mf_launch_t<float>(P, x, y, pq, z);
+}
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z) {
+    return mf_launch_t<double>(P, x, y, pq, z);
+}
 
 }  // namespace femgpu

@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_update_p(const T *__restrict__ z, T *__
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const size_t i = 3 * (size_t)n + d;
-        p[i] = first ? z[i] : z[i] + beta * p[i];
+        p[i] = first ? z[i] : dir_value(z[i], beta, p[i]);
     }
 }
 
@@ -348,6 +348,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         wg_combine<1, 0u>(partials, stride, w0, nw, o);
         if (threadIdx.x) return;
         P.rho = o[0];
+        if (what == FIN_RESTART_FIRST) P.beta = 0;  // p = z: a fused direction update keeps it
         if (what == FIN_RHO) P.beta = o[0] / P.rho_prev;
         if (what == FIN_RESTART) {
             P.beta = o[0] / P.rho_prev;
@@ -444,14 +445,16 @@ static WgTables tables(const Problem &P) {
     return WgTables{P.d_wg_row0, P.d_wg_row1, P.d_wg_sub, P.d_sub_wg0, P.d_sub_nwg};
 }
 
+// zdir (matrix-free MODE_PQ only): the direction update p = z + beta x is fused into the product.
 template <typename T, int MODE>
-static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int sel) {
+static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int sel,
+                          const T *zdir = nullptr) {
     const dim3 g(P.n_wg), blk(256);
     const WgTables wt = tables(P);
     const int pr = sizeof(T) == 4 ? 1 : 0;
     // matrix-free products; residuals keep the assembled FP64 storage (full format)
     if (P.spmv.format[pr] == FMT_MF && MODE != MODE_RES) {
-        (void)mf_launch(P, x, y, MODE == MODE_PQ);
+        (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr);
         return;
     }
     if (P.spmv.format[pr] == FMT_SYM) {
@@ -672,11 +675,20 @@ static T *zbuf(Problem &P) {
 }
 
 // One PCG iteration on precision T (K, vectors and the Jacobi factors or IC(0) buffers of T).
+// Jacobi preconditioners with the matrix-free product: the direction update p = z + beta p that
+// ends an iteration is fused into the next iteration's product (beta = 0 after the first restart,
+// where p = z already; a refresh leaves its beta for the next product).
+template <typename T>
+static bool fused_direction(const Problem &P, int pre) {
+    return pre != PRE_IC0 && P.spmv.format[sizeof(T) == 4 ? 1 : 0] == FMT_MF;
+}
+
 template <int PRE, typename T>
 static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, const T *Minv, T *z) {
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
-    launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE);
+    const bool fused = fused_direction<T>(P, PRE);
+    launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr);
     finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
     hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, q, Minv, z, wt, P.d_ps, P.d_partials);
     if (PRE == PRE_IC0) {
@@ -687,7 +699,7 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     } else {
         finalize(P, FIN_UPD);
     }
-    hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_ACTIVE, 0);
+    if (!fused) hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_ACTIVE, 0);
 }
 
 template <int PRE>
@@ -717,7 +729,7 @@ static void launch_restart(Problem &P, const double *r64, T *r, T *p, const T *M
     } else {
         hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps, P.d_partials, first);
         finalize(P, first ? FIN_RESTART_FIRST : FIN_RESTART);
-        if (!first)
+        if (!first && !fused_direction<T>(P, PRE))
             hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_PAUSE_DONE, 0);
     }
 }

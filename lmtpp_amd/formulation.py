@@ -312,11 +312,12 @@ class Formulation:
 
     def matrix_free_info(self):
         """Layout of the matrix-free product (Hexa, one sub-problem): element blocks, sub-colours,
-        block-local node entries, shared nodes, their partial slots, nodes of no element."""
-        info = (C.c_uint64 * 6)()
+        block-local node entries, shared nodes, their partial slots, nodes of no element, elements
+        that are axis-aligned boxes."""
+        info = (C.c_uint64 * 7)()
         A.check(self._lib.femgpu_matrix_free_info(self._h, info), "matrix_free_info")
-        return dict(zip(("blocks", "colors", "local_entries", "shared_nodes", "partial_slots", "isolated_nodes"),
-                        (int(v) for v in info)))
+        return dict(zip(("blocks", "colors", "local_entries", "shared_nodes", "partial_slots", "isolated_nodes",
+                         "box_elements"), (int(v) for v in info)))
 
     def time_matrix_free(self, precision=A.FP64, reps=20):
         """(element-block kernel ms, whole product ms), averaged over reps launches."""

@@ -6,7 +6,8 @@ whose element kernels are pinned in test_oracle_elem.py):
 * FP64: |K_mf x - K x| <= 1e-12 (|K| |x|) per DOF; the FP32 product to 2e-5 of the same scale
   (the stored FP32 copy is held to 1e-6 beside it);
 * the penalty diagonal, a node of no element (its row is the penalty alone) and the internal node
-  order are covered;
+  order are covered, on general (jittered, rotated) elements, on axis-aligned boxes (the lattice
+  and a rectilinear grid: the box-only kernel) and on a mesh mixing both in every block;
 * with assume_constant_matrix the product keeps the material of the last matrix assembly;
 * solutions with the product in the loop match the oracle's solve to 1e-10 (FP64) and 1e-8
   (MIXED), with block Jacobi and block IC(0), and repeat bitwise;
@@ -32,9 +33,23 @@ def say(*a):
     sys.stdout.flush()
 
 
-def _mesh(div, seed, isolated=False, shuffle=False):
+def _mesh(div, seed, isolated=False, shuffle=False, geometry="jittered"):
+    """geometry: "jittered" (interior nodes moved: general elements), "lattice" (make_rect's
+    axis-aligned boxes), "rectilinear" (boxes of unequal sizes), "rotated" (the lattice turned
+    about the x axis: boxes that are not axis-aligned), "mixed" (nodes with x < 0.5
+    jittered: box and general elements side by side in every element block)."""
     c, e = L.make_rect(L.HEXA, div)
-    c = jitter_interior(c, 1.0 / max(div), 0.2, seed)
+    if geometry == "jittered":
+        c = jitter_interior(c, 1.0 / max(div), 0.2, seed)
+    elif geometry == "rectilinear":
+        c = c ** 1.3
+    elif geometry == "rotated":
+        a = 0.3  # about the x axis, so the clamped and loaded planes x = 0, 1 stay in place
+        R = np.array([[1.0, 0.0, 0.0], [0.0, np.cos(a), -np.sin(a)], [0.0, np.sin(a), np.cos(a)]])
+        c = c @ R.T
+    elif geometry == "mixed":
+        j = jitter_interior(c, 1.0 / max(div), 0.2, seed)
+        c = np.where(c[:, :1] < 0.5, j, c)
     if shuffle:
         perm = np.random.default_rng(seed).permutation(len(c))
         new_of = np.empty(len(c), np.int64)
@@ -47,6 +62,7 @@ def _mesh(div, seed, isolated=False, shuffle=False):
 
 def _system(oracle, c, e, E, isolated, order=A.NODE_ORDER_NATURAL, nu=0.3):
     faces, tr, dofs = clamp_and_traction(L.HEXA, c, e, (0.0, 0.4, -1.0))
+    assert len(dofs) > 0
     if isolated:
         dofs = np.concatenate([dofs, 3 * (len(c) - 1) + np.arange(3)]).astype(np.uint32)
     with A.node_order_ctx(order):
@@ -65,13 +81,22 @@ def _rel_err(y, ref, scale):
     return float(np.max(np.abs(y - ref) / scale))
 
 
-@pytest.mark.parametrize("case", ["jittered", "isolated_node", "node_order"])
+@pytest.mark.parametrize("case", ["jittered", "isolated_node", "node_order", "lattice", "rectilinear", "rotated",
+                                  "mixed", "lattice_isolated_node"])
 def test_product_matches_oracle_matrix(oracle, case):
     say("matrix-free product", case)
-    iso = case == "isolated_node"
-    c, e, E = _mesh((7, 6, 5), 11, isolated=iso, shuffle=case == "node_order")
+    iso = case.endswith("isolated_node")
+    geometry = case.replace("_isolated_node", "") if case in ("lattice", "rectilinear", "rotated", "mixed",
+                                                             "lattice_isolated_node") else "jittered"
+    c, e, E = _mesh((7, 6, 5), 11, isolated=iso, shuffle=case == "node_order", geometry=geometry)
     order = A.NODE_ORDER_RCM if case == "node_order" else A.NODE_ORDER_NATURAL
     f, (rp, col, vals, F) = _system(oracle, c, e, E, iso, order)
+    boxes = f.matrix_free_info()["box_elements"]
+    want = {"lattice": len(e), "rectilinear": len(e), "lattice_isolated_node": len(e)}.get(case, 0)
+    if case == "mixed":
+        assert 0 < boxes < len(e), boxes
+    else:
+        assert boxes == want, (boxes, want)
     rng = np.random.default_rng(5)
     x = rng.standard_normal(3 * len(c))
     ref = oracle.spmv(rp, col, vals, x)
@@ -110,10 +135,11 @@ def test_product_keeps_the_assembled_material(oracle):
     f.close()
 
 
-@pytest.mark.parametrize("precond", [L.BLOCK_JACOBI, L.BLOCK_IC0])
-def test_solutions_match_oracle(oracle, precond):
-    say("matrix-free solve", precond)
-    c, e, E = _mesh((8, 6, 6), 13)
+@pytest.mark.parametrize("precond,geometry", [(L.BLOCK_JACOBI, "jittered"), (L.BLOCK_IC0, "jittered"),
+                                              (L.BLOCK_JACOBI, "lattice"), (L.BLOCK_JACOBI, "mixed")])
+def test_solutions_match_oracle(oracle, precond, geometry):
+    say("matrix-free solve", precond, geometry)
+    c, e, E = _mesh((8, 6, 6), 13, geometry=geometry)
     f, (rp, col, vals, F) = _system(oracle, c, e, E, False)
     x, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-14, block=True)
     assert rc == 0

@@ -691,6 +691,30 @@ int mf_build(Problem &P) {
             selem[i] = e;
             n_colors = std::max(n_colors, c + 1);
         }
+        // slots by sub-colour (stable): a wave then holds 64 elements of one or two sub-colours
+        // instead of a (z, y, x) layer spanning four, so it takes part in fewer accumulation phases
+        // with more of its lanes; every entry still receives its adds in sub-colour order
+        {
+            const size_t i0 = (size_t)b * MF_BLOCK, nk = blk.size();
+            std::vector<uint32_t> ord(nk);
+            for (uint32_t k = 0; k < nk; ++k) ord[k] = k;
+            std::stable_sort(ord.begin(), ord.end(), [&](uint32_t p, uint32_t q) {
+                return (scol[i0 + p] & MF_COLOR_BITS) < (scol[i0 + q] & MF_COLOR_BITS);
+            });
+            const std::vector<uint32_t> c0(sconn.begin() + 8 * i0, sconn.begin() + 8 * (i0 + nk)),
+                e0(selem.begin() + i0, selem.begin() + i0 + nk);
+            const std::vector<uint16_t> l0(sloc.begin() + 8 * i0, sloc.begin() + 8 * (i0 + nk));
+            const std::vector<uint8_t> s0(scol.begin() + i0, scol.begin() + i0 + nk);
+            for (size_t k = 0; k < nk; ++k) {
+                const size_t src = ord[k];
+                for (int a = 0; a < 8; ++a) {
+                    sconn[8 * (i0 + k) + a] = c0[8 * src + a];
+                    sloc[8 * (i0 + k) + a] = l0[8 * src + a];
+                }
+                scol[i0 + k] = s0[src];
+                selem[i0 + k] = e0[src];
+            }
+        }
         max_local = std::max<uint32_t>(max_local, (uint32_t)lnode.size() - lptr[b]);
     }
     lptr[nb] = (uint32_t)lnode.size();

@@ -466,7 +466,7 @@ def main():
     # the matrix-free one) is HBM-bound; the matrix-free element-block kernel is bound by vector
     # arithmetic.
     from lmtpp_amd import _capi as A
-    from lmtpp_amd.traffic import MF_FLOPS_PER_ELEM, mf_block_bytes
+    from lmtpp_amd.traffic import mf_block_bytes, mf_flops
     reps = 50
     mf = A.SPMV_MATRIX_FREE
     stored = {k: ((A.SPMV_FULL if v[0] == mf else v[0]), v[1], v[2]) for k, v in spmv_cfg.items()}
@@ -492,20 +492,29 @@ def main():
                      "kernel": "k_spmv<double,0,%d,%d,%d>" % (stored["fp64"][1], sfmt64, stored["fp64"][2])}
 
     def mf_roof(fp32):
+        # The element-block kernel of the matrix-free product.  Its arithmetic intensity (about
+        # 10 FLOP/B in FP32 on box elements) sits at the ridge of the unpacked vector rate against
+        # HBM (78.6 TFLOP/s / 8 TB/s), so both roofs are given; "achieved" is the HBM one
+        # (algorithmic bytes / launch time), the vector one beside it.
         tb, tp = f.time_matrix_free(L.MIXED if fp32 else L.FP64, reps)
         info = f.matrix_free_info()
-        flops = MF_FLOPS_PER_ELEM * sz["n_elem"]
+        flops = mf_flops(sz["n_elem"], info)
         nbytes = mf_block_bytes(sz["n_elem"], sz["n_nodes"], info, fp32)
         peak = FP32_VECTOR_PEAK_TFLOPS if fp32 else FP64_VECTOR_PEAK_TFLOPS
         ach = flops / (tb * 1e-3) / 1e12
-        r = {"bound": "valu", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 4),
+        gbs = nbytes / (tb * 1e-3) / 1e9
+        r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(gbs / HBM_PEAK_GBS, 4),
              "traffic": load_traffic("mf_block_" + ("fp32" if fp32 else "fp64")),
-             "kernel": "k_mf_block<%s,false> (%s matrix-free Hexa product, element blocks)" % (
-                 "float" if fp32 else "double", "FP32" if fp32 else "FP64"),
-             "avg_launch_ms": round(tb, 4), "algorithmic_flops": flops, "algorithmic_bytes": nbytes,
-             "achieved_gbs": round(nbytes / (tb * 1e-3) / 1e9, 1), "product_ms": round(tp, 4), "layout": info}
+             "kernel": "k_mf_block<%s,false,false,%s> (%s matrix-free Hexa product, element blocks)" % (
+                 "float" if fp32 else "double", "true" if info["box_elements"] == sz["n_elem"] else "false",
+                 "FP32" if fp32 else "FP64"),
+             "avg_launch_ms": round(tb, 4), "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
+             "flop_per_byte": round(flops / nbytes, 2),
+             "valu": {"achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 4)},
+             "product_ms": round(tp, 4), "layout": info}
         if fp32:  # packed FP32 issue is disabled in this build: the reachable vector rate is the FP64 one
-            r["frac_of_unpacked_fp32"] = round(ach / FP64_VECTOR_PEAK_TFLOPS, 4)
+            r["valu"]["frac_of_unpacked_fp32"] = round(ach / FP64_VECTOR_PEAK_TFLOPS, 4)
         return r
 
     roof32 = mf_roof(True) if spmv_cfg["fp32"][0] == mf else roof_stored32

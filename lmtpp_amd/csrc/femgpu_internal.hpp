@@ -86,6 +86,8 @@ struct SymDev {
 // is the assembled K up to rounding even when the material changed later
 // (assume_constant_matrix).
 constexpr uint32_t MF_BLOCK = 256;
+constexpr uint32_t MF_NODE_REGS = 2;
+constexpr uint32_t MF_NO_NODE = 0xffffffffu;  // local nodes per lane whose node and slot k_mf_block keeps in registers
 constexpr uint32_t MF_EMPTY = 0xffu, MF_BOX = 0x40u, MF_COLOR_BITS = 0x3fu;
 struct MatFree {
     bool built = false, snap = false;
@@ -100,8 +102,10 @@ struct MatFree {
     uint8_t *d_color = nullptr;      // sub-colour per slot | MF_BOX for axis-aligned boxes (MF_EMPTY: empty slot)
     uint32_t *d_elem = nullptr;      // element per slot (0 for empty slots)
     uint32_t *d_lptr = nullptr;      // block b: local nodes [lptr[b], lptr[b + 1])
-    uint32_t *d_lnode = nullptr;     // global node of each local node
-    uint32_t *d_lslot = nullptr;     // its partial slot when shared with another block, else ~0
+    uint32_t ls = 0;                 // stride of the per-block node lists (max_local rounded up to 64)
+    // block b, local node l < its count (lptr[b + 1] - lptr[b]): entry b * ls + l holds the global
+    // node (MF_NO_NODE past the count) and its partial slot when shared with another block (~0 if not)
+    uint32_t *d_lnode_pad = nullptr, *d_lslot_pad = nullptr;
     uint32_t *d_bnode = nullptr, *d_bptr = nullptr;  // shared node k: partials [bptr[k], bptr[k + 1])
     double *d_halo = nullptr;        // 3 values per partial slot (FP64, or FP32 in its first half)
     std::vector<uint32_t> node_blk, node_loc;  // first block holding each node, its local index (~0: none)

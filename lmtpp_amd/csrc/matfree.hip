@@ -13,7 +13,8 @@
 // factorisation), the transposed projection the same way.  With adj = detJ J^-1, the Gauss point
 // term is (w_q / detJ) sigma(Gr adj^T) adj: one division per point.  Elements that are
 // axis-aligned boxes (every lattice mesh) have a constant diagonal J: adj, w / detJ and the
-// Hooke coefficients come from a per-element record and each point needs a third of the work.
+// Hooke coefficients come from a per-element record, and the sums over the Gauss points take a
+// closed form (about a third of the general element's arithmetic).
 //
 // Scatter without atomics: elements sorted along a Morton curve of their centres form blocks of
 // MF_BLOCK, one workgroup each (compact bricks on lattices).  The workgroup stages the input at its
@@ -256,12 +257,49 @@ __device__ inline void hexa_apply(const double (&X)[8][3], const T (&u)[8][3], T
     project_out(Pd, w1, f);
 }
 
+// One axis of the box product: Pd[D][s][t][.] = b sum over the two Gauss points along xi_D of
+// sigma(G)[.][D], G[c][m] = Ud[m][.][c] ad_m, in closed form.  Column m of G does not vary along
+// xi_m (the reference derivative along xi_m of a Q1 field is constant along xi_m), so the sum
+// is twice the terms of column D plus the sums along xi_D of columns E and F (the other axes;
+// (s, t) are the Gauss point indices along E < F).
+template <int D, typename T>
+__device__ inline void box_axis(const T (&Ud)[3][2][2][3], const T (&ad)[3], T b, T cs, T c1, T c2,
+                                T (&Pd)[3][2][2][3]) {
+    constexpr int E = D == 0 ? 1 : 0, F = D == 2 ? 1 : 2;
+    // Ud[E] at (n_D, n_F) and Ud[F] at (n_D, n_E), each indexed by its own two axes in order
+    auto udE = [&](int nD, int nF, int c) -> T { return D < F ? Ud[E][nD][nF][c] : Ud[E][nF][nD][c]; };
+    auto udF = [&](int nD, int nE, int c) -> T { return D < E ? Ud[F][nD][nE][c] : Ud[F][nE][nD][c]; };
+    const T bc1 = b * c1, bc2 = b * c2;
+    const T A = b * cs * ad[D], BE = bc1 * ad[E], BF = bc1 * ad[F];
+    const T C = bc2 * ((T)2 * ad[D]), DE = bc2 * ad[E], DF = bc2 * ad[F];
+    T seE[2], seD[2], sfF[2], sfD[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        seE[t] = BE * (udE(0, t, E) + udE(1, t, E));
+        seD[t] = DE * (udE(0, t, D) + udE(1, t, D));
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        sfF[q] = BF * (udF(0, q, F) + udF(1, q, F));
+        sfD[q] = DF * (udF(0, q, D) + udF(1, q, D));
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            Pd[D][q][t][D] = A * Ud[D][q][t][D] + (seE[t] + sfF[q]);
+            Pd[D][q][t][E] = C * Ud[D][q][t][E] + seD[t];
+            Pd[D][q][t][F] = C * Ud[D][q][t][F] + sfD[q];
+        }
+}
+
 // The same for an axis-aligned box (every edge along xi_d is h_d e_d): J = diag(h) at every
-// Gauss point, so adj = diag(h1 h2, h0 h2, h0 h1) and w / detJ are constant, and the point term
-// (w / detJ) sigma(Gr adj^T) adj needs the diagonal only: 30 instead of about 90 operations per
-// point, and no geometry (the "Cartesian" cell of matrix-free FE codes).
+// Gauss point, so adj = diag(h1 h2, h0 h2, h0 h1) and w / detJ are constant, and the point terms
+// (w / detJ) sigma(Gr adj^T) adj need the diagonal only; their sums over the points along each
+// axis take the closed form of box_axis (no geometry: the "Cartesian" cell of matrix-free FE
+// codes).  c0, c1, c2: H11 - H12, H12, H44 / 2.
 template <typename T>
-__device__ inline void hexa_apply_box(const T (&h)[3], const T (&u)[8][3], T h01, T h1, T h2, T (&f)[8][3]) {
+__device__ inline void hexa_apply_box(const T (&h)[3], const T (&u)[8][3], T c0, T c1, T c2, T (&f)[8][3]) {
     double g0d, g1d, wq;
     hexa_gauss(g0d, g1d, wq);
     const T g[2] = {(T)g0d, (T)g1d};
@@ -270,43 +308,11 @@ __device__ inline void hexa_apply_box(const T (&h)[3], const T (&u)[8][3], T h01
     ref_gradients(u, g, Ud);
     const T ad[3] = {h[1] * h[2], h[0] * h[2], h[0] * h[1]};
     const T kq = wq_over(h[0] * ad[0], (T)wq);
-    const T bd[3] = {kq * ad[0], kq * ad[1], kq * ad[2]};
+    const T cs = (T)2 * (c0 + c1);
     T Pd[3][2][2][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) Pd[d][s][t][c] = 0;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                T G[3][3];  // detJ * d u_c / d x_m
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    G[c][0] = Ud[0][j][k][c] * ad[0];
-                    G[c][1] = Ud[1][i][k][c] * ad[1];
-                    G[c][2] = Ud[2][i][j][c] * ad[2];
-                }
-                const T htr = h1 * (G[0][0] + G[1][1] + G[2][2]);
-                T sg[3][3];  // sigma(G) / (w / detJ)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) sg[c][c] = h01 * G[c][c] + htr;
-                sg[0][1] = sg[1][0] = h2 * (G[0][1] + G[1][0]);
-                sg[0][2] = sg[2][0] = h2 * (G[0][2] + G[2][0]);
-                sg[1][2] = sg[2][1] = h2 * (G[1][2] + G[2][1]);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    Pd[0][j][k][c] += sg[c][0] * bd[0];
-                    Pd[1][i][k][c] += sg[c][1] * bd[1];
-                    Pd[2][i][j][c] += sg[c][2] * bd[2];
-                }
-            }
+    box_axis<0>(Ud, ad, kq * ad[0], cs, c1, c2, Pd);
+    box_axis<1>(Ud, ad, kq * ad[1], cs, c1, c2, Pd);
+    box_axis<2>(Ud, ad, kq * ad[2], cs, c1, c2, Pd);
     project_out(Pd, w1, f);
 }
 
@@ -368,8 +374,8 @@ inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
 template <typename T, bool PQ, bool DIR, bool BOXES>
 __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
-                                                  uint32_t n_colors, const uint32_t *__restrict__ lptr,
-                                                  const uint32_t *__restrict__ lnode, const uint32_t *__restrict__ lslot,
+                                                  uint32_t n_colors, uint32_t ls, const uint32_t *__restrict__ lnode,
+                                                  const uint32_t *__restrict__ lslot,
                                                   const uint32_t *__restrict__ pptr, const uint32_t *__restrict__ pidx,
                                                   const uint32_t *__restrict__ psrc, const double *__restrict__ fpen,
                                                   const double *__restrict__ coords, const double *__restrict__ E,
@@ -380,18 +386,37 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
     extern __shared__ double mf_lds[];
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
     const unsigned b = xcd_slab();
-    const uint32_t l0 = lptr[b], nl = lptr[b + 1] - l0;
-    T *Ps = reinterpret_cast<T *>(mf_lds);  // [nl][3] the product's input
-    T *acc = Ps + 3 * nl;                   // [nl][3] summed nodal forces
+    // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
+    // loads need no other load first
+    const size_t l0 = (size_t)b * ls;
+    T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][3] the product's input
+    T *acc = Ps + 3 * (size_t)ls;           // [ls][3] summed nodal forces
     const T beta = DIR ? (T)S[0].beta : (T)0;
-    for (uint32_t i = threadIdx.x; i < nl; i += MF_BLOCK) {
-        const size_t g = 3 * (size_t)lnode[l0 + i];
+    // everything the later phases read from memory is loaded here, ahead of the barriers (a load
+    // after a barrier cannot be hoisted above it): the penalty range, and the global node and
+    // partial slot of the lane's first MF_NODE_REGS local nodes (a brick's ~400; larger blocks
+    // reload the rest)
+    const uint32_t pk0 = pptr[b], pk1 = pptr[b + 1];
+    uint32_t gnode[MF_NODE_REGS], gslot[MF_NODE_REGS];
+#pragma unroll
+    for (int r = 0; r < (int)MF_NODE_REGS; ++r) {
+        const uint32_t i = threadIdx.x + r * MF_BLOCK;
+        gnode[r] = i < ls ? lnode[l0 + i] : MF_NO_NODE;
+        gslot[r] = i < ls ? lslot[l0 + i] : 0u;
+    }
+    auto stage = [&](uint32_t i, uint32_t gn) {
+        const size_t g = 3 * (size_t)gn;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             Ps[3 * i + d] = DIR ? dir_value(z[g + d], beta, x[g + d]) : x[g + d];
             acc[3 * i + d] = 0;
         }
-    }
+    };
+#pragma unroll
+    for (int r = 0; r < (int)MF_NODE_REGS; ++r)
+        if (gnode[r] != MF_NO_NODE) stage(threadIdx.x + r * MF_BLOCK, gnode[r]);
+    for (uint32_t i = threadIdx.x + MF_NODE_REGS * MF_BLOCK; i < ls; i += MF_BLOCK)
+        if (lnode[l0 + i] != MF_NO_NODE) stage(i, lnode[l0 + i]);
     const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
     const uint32_t cbits = color[slot];
     const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
@@ -449,11 +474,9 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
         __syncthreads();
     }
     // penalty terms of the constrained DOFs assigned to this block (distinct local entries)
-    const uint32_t p1 = pptr[b + 1];
-    for (uint32_t k = pptr[b] + threadIdx.x; k < p1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * Ps[pidx[k]];
+    for (uint32_t k = pk0 + threadIdx.x; k < pk1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * Ps[pidx[k]];
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nl; i += MF_BLOCK) {
-        const uint32_t s = lslot[l0 + i], g = lnode[l0 + i];
+    auto write_out = [&](uint32_t i, uint32_t s, uint32_t g) {
         const bool own = s == 0xffffffffu;
         T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)s;
         const T a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
@@ -469,7 +492,12 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
                 x[3 * (size_t)g + 2] = x2;
             }
         }
-    }
+    };
+#pragma unroll
+    for (int r = 0; r < (int)MF_NODE_REGS; ++r)
+        if (gnode[r] != MF_NO_NODE) write_out(threadIdx.x + r * MF_BLOCK, gslot[r], gnode[r]);
+    for (uint32_t i = threadIdx.x + MF_NODE_REGS * MF_BLOCK; i < ls; i += MF_BLOCK)
+        if (lnode[l0 + i] != MF_NO_NODE) write_out(i, lslot[l0 + i], lnode[l0 + i]);
     if (PQ) wg_partial<1, 0u>(pq, partials, 0u, b);
 }
 
@@ -551,8 +579,9 @@ bool mf_eligible(const Problem &P) {
 }
 
 void mf_free(MatFree &M) {
-    dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr); dfree(M.d_lnode);
-    dfree(M.d_lslot); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof); dfree(M.d_isrc);
+    dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr);
+    dfree(M.d_lnode_pad); dfree(M.d_lslot_pad); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof);
+    dfree(M.d_isrc);
     dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
     dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_wg0); dfree(M.d_nwg);
     M = MatFree();
@@ -718,7 +747,8 @@ int mf_build(Problem &P) {
         max_local = std::max<uint32_t>(max_local, (uint32_t)lnode.size() - lptr[b]);
     }
     lptr[nb] = (uint32_t)lnode.size();
-    if (mf_lds_bytes(max_local, 8) > 64 * 1024) return 1;  // LDS of the FP64 product's blocks
+    const uint32_t ls = (max_local + 63) / 64 * 64;  // fixed stride of the blocks' node lists
+    if (mf_lds_bytes(ls, 8) > 64 * 1024) return 1;  // LDS of the FP64 product's blocks
     // shared nodes: a partial slot per (node, block), grouped by node in block order
     std::vector<uint32_t> nshared(nn, 0);
     for (size_t l = 0; l < lnode.size(); ++l)
@@ -735,10 +765,16 @@ int mf_build(Problem &P) {
     std::vector<uint32_t> lslot(lnode.size(), 0xffffffffu);
     for (size_t l = 0; l < lnode.size(); ++l)  // blocks in order, so slots follow block order
         if (in_blk[l] != deg[lnode[l]]) lslot[l] = first[lnode[l]]++;
+    std::vector<uint32_t> lnode_pad((size_t)nb * ls, MF_NO_NODE), lslot_pad((size_t)nb * ls, 0xffffffffu);
+    for (uint32_t b = 0; b < nb; ++b)
+        for (uint32_t l = lptr[b]; l < lptr[b + 1]; ++l) {
+            lnode_pad[(size_t)b * ls + (l - lptr[b])] = lnode[l];
+            lslot_pad[(size_t)b * ls + (l - lptr[b])] = lslot[l];
+        }
     const std::vector<uint32_t> zero(1, 0);
     if (!upload_vec(&M.d_conn, sconn) || !upload_vec(&M.d_loc, sloc) || !upload_vec(&M.d_color, scol) ||
-        !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode, lnode) ||
-        !upload_vec(&M.d_lslot, lslot) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
+        !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode_pad, lnode_pad) ||
+        !upload_vec(&M.d_lslot_pad, lslot_pad) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
         !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
         hipMalloc(&M.d_rec64, sizeof(BoxRec<double>) * ns) != hipSuccess ||
@@ -752,6 +788,7 @@ int mf_build(Problem &P) {
     M.n_boxes = (uint32_t)std::count_if(scol.begin(), scol.end(), [](uint8_t c) { return c != MF_EMPTY && (c & MF_BOX); });
     M.all_boxes = M.n_boxes == ne;
     M.max_local = max_local;
+    M.ls = ls;
     M.n_bnodes = (uint32_t)bnode.size();
     M.n_halo = bptr.back();
     M.n_iso = (uint32_t)idof.size() / 3;
@@ -862,8 +899,8 @@ int mf_prepare(Problem &P) {
 template <typename T, bool PQ, bool DIR, bool BOXES>
 static void mf_block_launch(Problem &P, T *x, const T *z, T *y) {
     const MatFree &M = P.mf;
-    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES>), dim3(M.n_blocks), dim3(MF_BLOCK), mf_lds_bytes(M.max_local, sizeof(T)),
-                       P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.n_colors, M.d_lptr, M.d_lnode, M.d_lslot,
+    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES>), dim3(M.n_blocks), dim3(MF_BLOCK), mf_lds_bytes(M.ls, sizeof(T)),
+                       P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad,
                        M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat,
                        reinterpret_cast<const BoxRec<T> *>(sizeof(T) == 4 ? (void *)M.d_rec32 : (void *)M.d_rec64), x, z, y,
                        reinterpret_cast<T *>(M.d_halo), P.d_ps, M.d_partials);

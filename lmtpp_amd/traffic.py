@@ -85,23 +85,39 @@ def assembly(n):
     f.close()
 
 
-# Arithmetic of one element in the matrix-free product as written (k_mf_block, hexa_apply; an FMA
-# counts 2): lattice differences 72, 18 bilinear interpolations x 20 = 360, per Gauss point
-# adjugate and det 32 + w/det 1 + displacement gradient 45 + trace 2 + stress 19 + projection
-# onto the adjugate 54 = 153 (x 8 = 1224), 9 transposed projections x 24 = 216, nodal forces 48,
-# Hooke law 8.
+# Arithmetic of one element in the matrix-free product as written (k_mf_block; an FMA counts 2).
+# General element (hexa_apply): lattice differences 72, 18 bilinear interpolations x 20 = 360, per
+# Gauss point adjugate and det 32 + w/det 1 + displacement gradient 45 + trace 2 + stress 19 +
+# projection onto the adjugate 54 = 153 (x 8 = 1224), 9 transposed projections x 24 = 216, nodal
+# forces 48, Hooke law 8.
 MF_FLOPS_PER_ELEM = 72 + 360 + 1224 + 216 + 48 + 8
+# Axis-aligned box (hexa_apply_box, Hooke coefficients from the element's record): displacement
+# differences 36, 9 bilinear interpolations x 20 = 180, diagonal adjugate, w/det and the axis
+# scales 8, per axis (box_axis) coefficients 9 + four marginal sums 12 + 12 values 20 = 41
+# (x 3 = 123), 9 transposed projections x 24 = 216, nodal forces 48.
+MF_BOX_FLOPS_PER_ELEM = 36 + 180 + 8 + 123 + 216 + 48
+
+
+def mf_flops(n_elem, info):
+    """Arithmetic of one element-block launch: box elements on the reduced path, the rest on the
+    general one."""
+    nb = info.get("box_elements", 0)
+    return nb * MF_BOX_FLOPS_PER_ELEM + (n_elem - nb) * MF_FLOPS_PER_ELEM
 
 
 def mf_block_bytes(n_elem, n_nodes, info, fp32):
-    """Algorithmic bytes of one element-block launch: per element its 8 nodes, 8 block-local
-    indices, sub-colour, element id and modulus (61 B); per block its local-node range; per node its
-    coordinates and x once; per block-local node entry its global node and slot (8 B); the stores
-    of the nodes a block owns and of the shared nodes' partials."""
+    """Algorithmic bytes of one element-block launch: per element its 8 block-local indices and
+    sub-colour (17 B) and either its record (a box: edge lengths and Hooke coefficients, 24 B in
+    FP32, 48 B in FP64) or its 8 nodes, element id and modulus (44 B) with the coordinates of the
+    general elements' nodes once (24 B per node); per block its local-node range; per node x once;
+    per block-local node entry its global node and slot (8 B); the stores of the nodes a block owns
+    and of the shared nodes' partials."""
     s = 12 if fp32 else 24
+    nb = info.get("box_elements", 0)
     owned = n_nodes - info["shared_nodes"] - info["isolated_nodes"]
-    return (n_elem * 61 + info["blocks"] * 4 + n_nodes * (24 + s) + info["local_entries"] * 8 +
-            (owned + info["partial_slots"]) * s)
+    coords = n_nodes * 24 if nb < n_elem else 0
+    return (nb * (17 + 2 * s) + (n_elem - nb) * 61 + coords + info["blocks"] * 4 + n_nodes * s +
+            info["local_entries"] * 8 + (owned + info["partial_slots"]) * s)
 
 
 def mf_halo_bytes(info, fp32):
@@ -156,7 +172,7 @@ def _counters(d):
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
-        mf = re.search(r"k_mf_(block|halo)<(double|float), false>", name)
+        mf = re.search(r"k_mf_(block|halo)<(double|float), false, false(?:, (?:true|false))?>", name)
         if "k_copy" in name:
             key = "copy"
         elif mf:

@@ -110,8 +110,9 @@ def mf_block_bytes(n_elem, n_nodes, info, fp32):
     sub-colour (17 B) and either its record (a box: edge lengths and Hooke coefficients, 24 B in
     FP32, 48 B in FP64) or its 8 nodes, element id and modulus (44 B) with the coordinates of the
     general elements' nodes once (24 B per node); per block its local-node range; per node x once;
-    per block-local node entry its global node and slot (8 B); the stores of the nodes a block owns
-    and of the shared nodes' partials."""
+    per block-local node entry its global node and slot (8 B; the lists are stored at a fixed
+    stride per block, and the padding is not counted); the stores of the nodes a block owns and of
+    the shared nodes' partials."""
     s = 12 if fp32 else 24
     nb = info.get("box_elements", 0)
     owned = n_nodes - info["shared_nodes"] - info["isolated_nodes"]

@@ -159,6 +159,28 @@ __global__ __launch_bounds__(256) void k_spmv(SpmvArgs A, const T *__restrict__ 
     }
 }
 
+// y = b - y for y holding K x (the residual of MODE_RES after a matrix-free product), with the
+// partials of ||y||^2 and max |y| that k_spmv's MODE_RES writes
+__global__ __launch_bounds__(256) void k_residual(const double *__restrict__ b, double *__restrict__ y, WgTables wt,
+                                                  const ProbScalars *S, double *partials, int sel) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], sel)) return;
+    double acc[2] = {0.0, 0.0};
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = 3 * (size_t)n + d;
+            const double v = b[i] - y[i];
+            y[i] = v;
+            acc[0] += v * v;
+            acc[1] = fmax(acc[1], fabs(v));
+        }
+    }
+    wg_partial<2, 2u>(acc, partials, gridDim.x, L);
+}
+
 // Values of the symmetric storage from the full K: plane q of upper position p is K[q][src[p]]
 // (zero at padding positions).
 template <typename T>
@@ -452,9 +474,13 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     const dim3 g(P.n_wg), blk(256);
     const WgTables wt = tables(P);
     const int pr = sizeof(T) == 4 ? 1 : 0;
-    // matrix-free products; residuals keep the assembled FP64 storage (full format)
-    if (P.spmv.format[pr] == FMT_MF && MODE != MODE_RES) {
+    // matrix-free products; an FP64 residual is the product into y, then y = b - y with the
+    // residual's partials
+    if (P.spmv.format[pr] == FMT_MF) {
         (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr);
+        if (MODE == MODE_RES)
+            hipLaunchKernelGGL(k_residual, g, blk, 0, P.stream, b, reinterpret_cast<double *>(y), wt, P.d_ps, P.d_partials,
+                               sel);
         return;
     }
     if (P.spmv.format[pr] == FMT_SYM) {

@@ -386,11 +386,17 @@ FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2]
 /* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
  * into K (read-modify-write).  rows = FEMGPU_ASSEMBLY_STAGED: the same passes into a
  * block-contiguous copy of K (each block's 9 values adjacent; an extra K-sized buffer), then one
- * streaming pass into the SpMV's layout.  rows = R (a power of two, at most *max_rows): each
- * workgroup owns R consecutive block rows, accumulates their blocks from the incident elements
- * colour by colour in LDS and writes the rows once.  Every variant sums each entry in the same
- * (colour) order.  The default is FEMGPU_ASSEMBLY_STAGED; femgpu_autotune times them all. */
+ * streaming pass into the SpMV's layout.  rows = FEMGPU_ASSEMBLY_GATHER: one pass stores every
+ * element's blocks and loads (plain writes), a second builds every stored position of K and every
+ * node's load from its list of contributions, written once (buffers of the element blocks and the
+ * lists, about 2.6 GB and 0.4 GB for a million Hexa8; element blocks are indexed in 32 bits).
+ * rows = R (a power of two, at most *max_rows): each workgroup owns R consecutive block rows,
+ * accumulates their blocks from the incident elements colour by colour in LDS and writes the rows
+ * once.  Every variant sums each entry in the same (colour) order: the colour passes, the staged
+ * passes and the gather give bitwise the same K and F.  The default is FEMGPU_ASSEMBLY_STAGED;
+ * femgpu_autotune times them all. */
 #define FEMGPU_ASSEMBLY_STAGED 0xffffffffu
+#define FEMGPU_ASSEMBLY_GATHER 0xfffffffeu
 FEMGPU_API int femgpu_set_assembly_rows(femgpu_problem *p, uint32_t rows);
 FEMGPU_API int femgpu_get_assembly_rows(femgpu_problem *p, uint32_t *rows, uint32_t *max_rows);
 /* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */

@@ -21,6 +21,7 @@ LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
 ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
 NODE_ORDER_NATURAL, NODE_ORDER_RCM = 0, 1
 ASSEMBLY_STAGED = 0xFFFFFFFF
+ASSEMBLY_GATHER = 0xFFFFFFFE
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
 NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}

@@ -138,6 +138,90 @@ __global__ __launch_bounds__(256) void k_assemble(const uint32_t *__restrict__ o
 }
 
 // ---------------------------------------------------------------------------------------------
+// Gather element loop, pass 1: every element's upper 3x3 blocks (summed over its Gauss points,
+// exactly as k_assemble forms them) and nodal loads, stored with plain writes: block (a, b),
+// a <= b, at EM[(e NUP + upper_index(a, b)) 9 ..], load of node a at EF[(e NPE + a) 3 ..].
+// ---------------------------------------------------------------------------------------------
+template <int K>
+__device__ constexpr int upper_index(int a, int b) { return a * ET<K>::NPE - a * (a - 1) / 2 + (b - a); }
+
+template <int K>
+__global__ __launch_bounds__(256) void k_elem_mats(uint32_t n_elem, const uint32_t *__restrict__ conn,
+                                                   const double *__restrict__ coords, const double *__restrict__ E,
+                                                   const uint32_t *__restrict__ elem_sub,
+                                                   const SubMaterial *__restrict__ mat, double *__restrict__ EM,
+                                                   double *__restrict__ EF, int do_mat, int do_vec, int *flags) {
+    constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP, TEAM = ET<K>::TEAM, TPW = 256 / TEAM;
+    constexpr int NUP = NPE * (NPE + 1) / 2;
+    __shared__ ElemLds<K> lds[TPW];
+    const int team = threadIdx.x / TEAM, t = threadIdx.x % TEAM;
+    const uint32_t e = blockIdx.x * TPW + team;
+    const bool valid = e < n_elem;
+    ElemLds<K> &L = lds[team];
+    elem_geometry<K>(L, t, valid, valid ? e : 0, conn, coords, flags);
+    if (!valid) return;
+    const SubMaterial m = mat[elem_sub[e]];
+    if (do_mat && t < NPE * NPE && t / NPE <= t % NPE) {
+        const int a = t / NPE, b = t % NPE;
+        double H[3];
+        gen::hooke(E[e], m.nu, H);
+        double k[9];
+        elem_block<K>(L, a, b, H, k);
+        if (a == b) { k[3] = k[1]; k[6] = k[2]; k[7] = k[5]; }
+        double *dst = EM + ((size_t)e * NUP + upper_index<K>(a, b)) * 9;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) dst[c] = k[c];
+    }
+    if (do_vec && t < NPE * 3) {
+        const int a = t / 3, d = t % 3;
+        double f3[3] = {0, 0, 0};
+        for (int q = 0; q < NGP; ++q) gen::load(L.N[q][a], m.bf0, m.bf1, m.bf2, L.wd[q], f3);
+        EF[((size_t)e * NPE + a) * 3 + d] = f3[d];
+    }
+}
+
+// Pass 2: every stored position p of K (a lane each, the planes written once and coalesced) sums
+// its contributions ent[ptr[p] .. ptr[p + 1]) in the colour order of the colour passes; an entry is
+// (element block index << 1) | transposed.  Same adds in the same order from zero as the colour
+// passes, so K is bitwise theirs.
+__global__ __launch_bounds__(256) void k_gather_blocks(uint64_t ps, const uint32_t *__restrict__ ptr,
+                                                       const uint32_t *__restrict__ ent, const double *__restrict__ EM,
+                                                       double *__restrict__ Kv) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < ps; p += (uint64_t)gridDim.x * blockDim.x) {
+        double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t j = ptr[p]; j < ptr[p + 1]; ++j) {
+            const uint32_t c = ent[j];
+            const double *blk = EM + (size_t)(c >> 1) * 9;
+            if (c & 1u) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] += blk[3 * (q % 3) + q / 3];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] += blk[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Kv[q * ps + p] = v[q];
+    }
+}
+
+// The loads: node n sums the element loads ent[ptr[n] .. ptr[n + 1]) (entry e NPE + a), in colour order.
+__global__ __launch_bounds__(256) void k_gather_loads(uint32_t n_nodes, const uint32_t *__restrict__ ptr,
+                                                      const uint32_t *__restrict__ ent, const double *__restrict__ EF,
+                                                      double *__restrict__ F) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= n_nodes) return;
+    double v[3] = {0, 0, 0};
+    for (uint32_t j = ptr[n]; j < ptr[n + 1]; ++j) {
+        const double *f = EF + (size_t)ent[j] * 3;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] += f[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[3 * (size_t)n + d] = v[d];
+}
+
+// ---------------------------------------------------------------------------------------------
 // Row-group element loop: workgroup G owns storage rows [G R, G R + R) (R divides 64, so the group
 // lies in one SELL slice; storage row t holds block row perm[t], perm null for the identity).  It walks the elements incident to its rows colour by colour, the same
 // order as the colour-pass kernel above, computes each element's geometry in a team of lanes and
@@ -376,6 +460,20 @@ static hipError_t assemble_kind(Problem &P, int do_mat, int do_vec) {
 }
 
 template <int K>
+static hipError_t assemble_gather_kind(Problem &P, int do_mat, int do_vec) {
+    constexpr int TPW = 256 / ET<K>::TEAM;
+    hipLaunchKernelGGL(k_elem_mats<K>, dim3((P.n_elem + TPW - 1) / TPW), dim3(256), 0, P.stream, P.n_elem, P.d_conn,
+                       P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.ga.d_em, P.ga.d_ef, do_mat, do_vec, P.d_flags);
+    if (do_mat)
+        hipLaunchKernelGGL(k_gather_blocks, dim3(8192), dim3(256), 0, P.stream, P.ps, P.ga.d_kptr, P.ga.d_kent, P.ga.d_em,
+                           P.d_K);
+    if (do_vec)
+        hipLaunchKernelGGL(k_gather_loads, dim3((P.n_nodes + 255) / 256), dim3(256), 0, P.stream, P.n_nodes, P.ga.d_fptr,
+                           P.ga.d_fent, P.ga.d_ef, P.d_F);
+    return hipGetLastError();
+}
+
+template <int K>
 static void assemble_rows_kind(Problem &P, int do_mat, int do_vec) {
     hipLaunchKernelGGL(k_assemble_rows<K>, dim3(P.rg.n_groups), dim3(ASM_THREADS), 0, P.stream, P.rg.R, P.n_nodes,
                        P.rg.n_colors, P.d_gptr, P.d_gel, P.d_conn, P.d_coords, P.d_E, P.d_elem_sub, P.d_mat, P.d_slot,
@@ -412,6 +510,10 @@ hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
     if (!do_mat && !body_force) {
         // right-hand side only, no body force: every element contribution is exactly zero
         if (do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
+    } else if (P.asm_rows == ASM_GATHER && P.ga.d_em) {
+        if (P.kind == TETRA) e = assemble_gather_kind<TETRA>(P, do_mat, do_vec);
+        else if (P.kind == TETRA10) e = assemble_gather_kind<TETRA10>(P, do_mat, do_vec);
+        else e = assemble_gather_kind<HEXA>(P, do_mat, do_vec);
     } else if (P.asm_rows && P.rg.R == P.asm_rows) {
         if (P.kind == TETRA) assemble_rows_kind<TETRA>(P, do_mat, do_vec);
         else if (P.kind == TETRA10) assemble_rows_kind<TETRA10>(P, do_mat, do_vec);

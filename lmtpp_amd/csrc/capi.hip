@@ -81,6 +81,10 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+void free_gather(GatherAsm &G) {
+    dfree(G.d_em); dfree(G.d_ef); dfree(G.d_kptr); dfree(G.d_kent); dfree(G.d_fptr); dfree(G.d_fent);
+}
+
 void free_problem(Problem &P) {
     destroy_graph(P);
     ic0_free(P.ic0);
@@ -96,6 +100,7 @@ void free_problem(Problem &P) {
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
     dfree(P.d_gptr); dfree(P.d_gel); dfree(P.d_perm); dfree(P.d_iperm); dfree(P.d_nperm); dfree(P.d_ninv); dfree(P.d_Kb);
+    free_gather(P.ga);
     if (P.h_ps) (void)hipHostFree(P.h_ps);
     if (P.h_flags) (void)hipHostFree(P.h_flags);
     P.h_ps = nullptr;
@@ -125,9 +130,57 @@ void build_wg_tables(Problem &P, std::vector<uint32_t> &row0, std::vector<uint32
     P.n_wg = (uint32_t)row0.size();
 }
 
-// Element-loop variant: 0 = colour passes, R = row groups of R rows (a power of two <= 64 whose
-// LDS accumulator fits, see max_assembly_rows).
+// The gather lists of ASM_GATHER (host, once per problem): every (element, a, b) block goes to
+// the list of its position, every (element, a) load to the list of its node, walking the elements
+// in colour order, so each list is in the order the colour passes add.
+static int build_gather(Problem &P) {
+    if (P.ga.d_em) return FEMGPU_OK;
+    const uint32_t npe = (uint32_t)P.npe, nup = npe * (npe + 1) / 2;
+    if ((uint64_t)P.n_elem * nup * 2 >= (1ull << 32) || (uint64_t)P.n_elem * npe * npe >= (1ull << 32) ||
+        P.ps >= (1ull << 32) || P.ecol.order.size() != P.n_elem)
+        return fail(FEMGPU_BAD_INPUT, "assembly: the gather variant indexes at most 2^32 element blocks");
+    const uint32_t *conn = P.conn_h.data();
+    const std::vector<uint32_t> slot = build_slot_map(P.pat, P.n_elem, (int)npe, conn);
+    std::vector<uint32_t> kptr(P.ps + 1, 0), fptr(P.n_nodes + 1, 0);
+    for (size_t i = 0; i < slot.size(); ++i) ++kptr[(size_t)slot[i] + 1];
+    for (size_t i = 0; i < (size_t)P.n_elem * npe; ++i) ++fptr[(size_t)conn[i] + 1];
+    for (size_t p = 0; p < P.ps; ++p) kptr[p + 1] += kptr[p];
+    for (uint32_t n = 0; n < P.n_nodes; ++n) fptr[n + 1] += fptr[n];
+    std::vector<uint32_t> kent(kptr.back()), fent(fptr.back());
+    std::vector<uint32_t> kf(kptr.begin(), kptr.end() - 1), ff(fptr.begin(), fptr.end() - 1);
+    for (const uint32_t e : P.ecol.order) {
+        for (uint32_t a = 0; a < npe; ++a) {
+            for (uint32_t b = 0; b < npe; ++b) {
+                const uint32_t lo = std::min(a, b), hi = std::max(a, b);
+                const uint32_t k = lo * npe - lo * (lo - 1) / 2 + (hi - lo);  // lo (lo - 1) is even
+                kent[kf[slot[((size_t)e * npe + a) * npe + b]]++] = ((e * nup + k) << 1) | (a > b ? 1u : 0u);
+            }
+            fent[ff[conn[(size_t)e * npe + a]]++] = e * npe + a;
+        }
+    }
+    GatherAsm G;
+    if (dalloc(&G.d_em, (size_t)P.n_elem * nup * 9) != hipSuccess || dalloc(&G.d_ef, (size_t)P.n_elem * npe * 3) != hipSuccess ||
+        upload(&G.d_kptr, kptr.data(), kptr.size()) != hipSuccess || upload(&G.d_kent, kent.data(), kent.size()) != hipSuccess ||
+        upload(&G.d_fptr, fptr.data(), fptr.size()) != hipSuccess || upload(&G.d_fent, fent.data(), fent.size()) != hipSuccess) {
+        free_gather(G);
+        return fail(FEMGPU_HIP_ERROR, "assembly: gather buffers");
+    }
+    P.ga = G;
+    return FEMGPU_OK;
+}
+
+// Element-loop variant: 0 = colour passes, ASM_STAGED = colour passes through the block-contiguous
+// copy, ASM_GATHER = element blocks then per-position gathers, R = row groups of R rows (a power
+// of two <= 64 whose LDS accumulator fits, see max_assembly_rows).
 int set_assembly_rows(Problem &P, uint32_t R) {
+    if (R == ASM_GATHER) {
+        const int rc = build_gather(P);
+        if (rc) return rc;
+        dfree(P.d_Kb);
+        P.asm_rows = ASM_GATHER;
+        return FEMGPU_OK;
+    }
+    free_gather(P.ga);
     if (R == ASM_STAGED) {
         if (!P.d_Kb) HIPCHK(dalloc(&P.d_Kb, 9 * P.ps));
         P.asm_rows = ASM_STAGED;
@@ -1189,9 +1242,10 @@ int femgpu_autotune(femgpu_problem *h, int *unroll) {
     const uint32_t Rmax = max_assembly_rows(P);
     uint32_t best_R = P.asm_rows;
     double best = 1e30;
-    for (uint32_t R : {0u, ASM_STAGED, Rmax, Rmax / 2, Rmax / 4}) {
-        if (R && R != ASM_STAGED && R < 8) continue;
+    for (uint32_t R : {0u, ASM_STAGED, ASM_GATHER, Rmax, Rmax / 2, Rmax / 4}) {
+        if (R && R != ASM_STAGED && R != ASM_GATHER && R < 8) continue;
         int rc = set_assembly_rows(P, R);
+        if (rc == FEMGPU_BAD_INPUT && R == ASM_GATHER) continue;  // beyond the gather lists' index range
         if (rc) return rc;
         double t = 1e30;
         for (int rep = 0; rep < 4; ++rep) {

@@ -37,6 +37,7 @@ struct SubMaterial { double nu, bf0, bf1, bf2; };
 
 constexpr uint32_t ROWS_PER_WG = 256;
 constexpr uint32_t ASM_STAGED = 0xffffffffu;
+constexpr uint32_t ASM_GATHER = 0xfffffffeu;
 constexpr uint32_t NO_SUB = 0xffffffffu;
 // Storage order of the block rows, chosen when a problem is created: AUTO sorts rows by length
 // within windows of SORT_WINDOW_WGS workgroups when that saves more than 3% of the SELL
@@ -139,6 +140,16 @@ struct Ic0 {
     Ic0Args args() const { return Ic0Args{d_nat, d_sub, d_len, d_nlow, d_sptr, d_colN, d_colT, d_kpos, d_tpos, ps}; }
 };
 
+// Gather element loop (ASM_GATHER): pass 1 stores every element's upper blocks (d_em: 9 values
+// per block, NPE (NPE + 1) / 2 blocks per element) and nodal loads (d_ef: 3 per (element, node));
+// pass 2 sums, for every stored position of K, its entries d_kent[d_kptr[p] .. d_kptr[p + 1])
+// ((element block index << 1) | transposed), and for every node its loads d_fent[d_fptr[n] ..]
+// (element * NPE + local node); both lists in the colour order of the colour passes.
+struct GatherAsm {
+    double *d_em = nullptr, *d_ef = nullptr;
+    uint32_t *d_kptr = nullptr, *d_kent = nullptr, *d_fptr = nullptr, *d_fent = nullptr;
+};
+
 struct Problem {
     int kind = 0, npe = 0, device = 0;
     uint32_t n_sub = 1;
@@ -217,6 +228,7 @@ struct Problem {
     // ASM_STAGED = colour passes into the block-contiguous staging copy d_Kb, then the planes
     uint32_t asm_rows = 0;
     double *d_Kb = nullptr;
+    GatherAsm ga;                    // ASM_GATHER: element blocks and the per-position gather lists
     std::vector<uint32_t> conn_h;  // host copy of the (offset) connectivity, for the work lists
     RowGroups rg;
     uint32_t *d_gptr = nullptr, *d_gel = nullptr;

@@ -344,7 +344,8 @@ class Formulation:
 
     def set_assembly_rows(self, rows):
         """Element-loop variant: 0 = colour passes, ASSEMBLY_STAGED = colour passes through a
-        block-contiguous copy of K, R = row groups of R rows (see femgpu.h)."""
+        block-contiguous copy of K, ASSEMBLY_GATHER = element blocks then per-position gathers,
+        R = row groups of R rows (see femgpu.h)."""
         A.check(self._lib.femgpu_set_assembly_rows(self._h, int(rows)), "set_assembly_rows")
 
     def assembly_rows(self):

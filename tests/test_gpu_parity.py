@@ -376,8 +376,9 @@ def test_symmetric_spmv_format_matches_oracle(oracle, kind):
 
 @pytest.mark.parametrize("kind", [L.TETRA, L.TETRA10, L.HEXA])
 def test_assembly_variants_agree(oracle, kind):
-    """Colour-pass and row-group element loops (every admissible group size) give the oracle's K
-    and F and agree with one another to rounding (same per-entry summation order)."""
+    """Colour-pass, staged, gather and row-group element loops (every admissible group size) give
+    the oracle's K and F and agree with one another to rounding (same per-entry summation order);
+    the staged passes and the gather give bitwise the colour passes' K and F."""
     say("assembly variants", kind)
     c, e, E = small_case(kind, 40 + kind)
     bf = (0.1, 0.2, -0.3)
@@ -386,7 +387,7 @@ def test_assembly_variants_agree(oracle, kind):
     _, rmax = f.assembly_rows()
     assert rmax >= 8
     out = {}
-    for R in (0, L._capi.ASSEMBLY_STAGED, rmax, rmax // 2, 8):
+    for R in (0, L._capi.ASSEMBLY_STAGED, L._capi.ASSEMBLY_GATHER, rmax, rmax // 2, 8):
         f.set_assembly_rows(R)
         f.assemble()
         rp, col, vals = f.get_mat()
@@ -395,8 +396,8 @@ def test_assembly_variants_agree(oracle, kind):
         assert np.abs(vals - ovals).max() <= 1e-13 * scale, R
         assert np.abs(F - oF).max() <= 1e-13 * np.abs(oF).max(), R
         out[R] = (vals, F)
-    staged = out[L._capi.ASSEMBLY_STAGED]
-    assert np.array_equal(staged[0], out[0][0]) and np.array_equal(staged[1], out[0][1])   # same adds, same order
+    for V in (L._capi.ASSEMBLY_STAGED, L._capi.ASSEMBLY_GATHER):   # same adds, same order
+        assert np.array_equal(out[V][0], out[0][0]) and np.array_equal(out[V][1], out[0][1]), V
     for R, (v, F) in out.items():
         assert np.abs(v - out[0][0]).max() <= 1e-15 * np.abs(out[0][0]).max(), R
         assert np.abs(F - out[0][1]).max() <= 1e-15 * np.abs(out[0][1]).max(), R

@@ -52,7 +52,7 @@ def test_orders_give_the_same_system(oracle, kind):
     ref = None
     for f in (nat, srt):
         _, rmax = f.assembly_rows()
-        for R in (0, rmax, 8):
+        for R in (0, rmax, 8, A.ASSEMBLY_GATHER):
             f.set_assembly_rows(R)
             f.assemble()
             rp, col, vals = f.get_mat()

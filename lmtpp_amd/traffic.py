@@ -67,8 +67,8 @@ def run(n):
 
 
 def assembly(n):
-    """Workload of the element-loop counters: Hexa8 n^3 assembled three times with row groups of
-    64 rows, then three times with colour passes."""
+    """Workload of the element-loop counters: Hexa8 n^3 assembled three times with each element-loop
+    variant: row groups of 64 rows, colour passes, staged colour passes, gather."""
     here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     sys.path.insert(0, here)
     sys.path.insert(0, os.path.join(here, "tests"))
@@ -77,11 +77,11 @@ def assembly(n):
 
     c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
     f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
-    for R in (64, 0):
+    for R in (64, 0, L._capi.ASSEMBLY_STAGED, L._capi.ASSEMBLY_GATHER):
         f.set_assembly_rows(R)
         for _ in range(3):
             f.assemble()
-        print("rows %d: assembly %.2f ms" % (R, f.assemble_ms()), flush=True)
+        print("rows %#x: assembly %.2f ms" % (R, f.assemble_ms()), flush=True)
     f.close()
 
 

@@ -88,7 +88,8 @@ struct SymDev {
 // (assume_constant_matrix).
 constexpr uint32_t MF_BLOCK = 256;
 constexpr uint32_t MF_NODE_REGS = 2;
-constexpr uint32_t MF_NO_NODE = 0xffffffffu;  // local nodes per lane whose node and slot k_mf_block keeps in registers
+constexpr uint32_t MF_NO_NODE = 0xffffffffu;
+constexpr uint32_t MF_FIRST_HOLDER = 0x80000000u, MF_SLOT_BITS = 0x7fffffffu;  // shared-node slot: flag | slot  // local nodes per lane whose node and slot k_mf_block keeps in registers
 constexpr uint32_t MF_EMPTY = 0xffu, MF_BOX = 0x40u, MF_COLOR_BITS = 0x3fu;
 struct MatFree {
     bool built = false, snap = false;
@@ -106,7 +107,11 @@ struct MatFree {
     uint32_t ls = 0;                 // stride of the per-block node lists (max_local rounded up to 64)
     // block b, local node l < its count (lptr[b + 1] - lptr[b]): entry b * ls + l holds the global
     // node (MF_NO_NODE past the count) and its partial slot when shared with another block (~0 if not)
+    // (bit 31 of a shared node's slot: this block is the first holding the node and stores its
+    // updated CG direction)
     uint32_t *d_lnode_pad = nullptr, *d_lslot_pad = nullptr;
+    double *d_pb = nullptr;          // second CG direction buffers (with the problem's d_p / d_p32,
+    float *d_pb32 = nullptr;         // used in turn by iteration parity when the update is fused)
     uint32_t *d_bnode = nullptr, *d_bptr = nullptr;  // shared node k: partials [bptr[k], bptr[k + 1])
     double *d_halo = nullptr;        // 3 values per partial slot (FP64, or FP32 in its first half)
     std::vector<uint32_t> node_blk, node_loc;  // first block holding each node, its local index (~0: none)

@@ -367,9 +367,12 @@ inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
 // their block-local indices; the corner coordinates (FP64, read-only) are gathered from memory.
 // PQ: partial of sum_e x_e . (K_e x_e) = sum over the block's local nodes of x . (their summed
 // element forces), taken at the write-out; with the penalty pass's partials it is x . K x.
-// DIR: the CG direction update is fused in: the product is applied to p = z + beta x (beta of
-// the scalars), and the block stores p into x at the nodes it owns (no other block reads them;
-// k_mf_halo stores the shared ones after every block has read them).
+// DIR: the CG direction update is fused in.  The direction lives in two buffers, x and xb, used in
+// turn by the parity of the iteration count (device-side, so captured iterations, pauses and
+// restarts agree): the product is applied to p = z + beta s (beta of the scalars, s the stored
+// direction, in x for even iterations and xb for odd ones), and the block stores p into the other
+// buffer at the nodes it owns and at the shared nodes it holds first (lslot bit 31), so every node
+// is written once and no block reads a value another block has already replaced.
 // LDS: per local node 3 input values and 3 accumulators.
 template <typename T, bool PQ, bool DIR, bool BOXES>
 __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
@@ -380,11 +383,15 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
                                                   const uint32_t *__restrict__ psrc, const double *__restrict__ fpen,
                                                   const double *__restrict__ coords, const double *__restrict__ E,
                                                   const SubMaterial *__restrict__ mat,
-                                                  const BoxRec<T> *__restrict__ rec, T *x, const T *__restrict__ z,
-                                                  T *__restrict__ y, T *__restrict__ halo, const ProbScalars *S,
-                                                  double *partials) {
+                                                  const BoxRec<T> *__restrict__ rec, const T *x, T *xb,
+                                                  const T *__restrict__ z, T *__restrict__ y, T *__restrict__ halo,
+                                                  const ProbScalars *S, double *partials) {
     extern __shared__ double mf_lds[];
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    // DIR: stored direction in, updated direction out (the two buffers by iteration parity)
+    const bool odd = DIR && (S[0].iter & 1);
+    const T *xs = odd ? xb : x;
+    T *xo = odd ? const_cast<T *>(x) : xb;
     const unsigned b = xcd_slab();
     // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
     // loads need no other load first
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
         const size_t g = 3 * (size_t)gn;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            Ps[3 * i + d] = DIR ? dir_value(z[g + d], beta, x[g + d]) : x[g + d];
+            Ps[3 * i + d] = DIR ? dir_value(z[g + d], beta, xs[g + d]) : x[g + d];
             acc[3 * i + d] = 0;
         }
     };
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
     __syncthreads();
     auto write_out = [&](uint32_t i, uint32_t s, uint32_t g) {
         const bool own = s == 0xffffffffu;
-        T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)s;
+        T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)(s & MF_SLOT_BITS);
         const T a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
         dst[0] = a0;
         dst[1] = a1;
@@ -486,10 +493,10 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
         if (PQ || DIR) {
             const T x0 = Ps[3 * i], x1 = Ps[3 * i + 1], x2 = Ps[3 * i + 2];
             if (PQ) pq[0] += (double)x0 * (double)a0 + (double)x1 * (double)a1 + (double)x2 * (double)a2;
-            if (DIR && own) {
-                x[3 * (size_t)g] = x0;
-                x[3 * (size_t)g + 1] = x1;
-                x[3 * (size_t)g + 2] = x2;
+            if (DIR && (own || (s & MF_FIRST_HOLDER))) {
+                xo[3 * (size_t)g] = x0;
+                xo[3 * (size_t)g + 1] = x1;
+                xo[3 * (size_t)g + 2] = x2;
             }
         }
     };
@@ -504,12 +511,11 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
 // Shared nodes: y = sum of the blocks' partials, in block order.  PQ: the first workgroup also
 // turns the p.q partials of the element blocks and of the nodes-of-no-element pass (complete:
 // written by earlier kernels) into alpha, which the vector update reads after this kernel.
-// DIR: also stores the updated direction p = z + beta x of the shared nodes into x.
-template <typename T, bool PQ, bool DIR>
+template <typename T, bool PQ>
 __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__restrict__ bnode,
                                                  const uint32_t *__restrict__ bptr, const T *__restrict__ halo,
                                                  T *__restrict__ y, ProbScalars *S, const double *__restrict__ partials,
-                                                 uint32_t n_slots, T *__restrict__ x, const T *__restrict__ z) {
+                                                 uint32_t n_slots) {
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
     if (PQ && blockIdx.x == 0) {
         double o[1];
@@ -518,12 +524,6 @@ __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__r
     }
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    if (DIR) {
-        const T beta = (T)S[0].beta;
-        const size_t g = 3 * (size_t)bnode[i];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[g + d] = dir_value(z[g + d], beta, x[g + d]);
-    }
     T v0 = 0, v1 = 0, v2 = 0;
     for (uint32_t k = bptr[i]; k < bptr[i + 1]; ++k) {
         v0 += halo[3 * (size_t)k];
@@ -538,19 +538,22 @@ __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__r
 
 // Nodes of no element: their rows hold the penalty alone, y = M P x on their constrained DOFs and
 // 0 on the others.
-// DIR: applied to (and stores) the updated direction p = z + beta x, as k_mf_block.
+// DIR: applied to the updated direction, stored into the other buffer, as k_mf_block.
 template <typename T, bool PQ, bool DIR>
 __global__ __launch_bounds__(256) void k_mf_iso(uint32_t n, const uint32_t *__restrict__ dof,
                                                 const uint32_t *__restrict__ src, const double *__restrict__ fpen,
-                                                T *__restrict__ x, const T *__restrict__ z, T *__restrict__ y,
+                                                const T *x, T *xb, const T *__restrict__ z, T *__restrict__ y,
                                                 const ProbScalars *S, double *partials, uint32_t slot0) {
     if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    const bool odd = DIR && (S[0].iter & 1);
+    const T *xs = odd ? xb : x;
+    T *xo = odd ? const_cast<T *>(x) : xb;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     double acc[1] = {0.0};
     if (i < n) {
         const uint32_t d = dof[i], k = src[i];
-        const T xv = DIR ? dir_value(z[d], (T)S[0].beta, x[d]) : x[d];
-        if (DIR) x[d] = xv;
+        const T xv = DIR ? dir_value(z[d], (T)S[0].beta, xs[d]) : x[d];
+        if (DIR) xo[d] = xv;
         const T a = k == 0xffffffffu ? (T)0 : (T)fpen[k] * xv;
         y[d] = a;
         acc[0] = (double)a * (double)xv;
@@ -583,7 +586,7 @@ void mf_free(MatFree &M) {
     dfree(M.d_lnode_pad); dfree(M.d_lslot_pad); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof);
     dfree(M.d_isrc);
     dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
-    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_wg0); dfree(M.d_nwg);
+    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_wg0); dfree(M.d_nwg);
     M = MatFree();
 }
 
@@ -763,8 +766,11 @@ int mf_build(Problem &P) {
         bptr.push_back(bptr.back() + nshared[n]);
     }
     std::vector<uint32_t> lslot(lnode.size(), 0xffffffffu);
-    for (size_t l = 0; l < lnode.size(); ++l)  // blocks in order, so slots follow block order
-        if (in_blk[l] != deg[lnode[l]]) lslot[l] = first[lnode[l]]++;
+    for (uint32_t b = 0; b < nb; ++b)  // blocks in order, so slots follow block order
+        for (uint32_t l = lptr[b]; l < lptr[b + 1]; ++l)
+            if (in_blk[l] != deg[lnode[l]])
+                lslot[l] = first[lnode[l]]++ | (M.node_blk[lnode[l]] == b ? MF_FIRST_HOLDER : 0u);
+    if (bptr.back() > MF_SLOT_BITS) return 1;  // partial slots beyond the 31-bit slot field
     std::vector<uint32_t> lnode_pad((size_t)nb * ls, MF_NO_NODE), lslot_pad((size_t)nb * ls, 0xffffffffu);
     for (uint32_t b = 0; b < nb; ++b)
         for (uint32_t l = lptr[b]; l < lptr[b + 1]; ++l) {
@@ -778,6 +784,8 @@ int mf_build(Problem &P) {
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
         !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
         hipMalloc(&M.d_rec64, sizeof(BoxRec<double>) * ns) != hipSuccess ||
+        hipMalloc(&M.d_pb, sizeof(double) * 3 * (size_t)nn) != hipSuccess ||
+        hipMalloc(&M.d_pb32, sizeof(float) * 3 * (size_t)nn) != hipSuccess ||
         hipMalloc(&M.d_rec32, sizeof(BoxRec<float>) * ns) != hipSuccess ||
         hipMalloc(&M.d_halo, sizeof(double) * 3 * std::max<size_t>(1, bptr.back())) != hipSuccess) {
         mf_free(M);
@@ -897,37 +905,39 @@ int mf_prepare(Problem &P) {
 
 // Element-block kernel of one variant (BOXES: every element of the mesh is an axis-aligned box).
 template <typename T, bool PQ, bool DIR, bool BOXES>
-static void mf_block_launch(Problem &P, T *x, const T *z, T *y) {
+static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y) {
     const MatFree &M = P.mf;
     hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES>), dim3(M.n_blocks), dim3(MF_BLOCK), mf_lds_bytes(M.ls, sizeof(T)),
                        P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad,
                        M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat,
-                       reinterpret_cast<const BoxRec<T> *>(sizeof(T) == 4 ? (void *)M.d_rec32 : (void *)M.d_rec64), x, z, y,
-                       reinterpret_cast<T *>(M.d_halo), P.d_ps, M.d_partials);
+                       reinterpret_cast<const BoxRec<T> *>(sizeof(T) == 4 ? (void *)M.d_rec32 : (void *)M.d_rec64), x, xb,
+                       z, y, reinterpret_cast<T *>(M.d_halo), P.d_ps, M.d_partials);
 }
 
 template <typename T, bool PQ, bool DIR>
-static void mf_passes(Problem &P, T *x, const T *z, T *y) {
+static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y) {
     const MatFree &M = P.mf;
-    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, z, y);
-    else mf_block_launch<T, PQ, DIR, false>(P, x, z, y);
+    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, xb, z, y);
+    else mf_block_launch<T, PQ, DIR, false>(P, x, xb, z, y);
     if (M.n_iso)
         hipLaunchKernelGGL((k_mf_iso<T, PQ, DIR>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,
-                           M.d_idof, M.d_isrc, M.d_fpen, x, z, y, P.d_ps, M.d_partials, M.n_blocks);
+                           M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, P.d_ps, M.d_partials, M.n_blocks);
     if (M.n_bnodes)
-        hipLaunchKernelGGL((k_mf_halo<T, PQ, DIR>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,
+        hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,
                            M.d_bnode, M.d_bptr, reinterpret_cast<const T *>(M.d_halo), y, P.d_ps, M.d_partials,
-                           M.n_slots, x, z);
+                           M.n_slots);
 }
 
-// y = K x; with z (PQ only), y = K p for the updated direction p = z + beta x, which is stored
-// into x (the direction update of the CG step, fused into the product's passes).
+// y = K x; with z (PQ only), y = K p for the updated direction p = z + beta s, s the stored
+// direction in x or in the problem's second direction buffer by the parity of the iteration
+// count, p stored into the other one (the direction update of the CG step, fused into the
+// product's passes).
 template <typename T>
-static hipError_t mf_launch_t(Problem &P, const T *x_in, T *y, bool pq, const T *z) {
-    T *x = const_cast<T *>(x_in);  // written only when z is given
-    if (pq && z) mf_passes<T, true, true>(P, x, z, y);
-    else if (pq) mf_passes<T, true, false>(P, x, nullptr, y);
-    else mf_passes<T, false, false>(P, x, nullptr, y);
+static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z) {
+    T *xb = reinterpret_cast<T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
+    if (pq && z) mf_passes<T, true, true>(P, x, xb, z, y);
+    else if (pq) mf_passes<T, true, false>(P, x, nullptr, nullptr, y);
+    else mf_passes<T, false, false>(P, x, nullptr, nullptr, y);
     return hipGetLastError();
 }
 
@@ -939,11 +949,11 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     const MatFree &M = P.mf;
     auto block = [&]() {
         if (precision) {
-            if (M.all_boxes) mf_block_launch<float, false, false, true>(P, P.d_p32, nullptr, P.d_q32);
-            else mf_block_launch<float, false, false, false>(P, P.d_p32, nullptr, P.d_q32);
+            if (M.all_boxes) mf_block_launch<float, false, false, true>(P, P.d_p32, nullptr, nullptr, P.d_q32);
+            else mf_block_launch<float, false, false, false>(P, P.d_p32, nullptr, nullptr, P.d_q32);
         } else {
-            if (M.all_boxes) mf_block_launch<double, false, false, true>(P, P.d_p, nullptr, P.d_q);
-            else mf_block_launch<double, false, false, false>(P, P.d_p, nullptr, P.d_q);
+            if (M.all_boxes) mf_block_launch<double, false, false, true>(P, P.d_p, nullptr, nullptr, P.d_q);
+            else mf_block_launch<double, false, false, false>(P, P.d_p, nullptr, nullptr, P.d_q);
         }
     };
     auto product = [&]() {

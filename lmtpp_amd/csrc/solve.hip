@@ -211,14 +211,17 @@ __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3
 
 // x += alpha p; r -= alpha q; z = M^-1 r (stored for k_update_p); partials of r.z, r.r, max|r|
 // (k_finalize FIN_UPD advances the scalars and applies the stop test).  IC(0): z needs the
-// sweeps, so r.z is left to k_dot_rz (partial 0 stays zero, FIN_UPD_IC).
+// sweeps, so r.z is left to k_dot_rz (partial 0 stays zero, FIN_UPD_IC).  p_alt (direction
+// update fused into a matrix-free product): the product of an even iteration stored the
+// direction in p_alt, of an odd one in p.
 template <int PRE, typename T>
-__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
                                                 const T *__restrict__ q, const T *__restrict__ Minv, T *__restrict__ z,
                                                 WgTables wt, const ProbScalars *S, double *partials) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB || !selected(S[s], SEL_ACTIVE)) return;
+    if (p_alt && !(S[s].iter & 1)) p = p_alt;
     const T alpha = (T)S[s].alpha;
     double acc[3] = {0, 0, 0};
     const uint32_t n = wt.row0[L] + threadIdx.x;
@@ -716,7 +719,8 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const bool fused = fused_direction<T>(P, PRE);
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr);
     finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
-    hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, q, Minv, z, wt, P.d_ps, P.d_partials);
+    const T *p_alt = fused ? reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb) : nullptr;
+    hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q, Minv, z, wt, P.d_ps, P.d_partials);
     if (PRE == PRE_IC0) {
         finalize(P, FIN_UPD_IC);
         ic0_apply(P, r, z, SEL_ACTIVE);

@@ -113,6 +113,7 @@ struct MatFree {
     double *d_pb = nullptr;          // second CG direction buffers (with the problem's d_p / d_p32,
     float *d_pb32 = nullptr;         // used in turn by iteration parity when the update is fused)
     uint32_t *d_bnode = nullptr, *d_bptr = nullptr;  // shared node k: partials [bptr[k], bptr[k + 1])
+    uint32_t *d_hbeg = nullptr;      // node n: partials [hbeg[n], hbeg[n + 1]) (empty unless shared)
     double *d_halo = nullptr;        // 3 values per partial slot (FP64, or FP32 in its first half)
     std::vector<uint32_t> node_blk, node_loc;  // first block holding each node, its local index (~0: none)
     uint32_t *d_idof = nullptr;      // the 3 DOFs of each node of no element ...

@@ -586,7 +586,7 @@ void mf_free(MatFree &M) {
     dfree(M.d_lnode_pad); dfree(M.d_lslot_pad); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof);
     dfree(M.d_isrc);
     dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
-    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_wg0); dfree(M.d_nwg);
+    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_hbeg); dfree(M.d_wg0); dfree(M.d_nwg);
     M = MatFree();
 }
 
@@ -765,6 +765,10 @@ int mf_build(Problem &P) {
         bnode.push_back(n);
         bptr.push_back(bptr.back() + nshared[n]);
     }
+    std::vector<uint32_t> hbeg(first.begin(), first.end());  // first[n]: node n's first partial slot
+    hbeg.push_back(bptr.back());
+    for (uint32_t n = nn; n-- > 0;)
+        if (hbeg[n] == 0xffffffffu) hbeg[n] = hbeg[n + 1];  // no partials: an empty range
     std::vector<uint32_t> lslot(lnode.size(), 0xffffffffu);
     for (uint32_t b = 0; b < nb; ++b)  // blocks in order, so slots follow block order
         for (uint32_t l = lptr[b]; l < lptr[b + 1]; ++l)
@@ -780,7 +784,7 @@ int mf_build(Problem &P) {
     const std::vector<uint32_t> zero(1, 0);
     if (!upload_vec(&M.d_conn, sconn) || !upload_vec(&M.d_loc, sloc) || !upload_vec(&M.d_color, scol) ||
         !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode_pad, lnode_pad) ||
-        !upload_vec(&M.d_lslot_pad, lslot_pad) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
+        !upload_vec(&M.d_lslot_pad, lslot_pad) || !upload_vec(&M.d_hbeg, hbeg) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
         !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
         hipMalloc(&M.d_rec64, sizeof(BoxRec<double>) * ns) != hipSuccess ||
@@ -922,7 +926,8 @@ static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y) {
     if (M.n_iso)
         hipLaunchKernelGGL((k_mf_iso<T, PQ, DIR>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,
                            M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, P.d_ps, M.d_partials, M.n_blocks);
-    if (M.n_bnodes)
+    // DIR: the iteration's vector update sums the shared nodes' partials itself (k_update_mf)
+    if (M.n_bnodes && !DIR)
         hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,
                            M.d_bnode, M.d_bptr, reinterpret_cast<const T *>(M.d_halo), y, P.d_ps, M.d_partials,
                            M.n_slots);
@@ -931,7 +936,8 @@ static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y) {
 // y = K x; with z (PQ only), y = K p for the updated direction p = z + beta s, s the stored
 // direction in x or in the problem's second direction buffer by the parity of the iteration
 // count, p stored into the other one (the direction update of the CG step, fused into the
-// product's passes).
+// product's passes); then y holds the nodes of no shared block only, and the shared nodes'
+// partials are left to the caller's vector update (k_update_mf).
 template <typename T>
 static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z) {
     T *xb = reinterpret_cast<T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);

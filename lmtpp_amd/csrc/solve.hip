@@ -209,6 +209,31 @@ __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3
     }
 }
 
+// One node of the vector update: x += alpha p; r -= alpha q; z = M^-1 r; partials of r.z, r.r,
+// max|r| accumulated into acc.
+template <int PRE, typename T>
+__device__ inline void update_node(uint32_t n, T alpha, T *__restrict__ x, T *__restrict__ r, const T *p,
+                                   const T (&qn)[3], const T *__restrict__ Minv, T *__restrict__ z, double (&acc)[3]) {
+    T rv[3], zv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const size_t i = 3 * (size_t)n + d;
+        x[i] += alpha * p[i];
+        rv[d] = r[i] - alpha * qn[d];
+        r[i] = rv[d];
+    }
+    if (PRE != PRE_IC0) apply_m<PRE, T>(Minv, n, rv, zv);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (PRE != PRE_IC0) {
+            z[3 * (size_t)n + d] = zv[d];
+            acc[0] += (double)rv[d] * (double)zv[d];
+        }
+        acc[1] += (double)rv[d] * (double)rv[d];
+        acc[2] = fmax(acc[2], fabs((double)rv[d]));
+    }
+}
+
 // x += alpha p; r -= alpha q; z = M^-1 r (stored for k_update_p); partials of r.z, r.r, max|r|
 // (k_finalize FIN_UPD advances the scalars and applies the stop test).  IC(0): z needs the
 // sweeps, so r.z is left to k_dot_rz (partial 0 stays zero, FIN_UPD_IC).  p_alt (direction
@@ -226,24 +251,52 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
     double acc[3] = {0, 0, 0};
     const uint32_t n = wt.row0[L] + threadIdx.x;
     if (n < wt.row1[L]) {
-        T rv[3], zv[3];
+        const T qn[3] = {q[3 * (size_t)n], q[3 * (size_t)n + 1], q[3 * (size_t)n + 2]};
+        update_node<PRE, T>(n, alpha, x, r, p, qn, Minv, z, acc);
+    }
+    wg_partial<3, 4u>(acc, partials, gridDim.x, L);
+}
+
+// The same after a matrix-free product with the fused direction update (one sub-problem), with the
+// product's shared-node pass folded in: every workgroup combines the element blocks' p.q partials
+// itself (the same fixed order as k_mf_halo's first workgroup, so every workgroup holds the same
+// alpha; workgroup 0 records it), and a node shared by several blocks takes q as the sum of its
+// partials halo[hbeg[n] .. hbeg[n + 1]) in block order, any other node the product's y.
+template <int PRE, typename T>
+__global__ __launch_bounds__(256) void k_update_mf(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
+                                                   const T *__restrict__ q, const T *__restrict__ halo,
+                                                   const uint32_t *__restrict__ hbeg, const double *__restrict__ mfpart,
+                                                   uint32_t n_mfslots, const T *__restrict__ Minv, T *__restrict__ z,
+                                                   WgTables wt, ProbScalars *S, double *partials) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], SEL_ACTIVE)) return;
+    double o[1];
+    wg_combine<1, 0u>(mfpart, 0u, 0u, n_mfslots, o);
+    if (L == 0 && threadIdx.x == 0) finish_pq(S[s], o[0]);
+    if (!finite(o[0]) || !(o[0] > 0)) return;  // recorded by workgroup 0: the solve stops
+    if (!(S[s].iter & 1)) p = p_alt;
+    const T alpha = (T)(S[s].rho / o[0]);
+    double acc[3] = {0, 0, 0};
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
+        T qn[3];
+        const uint32_t k0 = hbeg[n], k1 = hbeg[n + 1];
+        if (k0 == k1) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const size_t i = 3 * (size_t)n + d;
-            x[i] += alpha * p[i];
-            rv[d] = r[i] - alpha * q[i];
-            r[i] = rv[d];
-        }
-        if (PRE != PRE_IC0) apply_m<PRE, T>(Minv, n, rv, zv);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (PRE != PRE_IC0) {
-                z[3 * (size_t)n + d] = zv[d];
-                acc[0] += (double)rv[d] * (double)zv[d];
+            for (int d = 0; d < 3; ++d) qn[d] = q[3 * (size_t)n + d];
+        } else {
+            T v0 = 0, v1 = 0, v2 = 0;
+            for (uint32_t k = k0; k < k1; ++k) {
+                v0 += halo[3 * (size_t)k];
+                v1 += halo[3 * (size_t)k + 1];
+                v2 += halo[3 * (size_t)k + 2];
             }
-            acc[1] += (double)rv[d] * (double)rv[d];
-            acc[2] = fmax(acc[2], fabs((double)rv[d]));
+            qn[0] = v0;
+            qn[1] = v1;
+            qn[2] = v2;
         }
+        update_node<PRE, T>(n, alpha, x, r, p, qn, Minv, z, acc);
     }
     wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
@@ -718,9 +771,19 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const dim3 g(P.n_wg), blk(256);
     const bool fused = fused_direction<T>(P, PRE);
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr);
-    finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
-    const T *p_alt = fused ? reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb) : nullptr;
-    hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q, Minv, z, wt, P.d_ps, P.d_partials);
+    if constexpr (PRE != PRE_IC0) {
+        if (fused) {
+            // the product's shared-node pass and alpha are folded into the update
+            const T *p_alt = reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
+            hipLaunchKernelGGL((k_update_mf<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
+                               reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.n_slots,
+                               Minv, z, wt, P.d_ps, P.d_partials);
+        }
+    }
+    if (!fused) {
+        finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
+        hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, nullptr, q, Minv, z, wt, P.d_ps, P.d_partials);
+    }
     if (PRE == PRE_IC0) {
         finalize(P, FIN_UPD_IC);
         ic0_apply(P, r, z, SEL_ACTIVE);

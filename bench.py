@@ -49,8 +49,9 @@ def load_traffic(name):
 
 def cpu_baseline(fp64_iters, ndof, n_elem, sample_n=64):
     """Oracle (plain C restatement, one host core) on a bounded sample of the same workload: full
-    assembly of a Hexa8 sample_n^3 cube and 100 block-Jacobi PCG iterations on it, scaled per element
-    and per DOF-iteration to the full cube and to the FP64 iteration count the GPU needed."""
+    assembly of a Hexa8 sample_n^3 cube and 500 block-Jacobi PCG iterations on it (about 10 s of CPU
+    work), scaled per element and per DOF-iteration to the full cube and to the FP64 iteration count
+    the GPU needed."""
     from oracle import oracle as O
     import lmtpp_amd as L
     from problems import clamp_and_traction, lognormal_E
@@ -63,7 +64,7 @@ def cpu_baseline(fp64_iters, ndof, n_elem, sample_n=64):
     rp, col, vals, F, _ = O.assemble(O.HEXA, c, e, E, 0.3, faces=faces, face_trac=tr, pat=pat)
     O.penalty(rp, col, vals, F, dofs, None, 1e5)
     t_asm = time.perf_counter() - t0
-    its = 100
+    its = 500
     t0 = time.perf_counter()
     O.pcg_jacobi(rp, col, vals, F, rel_tol=0.0, max_iter=its, block=True)
     t_it = (time.perf_counter() - t0) / its

@@ -264,6 +264,7 @@ def config4(n_models=256):
         b.set_traction(i, faces, trs)
         b.add_constraints(i, dofs, None, 1e5)
         ndof += 3 * len(c)
+    b.set_assembly_rows(L._capi.ASSEMBLY_GATHER)
     b.assemble()
     # FP64 references (tight), then the timed MIXED batch solve
     b.clear_result()
@@ -273,11 +274,14 @@ def config4(n_models=256):
     for _ in range(2):
         b.assemble()
         t_asm.append(b.assemble_ms())
-    # matrix loads: non-temporal (streamed once per product; with 16-bit column offsets) or plain
-    # (the few problems still iterating late fit the Infinity Cache); all timed, the fastest kept
+    # product variants: the stored matrix with non-temporal loads (streamed once per product; with
+    # 16-bit column offsets) or plain loads (the few problems still iterating late fit the Infinity
+    # Cache), and the matrix-free product (-1); all timed, the fastest kept
     by_nt = {}
-    for nt in (2, 1, 0):
-        b.set_spmv_config(fp64=(0, 2, nt), fp32=(0, 2, nt))
+    variants = {2: ((0, 2, 2), (0, 2, 2)), 1: ((0, 2, 1), (0, 2, 1)), 0: ((0, 2, 0), (0, 2, 0)),
+                -1: ((2, 2, 1), (2, 2, 1))}
+    for nt, (c64, c32) in variants.items():
+        b.set_spmv_config(fp64=c64, fp32=c32)
         times = []
         for _ in range(2):
             b.assemble()
@@ -287,7 +291,7 @@ def config4(n_models=256):
             times.append((time.perf_counter() - t0, ms))
         by_nt[nt] = (float(np.median([t[1] for t in times])), ok)
     best_nt = min(by_nt, key=lambda k: by_nt[k][0])
-    b.set_spmv_config(fp64=(0, 2, best_nt), fp32=(0, 2, best_nt))
+    b.set_spmv_config(fp64=variants[best_nt][0], fp32=variants[best_nt][1])
     times = []
     for _ in range(2):
         b.assemble()
@@ -300,14 +304,16 @@ def config4(n_models=256):
     iters = [st.iters for st in b.last_stats]
     return {"workload": "256 Hexa8 cubes, side ~ U{8..32} cells (seed 4), lognormal E (sigma 0.5), nu ~ U[0.2,0.4], "
                         "traction ~ N(0,1) on x=1, clamp x=0; one batch: per-problem MIXED block-Jacobi PCG to rel. res. 1e-10, "
-                        "iteration chunks replayed from a captured hipGraph",
+                        "iteration chunks replayed from a captured hipGraph; gather element loop",
             "models": n_models, "total_dof": ndof, "batch_setup_s": round(setup, 2),
             "assembly_ms": round(float(np.median(t_asm)), 3), "solve_ms": round(ms, 2),
             "aggregate_mdof_s": round(ndof / (ms * 1e-3) / 1e6, 3), "max_rel_err_vs_fp64": float(max(errs)),
             "median_rel_err": float(np.median(errs)), "iters_min": int(min(iters)), "iters_max": int(max(iters)),
-            "all_ok": bool(ok), "spmv_loads": int(best_nt),
-            "solve_ms_by_loads": {"nontemporal_col16": round(by_nt[2][0], 2), "nontemporal": round(by_nt[1][0], 2),
-                                  "plain": round(by_nt[0][0], 2)}}
+            "all_ok": bool(ok), "product": {2: "stored, nontemporal_col16", 1: "stored, nontemporal", 0: "stored, plain",
+                                            -1: "matrix-free"}[best_nt],
+            "solve_ms_by_product": {"nontemporal_col16": round(by_nt[2][0], 2), "nontemporal": round(by_nt[1][0], 2),
+                                    "plain": round(by_nt[0][0], 2), "matrix_free": round(by_nt[-1][0], 2)},
+            "assembly": "gather"}
 
 
 def main():

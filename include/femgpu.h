@@ -314,6 +314,9 @@ FEMGPU_API int femgpu_batch_after_solve(femgpu_batch *b, uint32_t i, double *eps
 FEMGPU_API int femgpu_batch_write_result_vtu(femgpu_batch *b, uint32_t i, const char *path);
 /* Device time (ms) of the last batched assembly. */
 FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
+/* The element-loop variant of a batch (femgpu_set_assembly_rows' values); every variant gives
+ * each sub-problem bitwise the K and F it gets alone. */
+FEMGPU_API int femgpu_batch_set_assembly_rows(femgpu_batch *b, uint32_t rows);
 FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);
 
 /* ---- diagnostics -------------------------------------------------------------------------- */

@@ -113,6 +113,7 @@ _SIGS = {
     "femgpu_batch_after_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "femgpu_get_spmv_config": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_set_assembly_rows": (C.c_int, [_P, C.c_uint32]),
+    "femgpu_batch_set_assembly_rows": (C.c_int, [_P, C.c_uint32]),
     "femgpu_get_assembly_rows": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "femgpu_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_batch_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),

@@ -95,6 +95,10 @@ class Batch:
         nt = (C.c_int * 2)(int(fp64[2]), int(fp32[2]))
         A.check(self._lib.femgpu_batch_set_spmv_config(self._h, f, u, nt), "batch_set_spmv_config")
 
+    def set_assembly_rows(self, rows):
+        """Element-loop variant, as Formulation.set_assembly_rows."""
+        A.check(self._lib.femgpu_batch_set_assembly_rows(self._h, int(rows)), "batch_set_assembly_rows")
+
     def clear_result(self):
         """Zero initial guess for every problem (solves otherwise start from the previous result)."""
         A.check(self._lib.femgpu_batch_clear_result(self._h), "batch_clear_result")

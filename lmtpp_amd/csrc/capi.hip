@@ -1300,9 +1300,15 @@ int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unr
             (format[i] == FMT_MF && !mf_eligible(h->P)))
             return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1/2, unroll 1..3, loads 0..2 (1, 2 not with "
                                           "format 1; 2 needs every column offset in 16 bits; format 2 needs a Hexa "
-                                          "problem with one sub-problem)");
+                                          "problem)");
     }
     Problem &P = h->P;
+    if (format[0] == FMT_MF || format[1] == FMT_MF) {
+        const int rc = mf_build(P);  // the block layout, once; 1: a layout this product does not serve
+        if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR,
+                            "set_spmv_config: matrix-free product not served (more than 32 sub-colours in a block, a "
+                            "batch with a node of no element, or block lists beyond their index range)");
+    }
     if ((format[0] == FMT_SYM || format[1] == FMT_SYM) && !P.sym.built) {
         const int rc = sym_ensure(P, format[0] == FMT_SYM ? 0 : 1);
         if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "set_spmv_config: symmetric storage");
@@ -1627,6 +1633,11 @@ int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms) {
     if (!b || !ms) return fail(FEMGPU_BAD_INPUT, "batch_get_assemble_ms: null");
     *ms = b->H.P.last_asm_ms;
     return FEMGPU_OK;
+}
+
+int femgpu_batch_set_assembly_rows(femgpu_batch *b, uint32_t rows) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_set_assembly_rows: null");
+    return femgpu_set_assembly_rows(&b->H, rows);
 }
 
 int femgpu_batch_destroy(femgpu_batch *b) {

@@ -114,6 +114,8 @@ struct MatFree {
     float *d_pb32 = nullptr;         // used in turn by iteration parity when the update is fused)
     uint32_t *d_bnode = nullptr, *d_bptr = nullptr;  // shared node k: partials [bptr[k], bptr[k + 1])
     uint32_t *d_hbeg = nullptr;      // node n: partials [hbeg[n], hbeg[n + 1]) (empty unless shared)
+    uint32_t *d_bsub = nullptr;      // sub-problem of each block (NO_SUB: padding); a block never straddles two
+    std::vector<uint32_t> sub_b0, sub_nb;  // each sub-problem's consecutive blocks
     double *d_halo = nullptr;        // 3 values per partial slot (FP64, or FP32 in its first half)
     std::vector<uint32_t> node_blk, node_loc;  // first block holding each node, its local index (~0: none)
     uint32_t *d_idof = nullptr;      // the 3 DOFs of each node of no element ...
@@ -127,7 +129,7 @@ struct MatFree {
     SubMaterial *d_mat = nullptr;    // material at the last matrix assembly
     double *d_fpen = nullptr;        // summed penalty M * P of each DOF of snap_dofs at that assembly
     double *d_partials = nullptr;    // one p.q partial per workgroup (element blocks, then isolated nodes)
-    uint32_t *d_wg0 = nullptr, *d_nwg = nullptr;  // {0}, {n_slots}: the finalize's slot range
+    uint32_t *d_wg0 = nullptr, *d_nwg = nullptr;  // per sub-problem: its p.q partial slots (first, count)
 };
 
 // Device side of the block IC(0) preconditioner (schedule in host_mesh.hpp).

@@ -339,6 +339,7 @@ __device__ inline void box_coefs(const double (&X0)[3], const double (&X6)[3], d
 // The records of every box slot, both precisions (after each snapshot of the material).
 __global__ __launch_bounds__(256) void k_mf_boxrec(size_t n, const uint8_t *__restrict__ color,
                                                    const uint32_t *__restrict__ conn, const uint32_t *__restrict__ elem,
+                                                   const uint32_t *__restrict__ bsub,
                                                    const double *__restrict__ coords, const double *__restrict__ E,
                                                    const SubMaterial *__restrict__ mat, BoxRec<double> *__restrict__ r64,
                                                    BoxRec<float> *__restrict__ r32) {
@@ -351,8 +352,9 @@ __global__ __launch_bounds__(256) void k_mf_boxrec(size_t n, const uint8_t *__re
         X0[d] = coords[3 * (size_t)conn[8 * s] + d];
         X6[d] = coords[3 * (size_t)conn[8 * s + 6] + d];
     }
-    box_coefs<double>(X0, X6, E[elem[s]], mat[0].nu, r64[s]);
-    box_coefs<float>(X0, X6, E[elem[s]], mat[0].nu, r32[s]);
+    const double nu = mat[bsub[s / MF_BLOCK]].nu;
+    box_coefs<double>(X0, X6, E[elem[s]], nu, r64[s]);
+    box_coefs<float>(X0, X6, E[elem[s]], nu, r32[s]);
 }
 
 // Dynamic LDS of k_mf_block for blocks of up to max_local nodes and values of vsize bytes.
@@ -377,6 +379,7 @@ inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
 template <typename T, bool PQ, bool DIR, bool BOXES>
 __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
+                                                  const uint32_t *__restrict__ bsub,
                                                   uint32_t n_colors, uint32_t ls, const uint32_t *__restrict__ lnode,
                                                   const uint32_t *__restrict__ lslot,
                                                   const uint32_t *__restrict__ pptr, const uint32_t *__restrict__ pidx,
@@ -387,18 +390,24 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
                                                   const T *__restrict__ z, T *__restrict__ y, T *__restrict__ halo,
                                                   const ProbScalars *S, double *partials) {
     extern __shared__ double mf_lds[];
-    if (PQ && !selected(S[0], SEL_ACTIVE)) return;
+    const unsigned b = xcd_slab();
+    const uint32_t sb = bsub[b];  // the block's sub-problem (NO_SUB: an empty padding block)
+    if (sb == NO_SUB) {
+        // a single problem's p.q combine spans the padding blocks too: they contribute zero
+        if (PQ && threadIdx.x == 0) partials[b] = 0.0;
+        return;
+    }
+    if (PQ && !selected(S[sb], SEL_ACTIVE)) return;
     // DIR: stored direction in, updated direction out (the two buffers by iteration parity)
-    const bool odd = DIR && (S[0].iter & 1);
+    const bool odd = DIR && (S[sb].iter & 1);
     const T *xs = odd ? xb : x;
     T *xo = odd ? const_cast<T *>(x) : xb;
-    const unsigned b = xcd_slab();
     // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
     // loads need no other load first
     const size_t l0 = (size_t)b * ls;
     T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][3] the product's input
     T *acc = Ps + 3 * (size_t)ls;           // [ls][3] summed nodal forces
-    const T beta = DIR ? (T)S[0].beta : (T)0;
+    const T beta = DIR ? (T)S[sb].beta : (T)0;
     // everything the later phases read from memory is loaded here, ahead of the barriers (a load
     // after a barrier cannot be hoisted above it): the penalty range, and the global node and
     // partial slot of the lane's first MF_NODE_REGS local nodes (a brick's ~400; larger blocks
@@ -444,7 +453,7 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
             const uint4 ca = reinterpret_cast<const uint4 *>(conn)[2 * slot];
             const uint4 cb = reinterpret_cast<const uint4 *>(conn)[2 * slot + 1];
             const uint32_t nd[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-            gen::hooke(E[elem[slot]], mat[0].nu, Hd);
+            gen::hooke(E[elem[slot]], mat[sb].nu, Hd);
 #pragma unroll
             for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -508,20 +517,12 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
     if (PQ) wg_partial<1, 0u>(pq, partials, 0u, b);
 }
 
-// Shared nodes: y = sum of the blocks' partials, in block order.  PQ: the first workgroup also
-// turns the p.q partials of the element blocks and of the nodes-of-no-element pass (complete:
-// written by earlier kernels) into alpha, which the vector update reads after this kernel.
-template <typename T, bool PQ>
+// Shared nodes: y = sum of the blocks' partials, in block order (a node of a finished
+// sub-problem sums stale partials into a q nobody reads).
+template <typename T>
 __global__ __launch_bounds__(256) void k_mf_halo(uint32_t n, const uint32_t *__restrict__ bnode,
                                                  const uint32_t *__restrict__ bptr, const T *__restrict__ halo,
-                                                 T *__restrict__ y, ProbScalars *S, const double *__restrict__ partials,
-                                                 uint32_t n_slots) {
-    if (PQ && !selected(S[0], SEL_ACTIVE)) return;
-    if (PQ && blockIdx.x == 0) {
-        double o[1];
-        wg_combine<1, 0u>(partials, 0u, 0u, n_slots, o);
-        if (threadIdx.x == 0) finish_pq(S[0], o[0]);
-    }
+                                                 T *__restrict__ y) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     T v0 = 0, v1 = 0, v2 = 0;
@@ -578,7 +579,7 @@ __global__ void k_mf_pen(uint32_t n_cdof, const uint32_t *__restrict__ cptr, con
 // ---------------------------------------------------------------------------------------------
 
 bool mf_eligible(const Problem &P) {
-    return P.kind == HEXA && P.n_sub == 1 && !P.from_bsr && P.n_elem > 0 && P.conn_h.size() == 8 * (size_t)P.n_elem;
+    return P.kind == HEXA && !P.from_bsr && P.n_elem > 0 && P.conn_h.size() == 8 * (size_t)P.n_elem;
 }
 
 void mf_free(MatFree &M) {
@@ -586,7 +587,7 @@ void mf_free(MatFree &M) {
     dfree(M.d_lnode_pad); dfree(M.d_lslot_pad); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof);
     dfree(M.d_isrc);
     dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
-    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_hbeg); dfree(M.d_wg0); dfree(M.d_nwg);
+    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_hbeg); dfree(M.d_bsub); dfree(M.d_wg0); dfree(M.d_nwg);
     M = MatFree();
 }
 
@@ -638,35 +639,52 @@ int mf_build(Problem &P) {
     const uint32_t *conn = P.conn_h.data();
     std::vector<double> X(3 * (size_t)nn);
     if (hipMemcpy(X.data(), P.d_coords, X.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return 5;
-    // element order: Morton code of the element centre on a grid of the typical element size (on a
-    // lattice, one cell per element, so aligned blocks are 8 x 8 x 4 bricks), ties by element id
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (uint32_t n = 0; n < nn; ++n)
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = std::min(lo[d], X[3 * (size_t)n + d]);
-            hi[d] = std::max(hi[d], X[3 * (size_t)n + d]);
-        }
-    double vol = 1.0, ext = 0.0;
-    for (int d = 0; d < 3; ++d) {
-        ext = std::max(ext, hi[d] - lo[d]);
-        vol *= std::max(hi[d] - lo[d], 0.0);
-    }
-    double h = vol > 0 ? std::cbrt(vol / ne) : ext / std::cbrt((double)ne);
-    if (!(h > 0)) h = 1.0;
+    // element order, per sub-problem (a block never straddles two): Morton code of the element
+    // centre on a grid of the sub-problem's typical element size (on a lattice, one cell per
+    // element, so aligned blocks are 8 x 8 x 4 bricks), ties by element id
     std::vector<std::pair<uint64_t, uint32_t>> key(ne);
-    for (uint32_t e = 0; e < ne; ++e) {
-        uint32_t q[3];
+    std::vector<uint32_t> bstart, bcount, bsub;  // per block: first sorted element, count, sub-problem
+    for (uint32_t sp = 0; sp < P.n_sub; ++sp) {
+        const uint32_t e0 = P.elem_off[sp], e1 = P.elem_off[sp + 1], n0 = P.node_off[sp], n1 = P.node_off[sp + 1];
+        if (e1 == e0) continue;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (uint32_t n = n0; n < n1; ++n)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], X[3 * (size_t)n + d]);
+                hi[d] = std::max(hi[d], X[3 * (size_t)n + d]);
+            }
+        double vol = 1.0, ext = 0.0;
         for (int d = 0; d < 3; ++d) {
-            double c = 0;
-            for (int a = 0; a < 8; ++a) c += X[3 * (size_t)conn[8 * (size_t)e + a] + d];
-            q[d] = (uint32_t)std::min(2097151.0, std::max(0.0, std::floor((c / 8 - lo[d]) / h)));
+            ext = std::max(ext, hi[d] - lo[d]);
+            vol *= std::max(hi[d] - lo[d], 0.0);
         }
-        key[e] = {morton3(q[0], q[1], q[2]), e};
+        double h = vol > 0 ? std::cbrt(vol / (e1 - e0)) : ext / std::cbrt((double)(e1 - e0));
+        if (!(h > 0)) h = 1.0;
+        for (uint32_t e = e0; e < e1; ++e) {
+            uint32_t q[3];
+            for (int d = 0; d < 3; ++d) {
+                double c = 0;
+                for (int a = 0; a < 8; ++a) c += X[3 * (size_t)conn[8 * (size_t)e + a] + d];
+                q[d] = (uint32_t)std::min(2097151.0, std::max(0.0, std::floor((c / 8 - lo[d]) / h)));
+            }
+            key[e] = {morton3(q[0], q[1], q[2]), e};
+        }
+        std::sort(key.begin() + e0, key.begin() + e1);
+        for (uint32_t k = e0; k < e1; k += MF_BLOCK) {
+            bstart.push_back(k);
+            bcount.push_back(std::min(MF_BLOCK, e1 - k));
+            bsub.push_back(sp);
+        }
     }
-    std::sort(key.begin(), key.end());
-    // blocks of MF_BLOCK consecutive elements, padded to a multiple of 8 blocks (the XCD mapping)
-    const uint32_t nb = ((ne + MF_BLOCK - 1) / MF_BLOCK + 7) / 8 * 8;
+    // blocks of up to MF_BLOCK elements of one sub-problem, padded to a multiple of 8 blocks (the
+    // XCD mapping) with empty blocks of no sub-problem
+    const uint32_t nb = ((uint32_t)bstart.size() + 7) / 8 * 8;
     const size_t ns = (size_t)nb * MF_BLOCK;
+    while (bstart.size() < nb) {
+        bstart.push_back(0);
+        bcount.push_back(0);
+        bsub.push_back(NO_SUB);
+    }
     std::vector<uint32_t> sconn(8 * ns, 0), selem(ns, 0), lptr(nb + 1, 0), lnode, deg(nn, 0), stamp(nn, 0xffffffffu),
         lid(nn, 0), in_blk;
     M.node_blk.assign(nn, 0xffffffffu);
@@ -687,8 +705,7 @@ int mf_build(Problem &P) {
         // the block's elements swept by (z, y, x) of their centres: greedy first-fit then finds the
         // 8-colour parity pattern on lattices (the Morton order alone needs more sub-colours)
         blk.clear();
-        for (uint32_t k = 0; k < MF_BLOCK && (size_t)b * MF_BLOCK + k < ne; ++k)
-            blk.push_back(key[(size_t)b * MF_BLOCK + k].second);
+        for (uint32_t k = 0; k < bcount[b]; ++k) blk.push_back(key[(size_t)bstart[b] + k].second);
         std::stable_sort(blk.begin(), blk.end(), [&](uint32_t p, uint32_t q) {
             for (int d = 2; d >= 0; --d)
                 if (cen[3 * (size_t)p + d] != cen[3 * (size_t)q + d]) return cen[3 * (size_t)p + d] < cen[3 * (size_t)q + d];
@@ -765,6 +782,9 @@ int mf_build(Problem &P) {
         bnode.push_back(n);
         bptr.push_back(bptr.back() + nshared[n]);
     }
+    // nodes of no element take a pass of their own, whose partials follow the blocks': served for
+    // a single problem only
+    if (P.n_sub > 1 && !idof.empty()) return 1;
     std::vector<uint32_t> hbeg(first.begin(), first.end());  // first[n]: node n's first partial slot
     hbeg.push_back(bptr.back());
     for (uint32_t n = nn; n-- > 0;)
@@ -786,7 +806,8 @@ int mf_build(Problem &P) {
         !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode_pad, lnode_pad) ||
         !upload_vec(&M.d_lslot_pad, lslot_pad) || !upload_vec(&M.d_hbeg, hbeg) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
-        !upload_vec(&M.d_wg0, zero) || !upload_vec(&M.d_nwg, zero) ||
+        !upload_vec(&M.d_bsub, bsub) || !upload_vec(&M.d_wg0, std::vector<uint32_t>(P.n_sub, 0)) ||
+        !upload_vec(&M.d_nwg, std::vector<uint32_t>(P.n_sub, 0)) ||
         hipMalloc(&M.d_rec64, sizeof(BoxRec<double>) * ns) != hipSuccess ||
         hipMalloc(&M.d_pb, sizeof(double) * 3 * (size_t)nn) != hipSuccess ||
         hipMalloc(&M.d_pb32, sizeof(float) * 3 * (size_t)nn) != hipSuccess ||
@@ -804,6 +825,14 @@ int mf_build(Problem &P) {
     M.n_bnodes = (uint32_t)bnode.size();
     M.n_halo = bptr.back();
     M.n_iso = (uint32_t)idof.size() / 3;
+    // each sub-problem's blocks are consecutive: their p.q partial slots
+    M.sub_b0.assign(P.n_sub, 0);
+    M.sub_nb.assign(P.n_sub, 0);
+    for (uint32_t b = nb; b-- > 0;)
+        if (bsub[b] != NO_SUB) {
+            M.sub_b0[bsub[b]] = b;
+            ++M.sub_nb[bsub[b]];
+        }
     M.built = true;
     return 0;
 }
@@ -813,7 +842,7 @@ hipError_t mf_snapshot(Problem &P) {
     MatFree &M = P.mf;
     hipError_t e = hipSuccess;
     if (!M.d_E) e = hipMalloc(&M.d_E, sizeof(double) * P.n_elem);
-    if (e == hipSuccess && !M.d_mat) e = hipMalloc(&M.d_mat, sizeof(SubMaterial));
+    if (e == hipSuccess && !M.d_mat) e = hipMalloc(&M.d_mat, sizeof(SubMaterial) * P.n_sub);
     if (e == hipSuccess && P.n_cdof > M.cap_fix) {
         dfree(M.d_fpen);
         e = hipMalloc(&M.d_fpen, sizeof(double) * P.n_cdof);
@@ -822,7 +851,8 @@ hipError_t mf_snapshot(Problem &P) {
         M.pen_built = false;
     }
     if (e == hipSuccess) e = hipMemcpyAsync(M.d_E, P.d_E, sizeof(double) * P.n_elem, hipMemcpyDeviceToDevice, P.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(M.d_mat, P.d_mat, sizeof(SubMaterial), hipMemcpyDeviceToDevice, P.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(M.d_mat, P.d_mat, sizeof(SubMaterial) * P.n_sub, hipMemcpyDeviceToDevice, P.stream);
     if (e == hipSuccess && P.n_cdof)
         hipLaunchKernelGGL(k_mf_pen, dim3((P.n_cdof + 255) / 256), dim3(256), 0, P.stream, P.n_cdof, P.d_cptr, P.d_csub,
                            P.d_cpen, P.d_ps, M.d_fpen);
@@ -885,7 +915,7 @@ int mf_prepare(Problem &P) {
     if (!M.rec_valid && M.n_boxes) {  // stream order: after the snapshot's copies
         const size_t ns = (size_t)M.n_blocks * MF_BLOCK;
         hipLaunchKernelGGL(k_mf_boxrec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, P.stream, ns, M.d_color, M.d_conn,
-                           M.d_elem, P.d_coords, M.d_E, M.d_mat, reinterpret_cast<BoxRec<double> *>(M.d_rec64),
+                           M.d_elem, M.d_bsub, P.d_coords, M.d_E, M.d_mat, reinterpret_cast<BoxRec<double> *>(M.d_rec64),
                            reinterpret_cast<BoxRec<float> *>(M.d_rec32));
         if (hipGetLastError() != hipSuccess) return 5;
     }
@@ -901,7 +931,16 @@ int mf_prepare(Problem &P) {
         destroy_graph(P);
     }
     if (s != M.n_slots) {
-        if (hipMemcpy(M.d_nwg, &s, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) return 5;
+        // a single problem: every slot (blocks, then the nodes-of-no-element pass); a batch: each
+        // sub-problem its blocks
+        std::vector<uint32_t> w0(P.n_sub, 0), nw(P.n_sub, s);
+        if (P.n_sub > 1) {
+            w0 = M.sub_b0;
+            nw = M.sub_nb;
+        }
+        if (hipMemcpy(M.d_wg0, w0.data(), 4 * w0.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(M.d_nwg, nw.data(), 4 * nw.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return 5;
         M.n_slots = s;
     }
     return 0;
@@ -912,7 +951,8 @@ template <typename T, bool PQ, bool DIR, bool BOXES>
 static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y) {
     const MatFree &M = P.mf;
     hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES>), dim3(M.n_blocks), dim3(MF_BLOCK), mf_lds_bytes(M.ls, sizeof(T)),
-                       P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad,
+                       P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.d_bsub, M.n_colors, M.ls, M.d_lnode_pad,
+                       M.d_lslot_pad,
                        M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat,
                        reinterpret_cast<const BoxRec<T> *>(sizeof(T) == 4 ? (void *)M.d_rec32 : (void *)M.d_rec64), x, xb,
                        z, y, reinterpret_cast<T *>(M.d_halo), P.d_ps, M.d_partials);
@@ -928,9 +968,8 @@ static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y) {
                            M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, P.d_ps, M.d_partials, M.n_blocks);
     // DIR: the iteration's vector update sums the shared nodes' partials itself (k_update_mf)
     if (M.n_bnodes && !DIR)
-        hipLaunchKernelGGL((k_mf_halo<T, PQ>), dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes,
-                           M.d_bnode, M.d_bptr, reinterpret_cast<const T *>(M.d_halo), y, P.d_ps, M.d_partials,
-                           M.n_slots);
+        hipLaunchKernelGGL(k_mf_halo<T>, dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes, M.d_bnode,
+                           M.d_bptr, reinterpret_cast<const T *>(M.d_halo), y);
 }
 
 // y = K x; with z (PQ only), y = K p for the updated direction p = z + beta s, s the stored

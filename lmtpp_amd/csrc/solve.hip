@@ -257,23 +257,24 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
     wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
 
-// The same after a matrix-free product with the fused direction update (one sub-problem), with the
-// product's shared-node pass folded in: every workgroup combines the element blocks' p.q partials
-// itself (the same fixed order as k_mf_halo's first workgroup, so every workgroup holds the same
-// alpha; workgroup 0 records it), and a node shared by several blocks takes q as the sum of its
-// partials halo[hbeg[n] .. hbeg[n + 1]) in block order, any other node the product's y.
+// The same after a matrix-free product with the fused direction update, with the product's
+// shared-node pass folded in: every workgroup combines its sub-problem's p.q partials itself (in
+// the fixed order of wg_combine, so every workgroup of a sub-problem holds the same alpha; the
+// sub-problem's first workgroup records it), and a node shared by several blocks takes q as the
+// sum of its partials halo[hbeg[n] .. hbeg[n + 1]) in block order, any other node the product's y.
 template <int PRE, typename T>
 __global__ __launch_bounds__(256) void k_update_mf(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
                                                    const T *__restrict__ q, const T *__restrict__ halo,
                                                    const uint32_t *__restrict__ hbeg, const double *__restrict__ mfpart,
-                                                   uint32_t n_mfslots, const T *__restrict__ Minv, T *__restrict__ z,
-                                                   WgTables wt, ProbScalars *S, double *partials) {
+                                                   const uint32_t *__restrict__ mf_w0, const uint32_t *__restrict__ mf_nw,
+                                                   const T *__restrict__ Minv, T *__restrict__ z, WgTables wt,
+                                                   ProbScalars *S, double *partials) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB || !selected(S[s], SEL_ACTIVE)) return;
     double o[1];
-    wg_combine<1, 0u>(mfpart, 0u, 0u, n_mfslots, o);
-    if (L == 0 && threadIdx.x == 0) finish_pq(S[s], o[0]);
+    wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
+    if (L == wt.sub_wg0[s] && threadIdx.x == 0) finish_pq(S[s], o[0]);
     if (!finite(o[0]) || !(o[0] > 0)) return;  // recorded by workgroup 0: the solve stops
     if (!(S[s].iter & 1)) p = p_alt;
     const T alpha = (T)(S[s].rho / o[0]);
@@ -509,12 +510,12 @@ static void finalize(Problem &P, int what, int sel = 0) {
 }
 
 // alpha = rho / p.q from the partials of the product just launched (the row-parallel formats
-// write one per row workgroup, the matrix-free passes one per pass workgroup; when the matrix-free
-// product has a shared-node pass, its first workgroup has done this already).
+// write one per row workgroup, the matrix-free passes one per element block and per workgroup of
+// the nodes-of-no-element pass, each sub-problem's in a range of its own).
 static void finalize_pq(Problem &P, int precision) {
     if (P.spmv.format[precision] != FMT_MF)
         finalize(P, FIN_PQ);
-    else if (!P.mf.n_bnodes)
+    else
         hipLaunchKernelGGL(k_finalize, dim3(P.n_sub), dim3(1024), 0, P.stream, (int)FIN_PQ, P.d_ps, P.mf.d_partials,
                            P.mf.n_slots, P.mf.d_wg0, P.mf.d_nwg, 0);
 }
@@ -776,8 +777,8 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
             // the product's shared-node pass and alpha are folded into the update
             const T *p_alt = reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
             hipLaunchKernelGGL((k_update_mf<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
-                               reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.n_slots,
-                               Minv, z, wt, P.d_ps, P.d_partials);
+                               reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
+                               P.mf.d_nwg, Minv, z, wt, P.d_ps, P.d_partials);
         }
     }
     if (!fused) {

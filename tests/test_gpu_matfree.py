@@ -175,8 +175,64 @@ def test_refused_where_it_cannot_serve():
     with pytest.raises(L.FemGpuError):
         f.set_spmv_config(fp64=MF, fp32=FULL)
     f.close()
-    meshes = [L.make_rect(L.HEXA, (3, 3, 3)), L.make_rect(L.HEXA, (4, 3, 3))]
+    # a batch whose meshes have a node of no element (its pass is served for a single problem only)
+    c, e = L.make_rect(L.HEXA, (3, 3, 3))
+    meshes = [(np.vstack([c, [[0.5, 0.5, 2.0]]]), e), L.make_rect(L.HEXA, (4, 3, 3))]
     b = L.Batch(L.HEXA, meshes)
     with pytest.raises(L.FemGpuError):
         b.set_spmv_config(fp64=MF, fp32=MF)
+    b.close()
+
+
+@pytest.mark.parametrize("precision,assembly", [(L.FP64, "staged"), (L.MIXED, "gather")])
+def test_batch_matches_individual_solves(oracle, precision, assembly):
+    """Matrix-free products in a batch: each sub-problem's blocks are its own (a block never
+    straddles two), so every sub-problem gives bitwise the solution and iteration count of the
+    same problem solved alone with the matrix-free product; lattices (box elements) and jittered
+    meshes of several sizes, moduli and Poisson ratios side by side, and the oracle's solution."""
+    from problems import clamp_and_traction
+    say("matrix-free batch", precision, assembly)
+    rng = np.random.default_rng(17)
+    items = []
+    for i in range(5):
+        div = tuple(int(v) for v in rng.integers(3, 8, size=3))
+        geometry = "lattice" if i % 2 == 0 else "jittered"
+        c, e, E = _mesh(div, 30 + i, geometry=geometry)
+        items.append((c, e, E, float(rng.uniform(0.2, 0.4)), rng.standard_normal(3)))
+    b = L.Batch(L.HEXA, [(c, e) for c, e, *_ in items])
+    singles = []
+    for i, (c, e, E, nu, tr) in enumerate(items):
+        faces, trs, dofs = clamp_and_traction(L.HEXA, c, e, tr)
+        b.set_material(i, E, nu)
+        b.set_traction(i, faces, trs)
+        b.add_constraints(i, dofs, None, 1e5)
+        f = L.Formulation(L.HEXA, c, e)
+        f.set_material(E, nu)
+        f.set_traction(faces, trs)
+        f.add_constraints(dofs, None, 1e5)
+        f.assemble()
+        f.set_spmv_config(fp64=MF, fp32=MF)
+        f.precision = precision
+        f.precond = L.JACOBI
+        f.set_result(np.zeros(3 * len(c)))
+        assert f.solve_system(rel_tol=1e-11)
+        singles.append((f.get_result(), f.last_stats.iters))
+        rp, col, vals, F, rc = oracle.assemble(L.HEXA, c, e, E, nu, faces=faces, face_trac=trs)
+        assert rc == 0
+        oracle.penalty(rp, col, vals, F, dofs, None, 1e5)
+        x, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-14)
+        assert rc == 0
+        tol = 1e-9 if precision == L.FP64 else 1e-8
+        assert np.linalg.norm(singles[-1][0] - x) <= tol * np.linalg.norm(x), i
+        f.close()
+    if assembly == "gather":   # the batch's K and F are bitwise the staged passes'
+        b.set_assembly_rows(A.ASSEMBLY_GATHER)
+    b.assemble()
+    b.set_spmv_config(fp64=MF, fp32=MF)
+    ok, ms = b.solve(rel_tol=1e-11, precision=precision, precond=L.JACOBI)
+    assert ok and ms > 0
+    for i, (u, it) in enumerate(singles):
+        say("  sub-problem", i, "iterations", b.last_stats[i].iters, it)
+        assert b.last_stats[i].iters == it
+        assert np.array_equal(b.get_result(i), u)
     b.close()

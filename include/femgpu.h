@@ -345,8 +345,8 @@ FEMGPU_API int femgpu_stream_read(size_t n, int reps, double *avg_ms);
 /* Block-SpMV storage formats: FEMGPU_SPMV_FULL stores both triangles (each row all its blocks);
  * FEMGPU_SPMV_SYM stores each row's upper blocks and reads the lower ones transposed from their
  * owner row, as MatWithTinyBlocks::partial_mul uses each stored block twice
- * (containers/MatWithTinyBlocks.h:303-378); FEMGPU_SPMV_MATRIX_FREE (Hexa problems with one
- * sub-problem) evaluates K x element by element from the mesh with the element loop's quadrature
+ * (containers/MatWithTinyBlocks.h:303-378); FEMGPU_SPMV_MATRIX_FREE (Hexa and Tetra_10 problems)
+ * evaluates K x element by element from the mesh with the element loop's quadrature
  * (formulation.h:626-647), the material and penalties of the last matrix assembly, colour by colour
  * without atomics; residual products of that precision keep the full storage with the configured
  * unroll and loads. */

@@ -1300,7 +1300,7 @@ int femgpu_set_spmv_config(femgpu_problem *h, const int format[2], const int unr
             (format[i] == FMT_MF && !mf_eligible(h->P)))
             return fail(FEMGPU_BAD_INPUT, "set_spmv_config: format 0/1/2, unroll 1..3, loads 0..2 (1, 2 not with "
                                           "format 1; 2 needs every column offset in 16 bits; format 2 needs a Hexa "
-                                          "problem)");
+                                          "or Tetra_10 problem)");
     }
     Problem &P = h->P;
     if (format[0] == FMT_MF || format[1] == FMT_MF) {

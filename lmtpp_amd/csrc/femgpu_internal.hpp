@@ -53,7 +53,7 @@ int node_order_default();
 // Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format, SELL slots per loop
 // trip (1, 2 or 3) and how the full format's matrix is loaded: plain loads, non-temporal loads,
 // or non-temporal loads with 16-bit column offsets (c - r; only when every offset fits).
-// FMT_MF: the product is evaluated element by element from the mesh (Hexa, one sub-problem; see
+// FMT_MF: the product is evaluated element by element from the mesh (Hexa and Tetra_10; see
 // MatFree); the residual products of that precision keep the full storage with its unroll and
 // loads.
 enum { FMT_FULL = 0, FMT_SYM = 1, FMT_MF = 2 };

@@ -1,4 +1,4 @@
-// Matrix-free block product y = K x for Hexa problems on gfx950.
+// Matrix-free block product y = K x for Hexa and Tetra_10 problems on gfx950.
 //
 // The assembled FP32 product reads about 1 KB of matrix per node (27 blocks of 36 bytes) and runs
 // at the HBM read bound.  The same product evaluated element by element reads a few tens of bytes
@@ -25,6 +25,13 @@
 // order: the product is deterministic.  The penalty diagonal of the constraints is added per
 // block, and nodes of no element get it in a pass of their own.  In the CG iteration the direction
 // update p = z + beta p is fused into these passes.
+//
+// Tetra_10 elements (10 nodes, 4 Gauss points, no product structure to factor) take a per-slot
+// record built on the device after each material snapshot: J^-1 and w detJ at each Gauss point
+// and the Hooke coefficients, 44 values (176 B FP32), so the element loop reads neither the
+// node coordinates nor the modulus; the reference gradients at the fixed Gauss points are
+// compile-time constants.  One block then reads ~40 B per element plus its staged nodes, against
+// ~39 stored 3x3 blocks per node for the assembled product.
 #include "device_common.hpp"
 #include "generated/elasticity_iso.inc"
 
@@ -357,6 +364,124 @@ __global__ __launch_bounds__(256) void k_mf_boxrec(size_t n, const uint8_t *__re
     box_coefs<float>(X0, X6, E[elem[s]], nu, r32[s]);
 }
 
+// Tetra_10 slot record: per Gauss point q of the element loop's 4-point rule
+// (element_Tetra_10.h:12-15), ji[q][3 m + j] = d xi_j / d x_m (J^-1 from the adjugate, J_ji =
+// d x_i / d xi_j from all 10 nodes, formal_lf/element.py:195-205) and c[q] = w_q detJ_q (signed,
+// formal_lf/formulation.py:553), then the Hooke coefficients (H11 - H12, H12, H44 / 2).
+template <typename T>
+struct TetRec {
+    T ji[4][9];
+    T c[4];
+    T h[3];
+    T pad;
+};
+
+template <typename T>
+__device__ inline void tet_coefs(const double (&X)[10][3], double E, double nu, TetRec<T> &r) {
+    double Hd[3];
+    gen::hooke(E, nu, Hd);
+    r.h[0] = (T)(Hd[0] - Hd[1]);
+    r.h[1] = (T)Hd[1];
+    r.h[2] = (T)(0.5 * Hd[2]);
+    r.pad = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double xi[3], w;
+        gen::vol<TETRA10>::gauss(q, xi, w);
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int a = 0; a < 10; ++a) {
+            double d[3];
+            (void)gen::vol<TETRA10>::shape(a, xi, d);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) J[j][i] += d[j] * X[a][i];
+        }
+        double A[3][3];
+        const double det = adjugate(J, A);  // A[m][j] = detJ d xi_j / d x_m
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) r.ji[q][3 * m + j] = (T)(A[m][j] / det);
+        r.c[q] = (T)(w * det);
+    }
+}
+
+// The records of every Tetra_10 slot, both precisions (after each snapshot of the material).
+__global__ __launch_bounds__(256) void k_mf_tetrec(size_t n, const uint8_t *__restrict__ color,
+                                                   const uint32_t *__restrict__ conn, const uint32_t *__restrict__ elem,
+                                                   const uint32_t *__restrict__ bsub,
+                                                   const double *__restrict__ coords, const double *__restrict__ E,
+                                                   const SubMaterial *__restrict__ mat, TetRec<double> *__restrict__ r64,
+                                                   TetRec<float> *__restrict__ r32) {
+    const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n || color[s] == MF_EMPTY) return;
+    double X[10][3];
+    for (int a = 0; a < 10; ++a)
+        for (int d = 0; d < 3; ++d) X[a][d] = coords[3 * (size_t)conn[10 * s + a] + d];
+    const double nu = mat[bsub[s / MF_BLOCK]].nu;
+    tet_coefs<double>(X, E[elem[s]], nu, r64[s]);
+    tet_coefs<float>(X, E[elem[s]], nu, r32[s]);
+}
+
+// f[a][c] = (K_e u_e) of one Tetra_10 from its record: per Gauss point, G = grad u (reference
+// gradients of the generated shape functions, constants at the fixed points, times J^-1),
+// sigma = H eps(G) scaled by w detJ, f_a += sigma grad N_a.
+template <typename T>
+__device__ inline void tet10_apply(const TetRec<T> &r, const T (&u)[10][3], T (&f)[10][3]) {
+#pragma unroll
+    for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[a][c] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double xi[3], w;
+        gen::vol<TETRA10>::gauss(q, xi, w);
+        T dr[10][3];
+#pragma unroll
+        for (int a = 0; a < 10; ++a) {
+            double d[3];
+            (void)gen::vol<TETRA10>::shape(a, xi, d);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dr[a][j] = (T)d[j];
+        }
+        T Gr[3][3];  // d u_c / d xi_j
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                T v = 0;
+#pragma unroll
+                for (int a = 0; a < 10; ++a) v += u[a][c] * dr[a][j];
+                Gr[c][j] = v;
+            }
+        T G[3][3];  // d u_c / d x_m
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                G[c][m] = Gr[c][0] * r.ji[q][3 * m] + Gr[c][1] * r.ji[q][3 * m + 1] + Gr[c][2] * r.ji[q][3 * m + 2];
+        const T kq = r.c[q], tr = G[0][0] + G[1][1] + G[2][2], ks = kq * r.h[2];
+        T sg[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sg[c][c] = kq * (r.h[0] * G[c][c] + r.h[1] * tr);
+        sg[0][1] = sg[1][0] = ks * (G[0][1] + G[1][0]);
+        sg[0][2] = sg[2][0] = ks * (G[0][2] + G[2][0]);
+        sg[1][2] = sg[2][1] = ks * (G[1][2] + G[2][1]);
+        T Sx[3][3];  // sum_m sigma_cm d xi_j / d x_m
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Sx[c][j] = sg[c][0] * r.ji[q][j] + sg[c][1] * r.ji[q][3 + j] + sg[c][2] * r.ji[q][6 + j];
+#pragma unroll
+        for (int a = 0; a < 10; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[a][c] += Sx[c][0] * dr[a][0] + Sx[c][1] * dr[a][1] + Sx[c][2] * dr[a][2];
+    }
+}
+
 // Dynamic LDS of k_mf_block for blocks of up to max_local nodes and values of vsize bytes.
 inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
     return (size_t)std::max<uint32_t>(1, max_local) * 6 * vsize;
@@ -376,7 +501,7 @@ inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
 // buffer at the nodes it owns and at the shared nodes it holds first (lslot bit 31), so every node
 // is written once and no block reads a value another block has already replaced.
 // LDS: per local node 3 input values and 3 accumulators.
-template <typename T, bool PQ, bool DIR, bool BOXES>
+template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA>
 __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   const uint32_t *__restrict__ bsub,
@@ -386,10 +511,11 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
                                                   const uint32_t *__restrict__ psrc, const double *__restrict__ fpen,
                                                   const double *__restrict__ coords, const double *__restrict__ E,
                                                   const SubMaterial *__restrict__ mat,
-                                                  const BoxRec<T> *__restrict__ rec, const T *x, T *xb,
+                                                  const void *__restrict__ recv, const T *x, T *xb,
                                                   const T *__restrict__ z, T *__restrict__ y, T *__restrict__ halo,
                                                   const ProbScalars *S, double *partials) {
     extern __shared__ double mf_lds[];
+    constexpr int NPE = K == TETRA10 ? 10 : 8;
     const unsigned b = xcd_slab();
     const uint32_t sb = bsub[b];  // the block's sub-problem (NO_SUB: an empty padding block)
     if (sb == NO_SUB) {
@@ -438,17 +564,22 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
     const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
     const bool box = BOXES || (cbits != MF_EMPTY && (cbits & MF_BOX));
     double pq[1] = {0.0};
-    T f[8][3];
-    uint32_t lo[8];
+    T f[NPE][3];
+    uint32_t lo[NPE];
     double X[8][3];  // corners of a general element
     double Hd[3] = {0, 0, 0};
     BoxRec<T> br;    // a box: edge lengths and Hooke coefficients
     if (col != MF_EMPTY) {
-        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[slot];
+        // block-local node indices: 8 u16 per Hexa slot, 16 (10 used) per Tetra_10 slot
+        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[K == TETRA10 ? 2 * slot : slot];
         lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
         lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
-        if (box) {
-            br = rec[slot];
+        if constexpr (K == TETRA10) {
+            const uint32_t l89 = reinterpret_cast<const uint32_t *>(loc)[8 * slot + 4];
+            lo[NPE - 2] = l89 & 0xffffu;
+            lo[NPE - 1] = l89 >> 16;
+        } else if (box) {
+            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
         } else {
             const uint4 ca = reinterpret_cast<const uint4 *>(conn)[2 * slot];
             const uint4 cb = reinterpret_cast<const uint4 *>(conn)[2 * slot + 1];
@@ -462,28 +593,32 @@ __global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ c
     }
     __syncthreads();
     if (col != MF_EMPTY) {
-        T u[8][3];
+        T u[NPE][3];
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int a = 0; a < NPE; ++a)
 #pragma unroll
             for (int c = 0; c < 3; ++c) u[a][c] = Ps[3 * lo[a] + c];
-        if (box) hexa_apply_box<T>(br.h, u, br.c[0], br.c[1], br.c[2], f);
-        else hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
+        if constexpr (K == TETRA10) {
+            tet10_apply<T>(reinterpret_cast<const TetRec<T> *>(recv)[slot], u, f);
+        } else {
+            if (box) hexa_apply_box<T>(br.h, u, br.c[0], br.c[1], br.c[2], f);
+            else hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
+        }
     }
     __syncthreads();
     // no two lanes of a sub-colour touch one entry, so the order of every sum is the sub-colour
     // order; plain read-add-write (LDS float atomics measured 2.6x slower here in FP32)
-    // (an element's 8 nodes are distinct, so all 24 entries are read before any is written: one
-    // LDS round trip per sub-colour instead of eight dependent ones)
+    // (an element's nodes are distinct, so all its entries are read before any is written: one
+    // LDS round trip per sub-colour instead of one per node)
     for (uint32_t c = 0; c < n_colors; ++c) {
         if (col == c) {
-            T cur[8][3];
+            T cur[NPE][3];
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
+            for (int a = 0; a < NPE; ++a)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) cur[a][d] = acc[3 * lo[a] + d];
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
+            for (int a = 0; a < NPE; ++a)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) acc[3 * lo[a] + d] = cur[a][d] + f[a][d];
         }
@@ -579,7 +714,9 @@ __global__ void k_mf_pen(uint32_t n_cdof, const uint32_t *__restrict__ cptr, con
 // ---------------------------------------------------------------------------------------------
 
 bool mf_eligible(const Problem &P) {
-    return P.kind == HEXA && !P.from_bsr && P.n_elem > 0 && P.conn_h.size() == 8 * (size_t)P.n_elem;
+    if (P.from_bsr || P.n_elem == 0) return false;
+    return (P.kind == HEXA && P.conn_h.size() == 8 * (size_t)P.n_elem) ||
+           (P.kind == TETRA10 && P.conn_h.size() == 10 * (size_t)P.n_elem);
 }
 
 void mf_free(MatFree &M) {
@@ -637,6 +774,9 @@ int mf_build(Problem &P) {
     if (!mf_eligible(P)) return 1;
     const uint32_t ne = P.n_elem, nn = P.n_nodes;
     const uint32_t *conn = P.conn_h.data();
+    // nodes per element, and the stride of the block-local index lists (16 u16 per Tetra_10 slot,
+    // so a slot's list is two aligned 16-byte loads)
+    const int npe = P.kind == TETRA10 ? 10 : 8, lst = P.kind == TETRA10 ? 16 : 8;
     std::vector<double> X(3 * (size_t)nn);
     if (hipMemcpy(X.data(), P.d_coords, X.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return 5;
     // element order, per sub-problem (a block never straddles two): Morton code of the element
@@ -664,8 +804,8 @@ int mf_build(Problem &P) {
             uint32_t q[3];
             for (int d = 0; d < 3; ++d) {
                 double c = 0;
-                for (int a = 0; a < 8; ++a) c += X[3 * (size_t)conn[8 * (size_t)e + a] + d];
-                q[d] = (uint32_t)std::min(2097151.0, std::max(0.0, std::floor((c / 8 - lo[d]) / h)));
+                for (int a = 0; a < npe; ++a) c += X[3 * (size_t)conn[npe * (size_t)e + a] + d];
+                q[d] = (uint32_t)std::min(2097151.0, std::max(0.0, std::floor((c / npe - lo[d]) / h)));
             }
             key[e] = {morton3(q[0], q[1], q[2]), e};
         }
@@ -685,19 +825,19 @@ int mf_build(Problem &P) {
         bcount.push_back(0);
         bsub.push_back(NO_SUB);
     }
-    std::vector<uint32_t> sconn(8 * ns, 0), selem(ns, 0), lptr(nb + 1, 0), lnode, deg(nn, 0), stamp(nn, 0xffffffffu),
+    std::vector<uint32_t> sconn(npe * ns, 0), selem(ns, 0), lptr(nb + 1, 0), lnode, deg(nn, 0), stamp(nn, 0xffffffffu),
         lid(nn, 0), in_blk;
     M.node_blk.assign(nn, 0xffffffffu);
     M.node_loc.assign(nn, 0);
-    std::vector<uint16_t> sloc(8 * ns, 0);
+    std::vector<uint16_t> sloc(lst * ns, 0);
     std::vector<uint8_t> scol(ns, 0xffu);
-    for (size_t s = 0; s < 8 * (size_t)ne; ++s) ++deg[conn[s]];
+    for (size_t s = 0; s < npe * (size_t)ne; ++s) ++deg[conn[s]];
     uint32_t n_colors = 0, max_local = 0;
     // element centres, for the sweep order of the sub-colouring
     std::vector<double> cen(3 * (size_t)ne, 0.0);
     for (uint32_t e = 0; e < ne; ++e)
-        for (int a = 0; a < 8; ++a)
-            for (int d = 0; d < 3; ++d) cen[3 * (size_t)e + d] += X[3 * (size_t)conn[8 * (size_t)e + a] + d] / 8;
+        for (int a = 0; a < npe; ++a)
+            for (int d = 0; d < 3; ++d) cen[3 * (size_t)e + d] += X[3 * (size_t)conn[npe * (size_t)e + a] + d] / npe;
     std::vector<uint32_t> blk;
     for (uint32_t b = 0; b < nb; ++b) {
         lptr[b] = (uint32_t)lnode.size();
@@ -715,8 +855,8 @@ int mf_build(Problem &P) {
             const size_t i = (size_t)b * MF_BLOCK + k;
             const uint32_t e = blk[k];
             uint32_t used = 0;
-            for (int a = 0; a < 8; ++a) {
-                const uint32_t n = conn[8 * (size_t)e + a];
+            for (int a = 0; a < npe; ++a) {
+                const uint32_t n = conn[npe * (size_t)e + a];
                 if (stamp[n] != b) {
                     stamp[n] = b;
                     lid[n] = (uint32_t)(lnode.size() - lptr[b]);
@@ -728,15 +868,15 @@ int mf_build(Problem &P) {
                         M.node_loc[n] = lid[n];
                     }
                 }
-                sconn[8 * i + a] = n;
-                sloc[8 * i + a] = (uint16_t)lid[n];
+                sconn[npe * i + a] = n;
+                sloc[lst * i + a] = (uint16_t)lid[n];
                 ++in_blk[lptr[b] + lid[n]];
                 used |= mask[lid[n]];
             }
             if (used == 0xffffffffu) return 1;  // more than 32 sub-colours: not served
             const uint32_t c = (uint32_t)__builtin_ctz(~used);
-            for (int a = 0; a < 8; ++a) mask[lid[conn[8 * (size_t)e + a]]] |= 1u << c;
-            scol[i] = (uint8_t)(c | (is_box(X.data(), conn + 8 * (size_t)e) ? MF_BOX : 0u));
+            for (int a = 0; a < npe; ++a) mask[lid[conn[npe * (size_t)e + a]]] |= 1u << c;
+            scol[i] = (uint8_t)(c | (P.kind == HEXA && is_box(X.data(), conn + 8 * (size_t)e) ? MF_BOX : 0u));
             selem[i] = e;
             n_colors = std::max(n_colors, c + 1);
         }
@@ -750,15 +890,15 @@ int mf_build(Problem &P) {
             std::stable_sort(ord.begin(), ord.end(), [&](uint32_t p, uint32_t q) {
                 return (scol[i0 + p] & MF_COLOR_BITS) < (scol[i0 + q] & MF_COLOR_BITS);
             });
-            const std::vector<uint32_t> c0(sconn.begin() + 8 * i0, sconn.begin() + 8 * (i0 + nk)),
+            const std::vector<uint32_t> c0(sconn.begin() + npe * i0, sconn.begin() + npe * (i0 + nk)),
                 e0(selem.begin() + i0, selem.begin() + i0 + nk);
-            const std::vector<uint16_t> l0(sloc.begin() + 8 * i0, sloc.begin() + 8 * (i0 + nk));
+            const std::vector<uint16_t> l0(sloc.begin() + lst * i0, sloc.begin() + lst * (i0 + nk));
             const std::vector<uint8_t> s0(scol.begin() + i0, scol.begin() + i0 + nk);
             for (size_t k = 0; k < nk; ++k) {
                 const size_t src = ord[k];
-                for (int a = 0; a < 8; ++a) {
-                    sconn[8 * (i0 + k) + a] = c0[8 * src + a];
-                    sloc[8 * (i0 + k) + a] = l0[8 * src + a];
+                for (int a = 0; a < npe; ++a) {
+                    sconn[npe * (i0 + k) + a] = c0[npe * src + a];
+                    sloc[lst * (i0 + k) + a] = l0[lst * src + a];
                 }
                 scol[i0 + k] = s0[src];
                 selem[i0 + k] = e0[src];
@@ -808,10 +948,10 @@ int mf_build(Problem &P) {
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
         !upload_vec(&M.d_bsub, bsub) || !upload_vec(&M.d_wg0, std::vector<uint32_t>(P.n_sub, 0)) ||
         !upload_vec(&M.d_nwg, std::vector<uint32_t>(P.n_sub, 0)) ||
-        hipMalloc(&M.d_rec64, sizeof(BoxRec<double>) * ns) != hipSuccess ||
+        hipMalloc(&M.d_rec64, (P.kind == TETRA10 ? sizeof(TetRec<double>) : sizeof(BoxRec<double>)) * ns) != hipSuccess ||
         hipMalloc(&M.d_pb, sizeof(double) * 3 * (size_t)nn) != hipSuccess ||
         hipMalloc(&M.d_pb32, sizeof(float) * 3 * (size_t)nn) != hipSuccess ||
-        hipMalloc(&M.d_rec32, sizeof(BoxRec<float>) * ns) != hipSuccess ||
+        hipMalloc(&M.d_rec32, (P.kind == TETRA10 ? sizeof(TetRec<float>) : sizeof(BoxRec<float>)) * ns) != hipSuccess ||
         hipMalloc(&M.d_halo, sizeof(double) * 3 * std::max<size_t>(1, bptr.back())) != hipSuccess) {
         mf_free(M);
         return 5;
@@ -912,7 +1052,13 @@ int mf_prepare(Problem &P) {
     if (M.n_fix && !M.d_fpen) return 5;
     rc = mf_penalties(P);
     if (rc) return rc;
-    if (!M.rec_valid && M.n_boxes) {  // stream order: after the snapshot's copies
+    if (!M.rec_valid && P.kind == TETRA10) {  // stream order: after the snapshot's copies
+        const size_t ns = (size_t)M.n_blocks * MF_BLOCK;
+        hipLaunchKernelGGL(k_mf_tetrec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, P.stream, ns, M.d_color, M.d_conn,
+                           M.d_elem, M.d_bsub, P.d_coords, M.d_E, M.d_mat, reinterpret_cast<TetRec<double> *>(M.d_rec64),
+                           reinterpret_cast<TetRec<float> *>(M.d_rec32));
+        if (hipGetLastError() != hipSuccess) return 5;
+    } else if (!M.rec_valid && M.n_boxes) {  // stream order: after the snapshot's copies
         const size_t ns = (size_t)M.n_blocks * MF_BLOCK;
         hipLaunchKernelGGL(k_mf_boxrec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, P.stream, ns, M.d_color, M.d_conn,
                            M.d_elem, M.d_bsub, P.d_coords, M.d_E, M.d_mat, reinterpret_cast<BoxRec<double> *>(M.d_rec64),
@@ -946,16 +1092,21 @@ int mf_prepare(Problem &P) {
     return 0;
 }
 
-// Element-block kernel of one variant (BOXES: every element of the mesh is an axis-aligned box).
+// Element-block kernel of one variant (BOXES: every element of the mesh is an axis-aligned box;
+// Tetra_10 problems take the Tetra_10 kernel).
 template <typename T, bool PQ, bool DIR, bool BOXES>
 static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y) {
     const MatFree &M = P.mf;
-    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES>), dim3(M.n_blocks), dim3(MF_BLOCK), mf_lds_bytes(M.ls, sizeof(T)),
-                       P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.d_bsub, M.n_colors, M.ls, M.d_lnode_pad,
-                       M.d_lslot_pad,
-                       M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen, P.d_coords, M.d_E, M.d_mat,
-                       reinterpret_cast<const BoxRec<T> *>(sizeof(T) == 4 ? (void *)M.d_rec32 : (void *)M.d_rec64), x, xb,
-                       z, y, reinterpret_cast<T *>(M.d_halo), P.d_ps, M.d_partials);
+    const void *rec = sizeof(T) == 4 ? M.d_rec32 : M.d_rec64;
+#define FEMGPU_MF_BLOCK(KIND)                                                                                         \
+    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND>), dim3(M.n_blocks), dim3(MF_BLOCK),        \
+                       mf_lds_bytes(M.ls, sizeof(T)), P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.d_bsub,      \
+                       M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
+                       P.d_coords, M.d_E, M.d_mat, rec, x, xb, z, y, reinterpret_cast<T *>(M.d_halo), P.d_ps,          \
+                       M.d_partials)
+    if (P.kind == TETRA10) FEMGPU_MF_BLOCK(TETRA10);
+    else FEMGPU_MF_BLOCK(HEXA);
+#undef FEMGPU_MF_BLOCK
 }
 
 template <typename T, bool PQ, bool DIR>

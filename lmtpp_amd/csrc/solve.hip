@@ -658,9 +658,9 @@ double time_spmv(Problem &P, int precision, int reps) {
 // Times every SpMV variant (storage format x SELL slots per loop trip) on this matrix, for each
 // precision (10 launches after 2 warm-up launches), and keeps the fastest.  The symmetric storage
 // holds a second copy of the values, so it is kept only when it beats the full storage by more
-// than 3%; storage no precision picked is released.  The matrix-free product (Hexa problems with
-// one sub-problem) is kept when it beats the stored formats; the full storage's fastest variant
-// then serves the residual products.
+// than 3%; storage no precision picked is released.  The matrix-free product (Hexa and Tetra_10
+// problems) is kept when it beats the stored formats; the full storage's fastest variant then
+// serves the residual products.
 int autotune_spmv(Problem &P) {
     for (int pr = 0; pr < 2; ++pr) {
         double best[2] = {1e30, 1e30};

@@ -311,7 +311,7 @@ class Formulation:
         return y
 
     def matrix_free_info(self):
-        """Layout of the matrix-free product (Hexa, one sub-problem): element blocks, sub-colours,
+        """Layout of the matrix-free product (Hexa and Tetra_10): element blocks, sub-colours,
         block-local node entries, shared nodes, their partial slots, nodes of no element, elements
         that are axis-aligned boxes."""
         info = (C.c_uint64 * 7)()

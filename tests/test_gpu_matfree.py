@@ -1,4 +1,5 @@
-"""Matrix-free Hexa product (femgpu_set_spmv_config FEMGPU_SPMV_MATRIX_FREE) through the C ABI.
+"""Matrix-free Hexa and Tetra_10 products (femgpu_set_spmv_config FEMGPU_SPMV_MATRIX_FREE) through
+the C ABI.
 
 The product K x is evaluated element by element from the mesh instead of read from the stored
 matrix.  Pinned here against the oracle's independently assembled K (oracle.assemble + penalty,
@@ -7,11 +8,12 @@ whose element kernels are pinned in test_oracle_elem.py):
   (the stored FP32 copy is held to 1e-6 beside it);
 * the penalty diagonal, a node of no element (its row is the penalty alone) and the internal node
   order are covered, on general (jittered, rotated) elements, on axis-aligned boxes (the lattice
-  and a rectilinear grid: the box-only kernel) and on a mesh mixing both in every block;
+  and a rectilinear grid: the box-only kernel) and on a mesh mixing both in every block; Tetra_10
+  elements with straight and with curved edges (mid-edge nodes off the chords);
 * with assume_constant_matrix the product keeps the material of the last matrix assembly;
 * solutions with the product in the loop match the oracle's solve to 1e-10 (FP64) and 1e-8
   (MIXED), with block Jacobi and block IC(0), and repeat bitwise;
-* problems it cannot serve (Tetra, batches) are refused.
+* problems it cannot serve (Tetra, batches with a node of no element) are refused.
 """
 import sys
 
@@ -20,7 +22,7 @@ import pytest
 
 import lmtpp_amd as L
 from lmtpp_amd import _capi as A
-from problems import clamp_and_traction, jitter_interior, lognormal_E
+from problems import clamp_and_traction, jitter_interior, lognormal_E, tetra10_jittered
 
 pytestmark = pytest.mark.gpu
 
@@ -60,18 +62,35 @@ def _mesh(div, seed, isolated=False, shuffle=False, geometry="jittered"):
     return c, e, lognormal_E(len(e), 0.4, seed)
 
 
-def _system(oracle, c, e, E, isolated, order=A.NODE_ORDER_NATURAL, nu=0.3):
-    faces, tr, dofs = clamp_and_traction(L.HEXA, c, e, (0.0, 0.4, -1.0))
+def _tet10(cells, seed, curved=False, shuffle=False):
+    """Tetra_10 on cells^3 cubes (corner nodes jittered); curved: interior mid-edge nodes moved
+    off the chords by up to 5% of a cube edge, so J varies over each element."""
+    c, e = tetra10_jittered(cells, seed=seed)
+    if curved:
+        mid = np.unique(e[:, 4:])
+        mid = mid[np.all((c[mid] > 1e-9) & (c[mid] < 1 - 1e-9), axis=1)]
+        c = c.copy()
+        c[mid] += np.random.default_rng(seed + 1).uniform(-0.05, 0.05, (len(mid), 3)) / cells
+    if shuffle:
+        perm = np.random.default_rng(seed).permutation(len(c))
+        new_of = np.empty(len(c), np.int64)
+        new_of[perm] = np.arange(len(c))
+        c, e = c[perm].copy(), new_of[e].astype(np.uint32)
+    return c, e, lognormal_E(len(e), 0.4, seed)
+
+
+def _system(oracle, c, e, E, isolated, order=A.NODE_ORDER_NATURAL, nu=0.3, kind=L.HEXA):
+    faces, tr, dofs = clamp_and_traction(kind, c, e, (0.0, 0.4, -1.0))
     assert len(dofs) > 0
     if isolated:
         dofs = np.concatenate([dofs, 3 * (len(c) - 1) + np.arange(3)]).astype(np.uint32)
     with A.node_order_ctx(order):
-        f = L.Formulation(L.HEXA, c, e)
+        f = L.Formulation(kind, c, e)
     f.set_material(E, nu, (0.1, -0.2, 0.3))
     f.set_traction(faces, tr)
     f.add_constraints(dofs, None, 1e5)
     f.assemble()
-    rp, col, vals, F, rc = oracle.assemble(L.HEXA, c, e, E, nu, bf=(0.1, -0.2, 0.3), faces=faces, face_trac=tr)
+    rp, col, vals, F, rc = oracle.assemble(kind, c, e, E, nu, bf=(0.1, -0.2, 0.3), faces=faces, face_trac=tr)
     assert rc == 0
     oracle.penalty(rp, col, vals, F, dofs, None, 1e5)
     return f, (rp, col, vals, F)
@@ -115,6 +134,56 @@ def test_product_matches_oracle_matrix(oracle, case):
     # repeated products are bitwise equal (fixed colour order, no atomics)
     assert np.array_equal(f.apply(x, A.FP64), f.apply(x, A.FP64))
     assert np.array_equal(f.apply(x, A.MIXED), f.apply(x, A.MIXED))
+    f.close()
+
+
+@pytest.mark.parametrize("case", ["straight", "curved", "node_order"])
+def test_tetra10_product_matches_oracle_matrix(oracle, case):
+    say("matrix-free Tetra_10 product", case)
+    c, e, E = _tet10(5, 21, curved=case == "curved", shuffle=case == "node_order")
+    order = A.NODE_ORDER_RCM if case == "node_order" else A.NODE_ORDER_NATURAL
+    f, (rp, col, vals, F) = _system(oracle, c, e, E, False, order, kind=L.TETRA10)
+    x = np.random.default_rng(7).standard_normal(3 * len(c))
+    ref = oracle.spmv(rp, col, vals, x)
+    scale = oracle.spmv(rp, col, np.abs(vals), np.abs(x))
+    f.set_spmv_config(fp64=MF, fp32=MF)
+    info = f.matrix_free_info()
+    assert info["box_elements"] == 0
+    e_mf64 = _rel_err(f.apply(x, A.FP64), ref, scale)
+    e_mf32 = _rel_err(f.apply(x, A.MIXED), ref, scale)
+    say("  errors mf64 %.2e mf32 %.2e" % (e_mf64, e_mf32), info)
+    assert e_mf64 <= 1e-12, e_mf64
+    assert e_mf32 <= 2e-5, e_mf32
+    assert np.array_equal(f.apply(x, A.FP64), f.apply(x, A.FP64))
+    assert np.array_equal(f.apply(x, A.MIXED), f.apply(x, A.MIXED))
+    f.close()
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_tetra10_solutions_match_oracle(oracle, curved):
+    say("matrix-free Tetra_10 solve", curved)
+    c, e, E = _tet10(5, 23, curved=curved)
+    f, (rp, col, vals, F) = _system(oracle, c, e, E, False, kind=L.TETRA10)
+    x, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-14, block=True)
+    assert rc == 0
+    f.precond = L.BLOCK_JACOBI
+    iters = {}
+    for cfg in (FULL, MF):
+        f.set_spmv_config(fp64=cfg, fp32=cfg)
+        for prec, tol in ((L.FP64, 1e-10), (L.MIXED, 1e-8)):
+            f.precision = prec
+            f.set_result(np.zeros_like(F))
+            assert f.solve_system(rel_tol=1e-12), (cfg, prec)
+            u = f.get_result()
+            assert np.linalg.norm(u - x) <= tol * np.linalg.norm(x), (cfg, prec)
+            iters[(cfg[0], prec)] = f.last_stats.iters
+            if cfg == MF:
+                f.set_result(np.zeros_like(F))
+                assert f.solve_system(rel_tol=1e-12)
+                assert np.array_equal(f.get_result(), u)
+    say("  iterations", iters)
+    a, b = iters[(A.SPMV_FULL, L.FP64)], iters[(A.SPMV_MATRIX_FREE, L.FP64)]
+    assert abs(a - b) <= 2, (a, b)
     f.close()
 
 
@@ -184,29 +253,32 @@ def test_refused_where_it_cannot_serve():
     b.close()
 
 
-@pytest.mark.parametrize("precision,assembly", [(L.FP64, "staged"), (L.MIXED, "gather")])
-def test_batch_matches_individual_solves(oracle, precision, assembly):
+@pytest.mark.parametrize("precision,assembly,kind", [(L.FP64, "staged", L.HEXA), (L.MIXED, "gather", L.HEXA),
+                                                    (L.MIXED, "staged", L.TETRA10)])
+def test_batch_matches_individual_solves(oracle, precision, assembly, kind):
     """Matrix-free products in a batch: each sub-problem's blocks are its own (a block never
     straddles two), so every sub-problem gives bitwise the solution and iteration count of the
     same problem solved alone with the matrix-free product; lattices (box elements) and jittered
     meshes of several sizes, moduli and Poisson ratios side by side, and the oracle's solution."""
     from problems import clamp_and_traction
-    say("matrix-free batch", precision, assembly)
+    say("matrix-free batch", precision, assembly, kind)
     rng = np.random.default_rng(17)
     items = []
     for i in range(5):
         div = tuple(int(v) for v in rng.integers(3, 8, size=3))
-        geometry = "lattice" if i % 2 == 0 else "jittered"
-        c, e, E = _mesh(div, 30 + i, geometry=geometry)
+        if kind == L.TETRA10:
+            c, e, E = _tet10(2 + i % 3, 30 + i, curved=i % 2 == 1)
+        else:
+            c, e, E = _mesh(div, 30 + i, geometry="lattice" if i % 2 == 0 else "jittered")
         items.append((c, e, E, float(rng.uniform(0.2, 0.4)), rng.standard_normal(3)))
-    b = L.Batch(L.HEXA, [(c, e) for c, e, *_ in items])
+    b = L.Batch(kind, [(c, e) for c, e, *_ in items])
     singles = []
     for i, (c, e, E, nu, tr) in enumerate(items):
-        faces, trs, dofs = clamp_and_traction(L.HEXA, c, e, tr)
+        faces, trs, dofs = clamp_and_traction(kind, c, e, tr)
         b.set_material(i, E, nu)
         b.set_traction(i, faces, trs)
         b.add_constraints(i, dofs, None, 1e5)
-        f = L.Formulation(L.HEXA, c, e)
+        f = L.Formulation(kind, c, e)
         f.set_material(E, nu)
         f.set_traction(faces, trs)
         f.add_constraints(dofs, None, 1e5)
@@ -217,7 +289,7 @@ def test_batch_matches_individual_solves(oracle, precision, assembly):
         f.set_result(np.zeros(3 * len(c)))
         assert f.solve_system(rel_tol=1e-11)
         singles.append((f.get_result(), f.last_stats.iters))
-        rp, col, vals, F, rc = oracle.assemble(L.HEXA, c, e, E, nu, faces=faces, face_trac=trs)
+        rp, col, vals, F, rc = oracle.assemble(kind, c, e, E, nu, faces=faces, face_trac=trs)
         assert rc == 0
         oracle.penalty(rp, col, vals, F, dofs, None, 1e5)
         x, _, _, rc = oracle.pcg_jacobi(rp, col, vals, F, rel_tol=1e-14)

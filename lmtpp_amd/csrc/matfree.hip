@@ -30,8 +30,9 @@
 // record built on the device after each material snapshot: J^-1 and w detJ at each Gauss point
 // and the Hooke coefficients, 44 values (176 B FP32), so the element loop reads neither the
 // node coordinates nor the modulus; the reference gradients at the fixed Gauss points are
-// compile-time constants.  One block then reads ~40 B per element plus its staged nodes, against
-// ~39 stored 3x3 blocks per node for the assembled product.
+// compile-time constants.  A block then reads ~210 B per element (FP32 record and block-local
+// indices) plus its staged nodes, against ~28 stored 3x3 blocks (~1 KB FP32) per node for the
+// assembled product.
 #include "device_common.hpp"
 #include "generated/elasticity_iso.inc"
 

@@ -332,6 +332,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="cells per side (Hexa8 n^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-configs", action="store_true", help="skip configurations 0, 2, 3, 4")
+    ap.add_argument("--configs", default="0,2,3,4", help="which of configurations 0, 2, 3, 4 to run (comma list)")
     ap.add_argument("--no-precond", action="store_true", help="skip the preconditioner comparison")
     ap.add_argument("--no-stepping", action="store_true", help="skip the time-stepping measurement")
     ap.add_argument("--no-natural-ic0", action="store_true",
@@ -579,7 +580,10 @@ def main():
     f.close()
     if not args.no_configs:
         out["configs"] = {}
+        wanted = set(args.configs.split(","))
         for name, fn in (("0", config0), ("2", config2), ("3", config3), ("4", config4)):
+            if name not in wanted:
+                continue
             try:
                 t0 = time.perf_counter()
                 out["configs"][name] = fn()

@@ -502,8 +502,10 @@ inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
 // buffer at the nodes it owns and at the shared nodes it holds first (lslot bit 31), so every node
 // is written once and no block reads a value another block has already replaced.
 // LDS: per local node 3 input values and 3 accumulators.
+// (FP64 Tetra_10: capped at 3 waves per SIMD, 168 VGPRs and 40 B of spills, against 184 VGPRs and
+// 2 waves uncapped: 80 against 84 us per product at 384k elements)
 template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA>
-__global__ __launch_bounds__(256) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
+__global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : 1) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   const uint32_t *__restrict__ bsub,
                                                   uint32_t n_colors, uint32_t ls, const uint32_t *__restrict__ lnode,

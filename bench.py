@@ -339,7 +339,8 @@ def main():
                     help="skip the natural-order block IC(0) solve (hundreds of level launches per iteration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; the path shards as described in DESIGN.md "
+        print("bench.py: measures one GPU only; a multi-GPU run would give each rank whole "
+              "sub-problems (batches) or a node-range shard with halo exchange, not run here "
               "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

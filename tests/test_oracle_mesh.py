@@ -43,7 +43,7 @@ def test_pattern_is_node_adjacency(oracle, kind, nb):
         assert list(col[rp[r]:rp[r + 1]]) == sorted(rows[r])
 
 
-def test_pattern_counts_match_survey(oracle):
+def test_pattern_counts_match_published_counts(oracle):
     # config 0: 17,191 full / 9,261 lower blocks; Hexa 21^3 (the published mesh): 118,121 lower
     # blocks = 1,035,306 lower scalars (include/mesh/exemple_probleme_elastique.txt:218-219)
     c, e = oracle.make_rect(oracle.TETRA, (11, 11, 11))

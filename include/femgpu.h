@@ -36,7 +36,9 @@ typedef enum {
     FEMGPU_MAX_ITER = 4,      /* iteration cap reached before the stop test */
     FEMGPU_HIP_ERROR = 5,     /* a HIP runtime call failed */
     FEMGPU_NAN = 6,           /* NaN or Inf met in a reduction */
-    FEMGPU_NOT_READY = 7      /* call order violated (e.g. solve before assemble) */
+    FEMGPU_NOT_READY = 7,     /* call order violated (e.g. solve before assemble) */
+    FEMGPU_STAGNATED = 8      /* the FP64 residual b - K u stopped decreasing above the
+                                 tolerance (the tolerance is below what FP64 attains here) */
 } femgpu_status;
 
 typedef enum { FEMGPU_TETRA = 0, FEMGPU_TETRA10 = 1, FEMGPU_HEXA = 2 } femgpu_elem_kind;
@@ -78,7 +80,12 @@ typedef struct {
     int precision;            /* femgpu_precision */
     int precond;              /* femgpu_precond */
     int crit_kind;            /* femgpu_crit_kind */
-    double tol;               /* REL_L2: ||r|| <= tol ||b||;  ABS_MAXNORM: max|r_i| <= tol */
+    double tol;               /* REL_L2: ||r|| <= tol ||b||;  ABS_MAXNORM: max|r_i| <= tol, with
+                                 r = b - K u formed in FP64 at the end: a recurrence residual that
+                                 meets the test only triggers that check; a solve whose true
+                                 residual stops decreasing above the tolerance ends with
+                                 FEMGPU_STAGNATED, one that runs out of iterations with
+                                 FEMGPU_MAX_ITER */
     int max_iter;             /* inner iterations cap (the reference has none) */
     double reliable_delta;    /* MIXED: FP64 residual refresh when the FP32 residual fell by this factor */
     int chunk;                /* iterations enqueued between host checks */
@@ -87,7 +94,8 @@ typedef struct {
 typedef struct {
     int status;
     int iters;                /* CG iterations (inner FP32 iterations in MIXED mode) */
-    int refreshes;            /* MIXED: FP64 residual refreshes (outer refinement steps) */
+    int refreshes;            /* FP64 true-residual checks that restarted or ended the
+                                 recurrence (MIXED: the outer refinement steps) */
     double rel_res;           /* final ||b - K u||_2 / ||b||_2, FP64 */
     double max_abs_res;       /* final max_i |b - K u|_i, FP64 */
     double solve_ms;          /* device time of the solve (HIP events on the problem's stream) */
@@ -148,6 +156,10 @@ FEMGPU_API int femgpu_get_result_device(femgpu_problem *p, double *d_u);
 FEMGPU_API int femgpu_set_result(femgpu_problem *p, const double *u, size_t n);
 /* get_sollicitation() (formulation_ancestor.h:236). */
 FEMGPU_API int femgpu_get_sollicitation(femgpu_problem *p, double *F, size_t n);
+/* Writes the sollicitation that the next solve_system solves for: the reference's
+ * get_sollicitation() returns Vec<T>& (formulation.h:2647), which callers modify in place
+ * between assemble() and solve_system().  A later vector assembly overwrites it. */
+FEMGPU_API int femgpu_set_sollicitation(femgpu_problem *p, const double *F, size_t n);
 /* get_mat() (formulation_ancestor.h:230): the assembled K as full 3x3-block CSR, block row-major
  * values (both triangles; the reference stores the lower one).  Any pointer may be NULL. */
 FEMGPU_API int femgpu_get_matrix(femgpu_problem *p, uint32_t *row_ptr, uint32_t *col, double *vals);

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfemgpu.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "femgpu.h")
 
-OK, BAD_INPUT, NEG_JACOBIAN, NOT_SPD, MAX_ITER, HIP_ERROR, NAN, NOT_READY = range(8)
+OK, BAD_INPUT, NEG_JACOBIAN, NOT_SPD, MAX_ITER, HIP_ERROR, NAN, NOT_READY, STAGNATED = range(9)
 TETRA, TETRA10, HEXA = 0, 1, 2
 TRI3, TRI6, QUAD4 = 0, 1, 2
 FP64, MIXED = 0, 1
@@ -76,6 +76,7 @@ _SIGS = {
     "femgpu_get_result_device": (C.c_int, [_P, _P]),
     "femgpu_set_result": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_get_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
+    "femgpu_set_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_get_matrix": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_sizes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint32)]),

@@ -44,7 +44,7 @@ class ElasticSolve(torch.autograd.Function):
         A.check(lib.femgpu_get_result_device(problem.handle, C.c_void_p(u.data_ptr())), "get_result")
         problem.forward_stats = st
         problem._adjoint_token = getattr(problem, "_adjoint_token", 0) + 1
-        ctx.problem, ctx.token, ctx.adj_rel_tol = problem, problem._adjoint_token, adj_rel_tol
+        ctx.problem, ctx.token, ctx.rel_tol, ctx.adj_rel_tol = problem, problem._adjoint_token, rel_tol, adj_rel_tol
         ctx.save_for_backward(E)
         return u
 
@@ -59,7 +59,7 @@ class ElasticSolve(torch.autograd.Function):
             # another forward ran on this problem since: restore K(E) and u before the adjoint
             A.check(lib.femgpu_set_elastic_modulus_device(problem.handle, C.c_void_p(E.data_ptr())), "set_E")
             problem.assemble(True, True)
-            o = _opts(lib, 1e-13)
+            o = _opts(lib, ctx.rel_tol)
             A.check(lib.femgpu_solve_system(problem.handle, C.byref(o), C.byref(A.SolveStats())), "solve_system")
             problem._adjoint_token = ctx.token
         out = torch.empty(problem.n_elem, dtype=torch.float64, device=g.device)

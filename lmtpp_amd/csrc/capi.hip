@@ -545,6 +545,22 @@ int check_opts(const femgpu_solve_opts *o) {
     return FEMGPU_OK;
 }
 
+// A solve other than solve_system_using_precond builds the preconditioner of the current K: a
+// preconditioner pinned by get_precond is released when K changed since (its factors no longer
+// match K) or when the solve asks for another kind (building that one replaces it), so that
+// solve_system_using_precond then reports NOT_READY instead of running with other factors.
+void unpin_precond(femgpu_problem &H, const femgpu_solve_opts *o) {
+    if (!H.precond_pinned) return;
+    const int kind = o ? o->precond : H.dopts.precond;
+    if (H.precond_stale || kind != H.pinned_kind) {
+        if (H.precond_stale) {
+            H.P.minv_kind = -1;
+            H.P.ic0.valid = H.P.ic0.valid32 = false;
+        }
+        H.precond_pinned = H.precond_stale = false;
+    }
+}
+
 int solve_system(femgpu_problem &H, const femgpu_solve_opts *o, femgpu_solve_stats *per, double *d_rhs, double *d_x) {
     int rc = check_opts(o);
     if (rc) return rc;
@@ -638,6 +654,7 @@ const char *femgpu_status_string(int s) {
         case FEMGPU_HIP_ERROR: return "HIP error";
         case FEMGPU_NAN: return "NaN or Inf";
         case FEMGPU_NOT_READY: return "not ready";
+        case FEMGPU_STAGNATED: return "residual stagnated above the tolerance";
         default: return "unknown status";
     }
 }
@@ -795,11 +812,7 @@ int femgpu_assemble(femgpu_problem *h, int do_mat, int do_vec) {
 
 int femgpu_solve_system(femgpu_problem *h, const femgpu_solve_opts *o, femgpu_solve_stats *st) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "solve_system: null");
-    if (h->precond_stale) {  // K changed since get_precond: solve_system builds its own
-        h->P.minv_kind = -1;
-        h->P.ic0.valid = h->P.ic0.valid32 = false;
-        h->precond_pinned = h->precond_stale = false;
-    }
+    unpin_precond(*h, o);
     femgpu_solve_stats tmp{};
     const int rc = solve_system(*h, o, st ? st : &tmp, nullptr, nullptr);
     if (st) st->status = rc;
@@ -844,6 +857,22 @@ int femgpu_set_result(femgpu_problem *h, const double *u, size_t n) {
 int femgpu_get_sollicitation(femgpu_problem *h, double *F, size_t n) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "get_sollicitation: null");
     return copy_sub(*h, 0, h->P.d_F, F, n);
+}
+
+int femgpu_set_sollicitation(femgpu_problem *h, const double *F, size_t n) {
+    if (!h || !F || n != 3 * (size_t)h->P.n_nodes) return fail(FEMGPU_BAD_INPUT, "set_sollicitation: size");
+    Problem &P = h->P;
+    std::vector<double> t;
+    if (!P.nperm.empty()) {  // to the internal node order
+        t.resize(n);
+        for (uint32_t i = 0; i < P.n_nodes; ++i)
+            for (int d = 0; d < 3; ++d) t[3 * (size_t)i + d] = F[3 * (size_t)P.nperm[i] + d];
+        F = t.data();
+    }
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipMemcpy(P.d_F, F, n * 8, hipMemcpyHostToDevice));
+    P.assembled_vec = true;
+    return FEMGPU_OK;
 }
 
 int femgpu_get_matrix(femgpu_problem *h, uint32_t *row_ptr, uint32_t *col, double *vals) {
@@ -1093,6 +1122,7 @@ int femgpu_adjoint_device(femgpu_problem *h, const double *d_dJdu, double *d_dJd
     if (e == hipSuccess) e = launch_fill(P.d_aux, n, 0.0, P.stream);
     if (e != hipSuccess) { (void)hipFree(d_rhs); return fail(FEMGPU_HIP_ERROR, hipGetErrorString(e)); }
     femgpu_solve_stats tmp{};
+    unpin_precond(*h, o);
     int rc = solve_system(*h, o, st ? st : &tmp, d_rhs, P.d_aux);
     if (rc == FEMGPU_OK) {
         e = launch_elem_grad(P, P.d_u, P.d_aux, d_dJdE);

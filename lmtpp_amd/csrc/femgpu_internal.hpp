@@ -25,7 +25,10 @@ struct ProbScalars {
     double bb, stop2, crit;       // stop test: rr <= stop2 (REL_L2) or rmax <= crit (ABS_MAXNORM)
     double rr_ref, delta2;        // mixed: pause for an FP64 refresh when rr <= delta2 * rr_ref
     double max_diag;              // assembly: max |diag K| of this sub-problem
+    double res_best;              // smallest true residual measure of the checks so far
     int iter, max_iter, done, pause, status, crit_kind, mixed, refreshes;
+    int checks, stalls;           // true-residual checks; consecutive ones without progress
+    int iter_best;                // iteration of the check that set res_best
 };
 
 // pause states: 0 running, 1 FP64 refresh requested, 2 refreshed (direction update pending)

@@ -1137,7 +1137,9 @@ static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z)
     if (pq && z) mf_passes<T, true, true>(P, x, xb, z, y);
     else if (pq) mf_passes<T, true, false>(P, x, nullptr, nullptr, y);
     else mf_passes<T, false, false>(P, x, nullptr, nullptr, y);
-    return hipGetLastError();
+    // peek, not get: a failed launch stays pending for the caller's own check (the solver
+    // tests hipGetLastError after each chunk and at the end of the solve)
+    return hipPeekAtLastError();
 }
 
 // Average device time of the element-block kernel alone and of the whole product over reps launches

@@ -374,6 +374,7 @@ __global__ __launch_bounds__(256) void k_dot_rz(const T *__restrict__ r, const T
 // FIN_UPD_IC: as FIN_UPD without r.z (rho_prev = rho; FIN_RHO then sets rho and beta).
 enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5,
        FIN_UPD_IC = 6, FIN_RHO = 7 };
+constexpr int STALL_CHECKS = 8, STALL_ITERS = 1000;
 
 __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
                                                   unsigned stride, const uint32_t *__restrict__ sub_wg0,
@@ -404,12 +405,14 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         P.rr = o[1];
         P.rmax = o[2];
         if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
+        // The recurrence residual only requests a check: the solve ends on the true residual
+        // b - K x (FIN_RES_CHECK), in FP64 as in MIXED, where an FP32 recurrence that has fallen
+        // by delta since the last FP64 residual also pauses for a refresh.
         const bool conv = P.crit_kind == 0 ? o[1] <= P.stop2 : o[2] <= P.crit;
-        if (P.mixed) {
-            if (conv || o[1] <= P.delta2 * P.rr_ref) P.pause = PAUSE_REQ;
-        } else if (conv) {
-            P.done = 1;
-        }
+        if (conv || (P.mixed && o[1] <= P.delta2 * P.rr_ref)) P.pause = PAUSE_REQ;
+        // no progress of the true residual since the last check that improved it, for as many
+        // iterations as all progress took (at least STALL_ITERS): stagnated above the tolerance
+        if (!P.done && P.checks >= 2 && P.iter - P.iter_best > max(P.iter_best, STALL_ITERS)) { P.status = 8; P.done = 1; }
         if (!P.done && P.iter >= P.max_iter) { P.status = 4; P.done = 1; }
     } else if (what == FIN_RES || what == FIN_RES_CHECK) {
         double o[2];
@@ -419,8 +422,20 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         P.rmax = o[1];
         if (what == FIN_RES_CHECK) {
             P.rr_ref = o[0];
+            // progress of the true residual: a check that does not halve the smallest one so far
+            // (in the norm of the stop test) is a stall; STALL_CHECKS in a row end the solve --
+            // the tolerance lies below what FP64 attains on this system
+            const double v = P.crit_kind == 0 ? o[0] : o[1] * o[1];
             if (!finite(o[0])) { P.status = 6; P.done = 1; }
             else if (P.crit_kind == 0 ? o[0] <= P.stop2 : o[1] <= P.crit) P.done = 1;
+            else if (P.checks > 0 && !(v < 0.25 * P.res_best)) {
+                if (++P.stalls >= STALL_CHECKS) { P.status = 8; P.done = 1; }
+            } else {
+                P.res_best = v;
+                P.stalls = 0;
+                P.iter_best = P.iter;
+            }
+            P.checks += 1;
         }
     } else {
         double o[1];
@@ -534,6 +549,7 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     // matrix-free products; an FP64 residual is the product into y, then y = b - y with the
     // residual's partials
     if (P.spmv.format[pr] == FMT_MF) {
+        // a launch error stays pending (mf_launch peeks) for run_chunk's and solve_impl's checks
         (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr);
         if (MODE == MODE_RES)
             hipLaunchKernelGGL(k_residual, g, blk, 0, P.stream, b, reinterpret_cast<double *>(y), wt, P.d_ps, P.d_partials,
@@ -939,13 +955,18 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             refresh |= !P.h_ps[s].done && P.h_ps[s].pause == PAUSE_REQ;
         }
         if (all_done) break;
-        if (mixed && refresh) {
-            // FP64 refresh of the paused sub-problems: x += x32; r = b - K x (FP64, stop test on
-            // the true residual); restart the FP32 recurrence from it, keeping the direction.
-            hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_PAUSE_REQ);
-            launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, P.d_r, SEL_PAUSE_REQ);
+        if (refresh) {
+            // Check of the paused sub-problems on the true residual: MIXED first adds the FP32
+            // correction (x += x32); r = b - K x in FP64 with the stop test on it; those that
+            // fail it restart the recurrence from r (FP32 or FP64), keeping the direction.
+            // FP64 forms that residual in q (dead between iterations), not in its recurrence
+            // residual r: the matrix-free product writes every sub-problem's rows, and the
+            // sub-problems still running keep their r; the restart copies r of the checked ones.
+            if (mixed) hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_PAUSE_REQ);
+            launch_spmv_t<double, MODE_RES>(P, P.d_K, d_x, d_rhs, mixed ? P.d_r : P.d_q, SEL_PAUSE_REQ);
             finalize(P, FIN_RES_CHECK, SEL_PAUSE_REQ);
-            launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, zbuf<PRE, float>(P), 0);
+            if (mixed) launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, zbuf<PRE, float>(P), 0);
+            else launch_restart<PRE, double>(P, P.d_q, P.d_r, P.d_p, P.d_Minv, zbuf<PRE, double>(P), 0);
             hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
         }
     }
@@ -973,6 +994,9 @@ int precond_build(Problem &P, int precond) {
     if (!P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
         return 5;
     std::memset(P.h_ps, 0, sizeof(ProbScalars) * P.n_sub);
+    // max |diag K| stays: an F-only assembly after this call scales the penalty terms with it
+    // (formulation.h:1029-1030,1059-1062)
+    for (uint32_t s = 0; s < P.n_sub; ++s) P.h_ps[s].max_diag = s < P.max_diag.size() ? P.max_diag[s] : 0.0;
     if (write_scalars(P)) return 5;
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);

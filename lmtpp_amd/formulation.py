@@ -252,6 +252,12 @@ class Formulation:
         A.check(self._lib.femgpu_get_sollicitation(self._h, _ptr(F), F.size), "get_sollicitation")
         return F
 
+    def set_sollicitation(self, F):
+        """The right-hand side of the next solve_system (the reference's get_sollicitation()
+        returns a mutable Vec&, formulation.h:2647); a later vector assembly overwrites it."""
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+        A.check(self._lib.femgpu_set_sollicitation(self._h, _ptr(F), F.size), "set_sollicitation")
+
     def sizes(self):
         nn, ne, nz, nc = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint32()
         A.check(self._lib.femgpu_get_sizes(self._h, C.byref(nn), C.byref(ne), C.byref(nz), C.byref(nc)), "sizes")

@@ -12,6 +12,11 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <omp.h>
+
+/* Host threads of the parallel paths (or_set_threads); 1 = the serial restatement. */
+static int g_threads = 1;
+int or_set_threads(int n) { g_threads = n > 0 ? n : omp_get_max_threads(); return g_threads; }
 
 /* ------------------------------------------------------------------------- */
 /* Element tables                                                             */
@@ -563,39 +568,88 @@ static inline size_t find_block(const uint32_t *row_ptr, const uint32_t *col, ui
     return lo;
 }
 
+/* One element of the loop below: its Gauss points' local upper triangles (i <= j) added
+ * straight into the global matrix, mirrored to the symmetric position
+ * (formal_lf/calculate_matrix.py:64-98), and its body load.  Returns 2 for det J <= 0. */
+static int assemble_elem(int kind, const double *coords, const uint32_t *conn, uint32_t e, const double *E_e, double nu,
+                         const double bf[3], const uint32_t *row_ptr, const uint32_t *col, double *vals, double *F)
+{
+    const int npe = or_nodes_per_elem(kind), nd = 3 * npe;
+    double xe[30];
+    const uint32_t *en = conn + (size_t)e * npe;
+    for (int a = 0; a < npe; ++a) for (int d = 0; d < 3; ++d) xe[3 * a + d] = coords[3 * (size_t)en[a] + d];
+    gp_data G; element_gp(kind, xe, &G);
+    const int status = G.detj_min > 0 ? 0 : 2;
+    double lam, mu; lame(E_e[e], nu, &lam, &mu);
+    size_t slot[10][10];
+    for (int a = 0; a < npe; ++a) for (int b = 0; b < npe; ++b) slot[a][b] = find_block(row_ptr, col, en[a], en[b]);
+    for (int q = 0; q < G.ngp; ++q) {
+        for (int r = 0; r < nd; ++r)
+            for (int c = r; c < nd; ++c) {
+                const int a = r / 3, i = r % 3, b = c / 3, j = c % 3;
+                const double v = kentry(&G, q, a, i, b, j, lam, mu);
+                vals[9 * slot[a][b] + 3 * i + j] += v;
+                if (r != c) vals[9 * slot[b][a] + 3 * j + i] += v;
+            }
+        if (bf)
+            for (int a = 0; a < npe; ++a)
+                for (int d = 0; d < 3; ++d) F[3 * (size_t)en[a] + d] += G.wd[q] * G.N[q][a] * bf[d];
+    }
+    return status;
+}
+
 /* Element loop (formulation.h:626-647) with the default per-Gauss-point
- * integration (formulation.h:2917-2933): each Gauss point's local upper
- * triangle (i <= j) is added straight into the global matrix, mirrored to the
- * symmetric position, in element order (formal_lf/calculate_matrix.py:64-98);
- * then the load of the element, then the skin faces (formulation.h:584-594). */
+ * integration (formulation.h:2917-2933), in element order; then the skin faces
+ * (formulation.h:584-594).  With more than one thread (or_set_threads) the
+ * elements are greedily coloured so that no two of a colour share a node, and
+ * the colours run in turn, each on all threads: every entry then receives its
+ * additions in colour order instead of element order (the multi-core CPU line). */
 int or_assemble(int kind, uint32_t n_nodes, const double *coords, uint32_t n_elem, const uint32_t *conn,
                 const double *E_e, double nu, const double bf[3], uint32_t n_faces, const uint32_t *faces,
                 const double *face_trac, const uint32_t *row_ptr, const uint32_t *col, double *vals, double *F)
 {
-    const int npe = or_nodes_per_elem(kind), nd = 3 * npe;
+    const int npe = or_nodes_per_elem(kind);
     memset(vals, 0, sizeof(double) * 9 * (size_t)row_ptr[n_nodes]);
     memset(F, 0, sizeof(double) * 3 * (size_t)n_nodes);
     int status = 0;
-    for (uint32_t e = 0; e < n_elem; ++e) {
-        double xe[30];
-        const uint32_t *en = conn + (size_t)e * npe;
-        for (int a = 0; a < npe; ++a) for (int d = 0; d < 3; ++d) xe[3 * a + d] = coords[3 * (size_t)en[a] + d];
-        gp_data G; element_gp(kind, xe, &G);
-        if (!(G.detj_min > 0)) status = 2;
-        double lam, mu; lame(E_e[e], nu, &lam, &mu);
-        size_t slot[10][10];
-        for (int a = 0; a < npe; ++a) for (int b = 0; b < npe; ++b) slot[a][b] = find_block(row_ptr, col, en[a], en[b]);
-        for (int q = 0; q < G.ngp; ++q) {
-            for (int r = 0; r < nd; ++r)
-                for (int c = r; c < nd; ++c) {
-                    const int a = r / 3, i = r % 3, b = c / 3, j = c % 3;
-                    const double v = kentry(&G, q, a, i, b, j, lam, mu);
-                    vals[9 * slot[a][b] + 3 * i + j] += v;
-                    if (r != c) vals[9 * slot[b][a] + 3 * j + i] += v;
-                }
-            if (bf)
-                for (int a = 0; a < npe; ++a)
-                    for (int d = 0; d < 3; ++d) F[3 * (size_t)en[a] + d] += G.wd[q] * G.N[q][a] * bf[d];
+    if (g_threads > 1) {
+        uint32_t *color = (uint32_t *)malloc(4 * (size_t)(n_elem ? n_elem : 1));
+        uint64_t *used = (uint64_t *)calloc(n_nodes ? n_nodes : 1, 8);
+        uint32_t n_col = 0;
+        for (uint32_t e = 0; e < n_elem && status == 0; ++e) {
+            const uint32_t *en = conn + (size_t)e * npe;
+            uint64_t m = 0;
+            for (int a = 0; a < npe; ++a) m |= used[en[a]];
+            uint32_t c = 0;
+            while (c < 64 && (m >> c & 1)) ++c;
+            if (c == 64) { status = 1; break; }
+            color[e] = c;
+            if (c + 1 > n_col) n_col = c + 1;
+            for (int a = 0; a < npe; ++a) used[en[a]] |= 1ull << c;
+        }
+        uint32_t *start = (uint32_t *)calloc(n_col + 2, 4), *order = (uint32_t *)malloc(4 * (size_t)(n_elem ? n_elem : 1));
+        if (status == 0) {
+            for (uint32_t e = 0; e < n_elem; ++e) ++start[color[e] + 1];
+            for (uint32_t c = 0; c < n_col; ++c) start[c + 1] += start[c];
+            for (uint32_t e = 0; e < n_elem; ++e) order[start[color[e]]++] = e;   /* start[c] -> end of c */
+            for (uint32_t c = n_col; c > 0; --c) start[c] = start[c - 1];
+            start[0] = 0;
+        }
+        for (uint32_t c = 0; c < n_col && status == 0; ++c) {
+            int st = 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(max : st)
+            for (uint32_t k = start[c]; k < start[c + 1]; ++k) {
+                const int s_ = assemble_elem(kind, coords, conn, order[k], E_e, nu, bf, row_ptr, col, vals, F);
+                if (s_ > st) st = s_;
+            }
+            if (st) status = st;
+        }
+        free(color); free(used); free(start); free(order);
+        if (status == 1) return 1;
+    } else {
+        for (uint32_t e = 0; e < n_elem; ++e) {
+            const int st = assemble_elem(kind, coords, conn, e, E_e, nu, bf, row_ptr, col, vals, F);
+            if (st) status = st;
         }
     }
     if (n_faces && faces && face_trac) {
@@ -607,7 +661,6 @@ int or_assemble(int kind, uint32_t n_nodes, const double *coords, uint32_t n_ele
             for (int a = 0; a < npf; ++a) for (int d = 0; d < 3; ++d) F[3 * (size_t)faces[(size_t)f * npf + a] + d] += Ff[3 * a + d];
         }
     }
-    (void)nd;
     return status;
 }
 
@@ -638,6 +691,7 @@ int or_penalty(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, d
 
 void or_spmv(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, const double *vals, const double *x, double *y)
 {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (uint32_t r = 0; r < n_nodes; ++r) {
         double s0 = 0, s1 = 0, s2 = 0;
         for (uint32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
@@ -651,6 +705,25 @@ void or_spmv(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, con
 }
 
 static double dotv(size_t n, const double *a, const double *b) { double s = 0; for (size_t i = 0; i < n; ++i) s += a[i] * b[i]; return s; }
+
+/* Dot product of the multi-threaded PCG: DOT_CHUNKS fixed index ranges summed in order, the
+ * range sums added in range order -- the same value for every thread count. */
+#define DOT_CHUNKS 1024
+static double dotv_mt(size_t n, const double *a, const double *b)
+{
+    double part[DOT_CHUNKS];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int k = 0; k < DOT_CHUNKS; ++k) {
+        const size_t i0 = n * (size_t)k / DOT_CHUNKS, i1 = n * (size_t)(k + 1) / DOT_CHUNKS;
+        double t = 0;
+        for (size_t i = i0; i < i1; ++i) t += a[i] * b[i];
+        part[k] = t;
+    }
+    double s = 0;
+    for (int k = 0; k < DOT_CHUNKS; ++k) s += part[k];
+    return s;
+}
+#define DOTP(n_, a_, b_) (g_threads > 1 ? dotv_mt(n_, a_, b_) : dotv(n_, a_, b_))
 
 static int inv3(const double *A, double *Ai)
 {
@@ -669,6 +742,14 @@ static int inv3(const double *A, double *Ai)
 int or_pcg_jacobi(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, const double *vals, const double *b,
                   double *x, double rel_tol, int max_iter, int block_jacobi, int *iters, double *rel_res)
 {
+    return or_pcg_jacobi_hist(n_nodes, row_ptr, col, vals, b, x, rel_tol, max_iter, block_jacobi, iters, rel_res, NULL, 0);
+}
+
+int or_pcg_jacobi_hist(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, const double *vals,
+                       const double *b, double *x, double rel_tol, int max_iter, int block_jacobi, int *iters,
+                       double *rel_res, double *hist, int hist_cap)
+{
+    const double t_start = omp_get_wtime();
     const size_t n = 3 * (size_t)n_nodes;
     double *r = (double *)malloc(8 * n), *z = (double *)malloc(8 * n), *p = (double *)malloc(8 * n), *q = (double *)malloc(8 * n);
     double *Dinv = (double *)malloc(8 * 9 * (size_t)n_nodes);
@@ -684,29 +765,38 @@ int or_pcg_jacobi(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col
     }
     if (status) goto done;
     or_spmv(n_nodes, row_ptr, col, vals, x, q);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (size_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
-#define APPLY_M(dst, src) for (uint32_t k = 0; k < n_nodes; ++k) { const double *Di = Dinv + 9 * (size_t)k; const double *s_ = src + 3 * (size_t)k; \
+#define APPLY_M(dst, src) \
+    _Pragma("omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)") \
+    for (uint32_t k = 0; k < n_nodes; ++k) { const double *Di = Dinv + 9 * (size_t)k; const double *s_ = src + 3 * (size_t)k; \
         for (int d = 0; d < 3; ++d) dst[3 * (size_t)k + d] = Di[3 * d] * s_[0] + Di[3 * d + 1] * s_[1] + Di[3 * d + 2] * s_[2]; }
-    const double bn = sqrt(dotv(n, b, b));
+    const double bn = sqrt(DOTP(n, b, b));
     const double stop = rel_tol * (bn > 0 ? bn : 1.0);
-    double rn = sqrt(dotv(n, r, r));
+    double rn = sqrt(DOTP(n, r, r));
     int it = 0;
     APPLY_M(z, r);
     memcpy(p, z, 8 * n);
-    double rho = dotv(n, r, z);
+    double rho = DOTP(n, r, z);
     while (rn > stop) {
         if (it >= max_iter) { status = 4; break; }
         or_spmv(n_nodes, row_ptr, col, vals, p, q);
-        const double pq = dotv(n, p, q);
+        const double pq = DOTP(n, p, q);
         if (!(pq > 0)) { status = 3; break; }
         const double alpha = rho / pq;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
         for (size_t i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
         APPLY_M(z, r);
-        const double rho1 = dotv(n, r, z);
+        const double rho1 = DOTP(n, r, z);
         const double beta = rho1 / rho;
         rho = rho1;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
         for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
-        rn = sqrt(dotv(n, r, r));
+        rn = sqrt(DOTP(n, r, r));
+        if (hist && it < hist_cap) {
+            hist[2 * it] = rn / (bn > 0 ? bn : 1.0);
+            hist[2 * it + 1] = omp_get_wtime() - t_start;
+        }
         ++it;
     }
 #undef APPLY_M

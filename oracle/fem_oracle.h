@@ -52,6 +52,11 @@ extern "C" {
 #endif
 
 enum { OR_TETRA = 0, OR_TETRA10 = 1, OR_HEXA = 2 };
+
+/* Host threads of or_assemble, or_spmv and or_pcg_jacobi (OpenMP; n <= 0: all).  1, the
+ * default, is the serial restatement in the reference's order; more threads colour the element
+ * loop and chunk the dot products (the multi-core CPU line).  Returns the count set. */
+int or_set_threads(int n);
 enum { OR_TRI3 = 0, OR_TRI6 = 1, OR_QUAD4 = 2 };
 
 int or_nodes_per_elem(int kind);
@@ -100,6 +105,12 @@ void or_spmv(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, con
 int or_pcg_jacobi(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, const double *vals,
                   const double *b, double *x, double rel_tol, int max_iter, int block_jacobi,
                   int *iters, double *rel_res);
+
+/* The same, recording after iteration i (i < hist_cap) the relative recurrence residual in
+ * hist[2i] and the seconds since the call began (set-up included) in hist[2i+1]. */
+int or_pcg_jacobi_hist(uint32_t n_nodes, const uint32_t *row_ptr, const uint32_t *col, const double *vals,
+                       const double *b, double *x, double rel_tol, int max_iter, int block_jacobi, int *iters,
+                       double *rel_res, double *hist, int hist_cap);
 
 /* Generic scalar CSR CG with scalar-Jacobi or no preconditioner (KAT tests). */
 int or_cg_csr(uint32_t n, const uint32_t *row_ptr, const uint32_t *col, const double *vals,

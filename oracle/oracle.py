@@ -48,6 +48,11 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def set_threads(n):
+    """Host threads of assemble, spmv and pcg_jacobi (n <= 0: all); 1 is the serial restatement."""
+    return int(lib().or_set_threads(int(n)))
+
+
 def make_rect(kind, nb_points, X0=(0.0, 0.0, 0.0), X1=(1.0, 1.0, 1.0)):
     nb = (C.c_uint * 3)(*nb_points)
     nn, ne = C.c_uint32(), C.c_uint32()
@@ -153,6 +158,18 @@ def pcg_jacobi(row_ptr, col, vals, b, x0=None, rel_tol=1e-10, max_iter=100000, b
     rc = lib().or_pcg_jacobi(len(row_ptr) - 1, _p(row_ptr), _p(col), _p(vals), _p(b), _p(x), C.c_double(rel_tol),
                              max_iter, int(block), C.byref(it), C.byref(rr))
     return x, it.value, rr.value, rc
+
+
+def pcg_jacobi_hist(row_ptr, col, vals, b, x0=None, rel_tol=1e-10, max_iter=100000, block=False):
+    """pcg_jacobi, also returning per iteration (relative recurrence residual, seconds since the
+    call began): an array of shape (iters, 2)."""
+    b = _f64(b)
+    x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
+    it, rr = C.c_int(), C.c_double()
+    hist = np.zeros((max(1, min(max_iter, 1 << 20)), 2))
+    rc = lib().or_pcg_jacobi_hist(len(row_ptr) - 1, _p(row_ptr), _p(col), _p(vals), _p(b), _p(x), C.c_double(rel_tol),
+                                  max_iter, int(block), C.byref(it), C.byref(rr), _p(hist), hist.shape[0])
+    return x, it.value, rr.value, rc, hist[:min(it.value, hist.shape[0])].copy()
 
 
 def cg_csr(row_ptr, col, vals, b, rel_tol=1e-14, max_iter=1000):

@@ -14,7 +14,7 @@ import pytest
 
 import lmtpp_amd as L
 from conftest import GOLDEN
-from problems import E0, NU0, build, jitter_interior, lognormal_E, tetra10_jittered
+from problems import E0, NU0, build, fd_gradient, jitter_interior, lognormal_E, tetra10_jittered
 
 pytestmark = pytest.mark.gpu
 
@@ -89,7 +89,7 @@ def test_config0_solution_matches_reference_and_oracle(oracle):
         f.precond = prec
         f.precision = L.FP64
         f.set_result(np.zeros_like(F))
-        assert f.solve_system(rel_tol=1e-13), f.last_stats.status
+        assert f.solve_system(rel_tol=1e-12), f.last_stats.status
         u = f.get_result()
         assert f.last_stats.rel_res <= 1e-12, f.last_stats.rel_res
         assert np.linalg.norm(u - xs) <= 1e-10 * np.linalg.norm(xs)
@@ -155,7 +155,8 @@ def test_block_system_known_answers():
     vals = np.array([A12[3 * r:3 * r + 3, 3 * c:3 * c + 3] for r in range(4) for c in range(4)])
     s = L.BlockSystem(rp, col, vals, b12)
     s.precond = L.BLOCK_JACOBI
-    assert s.solve_system(rel_tol=1e-15)
+    # cond(A10) = 1.9e4: FP64 attains about ||A|| ||x|| eps / ||b|| = 4e-13 of relative residual
+    assert s.solve_system(rel_tol=2e-12), s.last_stats.status
     x = s.get_result()[:10]
     xr = np.array(k["dense10"]["x"])
     assert (np.abs(x - xr) / (1e-20 + np.abs(x) + np.abs(xr))).max() < 1e-10
@@ -167,41 +168,41 @@ def test_adjoint_gradient_matches_oracle_and_fd(oracle):
     c, e = L.make_rect(kind, (6, 6, 6))
     E = lognormal_E(len(e), 0.3, 5)
     f, faces, tr, dofs = build(kind, c, e, E)
-    assert f.solve(rel_tol=1e-14)
+    assert f.solve(rel_tol=1e-11)
     u = f.get_result()
     rng = np.random.default_rng(1)
     top = np.where(np.abs(c[:, 2] - 1.0) < 1e-9)[0]
     obs = rng.choice(top, size=max(3, len(top) // 10), replace=False)
     odofs = (3 * obs[:, None] + np.arange(3)).ravel()
     u_meas = u[odofs] * (1 + 0.05 * rng.standard_normal(len(odofs)))
-
-    def J_of(uu):
-        return 0.5 * np.sum((uu[odofs] - u_meas) ** 2)
-
     dJdu = np.zeros_like(u)
     dJdu[odofs] = u[odofs] - u_meas
-    g = f.adjoint(dJdu, rel_tol=1e-14)
+    g = f.adjoint(dJdu, rel_tol=1e-11)
     # oracle adjoint: dense solve with the oracle's K (penalty scale held fixed)
     orp, ocol, ovals, oF, _, _ = oracle_system(oracle, kind, c, e, E, NU0, (0, 0, 0), faces, tr, dofs)
     Ad = dense(orp, ocol, ovals)
     lam = np.linalg.solve(Ad, dJdu)
     ug = np.linalg.solve(Ad, oF)
     go = oracle.elem_grad_E(kind, e, c, E, NU0, ug, lam)
-    assert np.abs(g - go).max() <= 1e-9 * np.abs(go).max()
+    gmax = np.abs(go).max()
+    big = np.abs(go) >= 1e-3 * gmax
+    assert (np.abs(g - go)[big] / np.abs(go)[big]).max() <= 1e-8
+    assert np.abs(g - go).max() <= 1e-9 * gmax
     # element-gradient kernel alone, same u and lambda
-    assert np.abs(f.elem_grad(ug, lam) - go).max() <= 1e-12 * np.abs(go).max()
-    # central finite differences through the GPU solve (step 1e-4 E_e)
-    for el in rng.choice(len(e), size=6, replace=False):
-        Jp = []
-        for s in (+1, -1):
-            E2 = E.copy()
-            E2[el] *= 1 + s * 1e-4
-            f.set_material(E2, NU0)
-            f.set_result(u)
-            assert f.solve(rel_tol=1e-14)
-            Jp.append(J_of(f.get_result()))
-        fd = (Jp[0] - Jp[1]) / (2e-4 * E[el])
-        assert abs(fd - g[el]) <= 1e-5 * abs(g).max()
+    assert np.abs(f.elem_grad(ug, lam) - go).max() <= 1e-12 * gmax
+    # central finite differences through the GPU solve (step 1e-4 E_e), per-element relative error
+    res = u[odofs] - u_meas
+    n_rel = 0
+    for el in rng.choice(len(e), size=8, replace=False):
+        fd = fd_gradient(f, oracle, E, u, res, odofs, el)
+        if fd is None:
+            continue
+        if abs(g[el]) >= 1e-3 * gmax:
+            assert abs(fd - g[el]) <= 1e-5 * abs(g[el]), (el, fd, g[el])
+            n_rel += 1
+        else:
+            assert abs(fd - g[el]) <= 1e-8 * gmax, (el, fd, g[el])
+    assert n_rel >= 4
 
 
 def test_error_paths():

@@ -22,7 +22,15 @@
  *                       No bit-level pin exists: the reference's generated
  *                       element kernels need its Python-2 generator.
  *   or_face_load        pinned by closed forms (A/3, A/4, Tri6 0 and A/3).
- *   or_penalty          pinned by reproducing max|diag K| * P on the same K.
+ *   or_skin_faces       pinned by geometry (test_oracle_pins.py): every boundary face of
+ *                       the unit cube once (total area 6), oriented outward by its first
+ *                       three nodes; Tetra_10 faces carry their mid-edge nodes at the
+ *                       midpoints of edges (0,1), (1,2), (2,0) (element_Triangle_6.py);
+ *                       a uniform traction loads only the mid-edge nodes, t A/3 per face.
+ *   or_penalty          pinned against a dense K summed from or_elem_matrix: exactly
+ *                       P * max|diag K| added at the constrained diagonal entries and
+ *                       P * max|diag K| * value to F, nothing else; the solution holds
+ *                       the prescribed values to O(1/P) (100x smaller defect at 100x P).
  *   or_block_ic0 / or_pcg_ic0
  *                       pinned against the reference's own MatWithTinyBlocks
  *                       chol_incomp + solve_using_incomplete_chol_factorize
@@ -38,9 +46,11 @@
  *                       constant strain, sigma = lambda tr I + 2 mu eps) and the
  *                       analytic gradient of a quadratic field on a straight
  *                       Tetra_10 at the point xi = (1/2,1/2,1/2).
- *   or_elem_grad_E      pinned by central finite differences of the oracle's
- *                       own assembled objective (tests), no reference
- *                       counterpart exists (the reference has no adjoint).
+ *   or_elem_grad_E      pinned by -lambda_e^T K_e(E_e) u_e / E_e from or_elem_matrix
+ *                       and by central differences (step 1e-4 E_e) of the objective
+ *                       with dense LAPACK solves, penalty scale held fixed
+ *                       (test_oracle_pins.py); no reference counterpart exists (the
+ *                       reference has no adjoint).
  */
 #ifndef FEM_ORACLE_H
 #define FEM_ORACLE_H

@@ -205,3 +205,29 @@ def test_rhs_only_assembly_equals_full_assembly():
         f.assemble(False, True)
         assert np.array_equal(f.get_sollicitation(), F_full), bf
     f.close()
+
+
+@pytest.mark.parametrize("precond", [L.BLOCK_JACOBI, L.BLOCK_IC0])
+def test_rhs_only_assembly_after_get_precond_keeps_penalty_scale(oracle, precond):
+    """assemble(); get_precond(); assemble(False, True); solve_system_using_precond with non-zero
+    prescribed displacements: the F-only assembly scales the penalty right-hand side with the
+    max|diag K| of the last matrix assembly (formulation.h:1029-1030,1059-1062), so the solution is
+    the oracle's and the constrained DOFs hold their values to O(1/P)."""
+    say("stepping rhs-only after get_precond", precond)
+    c, e, E = _mesh(8)
+    f, faces, tr, dofs = _formulation(c, e, E, precond=precond)
+    vals = 1e-4 * np.cos(np.arange(len(dofs)))
+    f.clear_constraints()
+    f.add_constraints(dofs, vals, 1e5)
+    f.assemble()
+    f.get_precond(rel_tol=1e-12)
+    f.assemble(False, True)
+    f.set_result(np.zeros(3 * f.n_nodes))
+    assert f.solve_system_using_precond(rel_tol=1e-12)
+    u = f.get_result()
+    rp, col, ovals, F, rc = oracle.assemble(L.HEXA, c, e, E, 0.3, faces=faces, face_trac=tr)
+    oracle.penalty(rp, col, ovals, F, dofs, vals, 1e5)
+    x, _, _, rc = oracle.pcg_jacobi(rp, col, ovals, F, rel_tol=1e-12, block=True)
+    assert rc == 0 and _close(u, x)
+    assert np.abs(u[dofs] - vals).max() <= 1e-3 * np.abs(vals).max()
+    f.close()

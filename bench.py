@@ -2,12 +2,15 @@
 
 One step = assembly of K and F on the GPU from the device-resident mesh and material, then the
 mixed-precision (FP32 inner / FP64 refreshed) block-Jacobi PCG solve from a zero initial guess to
-the relative residual that was calibrated, before timing, to give <= 1e-8 relative L2 error
-against an FP64 solve converged to 1e-13.  ``value`` is DOF / step seconds (MDOF/s) of the whole
-step; the FP64-only solve with the same preconditioner, the assembly rate, the SpMV roofline, the
-other preconditioners (scalar Jacobi, block IC(0) in the reference's natural order and in
-multicolour order), the after-solve recovery and a CPU baseline (the oracle on the host cores,
-bounded sample) are reported beside it.
+the relative residual that was calibrated, before timing, to give <= 0.5e-8 relative L2 error in
+FP64 against the FP64 oracle solution.  That solution comes from the CPU leg: the oracle (the
+plain-C restatement of the reference's path) assembles the same system on the host cores and
+solves it by block-Jacobi PCG to a relative residual of 1e-12, and its K and F are compared with
+the GPU's block by block.  ``value`` is DOF / step seconds (MDOF/s) of the whole step; the
+FP64-only solve with the same preconditioner, the assembly rate, the product rooflines, the other
+preconditioners (scalar Jacobi, block IC(0) in the reference's natural order and in multicolour
+order), the after-solve recovery, configurations 0, 2, 3, 4 and the CPU baseline (the CPU leg's
+time for the same workload and tolerance) are reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100]
 """
@@ -47,44 +50,135 @@ def load_traffic(name):
     return (d.get(name) or {}).get("traffic")
 
 
-def cpu_baseline(fp64_iters, ndof, n_elem, sample_n=64):
-    """Oracle (plain C restatement, one host core) on a bounded sample of the same workload: full
-    assembly of a Hexa8 sample_n^3 cube and 500 block-Jacobi PCG iterations on it (about 10 s of CPU
-    work), scaled per element and per DOF-iteration to the full cube and to the FP64 iteration count
-    the GPU needed."""
+def host_cpu():
+    """(logical CPUs of this process, CPU model name) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return n, model
+
+
+def cpu_leg(kind, c, e, E, faces, tr, dofs, f, batch_items=()):
+    """The CPU leg: the oracle (oracle/fem_oracle.c, the plain-C restatement of the reference's
+    path) on the host cores, in the same run as the GPU.
+
+    * All host cores (at most 16, the GPU box's CPU share): the full config-1 workload --
+      coloured element loop, penalty, block-Jacobi PCG to a relative residual of 1e-12 with the
+      time of every iteration recorded, so the time to any tolerance is read off afterwards.
+      Its solution is the FP64 oracle solution the GPU results are compared with; its K and F are
+      compared with the GPU's assembled K and F, block by block.
+    * One core: the serial restatement (the reference's element-order scatter) on a Hexa8 48^3
+      sample, assembly and 300 PCG iterations, scaled to the full workload.
+    * The batch members batch_items = [(index, c, e, E, nu, faces, trac, dofs)] solved by the
+      oracle, for the batch's error against the oracle.
+    Returns a dict; ["u"] is the oracle solution."""
     from oracle import oracle as O
     import lmtpp_amd as L
     from problems import clamp_and_traction, lognormal_E
     O.build()
-    c, e = L.make_rect(L.HEXA, (sample_n + 1,) * 3)
-    E = lognormal_E(len(e), 0.5, seed=0)
-    faces, tr, dofs = clamp_and_traction(L.HEXA, c, e)
-    pat = O.pattern(O.HEXA, len(c), e)
+    n_cpu, model = host_cpu()
+    threads = O.set_threads(min(16, n_cpu))
+    out = {"threads": threads, "host_cpus": n_cpu, "cpu_model": model,
+           "build": "gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"}
+    pat = O.pattern(kind, len(c), e)
     t0 = time.perf_counter()
-    rp, col, vals, F, _ = O.assemble(O.HEXA, c, e, E, 0.3, faces=faces, face_trac=tr, pat=pat)
-    O.penalty(rp, col, vals, F, dofs, None, 1e5)
-    t_asm = time.perf_counter() - t0
-    its = 500
+    rp, col, vals, F, rc = O.assemble(kind, c, e, E, 0.3, faces=faces, face_trac=tr, pat=pat)
+    M = O.penalty(rp, col, vals, F, dofs, None, 1e5)
+    out["assembly_s"] = time.perf_counter() - t0
+    assert rc == 0
+    x, it, rr, rc, hist = O.pcg_jacobi_hist(rp, col, vals, F, rel_tol=1e-12, block=True, max_iter=20000)
+    out["pcg_iters"], out["pcg_s"], out["pcg_status"] = it, float(hist[-1, 1]) if len(hist) else 0.0, rc
+    out["hist"] = hist
+    out["true_rel_res"] = float(np.linalg.norm(F - O.spmv(rp, col, vals, x)) / np.linalg.norm(F))
+    out["u"] = x
+    log("CPU leg (%d threads, %s): oracle assembly %.2f s, block-Jacobi PCG %d iterations to 1e-12 in %.1f s, "
+        "true rel. res %.2e" % (threads, model, out["assembly_s"], it, out["pcg_s"], out["true_rel_res"]))
+    # the GPU's assembled K and F against the oracle's, every block
+    grp, gcol, gvals = f.get_mat()
+    same = np.array_equal(grp, rp) and np.array_equal(gcol, col)
+    # each value's difference over max(max|diag K| before the constraints, |value|): the matrix
+    # scale, or the value's own size on the penalty-carrying diagonal entries
+    dk = 0.0
+    step = 1 << 22
+    for i in range(0, len(col), step):
+        ov = vals[i:i + step]
+        dk = max(dk, float((np.abs(gvals[i:i + step] - ov) / np.maximum(M, np.abs(ov))).max()))
+    out["K_vs_oracle"] = {"pattern_equal": bool(same), "blocks": int(len(col)),
+                          "max_rel_diff": dk,
+                          "F_max_abs_diff_over_max_F": float(np.abs(f.get_sollicitation() - F).max() / np.abs(F).max())}
+    del grp, gcol, gvals, vals
+    log("GPU K vs oracle K: pattern %s, max |dK| / max(max|diag K|, |K|) = %.2e" % (
+        "equal" if same else "DIFFERS", out["K_vs_oracle"]["max_rel_diff"]))
+    # batch members
+    out["batch"] = []
+    for i, bc, be, bE, bnu, bfaces, btr, bdofs in batch_items:
+        brp, bcol, bvals, bF, _ = O.assemble(L.HEXA, bc, be, bE, bnu, faces=bfaces, face_trac=btr)
+        O.penalty(brp, bcol, bvals, bF, bdofs, None, 1e5)
+        bx, _, _, brc = O.pcg_jacobi(brp, bcol, bvals, bF, rel_tol=1e-12, block=True)
+        out["batch"].append((i, bx, brc))
+    # one core, serial restatement, bounded sample
+    O.set_threads(1)
+    sn = 48
+    sc, se = L.make_rect(L.HEXA, (sn + 1,) * 3)
+    sE = lognormal_E(len(se), 0.5, seed=0)
+    sfaces, str_, sdofs = clamp_and_traction(L.HEXA, sc, se)
+    spat = O.pattern(O.HEXA, len(sc), se)
     t0 = time.perf_counter()
-    O.pcg_jacobi(rp, col, vals, F, rel_tol=0.0, max_iter=its, block=True)
-    t_it = (time.perf_counter() - t0) / its
-    s_ndof = 3 * len(c)
-    t_total = t_asm * n_elem / len(e) + t_it * (ndof / s_ndof) * fp64_iters
-    return {"value": round(ndof / t_total / 1e6, 5), "unit": "MDOF/s", "cores": 1, "kind": "port",
-            "sample": ("oracle/fem_oracle.c (gcc -O2, 1 thread) on Hexa8 %d^3 (%d elements, %d DOF): assembly %.2f s and "
-                       "%.4f s per block-Jacobi PCG iteration (mean of %d); scaled to %d elements and %d DOF x %d FP64 iterations "
-                       "(the count the GPU needed for 1e-8) = %.1f s" % (sample_n, len(e), s_ndof, t_asm, t_it, its, n_elem,
-                                                                       ndof, fp64_iters, t_total)),
-            "est_total_s": round(t_total, 2)}
+    srp, scol, svals, sF, _ = O.assemble(O.HEXA, sc, se, sE, 0.3, faces=sfaces, face_trac=str_, pat=spat)
+    O.penalty(srp, scol, svals, sF, sdofs, None, 1e5)
+    t_asm1 = time.perf_counter() - t0
+    its = 300
+    t0 = time.perf_counter()
+    O.pcg_jacobi(srp, scol, svals, sF, rel_tol=0.0, max_iter=its, block=True)
+    out["one_core"] = {"sample": "Hexa8 %d^3 (%d elements, %d DOF)" % (sn, len(se), 3 * len(sc)),
+                       "assembly_s": t_asm1, "s_per_iter": (time.perf_counter() - t0) / its,
+                       "elements": len(se), "dof": 3 * len(sc)}
+    return out
 
 
-def calibrated_solves(f, ndof, log_tag, rel_ref=1e-13):
-    """FP64 reference solve, then the largest relative-residual tolerance whose FP64 solution is
-    within 0.5e-8 of it, then the MIXED tolerance reaching 1e-8 (tightened by 3x if needed)."""
+def cpu_baseline(cpu, tol, ndof, n_elem):
+    """cpu_baseline from the CPU leg: the all-core oracle's assembly plus its PCG time to the
+    relative residual tol the GPU headline solves to, on the full workload."""
+    hist = cpu["hist"]
+    k = int(np.argmax(hist[:, 0] <= tol)) if np.any(hist[:, 0] <= tol) else len(hist) - 1
+    t_solve = float(hist[k, 1])
+    t_total = cpu["assembly_s"] + t_solve
+    one = cpu["one_core"]
+    t_one = one["assembly_s"] * n_elem / one["elements"] + one["s_per_iter"] * (ndof / one["dof"]) * (k + 1)
+    return {"value": round(ndof / t_total / 1e6, 4), "unit": "MDOF/s", "cores": cpu["threads"], "kind": "port",
+            "sample": ("the whole workload (%d elements, %d DOF) on %d host threads (%s, %d CPUs visible): "
+                       "oracle/fem_oracle.c (%s), coloured element loop %.2f s + block-Jacobi PCG to rel. res. %.0e "
+                       "in %d iterations, %.2f s (from the PCG's per-iteration clock)" % (
+                           n_elem, ndof, cpu["threads"], cpu["cpu_model"], cpu["host_cpus"], cpu["build"],
+                           cpu["assembly_s"], tol, k + 1, t_solve)),
+            "total_s": round(t_total, 3),
+            "one_core": {"value": round(ndof / t_one / 1e6, 5), "unit": "MDOF/s", "cores": 1,
+                         "sample": ("serial restatement (element-order scatter, as the reference's default "
+                                    "nb_threads_assemble_matrix = 1) on %s: assembly %.2f s, %.4f s per PCG iteration "
+                                    "(300 timed), scaled to the full workload and %d iterations = %.1f s" % (
+                                        one["sample"], one["assembly_s"], one["s_per_iter"], k + 1, t_one)),
+                         "est_total_s": round(t_one, 2)}}
+
+
+def calibrated_solves(f, ndof, log_tag, rel_ref=1e-11):
+    """FP64 reference solve (to rel_ref, or to where FP64 stagnates above it), then the largest
+    relative-residual tolerance whose FP64 solution is within 0.5e-8 of it, then the MIXED
+    tolerance reaching 1e-8 (tightened by 3x if needed)."""
     import lmtpp_amd as L
+    from lmtpp_amd import _capi as A
     f.precision = L.FP64
     f.set_result(np.zeros(ndof))
-    assert f.solve_system(rel_tol=rel_ref), f.last_stats.status
+    ok = f.solve_system(rel_tol=rel_ref)
+    assert ok or f.last_stats.status == A.STAGNATED, f.last_stats.status
+    log("%s: FP64 reference %d iters, status %d, rel res %.2e" % (log_tag, f.last_stats.iters, f.last_stats.status,
+                                                                  f.last_stats.rel_res))
     u_ref = f.get_result()
     nref = np.linalg.norm(u_ref)
 
@@ -137,7 +231,8 @@ def config0():
             "max_abs_res": st.max_abs_res,
             "block_ic0_natural": {"ok": bool(ok_ic), "iters": st_ic.iters, "solve_ms": round(st_ic.solve_ms, 3),
                                   "factor_ms": round(setup_ic, 3), "max_abs_res": st_ic.max_abs_res},
-            "reference_ic0_pcg_iters": 47}
+            "reference_ic0_pcg_iters": int(np.load(os.path.join(ROOT, "tests", "golden",
+                                                               "ref_tinyblock_pcg_tetra11.npz"))["iters"])}
 
 
 def config2():
@@ -180,39 +275,17 @@ def config2():
                              "mixed_solve_ms": round(bstm.solve_ms, 2), "mixed_iters": bstm.iters, "rel_err_mixed": berrm}}
 
 
-def box_filter27(z, n, passes=3):
-    """3 passes of the 27-point box filter on an n^3 cell field, zero-flux (replicated) edges."""
-    a = z.reshape(n, n, n)
-    for _ in range(passes):
-        p = np.pad(a, 1, mode="edge")
-        acc = np.zeros_like(a)
-        for dz in range(3):
-            for dy in range(3):
-                for dx in range(3):
-                    acc += p[dz:dz + n, dy:dy + n, dx:dx + n]
-        a = acc / 27.0
-    return a.reshape(-1)
-
-
 def config3(steps=50):
     """Adjoint identification: Hexa8 64^3, 262,144 per-element moduli; Adam on log E with
     gradients from femgpu_adjoint (one forward + one adjoint solve per step)."""
     import lmtpp_amd as L
-    from problems import build
-    n = 64
-    c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
-    rng = np.random.default_rng(3)
-    zt = box_filter27(rng.standard_normal(len(e)), n)
-    zt = zt / zt.std()
-    E_true = 210000.0 * np.exp(0.3 * zt)
+    from problems import build, config3 as config3_problem
+    c, e, E_true, odofs, _ = config3_problem(n=64)
     f, *_ = build(L.HEXA, c, e, E_true)
     f.assemble()
     f.autotune()                             # product variant for this mesh (matrix-free on Hexa)
-    assert f.solve(rel_tol=1e-12)
+    assert f.solve(rel_tol=1e-11)
     u_true = f.get_result()
-    top = np.where(np.abs(c[:, 2] - 1.0) < 1e-9)[0]
-    obs = rng.choice(top, size=int(0.05 * len(top)), replace=False)
-    odofs = (3 * obs[:, None] + np.arange(3)).ravel()
     u_meas = u_true[odofs]
     logE = np.full(len(e), np.log(210000.0))
     m1 = np.zeros_like(logE)
@@ -248,10 +321,9 @@ def config3(steps=50):
             "rel_err_E_first": None, "rel_err_E_last": float(np.linalg.norm(np.exp(logE) - E_true) / np.linalg.norm(E_true))}
 
 
-def config4(n_models=256):
-    """256 independent Hexa8 cubes of uneven size, assembled and solved (MIXED) in one batch."""
+def config4_inputs(n_models=256):
+    """The 256 cubes of the batch: (meshes [(coords, conn)], items [(E, nu, traction)])."""
     import lmtpp_amd as L
-    from problems import clamp_and_traction
     rng = np.random.default_rng(4)
     meshes, items = [], []
     for i in range(n_models):
@@ -262,6 +334,27 @@ def config4(n_models=256):
         tr = rng.standard_normal(3)
         meshes.append((c, e))
         items.append((E, nu, tr))
+    return meshes, items
+
+
+def config4_oracle_members(meshes, items, k=8, seed=5):
+    """k seeded members of the batch with their clamp and traction, for the oracle's solves."""
+    import lmtpp_amd as L
+    from problems import clamp_and_traction
+    pick = np.random.default_rng(seed).choice(len(meshes), size=k, replace=False)
+    out = []
+    for i in sorted(int(v) for v in pick):
+        (c, e), (E, nu, tr) = meshes[i], items[i]
+        faces, trs, dofs = clamp_and_traction(L.HEXA, c, e, tr)
+        out.append((i, c, e, E, nu, faces, trs, dofs))
+    return out
+
+
+def config4(meshes, items, oracle_refs=()):
+    """256 independent Hexa8 cubes of uneven size, assembled and solved (MIXED) in one batch."""
+    import lmtpp_amd as L
+    from problems import clamp_and_traction
+    n_models = len(meshes)
     t0 = time.perf_counter()
     b = L.Batch(L.HEXA, meshes)
     setup = time.perf_counter() - t0
@@ -276,7 +369,7 @@ def config4(n_models=256):
     b.assemble()
     # FP64 references (tight), then the timed MIXED batch solve
     b.clear_result()
-    ok, _ = b.solve(rel_tol=1e-13, precision=L.FP64, max_iter=40000)
+    ok, _ = b.solve(rel_tol=1e-11, precision=L.FP64, max_iter=40000)
     refs = [b.get_result(i) for i in range(n_models)]
     t_asm = []
     for _ in range(2):
@@ -308,6 +401,7 @@ def config4(n_models=256):
         ok, ms = b.solve(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI, max_iter=40000)
         times.append((time.perf_counter() - t0, ms))
     errs = [np.linalg.norm(b.get_result(i) - refs[i]) / np.linalg.norm(refs[i]) for i in range(n_models)]
+    oerr = {int(i): float(np.linalg.norm(b.get_result(i) - x) / np.linalg.norm(x)) for i, x, rc in oracle_refs if rc == 0}
     ms = float(np.median([t[1] for t in times]))
     iters = [st.iters for st in b.last_stats]
     return {"workload": "256 Hexa8 cubes, side ~ U{8..32} cells (seed 4), lognormal E (sigma 0.5), nu ~ U[0.2,0.4], "
@@ -316,6 +410,7 @@ def config4(n_models=256):
             "models": n_models, "total_dof": ndof, "batch_setup_s": round(setup, 2),
             "assembly_ms": round(float(np.median(t_asm)), 3), "solve_ms": round(ms, 2),
             "aggregate_mdof_s": round(ndof / (ms * 1e-3) / 1e6, 3), "max_rel_err_vs_fp64": float(max(errs)),
+            "rel_err_vs_oracle": oerr, "max_rel_err_vs_oracle": max(oerr.values()) if oerr else None,
             "median_rel_err": float(np.median(errs)), "iters_min": int(min(iters)), "iters_max": int(max(iters)),
             "all_ok": bool(ok), "product": {2: "stored, nontemporal_col16", 1: "stored, nontemporal", 0: "stored, plain",
                                             -1: "matrix-free"}[best_nt],
@@ -358,7 +453,7 @@ def main():
     log("setup %.1f s: %d elem, %d nodes, %d DOF, %d blocks, %d colours" % (
         time.perf_counter() - t0, sz["n_elem"], sz["n_nodes"], ndof, sz["nnzb"], sz["n_colors"]))
 
-    # -- calibration (untimed) ---------------------------------------------------------------
+    # -- references (untimed) ------------------------------------------------------------------
     f.assemble()
     spmv_cfg = f.autotune()
     fmt_name = {0: "full", 1: "sym", 2: "matrix-free"}
@@ -368,11 +463,30 @@ def main():
     f.precision, f.precond = L.FP64, L.BLOCK_JACOBI
     f.max_iter = 40000
     f.set_result(np.zeros(ndof))
-    assert f.solve_system(rel_tol=1e-13), f.last_stats.status
-    u_ref = f.get_result()
-    ref_iters = f.last_stats.iters
-    log("FP64 reference solve: %d iters, rel res %.2e, %.1f ms" % (ref_iters, f.last_stats.rel_res,
-                                                                     f.last_stats.solve_ms))
+    gpu_ref_tol = 3e-11
+    assert f.solve_system(rel_tol=gpu_ref_tol), f.last_stats.status
+    assert f.last_stats.rel_res <= gpu_ref_tol
+    u_gpu64 = f.get_result()
+    gpu_ref = {"rel_tol": gpu_ref_tol, "iters": int(f.last_stats.iters), "rel_res": f.last_stats.rel_res,
+               "true_residual_checks": int(f.last_stats.refreshes), "solve_ms": round(f.last_stats.solve_ms, 2)}
+    log("FP64 GPU reference solve: %d iters, true rel res %.2e (tol %.0e), %.1f ms" % (
+        f.last_stats.iters, f.last_stats.rel_res, gpu_ref_tol, f.last_stats.solve_ms))
+    # the CPU leg: the oracle on the host cores, whose solution is the FP64 oracle solution
+    wanted = set() if args.no_configs else set(args.configs.split(","))
+    c4 = config4_inputs() if "4" in wanted else None
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_leg(kind, c, e, E, faces, tr, dofs, f,
+                          config4_oracle_members(*c4) if c4 else ())
+        except Exception as ex:  # the baseline is reported, never the measured value
+            log("CPU leg failed: %r" % (ex,))
+    if cpu is not None:
+        u_ref, ref_name = cpu["u"], "FP64 oracle (oracle/fem_oracle.c block-Jacobi PCG to 1e-12, host CPU)"
+        gpu_ref["rel_err_vs_oracle"] = float(np.linalg.norm(u_gpu64 - u_ref) / np.linalg.norm(u_ref))
+        log("FP64 GPU reference vs oracle: rel. err %.2e" % gpu_ref["rel_err_vs_oracle"])
+    else:
+        u_ref, ref_name = u_gpu64, "FP64 GPU solve to a true relative residual of %.0e" % gpu_ref_tol
     nref = np.linalg.norm(u_ref)
 
     def solve(prec, tol, precond=L.BLOCK_JACOBI):
@@ -554,7 +668,8 @@ def main():
         "solve_mdof_s_fp64": round(ndof / (np.median(fp64_ms) * 1e-3) / 1e6, 3),
         "assembly_ms": round(float(np.median(asm_ms)), 3),
         "assembly_melem_s": round(sz["n_elem"] / (np.median(asm_ms) * 1e-3) / 1e6, 3),
-        "rel_err_mixed": float(err_timed), "rel_err_fp64": float(err64),
+        "rel_err_mixed": float(err_timed), "rel_err_fp64": float(err64), "rel_err_against": ref_name,
+        "fp64_gpu_reference": gpu_ref,
         "rel_res_tol": mtol, "iters_mixed": int(stm.iters), "refreshes_mixed": int(stm.refreshes),
         "iters_fp64": int(st64.iters),
         "spmv_formats": {"fp64": fmt_name[spmv_cfg["fp64"][0]], "fp32": fmt_name[spmv_cfg["fp32"][0]]},
@@ -579,10 +694,11 @@ def main():
     out["roofline_stored_spmv"]["frac_of_measured_copy"] = round(ach32 / gbs, 4)
     out["roofline_stored_spmv"]["frac_of_measured_read"] = round(ach32 / rgbs, 4)
     f.close()
-    if not args.no_configs:
+    if wanted:
         out["configs"] = {}
-        wanted = set(args.configs.split(","))
-        for name, fn in (("0", config0), ("2", config2), ("3", config3), ("4", config4)):
+        oracle_refs = cpu["batch"] if cpu is not None else ()
+        for name, fn in (("0", config0), ("2", config2), ("3", config3),
+                         ("4", lambda: config4(c4[0], c4[1], oracle_refs))):
             if name not in wanted:
                 continue
             try:
@@ -592,12 +708,12 @@ def main():
             except Exception as ex:  # a secondary configuration never hides the headline line
                 log("config %s failed: %r" % (name, ex))
                 out["configs"][name] = {"error": repr(ex)}
-    if not args.no_cpu:
-        try:
-            out["cpu_baseline"] = cpu_baseline(st64.iters, ndof, sz["n_elem"])
-        except Exception as ex:  # the baseline is reported, never the measured value
-            log("cpu baseline failed: %r" % (ex,))
-            out["cpu_baseline"] = None
+    if cpu is not None:
+        out["cpu_baseline"] = cpu_baseline(cpu, mtol, ndof, sz["n_elem"])
+        out["oracle_check"] = {"true_rel_res": cpu["true_rel_res"], "pcg_iters": cpu["pcg_iters"],
+                               "K_vs_oracle": cpu["K_vs_oracle"]}
+    else:
+        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 

@@ -367,6 +367,19 @@ typedef enum { FEMGPU_SPMV_FULL = 0, FEMGPU_SPMV_SYM = 1, FEMGPU_SPMV_MATRIX_FRE
  * FEMGPU_FP64 or the FP32 one (x rounded to float, y widened): the operator the solver iterates
  * with, as Mat * Vec computes the stored matrix's product (containers/MatWithTinyBlocks.h:303-378). */
 FEMGPU_API int femgpu_apply(femgpu_problem *p, int precision, const double *x, double *y, size_t n);
+/* No counterpart in the reference: femgpu_apply without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  d_y = K d_x on device pointers of the problem's GPU, n = 3 * n_nodes values each,
+ * in the caller's node order, with the product variant the solver uses for precision (FP64, or
+ * MIXED for the FP32 one: x rounded to float, the result widened).  Every temporary lives in a
+ * static 64 MiB device array per GPU: the call only enqueues kernels on the problem's stream
+ * (synchronise it before reading d_y) and never synchronises or allocates.  FEMGPU_NOT_READY until
+ * the product variant was built for the current matrix (by femgpu_apply or a solve of that
+ * precision, which allocate); FEMGPU_BAD_INPUT when the product needs more scratch than the array
+ * holds (the FP32 copies of x and y, 24 n bytes in all, the matrix-free product's shared-node
+ * partials, and 16 n bytes more with an internal node order).  Calls on one GPU share the array:
+ * calls that overlap on one GPU race. */
+FEMGPU_API int femgpu_apply_noalloc(femgpu_problem *p, int precision, const double *d_x, double *d_y);
 /* The matrix-free product's layout: info[0] element blocks (256 element slots each, sorted along a
  * Morton curve), [1] sub-colours, [2] block-local node entries, [3] nodes shared by several blocks,
  * [4] their partial slots, [5] nodes of no element, [6] elements that are axis-aligned boxes (their

@@ -77,6 +77,7 @@ _SIGS = {
     "femgpu_set_result": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_get_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_set_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
+    "femgpu_apply_noalloc": (C.c_int, [_P, C.c_int, _P, _P]),
     "femgpu_get_matrix": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_sizes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint32)]),

@@ -271,6 +271,13 @@ double time_spmv(Problem &P, int precision, int reps);
 // d_y = K d_x (device, internal order) with the product variant of precision 0 (FP64) or 1 (FP32);
 // returns 0, 1 (the variant cannot serve this problem) or 5.
 int apply_spmv(Problem &P, int precision, const double *d_x, double *d_y);
+// Bytes of scratch apply_spmv_scratch needs for precision (FP32 copies of x and y, the matrix-free
+// product's shared-node partials), or -1 while the product variant's data is not built for the
+// current K (building it allocates: apply_spmv or a solve builds it).
+int64_t apply_scratch_bytes(const Problem &P, int precision);
+// apply_spmv with every temporary in the caller's scratch (apply_scratch_bytes of it, 256-byte
+// aligned); allocates nothing and does not synchronise.  Returns 0 or 5.
+int apply_spmv_scratch(Problem &P, int precision, const double *d_x, double *d_y, char *scratch);
 // Preconditioner precond (femgpu_precond) of the current K with its FP32 copy, as get_precond
 // does before the solves that reuse it; returns 0, 1 (bad input), 3 (non-positive pivot) or 5.
 int precond_build(Problem &P, int precond);
@@ -295,8 +302,14 @@ hipError_t mf_snapshot(Problem &P);
 int mf_prepare(Problem &P);
 // y = K x (pq: on the active sub-problem, with the p.q partials in mf.d_partials).  With z
 // (pq only): y = K p for the CG direction p = z + beta x (beta of the scalars), stored into x.
-hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z = nullptr);
-hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z = nullptr);
+// halo: where the shared nodes' partials go (3 values of T per slot); null: the problem's buffer.
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z = nullptr, void *halo = nullptr);
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z = nullptr, void *halo = nullptr);
+// The product's arrays are built for the current matrix assembly (mf_prepare has nothing left to
+// allocate or upload for a plain product).
+bool mf_ready(const Problem &P);
+// Values in the shared-node partial slots (3 per slot).
+size_t mf_halo_values(const Problem &P);
 void mf_free(MatFree &M);
 // Device time (ms) of the element-block kernel alone and of the whole product; returns 0, 1 or 5.
 int mf_time(Problem &P, int precision, int reps, double *block_ms, double *product_ms);

@@ -1098,32 +1098,33 @@ int mf_prepare(Problem &P) {
 // Element-block kernel of one variant (BOXES: every element of the mesh is an axis-aligned box;
 // Tetra_10 problems take the Tetra_10 kernel).
 template <typename T, bool PQ, bool DIR, bool BOXES>
-static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y) {
+static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo) {
     const MatFree &M = P.mf;
     const void *rec = sizeof(T) == 4 ? M.d_rec32 : M.d_rec64;
 #define FEMGPU_MF_BLOCK(KIND)                                                                                         \
     hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND>), dim3(M.n_blocks), dim3(MF_BLOCK),        \
                        mf_lds_bytes(M.ls, sizeof(T)), P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.d_bsub,      \
                        M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
-                       P.d_coords, M.d_E, M.d_mat, rec, x, xb, z, y, reinterpret_cast<T *>(M.d_halo), P.d_ps,          \
+                       P.d_coords, M.d_E, M.d_mat, rec, x, xb, z, y, halo, P.d_ps,                                      \
                        M.d_partials)
     if (P.kind == TETRA10) FEMGPU_MF_BLOCK(TETRA10);
     else FEMGPU_MF_BLOCK(HEXA);
 #undef FEMGPU_MF_BLOCK
 }
 
+// halo: the shared-node partial slots (3 values of T each), the problem's own buffer or a caller's
 template <typename T, bool PQ, bool DIR>
-static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y) {
+static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo) {
     const MatFree &M = P.mf;
-    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, xb, z, y);
-    else mf_block_launch<T, PQ, DIR, false>(P, x, xb, z, y);
+    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, xb, z, y, halo);
+    else mf_block_launch<T, PQ, DIR, false>(P, x, xb, z, y, halo);
     if (M.n_iso)
         hipLaunchKernelGGL((k_mf_iso<T, PQ, DIR>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,
                            M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, P.d_ps, M.d_partials, M.n_blocks);
     // DIR: the iteration's vector update sums the shared nodes' partials itself (k_update_mf)
     if (M.n_bnodes && !DIR)
         hipLaunchKernelGGL(k_mf_halo<T>, dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes, M.d_bnode,
-                           M.d_bptr, reinterpret_cast<const T *>(M.d_halo), y);
+                           M.d_bptr, halo, y);
 }
 
 // y = K x; with z (PQ only), y = K p for the updated direction p = z + beta s, s the stored
@@ -1132,11 +1133,12 @@ static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y) {
 // product's passes); then y holds the nodes of no shared block only, and the shared nodes'
 // partials are left to the caller's vector update (k_update_mf).
 template <typename T>
-static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z) {
+static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z, void *halo) {
     T *xb = reinterpret_cast<T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
-    if (pq && z) mf_passes<T, true, true>(P, x, xb, z, y);
-    else if (pq) mf_passes<T, true, false>(P, x, nullptr, nullptr, y);
-    else mf_passes<T, false, false>(P, x, nullptr, nullptr, y);
+    T *h = reinterpret_cast<T *>(halo ? halo : (void *)P.mf.d_halo);
+    if (pq && z) mf_passes<T, true, true>(P, x, xb, z, y, h);
+    else if (pq) mf_passes<T, true, false>(P, x, nullptr, nullptr, y, h);
+    else mf_passes<T, false, false>(P, x, nullptr, nullptr, y, h);
     // peek, not get: a failed launch stays pending for the caller's own check (the solver
     // tests hipGetLastError after each chunk and at the end of the solve)
     return hipPeekAtLastError();
@@ -1150,16 +1152,18 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     const MatFree &M = P.mf;
     auto block = [&]() {
         if (precision) {
-            if (M.all_boxes) mf_block_launch<float, false, false, true>(P, P.d_p32, nullptr, nullptr, P.d_q32);
-            else mf_block_launch<float, false, false, false>(P, P.d_p32, nullptr, nullptr, P.d_q32);
+            float *h = reinterpret_cast<float *>(M.d_halo);
+            if (M.all_boxes) mf_block_launch<float, false, false, true>(P, P.d_p32, nullptr, nullptr, P.d_q32, h);
+            else mf_block_launch<float, false, false, false>(P, P.d_p32, nullptr, nullptr, P.d_q32, h);
         } else {
-            if (M.all_boxes) mf_block_launch<double, false, false, true>(P, P.d_p, nullptr, nullptr, P.d_q);
-            else mf_block_launch<double, false, false, false>(P, P.d_p, nullptr, nullptr, P.d_q);
+            double *h = M.d_halo;
+            if (M.all_boxes) mf_block_launch<double, false, false, true>(P, P.d_p, nullptr, nullptr, P.d_q, h);
+            else mf_block_launch<double, false, false, false>(P, P.d_p, nullptr, nullptr, P.d_q, h);
         }
     };
     auto product = [&]() {
-        if (precision) (void)mf_launch_t<float>(P, P.d_p32, P.d_q32, false, nullptr);
-        else (void)mf_launch_t<double>(P, P.d_p, P.d_q, false, nullptr);
+        if (precision) (void)mf_launch_t<float>(P, P.d_p32, P.d_q32, false, nullptr, nullptr);
+        else (void)mf_launch_t<double>(P, P.d_p, P.d_q, false, nullptr, nullptr);
     };
     float ms[2] = {0, 0};
     product();
@@ -1176,11 +1180,18 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     return 0;
 }
 
-hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z) {
-    return mf_launch_t<float>(P, x, y, pq, z);
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z, void *halo) {
+    return mf_launch_t<float>(P, x, y, pq, z, halo);
 }
-hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z) {
-    return mf_launch_t<double>(P, x, y, pq, z);
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z, void *halo) {
+    return mf_launch_t<double>(P, x, y, pq, z, halo);
 }
+
+bool mf_ready(const Problem &P) {
+    const MatFree &M = P.mf;
+    return M.built && M.snap && M.rec_valid && M.pen_built && M.pen_dofs == M.snap_dofs;
+}
+
+size_t mf_halo_values(const Problem &P) { return 3 * (size_t)P.mf.n_halo; }
 
 }  // namespace femgpu

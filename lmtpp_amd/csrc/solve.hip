@@ -542,7 +542,7 @@ static WgTables tables(const Problem &P) {
 // zdir (matrix-free MODE_PQ only): the direction update p = z + beta x is fused into the product.
 template <typename T, int MODE>
 static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int sel,
-                          const T *zdir = nullptr) {
+                          const T *zdir = nullptr, void *halo = nullptr) {
     const dim3 g(P.n_wg), blk(256);
     const WgTables wt = tables(P);
     const int pr = sizeof(T) == 4 ? 1 : 0;
@@ -550,7 +550,7 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     // residual's partials
     if (P.spmv.format[pr] == FMT_MF) {
         // a launch error stays pending (mf_launch peeks) for run_chunk's and solve_impl's checks
-        (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr);
+        (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr, halo);
         if (MODE == MODE_RES)
             hipLaunchKernelGGL(k_residual, g, blk, 0, P.stream, b, reinterpret_cast<double *>(y), wt, P.d_ps, P.d_partials,
                                sel);
@@ -742,6 +742,35 @@ int apply_spmv(Problem &P, int precision, const double *d_x, double *d_y) {
         hipLaunchKernelGGL(k_vec_to_f64, g, blk, 0, P.stream, P.d_q32, d_y, n);
     } else {
         launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, d_x, nullptr, d_y, 0);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+int64_t apply_scratch_bytes(const Problem &P, int precision) {
+    const size_t n = 3 * (size_t)P.n_nodes;
+    size_t b = precision == 1 ? align256(2 * n * sizeof(float)) : 0;
+    switch (P.spmv.format[precision]) {
+        case FMT_MF:
+            if (!mf_ready(P)) return -1;
+            return (int64_t)(b + align256(mf_halo_values(P) * (precision == 1 ? sizeof(float) : sizeof(double))));
+        case FMT_SYM: return P.sym.valid[precision] ? (int64_t)b : -1;
+        default: return precision == 1 && !P.k32_valid ? -1 : (int64_t)b;
+    }
+}
+
+int apply_spmv_scratch(Problem &P, int precision, const double *d_x, double *d_y, char *scratch) {
+    const size_t n = 3 * (size_t)P.n_nodes;
+    const dim3 g((unsigned)std::max<size_t>(1, std::min<size_t>(8192, (n + 255) / 256))), blk(256);
+    if (precision == 1) {
+        float *x32 = reinterpret_cast<float *>(scratch), *y32 = x32 + n;
+        char *halo = scratch + align256(2 * n * sizeof(float));
+        hipLaunchKernelGGL(k_vec_to_f32, g, blk, 0, P.stream, d_x, x32, n);
+        launch_spmv_t<float, MODE_PLAIN>(P, P.d_K32, x32, nullptr, y32, 0, nullptr, halo);
+        hipLaunchKernelGGL(k_vec_to_f64, g, blk, 0, P.stream, y32, d_y, n);
+    } else {
+        launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, d_x, nullptr, d_y, 0, nullptr, scratch);
     }
     return hipGetLastError() == hipSuccess ? 0 : 5;
 }

@@ -418,6 +418,12 @@ FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2]
  * element's blocks and loads (plain writes), a second builds every stored position of K and every
  * node's load from its list of contributions, written once (buffers of the element blocks and the
  * lists, about 2.6 GB and 0.4 GB for a million Hexa8; element blocks are indexed in 32 bits).
+ * rows = FEMGPU_ASSEMBLY_GATHER_BOX (Hexa meshes whose elements are all axis-aligned boxes, of at
+ * most 4096 distinct edge-length triples per sub-problem, as every make_rect lattice): the gather
+ * with no stored element blocks -- the element matrix of a box is linear in its modulus, so each
+ * class of equal edge lengths gets its unit-modulus matrix from the generated per-point blocks
+ * once per assembly, and every position sums E_e times its class's blocks (K agrees with the
+ * other variants to rounding, 1e-15 of the matrix scale, not bitwise).
  * rows = R (a power of two, at most *max_rows): each workgroup owns R consecutive block rows,
  * accumulates their blocks from the incident elements colour by colour in LDS and writes the rows
  * once.  Every variant sums each entry in the same (colour) order: the colour passes, the staged
@@ -425,6 +431,7 @@ FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2]
  * femgpu_autotune times them all. */
 #define FEMGPU_ASSEMBLY_STAGED 0xffffffffu
 #define FEMGPU_ASSEMBLY_GATHER 0xfffffffeu
+#define FEMGPU_ASSEMBLY_GATHER_BOX 0xfffffffdu
 FEMGPU_API int femgpu_set_assembly_rows(femgpu_problem *p, uint32_t rows);
 FEMGPU_API int femgpu_get_assembly_rows(femgpu_problem *p, uint32_t *rows, uint32_t *max_rows);
 /* Replays each chunk of CG iterations from a captured hipGraph (on by default for batches). */

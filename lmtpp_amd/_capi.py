@@ -22,6 +22,7 @@ ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
 NODE_ORDER_NATURAL, NODE_ORDER_RCM = 0, 1
 ASSEMBLY_STAGED = 0xFFFFFFFF
 ASSEMBLY_GATHER = 0xFFFFFFFE
+ASSEMBLY_GATHER_BOX = 0xFFFFFFFD
 NPE = {TETRA: 4, TETRA10: 10, HEXA: 8}
 FACE_OF = {TETRA: TRI3, TETRA10: TRI6, HEXA: QUAD4}
 NPF = {TRI3: 3, TRI6: 6, QUAD4: 4}

@@ -33,6 +33,7 @@ struct ElemLds {
     double g[ET<K>::NGP][ET<K>::NPE][3];
     double wd[ET<K>::NGP];
     double N[ET<K>::NGP][ET<K>::NPE];
+    double ji[ET<K>::NGP][9];   // J^-1 at each Gauss point (ji[q][3 j + i] = Ji[j][i])
 };
 
 // Phase B alone, coordinates already in L.x (written before the barrier this function starts with).
@@ -47,14 +48,15 @@ __device__ inline void elem_geometry(ElemLds<K> &L, int t, bool valid, uint32_t 
     elem_gradients<K>(L, t, valid, flags);
 }
 
+// J, det J and J^-1 once per Gauss point (a lane each), then the gradient of every (point, node)
+// pair (a lane each).
 template <int K>
 __device__ inline void elem_gradients(ElemLds<K> &L, int t, bool valid, int *flags) {
     constexpr int NPE = ET<K>::NPE, NGP = ET<K>::NGP;
     __syncthreads();
-    if (valid && t < NGP * NPE) {
-        const int q = t / NPE, a = t % NPE;
+    if (valid && t < NGP) {
         double xi[3], w;
-        gen::vol<K>::gauss(q, xi, w);
+        gen::vol<K>::gauss(t, xi, w);
         double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         for (int b = 0; b < NPE; ++b) {
             double d[3];
@@ -64,13 +66,20 @@ __device__ inline void elem_gradients(ElemLds<K> &L, int t, bool valid, int *fla
         }
         double Ji[3][3];
         const double det = jac_inverse(J, Ji);
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) L.ji[t][3 * j + i] = Ji[j][i];
+        L.wd[t] = w * det;
+        if (!(det > 0)) atomicOr(flags, 1);
+    }
+    __syncthreads();
+    if (valid && t < NGP * NPE) {
+        const int q = t / NPE, a = t % NPE;
+        double xi[3], w;
+        gen::vol<K>::gauss(q, xi, w);
         double d[3];
         L.N[q][a] = gen::vol<K>::shape(a, xi, d);
-        for (int j = 0; j < 3; ++j) L.g[q][a][j] = Ji[j][0] * d[0] + Ji[j][1] * d[1] + Ji[j][2] * d[2];
-        if (a == 0) {
-            L.wd[q] = w * det;
-            if (!(det > 0)) atomicOr(flags, 1);
-        }
+        const double *Ji = L.ji[q];
+        for (int j = 0; j < 3; ++j) L.g[q][a][j] = Ji[3 * j] * d[0] + Ji[3 * j + 1] * d[1] + Ji[3 * j + 2] * d[2];
     }
     __syncthreads();
 }
@@ -203,6 +212,87 @@ __global__ __launch_bounds__(256) void k_gather_blocks(uint64_t ps, const uint32
 #pragma unroll
         for (int q = 0; q < 9; ++q) Kv[q * ps + p] = v[q];
     }
+}
+
+// Box classes (ASM_GATHER_BOX): the unit-modulus element matrix K_c(1) and the body loads of every
+// class of axis-aligned Hexa boxes, a team of 64 lanes per class, from a box at the origin with the
+// class's edge lengths through the same phases and generated per-point blocks as the element loop.
+__global__ __launch_bounds__(64) void k_box_ref(const double *__restrict__ ch, const uint32_t *__restrict__ csub,
+                                                const SubMaterial *__restrict__ mat, double *__restrict__ kref,
+                                                double *__restrict__ fref, int *flags) {
+    constexpr int NPE = 8, NGP = 8, NUP = NPE * (NPE + 1) / 2;
+    __shared__ ElemLds<HEXA> L;
+    const uint32_t c = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t < NPE * 3) {
+        // node a of element_Hexa.py at corner (x, y, z) = ((a + 1) / 2 % 2, a / 2 % 2, a / 4)
+        const int a = t / 3, i = t % 3;
+        const int bit = i == 0 ? ((a + 1) >> 1) & 1 : i == 1 ? (a >> 1) & 1 : a >> 2;
+        L.x[t] = bit ? ch[3 * (size_t)c + i] : 0.0;
+    }
+    elem_gradients<HEXA>(L, t, true, flags);
+    const SubMaterial m = mat[csub[c]];
+    if (t / NPE <= t % NPE) {
+        const int a = t / NPE, b = t % NPE;
+        double H[3];
+        gen::hooke(1.0, m.nu, H);
+        double k[9];
+        elem_block<HEXA>(L, a, b, H, k);
+        if (a == b) { k[3] = k[1]; k[6] = k[2]; k[7] = k[5]; }
+        double *dst = kref + ((size_t)c * NUP + upper_index<HEXA>(a, b)) * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dst[q] = k[q];
+    }
+    if (t < NPE * 3) {
+        const int a = t / 3, d = t % 3;
+        double f3[3] = {0, 0, 0};
+        for (int q = 0; q < NGP; ++q) gen::load(L.N[q][a], m.bf0, m.bf1, m.bf2, L.wd[q], f3);
+        fref[((size_t)c * NPE + a) * 3 + d] = f3[d];
+    }
+}
+
+// Pass 2 of ASM_GATHER_BOX: as k_gather_blocks, each entry's block being E_e times its class's
+// unit-modulus block (entry (e NUP + pair) << 1 | transposed).
+__global__ __launch_bounds__(256) void k_gather_blocks_box(uint64_t ps, const uint32_t *__restrict__ ptr,
+                                                           const uint32_t *__restrict__ ent, const double *__restrict__ E,
+                                                           const uint16_t *__restrict__ cls,
+                                                           const double *__restrict__ kref, double *__restrict__ Kv) {
+    constexpr uint32_t NUP = 36;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < ps; p += (uint64_t)gridDim.x * blockDim.x) {
+        double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t j = ptr[p]; j < ptr[p + 1]; ++j) {
+            const uint32_t c = ent[j], blk = c >> 1, e = blk / NUP, pair = blk - e * NUP;
+            const double Ee = E[e];
+            const double *r = kref + ((size_t)cls[e] * NUP + pair) * 9;
+            if (c & 1u) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] += Ee * r[3 * (q % 3) + q / 3];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] += Ee * r[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Kv[q * ps + p] = v[q];
+    }
+}
+
+// Loads of ASM_GATHER_BOX: node n sums its class loads (entry e NPE + a) in colour order.
+__global__ __launch_bounds__(256) void k_gather_loads_box(uint32_t n_nodes, const uint32_t *__restrict__ ptr,
+                                                          const uint32_t *__restrict__ ent,
+                                                          const uint16_t *__restrict__ cls,
+                                                          const double *__restrict__ fref, double *__restrict__ F) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= n_nodes) return;
+    double v[3] = {0, 0, 0};
+    for (uint32_t j = ptr[n]; j < ptr[n + 1]; ++j) {
+        const uint32_t c = ent[j];
+        const double *f = fref + ((size_t)cls[c >> 3] * 8 + (c & 7u)) * 3;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] += f[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[3 * (size_t)n + d] = v[d];
 }
 
 // The loads: node n sums the element loads ent[ptr[n] .. ptr[n + 1]) (entry e NPE + a), in colour order.
@@ -473,6 +563,19 @@ static hipError_t assemble_gather_kind(Problem &P, int do_mat, int do_vec) {
     return hipGetLastError();
 }
 
+static hipError_t assemble_gather_box(Problem &P, int do_mat, int do_vec) {
+    const GatherAsm &G = P.ga;
+    hipLaunchKernelGGL(k_box_ref, dim3(G.n_classes), dim3(64), 0, P.stream, G.d_ch, G.d_csub, P.d_mat, G.d_kref, G.d_fref,
+                       P.d_flags);
+    if (do_mat)
+        hipLaunchKernelGGL(k_gather_blocks_box, dim3(8192), dim3(256), 0, P.stream, P.ps, G.d_kptr, G.d_kent, P.d_E,
+                           G.d_cls, G.d_kref, P.d_K);
+    if (do_vec)
+        hipLaunchKernelGGL(k_gather_loads_box, dim3((P.n_nodes + 255) / 256), dim3(256), 0, P.stream, P.n_nodes, G.d_fptr,
+                           G.d_fent, G.d_cls, G.d_fref, P.d_F);
+    return hipGetLastError();
+}
+
 template <int K>
 static void assemble_rows_kind(Problem &P, int do_mat, int do_vec) {
     hipLaunchKernelGGL(k_assemble_rows<K>, dim3(P.rg.n_groups), dim3(ASM_THREADS), 0, P.stream, P.rg.R, P.n_nodes,
@@ -510,6 +613,8 @@ hipError_t launch_assemble(Problem &P, int do_mat, int do_vec) {
     if (!do_mat && !body_force) {
         // right-hand side only, no body force: every element contribution is exactly zero
         if (do_vec) e = launch_fill(P.d_F, 3 * (size_t)P.n_nodes, 0.0, P.stream);
+    } else if (P.asm_rows == ASM_GATHER_BOX && P.ga.box) {
+        e = assemble_gather_box(P, do_mat, do_vec);
     } else if (P.asm_rows == ASM_GATHER && P.ga.d_em) {
         if (P.kind == TETRA) e = assemble_gather_kind<TETRA>(P, do_mat, do_vec);
         else if (P.kind == TETRA10) e = assemble_gather_kind<TETRA10>(P, do_mat, do_vec);

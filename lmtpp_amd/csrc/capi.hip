@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <tuple>
 #include <new>
 
 #include "../../include/femgpu.h"
@@ -53,6 +55,8 @@ void dfree(T *&p) {
 
 void free_gather(GatherAsm &G) {
     dfree(G.d_em); dfree(G.d_ef); dfree(G.d_kptr); dfree(G.d_kent); dfree(G.d_fptr); dfree(G.d_fent);
+    dfree(G.d_cls); dfree(G.d_ch); dfree(G.d_csub); dfree(G.d_kref); dfree(G.d_fref);
+    G = GatherAsm();
 }
 
 void free_problem(Problem &P) {
@@ -100,11 +104,48 @@ void build_wg_tables(Problem &P, std::vector<uint32_t> &row0, std::vector<uint32
     P.n_wg = (uint32_t)row0.size();
 }
 
+// Classes of ASM_GATHER_BOX: every element an axis-aligned Hexa box (else 1), classed by its exact
+// edge lengths and its sub-problem, at most MAX_BOX_CLASSES classes (else 1).
+static int box_classes(const Problem &P, std::vector<uint16_t> &cls, std::vector<double> &ch, std::vector<uint32_t> &csub) {
+    if (P.kind != HEXA || P.n_elem == 0) return 1;
+    std::vector<double> X(3 * (size_t)P.n_nodes);
+    if (hipMemcpy(X.data(), P.d_coords, X.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return 5;
+    std::map<std::tuple<double, double, double, uint32_t>, uint16_t> ids;
+    cls.resize(P.n_elem);
+    uint32_t s = 0;
+    for (uint32_t e = 0; e < P.n_elem; ++e) {
+        while (e >= P.elem_off[s + 1]) ++s;
+        const uint32_t *nd = P.conn_h.data() + 8 * (size_t)e;
+        if (!hexa_is_box(X.data(), nd)) return 1;
+        const double *x0 = &X[3 * (size_t)nd[0]], *x6 = &X[3 * (size_t)nd[6]];
+        const auto key = std::make_tuple(x6[0] - x0[0], x6[1] - x0[1], x6[2] - x0[2], s);
+        auto it = ids.find(key);
+        if (it == ids.end()) {
+            if (ids.size() >= MAX_BOX_CLASSES) return 1;
+            it = ids.emplace(key, (uint16_t)ids.size()).first;
+            ch.insert(ch.end(), {std::get<0>(key), std::get<1>(key), std::get<2>(key)});
+            csub.push_back(s);
+        }
+        cls[e] = it->second;
+    }
+    return 0;
+}
+
 // The gather lists of ASM_GATHER (host, once per problem): every (element, a, b) block goes to
 // the list of its position, every (element, a) load to the list of its node, walking the elements
-// in colour order, so each list is in the order the colour passes add.
-static int build_gather(Problem &P) {
-    if (P.ga.d_em) return FEMGPU_OK;
+// in colour order, so each list is in the order the colour passes add.  box: the lists of
+// ASM_GATHER_BOX, with the element classes instead of the stored element blocks.
+static int build_gather(Problem &P, bool box) {
+    if (box ? P.ga.box : (P.ga.d_em != nullptr)) return FEMGPU_OK;
+    std::vector<uint16_t> cls;
+    std::vector<double> ch;
+    std::vector<uint32_t> csub;
+    if (box) {
+        const int rc = box_classes(P, cls, ch, csub);
+        if (rc == 1) return fail(FEMGPU_BAD_INPUT, "assembly: the box gather variant needs Hexa elements that are all "
+                                                   "axis-aligned boxes, of at most 4096 shapes");
+        if (rc) return fail(FEMGPU_HIP_ERROR, "assembly: box classes");
+    }
     const uint32_t npe = (uint32_t)P.npe, nup = npe * (npe + 1) / 2;
     if ((uint64_t)P.n_elem * nup * 2 >= (1ull << 32) || (uint64_t)P.n_elem * npe * npe >= (1ull << 32) ||
         P.ps >= (1ull << 32) || P.ecol.order.size() != P.n_elem)
@@ -128,10 +169,21 @@ static int build_gather(Problem &P) {
             fent[ff[conn[(size_t)e * npe + a]]++] = e * npe + a;
         }
     }
+    free_gather(P.ga);
     GatherAsm G;
-    if (dalloc(&G.d_em, (size_t)P.n_elem * nup * 9) != hipSuccess || dalloc(&G.d_ef, (size_t)P.n_elem * npe * 3) != hipSuccess ||
-        upload(&G.d_kptr, kptr.data(), kptr.size()) != hipSuccess || upload(&G.d_kent, kent.data(), kent.size()) != hipSuccess ||
-        upload(&G.d_fptr, fptr.data(), fptr.size()) != hipSuccess || upload(&G.d_fent, fent.data(), fent.size()) != hipSuccess) {
+    bool ok = upload(&G.d_kptr, kptr.data(), kptr.size()) == hipSuccess && upload(&G.d_kent, kent.data(), kent.size()) == hipSuccess &&
+              upload(&G.d_fptr, fptr.data(), fptr.size()) == hipSuccess && upload(&G.d_fent, fent.data(), fent.size()) == hipSuccess;
+    if (ok && box) {
+        G.box = true;
+        G.n_classes = (uint32_t)csub.size();
+        ok = upload(&G.d_cls, cls.data(), cls.size()) == hipSuccess && upload(&G.d_ch, ch.data(), ch.size()) == hipSuccess &&
+             upload(&G.d_csub, csub.data(), csub.size()) == hipSuccess &&
+             dalloc(&G.d_kref, (size_t)G.n_classes * nup * 9) == hipSuccess &&
+             dalloc(&G.d_fref, (size_t)G.n_classes * npe * 3) == hipSuccess;
+    } else if (ok) {
+        ok = dalloc(&G.d_em, (size_t)P.n_elem * nup * 9) == hipSuccess && dalloc(&G.d_ef, (size_t)P.n_elem * npe * 3) == hipSuccess;
+    }
+    if (!ok) {
         free_gather(G);
         return fail(FEMGPU_HIP_ERROR, "assembly: gather buffers");
     }
@@ -143,11 +195,11 @@ static int build_gather(Problem &P) {
 // copy, ASM_GATHER = element blocks then per-position gathers, R = row groups of R rows (a power
 // of two <= 64 whose LDS accumulator fits, see max_assembly_rows).
 int set_assembly_rows(Problem &P, uint32_t R) {
-    if (R == ASM_GATHER) {
-        const int rc = build_gather(P);
+    if (R == ASM_GATHER || R == ASM_GATHER_BOX) {
+        const int rc = build_gather(P, R == ASM_GATHER_BOX);
         if (rc) return rc;
         dfree(P.d_Kb);
-        P.asm_rows = ASM_GATHER;
+        P.asm_rows = R;
         return FEMGPU_OK;
     }
     free_gather(P.ga);
@@ -1242,10 +1294,11 @@ int femgpu_autotune(femgpu_problem *h, int *unroll) {
     const uint32_t Rmax = max_assembly_rows(P);
     uint32_t best_R = P.asm_rows;
     double best = 1e30;
-    for (uint32_t R : {0u, ASM_STAGED, ASM_GATHER, Rmax, Rmax / 2, Rmax / 4}) {
-        if (R && R != ASM_STAGED && R != ASM_GATHER && R < 8) continue;
+    for (uint32_t R : {0u, ASM_STAGED, ASM_GATHER, ASM_GATHER_BOX, Rmax, Rmax / 2, Rmax / 4}) {
+        if (R && R != ASM_STAGED && R != ASM_GATHER && R != ASM_GATHER_BOX && R < 8) continue;
         int rc = set_assembly_rows(P, R);
-        if (rc == FEMGPU_BAD_INPUT && R == ASM_GATHER) continue;  // beyond the gather lists' index range
+        // beyond the gather lists' index range, or not a mesh of box elements
+        if (rc == FEMGPU_BAD_INPUT && (R == ASM_GATHER || R == ASM_GATHER_BOX)) continue;
         if (rc) return rc;
         double t = 1e30;
         for (int rep = 0; rep < 4; ++rep) {

@@ -41,6 +41,8 @@ struct SubMaterial { double nu, bf0, bf1, bf2; };
 constexpr uint32_t ROWS_PER_WG = 256;
 constexpr uint32_t ASM_STAGED = 0xffffffffu;
 constexpr uint32_t ASM_GATHER = 0xfffffffeu;
+constexpr uint32_t ASM_GATHER_BOX = 0xfffffffdu;
+constexpr uint32_t MAX_BOX_CLASSES = 4096;
 constexpr uint32_t NO_SUB = 0xffffffffu;
 // Storage order of the block rows, chosen when a problem is created: AUTO sorts rows by length
 // within windows of SORT_WINDOW_WGS workgroups when that saves more than 3% of the SELL
@@ -159,6 +161,17 @@ struct Ic0 {
 struct GatherAsm {
     double *d_em = nullptr, *d_ef = nullptr;
     uint32_t *d_kptr = nullptr, *d_kent = nullptr, *d_fptr = nullptr, *d_fent = nullptr;
+    // ASM_GATHER_BOX, when every element is an axis-aligned Hexa box: elements fall into classes
+    // of equal edge lengths and sub-problem; a class's unit-modulus element matrix K_c(1) and body
+    // loads are formed once per matrix assembly, and pass 2 sums E_e K_c(1) (K_e is linear in E_e)
+    // instead of reading stored element blocks.
+    bool box = false;
+    uint32_t n_classes = 0;
+    uint16_t *d_cls = nullptr;        // class of each element
+    double *d_ch = nullptr;           // edge lengths of each class (3)
+    uint32_t *d_csub = nullptr;       // sub-problem of each class
+    double *d_kref = nullptr;         // per class: the NUP upper 3x3 blocks of K_c(1)
+    double *d_fref = nullptr;         // per class: the loads of its 8 nodes (3 each)
 };
 
 struct Problem {
@@ -256,6 +269,9 @@ hipError_t launch_permute_nodes(double *dst, const double *src, const uint32_t *
 hipError_t launch_assemble(Problem &P, int assemble_mat, int assemble_vec);
 // Largest row-group size (power of two <= 64) whose accumulator fits the LDS budget; 0 if none.
 uint32_t max_assembly_rows(const Problem &P);
+// An axis-aligned Hexa box: every edge along xi_d is (h_d > 0) e_d, one h_d per axis, compared
+// exactly; X the node coordinates, nd the element's 8 nodes.
+bool hexa_is_box(const double *X, const uint32_t *nd);
 hipError_t launch_constraints(Problem &P, int assemble_mat, int assemble_vec);
 hipError_t launch_to_fp32(Problem &P);
 hipError_t launch_elem_grad(Problem &P, const double *d_u, const double *d_lam, double *d_g);

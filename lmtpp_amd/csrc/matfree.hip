@@ -483,6 +483,12 @@ __device__ inline void tet10_apply(const TetRec<T> &r, const T (&u)[10][3], T (&
     }
 }
 
+#ifndef MF_WG_F32
+#define MF_WG_F32 1
+#endif
+#ifndef MF_WG_F64
+#define MF_WG_F64 1
+#endif
 // Dynamic LDS of k_mf_block for blocks of up to max_local nodes and values of vsize bytes.
 inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
     return (size_t)std::max<uint32_t>(1, max_local) * 6 * vsize;
@@ -505,7 +511,7 @@ inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
 // (FP64 Tetra_10: capped at 3 waves per SIMD, 168 VGPRs and 40 B of spills, against 184 VGPRs and
 // 2 waves uncapped: 80 against 84 us per product at 384k elements)
 template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA>
-__global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : 1) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
+__global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES ? (sizeof(T) == 4 ? MF_WG_F32 : MF_WG_F64) : 1)) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   const uint32_t *__restrict__ bsub,
                                                   uint32_t n_colors, uint32_t ls, const uint32_t *__restrict__ lnode,
@@ -731,11 +737,9 @@ void mf_free(MatFree &M) {
     M = MatFree();
 }
 
-namespace {
-
 // An axis-aligned box: every edge along xi_d is (h_d > 0) e_d, with one h_d per axis, compared
 // exactly (so the box product sees the coordinate differences of the general one).
-bool is_box(const double *X, const uint32_t *nd) {
+bool hexa_is_box(const double *X, const uint32_t *nd) {
     for (int d = 0; d < 3; ++d) {
         double hd = 0;
         for (int p = 0; p < 2; ++p)
@@ -753,6 +757,8 @@ bool is_box(const double *X, const uint32_t *nd) {
     }
     return true;
 }
+
+namespace {
 
 // 21-bit-per-axis Morton code (bit 3k + d holds bit k of axis d).
 uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
@@ -879,7 +885,7 @@ int mf_build(Problem &P) {
             if (used == 0xffffffffu) return 1;  // more than 32 sub-colours: not served
             const uint32_t c = (uint32_t)__builtin_ctz(~used);
             for (int a = 0; a < npe; ++a) mask[lid[conn[npe * (size_t)e + a]]] |= 1u << c;
-            scol[i] = (uint8_t)(c | (P.kind == HEXA && is_box(X.data(), conn + 8 * (size_t)e) ? MF_BOX : 0u));
+            scol[i] = (uint8_t)(c | (P.kind == HEXA && hexa_is_box(X.data(), conn + 8 * (size_t)e) ? MF_BOX : 0u));
             selem[i] = e;
             n_colors = std::max(n_colors, c + 1);
         }

@@ -405,3 +405,47 @@ def test_assembly_variants_agree(oracle, kind):
     assert f.assembly_rows()[0] == 8
     with pytest.raises(L.FemGpuError):
         f.set_assembly_rows(3)
+
+
+@pytest.mark.parametrize("geometry", ["lattice", "rectilinear"])
+def test_box_gather_assembly_matches_oracle(oracle, geometry):
+    """FEMGPU_ASSEMBLY_GATHER_BOX (every element an axis-aligned box): K and F against the oracle's
+    assembly to 1e-13 of their scales and against the element-block gather to 1e-15 (E_e times the
+    unit-modulus matrix of the element's edge-length class, rounding aside), with a body force, a
+    traction and per-element moduli; refused on a mesh with a non-box element."""
+    say("box gather", geometry)
+    c, e = L.make_rect(L.HEXA, (7, 6, 5))
+    if geometry == "rectilinear":          # monotone maps of each axis keep the elements boxes
+        c = np.stack([c[:, 0] ** 2, 0.5 * (c[:, 1] + c[:, 1] ** 3), np.sqrt(c[:, 2])], axis=1)
+    E = lognormal_E(len(e), 0.5, 11)
+    bf = (0.3, -0.2, 0.9)
+    f, faces, tr, dofs = build(L.HEXA, c, e, E, bf=bf)
+    rp, col, ovals, oF, rc = oracle.assemble(L.HEXA, c, e, E, NU0, bf=bf, faces=faces, face_trac=tr)
+    assert rc == 0
+    oracle.penalty(rp, col, ovals, oF, dofs, None, 1e5)
+    got = {}
+    for R in (L._capi.ASSEMBLY_GATHER, L._capi.ASSEMBLY_GATHER_BOX):
+        f.set_assembly_rows(R)
+        f.assemble()
+        _, _, vals = f.get_mat()
+        F = f.get_sollicitation()
+        scale = f.max_diag()
+        d = np.abs(vals - ovals) / np.maximum(scale, np.abs(ovals))
+        assert d.max() <= 1e-13, (R, d.max())
+        assert np.abs(F - oF).max() <= 1e-13 * np.abs(oF).max(), R
+        got[R] = (vals, F)
+    (v0, F0), (v1, F1) = got[L._capi.ASSEMBLY_GATHER], got[L._capi.ASSEMBLY_GATHER_BOX]
+    assert (np.abs(v1 - v0) / np.maximum(f.max_diag(), np.abs(v0))).max() <= 1e-15
+    assert np.abs(F1 - F0).max() <= 1e-15 * np.abs(F0).max()
+    # exact symmetry: a position and its transpose sum the same class blocks
+    n = len(c)
+    for r in range(0, n, 7):
+        for p in range(rp[r], rp[r + 1]):
+            q = rp[col[p]] + np.searchsorted(col[rp[col[p]]:rp[col[p] + 1]], r)
+            assert np.array_equal(v1[p], v1[q].T)
+    f.close()
+    cj = jitter_interior(L.make_rect(L.HEXA, (5, 4, 4))[0], 0.25, 0.1, 2)
+    g, *_ = build(L.HEXA, cj, L.make_rect(L.HEXA, (5, 4, 4))[1], 210000.0)
+    with pytest.raises(L.FemGpuError):
+        g.set_assembly_rows(L._capi.ASSEMBLY_GATHER_BOX)
+    g.close()

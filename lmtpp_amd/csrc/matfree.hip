@@ -489,9 +489,37 @@ __device__ inline void tet10_apply(const TetRec<T> &r, const T (&u)[10][3], T (&
 #ifndef MF_WG_F64
 #define MF_WG_F64 1
 #endif
-// Dynamic LDS of k_mf_block for blocks of up to max_local nodes and values of vsize bytes.
+// LDS of k_mf_block: per local node its 3 input values and 3 accumulators, each triple at a
+// stride of MF_NS values.  (A stride of 4, one 16-byte access per node, was measured: 29.7 us
+// against 29.3 us per FP32 element-block launch at Hexa8 100^3, 49.0 against 46.0 us in FP64.)
+constexpr int MF_NS = 3;
 inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
-    return (size_t)std::max<uint32_t>(1, max_local) * 6 * vsize;
+    return (size_t)std::max<uint32_t>(1, max_local) * 2 * MF_NS * vsize;
+}
+
+// The 3 values of local node i at LDS triple p (stride MF_NS), as vector accesses when the stride is 4.
+template <typename T>
+__device__ inline void lds_get3(const T *p, T &a, T &b, T &c) {
+    if constexpr (MF_NS == 4 && sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        a = v.x; b = v.y; c = v.z;
+    } else if constexpr (MF_NS == 4 && sizeof(T) == 8) {
+        const double2 v = *reinterpret_cast<const double2 *>(p);
+        a = v.x; b = v.y; c = p[2];
+    } else {
+        a = p[0]; b = p[1]; c = p[2];
+    }
+}
+template <typename T>
+__device__ inline void lds_put3(T *p, T a, T b, T c) {
+    if constexpr (MF_NS == 4 && sizeof(T) == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, 0.0f);
+    } else if constexpr (MF_NS == 4 && sizeof(T) == 8) {
+        *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+        p[2] = c;
+    } else {
+        p[0] = a; p[1] = b; p[2] = c;
+    }
 }
 
 // One block of MF_BLOCK element slots per workgroup (logical block b = xcd_slab(), so each XCD
@@ -540,8 +568,8 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
     // loads need no other load first
     const size_t l0 = (size_t)b * ls;
-    T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][3] the product's input
-    T *acc = Ps + 3 * (size_t)ls;           // [ls][3] summed nodal forces
+    T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][MF_NS] the product's input
+    T *acc = Ps + MF_NS * (size_t)ls;       // [ls][MF_NS] summed nodal forces
     const T beta = DIR ? (T)S[sb].beta : (T)0;
     // everything the later phases read from memory is loaded here, ahead of the barriers (a load
     // after a barrier cannot be hoisted above it): the penalty range, and the global node and
@@ -557,11 +585,11 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     }
     auto stage = [&](uint32_t i, uint32_t gn) {
         const size_t g = 3 * (size_t)gn;
+        T v[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            Ps[3 * i + d] = DIR ? dir_value(z[g + d], beta, xs[g + d]) : x[g + d];
-            acc[3 * i + d] = 0;
-        }
+        for (int d = 0; d < 3; ++d) v[d] = DIR ? dir_value(z[g + d], beta, xs[g + d]) : x[g + d];
+        lds_put3(Ps + MF_NS * i, v[0], v[1], v[2]);
+        lds_put3(acc + MF_NS * i, (T)0, (T)0, (T)0);
     };
 #pragma unroll
     for (int r = 0; r < (int)MF_NODE_REGS; ++r)
@@ -604,9 +632,7 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     if (col != MF_EMPTY) {
         T u[NPE][3];
 #pragma unroll
-        for (int a = 0; a < NPE; ++a)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) u[a][c] = Ps[3 * lo[a] + c];
+        for (int a = 0; a < NPE; ++a) lds_get3(Ps + MF_NS * lo[a], u[a][0], u[a][1], u[a][2]);
         if constexpr (K == TETRA10) {
             tet10_apply<T>(reinterpret_cast<const TetRec<T> *>(recv)[slot], u, f);
         } else {
@@ -623,13 +649,10 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         if (col == c) {
             T cur[NPE][3];
 #pragma unroll
-            for (int a = 0; a < NPE; ++a)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) cur[a][d] = acc[3 * lo[a] + d];
+            for (int a = 0; a < NPE; ++a) lds_get3(acc + MF_NS * lo[a], cur[a][0], cur[a][1], cur[a][2]);
 #pragma unroll
             for (int a = 0; a < NPE; ++a)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) acc[3 * lo[a] + d] = cur[a][d] + f[a][d];
+                lds_put3(acc + MF_NS * lo[a], cur[a][0] + f[a][0], cur[a][1] + f[a][1], cur[a][2] + f[a][2]);
         }
         __syncthreads();
     }
@@ -639,12 +662,14 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     auto write_out = [&](uint32_t i, uint32_t s, uint32_t g) {
         const bool own = s == 0xffffffffu;
         T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)(s & MF_SLOT_BITS);
-        const T a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
+        T a0, a1, a2;
+        lds_get3(acc + MF_NS * i, a0, a1, a2);
         dst[0] = a0;
         dst[1] = a1;
         dst[2] = a2;
         if (PQ || DIR) {
-            const T x0 = Ps[3 * i], x1 = Ps[3 * i + 1], x2 = Ps[3 * i + 2];
+            T x0, x1, x2;
+            lds_get3(Ps + MF_NS * i, x0, x1, x2);
             if (PQ) pq[0] += (double)x0 * (double)a0 + (double)x1 * (double)a1 + (double)x2 * (double)a2;
             if (DIR && (own || (s & MF_FIRST_HOLDER))) {
                 xo[3 * (size_t)g] = x0;
@@ -1035,7 +1060,7 @@ static int mf_penalties(Problem &P) {
     for (uint32_t b = 0; b < M.n_blocks; ++b) {
         for (uint32_t i : per[b]) {
             const uint32_t dof = M.snap_dofs[i], n = dof / 3;
-            pidx.push_back(3 * M.node_loc[n] + dof % 3);
+            pidx.push_back(MF_NS * M.node_loc[n] + dof % 3);
             psrc.push_back(i);
         }
         pptr.push_back((uint32_t)pidx.size());

@@ -129,6 +129,23 @@ FEMGPU_API int femgpu_add_constraint(femgpu_problem *p, uint32_t dof, double val
 FEMGPU_API int femgpu_add_constraints(femgpu_problem *p, uint32_t n, const uint32_t *dofs, const double *values,
                                       double penalty);
 FEMGPU_API int femgpu_clear_constraints(femgpu_problem *p);
+/* add_constraint(txt, penalty_value) (formulation_ancestor.h:197, formulation.h:2618; parsed as
+ * Constraint, constraint.h:46-93): txt is an expression whose zero is the constraint, linear in
+ * the unknowns node[i].dep[d] (caller's node numbering), with numbers, `time`, pi, + - * /
+ * ** (or ^), parentheses and sin cos tan exp log sqrt abs of unknown-free arguments, e.g.
+ * "node[12].dep[2] + 0.001*time".  Its penalty terms are those of assemble_constraints
+ * (formulation.h:1058-1070): with the expression c u + b, K += M P c^2 and F += M P c (-b),
+ * M = max|diag K|; an expression depending on time is re-evaluated at each assembly at the
+ * formulation's time.  Expressions with more than one unknown (couplings between DOFs) are
+ * refused (FEMGPU_BAD_INPUT), as is penalty <= 0 (the Lagrange form).  *number: the constraint's
+ * index among all constraints, as the reference returns it. */
+FEMGPU_API int femgpu_add_constraint_text(femgpu_problem *p, const char *txt, double penalty, uint32_t *number);
+/* The parsing of femgpu_add_constraint_text alone (host only, no problem): at time t the
+ * expression is coeff[k] u[dof[k]] summed over k < *n_terms, plus *b; dof / coeff hold cap
+ * entries (*n_terms may exceed cap: then only the count is written).  FEMGPU_BAD_INPUT with the
+ * message in femgpu_last_error() for a malformed or non-linear expression. */
+FEMGPU_API int femgpu_constraint_parse(const char *txt, double t, uint32_t cap, uint64_t *dof, double *coeff,
+                                       uint32_t *n_terms, double *b);
 
 /* ---- hot path ----------------------------------------------------------------------------- */
 
@@ -154,6 +171,18 @@ FEMGPU_API int femgpu_get_result(femgpu_problem *p, double *u, size_t n);
 FEMGPU_API int femgpu_get_result_device(femgpu_problem *p, double *d_u);
 /* Writes vectors[0], the initial guess of the next solve (shift(), formulation.h:424-433). */
 FEMGPU_API int femgpu_set_result(femgpu_problem *p, const double *u, size_t n);
+/* get_result(num) (formulation_ancestor.h:237, formulation.h:2648): vectors[num], num <
+ * nb_vectors = 20 (formulation.h:112): num = 0 the current solution, num = k the solution k shifts
+ * ago (zeros where no shift has filled the slot yet). */
+FEMGPU_API int femgpu_get_result_history(femgpu_problem *p, uint32_t num, double *u, size_t n);
+/* get_indice_noda(num_node) (formulation_ancestor.h:188, formulation.h:2655): the index of the
+ * node's first unknown in every vector of this API (3 num_node; with FEMGPU_NODE_ORDER_RCM the
+ * internal storage order is private and converted at the API, so the index stays 3 num_node). */
+FEMGPU_API int femgpu_get_indice_noda(femgpu_problem *p, uint32_t node, uint32_t *index);
+/* get_nodal_forces() (formulation_ancestor.h:65, formulation.h:447-458): the sollicitation of the
+ * element and skin loops alone (assemble_clean_mat(false, true): body and surface loads, no
+ * constraint terms), the assembled sollicitation left unchanged. */
+FEMGPU_API int femgpu_get_nodal_forces(femgpu_problem *p, double *F, size_t n);
 /* get_sollicitation() (formulation_ancestor.h:236). */
 FEMGPU_API int femgpu_get_sollicitation(femgpu_problem *p, double *F, size_t n);
 /* Writes the sollicitation that the next solve_system solves for: the reference's
@@ -253,9 +282,13 @@ FEMGPU_API int femgpu_write_result_vtu(femgpu_problem *p, const char *path);
 
 /* ---- time steps, non-linear loop, preconditioner reuse ------------------------------------- */
 
-/* shift(nb) (formulation.h:424-433): time += time step, and the solution history moves one step
- * (vectors[1] = vectors[0]); vectors[0] stays, the initial guess of the next solve. */
+/* shift(nb) (formulation.h:424-433): time += time_steps[0], time_steps and the solution history
+ * vectors[0..19] move up one slot (vectors[i] = vectors[i-1]); vectors[0] stays, the initial guess
+ * of the next solve. */
 FEMGPU_API int femgpu_shift(femgpu_problem *p, int nb);
+/* unshift(nb) (formulation_ancestor.h:58, formulation.h:435-445): time -= time_steps[1],
+ * time_steps and vectors move down one slot (vectors[0] takes vectors[1]), the last slot kept. */
+FEMGPU_API int femgpu_unshift(femgpu_problem *p, int nb);
 /* set_initial_time_step(ts) (formulation.h:2624): time step ts, time = -2 ts.  The time step
  * starts at 1e40 and time at 0 (formulation.h:128-129). */
 FEMGPU_API int femgpu_set_initial_time_step(femgpu_problem *p, double ts);
@@ -265,7 +298,8 @@ FEMGPU_API int femgpu_set_next_time_step(femgpu_problem *p, double ts);
 FEMGPU_API int femgpu_get_next_time_step(femgpu_problem *p, double *ts);
 FEMGPU_API int femgpu_get_time(femgpu_problem *p, double *time);
 FEMGPU_API int femgpu_set_time(femgpu_problem *p, double time);
-/* vectors[1]: the solution before the last shift (node-major, 3 n_nodes doubles). */
+/* vectors[1]: the solution before the last shift (node-major, 3 n_nodes doubles; zeros before the
+ * first shift), femgpu_get_result_history(p, 1, ...). */
 FEMGPU_API int femgpu_get_previous_result(femgpu_problem *p, double *u, size_t n);
 /* assume_constant_matrix (formulation_ancestor.h:31,244): ProblemAncestor::iteration then
  * assembles K once, builds the preconditioner once (get_precond) and afterwards assembles only the

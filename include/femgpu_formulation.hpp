@@ -9,8 +9,12 @@
 //     solve(crit) -> bool            formulation_ancestor.h:49
 //     update_variables()             formulation_ancestor.h:179
 //     add_constraint(dof, penalty)   formulation_ancestor.h:197  (penalty form)
-//     get_sollicitation()            formulation_ancestor.h:236
-//     get_result()                   formulation_ancestor.h:237
+//     add_constraint(txt, penalty)   formulation_ancestor.h:197; constraint.h:46-93 (one unknown)
+//     unshift(nb)                    formulation_ancestor.h:58; formulation.h:435-445
+//     get_sollicitation()            formulation_ancestor.h:236 (set_sollicitation for writes)
+//     get_result(num)                formulation_ancestor.h:237
+//     get_indice_noda(node)          formulation_ancestor.h:188
+//     get_nodal_forces()             formulation_ancestor.h:65; formulation.h:447-458
 //     call_after_solve()             formulation.h:2026-2050 (elem.epsilon / sigma / tr_epsilon)
 //     default_iterative_criterium    formulation_ancestor.h:244-257
 //     assume_constant_matrix, non_linear_iterative_criterium, max_non_linear_iteration
@@ -92,11 +96,18 @@ public:
     void add_constraint(uint32_t dof, double value, double penalty) {
         check(femgpu_add_constraint(h_, dof, value, penalty), "add_constraint");
     }
+    // e.g. add_constraint("node[12].dep[2] + 0.001*time", 1e5); returns the constraint's number
+    unsigned add_constraint(const std::string &txt, double penalty_value) {
+        uint32_t num = 0;
+        check(femgpu_add_constraint_text(h_, txt.c_str(), penalty_value, &num), "add_constraint");
+        return num;
+    }
 
     void allocate_matrices(bool = true, bool = true) {}
     // time += time step; the previous solution moves to vectors[1]; vectors[0] stays the next
     // initial guess
     void shift(int nb = 1) { check(femgpu_shift(h_, nb), "shift"); }
+    void unshift(int nb = 1) { check(femgpu_unshift(h_, nb), "unshift"); }
     void set_initial_time_step(double ts) { check(femgpu_set_initial_time_step(h_, ts), "set_initial_time_step"); }
     double get_next_time_step() const {
         double v = 0;
@@ -158,14 +169,27 @@ public:
         check(femgpu_get_result(h_, dep_.data(), dep_.size()), "get_result");
         return dep_;
     }
-    std::vector<double> get_result() const {
+    std::vector<double> get_result(unsigned num = 0) const {
         std::vector<double> u(3 * size_t(n_nodes_));
-        check(femgpu_get_result(h_, u.data(), u.size()), "get_result");
+        check(femgpu_get_result_history(h_, num, u.data(), u.size()), "get_result");
         return u;
     }
     std::vector<double> get_sollicitation() const {
         std::vector<double> F(3 * size_t(n_nodes_));
         check(femgpu_get_sollicitation(h_, F.data(), F.size()), "get_sollicitation");
+        return F;
+    }
+    void set_sollicitation(const std::vector<double> &F) {
+        check(femgpu_set_sollicitation(h_, F.data(), F.size()), "set_sollicitation");
+    }
+    unsigned get_indice_noda(unsigned num_node) const {
+        uint32_t i = 0;
+        check(femgpu_get_indice_noda(h_, num_node, &i), "get_indice_noda");
+        return i;
+    }
+    std::vector<double> get_nodal_forces() const {
+        std::vector<double> F(3 * size_t(n_nodes_));
+        check(femgpu_get_nodal_forces(h_, F.data(), F.size()), "get_nodal_forces");
         return F;
     }
     // Per-element strain and stress at xi = (1/2,1/2,1/2) from the current result: 6 values per

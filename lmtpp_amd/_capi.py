@@ -79,6 +79,13 @@ _SIGS = {
     "femgpu_get_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_set_sollicitation": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_apply_noalloc": (C.c_int, [_P, C.c_int, _P, _P]),
+    "femgpu_add_constraint_text": (C.c_int, [_P, C.c_char_p, C.c_double, C.POINTER(C.c_uint32)]),
+    "femgpu_unshift": (C.c_int, [_P, C.c_int]),
+    "femgpu_constraint_parse": (C.c_int, [C.c_char_p, C.c_double, C.c_uint32, _P, _P, C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_double)]),
+    "femgpu_get_result_history": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "femgpu_get_indice_noda": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "femgpu_get_nodal_forces": (C.c_int, [_P, _P, C.c_size_t]),
     "femgpu_get_matrix": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_sizes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint32)]),
@@ -181,6 +188,19 @@ def stream_copy_gbs(n_doubles=1 << 27, reps=20):
     ms = C.c_double()
     check(lib.femgpu_stream_copy(n_doubles, reps, C.byref(ms)), "stream_copy")
     return 16.0 * n_doubles / (ms.value * 1e-3) / 1e9, ms.value
+
+
+def constraint_parse(txt, time=0.0):
+    """The affine form of a text constraint at time (host only): ({dof: coefficient}, constant)."""
+    import numpy as np
+    lib = load()
+    n, b = C.c_uint32(), C.c_double()
+    check(lib.femgpu_constraint_parse(txt.encode(), float(time), 0, None, None, C.byref(n), C.byref(b)), "constraint_parse")
+    dof = np.zeros(max(1, n.value), np.uint64)
+    co = np.zeros(max(1, n.value))
+    check(lib.femgpu_constraint_parse(txt.encode(), float(time), n.value, dof.ctypes.data, co.ctypes.data, C.byref(n),
+                                      C.byref(b)), "constraint_parse")
+    return {int(d): float(v) for d, v in zip(dof[:n.value], co[:n.value])}, b.value
 
 
 def ic0_schedule(row_ptr, col, precond=BLOCK_IC0_COLOURED):

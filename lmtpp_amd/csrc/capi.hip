@@ -504,11 +504,41 @@ int upload_constraints(Problem &P) {
     return FEMGPU_OK;
 }
 
+// Value and penalty of a text constraint at time t: its expression is c u_dof + b, so the penalty
+// terms M P c^2 on the diagonal and M P c (-b) on the right-hand side of assemble_constraints
+// (formulation.h:1058-1070) are those of u_dof = -b / c with penalty P c^2.
+int eval_text_constraint(const femgpu_problem::TextConstraint &tc, double t, double &value, double &pen) {
+    std::map<uint64_t, double> co;
+    double b = 0;
+    const std::string err = tc.expr.eval(t, co, b);
+    if (!err.empty()) return fail(FEMGPU_BAD_INPUT, "constraint '" + tc.expr.text() + "': " + err);
+    if (co.size() != 1 || co.begin()->first != tc.dof)
+        return fail(FEMGPU_BAD_INPUT, "constraint '" + tc.expr.text() + "': no unknown left at time " + std::to_string(t));
+    const double c = co.begin()->second;
+    value = -b / c;
+    pen = tc.penalty * c * c;
+    if (!std::isfinite(value) || !std::isfinite(pen)) return fail(FEMGPU_NAN, "constraint '" + tc.expr.text() + "': not finite");
+    return FEMGPU_OK;
+}
+
 int assemble(femgpu_problem &H, int do_mat, int do_vec) {
     Problem &P = H.P;
     if (P.from_bsr) return fail(FEMGPU_BAD_INPUT, "assemble: problem built from a matrix");
     for (uint32_t s = 0; s < P.n_sub; ++s)
         if (!P.material_set[s]) return fail(FEMGPU_NOT_READY, "assemble: material not set");
+    // text constraints depending on time take their values at the current time
+    for (const auto &tc : H.text_cons) {
+        if (!tc.expr.uses_time()) continue;
+        double v = 0, pen = 0;
+        const int rc = eval_text_constraint(tc, H.time, v, pen);
+        if (rc) return rc;
+        Problem::Cons &c = P.cons[tc.slot];
+        if (c.val != v || c.pen != pen) {
+            c.val = v;
+            c.pen = pen;
+            P.cons_dirty = true;
+        }
+    }
     int rc = upload_constraints(P);
     if (!rc) rc = upload_faces(H);
     if (rc) return rc;
@@ -786,7 +816,7 @@ int femgpu_create_from_bsr(uint32_t n_brows, const uint32_t *row_ptr, const uint
 int femgpu_destroy(femgpu_problem *h) {
     if (!h) return FEMGPU_OK;
     free_problem(h->P);
-    dfree(h->d_prev);
+    for (double *&v : h->d_hist) dfree(v);
     dfree(h->d_old);
     delete h;
     return FEMGPU_OK;
@@ -820,9 +850,64 @@ int femgpu_add_constraints(femgpu_problem *h, uint32_t n, const uint32_t *dofs, 
     return add_constraints(*h, 0, n, dofs, values, penalty);
 }
 
+int femgpu_constraint_parse(const char *txt, double t, uint32_t cap, uint64_t *dof, double *coeff, uint32_t *n_terms,
+                            double *b) {
+    if (!txt || !n_terms || !b || (cap && (!dof || !coeff))) return fail(FEMGPU_BAD_INPUT, "constraint_parse: null");
+    ConstraintExpr ex;
+    std::string err = ex.parse(txt);
+    std::map<uint64_t, double> co;
+    if (err.empty()) err = ex.eval(t, co, *b);
+    if (!err.empty()) return fail(FEMGPU_BAD_INPUT, std::string("constraint '") + txt + "': " + err);
+    *n_terms = (uint32_t)co.size();
+    uint32_t k = 0;
+    for (const auto &kv : co) {
+        if (k >= cap) break;
+        dof[k] = kv.first;
+        coeff[k] = kv.second;
+        ++k;
+    }
+    return FEMGPU_OK;
+}
+
+int femgpu_add_constraint_text(femgpu_problem *h, const char *txt, double penalty, uint32_t *number) {
+    if (!h || !txt) return fail(FEMGPU_BAD_INPUT, "add_constraint: null");
+    Problem &P = h->P;
+    if (P.from_bsr) return fail(FEMGPU_BAD_INPUT, "add_constraint: problem built from a matrix");
+    if (!(penalty > 0) || !std::isfinite(penalty))
+        return fail(FEMGPU_BAD_INPUT, "add_constraint: only the penalty form (penalty > 0) keeps K SPD");
+    femgpu_problem::TextConstraint tc;
+    std::string err = tc.expr.parse(txt);
+    if (!err.empty()) return fail(FEMGPU_BAD_INPUT, std::string("add_constraint '") + txt + "': " + err);
+    std::map<uint64_t, double> co;
+    double b = 0;
+    err = tc.expr.eval(h->time, co, b);
+    if (!err.empty()) return fail(FEMGPU_BAD_INPUT, std::string("add_constraint '") + txt + "': " + err);
+    if (co.size() != 1)
+        return fail(FEMGPU_BAD_INPUT, std::string("add_constraint '") + txt + "': " +
+                                          (co.empty() ? "no unknown"
+                                                      : "couples several unknowns; its penalty terms between DOFs lie "
+                                                        "outside the node-adjacency pattern and the matrix-free product"));
+    const uint64_t dof = co.begin()->first;
+    if (dof >= 3 * (uint64_t)(P.node_off[1] - P.node_off[0]))
+        return fail(FEMGPU_BAD_INPUT, std::string("add_constraint '") + txt + "': node beyond the mesh");
+    tc.dof = dof;
+    tc.penalty = penalty;
+    double v = 0, pen = 0;
+    const int rc = eval_text_constraint(tc, h->time, v, pen);
+    if (rc) return rc;
+    const uint32_t g = (uint32_t)dof;
+    tc.slot = P.cons.size();
+    P.cons.push_back({P.ninv.empty() ? g : 3 * P.ninv[g / 3] + g % 3, v, pen});
+    P.cons_dirty = true;
+    h->text_cons.push_back(tc);
+    if (number) *number = (uint32_t)(P.cons.size() - 1);
+    return FEMGPU_OK;
+}
+
 int femgpu_clear_constraints(femgpu_problem *h) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "clear_constraints: null");
     h->P.cons.clear();
+    h->text_cons.clear();
     h->P.cons_dirty = true;
     return FEMGPU_OK;
 }
@@ -1446,10 +1531,44 @@ int femgpu_shift(femgpu_problem *h, int nb) {
     Problem &P = h->P;
     const size_t n = 3 * (size_t)P.n_nodes;
     while (nb--) {
-        h->time += h->time_step;
-        if (!h->d_prev) HIPCHK(dalloc(&h->d_prev, n));
-        HIPCHK(hipMemcpyAsync(h->d_prev, P.d_u, n * 8, hipMemcpyDeviceToDevice, P.stream));
-        h->has_prev = true;
+        h->time += h->time_steps[0];
+        for (int i = NB_VECTORS - 1; i > 0; --i) h->time_steps[i] = h->time_steps[i - 1];
+        h->time_steps[0] = h->time_steps[1];
+        // vectors[i] = vectors[i - 1]: the buffers move up one slot, the last one's buffer takes
+        // vectors[0] (which stays as the next solve's initial guess)
+        double *last = h->d_hist[NB_VECTORS - 1];
+        for (int i = NB_VECTORS - 1; i > 1; --i) h->d_hist[i] = h->d_hist[i - 1];
+        h->d_hist[1] = nullptr;
+        if (!last) HIPCHK(dalloc(&last, n));
+        h->d_hist[1] = last;
+        HIPCHK(hipMemcpyAsync(last, P.d_u, n * 8, hipMemcpyDeviceToDevice, P.stream));
+    }
+    HIPCHK(hipStreamSynchronize(P.stream));
+    return FEMGPU_OK;
+}
+
+int femgpu_unshift(femgpu_problem *h, int nb) {
+    if (!h || nb < 0) return fail(FEMGPU_BAD_INPUT, "unshift: bad input");
+    Problem &P = h->P;
+    const size_t n = 3 * (size_t)P.n_nodes;
+    while (nb--) {
+        h->time -= h->time_steps[1];
+        for (int i = 0; i < NB_VECTORS - 2; ++i) h->time_steps[i] = h->time_steps[i + 1];
+        h->time_steps[NB_VECTORS - 1] = h->time_steps[NB_VECTORS - 2];
+        // vectors[i] = vectors[i + 1] for i < nb_vectors - 2, vectors[last] = vectors[last - 1]
+        if (h->d_hist[1]) HIPCHK(hipMemcpyAsync(P.d_u, h->d_hist[1], n * 8, hipMemcpyDeviceToDevice, P.stream));
+        else HIPCHK(launch_fill(P.d_u, n, 0.0, P.stream));
+        double *spare = h->d_hist[1];
+        for (int i = 1; i < NB_VECTORS - 1; ++i) h->d_hist[i] = h->d_hist[i + 1];
+        h->d_hist[NB_VECTORS - 1] = nullptr;
+        if (h->d_hist[NB_VECTORS - 2]) {
+            if (!spare) HIPCHK(dalloc(&spare, n));
+            HIPCHK(hipMemcpyAsync(spare, h->d_hist[NB_VECTORS - 2], n * 8, hipMemcpyDeviceToDevice, P.stream));
+            h->d_hist[NB_VECTORS - 1] = spare;
+        } else if (spare) {
+            HIPCHK(hipStreamSynchronize(P.stream));
+            dfree(spare);
+        }
     }
     HIPCHK(hipStreamSynchronize(P.stream));
     return FEMGPU_OK;
@@ -1457,20 +1576,20 @@ int femgpu_shift(femgpu_problem *h, int nb) {
 
 int femgpu_set_initial_time_step(femgpu_problem *h, double ts) {
     if (!h || !std::isfinite(ts)) return fail(FEMGPU_BAD_INPUT, "set_initial_time_step: bad input");
-    h->time_step = ts;
+    for (double &t : h->time_steps) t = ts;
     h->time = -2.0 * ts;
     return FEMGPU_OK;
 }
 
 int femgpu_set_next_time_step(femgpu_problem *h, double ts) {
     if (!h || !std::isfinite(ts)) return fail(FEMGPU_BAD_INPUT, "set_next_time_step: bad input");
-    h->time_step = ts;
+    h->time_steps[0] = ts;
     return FEMGPU_OK;
 }
 
 int femgpu_get_next_time_step(femgpu_problem *h, double *ts) {
     if (!h || !ts) return fail(FEMGPU_BAD_INPUT, "get_next_time_step: null");
-    *ts = h->time_step;
+    *ts = h->time_steps[0];
     return FEMGPU_OK;
 }
 
@@ -1486,10 +1605,45 @@ int femgpu_set_time(femgpu_problem *h, double t) {
     return FEMGPU_OK;
 }
 
+int femgpu_get_result_history(femgpu_problem *h, uint32_t num, double *u, size_t n) {
+    if (!h || num >= (uint32_t)NB_VECTORS) return fail(FEMGPU_BAD_INPUT, "get_result: num beyond nb_vectors");
+    if (num == 0) return copy_sub(*h, 0, h->P.d_u, u, n);
+    if (!h->d_hist[num]) {
+        if (!u || n != 3 * (size_t)(h->P.node_off[1] - h->P.node_off[0])) return fail(FEMGPU_BAD_INPUT, "get_result: size");
+        std::fill(u, u + n, 0.0);
+        return FEMGPU_OK;
+    }
+    return copy_sub(*h, 0, h->d_hist[num], u, n);
+}
+
 int femgpu_get_previous_result(femgpu_problem *h, double *u, size_t n) {
-    if (!h) return fail(FEMGPU_BAD_INPUT, "get_previous_result: null");
-    if (!h->has_prev) return fail(FEMGPU_NOT_READY, "get_previous_result: no shift yet");
-    return copy_sub(*h, 0, h->d_prev, u, n);
+    return femgpu_get_result_history(h, 1, u, n);
+}
+
+int femgpu_get_indice_noda(femgpu_problem *h, uint32_t node, uint32_t *index) {
+    if (!h || !index || node >= h->P.node_off[1]) return fail(FEMGPU_BAD_INPUT, "get_indice_noda: bad input");
+    *index = 3 * node;
+    return FEMGPU_OK;
+}
+
+int femgpu_get_nodal_forces(femgpu_problem *h, double *F, size_t n) {
+    if (!h || !F || n != 3 * (size_t)(h->P.node_off[1] - h->P.node_off[0]))
+        return fail(FEMGPU_BAD_INPUT, "get_nodal_forces: size");
+    Problem &P = h->P;
+    if (P.from_bsr) return fail(FEMGPU_BAD_INPUT, "get_nodal_forces: problem built from a matrix");
+    for (uint32_t s = 0; s < P.n_sub; ++s)
+        if (!P.material_set[s]) return fail(FEMGPU_NOT_READY, "get_nodal_forces: material not set");
+    int rc = upload_faces(*h);
+    if (rc) return rc;
+    const size_t nv = 3 * (size_t)P.n_nodes;
+    // the sollicitation of the element and skin loops alone (assemble_clean_mat(false, true),
+    // formulation.h:447-458); the assembled one is kept aside and restored
+    HIPCHK(hipMemcpyAsync(P.d_aux, P.d_F, nv * 8, hipMemcpyDeviceToDevice, P.stream));
+    HIPCHK(launch_assemble(P, 0, 1));
+    rc = copy_sub(*h, 0, P.d_F, F, n);
+    HIPCHK(hipMemcpyAsync(P.d_F, P.d_aux, nv * 8, hipMemcpyDeviceToDevice, P.stream));
+    HIPCHK(hipStreamSynchronize(P.stream));
+    return rc;
 }
 
 int femgpu_set_assume_constant_matrix(femgpu_problem *h, int flag) {

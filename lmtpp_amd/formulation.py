@@ -84,9 +84,19 @@ class Formulation:
         tr = np.ascontiguousarray(np.broadcast_to(np.asarray(traction, dtype=np.float64), (faces.shape[0], 3)))
         A.check(self._lib.femgpu_set_traction(self._h, faces.shape[0], _ptr(faces), _ptr(tr)), "set_traction")
 
-    def add_constraint(self, dof, value=0.0, penalty=1e5):
-        """Penalty constraint u[dof] = value (formulation_ancestor.h:197, formulation.h:1028-1070)."""
-        A.check(self._lib.femgpu_add_constraint(self._h, int(dof), float(value), float(penalty)), "add_constraint")
+    def add_constraint(self, dof, value=0.0, penalty=None):
+        """Penalty constraint u[dof] = value (formulation.h:1028-1070), penalty 1e5 by default.
+        With a string first, add_constraint(txt, penalty_value) as the reference's
+        (formulation_ancestor.h:197): txt is an expression in node[i].dep[d] and time whose zero
+        is the constraint, e.g. "node[3].dep[2] + 0.01*time"; returns the constraint's number.
+        penalty_value must be > 0 (the reference's default 0, the Lagrange form, is refused)."""
+        if isinstance(dof, str):
+            pen = float(value) if penalty is None else float(penalty)
+            num = C.c_uint32()
+            A.check(self._lib.femgpu_add_constraint_text(self._h, dof.encode(), pen, C.byref(num)), "add_constraint")
+            return num.value
+        A.check(self._lib.femgpu_add_constraint(self._h, int(dof), float(value), 1e5 if penalty is None else float(penalty)),
+                "add_constraint")
 
     def add_constraints(self, dofs, values=None, penalty=1e5):
         dofs = np.ascontiguousarray(dofs, dtype=np.uint32)
@@ -202,6 +212,28 @@ class Formulation:
 
     def set_time(self, t):
         A.check(self._lib.femgpu_set_time(self._h, float(t)), "set_time")
+
+    def unshift(self, nb=1):
+        """unshift(nb) (formulation.h:435-445): time -= time_steps[1]; the history moves down."""
+        A.check(self._lib.femgpu_unshift(self._h, int(nb)), "unshift")
+
+    def get_result_num(self, num=0):
+        """get_result(num) (formulation_ancestor.h:237): vectors[num], 0 <= num < 20."""
+        u = np.zeros(3 * self.n_nodes)
+        A.check(self._lib.femgpu_get_result_history(self._h, int(num), _ptr(u), u.size), "get_result")
+        return u
+
+    def get_indice_noda(self, node):
+        """get_indice_noda(num_node) (formulation_ancestor.h:188): index of the node's first unknown."""
+        v = C.c_uint32()
+        A.check(self._lib.femgpu_get_indice_noda(self._h, int(node), C.byref(v)), "get_indice_noda")
+        return v.value
+
+    def get_nodal_forces(self):
+        """get_nodal_forces() (formulation.h:447-458): body and surface loads without constraint terms."""
+        F = np.zeros(3 * self.n_nodes)
+        A.check(self._lib.femgpu_get_nodal_forces(self._h, _ptr(F), F.size), "get_nodal_forces")
+        return F
 
     def get_previous_result(self):
         """vectors[1]: the solution before the last shift."""

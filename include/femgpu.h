@@ -280,6 +280,34 @@ FEMGPU_API int femgpu_write_vtu(const char *path, int elem_kind, uint32_t n_node
  * tr_epsilon (call_after_solve of the current result); poisson_ratio. */
 FEMGPU_API int femgpu_write_result_vtu(femgpu_problem *p, const char *path);
 
+/* One named field of an AVS file.  nest is the shape of the value in the reference's mesh data,
+ * which fixes its spacing in the file: 0 a scalar (n_comp 1), 1 a Vec<T,n_comp>, 2 a
+ * Vec<Vec<T,n_comp>,1> (the element fields epsilon and sigma). */
+typedef struct {
+    const char *name;
+    uint32_t n_comp;
+    uint32_t nest;
+    const double *data;
+} femgpu_avs_field;
+/* write_avs(mesh, name, display_fields, Ascii()) (mesh/write_avs.h:389-530): an AVS UCD ASCII
+ * file with the reference writer's bytes -- the header (nodes, elements, then the component sums
+ * of the node, cell and global fields written, 2-component node and global fields counted as 3),
+ * the nodes, the cells ("hex" / "tet"; a Tetra_10 cell, which that writer has no type for, as its
+ * number and material 0 alone, with no line end), then the node fields in the given order, the
+ * cell fields in name order and the global fields in the given order, each written when its name
+ * is in display[0..n_display) or display[0] is "all".  Host only. */
+FEMGPU_API int femgpu_write_avs(const char *path, int elem_kind, uint32_t n_nodes, const double *coords,
+                                uint32_t n_elem, const uint32_t *conn, uint32_t n_point,
+                                const femgpu_avs_field *point, uint32_t n_cell, const femgpu_avs_field *cell,
+                                uint32_t n_global, const femgpu_avs_field *global, uint32_t n_display,
+                                const char *const *display);
+/* The problem's mesh and current result in that layout (write_avs(mesh, num, structure, Ascii()),
+ * write_avs.h:548-562, without its path and numbering): the node fields dep, pos; the cell fields
+ * elastic_modulus, epsilon, sigma, tr_epsilon (call_after_solve of the current result); the
+ * global fields poisson_ratio, f_vol; display as for femgpu_write_avs (NULL or none: "all"). */
+FEMGPU_API int femgpu_write_result_avs(femgpu_problem *p, const char *path, uint32_t n_display,
+                                       const char *const *display);
+
 /* ---- time steps, non-linear loop, preconditioner reuse ------------------------------------- */
 
 /* shift(nb) (formulation.h:424-433): time += time_steps[0], time_steps and the solution history
@@ -358,6 +386,9 @@ FEMGPU_API int femgpu_batch_clear_result(femgpu_batch *b);
 FEMGPU_API int femgpu_batch_after_solve(femgpu_batch *b, uint32_t i, double *eps, double *sigma, double *tr_eps);
 /* femgpu_write_result_vtu for problem i of a batch (its own mesh, group = i). */
 FEMGPU_API int femgpu_batch_write_result_vtu(femgpu_batch *b, uint32_t i, const char *path);
+/* femgpu_write_result_avs for problem i of a batch (its own mesh and material). */
+FEMGPU_API int femgpu_batch_write_result_avs(femgpu_batch *b, uint32_t i, const char *path, uint32_t n_display,
+                                             const char *const *display);
 /* Device time (ms) of the last batched assembly. */
 FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
 /* The element-loop variant of a batch (femgpu_set_assembly_rows' values); every variant gives

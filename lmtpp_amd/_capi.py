@@ -51,6 +51,10 @@ class VtuField(C.Structure):
     _fields_ = [("name", C.c_char_p), ("n_comp", C.c_uint32), ("data", C.c_void_p)]
 
 
+class AvsField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("n_comp", C.c_uint32), ("nest", C.c_uint32), ("data", C.c_void_p)]
+
+
 class FemGpuError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s (status %d)" % (msg, status))
@@ -138,6 +142,10 @@ _SIGS = {
                                    _P, C.c_uint32, _P]),
     "femgpu_write_result_vtu": (C.c_int, [_P, C.c_char_p]),
     "femgpu_batch_write_result_vtu": (C.c_int, [_P, C.c_uint32, C.c_char_p]),
+    "femgpu_write_avs": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P,
+                                   C.c_uint32, _P, C.c_uint32, _P]),
+    "femgpu_write_result_avs": (C.c_int, [_P, C.c_char_p, C.c_uint32, _P]),
+    "femgpu_batch_write_result_avs": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32, _P]),
     "femgpu_shift": (C.c_int, [_P, C.c_int]),
     "femgpu_set_initial_time_step": (C.c_int, [_P, C.c_double]),
     "femgpu_set_next_time_step": (C.c_int, [_P, C.c_double]),
@@ -261,6 +269,45 @@ def write_vtu(path, kind, coords, conn, point=(), cell=(), group=None, glob=()):
     check(lib.femgpu_write_vtu(str(path).encode(), int(kind), len(coords), coords.ctypes.data, len(conn),
                                conn.ctypes.data, len(point), pf, len(cell), cf, None if g is None else g.ctypes.data,
                                len(glob), gf), "write_vtu")
+
+
+def display_names(display):
+    """A display list for the AVS writers: (count, char** array or None); None means "all"."""
+    if display is None:
+        return 0, None
+    if isinstance(display, str):
+        display = [display]
+    arr = (C.c_char_p * max(len(display), 1))(*[d.encode() for d in display])
+    return len(display), arr
+
+
+def write_avs(path, kind, coords, conn, point=(), cell=(), glob=(), display=("all",)):
+    """AVS UCD ASCII file in the reference writer's layout and bytes (femgpu_write_avs, host only).
+    point / cell / glob: sequences of (name, array) or (name, array, nest) with one row of
+    components per node / element, or one row for a global field; nest 0 is a scalar, 1 (the
+    default) a Vec, 2 a Vec of one Vec (epsilon, sigma).  display: the field names to write, or
+    ("all",)."""
+    import numpy as np
+    lib = load()
+    keep = []
+
+    def fields(items, rows):
+        arr = (AvsField * max(len(items), 1))()
+        for i, it in enumerate(items):
+            name, a = it[0], it[1]
+            nest = it[2] if len(it) > 2 else 1
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            a = a.reshape(rows, -1) if rows else a.reshape(1, -1)
+            keep.append(a)
+            arr[i] = AvsField(name.encode(), a.shape[1], nest, a.ctypes.data)
+        return arr
+
+    coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(-1, NPE[int(kind)])
+    pf, cf, gf = fields(point, len(coords)), fields(cell, len(conn)), fields(glob, 0)
+    nd, dp = display_names(display)
+    check(lib.femgpu_write_avs(str(path).encode(), int(kind), len(coords), coords.ctypes.data, len(conn),
+                               conn.ctypes.data, len(point), pf, len(cell), cf, len(glob), gf, nd, dp), "write_avs")
 
 
 def node_order(row_ptr, col):

@@ -114,6 +114,12 @@ class Batch:
         """Problem i's mesh and result as a VTK unstructured grid (Formulation.write_vtu)."""
         A.check(self._lib.femgpu_batch_write_result_vtu(self._h, int(i), str(path).encode()), "batch_write_result_vtu")
 
+    def write_avs(self, i, path, display=None):
+        """Formulation.write_avs for problem i of the batch."""
+        nd, dp = A.display_names(display)
+        A.check(self._lib.femgpu_batch_write_result_avs(self._h, int(i), str(path).encode(), nd, dp),
+                "batch_write_result_avs")
+
     def get_result(self, i):
         u = np.zeros(3 * self.sizes[i][0])
         A.check(self._lib.femgpu_batch_get_result(self._h, i, _ptr(u), u.size), "batch_get_result")

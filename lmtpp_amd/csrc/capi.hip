@@ -13,6 +13,7 @@
 #include "../../include/femgpu.h"
 #include "capi_handles.hpp"
 #include "femgpu_internal.hpp"
+#include "io_avs.hpp"
 #include "io_vtu.hpp"
 
 namespace femgpu {
@@ -1149,31 +1150,110 @@ int femgpu_write_vtu(const char *path, int elem_kind, uint32_t n_nodes, const do
     return FEMGPU_OK;
 }
 
+int femgpu_write_avs(const char *path, int elem_kind, uint32_t n_nodes, const double *coords, uint32_t n_elem,
+                     const uint32_t *conn, uint32_t n_point, const femgpu_avs_field *point, uint32_t n_cell,
+                     const femgpu_avs_field *cell, uint32_t n_global, const femgpu_avs_field *global, uint32_t n_display,
+                     const char *const *display) {
+    const int npe = nodes_per_elem(elem_kind);
+    if (!path || !npe || (n_nodes && !coords) || (n_elem && !conn) || (n_point && !point) || (n_cell && !cell) ||
+        (n_global && !global) || (n_display && !display))
+        return fail(FEMGPU_BAD_INPUT, "write_avs: bad input");
+    for (size_t k = 0; k < (size_t)n_elem * npe; ++k)
+        if (conn[k] >= n_nodes) return fail(FEMGPU_BAD_INPUT, "write_avs: node index out of range");
+    auto fields = [](uint32_t n, const femgpu_avs_field *f, size_t items, std::vector<AvsField> &out) {
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!f[i].name || !f[i].n_comp || f[i].nest > 2 || (f[i].nest == 0 && f[i].n_comp != 1) ||
+                (items && !f[i].data))
+                return false;
+            out.push_back(AvsField{f[i].name, f[i].n_comp, f[i].nest, f[i].data});
+        }
+        return true;
+    };
+    std::vector<AvsField> pf, cf, gf;
+    if (!fields(n_point, point, n_nodes, pf) || !fields(n_cell, cell, n_elem, cf) || !fields(n_global, global, 1, gf))
+        return fail(FEMGPU_BAD_INPUT, "write_avs: a field needs a name, data, n_comp >= 1 and nest 0 (n_comp 1), 1 or 2");
+    std::vector<std::string> disp;
+    for (uint32_t i = 0; i < n_display; ++i) {
+        if (!display[i]) return fail(FEMGPU_BAD_INPUT, "write_avs: null display name");
+        disp.emplace_back(display[i]);
+    }
+    if (!write_avs(path, elem_kind, n_nodes, coords, n_elem, conn, pf, cf, gf, disp))
+        return fail(FEMGPU_BAD_INPUT, std::string("write_avs: cannot write ") + path);
+    return FEMGPU_OK;
+}
+
 namespace {
-// Sub-problem s of a mesh problem as a result file: dep; elastic_modulus, epsilon, sigma,
-// tr_epsilon (the after-solve recovery of the current result); group = s; poisson_ratio.
-int write_result(femgpu_problem &H, uint32_t s, const char *path) {
+// Sub-problem s of a mesh problem, gathered for a result file in the API's node order.
+struct ResultData {
+    uint32_t nn = 0, ne = 0;
+    std::vector<double> coords, u, E, eps, sig, tr;
+    std::vector<uint32_t> conn;
+    double nu = 0, bf[3] = {0, 0, 0};
+};
+
+int gather_result(femgpu_problem &H, uint32_t s, const char *path, ResultData &R) {
     Problem &P = H.P;
-    if (!path || P.from_bsr || s >= P.n_sub) return fail(FEMGPU_BAD_INPUT, "write_result_vtu: bad input");
-    const uint32_t n0 = P.node_off[s], nn = P.node_off[s + 1] - n0, e0 = P.elem_off[s], ne = P.elem_off[s + 1] - e0;
-    std::vector<double> coords(3 * (size_t)nn), u(3 * (size_t)nn), E(ne), eps(6 * (size_t)ne), sig(6 * (size_t)ne), tr(ne);
+    if (!path || P.from_bsr || s >= P.n_sub) return fail(FEMGPU_BAD_INPUT, "write_result: bad input");
+    const uint32_t n0 = P.node_off[s], e0 = P.elem_off[s];
+    R.nn = P.node_off[s + 1] - n0;
+    R.ne = P.elem_off[s + 1] - e0;
+    R.coords.resize(3 * (size_t)R.nn);
+    R.u.resize(3 * (size_t)R.nn);
+    R.E.resize(R.ne);
+    R.eps.resize(6 * (size_t)R.ne);
+    R.sig.resize(6 * (size_t)R.ne);
+    R.tr.resize(R.ne);
     HIPCHK(hipStreamSynchronize(P.stream));
-    HIPCHK(hipMemcpy(coords.data(), P.d_coords + 3 * (size_t)n0, coords.size() * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(u.data(), P.d_u + 3 * (size_t)n0, u.size() * 8, hipMemcpyDeviceToHost));
-    to_api_order(P, s, coords.data());
-    to_api_order(P, s, u.data());
-    HIPCHK(hipMemcpy(E.data(), P.d_E + e0, E.size() * 8, hipMemcpyDeviceToHost));
-    int rc = after_solve_host(&H, e0, ne, eps.data(), sig.data(), tr.data());
+    HIPCHK(hipMemcpy(R.coords.data(), P.d_coords + 3 * (size_t)n0, R.coords.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(R.u.data(), P.d_u + 3 * (size_t)n0, R.u.size() * 8, hipMemcpyDeviceToHost));
+    to_api_order(P, s, R.coords.data());
+    to_api_order(P, s, R.u.data());
+    HIPCHK(hipMemcpy(R.E.data(), P.d_E + e0, R.E.size() * 8, hipMemcpyDeviceToHost));
+    int rc = after_solve_host(&H, e0, R.ne, R.eps.data(), R.sig.data(), R.tr.data());
     if (rc) return rc;
-    std::vector<uint32_t> conn(P.conn_h.begin() + (size_t)e0 * P.npe, P.conn_h.begin() + (size_t)(e0 + ne) * P.npe);
-    for (uint32_t &v : conn) v = (P.nperm.empty() ? v : P.nperm[v]) - n0;
-    const std::vector<uint32_t> group(ne, s);
-    const double nu = P.mat_h[s].nu;
-    const femgpu_vtu_field pf[] = {{"dep", 3, u.data()}};
-    const femgpu_vtu_field cf[] = {{"elastic_modulus", 1, E.data()}, {"epsilon", 6, eps.data()},
-                                   {"sigma", 6, sig.data()}, {"tr_epsilon", 1, tr.data()}};
-    const femgpu_vtu_field gf[] = {{"poisson_ratio", 1, &nu}};
-    return femgpu_write_vtu(path, P.kind, nn, coords.data(), ne, conn.data(), 1, pf, 4, cf, group.data(), 1, gf);
+    R.conn.assign(P.conn_h.begin() + (size_t)e0 * P.npe, P.conn_h.begin() + (size_t)(e0 + R.ne) * P.npe);
+    for (uint32_t &v : R.conn) v = (P.nperm.empty() ? v : P.nperm[v]) - n0;
+    const SubMaterial &m = P.mat_h[s];
+    R.nu = m.nu;
+    R.bf[0] = m.bf0;
+    R.bf[1] = m.bf1;
+    R.bf[2] = m.bf2;
+    return FEMGPU_OK;
+}
+
+// Sub-problem s as a VTU file: dep; elastic_modulus, epsilon, sigma, tr_epsilon (the after-solve
+// recovery of the current result); group = s; poisson_ratio.
+int write_result(femgpu_problem &H, uint32_t s, const char *path) {
+    ResultData R;
+    int rc = gather_result(H, s, path, R);
+    if (rc) return rc;
+    const std::vector<uint32_t> group(R.ne, s);
+    const femgpu_vtu_field pf[] = {{"dep", 3, R.u.data()}};
+    const femgpu_vtu_field cf[] = {{"elastic_modulus", 1, R.E.data()}, {"epsilon", 6, R.eps.data()},
+                                   {"sigma", 6, R.sig.data()}, {"tr_epsilon", 1, R.tr.data()}};
+    const femgpu_vtu_field gf[] = {{"poisson_ratio", 1, &R.nu}};
+    return femgpu_write_vtu(path, H.P.kind, R.nn, R.coords.data(), R.ne, R.conn.data(), 1, pf, 4, cf, group.data(), 1,
+                            gf);
+}
+
+// Sub-problem s as an AVS file, with the shapes of the reference's elasticity mesh data: dep, pos
+// (Vec<T,3>); elastic_modulus (a scalar per element), epsilon, sigma (Vec<Vec<T,6>,1>),
+// tr_epsilon (Vec<T,1>); poisson_ratio (scalar), f_vol (Vec<T,3>).
+int write_result_avs(femgpu_problem &H, uint32_t s, const char *path, uint32_t n_display, const char *const *display) {
+    ResultData R;
+    int rc = gather_result(H, s, path, R);
+    if (rc) return rc;
+    const femgpu_avs_field pf[] = {{"dep", 3, 1, R.u.data()}, {"pos", 3, 1, R.coords.data()}};
+    const femgpu_avs_field cf[] = {{"elastic_modulus", 1, 0, R.E.data()}, {"epsilon", 6, 2, R.eps.data()},
+                                   {"sigma", 6, 2, R.sig.data()}, {"tr_epsilon", 1, 1, R.tr.data()}};
+    const femgpu_avs_field gf[] = {{"poisson_ratio", 1, 0, &R.nu}, {"f_vol", 3, 1, R.bf}};
+    static const char *const all[] = {"all"};
+    if (!n_display || !display) {
+        n_display = 1;
+        display = all;
+    }
+    return femgpu_write_avs(path, H.P.kind, R.nn, R.coords.data(), R.ne, R.conn.data(), 2, pf, 4, cf, 2, gf, n_display,
+                            display);
 }
 }  // namespace
 
@@ -1185,6 +1265,17 @@ int femgpu_write_result_vtu(femgpu_problem *h, const char *path) {
 int femgpu_batch_write_result_vtu(femgpu_batch *b, uint32_t i, const char *path) {
     if (!b) return fail(FEMGPU_BAD_INPUT, "batch_write_result_vtu: null");
     return write_result(b->H, i, path);
+}
+
+int femgpu_write_result_avs(femgpu_problem *h, const char *path, uint32_t n_display, const char *const *display) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "write_result_avs: null");
+    return write_result_avs(*h, 0, path, n_display, display);
+}
+
+int femgpu_batch_write_result_avs(femgpu_batch *b, uint32_t i, const char *path, uint32_t n_display,
+                                  const char *const *display) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_write_result_avs: null");
+    return write_result_avs(b->H, i, path, n_display, display);
 }
 
 int femgpu_elem_grad(femgpu_problem *h, const double *u, const double *lam, double *g) {

@@ -435,6 +435,14 @@ class Formulation:
         tr_epsilon; poisson_ratio."""
         A.check(self._lib.femgpu_write_result_vtu(self._h, str(path).encode()), "write_result_vtu")
 
+    def write_avs(self, path, display=None):
+        """Mesh and current result as an AVS UCD ASCII file in the reference writer's layout
+        (write_avs(mesh, name, display_fields, Ascii()), mesh/write_avs.h:389-530): node fields
+        dep, pos; cell fields elastic_modulus, epsilon, sigma, tr_epsilon; global fields
+        poisson_ratio, f_vol; display: the names to write (None: every field)."""
+        nd, dp = A.display_names(display)
+        A.check(self._lib.femgpu_write_result_avs(self._h, str(path).encode(), nd, dp), "write_result_avs")
+
     def after_solve_ms(self):
         """Device time of the last after-solve kernel."""
         v = C.c_double()

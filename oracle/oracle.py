@@ -277,6 +277,34 @@ def ref_write_vtu_hexa(nb_points, coords, u, E, nu, tmpdir):
     return data, d[:, :6].copy(), d[:, 6:12].copy(), d[:, 12].copy()
 
 
+def ref_write_avs(kind_name, nb_points, coords, u, E, nu, display, tmpdir):
+    """The reference's ASCII write_avs for make_rect(kind, nb_points) with node positions coords:
+    the elasticity_dep_static Hexa mesh with displacements u, E, nu and the strain and stress of
+    its generated after_solve, or (tetra, tetra10) the field-less standard mesh.  Returns (file
+    bytes, eps[n,6], sigma[n,6], tr[n]); the last three are None for the field-less meshes."""
+    inp = os.path.join(tmpdir, "avs_in.bin")
+    out = os.path.join(tmpdir, "avs_out.inp")
+    flds = os.path.join(tmpdir, "avs_fields.bin")
+    if os.path.exists(flds):
+        os.remove(flds)
+    coords, u = _f64(coords), _f64(u)
+    with open(inp, "wb") as f:
+        f.write(np.array([coords.shape[0]], dtype=np.uint32).tobytes())
+        f.write(coords.tobytes())
+        f.write(u.tobytes())
+        f.write(np.array([E, nu], dtype=np.float64).tobytes())
+    subprocess.check_call([os.path.join(REF_DIR, "ref_write_avs"), kind_name] + [str(v) for v in nb_points] +
+                          [inp, out, flds] + list(display), stdout=subprocess.DEVNULL)
+    with open(out, "rb") as f:
+        data = f.read()
+    if not os.path.exists(flds):
+        return data, None, None, None
+    with open(flds, "rb") as f:
+        ne = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
+        d = np.frombuffer(f.read(8 * 13 * ne), dtype=np.float64).reshape(ne, 13)
+    return data, d[:, :6].copy(), d[:, 6:12].copy(), d[:, 12].copy()
+
+
 def ref_symrcm(kind_name, nb_points, tmpdir):
     """The reference's symrcm(m) order on make_rect(kind, nb_points)."""
     out = os.path.join(tmpdir, "symrcm.bin")

@@ -13,10 +13,14 @@ run where those sources exist.  Fixtures are data only (inputs and the reference
            definition_probleme_elasticity_dep_static.h:721-789 on a jittered make_rect mesh
   ref_write_vtu_hexa_3_4_2.npz         coords, conn, u, E, nu, eps, sigma, tr -> vtu (file bytes)
         -- write_mesh_vtk(filename, m) (mesh/write_mesh_vtk.h:434-656,827-831) on that mesh
+  ref_write_avs.npz                    per case <name>_{coords,conn,u,E,nu,eps,sigma,tr,display,avs}:
+        write_avs(mesh, name, display_fields, Ascii()) (mesh/write_avs.h:389-530) on that Hexa
+        mesh (displays all / dep,sigma,poisson_ratio,tr_epsilon / f_vol,epsilon / none) and on
+        field-less Tetra and Tetra_10 meshes (all)
   ref_symrcm.npz                       tetra_7_6_5, hexa_7_6_5: symrcm(m) (util/symrcm.cpp:38-101)
         node orders of make_rect meshes
 
-    python tests/golden/make_golden.py [after_solve|vtu|symrcm]   (no argument: every fixture)
+    python tests/golden/make_golden.py [after_solve|vtu|avs|symrcm]   (no argument: every fixture)
 """
 import json
 import os
@@ -74,6 +78,30 @@ def vtu_fixture(tmp):
                         eps=eps, sigma=sig, tr=tr, vtu=np.frombuffer(data, dtype=np.uint8))
 
 
+def avs_fixture(tmp):
+    out = {}
+    rng = np.random.default_rng(91)
+    cases = [("hexa_all", "hexa", (4, 3, 3), ["all"]),
+             ("hexa_some", "hexa", (4, 3, 3), ["dep", "sigma", "poisson_ratio", "tr_epsilon"]),
+             ("hexa_vol_eps", "hexa", (4, 3, 3), ["f_vol", "epsilon"]),
+             ("hexa_none", "hexa", (4, 3, 3), ["none"]),
+             ("tetra_all", "tetra", (3, 3, 3), ["all"]),
+             ("tetra10_all", "tetra10", (3, 3, 3), ["all"])]
+    for name, kind, nb, display in cases:
+        c, e = O.ref_mesh(kind, nb, tmp)
+        inside = np.all((c > 1e-9) & (c < 1 - 1e-9), axis=1)
+        c = c.copy()
+        c[inside] += rng.uniform(-0.05, 0.05, size=(int(inside.sum()), 3))
+        u = 1e-3 * rng.standard_normal(3 * len(c))
+        E, nu = 201000.0, 0.29
+        data, eps, sig, tr = O.ref_write_avs(kind, nb, c, u, E, nu, display, tmp)
+        out.update({name + "_coords": c, name + "_conn": e, name + "_u": u, name + "_E": E, name + "_nu": nu,
+                    name + "_display": np.array(display), name + "_avs": np.frombuffer(data, dtype=np.uint8)})
+        if eps is not None:
+            out.update({name + "_eps": eps, name + "_sigma": sig, name + "_tr": tr})
+    np.savez_compressed(os.path.join(HERE, "ref_write_avs.npz"), **out)
+
+
 def symrcm_fixture(tmp):
     out = {}
     for name, nb in (("tetra", (7, 6, 5)), ("hexa", (7, 6, 5))):
@@ -93,8 +121,12 @@ def main():
     if sys.argv[1:] == ["vtu"]:
         vtu_fixture(tmp)
         return
+    if sys.argv[1:] == ["avs"]:
+        avs_fixture(tmp)
+        return
     after_solve_fixture(tmp)
     vtu_fixture(tmp)
+    avs_fixture(tmp)
     symrcm_fixture(tmp)
     for name, kind, nb in MESHES:
         c, e = O.ref_mesh(name, nb, tmp)

@@ -107,3 +107,27 @@ def test_result_file_holds_result_and_recovery(tmp_path):
     assert (arr(cd["poisson_ratio"], len(e)) == 0.3).all()
     assert np.array_equal(arr(piece.find("Points/DataArray"), len(c)), c)
     f.close()
+
+
+def test_result_avs_holds_result_and_recovery(tmp_path):
+    """femgpu_write_result_avs: the problem's mesh, dep, pos, E and the after-solve fields in the
+    reference writer's layout, byte for byte what femgpu_write_avs writes from the same values
+    read back through the API (its byte equality with the reference's writer is a CPU test)."""
+    from lmtpp_amd import _capi as A
+    say("write_result_avs")
+    c, e = L.make_rect(L.HEXA, (5, 4, 4))
+    c = jitter_interior(c, 0.2, 0.1, 4)
+    E = lognormal_E(len(e), 0.3, 4)
+    f, *_ = build(L.HEXA, c, e, E, bf=(0.0, 0.1, -0.2))
+    assert f.solve(rel_tol=1e-11)
+    eps, sig, tr = f.call_after_solve()
+    u = f.get_result()
+    for display in (None, ["sigma", "dep", "poisson_ratio"]):
+        got, ref = tmp_path / "res.inp", tmp_path / "ref.inp"
+        f.write_avs(got, display)
+        A.write_avs(ref, L.HEXA, c, e, point=[("dep", u), ("pos", c)],
+                    cell=[("elastic_modulus", E, 0), ("epsilon", eps, 2), ("sigma", sig, 2), ("tr_epsilon", tr)],
+                    glob=[("poisson_ratio", [0.3], 0), ("f_vol", [0.0, 0.1, -0.2])],
+                    display=display or ["all"])
+        assert got.read_bytes() == ref.read_bytes()
+    f.close()

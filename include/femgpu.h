@@ -517,13 +517,20 @@ typedef enum {
 FEMGPU_API int femgpu_set_row_order(int order);
 /* Order a problem uses (NATURAL or BY_LENGTH) and its SELL positions (blocks + padding). */
 FEMGPU_API int femgpu_get_row_order(femgpu_problem *p, int *order, uint64_t *n_positions);
-/* want_rcm (formulation.h:319-323, util/symrcm.h:30-35): problems and batches created afterwards
- * store their nodes in the Cuthill-McKee order of femgpu_node_order (per batch member), so that
- * the block SpMV gathers neighbouring values from neighbouring memory whatever the caller's node
- * numbering.  The order is internal: every API array (results, loads, constraints, faces, the
- * CSR view, result files) keeps the caller's numbering, and FEMGPU_BLOCK_IC0 still eliminates in
- * the caller's node order (the reference's factor).  Process-wide; default NATURAL. */
-typedef enum { FEMGPU_NODE_ORDER_NATURAL = 0, FEMGPU_NODE_ORDER_RCM = 1 } femgpu_node_order_kind;
+/* want_rcm / want_amd (formulation.h:314-323, util/symrcm.h:30-35, util/symamd.h:27-36):
+ * problems and batches created afterwards store their nodes in the Cuthill-McKee order of
+ * femgpu_node_order (RCM), so that the block SpMV gathers neighbouring values from neighbouring
+ * memory whatever the caller's node numbering, or in the approximate minimum degree order of
+ * femgpu_node_order_amd (AMD: a fill-reducing order for direct factorisations, which scatters
+ * neighbours in memory and slows the iterative path), per batch member.  The order is internal:
+ * every API array (results, loads, constraints, faces, the CSR view, result files) keeps the
+ * caller's numbering, and FEMGPU_BLOCK_IC0 still eliminates in the caller's node order (the
+ * reference's factor).  Process-wide; default NATURAL. */
+typedef enum {
+    FEMGPU_NODE_ORDER_NATURAL = 0,
+    FEMGPU_NODE_ORDER_RCM = 1,
+    FEMGPU_NODE_ORDER_AMD = 2
+} femgpu_node_order_kind;
 FEMGPU_API int femgpu_set_node_order(int order);
 /* Order a problem uses; perm (n_nodes, may be NULL) receives the caller's node held at each
  * internal position. */
@@ -533,6 +540,14 @@ FEMGPU_API int femgpu_get_node_order(femgpu_problem *p, int *order, uint32_t *pe
  * order), unreached components restarted from their lowest-degree node.  row_ptr / col: node
  * adjacency, every row holding its own node (the 3x3-block pattern).  order: n_brows nodes. */
 FEMGPU_API int femgpu_node_order(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, uint32_t *order);
+/* Host only: the approximate minimum degree order symamd computes (util/symamd.cpp:16-36 with the
+ * AMD 1.1 package the reference vendors, default controls: rows of degree above
+ * max(16, 10 sqrt(n)) ordered last, aggressive absorption, assembly-tree postorder).  row_ptr /
+ * col: a symmetric node adjacency, rows strictly increasing, every row holding its own node.
+ * order: n_brows nodes, the node at each position (symamd(m) returns the inverse, the position
+ * of each node, which want_amd multiplies into indice_noda). */
+FEMGPU_API int femgpu_node_order_amd(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col,
+                                     uint32_t *order);
 /* Host only: the storage order the library would choose for a block-row length profile
  * (row_ptr, n_brows + 1 entries; one sub-problem).  perm[t] = block row stored at t (may be
  * NULL); *n_natural and *n_positions receive the SELL positions of the natural and the chosen

@@ -19,7 +19,7 @@ REL_L2, ABS_MAXNORM = 0, 1
 SPMV_FULL, SPMV_SYM, SPMV_MATRIX_FREE = 0, 1, 2
 LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
 ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
-NODE_ORDER_NATURAL, NODE_ORDER_RCM = 0, 1
+NODE_ORDER_NATURAL, NODE_ORDER_RCM, NODE_ORDER_AMD = 0, 1, 2
 ASSEMBLY_STAGED = 0xFFFFFFFF
 ASSEMBLY_GATHER = 0xFFFFFFFE
 ASSEMBLY_GATHER_BOX = 0xFFFFFFFD
@@ -136,6 +136,7 @@ _SIGS = {
     "femgpu_ic0_schedule": (C.c_int, [C.c_uint32, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_uint32)]),
     "femgpu_set_row_order": (C.c_int, [C.c_int]),
     "femgpu_node_order": (C.c_int, [C.c_uint32, _P, _P, _P]),
+    "femgpu_node_order_amd": (C.c_int, [C.c_uint32, _P, _P, _P]),
     "femgpu_set_node_order": (C.c_int, [C.c_int]),
     "femgpu_get_node_order": (C.c_int, [_P, C.POINTER(C.c_int), _P]),
     "femgpu_write_vtu": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P,
@@ -318,6 +319,18 @@ def node_order(row_ptr, col):
     cl = np.ascontiguousarray(col, dtype=np.uint32)
     o = np.zeros(len(rp) - 1, np.uint32)
     check(lib.femgpu_node_order(len(rp) - 1, rp.ctypes.data, cl.ctypes.data, o.ctypes.data), "node_order")
+    return o
+
+
+def node_order_amd(row_ptr, col):
+    """Approximate minimum degree order of a symmetric node graph, the node at each position
+    (femgpu_node_order_amd, host only; symamd's result is its inverse)."""
+    import numpy as np
+    lib = load()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    cl = np.ascontiguousarray(col, dtype=np.uint32)
+    o = np.zeros(len(rp) - 1, np.uint32)
+    check(lib.femgpu_node_order_amd(len(rp) - 1, rp.ctypes.data, cl.ctypes.data, o.ctypes.data), "node_order_amd")
     return o
 
 

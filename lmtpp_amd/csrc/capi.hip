@@ -345,13 +345,16 @@ int build_problem(femgpu_problem &H, uint32_t n_sub, const femgpu_mesh *meshes) 
         std::fill(esub.begin() + P.elem_off[s], esub.begin() + P.elem_off[s + 1], s);
     }
     (void)hipGetDevice(&P.device);
-    if (node_order_default() == NODE_ORDER_CM) {
-        // symrcm-style renumbering (formulation.h:319-323), per sub-problem
+    const int node_order = node_order_default();
+    if (node_order != NODE_ORDER_NATURAL) {
+        // want_rcm / want_amd renumbering (formulation.h:314-323), per sub-problem
         const Pattern api = build_pattern(P.n_nodes, P.n_elem, npe, conn.data());
         std::vector<uint32_t> perm;
         perm.reserve(P.n_nodes);
         for (uint32_t s = 0; s < n_sub; ++s) {
-            const std::vector<uint32_t> o = cm_order(api, P.node_off[s], P.node_off[s + 1]);
+            const std::vector<uint32_t> o = node_order == NODE_ORDER_AMD
+                                                ? amd_order(api, P.node_off[s], P.node_off[s + 1])
+                                                : cm_order(api, P.node_off[s], P.node_off[s + 1]);
             perm.insert(perm.end(), o.begin(), o.end());
         }
         bool identity = true;
@@ -360,6 +363,7 @@ int build_problem(femgpu_problem &H, uint32_t n_sub, const femgpu_mesh *meshes) 
             P.ninv.assign(P.n_nodes, 0);
             for (uint32_t i = 0; i < P.n_nodes; ++i) P.ninv[perm[i]] = i;
             P.nperm = std::move(perm);
+            P.node_order = node_order;
             for (uint32_t &v : conn) v = P.ninv[v];
             std::vector<double> c2(coords.size());
             for (uint32_t i = 0; i < P.n_nodes; ++i)
@@ -1566,7 +1570,8 @@ int femgpu_get_row_order(femgpu_problem *h, int *order, uint64_t *n_positions) {
 }
 
 int femgpu_set_node_order(int order) {
-    if (order != NODE_ORDER_NATURAL && order != NODE_ORDER_CM) return fail(FEMGPU_BAD_INPUT, "set_node_order: 0 or 1");
+    if (order != NODE_ORDER_NATURAL && order != NODE_ORDER_CM && order != NODE_ORDER_AMD)
+        return fail(FEMGPU_BAD_INPUT, "set_node_order: 0, 1 or 2");
     g_node_order.store(order);
     return FEMGPU_OK;
 }
@@ -1574,27 +1579,58 @@ int femgpu_set_node_order(int order) {
 int femgpu_get_node_order(femgpu_problem *h, int *order, uint32_t *perm) {
     if (!h) return fail(FEMGPU_BAD_INPUT, "get_node_order: null");
     const Problem &P = h->P;
-    if (order) *order = P.nperm.empty() ? NODE_ORDER_NATURAL : NODE_ORDER_CM;
+    if (order) *order = P.nperm.empty() ? NODE_ORDER_NATURAL : P.node_order;
     if (perm)
         for (uint32_t i = 0; i < P.n_nodes; ++i) perm[i] = P.nperm.empty() ? i : P.nperm[i];
     return FEMGPU_OK;
 }
 
-int femgpu_node_order(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, uint32_t *order) {
-    if (!n_brows || !row_ptr || !col || !order) return fail(FEMGPU_BAD_INPUT, "node_order: bad input");
-    Pattern pat;
-    pat.n_rows = n_brows;
-    pat.row_ptr.assign(row_ptr, row_ptr + n_brows + 1);
+namespace {
+// A node graph given as CSR: every row holds its node; with sym, rows strictly increasing and
+// the pattern symmetric.
+int graph_pattern(const char *what, uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, bool sym,
+                  Pattern &pat) {
+    if (!n_brows || !row_ptr || !col) return fail(FEMGPU_BAD_INPUT, std::string(what) + ": bad input");
+    if (row_ptr[0] != 0) return fail(FEMGPU_BAD_INPUT, std::string(what) + ": row_ptr[0] must be 0");
     for (uint32_t r = 0; r < n_brows; ++r) {
+        if (row_ptr[r + 1] < row_ptr[r]) return fail(FEMGPU_BAD_INPUT, std::string(what) + ": row_ptr decreasing");
         bool has_diag = false;
         for (uint32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
-            if (col[p] >= n_brows) return fail(FEMGPU_BAD_INPUT, "node_order: column out of range");
+            if (col[p] >= n_brows) return fail(FEMGPU_BAD_INPUT, std::string(what) + ": column out of range");
+            if (sym && p > row_ptr[r] && col[p] <= col[p - 1])
+                return fail(FEMGPU_BAD_INPUT, std::string(what) + ": rows must be strictly increasing");
             has_diag |= col[p] == r;
         }
-        if (!has_diag) return fail(FEMGPU_BAD_INPUT, "node_order: every row must hold its node");
+        if (!has_diag) return fail(FEMGPU_BAD_INPUT, std::string(what) + ": every row must hold its node");
     }
+    if (sym)
+        for (uint32_t r = 0; r < n_brows; ++r)
+            for (uint32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+                if (!std::binary_search(col + row_ptr[col[p]], col + row_ptr[col[p] + 1], r))
+                    return fail(FEMGPU_BAD_INPUT, std::string(what) + ": the pattern must be symmetric");
+    pat.n_rows = n_brows;
+    pat.row_ptr.assign(row_ptr, row_ptr + n_brows + 1);
     pat.col.assign(col, col + row_ptr[n_brows]);
+    return FEMGPU_OK;
+}
+}  // namespace
+
+int femgpu_node_order(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, uint32_t *order) {
+    Pattern pat;
+    if (!order) return fail(FEMGPU_BAD_INPUT, "node_order: bad input");
+    int rc = graph_pattern("node_order", n_brows, row_ptr, col, false, pat);
+    if (rc) return rc;
     const std::vector<uint32_t> o = cm_order(pat, 0, n_brows);
+    std::memcpy(order, o.data(), 4 * (size_t)n_brows);
+    return FEMGPU_OK;
+}
+
+int femgpu_node_order_amd(uint32_t n_brows, const uint32_t *row_ptr, const uint32_t *col, uint32_t *order) {
+    Pattern pat;
+    if (!order) return fail(FEMGPU_BAD_INPUT, "node_order_amd: bad input");
+    int rc = graph_pattern("node_order_amd", n_brows, row_ptr, col, true, pat);
+    if (rc) return rc;
+    const std::vector<uint32_t> o = amd_order(pat, 0, n_brows);
     std::memcpy(order, o.data(), 4 * (size_t)n_brows);
     return FEMGPU_OK;
 }

@@ -51,8 +51,9 @@ enum { ROW_ORDER_AUTO = 0, ROW_ORDER_NATURAL = 1, ROW_ORDER_BY_LENGTH = 2 };
 constexpr uint32_t SORT_WINDOW_WGS = 4;
 int row_order_default();
 // Node order chosen when a mesh problem is created: NATURAL keeps the caller's numbering, CM
-// stores nodes in the Cuthill-McKee order of symrcm (cm_order) for locality.
-enum { NODE_ORDER_NATURAL = 0, NODE_ORDER_CM = 1 };
+// stores nodes in the Cuthill-McKee order of symrcm (cm_order) for locality, AMD in the
+// approximate minimum degree order of symamd (amd_order).
+enum { NODE_ORDER_NATURAL = 0, NODE_ORDER_CM = 1, NODE_ORDER_AMD = 2 };
 int node_order_default();
 
 // Block-SpMV variant per precision (index 0 = FP64, 1 = FP32): storage format, SELL slots per loop
@@ -201,6 +202,7 @@ struct Problem {
     // the caller's (block-diagonal over sub-problems)
     std::vector<uint32_t> nperm, ninv;
     uint32_t *d_nperm = nullptr, *d_ninv = nullptr;
+    int node_order = NODE_ORDER_NATURAL;  // the order nperm follows (NATURAL when empty)
     uint32_t *d_conn = nullptr, *d_rp = nullptr, *d_col = nullptr, *d_diag = nullptr, *d_slot = nullptr,
              *d_order = nullptr, *d_sptr = nullptr, *d_elem_sub = nullptr;
     SubMaterial *d_mat = nullptr;

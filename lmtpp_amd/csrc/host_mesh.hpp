@@ -91,6 +91,11 @@ RowGroups build_row_groups(uint32_t n_rows, uint32_t R, uint32_t n_elem, int npe
 // reference numbers only the first component).  Degrees count neighbours other than the node.
 std::vector<uint32_t> cm_order(const Pattern &pat, uint32_t r0, uint32_t r1);
 
+// Approximate minimum degree order of the nodes of [r0, r1) (rows with no coupling outside the
+// range): the order symamd (util/symamd.h:27-36) computes with the AMD 1.1 package it vendors,
+// default controls.  Returns the node at each position.
+std::vector<uint32_t> amd_order(const Pattern &pat, uint32_t r0, uint32_t r1);
+
 // Contiguous row ranges, balanced by stored blocks, for n_parts workgroups.
 std::vector<uint32_t> balance_rows(const std::vector<uint32_t> &row_ptr, uint32_t n_parts);
 

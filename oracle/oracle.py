@@ -314,6 +314,29 @@ def ref_symrcm(kind_name, nb_points, tmpdir):
     return d[2:2 + d[1]].copy()
 
 
+def ref_symamd(kind_name, nb_points, tmpdir):
+    """The reference's symamd(m) on make_rect(kind, nb_points): the position of each node."""
+    out = os.path.join(tmpdir, "symamd.bin")
+    subprocess.check_call([os.path.join(REF_DIR, "ref_symamd"), kind_name] + [str(v) for v in nb_points] + [out],
+                          stdout=subprocess.DEVNULL)
+    d = np.fromfile(out, dtype=np.uint32)
+    return d[2:2 + d[1]].copy()
+
+
+def ref_symamd_graph(row_ptr, col, tmpdir):
+    """The reference's symamd(ind) on a node graph (rows sorted, each holding its node): the
+    position of each node."""
+    inp = os.path.join(tmpdir, "symamd_graph.bin")
+    out = os.path.join(tmpdir, "symamd.bin")
+    with open(inp, "wb") as f:
+        f.write(np.array([len(row_ptr) - 1], dtype=np.uint32).tobytes())
+        f.write(_u32(row_ptr).tobytes())
+        f.write(_u32(col).tobytes())
+    subprocess.check_call([os.path.join(REF_DIR, "ref_symamd"), "graph", inp, out], stdout=subprocess.DEVNULL)
+    d = np.fromfile(out, dtype=np.uint32)
+    return d[2:2 + d[1]].copy()
+
+
 def ref_tinyblock_pcg(row_ptr, col, vals, b, x0, crit, tmpdir):
     inp = os.path.join(tmpdir, "pcg_in.bin")
     out = os.path.join(tmpdir, "pcg_out.bin")

@@ -17,10 +17,13 @@ run where those sources exist.  Fixtures are data only (inputs and the reference
         write_avs(mesh, name, display_fields, Ascii()) (mesh/write_avs.h:389-530) on that Hexa
         mesh (displays all / dep,sigma,poisson_ratio,tr_epsilon / f_vol,epsilon / none) and on
         field-less Tetra and Tetra_10 meshes (all)
+  ref_symamd.npz                       <mesh>: symamd(m) (util/symamd.h:27-36 with the vendored AMD 1.1)
+        node positions of make_rect meshes; graph<k>_{row_ptr,col,pos}: symamd(ind)
+        (util/symamd.cpp:16-36) on random symmetric graphs, some with dense rows and empty ones
   ref_symrcm.npz                       tetra_7_6_5, hexa_7_6_5: symrcm(m) (util/symrcm.cpp:38-101)
         node orders of make_rect meshes
 
-    python tests/golden/make_golden.py [after_solve|vtu|avs|symrcm]   (no argument: every fixture)
+    python tests/golden/make_golden.py [after_solve|vtu|avs|symrcm|symamd]   (no argument: every fixture)
 """
 import json
 import os
@@ -102,6 +105,28 @@ def avs_fixture(tmp):
     np.savez_compressed(os.path.join(HERE, "ref_write_avs.npz"), **out)
 
 
+def symamd_fixture(tmp):
+    out = {}
+    for name, nb in (("tetra", (7, 6, 5)), ("hexa", (7, 6, 5)), ("tetra10", (4, 3, 3)), ("hexa", (13, 12, 11))):
+        out["%s_%d_%d_%d" % ((name,) + nb)] = O.ref_symamd(name, nb, tmp)
+    rng = np.random.default_rng(5)
+    for k, n in enumerate((40, 150, 300, 500)):
+        M = rng.random((n, n)) < 6.0 / n
+        if k >= 2:   # rows denser than max(16, 10 sqrt(n)), and rows with no neighbour
+            for h in rng.integers(0, n, 3):
+                M[h, :] = rng.random(n) < 0.9
+            for z in rng.integers(0, n, 4):
+                M[z, :] = False
+                M[:, z] = False
+        M = M | M.T
+        np.fill_diagonal(M, True)
+        rp = np.concatenate([[0], np.cumsum(M.sum(1))]).astype(np.uint32)
+        col = np.concatenate([np.nonzero(row)[0] for row in M]).astype(np.uint32)
+        out.update({"graph%d_row_ptr" % k: rp, "graph%d_col" % k: col,
+                    "graph%d_pos" % k: O.ref_symamd_graph(rp, col, tmp)})
+    np.savez_compressed(os.path.join(HERE, "ref_symamd.npz"), **out)
+
+
 def symrcm_fixture(tmp):
     out = {}
     for name, nb in (("tetra", (7, 6, 5)), ("hexa", (7, 6, 5))):
@@ -121,6 +146,9 @@ def main():
     if sys.argv[1:] == ["vtu"]:
         vtu_fixture(tmp)
         return
+    if sys.argv[1:] == ["symamd"]:
+        symamd_fixture(tmp)
+        return
     if sys.argv[1:] == ["avs"]:
         avs_fixture(tmp)
         return
@@ -128,6 +156,7 @@ def main():
     vtu_fixture(tmp)
     avs_fixture(tmp)
     symrcm_fixture(tmp)
+    symamd_fixture(tmp)
     for name, kind, nb in MESHES:
         c, e = O.ref_mesh(name, nb, tmp)
         np.savez_compressed(os.path.join(HERE, "ref_mesh_%s_%d_%d_%d.npz" % ((name,) + nb)), coords=c, conn=e)

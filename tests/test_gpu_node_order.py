@@ -1,5 +1,5 @@
-"""Nodes stored in Cuthill-McKee order (femgpu_set_node_order RCM, the reference's want_rcm)
-against the caller's order, through the C ABI, on meshes whose node numbering is shuffled.
+"""Nodes stored in Cuthill-McKee order (femgpu_set_node_order RCM, the reference's want_rcm) or in
+approximate minimum degree order (AMD, the reference's want_amd) against the caller's order, through the C ABI, on meshes whose node numbering is shuffled.
 
 The node order is internal: K (as the CSR view in the caller's numbering) and F must match the
 oracle's assembly of the shuffled mesh (pattern exact, values to 1e-13 of the scale), solutions
@@ -47,13 +47,17 @@ def _make(kind, c, e, E, order, bf=(0.1, -0.2, 0.3)):
     return f, faces, tr, dofs
 
 
-@pytest.mark.parametrize("kind", [L.TETRA, L.TETRA10, L.HEXA])
-def test_rcm_system_and_solution_match_the_oracle(oracle, kind):
-    say("node order system", kind)
+@pytest.mark.parametrize("kind,node_order", [(L.TETRA, A.NODE_ORDER_RCM), (L.TETRA10, A.NODE_ORDER_RCM),
+                                             (L.HEXA, A.NODE_ORDER_RCM), (L.TETRA10, A.NODE_ORDER_AMD),
+                                             (L.HEXA, A.NODE_ORDER_AMD)])
+def test_reordered_system_and_solution_match_the_oracle(oracle, kind, node_order):
+    say("node order system", kind, node_order)
     c, e, E = _shuffled(kind, 80 + kind)
-    f, faces, tr, dofs = _make(kind, c, e, E, A.NODE_ORDER_RCM)
+    f, faces, tr, dofs = _make(kind, c, e, E, node_order)
     order, perm = f.node_order()
-    assert order == A.NODE_ORDER_RCM and np.array_equal(np.sort(perm), np.arange(len(c)))
+    assert order == node_order and np.array_equal(np.sort(perm), np.arange(len(c)))
+    if node_order == A.NODE_ORDER_AMD:     # the internal order is the library's symamd order
+        assert np.array_equal(perm, A.node_order_amd(*oracle.pattern(kind, len(c), e)))
     f.assemble()
     rp, col, vals = f.get_mat()
     F = f.get_sollicitation()
@@ -82,7 +86,7 @@ def test_order_does_not_change_factor_recovery_gradient_or_file(tmp_path):
     kind = L.HEXA
     c, e, E = _shuffled(kind, 91)
     out = {}
-    for order in (A.NODE_ORDER_NATURAL, A.NODE_ORDER_RCM):
+    for order in (A.NODE_ORDER_NATURAL, A.NODE_ORDER_RCM, A.NODE_ORDER_AMD):
         f, *_ = _make(kind, c, e, E, order)
         f.precond = L.BLOCK_IC0
         assert f.solve(rel_tol=1e-12)
@@ -98,14 +102,15 @@ def test_order_does_not_change_factor_recovery_gradient_or_file(tmp_path):
         g = f.adjoint(rng.standard_normal(u.size), rel_tol=1e-13)
         out[order] = (u, fac, eps, sig, tr, path.read_bytes(), g)
         f.close()
-    a, b = out[A.NODE_ORDER_NATURAL], out[A.NODE_ORDER_RCM]
-    assert np.linalg.norm(a[0] - b[0]) <= 1e-10 * np.linalg.norm(a[0])
-    for x, y in zip(a[1], b[1]):
-        assert np.array_equal(x, y)                        # the reference's factor in both orders
-    for x, y in zip(a[2:5], b[2:5]):
-        assert np.array_equal(x, y)
-    assert a[5] == b[5]
-    assert np.linalg.norm(a[6] - b[6]) <= 1e-9 * np.linalg.norm(a[6])
+    a = out[A.NODE_ORDER_NATURAL]
+    for b in (out[A.NODE_ORDER_RCM], out[A.NODE_ORDER_AMD]):
+        assert np.linalg.norm(a[0] - b[0]) <= 1e-10 * np.linalg.norm(a[0])
+        for x, y in zip(a[1], b[1]):
+            assert np.array_equal(x, y)                    # the reference's factor in every order
+        for x, y in zip(a[2:5], b[2:5]):
+            assert np.array_equal(x, y)
+        assert a[5] == b[5]
+        assert np.linalg.norm(a[6] - b[6]) <= 1e-9 * np.linalg.norm(a[6])
 
 
 def test_batch_members_renumbered_like_single_problems():

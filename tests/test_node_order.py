@@ -1,15 +1,26 @@
-"""Cuthill-McKee node order (femgpu_node_order, host only) against the reference's symrcm.
+"""Node orders (femgpu_node_order, femgpu_node_order_amd, host only) against the reference's
+symrcm and symamd.
 
 The golden orders are the reference's symrcm(m) (util/symrcm.cpp:38-101 with
 util/GetLinkForOrdering.h), compiled from source, on make_rect Tetra and Hexa meshes; the library
 must give the same visiting order.  Further checks: a permutation of the nodes, a breadth-first
 visit (every node after the first is adjacent to an earlier one within its component), a start at
 the lowest degree, unreached components restarted, and a profile (sum of squared row spans, the
-reference's symrcm_cost) much smaller than the one of a shuffled numbering.  No GPU compute.
+reference's symrcm_cost) much smaller than the one of a shuffled numbering.
+
+The golden AMD orders are the reference's symamd(m) (util/symamd.h:27-36) and symamd(ind)
+(util/symamd.cpp:16-36) with the AMD 1.1 package the reference vendors (amd/, built with -DDINT
+as its SConscript does), compiled from source, on make_rect Tetra, Tetra_10 and Hexa meshes and
+on random symmetric graphs with rows denser than the package's threshold and rows with no
+neighbour; the library must give the same positions.  Further checks from the mathematics: a
+tree is ordered with no fill at all (minimum degree eliminates leaves first), and on a mesh the
+fill of the Cholesky factor (counted by the elimination game) falls far below the natural
+order's.  No GPU compute.
 """
 import os
 
 import numpy as np
+import pytest
 
 import lmtpp_amd as L
 from lmtpp_amd import _capi as A
@@ -64,3 +75,65 @@ def test_components_are_restarted():
     rp = np.cumsum([0] + [len(adj[i]) for i in range(5)]).astype(np.uint32)
     col = np.concatenate([adj[i] for i in range(5)]).astype(np.uint32)
     assert A.node_order(rp, col).tolist() == [0, 2, 4, 1, 3]
+
+
+def _positions(order):
+    pos = np.empty(len(order), np.int64)
+    pos[order] = np.arange(len(order))
+    return pos
+
+
+def _fill(rp, col, order):
+    """Entries the elimination game adds to the pattern when nodes are eliminated in order."""
+    adj = [set(col[rp[i]:rp[i + 1]].tolist()) - {i} for i in range(len(rp) - 1)]
+    added = 0
+    for v in order:
+        nb = list(adj[v])
+        for a in nb:
+            adj[a].discard(v)
+        for x in range(len(nb)):
+            for y in range(x + 1, len(nb)):
+                if nb[y] not in adj[nb[x]]:
+                    adj[nb[x]].add(nb[y])
+                    adj[nb[y]].add(nb[x])
+                    added += 1
+    return added
+
+
+def test_amd_matches_reference_symamd(oracle):
+    g = np.load(os.path.join(GOLDEN, "ref_symamd.npz"))
+    for key, kind, nb in (("tetra_7_6_5", L.TETRA, (7, 6, 5)), ("hexa_7_6_5", L.HEXA, (7, 6, 5)),
+                          ("tetra10_4_3_3", L.TETRA10, (4, 3, 3)), ("hexa_13_12_11", L.HEXA, (13, 12, 11))):
+        rp, col = _graph(oracle, kind, nb)
+        assert np.array_equal(_positions(A.node_order_amd(rp, col)), g[key]), key
+    for k in range(4):
+        rp, col = g["graph%d_row_ptr" % k], g["graph%d_col" % k]
+        assert np.array_equal(_positions(A.node_order_amd(rp, col)), g["graph%d_pos" % k]), k
+
+
+def test_amd_fill_properties(oracle):
+    # a random tree: no fill in the order
+    rng = np.random.default_rng(4)
+    n = 200
+    par = [int(rng.integers(0, i)) for i in range(1, n)]
+    adj = [{i} for i in range(n)]
+    for i, p in enumerate(par, start=1):
+        adj[i].add(p)
+        adj[p].add(i)
+    rp = np.cumsum([0] + [len(a) for a in adj]).astype(np.uint32)
+    col = np.concatenate([sorted(a) for a in adj]).astype(np.uint32)
+    o = A.node_order_amd(rp, col)
+    assert np.array_equal(np.sort(o), np.arange(n))
+    assert _fill(rp, col, o) == 0
+    # a mesh: far less fill than the natural order
+    rp, col = _graph(oracle, L.HEXA, (8, 8, 6))
+    o = A.node_order_amd(rp, col)
+    assert _fill(rp, col, o) < 0.6 * _fill(rp, col, np.arange(len(rp) - 1))
+
+
+def test_amd_bad_input():
+    rp = np.array([0, 2, 3], np.uint32)
+    with pytest.raises(A.FemGpuError):      # 0 -> 1 without 1 -> 0
+        A.node_order_amd(rp, np.array([0, 1, 1], np.uint32))
+    with pytest.raises(A.FemGpuError):      # unsorted row
+        A.node_order_amd(np.array([0, 2, 4], np.uint32), np.array([1, 0, 0, 1], np.uint32))

@@ -1,29 +1,16 @@
-"""HBM traffic of the block SpMV, from rocprofv3 FETCH_SIZE / WRITE_SIZE counters.
+"""Byte and arithmetic model of the product kernels, and the reader of rocprofv3 FETCH_SIZE /
+WRITE_SIZE counters that compares them with measured HBM traffic.
 
-Workload (run it under rocprofv3 --pmc, one counter per run):
+The algorithmic bytes and flops of a launch (spmv_bytes, mf_block_bytes, mf_halo_bytes, mf_flops)
+are what the roofline fractions are computed on.  The counter workloads that run under rocprofv3
+live outside the package (profile_workloads.py at the repository root); their counters are read
+back with
 
-    python -m lmtpp_amd.traffic run [--n 100]
+    python -m lmtpp_amd.traffic parse <fetch_dir> <write_dir> [nnzb n_nodes] > profiles/spmv_traffic.json
 
-launches the plain copy kernel (k_copy, 2^26 doubles: exactly 512 MiB read and 512 MiB written),
-then 5 FP64 and 5 FP32 block SpMV launches on Hexa8 n^3 for each variant: full storage with
-non-temporal matrix loads (the default), full storage with plain loads, and symmetric storage
-(upper blocks stored, lower ones read transposed); then the matrix-free product (its element-block
-kernel and its shared-node pass).  Then
-
-    python -m lmtpp_amd.traffic parse <fetch_dir> <write_dir> > profiles/spmv_traffic.json
-
-reads the per-dispatch counters (KiB), scales each counter by the copy kernel's known bytes
-(factor = known KiB / counted KiB), and reports bytes per SpMV launch next to the algorithmic
-bytes.
-
-    python -m lmtpp_amd.traffic assembly [--n 100]
-
-is the workload of the element-loop counters (both assembly variants, three times each), and
-
-    python -m lmtpp_amd.traffic matfree [--n 100]
-
-that of the matrix-free product's counters (its element-block kernel and whole product, FP32 then
-FP64).
+which scales each per-dispatch counter (KiB) by the plain copy kernel's known bytes (k_copy of
+COPY_DOUBLES doubles: exactly 512 MiB read and 512 MiB written; factor = known KiB / counted KiB)
+and reports bytes per launch next to the algorithmic bytes.
 """
 import csv
 import glob
@@ -33,56 +20,6 @@ import re
 import sys
 
 COPY_DOUBLES = 1 << 26
-
-
-def run(n):
-    import numpy as np
-
-    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    sys.path.insert(0, here)
-    sys.path.insert(0, os.path.join(here, "tests"))
-    import lmtpp_amd as L
-    from lmtpp_amd import _capi as A
-    from problems import build, lognormal_E
-
-    gbs, ms = A.stream_copy_gbs(COPY_DOUBLES, 5)
-    print("copy: %.1f GB/s (%.3f ms per launch)" % (gbs, ms), flush=True)
-    c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
-    f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
-    f.assemble()
-    f.set_result(np.zeros(3 * len(c)))
-    sz = f.sizes()
-    for fmt, nt in ((A.SPMV_FULL, A.LOADS_NT), (A.SPMV_FULL, A.LOADS_NT_COL16), (A.SPMV_FULL, A.LOADS_PLAIN),
-                    (A.SPMV_SYM, A.LOADS_PLAIN)):
-        f.set_spmv_config(fp64=(fmt, 2, nt), fp32=(fmt, 2, nt))
-        t64 = f.time_spmv(L.FP64, 5)
-        t32 = f.time_spmv(L.MIXED, 5)
-        print(json.dumps({"format": fmt, "loads": nt, "nnzb": sz["nnzb"], "n_nodes": sz["n_nodes"], "spmv_fp64_ms": t64,
-                          "spmv_fp32_ms": t32}), flush=True)
-    f.set_spmv_config(fp64=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT), fp32=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT))
-    b64, p64 = f.time_matrix_free(L.FP64, 5)
-    b32, p32 = f.time_matrix_free(L.MIXED, 5)
-    print(json.dumps({"format": A.SPMV_MATRIX_FREE, "info": f.matrix_free_info(), "block_fp64_ms": b64,
-                      "product_fp64_ms": p64, "block_fp32_ms": b32, "product_fp32_ms": p32}), flush=True)
-
-
-def assembly(n):
-    """Workload of the element-loop counters: Hexa8 n^3 assembled three times with each element-loop
-    variant: row groups of 64 rows, colour passes, staged colour passes, gather."""
-    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    sys.path.insert(0, here)
-    sys.path.insert(0, os.path.join(here, "tests"))
-    import lmtpp_amd as L
-    from problems import build, lognormal_E
-
-    c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
-    f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
-    for R in (64, 0, L._capi.ASSEMBLY_STAGED, L._capi.ASSEMBLY_GATHER):
-        f.set_assembly_rows(R)
-        for _ in range(3):
-            f.assemble()
-        print("rows %#x: assembly %.2f ms" % (R, f.assemble_ms()), flush=True)
-    f.close()
 
 
 # Arithmetic of one element in the matrix-free product as written (k_mf_block; an FMA counts 2).
@@ -126,27 +63,6 @@ def mf_halo_bytes(info, fp32):
     partials read once, its y written once."""
     s = 12 if fp32 else 24
     return info["shared_nodes"] * (8 + s) + info["partial_slots"] * s
-
-
-def matfree(n):
-    """Workload of the matrix-free counters: Hexa8 n^3, the element-block kernel alone and the whole
-    product, 3 launches each per precision (after one untimed product)."""
-    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    sys.path.insert(0, here)
-    sys.path.insert(0, os.path.join(here, "tests"))
-    import lmtpp_amd as L
-    from lmtpp_amd import _capi as A
-    from problems import build, lognormal_E
-
-    c, e = L.make_rect(L.HEXA, (n + 1,) * 3)
-    f, *_ = build(L.HEXA, c, e, lognormal_E(len(e), 0.5, 0))
-    f.assemble()
-    f.set_spmv_config(fp64=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT), fp32=(A.SPMV_MATRIX_FREE, 2, A.LOADS_NT))
-    for prec, name in ((L.MIXED, "fp32"), (L.FP64, "fp64")):
-        b, p = f.time_matrix_free(prec, 3)
-        print("%s: element blocks %.1f us, product %.1f us" % (name, b * 1e3, p * 1e3), flush=True)
-    print(json.dumps(f.matrix_free_info()), flush=True)
-    f.close()
 
 
 def spmv_bytes(nnzb, n_nodes, fp32, fmt, col16=0):
@@ -216,10 +132,7 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) >= 2 and sys.argv[1] in ("run", "assembly", "matfree"):
-        n = int(sys.argv[sys.argv.index("--n") + 1]) if "--n" in sys.argv else 100
-        {"run": run, "assembly": assembly, "matfree": matfree}[sys.argv[1]](n)
-    elif len(sys.argv) >= 4 and sys.argv[1] == "parse":
+    if len(sys.argv) >= 4 and sys.argv[1] == "parse":
         nnzb = int(sys.argv[4]) if len(sys.argv) > 4 else 27270901
         nn = int(sys.argv[5]) if len(sys.argv) > 5 else 1030301
         print(json.dumps(parse(sys.argv[2], sys.argv[3], nnzb, nn), indent=1))

@@ -1,7 +1,8 @@
 // Runs the reference's own block IC(0) + PCG (containers/MatWithTinyBlocks.h:
 // conversion :224-269, chol_incomp :426-523, solve_using_incomplete_chol_factorize
 // :1101-1151) on a matrix and right-hand side read from a file, for golden
-// fixtures.  Builds against the LMT++ headers where they lie.
+// fixtures and for timing the reference's solve on this host (it prints the iteration count and
+// the seconds of chol_incomp and of the PCG).  Builds against the LMT++ headers where they lie.
 // in.bin : uint32 n_nodes, uint32 nnzb, uint32 row_ptr[n_nodes+1], uint32 col[nnzb],
 //          double vals[nnzb*9] (full block CSR, block row-major), double b[3n],
 //          double x0[3n], double crit
@@ -13,10 +14,17 @@
 #include "containers/matsparse.h"
 #include "containers/MatWithTinyBlocks.h"
 #include <cstdio>
+#include <time.h>
 #include <vector>
 #include <sstream>
 #include <iostream>
 using namespace LMT;
+
+static double now_s() {
+    timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
 
 int main(int argc, char **argv) {
     if (argc != 3) return 1;
@@ -43,12 +51,15 @@ int main(int argc, char **argv) {
                 }
     MatWithTinyBlocks<double, Sym<3> > M(K);
     MatWithTinyBlocks<double, Sym<3> > F = M;
+    const double t0 = now_s();
     F.chol_incomp();
+    const double t1 = now_s();
     Vec<double> bb; bb.resize(n); for (unsigned i = 0; i < n; ++i) bb[i] = b[i];
     Vec<double> x; x.resize(n); for (unsigned i = 0; i < n; ++i) x[i] = x0[i];
     // count the residual prints (one before the loop, one per iteration)
     std::stringstream ss; std::streambuf *old = std::cout.rdbuf(ss.rdbuf());
     solve_using_incomplete_chol_factorize(F, M, bb, x, crit, true);
+    const double t2 = now_s();
     std::cout.rdbuf(old);
     unsigned lines = 0; std::string line; while (std::getline(ss, line)) ++lines;
     unsigned iters = lines ? lines - 1 : 0;
@@ -66,6 +77,6 @@ int main(int argc, char **argv) {
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) { double v = j <= i ? F.diags[r](i, j) : 0.0; fwrite(&v, 8, 1, o); }
     }
     fclose(o);
-    printf("iters %u\n", iters);
+    printf("iters %u factor_s %.6f solve_s %.6f\n", iters, t1 - t0, t2 - t1);
     return 0;
 }

@@ -365,58 +365,48 @@ def config4(meshes, items, oracle_refs=()):
         b.set_traction(i, faces, trs)
         b.add_constraints(i, dofs, None, 1e5)
         ndof += 3 * len(c)
-    b.set_assembly_rows(L._capi.ASSEMBLY_GATHER)
     b.assemble()
     # FP64 references (tight), then the timed MIXED batch solve
     b.clear_result()
     ok, _ = b.solve(rel_tol=1e-11, precision=L.FP64, max_iter=40000)
     refs = [b.get_result(i) for i in range(n_models)]
+    # product variant and element loop chosen inside the library by timing whole batched solves
+    prod = b.autotune(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI, max_iter=40000)
     t_asm = []
     for _ in range(2):
         b.assemble()
         t_asm.append(b.assemble_ms())
-    # product variants: the stored matrix with non-temporal loads (streamed once per product; with
-    # 16-bit column offsets) or plain loads (the few problems still iterating late fit the Infinity
-    # Cache), and the matrix-free product (-1); all timed, the fastest kept
-    by_nt = {}
-    variants = {2: ((0, 2, 2), (0, 2, 2)), 1: ((0, 2, 1), (0, 2, 1)), 0: ((0, 2, 0), (0, 2, 0)),
-                -1: ((2, 2, 1), (2, 2, 1))}
-    for nt, (c64, c32) in variants.items():
-        b.set_spmv_config(fp64=c64, fp32=c32)
-        times = []
-        for _ in range(2):
-            b.assemble()
-            b.clear_result()
-            t0 = time.perf_counter()
-            ok, ms = b.solve(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI, max_iter=40000)
-            times.append((time.perf_counter() - t0, ms))
-        by_nt[nt] = (float(np.median([t[1] for t in times])), ok)
-    best_nt = min(by_nt, key=lambda k: by_nt[k][0])
-    b.set_spmv_config(fp64=variants[best_nt][0], fp32=variants[best_nt][1])
-    times = []
+    times, polls = [], []
     for _ in range(2):
         b.assemble()
         b.clear_result()
         t0 = time.perf_counter()
         ok, ms = b.solve(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI, max_iter=40000)
         times.append((time.perf_counter() - t0, ms))
+        polls.append(b.chunk_stats())
     errs = [np.linalg.norm(b.get_result(i) - refs[i]) / np.linalg.norm(refs[i]) for i in range(n_models)]
     oerr = {int(i): float(np.linalg.norm(b.get_result(i) - x) / np.linalg.norm(x)) for i, x, rc in oracle_refs if rc == 0}
     ms = float(np.median([t[1] for t in times]))
     iters = [st.iters for st in b.last_stats]
     return {"workload": "256 Hexa8 cubes, side ~ U{8..32} cells (seed 4), lognormal E (sigma 0.5), nu ~ U[0.2,0.4], "
                         "traction ~ N(0,1) on x=1, clamp x=0; one batch: per-problem MIXED block-Jacobi PCG to rel. res. 1e-10, "
-                        "iteration chunks replayed from a captured hipGraph; gather element loop",
+                        "iteration chunks replayed from a captured hipGraph; product and element loop from femgpu_batch_autotune",
             "models": n_models, "total_dof": ndof, "batch_setup_s": round(setup, 2),
             "assembly_ms": round(float(np.median(t_asm)), 3), "solve_ms": round(ms, 2),
             "aggregate_mdof_s": round(ndof / (ms * 1e-3) / 1e6, 3), "max_rel_err_vs_fp64": float(max(errs)),
             "rel_err_vs_oracle": oerr, "max_rel_err_vs_oracle": max(oerr.values()) if oerr else None,
             "median_rel_err": float(np.median(errs)), "iters_min": int(min(iters)), "iters_max": int(max(iters)),
             "all_ok": bool(ok), "product": {2: "stored, nontemporal_col16", 1: "stored, nontemporal", 0: "stored, plain",
-                                            -1: "matrix-free"}[best_nt],
-            "solve_ms_by_product": {"nontemporal_col16": round(by_nt[2][0], 2), "nontemporal": round(by_nt[1][0], 2),
-                                    "plain": round(by_nt[0][0], 2), "matrix_free": round(by_nt[-1][0], 2)},
-            "assembly": "gather"}
+                                            -1: "matrix-free"}[prod],
+            "product_choice": "femgpu_batch_autotune",
+            "host_poll": {"chunks": polls[-1][0], "device_ms_in_chunks": round(polls[-1][1], 2),
+                          "gaps": polls[-1][2], "gap_ms": round(polls[-1][3], 3),
+                          "gap_us_per_chunk": round(1e3 * polls[-1][3] / max(1, polls[-1][2]), 1),
+                          # that gap at every chunk transition (refreshes add device work there)
+                          "est_fraction_of_solve": round(polls[-1][3] / max(1, polls[-1][2]) *
+                                                         max(0, polls[-1][0] - 1) / ms, 4)},
+            "assembly": {0xffffffff: "staged", 0xfffffffe: "gather", 0xfffffffd: "gather_box"}.get(
+                b.assembly_rows()[0], "rows %d" % b.assembly_rows()[0])}
 
 
 def main():

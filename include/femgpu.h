@@ -394,6 +394,7 @@ FEMGPU_API int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms);
 /* The element-loop variant of a batch (femgpu_set_assembly_rows' values); every variant gives
  * each sub-problem bitwise the K and F it gets alone. */
 FEMGPU_API int femgpu_batch_set_assembly_rows(femgpu_batch *b, uint32_t rows);
+FEMGPU_API int femgpu_batch_get_assembly_rows(femgpu_batch *b, uint32_t *rows, uint32_t *max_rows);
 FEMGPU_API int femgpu_batch_destroy(femgpu_batch *b);
 
 /* ---- diagnostics -------------------------------------------------------------------------- */
@@ -476,6 +477,30 @@ FEMGPU_API int femgpu_set_spmv_config(femgpu_problem *p, const int format[2], co
  * plain loads keep them resident across iterations. */
 FEMGPU_API int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2],
                                             const int loads[2]);
+/* Batch product variants femgpu_batch_autotune chooses among. */
+typedef enum {
+    FEMGPU_BATCH_STORED_PLAIN = 0,
+    FEMGPU_BATCH_STORED_NT = 1,
+    FEMGPU_BATCH_STORED_NT_COL16 = 2,
+    FEMGPU_BATCH_MATRIX_FREE = -1
+} femgpu_batch_product;
+/* Times whole batched solves with the options o (NULL: defaults) from a zero initial guess, two
+ * per product variant (the stored matrix with non-temporal loads and 16-bit column offsets when
+ * every offset fits, with non-temporal loads, with plain loads -- the few sub-problems still
+ * iterating late may stay in the 256 MB Infinity Cache --, and the matrix-free product for Hexa
+ * and Tetra_10 batches), keeps the variant of the fastest second solve in both precisions
+ * (*product, may be NULL), then times the element-loop variants as femgpu_autotune does and
+ * re-assembles K and F with the fastest.  The results of the batch are left as they were.
+ * Needs an assembled batch. */
+FEMGPU_API int femgpu_batch_autotune(femgpu_batch *b, const femgpu_solve_opts *o, int *product);
+/* Iteration chunks of the last solve: how many, the device time inside them, and the time the
+ * device waited between consecutive chunks with no refresh between them (the host's read of the
+ * per-problem scalars and the next launch) over n_gaps such transitions.  Any pointer may be
+ * NULL. */
+FEMGPU_API int femgpu_get_chunk_stats(femgpu_problem *p, int *n_chunks, double *chunk_ms, int *n_gaps,
+                                      double *gap_ms);
+FEMGPU_API int femgpu_batch_get_chunk_stats(femgpu_batch *b, int *n_chunks, double *chunk_ms, int *n_gaps,
+                                            double *gap_ms);
 /* Element-loop variant.  rows = 0: one pass per element colour, each element adding its blocks
  * into K (read-modify-write).  rows = FEMGPU_ASSEMBLY_STAGED: the same passes into a
  * block-contiguous copy of K (each block's 9 values adjacent; an extra K-sized buffer), then one

@@ -20,6 +20,7 @@ SPMV_FULL, SPMV_SYM, SPMV_MATRIX_FREE = 0, 1, 2
 LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2
 ROW_ORDER_AUTO, ROW_ORDER_NATURAL, ROW_ORDER_BY_LENGTH = 0, 1, 2
 NODE_ORDER_NATURAL, NODE_ORDER_RCM, NODE_ORDER_AMD = 0, 1, 2
+BATCH_STORED_PLAIN, BATCH_STORED_NT, BATCH_STORED_NT_COL16, BATCH_MATRIX_FREE = 0, 1, 2, -1
 ASSEMBLY_STAGED = 0xFFFFFFFF
 ASSEMBLY_GATHER = 0xFFFFFFFE
 ASSEMBLY_GATHER_BOX = 0xFFFFFFFD
@@ -129,7 +130,13 @@ _SIGS = {
     "femgpu_set_assembly_rows": (C.c_int, [_P, C.c_uint32]),
     "femgpu_batch_set_assembly_rows": (C.c_int, [_P, C.c_uint32]),
     "femgpu_get_assembly_rows": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "femgpu_batch_get_assembly_rows": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "femgpu_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),
+    "femgpu_batch_autotune": (C.c_int, [_P, _P, C.POINTER(C.c_int)]),
+    "femgpu_get_chunk_stats": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double)]),
+    "femgpu_batch_get_chunk_stats": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_double)]),
     "femgpu_batch_set_spmv_config": (C.c_int, [_P, _P, _P, _P]),
     "femgpu_get_precond_setup_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "femgpu_get_ic0_factor": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),

@@ -75,11 +75,15 @@ class Batch:
         A.check(self._lib.femgpu_batch_get_assemble_ms(self._h, C.byref(v)), "batch_assemble_ms")
         return v.value
 
-    def solve(self, rel_tol=1e-10, precision=A.MIXED, precond=A.JACOBI, max_iter=100000, chunk=32):
+    def _opts(self, rel_tol, precision, precond, max_iter, chunk):
         o = A.SolveOpts()
         self._lib.femgpu_default_solve_opts(C.byref(o))
         o.precision, o.precond, o.crit_kind, o.tol, o.max_iter, o.chunk = (
             int(precision), int(precond), A.REL_L2, float(rel_tol), int(max_iter), int(chunk))
+        return o
+
+    def solve(self, rel_tol=1e-10, precision=A.MIXED, precond=A.JACOBI, max_iter=100000, chunk=32):
+        o = self._opts(rel_tol, precision, precond, max_iter, chunk)
         st = (A.SolveStats * self.n)()
         ms = C.c_double()
         rc = self._lib.femgpu_batch_solve(self._h, C.byref(o), st, C.byref(ms))
@@ -95,9 +99,33 @@ class Batch:
         nt = (C.c_int * 2)(int(fp64[2]), int(fp32[2]))
         A.check(self._lib.femgpu_batch_set_spmv_config(self._h, f, u, nt), "batch_set_spmv_config")
 
+    def autotune(self, rel_tol=1e-10, precision=A.MIXED, precond=A.JACOBI, max_iter=100000, chunk=32):
+        """Product variant and element loop chosen by timing whole batched solves with these
+        options (femgpu_batch_autotune); returns the chosen product (A.BATCH_*).  The batch's
+        results are left as they were."""
+        o = self._opts(rel_tol, precision, precond, max_iter, chunk)
+        prod = C.c_int()
+        A.check(self._lib.femgpu_batch_autotune(self._h, C.byref(o), C.byref(prod)), "batch_autotune")
+        return prod.value
+
+    def chunk_stats(self):
+        """Iteration chunks of the last solve: (chunks, device ms inside them, gaps between
+        chunks with no refresh, device ms waited in those gaps)."""
+        n, g = C.c_int(), C.c_int()
+        cm, gm = C.c_double(), C.c_double()
+        A.check(self._lib.femgpu_batch_get_chunk_stats(self._h, C.byref(n), C.byref(cm), C.byref(g), C.byref(gm)),
+                "batch_get_chunk_stats")
+        return n.value, cm.value, g.value, gm.value
+
     def set_assembly_rows(self, rows):
         """Element-loop variant, as Formulation.set_assembly_rows."""
         A.check(self._lib.femgpu_batch_set_assembly_rows(self._h, int(rows)), "batch_set_assembly_rows")
+
+    def assembly_rows(self):
+        """(current element-loop variant, largest admissible R), as Formulation.assembly_rows."""
+        r, m = C.c_uint32(), C.c_uint32()
+        A.check(self._lib.femgpu_batch_get_assembly_rows(self._h, C.byref(r), C.byref(m)), "batch_get_assembly_rows")
+        return r.value, m.value
 
     def clear_result(self):
         """Zero initial guess for every problem (solves otherwise start from the previous result)."""

@@ -1462,15 +1462,13 @@ static int stream_bench(size_t n, int reps, double *avg_ms, bool read_only) {
 int femgpu_stream_copy(size_t n, int reps, double *avg_ms) { return stream_bench(n, reps, avg_ms, false); }
 int femgpu_stream_read(size_t n, int reps, double *avg_ms) { return stream_bench(n, reps, avg_ms, true); }
 
-int femgpu_autotune(femgpu_problem *h, int *unroll) {
-    if (!h) return fail(FEMGPU_BAD_INPUT, "autotune: null");
+}  // extern "C"
+
+namespace {
+// Element loop: colour passes (direct or staged), the gathers, or row groups of Rmax, Rmax/2,
+// Rmax/4 rows (min of 3 timings); the fastest is kept and K and F re-assembled with it.
+int tune_assembly(femgpu_problem *h) {
     Problem &P = h->P;
-    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "autotune: assemble first");
-    if (autotune_spmv(P)) return fail(FEMGPU_HIP_ERROR, "autotune failed");
-    if (unroll) *unroll = P.spmv.unroll[0];
-    if (P.from_bsr) return FEMGPU_OK;
-    // element loop: colour passes (direct or staged) or row groups of Rmax, Rmax/2, Rmax/4 rows
-    // (min of 3 timings)
     const uint32_t Rmax = max_assembly_rows(P);
     uint32_t best_R = P.asm_rows;
     double best = 1e30;
@@ -1495,6 +1493,86 @@ int femgpu_autotune(femgpu_problem *h, int *unroll) {
     int rc = set_assembly_rows(P, best_R);
     if (!rc) rc = assemble(*h, 1, P.assembled_vec ? 1 : 0);   // K and F as the chosen variant makes them
     return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int femgpu_autotune(femgpu_problem *h, int *unroll) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "autotune: null");
+    Problem &P = h->P;
+    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "autotune: assemble first");
+    if (autotune_spmv(P)) return fail(FEMGPU_HIP_ERROR, "autotune failed");
+    if (unroll) *unroll = P.spmv.unroll[0];
+    if (P.from_bsr) return FEMGPU_OK;
+    return tune_assembly(h);
+}
+
+int femgpu_batch_autotune(femgpu_batch *b, const femgpu_solve_opts *o, int *product) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_autotune: null");
+    int rc = check_opts(o);
+    if (rc) return rc;
+    femgpu_problem &H = b->H;
+    Problem &P = H.P;
+    if (!P.assembled_mat) return fail(FEMGPU_NOT_READY, "batch_autotune: assemble first");
+    // the caller's current results are kept
+    const size_t n = 3 * (size_t)P.n_nodes;
+    double *d_keep = nullptr;
+    HIPCHK(dalloc(&d_keep, n));
+    hipError_t e = hipMemcpyAsync(d_keep, P.d_u, n * 8, hipMemcpyDeviceToDevice, P.stream);
+    // candidates: stored full matrix with non-temporal loads and 16-bit column offsets (when
+    // every offset fits), with non-temporal loads, with plain loads; the matrix-free product
+    struct Variant { int id, format, loads; };
+    std::vector<Variant> vs;
+    if (P.d_col16) vs.push_back({FEMGPU_BATCH_STORED_NT_COL16, FMT_FULL, LOAD_NT16});
+    vs.push_back({FEMGPU_BATCH_STORED_NT, FMT_FULL, LOAD_NT});
+    vs.push_back({FEMGPU_BATCH_STORED_PLAIN, FMT_FULL, LOAD_PLAIN});
+    if (mf_eligible(P)) vs.push_back({FEMGPU_BATCH_MATRIX_FREE, FMT_MF, LOAD_NT});
+    double best = 1e30;
+    int best_i = -1;
+    std::vector<femgpu_solve_stats> st(P.n_sub);
+    for (size_t i = 0; i < vs.size() && e == hipSuccess && !rc; ++i) {
+        const int fmt[2] = {vs[i].format, vs[i].format}, unroll[2] = {2, 2}, loads[2] = {vs[i].loads, vs[i].loads};
+        if ((rc = femgpu_set_spmv_config(&H, fmt, unroll, loads))) break;
+        double t = 1e30;
+        for (int rep = 0; rep < 2 && !rc; ++rep) {  // the first solve also captures the chunk graph
+            if ((e = launch_fill(P.d_u, n, 0.0, P.stream)) != hipSuccess) break;
+            rc = solve_system(H, o, st.data(), nullptr, nullptr);
+            if (rc == FEMGPU_MAX_ITER || rc == FEMGPU_STAGNATED) rc = FEMGPU_OK;  // timed all the same
+            if (!rc && rep) t = st[0].solve_ms;
+        }
+        if (!rc && t < best) {
+            best = t;
+            best_i = (int)i;
+        }
+    }
+    if (!rc && e == hipSuccess && best_i >= 0) {
+        const int fmt[2] = {vs[best_i].format, vs[best_i].format}, unroll[2] = {2, 2},
+                  loads[2] = {vs[best_i].loads, vs[best_i].loads};
+        rc = femgpu_set_spmv_config(&H, fmt, unroll, loads);
+        if (!rc && !P.from_bsr) rc = tune_assembly(&H);
+        if (product) *product = vs[best_i].id;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(P.d_u, d_keep, n * 8, hipMemcpyDeviceToDevice, P.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(P.stream);
+    (void)hipFree(d_keep);
+    if (e != hipSuccess) return fail(FEMGPU_HIP_ERROR, hipGetErrorString(e));
+    return rc;
+}
+
+int femgpu_get_chunk_stats(femgpu_problem *h, int *n_chunks, double *chunk_ms, int *n_gaps, double *gap_ms) {
+    if (!h) return fail(FEMGPU_BAD_INPUT, "get_chunk_stats: null");
+    const Problem &P = h->P;
+    if (n_chunks) *n_chunks = P.last_chunks;
+    if (chunk_ms) *chunk_ms = P.last_chunk_ms;
+    if (n_gaps) *n_gaps = P.last_gaps;
+    if (gap_ms) *gap_ms = P.last_gap_ms;
+    return FEMGPU_OK;
+}
+
+int femgpu_batch_get_chunk_stats(femgpu_batch *b, int *n_chunks, double *chunk_ms, int *n_gaps, double *gap_ms) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_get_chunk_stats: null");
+    return femgpu_get_chunk_stats(&b->H, n_chunks, chunk_ms, n_gaps, gap_ms);
 }
 
 int femgpu_batch_set_spmv_config(femgpu_batch *b, const int format[2], const int unroll[2], const int loads[2]) {
@@ -1972,6 +2050,11 @@ int femgpu_batch_get_assemble_ms(femgpu_batch *b, double *ms) {
 int femgpu_batch_set_assembly_rows(femgpu_batch *b, uint32_t rows) {
     if (!b) return fail(FEMGPU_BAD_INPUT, "batch_set_assembly_rows: null");
     return femgpu_set_assembly_rows(&b->H, rows);
+}
+
+int femgpu_batch_get_assembly_rows(femgpu_batch *b, uint32_t *rows, uint32_t *max_rows) {
+    if (!b) return fail(FEMGPU_BAD_INPUT, "batch_get_assembly_rows: null");
+    return femgpu_get_assembly_rows(&b->H, rows, max_rows);
 }
 
 int femgpu_batch_destroy(femgpu_batch *b) {

@@ -182,7 +182,7 @@ struct Problem {
     std::vector<uint32_t> node_off, elem_off;                // n_sub + 1 offsets
     uint64_t nnzb = 0, ps = 0;  // stored blocks, plane stride (SELL positions, padded to 64)
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[7] = {};  // 0-3 solve and kernel timing, 4-6 the ring timing iteration chunks
 
     Pattern pat;
     Coloring ecol, fcol;
@@ -242,6 +242,11 @@ struct Problem {
     double last_precond_ms = 0;
     double last_after_solve_ms = 0;    // device time of the last preconditioner set-up (0: reused)
     bool from_bsr = false;
+    // iteration chunks of the last solve: count, device time inside them, and the time the device
+    // waited between consecutive chunks with no refresh between them (the host's poll of the
+    // scalars and the next launch), over n_gaps such transitions
+    int last_chunks = 0, last_gaps = 0;
+    double last_chunk_ms = 0, last_gap_ms = 0;
     bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;

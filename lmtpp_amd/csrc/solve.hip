@@ -1019,13 +1019,31 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     }
     int chunk = 2;
     const long max_chunks = (long)o.max_iter / 2 + 1000;
+    // chunk timing: events 4-6 as a ring (the previous chunk's end stays for the gap)
+    P.last_chunks = P.last_gaps = 0;
+    P.last_chunk_ms = P.last_gap_ms = 0;
+    int prev_end = -1;
+    bool prev_refresh = false;
     for (long c = 0;; ++c) {
         if (c > max_chunks) {  // cannot happen while every sub-problem honours max_iter
             for (uint32_t s = 0; s < P.n_sub; ++s) if (!P.h_ps[s].done) P.h_ps[s].status = 4;
             break;
         }
+        int ev_s = 4, ev_e = 5;  // the two events other than the previous chunk's end
+        if (prev_end == 4) ev_s = 6;
+        else if (prev_end == 5) ev_e = 6;
+        (void)hipEventRecord(P.ev[ev_s], P.stream);
         if (run_chunk<PRE>(P, mixed, d_x, P.use_graph ? std::max(2, o.chunk) : chunk)) return 5;
+        (void)hipEventRecord(P.ev[ev_e], P.stream);
         if (read_scalars(P)) return 5;
+        float t = 0;
+        if (hipEventElapsedTime(&t, P.ev[ev_s], P.ev[ev_e]) == hipSuccess) P.last_chunk_ms += t;
+        if (prev_end >= 0 && !prev_refresh && hipEventElapsedTime(&t, P.ev[prev_end], P.ev[ev_s]) == hipSuccess) {
+            P.last_gap_ms += t;
+            ++P.last_gaps;
+        }
+        ++P.last_chunks;
+        prev_end = ev_e;
         chunk = std::min(chunk * 2, std::max(2, o.chunk));
         bool all_done = true, refresh = false;
         for (uint32_t s = 0; s < P.n_sub; ++s) {
@@ -1033,6 +1051,7 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             refresh |= !P.h_ps[s].done && P.h_ps[s].pause == PAUSE_REQ;
         }
         if (all_done) break;
+        prev_refresh = refresh;
         if (refresh) {
             // Check of the paused sub-problems on the true residual: MIXED first adds the FP32
             // correction (x += x32); r = b - K x in FP64 with the stop test on it; those that

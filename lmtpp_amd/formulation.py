@@ -138,6 +138,15 @@ class Formulation:
             A.check(rc, "solve_system")
         return rc == A.OK
 
+    def chunk_stats(self):
+        """Iteration chunks of the last solve: (chunks, device ms inside them, gaps between
+        chunks with no refresh, device ms waited in those gaps) (femgpu_get_chunk_stats)."""
+        n, g = C.c_int(), C.c_int()
+        cm, gm = C.c_double(), C.c_double()
+        A.check(self._lib.femgpu_get_chunk_stats(self._h, C.byref(n), C.byref(cm), C.byref(g), C.byref(gm)),
+                "get_chunk_stats")
+        return n.value, cm.value, g.value, gm.value
+
     def solve(self, crit=0.0, rel_tol=None):
         """solve(crit) (formulation.h:1517-1568): shift, then assemble + solve_system, repeated
         while non_linear_iterative_criterium is set and the iterate still moves; True on success.
@@ -393,7 +402,7 @@ class Formulation:
         return r.value, m.value
 
     def node_order(self):
-        """(NODE_ORDER_NATURAL or NODE_ORDER_RCM, perm): perm[i] = the caller's node stored at
+        """(NODE_ORDER_NATURAL, _RCM or _AMD, perm): perm[i] = the caller's node stored at
         internal position i."""
         o = C.c_int()
         perm = np.zeros(self.n_nodes, dtype=np.uint32)

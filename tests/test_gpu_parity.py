@@ -14,6 +14,7 @@ import pytest
 
 import lmtpp_amd as L
 from conftest import GOLDEN
+from lmtpp_amd import _capi as A
 from problems import E0, NU0, build, fd_gradient, jitter_interior, lognormal_E, tetra10_jittered
 
 pytestmark = pytest.mark.gpu
@@ -316,6 +317,45 @@ def test_batch_matches_individual_solves(precision):
         u = b.get_result(i)
         assert b.last_stats[i].iters == singles[i][1]
         assert np.array_equal(u, singles[i][0])
+
+
+def test_batch_autotune_keeps_results_and_picks_a_variant():
+    """femgpu_batch_autotune: the batch's results are left as they were, the chosen product is
+    one of the timed variants and is the configuration in force, a solve after it matches the
+    single problems' FP64 solutions, and the chunk timing of that solve is consistent."""
+    from problems import clamp_and_traction
+    say("batch autotune")
+    items = _batch_inputs(seed=12)
+    b = L.Batch(L.HEXA, [(c, e) for c, e, *_ in items])
+    refs = []
+    for i, (c, e, E, nu, tr) in enumerate(items):
+        faces, trs, dofs = clamp_and_traction(L.HEXA, c, e, tr)
+        b.set_material(i, E, nu)
+        b.set_traction(i, faces, trs)
+        b.add_constraints(i, dofs, None, 1e5)
+        f = L.Formulation(L.HEXA, c, e)
+        f.set_material(E, nu)
+        f.set_traction(faces, trs)
+        f.add_constraints(dofs, None, 1e5)
+        assert f.solve(rel_tol=1e-12)
+        refs.append(f.get_result())
+        f.close()
+    b.assemble()
+    ok, _ = b.solve(rel_tol=1e-6, precision=L.FP64)
+    assert ok
+    before = [b.get_result(i) for i in range(len(items))]
+    prod = b.autotune(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI)
+    assert prod in (A.BATCH_STORED_PLAIN, A.BATCH_STORED_NT, A.BATCH_STORED_NT_COL16, A.BATCH_MATRIX_FREE)
+    for i in range(len(items)):
+        assert np.array_equal(b.get_result(i), before[i])
+    b.clear_result()
+    ok, ms = b.solve(rel_tol=1e-10, precision=L.MIXED, precond=L.BLOCK_JACOBI)
+    assert ok
+    for i in range(len(items)):
+        assert np.linalg.norm(b.get_result(i) - refs[i]) <= 1e-8 * np.linalg.norm(refs[i])
+    n, chunk_ms, gaps, gap_ms = b.chunk_stats()
+    assert n >= 1 and 0 < chunk_ms <= ms * 1.01 and 0 <= gaps <= n - 1 and gap_ms >= 0
+    b.close()
 
 
 def test_graph_replay_is_bitwise_equal():

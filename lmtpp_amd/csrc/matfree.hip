@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <tuple>
 #include <type_traits>
 
 namespace femgpu {
@@ -785,6 +786,46 @@ bool hexa_is_box(const double *X, const uint32_t *nd) {
 
 namespace {
 
+// Renumbers the local nodes of block b (lnode[L0..]) by the parity class of their lattice
+// position -- per axis the rank of the node's coordinate among the block's distinct ones -- then
+// by (z, y, x) rank, and updates the block's corner indices (sloc of slots i0..i0+nk), its
+// per-node element counts and the local index of the nodes the block holds first.
+void renumber_by_parity(const double *X, uint32_t L0, std::vector<uint32_t> &lnode, std::vector<uint32_t> &in_blk,
+                        std::vector<uint16_t> &sloc, size_t i0, size_t nk, uint32_t lst, uint32_t b, MatFree &M) {
+    const uint32_t nl = (uint32_t)lnode.size() - L0;
+    std::vector<uint32_t> rk(3 * (size_t)nl);
+    std::vector<double> v(nl), u;
+    for (int d = 0; d < 3; ++d) {
+        for (uint32_t i = 0; i < nl; ++i) v[i] = X[3 * (size_t)lnode[L0 + i] + d];
+        u = v;
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (uint32_t i = 0; i < nl; ++i)
+            rk[3 * (size_t)i + d] = (uint32_t)(std::lower_bound(u.begin(), u.end(), v[i]) - u.begin());
+    }
+    auto key = [&](uint32_t i) {
+        const uint32_t *r = &rk[3 * (size_t)i];
+        const uint32_t cls = (r[0] & 1u) | (r[1] & 1u) << 1 | (r[2] & 1u) << 2;
+        return std::make_tuple(cls, r[2], r[1], r[0]);
+    };
+    std::vector<uint32_t> ord(nl), nid(nl);
+    for (uint32_t i = 0; i < nl; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t p, uint32_t q) { return key(p) < key(q); });
+    for (uint32_t t = 0; t < nl; ++t) nid[ord[t]] = t;
+    const std::vector<uint32_t> ln(lnode.begin() + L0, lnode.end()), ib(in_blk.begin() + L0, in_blk.end());
+    for (uint32_t t = 0; t < nl; ++t) {
+        lnode[L0 + t] = ln[ord[t]];
+        in_blk[L0 + t] = ib[ord[t]];
+    }
+    for (size_t k = 0; k < nk; ++k)
+        for (uint32_t a = 0; a < 8; ++a) {
+            uint16_t &l = sloc[lst * (i0 + k) + a];
+            l = (uint16_t)nid[l];
+        }
+    for (uint32_t i = 0; i < nl; ++i)
+        if (M.node_blk[ln[i]] == b) M.node_loc[ln[i]] = nid[i];
+}
+
 // 21-bit-per-axis Morton code (bit 3k + d holds bit k of axis d).
 uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
     uint64_t m = 0;
@@ -937,6 +978,17 @@ int mf_build(Problem &P) {
                 scol[i0 + k] = s0[src];
                 selem[i0 + k] = e0[src];
             }
+        }
+        // a block of box elements (a brick of a lattice) numbers its local nodes by lattice parity
+        // class, then (z, y, x): the lanes of one sub-colour reach, at one corner, nodes of one
+        // class two lattice steps apart, which then lie at consecutive triples of the staged
+        // input and the accumulator and so in distinct LDS banks (first-appearance numbering
+        // put them 6 words apart, two to four lanes per bank)
+        if (P.kind == HEXA && !blk.empty()) {
+            const size_t i0 = (size_t)b * MF_BLOCK, nk = blk.size();
+            bool boxes = true;
+            for (size_t k = 0; k < nk && boxes; ++k) boxes = (scol[i0 + k] & MF_BOX) != 0;
+            if (boxes) renumber_by_parity(X.data(), lptr[b], lnode, in_blk, sloc, i0, nk, lst, b, M);
         }
         max_local = std::max<uint32_t>(max_local, (uint32_t)lnode.size() - lptr[b]);
     }

@@ -14,7 +14,8 @@ clamped and a traction on x=1 (the elastic-cube workload of bench.py):
   matfree    the matrix-free product: its element-block kernel and the whole product, 3 launches
              each, FP32 then FP64;
   iteration  the solves bench.py times: the box matrix-free product with block Jacobi, MIXED then
-             FP64, to a relative residual of 1e-8 from a zero initial guess.
+             FP64 (--precision mixed or fp64: that one alone), to a relative residual of 1e-8 from
+             a zero initial guess.
 
 FETCH_SIZE / WRITE_SIZE runs of `traffic` are read back with `python -m lmtpp_amd.traffic parse`.
 """
@@ -86,13 +87,15 @@ def matfree(n):
     f.close()
 
 
-def iteration(n):
+def iteration(n, only=None):
     f, c = _cube(n)
     f.assemble()
     mf = (A.SPMV_MATRIX_FREE, 2, A.LOADS_NT)
     f.set_spmv_config(fp64=mf, fp32=mf)
     f.precond = L.BLOCK_JACOBI
     for prec, name in ((L.MIXED, "mixed"), (L.FP64, "fp64")):
+        if only and name != only:
+            continue
         f.precision = prec
         f.set_result(np.zeros(3 * len(c)))
         f.solve_system(rel_tol=1e-8)
@@ -108,4 +111,8 @@ if __name__ == "__main__":
     if len(sys.argv) < 2 or sys.argv[1] not in WORKLOADS:
         print(__doc__)
         sys.exit(2)
-    WORKLOADS[sys.argv[1]](int(sys.argv[sys.argv.index("--n") + 1]) if "--n" in sys.argv else 100)
+    n = int(sys.argv[sys.argv.index("--n") + 1]) if "--n" in sys.argv else 100
+    if sys.argv[1] == "iteration" and "--precision" in sys.argv:
+        iteration(n, sys.argv[sys.argv.index("--precision") + 1])
+    else:
+        WORKLOADS[sys.argv[1]](n)

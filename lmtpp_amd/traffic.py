@@ -89,7 +89,8 @@ def _counters(d):
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
-        mf = re.search(r"k_mf_(block|halo)<(double|float), false(?:, false)?(?:, (?:true|false))?>", name)
+        mf = re.search(r"k_mf_(block)<(double|float), false, false, (?:true|false)(?:, \d+)?>", name) or \
+            re.search(r"k_mf_(halo)<(double|float)(?:, false)?>", name)
         if "k_copy" in name:
             key = "copy"
         elif mf:

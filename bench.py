@@ -627,7 +627,7 @@ def main():
         r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(gbs / HBM_PEAK_GBS, 4),
              "traffic": load_traffic("mf_block_" + ("fp32" if fp32 else "fp64")),
-             "kernel": "k_mf_block<%s,false,false,%s> (%s matrix-free Hexa product, element blocks)" % (
+             "kernel": "k_mf_block<%s,false,false,%s,2> (%s matrix-free Hexa product, element blocks)" % (
                  "float" if fp32 else "double", "true" if info["box_elements"] == sz["n_elem"] else "false",
                  "FP32" if fp32 else "FP64"),
              "avg_launch_ms": round(tb, 4), "algorithmic_bytes": nbytes, "algorithmic_flops": flops,

@@ -48,12 +48,19 @@ typedef enum { FEMGPU_FP64 = 0, FEMGPU_MIXED = 1 } femgpu_precision;
 /* FEMGPU_BLOCK_IC0: the reference's block incomplete Cholesky (chol_incomp,
  * containers/MatWithTinyBlocks.h:426-523) in natural node order -- the reference's factor,
  * level-scheduled; FEMGPU_BLOCK_IC0_COLOURED: the same factorisation in a multicolour node
- * order (a different, GPU-friendly preconditioner with a few wide levels). */
+ * order (a different, GPU-friendly preconditioner with a few wide levels).
+ * FEMGPU_CHEBYSHEV_BLOCK_JACOBI (no counterpart in the reference): z = p(D K) D r with D the
+ * block-Jacobi inverse and p the degree-1 polynomial whose residual 1 - t p(t) is the scaled
+ * Chebyshev polynomial of degree 2 on [b/20, b], b = 1.15 x a power-iteration estimate of the
+ * largest eigenvalue of D K (20 FP64 steps at set-up); one more product per iteration, about half
+ * the iterations of block Jacobi on elasticity.  A sub-problem on which it is not positive
+ * definite stops with FEMGPU_NOT_SPD. */
 typedef enum {
     FEMGPU_JACOBI = 0,
     FEMGPU_BLOCK_JACOBI = 1,
     FEMGPU_BLOCK_IC0 = 2,
-    FEMGPU_BLOCK_IC0_COLOURED = 3
+    FEMGPU_BLOCK_IC0_COLOURED = 3,
+    FEMGPU_CHEBYSHEV_BLOCK_JACOBI = 4
 } femgpu_precond;
 /* stop tests: relative L2 residual, or the reference's max_i |r_i| <= crit
  * (containers/MatWithTinyBlocks.h:1115,1136) */
@@ -403,8 +410,9 @@ FEMGPU_API const char *femgpu_status_string(int status);
 FEMGPU_API const char *femgpu_last_error(void);
 /* Device time (ms) of the last assembly (element + skin + constraint kernels). */
 FEMGPU_API int femgpu_get_assemble_ms(femgpu_problem *p, double *ms);
-/* Device time (ms) of the preconditioner set-up of the last solve (Jacobi factors, or the block
- * IC(0) factorisation and its FP32 copy); 0 when the previous set-up was reused. */
+/* Device time (ms) of the preconditioner set-up of the last solve (Jacobi factors, the block
+ * IC(0) factorisation and its FP32 copy, or the block-Jacobi factors and the power iteration of
+ * the Chebyshev preconditioner); 0 when the previous set-up was reused. */
 FEMGPU_API int femgpu_get_precond_setup_ms(femgpu_problem *p, double *ms);
 /* Average device time (ms) of one block-SpMV launch (precision FP64 or the FP32 copy), measured
  * with HIP events around reps back-to-back launches on the problem's stream, after one untimed

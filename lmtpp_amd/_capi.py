@@ -14,7 +14,7 @@ OK, BAD_INPUT, NEG_JACOBIAN, NOT_SPD, MAX_ITER, HIP_ERROR, NAN, NOT_READY, STAGN
 TETRA, TETRA10, HEXA = 0, 1, 2
 TRI3, TRI6, QUAD4 = 0, 1, 2
 FP64, MIXED = 0, 1
-JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED = 0, 1, 2, 3
+JACOBI, BLOCK_JACOBI, BLOCK_IC0, BLOCK_IC0_COLOURED, CHEBYSHEV_BLOCK_JACOBI = 0, 1, 2, 3, 4
 REL_L2, ABS_MAXNORM = 0, 1
 SPMV_FULL, SPMV_SYM, SPMV_MATRIX_FREE = 0, 1, 2
 LOADS_PLAIN, LOADS_NT, LOADS_NT_COL16 = 0, 1, 2

@@ -595,7 +595,7 @@ void to_stats(const SolveOut &s, femgpu_solve_stats *st) {
 
 int check_opts(const femgpu_solve_opts *o) {
     if (!o) return FEMGPU_OK;
-    if (o->precision < 0 || o->precision > 1 || o->precond < 0 || o->precond > 3 || o->crit_kind < 0 ||
+    if (o->precision < 0 || o->precision > 1 || o->precond < 0 || o->precond > 4 || o->crit_kind < 0 ||
         o->crit_kind > 1 || !(o->tol >= 0) || o->max_iter < 0 || o->chunk < 1 || !(o->reliable_delta > 0) ||
         !(o->reliable_delta < 1))
         return fail(FEMGPU_BAD_INPUT, "invalid solve options");

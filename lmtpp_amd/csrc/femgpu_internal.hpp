@@ -29,6 +29,8 @@ struct ProbScalars {
     int iter, max_iter, done, pause, status, crit_kind, mixed, refreshes;
     int checks, stalls;           // true-residual checks; consecutive ones without progress
     int iter_best;                // iteration of the check that set res_best
+    double cheb_c0, cheb_c1;      // Chebyshev preconditioner: z = c0 D r + c1 D K D r
+    double pw_norm, pw_rq;        // power iteration: norm of its vector, Rayleigh quotient
 };
 
 // pause states: 0 running, 1 FP64 refresh requested, 2 refreshed (direction update pending)
@@ -236,6 +238,9 @@ struct Problem {
     ProbScalars *d_ps = nullptr, *h_ps = nullptr;
     int *d_flags = nullptr, *h_flags = nullptr;  // [0]: element with det J <= 0; [1]: IC(0) pivot <= 0
     int minv_kind = -1;  // preconditioner currently stored in d_Minv
+    // Chebyshev preconditioner: per sub-problem coefficients (c0, c1) of its polynomial, from the
+    // power iteration run with the block-Jacobi factors it was built with (minv_kind == 3)
+    std::vector<double> cheb_c;
     SpmvConfig spmv;
     std::vector<double> max_diag;  // per sub-problem, of the last matrix assembly
     double last_asm_ms = 0;
@@ -328,6 +333,10 @@ int mf_prepare(Problem &P);
 // halo: where the shared nodes' partials go (3 values of T per slot); null: the problem's buffer.
 hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z = nullptr, void *halo = nullptr);
 hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z = nullptr, void *halo = nullptr);
+// y = K x with the shared nodes' partials left in the halo buffer (summed by the caller's vector
+// kernel); pq: active sub-problems only, with the p.q partials.
+hipError_t mf_launch_partial(Problem &P, const float *x, float *y, bool pq);
+hipError_t mf_launch_partial(Problem &P, const double *x, double *y, bool pq);
 // The product's arrays are built for the current matrix assembly (mf_prepare has nothing left to
 // allocate or upload for a plain product).
 bool mf_ready(const Problem &P);

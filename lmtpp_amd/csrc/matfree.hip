@@ -1197,15 +1197,16 @@ static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y, T *
 
 // halo: the shared-node partial slots (3 values of T each), the problem's own buffer or a caller's
 template <typename T, bool PQ, bool DIR>
-static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo) {
+static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo, bool halo_pass = true) {
     const MatFree &M = P.mf;
     if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, xb, z, y, halo);
     else mf_block_launch<T, PQ, DIR, false>(P, x, xb, z, y, halo);
     if (M.n_iso)
         hipLaunchKernelGGL((k_mf_iso<T, PQ, DIR>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,
                            M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, P.d_ps, M.d_partials, M.n_blocks);
-    // DIR: the iteration's vector update sums the shared nodes' partials itself (k_update_mf)
-    if (M.n_bnodes && !DIR)
+    // DIR (or !halo_pass): the caller's vector kernel sums the shared nodes' partials itself
+    // (k_update_mf, k_cheb)
+    if (M.n_bnodes && !DIR && halo_pass)
         hipLaunchKernelGGL(k_mf_halo<T>, dim3((M.n_bnodes + 255) / 256), dim3(256), 0, P.stream, M.n_bnodes, M.d_bnode,
                            M.d_bptr, halo, y);
 }
@@ -1262,6 +1263,18 @@ int mf_time(Problem &P, int precision, int reps, double *block_ms, double *produ
     if (product_ms) *product_ms = (double)ms[1] / reps;
     return 0;
 }
+
+// y = K x leaving the shared nodes' partials in the halo buffer for the caller's vector kernel
+// (pq: on the active sub-problems, with the p.q partials; else on every sub-problem).
+template <typename T>
+static hipError_t mf_launch_partial_t(Problem &P, const T *x, T *y, bool pq) {
+    T *h = reinterpret_cast<T *>(P.mf.d_halo);
+    if (pq) mf_passes<T, true, false>(P, x, nullptr, nullptr, y, h, false);
+    else mf_passes<T, false, false>(P, x, nullptr, nullptr, y, h, false);
+    return hipPeekAtLastError();
+}
+hipError_t mf_launch_partial(Problem &P, const float *x, float *y, bool pq) { return mf_launch_partial_t<float>(P, x, y, pq); }
+hipError_t mf_launch_partial(Problem &P, const double *x, double *y, bool pq) { return mf_launch_partial_t<double>(P, x, y, pq); }
 
 hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z, void *halo) {
     return mf_launch_t<float>(P, x, y, pq, z, halo);

@@ -4,8 +4,9 @@
 // The iteration is the PCG of containers/MatWithTinyBlocks.h:1101-1151 /
 // containers/conjugate_gradient.h:27-68:  q = K d; alpha = rho / d.q; x += alpha d;
 // r -= alpha q; z = M^-1 r; rho' = r.z; beta = rho'/rho; d = z + beta d, with M = scalar or 3x3
-// block Jacobi (fused into the vector updates) or the block IC(0) factor of ic0.hip (level
-// sweeps between the update and the r.z reduction).
+// block Jacobi (fused into the vector updates), the block IC(0) factor of ic0.hip (level
+// sweeps between the update and the r.z reduction) or a degree-2 Chebyshev polynomial in the
+// block-Jacobi operator (a second product per iteration, k_cheb).
 //
 // Every row-parallel kernel uses the same workgroup tables: logical workgroup L covers rows
 // [wg_row0[L], wg_row1[L]) of sub-problem wg_sub[L].  Reductions are segmented by sub-problem and
@@ -195,8 +196,12 @@ __global__ void k_to_sym(const double *__restrict__ K, uint64_t ps, const uint32
 
 // z = M^-1 r for one node; PRE 0: scalar Jacobi (3 values per node), 1: 3x3 block Jacobi, the
 // inverse of a symmetric diagonal block stored as its 6 upper values (m00 m01 m02 m11 m12 m22),
-// 2: block IC(0), whose z the level sweeps write (not applied here).
-enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2 };
+// 2: block IC(0), whose z the level sweeps write (not applied here), 3: the Chebyshev polynomial
+// z = c0 D r + c1 D K D r in the block-Jacobi D (D r here; k_cheb adds the second term).
+enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2, PRE_CHEB = 3 };
+// the block-Jacobi factors of D serve both
+template <int PRE>
+constexpr int jacobi_kind() { return PRE == PRE_CHEB ? PRE_BLOCK_JACOBI : PRE; }
 // The preconditioner's values of node n (3 for Jacobi, 6 for block Jacobi), loaded ahead of their use.
 template <int PRE, typename T>
 __device__ inline void load_m(const T *__restrict__ Minv, size_t n, T (&m)[6]) {
@@ -418,11 +423,125 @@ __global__ __launch_bounds__(256) void k_dot_rz(const T *__restrict__ r, const T
     wg_partial<1, 0u>(acc, partials, gridDim.x, L);
 }
 
+// Second stage of the Chebyshev preconditioner: with y = D r (stored in z by the vector update or
+// the restart) and v = K y (q, or for a node shared by several matrix-free blocks the sum of its
+// partials halo[hbeg[n] .. hbeg[n + 1]) when hbeg is given), z = c0 y + c1 D v, in place.  first:
+// also p = z.  NV 3 (iteration): partials of r.z, r.r, max|r| for FIN_UPD; NV 1 (restart): of r.z.
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_cheb(const T *__restrict__ r, T *z, const T *__restrict__ q,
+                                              const T *__restrict__ halo, const uint32_t *__restrict__ hbeg,
+                                              const T *__restrict__ Minv, T *__restrict__ p, WgTables wt,
+                                              const ProbScalars *S, double *partials, int sel, int first) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB || !selected(S[s], sel)) return;
+    const T c0 = (T)S[s].cheb_c0, c1 = (T)S[s].cheb_c1;
+    double acc[3] = {0, 0, 0};
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
+        T m[6], v[3] = {0, 0, 0}, dv[3];
+        load_m<PRE_BLOCK_JACOBI, T>(Minv, n, m);
+        const uint32_t k0 = hbeg ? hbeg[n] : 0u, k1 = hbeg ? hbeg[n + 1] : 0u;
+        if (k0 == k1) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = q[3 * (size_t)n + d];
+        } else {
+            for (uint32_t k = k0; k < k1; ++k) {
+                v[0] += halo[3 * (size_t)k];
+                v[1] += halo[3 * (size_t)k + 1];
+                v[2] += halo[3 * (size_t)k + 2];
+            }
+        }
+        apply_mv<PRE_BLOCK_JACOBI, T>(m, v, dv);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = 3 * (size_t)n + d;
+            const T rv = r[i], zv = c0 * z[i] + c1 * dv[d];
+            z[i] = zv;
+            if (first) p[i] = zv;
+            acc[0] += (double)rv * (double)zv;
+            acc[1] += (double)rv * (double)rv;
+            acc[2] = fmax(acc[2], fabs((double)rv));
+        }
+    }
+    if (NV == 3) {
+        wg_partial<3, 4u>(acc, partials, gridDim.x, L);
+    } else {
+        double a1[1] = {acc[0]};
+        wg_partial<1, 0u>(a1, partials, gridDim.x, L);
+    }
+}
+
+// Start vector of the power iteration for the largest eigenvalue of D K: a fixed pseudo-random
+// value in [-1, 1) per DOF (the MurmurHash3 32-bit finalizer of its index within its
+// sub-problem, so a sub-problem of a batch starts as it would alone), with the partials
+// (0, 0, ||v||^2) for FIN_POWER.
+__device__ inline double unit_hash(uint32_t i) {
+    uint32_t h = i * 0x9e3779b1u;
+    h ^= h >> 16; h *= 0x85ebca6bu;
+    h ^= h >> 13; h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return (double)h * (2.0 / 4294967296.0) - 1.0;
+}
+
+__global__ __launch_bounds__(256) void k_power_init(double *__restrict__ v, WgTables wt, double *partials) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB) return;
+    double acc[3] = {0, 0, 0};
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
+        const uint32_t local = n - wt.row0[wt.sub_wg0[s]];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double h = unit_hash(3 * local + d);
+            v[3 * (size_t)n + d] = h;
+            acc[2] += h * h;
+        }
+    }
+    wg_partial<3, 0u>(acc, partials, gridDim.x, L);
+}
+
+// One power step from u = K v: partials of v.u and u.D u (their ratio, the Rayleigh quotient of
+// K^1/2 D K^1/2 at K^1/2 v, bounds the largest eigenvalue of D K from below) and v <- D u / ||v||
+// with the partial of its squared norm.
+__global__ __launch_bounds__(256) void k_power(double *__restrict__ v, const double *__restrict__ u,
+                                               const double *__restrict__ Minv, WgTables wt, const ProbScalars *S,
+                                               double *partials) {
+    const unsigned L = xcd_slab();
+    const uint32_t s = wt.sub[L];
+    if (s == NO_SUB) return;
+    const double inv = 1.0 / S[s].pw_norm;
+    double acc[3] = {0, 0, 0};
+    const uint32_t n = wt.row0[L] + threadIdx.x;
+    if (n < wt.row1[L]) {
+        double m[6], un[3], w[3];
+        load_m<PRE_BLOCK_JACOBI, double>(Minv, n, m);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) un[d] = u[3 * (size_t)n + d];
+        apply_mv<PRE_BLOCK_JACOBI, double>(m, un, w);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = 3 * (size_t)n + d;
+            acc[0] += v[i] * un[d];
+            acc[1] += un[d] * w[d];
+            const double vn = w[d] * inv;
+            v[i] = vn;
+            acc[2] += vn * vn;
+        }
+    }
+    wg_partial<3, 0u>(acc, partials, gridDim.x, L);
+}
+
 // Finalize of a row kernel's partials, one workgroup per sub-problem (grid = n_sub); selects the
 // same sub-problems as the row kernel it follows (their flags are unchanged in between).
 // FIN_UPD_IC: as FIN_UPD without r.z (rho_prev = rho; FIN_RHO then sets rho and beta).
+// FIN_POWER: a power step's Rayleigh quotient and vector norm (every sub-problem).
+// Chebyshev preconditioner (cheb_c1 != 0): a negative r.z (FIN_UPD, FIN_RESTART*) stops the
+// sub-problem as NOT_SPD -- the polynomial is not positive on the spectrum of D K (a bound below
+// its largest eigenvalue).
 enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5,
-       FIN_UPD_IC = 6, FIN_RHO = 7 };
+       FIN_UPD_IC = 6, FIN_RHO = 7, FIN_POWER = 8 };
 constexpr int STALL_CHECKS = 8, STALL_ITERS = 1000;
 
 __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
@@ -454,6 +573,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         P.rr = o[1];
         P.rmax = o[2];
         if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
+        if (what == FIN_UPD && P.cheb_c1 != 0 && o[0] < 0) { P.status = 3; P.done = 1; return; }
         // The recurrence residual only requests a check: the solve ends on the true residual
         // b - K x (FIN_RES_CHECK), in FP64 as in MIXED, where an FP32 recurrence that has fallen
         // by delta since the last FP64 residual also pauses for a refresh.
@@ -486,6 +606,12 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
             }
             P.checks += 1;
         }
+    } else if (what == FIN_POWER) {
+        double o[3];
+        wg_combine<3, 0u>(partials, stride, w0, nw, o);
+        if (threadIdx.x) return;
+        if (o[0] > 0 && finite(o[1])) P.pw_rq = o[1] / o[0];
+        P.pw_norm = o[2] > 0 && finite(o[2]) ? sqrt(o[2]) : 1.0;
     } else {
         double o[1];
         wg_combine<1, 0u>(partials, stride, w0, nw, o);
@@ -499,6 +625,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
             P.refreshes += 1;
         }
         if (!finite(o[0])) { P.status = 6; P.done = 1; }
+        else if (P.cheb_c1 != 0 && o[0] < 0) { P.status = 3; P.done = 1; }
     }
 }
 
@@ -860,25 +987,46 @@ static bool fused_direction(const Problem &P, int pre) {
     return pre != PRE_IC0 && P.spmv.format[sizeof(T) == 4 ? 1 : 0] == FMT_MF;
 }
 
+// Chebyshev preconditioner, after y = D r was stored in z: v = K y into q (matrix-free: the
+// shared nodes' partials stay in the halo buffer for k_cheb), then z = c0 y + c1 D v with the
+// partials of r.z, r.r, max|r| (iteration, NV 3: the active sub-problems) or of r.z (restart, NV
+// 1: sel, with p = z when first).
+template <typename T, int NV>
+static void launch_cheb(Problem &P, const T *K, const T *r, T *q, const T *Minv, T *z, T *p, int sel, int first) {
+    const WgTables wt = tables(P);
+    const dim3 g(P.n_wg), blk(256);
+    const bool mf = P.spmv.format[sizeof(T) == 4 ? 1 : 0] == FMT_MF;
+    // the product runs on the iteration's active sub-problems (MODE_PQ; its p.q partials are not
+    // used) or on every sub-problem
+    if (mf) (void)mf_launch_partial(P, z, q, NV == 3);
+    else if (NV == 3) launch_spmv_t<T, MODE_PQ>(P, K, z, nullptr, q, SEL_ACTIVE);
+    else launch_spmv_t<T, MODE_PLAIN>(P, K, z, nullptr, q, SEL_ALL);
+    hipLaunchKernelGGL((k_cheb<T, NV>), g, blk, 0, P.stream, r, z, q, reinterpret_cast<const T *>(P.mf.d_halo),
+                       mf ? (const uint32_t *)P.mf.d_hbeg : nullptr, Minv, p, wt, P.d_ps, P.d_partials, sel, first);
+}
+
 template <int PRE, typename T>
 static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, const T *Minv, T *z) {
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
     const bool fused = fused_direction<T>(P, PRE);
+    constexpr int JP = jacobi_kind<PRE>();
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr);
     if constexpr (PRE != PRE_IC0) {
         if (fused) {
             // the product's shared-node pass and alpha are folded into the update
             const T *p_alt = reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
-            hipLaunchKernelGGL((k_update_mf<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
+            hipLaunchKernelGGL((k_update_mf<JP, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
                                reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
                                P.mf.d_nwg, Minv, z, wt, P.d_ps, P.d_partials);
         }
     }
     if (!fused) {
         finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
-        hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, nullptr, q, Minv, z, wt, P.d_ps, P.d_partials);
+        hipLaunchKernelGGL((k_update<JP, T>), g, blk, 0, P.stream, x, r, p, nullptr, q, Minv, z, wt, P.d_ps, P.d_partials);
     }
+    // Chebyshev: the update stored y = D r in z (its partials are replaced by k_cheb's)
+    if (PRE == PRE_CHEB) launch_cheb<T, 3>(P, K, r, q, Minv, z, nullptr, (int)SEL_ACTIVE, 0);
     if (PRE == PRE_IC0) {
         finalize(P, FIN_UPD_IC);
         ic0_apply(P, r, z, SEL_ACTIVE);
@@ -915,7 +1063,15 @@ static void launch_restart(Problem &P, const double *r64, T *r, T *p, const T *M
         hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, first ? (int)SEL_ALL : (int)SEL_PAUSE_DONE,
                            first);
     } else {
-        hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps, P.d_partials, first);
+        hipLaunchKernelGGL((k_restart<jacobi_kind<PRE>(), T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps,
+                           P.d_partials, first);
+        // Chebyshev: k_restart stored y = D r in z; the product of y goes to the dead q buffer (in
+        // FP64 r64 itself, read by k_restart already)
+        if (PRE == PRE_CHEB) {
+            T *q = reinterpret_cast<T *>(sizeof(T) == 4 ? (void *)P.d_q32 : (void *)P.d_q);
+            const T *K = reinterpret_cast<const T *>(sizeof(T) == 4 ? (const void *)P.d_K32 : (const void *)P.d_K);
+            launch_cheb<T, 1>(P, K, r, q, Minv, z, p, sel, first);
+        }
         finalize(P, first ? FIN_RESTART_FIRST : FIN_RESTART);
         if (!first && !fused_direction<T>(P, PRE))
             hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_PAUSE_DONE, 0);
@@ -948,6 +1104,49 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
         P.graph_x = d_x;
     }
     return hipGraphLaunch(P.graph_exec, P.stream) == hipSuccess ? 0 : 5;
+}
+
+// Chebyshev preconditioner: z = p(D K) D r with p(t) = c0 + c1 t, the degree-2 residual
+// polynomial 1 - t p(t) = T2((b + a - 2t) / (b - a)) / T2((b + a) / (b - a)) (T2(x) = 2x^2 - 1)
+// smallest in magnitude on [a, b], a = CHEB_LOWER b.  b = CHEB_SAFETY x the Rayleigh quotient
+// after CHEB_POWER_STEPS power steps from a fixed start (a lower bound of the largest eigenvalue
+// of D K).  The preconditioner is positive definite while every eigenvalue lies below
+// a + b = 1.05 b, so b may undershoot the largest eigenvalue by about 5% (the quotient by about
+// 20%) before r.z can turn negative.
+constexpr int CHEB_POWER_STEPS = 20;
+constexpr double CHEB_SAFETY = 1.15, CHEB_LOWER = 0.05;
+
+// Power iteration on D K in FP64 (block-Jacobi factors in d_Minv, vectors in d_p and d_q, dead
+// before a solve), every sub-problem at once; then (c0, c1) per sub-problem into P.cheb_c and
+// the host scalars (written back).  Returns 3 when a sub-problem's quotient is not positive.
+static int cheb_setup(Problem &P) {
+    const WgTables wt = tables(P);
+    const dim3 g(P.n_wg), blk(256);
+    hipLaunchKernelGGL(k_power_init, g, blk, 0, P.stream, P.d_p, wt, P.d_partials);
+    finalize(P, FIN_POWER, SEL_EVERY);
+    for (int k = 0; k < CHEB_POWER_STEPS; ++k) {
+        launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, P.d_p, nullptr, P.d_q, SEL_EVERY);
+        hipLaunchKernelGGL(k_power, g, blk, 0, P.stream, P.d_p, P.d_q, P.d_Minv, wt, P.d_ps, P.d_partials);
+        finalize(P, FIN_POWER, SEL_EVERY);
+    }
+    if (hipGetLastError() != hipSuccess || read_scalars(P)) return 5;
+    P.cheb_c.assign(2 * (size_t)P.n_sub, 0.0);
+    int rc = 0;
+    for (uint32_t s = 0; s < P.n_sub; ++s) {
+        ProbScalars &S = P.h_ps[s];
+        const double lam = S.pw_rq;
+        if (!(lam > 0) || !std::isfinite(lam)) {
+            S.status = 3;
+            rc = 3;
+            continue;
+        }
+        const double b = CHEB_SAFETY * lam, a = CHEB_LOWER * b;
+        const double th = 0.5 * (b + a), de = 0.5 * (b - a), t2 = 2 * (th / de) * (th / de) - 1;
+        S.cheb_c0 = P.cheb_c[2 * s] = 4 * th / (de * de) / t2;
+        S.cheb_c1 = P.cheb_c[2 * s + 1] = -2 / (de * de) / t2;
+    }
+    if (write_scalars(P)) return 5;
+    return rc;
 }
 
 template <int PRE>
@@ -989,7 +1188,8 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         if (mixed && !P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
             return 5;
         (void)hipEventRecord(P.ev[2], P.stream);
-        hipLaunchKernelGGL(k_build_minv<PRE>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
+        hipLaunchKernelGGL(k_build_minv<jacobi_kind<PRE>()>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv,
+                           P.d_Minv32, wt, P.d_ps);
         (void)hipEventRecord(P.ev[3], P.stream);
         P.minv_kind = PRE;
         built_minv = true;
@@ -997,6 +1197,24 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         P.last_precond_ms = 0;
     }
     if (read_scalars(P)) return 5;
+    // Chebyshev: its coefficients from a power iteration with the factors just built, else those
+    // kept from the build
+    if (PRE == PRE_CHEB) {
+        bool ok = true;
+        for (uint32_t s = 0; s < P.n_sub; ++s) ok &= !P.h_ps[s].status;
+        if (ok && (built_minv || P.cheb_c.size() != 2 * (size_t)P.n_sub)) {
+            const int rc = cheb_setup(P);
+            if (rc == 5) return 5;
+            (void)hipEventRecord(P.ev[3], P.stream);
+            if (hipEventSynchronize(P.ev[3]) != hipSuccess) return 5;
+        } else if (ok) {
+            for (uint32_t s = 0; s < P.n_sub; ++s) {
+                P.h_ps[s].cheb_c0 = P.cheb_c[2 * s];
+                P.h_ps[s].cheb_c1 = P.cheb_c[2 * s + 1];
+            }
+            if (write_scalars(P)) return 5;
+        }
+    }
     if (built_minv) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, P.ev[2], P.ev[3]) == hipSuccess) P.last_precond_ms = ms;
@@ -1087,7 +1305,11 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
 
 int precond_build(Problem &P, int precond) {
     if (precond == 2 || precond == 3) return ic0_ensure(P, precond == 2 ? IC0_NATURAL : IC0_COLOURED, true);
-    if (precond != 0 && precond != 1) return 1;
+    if (precond < 0 || precond > 4) return 1;
+    if (precond == 4) {
+        const int rc = spmv_ensure(P, 0);  // the power iteration's FP64 product
+        if (rc) return rc;
+    }
     if (!P.d_Minv32 && hipMalloc(&P.d_Minv32, sizeof(float) * 9 * std::max<uint32_t>(1, P.n_nodes)) != hipSuccess)
         return 5;
     std::memset(P.h_ps, 0, sizeof(ProbScalars) * P.n_sub);
@@ -1098,18 +1320,26 @@ int precond_build(Problem &P, int precond) {
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
     (void)hipEventRecord(P.ev[2], P.stream);
-    if (precond == 1)
+    if (precond != 0)
         hipLaunchKernelGGL(k_build_minv<PRE_BLOCK_JACOBI>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
     else
         hipLaunchKernelGGL(k_build_minv<PRE_JACOBI>, g, blk, 0, P.stream, P.d_diag, P.d_K, P.ps, P.d_Minv, P.d_Minv32, wt, P.d_ps);
     (void)hipEventRecord(P.ev[3], P.stream);
     if (hipGetLastError() != hipSuccess || read_scalars(P)) return 5;
+    bool ok = true;
+    for (uint32_t s = 0; s < P.n_sub; ++s) ok &= !P.h_ps[s].status;
+    if (ok && precond == 4) {
+        const int rc = cheb_setup(P);
+        if (rc == 5) return 5;
+        (void)hipEventRecord(P.ev[3], P.stream);
+        if (hipEventSynchronize(P.ev[3]) != hipSuccess) return 5;
+    }
     for (uint32_t s = 0; s < P.n_sub; ++s)
         if (P.h_ps[s].status) {
             P.minv_kind = -1;
             return P.h_ps[s].status;
         }
-    P.minv_kind = precond == 1 ? PRE_BLOCK_JACOBI : PRE_JACOBI;
+    P.minv_kind = precond == 4 ? PRE_CHEB : precond == 1 ? PRE_BLOCK_JACOBI : PRE_JACOBI;
     float ms = 0;
     if (hipEventElapsedTime(&ms, P.ev[2], P.ev[3]) == hipSuccess) P.last_precond_ms = ms;
     return 0;
@@ -1170,6 +1400,7 @@ int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, 
         case 1: return solve_impl<PRE_BLOCK_JACOBI>(P, d_rhs, d_x, o, out, 0);
         case 2: return solve_impl<PRE_IC0>(P, d_rhs, d_x, o, out, IC0_NATURAL);
         case 3: return solve_impl<PRE_IC0>(P, d_rhs, d_x, o, out, IC0_COLOURED);
+        case 4: return solve_impl<PRE_CHEB>(P, d_rhs, d_x, o, out, 0);
         default: return solve_impl<PRE_JACOBI>(P, d_rhs, d_x, o, out, 0);
     }
 }

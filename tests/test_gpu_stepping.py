@@ -207,7 +207,7 @@ def test_rhs_only_assembly_equals_full_assembly():
     f.close()
 
 
-@pytest.mark.parametrize("precond", [L.BLOCK_JACOBI, L.BLOCK_IC0])
+@pytest.mark.parametrize("precond", [L.BLOCK_JACOBI, L.BLOCK_IC0, L.CHEBYSHEV_BLOCK_JACOBI])
 def test_rhs_only_assembly_after_get_precond_keeps_penalty_scale(oracle, precond):
     """assemble(); get_precond(); assemble(False, True); solve_system_using_precond with non-zero
     prescribed displacements: the F-only assembly scales the penalty right-hand side with the

@@ -253,9 +253,12 @@ struct Problem {
     int last_chunks = 0, last_gaps = 0;
     double last_chunk_ms = 0, last_gap_ms = 0;
     bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
-    hipGraphExec_t graph_exec = nullptr;
-    uint64_t graph_key = 0;
-    const double *graph_x = nullptr;   // solution vector the captured chunk updates
+    // captured chunks: slot 0 holds a chunk of the solve's maximum length, slot k (1..5) one of
+    // 2^k iterations (the shorter chunks that end at a predicted pause are sums of these)
+    static constexpr int N_GRAPHS = 6;
+    hipGraphExec_t graph_exec[N_GRAPHS] = {};
+    uint64_t graph_key[N_GRAPHS] = {};
+    const double *graph_x[N_GRAPHS] = {};  // solution vector each captured chunk updates
     Ic0 ic0;
     SymDev sym;
     MatFree mf;

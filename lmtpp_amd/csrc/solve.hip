@@ -202,13 +202,15 @@ enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2, PRE_CHEB = 3 };
 // the block-Jacobi factors of D serve both
 template <int PRE>
 constexpr int jacobi_kind() { return PRE == PRE_CHEB ? PRE_BLOCK_JACOBI : PRE; }
+// PRE_CHEB in the vector updates and the restart: z = D r (k_cheb forms the preconditioned
+// residual and its partials afterwards, so these write none)
 // The preconditioner's values of node n (3 for Jacobi, 6 for block Jacobi), loaded ahead of their use.
 template <int PRE, typename T>
 __device__ inline void load_m(const T *__restrict__ Minv, size_t n, T (&m)[6]) {
     if (PRE == 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) m[c] = Minv[3 * n + c];
-    } else if (PRE == 1) {
+    } else if (PRE == PRE_BLOCK_JACOBI || PRE == PRE_CHEB) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) m[c] = Minv[6 * n + c];
     }
@@ -219,7 +221,7 @@ template <int PRE, typename T>
 __device__ inline void apply_mv(const T (&m)[6], const T r[3], T z[3]) {
     if (PRE == 0) {
         z[0] = m[0] * r[0]; z[1] = m[1] * r[1]; z[2] = m[2] * r[2];
-    } else if (PRE == 1) {
+    } else if (PRE == PRE_BLOCK_JACOBI || PRE == PRE_CHEB) {
         z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
         z[1] = m[1] * r[0] + m[3] * r[1] + m[4] * r[2];
         z[2] = m[2] * r[0] + m[4] * r[1] + m[5] * r[2];
@@ -230,7 +232,7 @@ template <int PRE, typename T>
 __device__ inline void apply_m(const T *__restrict__ Minv, size_t n, const T r[3], T z[3]) {
     if (PRE == 0) {
         z[0] = Minv[3 * n] * r[0]; z[1] = Minv[3 * n + 1] * r[1]; z[2] = Minv[3 * n + 2] * r[2];
-    } else if (PRE == 1) {
+    } else if (PRE == PRE_BLOCK_JACOBI || PRE == PRE_CHEB) {
         const T *m = Minv + 6 * n;
         z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
         z[1] = m[1] * r[0] + m[3] * r[1] + m[4] * r[2];
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
         const T qn[3] = {q[3 * (size_t)n], q[3 * (size_t)n + 1], q[3 * (size_t)n + 2]};
         update_node<PRE, T>(n, alpha, x, r, p, qn, Minv, z, acc);
     }
-    wg_partial<3, 4u>(acc, partials, gridDim.x, L);
+    if (PRE != PRE_CHEB) wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
 
 // The same after a matrix-free product with the fused direction update, with the product's
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(256) void k_update_mf(T *__restrict__ x, T *__restr
             acc[2] = fmax(acc[2], fabs((double)rv[d]));
         }
     }
-    wg_partial<3, 4u>(acc, partials, gridDim.x, L);
+    if (PRE != PRE_CHEB) wg_partial<3, 4u>(acc, partials, gridDim.x, L);
 }
 
 // p = z + beta p, or p = z when first, with z = M^-1 r as the update, the restart or the IC(0)
@@ -403,7 +405,7 @@ __global__ __launch_bounds__(256) void k_restart(const double *r64, T *r, T *__r
             }
         }
     }
-    if (PRE != PRE_IC0) wg_partial<1, 0u>(acc, partials, gridDim.x, L);
+    if (PRE != PRE_IC0 && PRE != PRE_CHEB) wg_partial<1, 0u>(acc, partials, gridDim.x, L);
 }
 
 // Partials of r.z (IC(0): after the sweeps); finalized by FIN_RHO, FIN_RESTART_FIRST or
@@ -423,12 +425,13 @@ __global__ __launch_bounds__(256) void k_dot_rz(const T *__restrict__ r, const T
     wg_partial<1, 0u>(acc, partials, gridDim.x, L);
 }
 
-// Second stage of the Chebyshev preconditioner: with y = D r (stored in z by the vector update or
-// the restart) and v = K y (q, or for a node shared by several matrix-free blocks the sum of its
-// partials halo[hbeg[n] .. hbeg[n + 1]) when hbeg is given), z = c0 y + c1 D v, in place.  first:
-// also p = z.  NV 3 (iteration): partials of r.z, r.r, max|r| for FIN_UPD; NV 1 (restart): of r.z.
+// Second stage of the Chebyshev preconditioner: with v = K y for y = D r (y stored in z by the
+// vector update or the restart; v in q, or for a node shared by several matrix-free blocks the sum
+// of its partials halo[hbeg[n] .. hbeg[n + 1]) when hbeg is given), z = c0 y + c1 D v =
+// D (c0 r + c1 v), written over y.  first: also p = z.  NV 3 (iteration): partials of r.z, r.r,
+// max|r| for FIN_UPD; NV 1 (restart): of r.z.
 template <typename T, int NV>
-__global__ __launch_bounds__(256) void k_cheb(const T *__restrict__ r, T *z, const T *__restrict__ q,
+__global__ __launch_bounds__(256) void k_cheb(const T *__restrict__ r, T *__restrict__ z, const T *__restrict__ q,
                                               const T *__restrict__ halo, const uint32_t *__restrict__ hbeg,
                                               const T *__restrict__ Minv, T *__restrict__ p, WgTables wt,
                                               const ProbScalars *S, double *partials, int sel, int first) {
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(256) void k_cheb(const T *__restrict__ r, T *z, con
     double acc[3] = {0, 0, 0};
     const uint32_t n = wt.row0[L] + threadIdx.x;
     if (n < wt.row1[L]) {
-        T m[6], v[3] = {0, 0, 0}, dv[3];
+        T m[6], v[3] = {0, 0, 0}, rv[3], w[3], zv[3];
         load_m<PRE_BLOCK_JACOBI, T>(Minv, n, m);
         const uint32_t k0 = hbeg ? hbeg[n] : 0u, k1 = hbeg ? hbeg[n + 1] : 0u;
         if (k0 == k1) {
@@ -452,16 +455,20 @@ __global__ __launch_bounds__(256) void k_cheb(const T *__restrict__ r, T *z, con
                 v[2] += halo[3 * (size_t)k + 2];
             }
         }
-        apply_mv<PRE_BLOCK_JACOBI, T>(m, v, dv);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            rv[d] = r[3 * (size_t)n + d];
+            w[d] = c0 * rv[d] + c1 * v[d];
+        }
+        apply_mv<PRE_BLOCK_JACOBI, T>(m, w, zv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const size_t i = 3 * (size_t)n + d;
-            const T rv = r[i], zv = c0 * z[i] + c1 * dv[d];
-            z[i] = zv;
-            if (first) p[i] = zv;
-            acc[0] += (double)rv * (double)zv;
-            acc[1] += (double)rv * (double)rv;
-            acc[2] = fmax(acc[2], fabs((double)rv));
+            z[i] = zv[d];
+            if (first) p[i] = zv[d];
+            acc[0] += (double)rv[d] * (double)zv[d];
+            acc[1] += (double)rv[d] * (double)rv[d];
+            acc[2] = fmax(acc[2], fabs((double)rv[d]));
         }
     }
     if (NV == 3) {
@@ -484,7 +491,8 @@ __device__ inline double unit_hash(uint32_t i) {
     return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
-__global__ __launch_bounds__(256) void k_power_init(double *__restrict__ v, WgTables wt, double *partials) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_power_init(T *__restrict__ v, WgTables wt, double *partials) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB) return;
@@ -494,9 +502,9 @@ __global__ __launch_bounds__(256) void k_power_init(double *__restrict__ v, WgTa
         const uint32_t local = n - wt.row0[wt.sub_wg0[s]];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double h = unit_hash(3 * local + d);
+            const T h = (T)unit_hash(3 * local + d);
             v[3 * (size_t)n + d] = h;
-            acc[2] += h * h;
+            acc[2] += (double)h * (double)h;
         }
     }
     wg_partial<3, 0u>(acc, partials, gridDim.x, L);
@@ -505,29 +513,29 @@ __global__ __launch_bounds__(256) void k_power_init(double *__restrict__ v, WgTa
 // One power step from u = K v: partials of v.u and u.D u (their ratio, the Rayleigh quotient of
 // K^1/2 D K^1/2 at K^1/2 v, bounds the largest eigenvalue of D K from below) and v <- D u / ||v||
 // with the partial of its squared norm.
-__global__ __launch_bounds__(256) void k_power(double *__restrict__ v, const double *__restrict__ u,
-                                               const double *__restrict__ Minv, WgTables wt, const ProbScalars *S,
-                                               double *partials) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_power(T *__restrict__ v, const T *__restrict__ u, const T *__restrict__ Minv,
+                                               WgTables wt, const ProbScalars *S, double *partials) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
     if (s == NO_SUB) return;
-    const double inv = 1.0 / S[s].pw_norm;
+    const T inv = (T)(1.0 / S[s].pw_norm);
     double acc[3] = {0, 0, 0};
     const uint32_t n = wt.row0[L] + threadIdx.x;
     if (n < wt.row1[L]) {
-        double m[6], un[3], w[3];
-        load_m<PRE_BLOCK_JACOBI, double>(Minv, n, m);
+        T m[6], un[3], w[3];
+        load_m<PRE_BLOCK_JACOBI, T>(Minv, n, m);
 #pragma unroll
         for (int d = 0; d < 3; ++d) un[d] = u[3 * (size_t)n + d];
-        apply_mv<PRE_BLOCK_JACOBI, double>(m, un, w);
+        apply_mv<PRE_BLOCK_JACOBI, T>(m, un, w);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const size_t i = 3 * (size_t)n + d;
-            acc[0] += v[i] * un[d];
-            acc[1] += un[d] * w[d];
-            const double vn = w[d] * inv;
+            acc[0] += (double)v[i] * (double)un[d];
+            acc[1] += (double)un[d] * (double)w[d];
+            const T vn = w[d] * inv;
             v[i] = vn;
-            acc[2] += vn * vn;
+            acc[2] += (double)vn * (double)vn;
         }
     }
     wg_partial<3, 0u>(acc, partials, gridDim.x, L);
@@ -952,10 +960,12 @@ int apply_spmv_scratch(Problem &P, int precision, const double *d_x, double *d_y
 }
 
 void destroy_graph(Problem &P) {
-    if (P.graph_exec) (void)hipGraphExecDestroy(P.graph_exec);
-    P.graph_exec = nullptr;
-    P.graph_key = 0;
-    P.graph_x = nullptr;
+    for (int k = 0; k < Problem::N_GRAPHS; ++k) {
+        if (P.graph_exec[k]) (void)hipGraphExecDestroy(P.graph_exec[k]);
+        P.graph_exec[k] = nullptr;
+        P.graph_key[k] = 0;
+        P.graph_x[k] = nullptr;
+    }
 }
 
 static int read_scalars(Problem &P) {
@@ -1010,20 +1020,19 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
     const bool fused = fused_direction<T>(P, PRE);
-    constexpr int JP = jacobi_kind<PRE>();
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr);
     if constexpr (PRE != PRE_IC0) {
         if (fused) {
             // the product's shared-node pass and alpha are folded into the update
             const T *p_alt = reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
-            hipLaunchKernelGGL((k_update_mf<JP, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
+            hipLaunchKernelGGL((k_update_mf<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
                                reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
                                P.mf.d_nwg, Minv, z, wt, P.d_ps, P.d_partials);
         }
     }
     if (!fused) {
         finalize_pq(P, sizeof(T) == 4 ? 1 : 0);
-        hipLaunchKernelGGL((k_update<JP, T>), g, blk, 0, P.stream, x, r, p, nullptr, q, Minv, z, wt, P.d_ps, P.d_partials);
+        hipLaunchKernelGGL((k_update<PRE, T>), g, blk, 0, P.stream, x, r, p, nullptr, q, Minv, z, wt, P.d_ps, P.d_partials);
     }
     // Chebyshev: the update stored y = D r in z (its partials are replaced by k_cheb's)
     if (PRE == PRE_CHEB) launch_cheb<T, 3>(P, K, r, q, Minv, z, nullptr, (int)SEL_ACTIVE, 0);
@@ -1063,7 +1072,7 @@ static void launch_restart(Problem &P, const double *r64, T *r, T *p, const T *M
         hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, first ? (int)SEL_ALL : (int)SEL_PAUSE_DONE,
                            first);
     } else {
-        hipLaunchKernelGGL((k_restart<jacobi_kind<PRE>(), T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps,
+        hipLaunchKernelGGL((k_restart<PRE, T>), g, blk, 0, P.stream, r64, r, p, Minv, z, wt, P.d_ps,
                            P.d_partials, first);
         // Chebyshev: k_restart stored y = D r in z; the product of y goes to the dead q buffer (in
         // FP64 r64 itself, read by k_restart already)
@@ -1078,32 +1087,68 @@ static void launch_restart(Problem &P, const double *r64, T *r, T *p, const T *M
     }
 }
 
-// One chunk of iterations: replayed from a captured hipGraph when enabled (the kernels read all
-// state from device memory, so the same graph serves every chunk of every solve on d_x).
+// n iterations replayed from the captured graph of slot (captured on first use or when the
+// configuration or the solution vector changed; the kernels read all state from device memory, so
+// one graph serves every chunk of every solve on d_x).
 template <int PRE>
-static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
-    // level-scheduled IC(0) in natural order has hundreds of sweep launches per iteration: no graph
-    if (!P.use_graph || (PRE == PRE_IC0 && P.ic0.sched.n_levels() > 64)) {
-        for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, fp32, d_x);
-        return hipGetLastError() == hipSuccess ? 0 : 5;
-    }
+static int run_graph(Problem &P, bool fp32, double *d_x, int n, int slot) {
     const int pr = fp32 ? 1 : 0;
     // every field in its own bits; the solution pointer is compared separately
     const uint64_t key = (uint64_t)fp32 | ((uint64_t)PRE << 1) | ((uint64_t)P.spmv.format[pr] << 3) |
-                         ((uint64_t)P.spmv.unroll[pr] << 5) | ((uint64_t)P.spmv.loads[pr] << 7) | ((uint64_t)chunk << 9);
-    if (!P.graph_exec || P.graph_key != key || P.graph_x != d_x) {
-        destroy_graph(P);
+                         ((uint64_t)P.spmv.unroll[pr] << 5) | ((uint64_t)P.spmv.loads[pr] << 7) | ((uint64_t)n << 9);
+    hipGraphExec_t &ge = P.graph_exec[slot];
+    if (!ge || P.graph_key[slot] != key || P.graph_x[slot] != d_x) {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        ge = nullptr;
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 5;
-        for (int k = 0; k < chunk; ++k) launch_iteration<PRE>(P, fp32, d_x);
+        for (int k = 0; k < n; ++k) launch_iteration<PRE>(P, fp32, d_x);
         if (hipStreamEndCapture(P.stream, &g) != hipSuccess) return 5;
-        const hipError_t e = hipGraphInstantiate(&P.graph_exec, g, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (e != hipSuccess) { P.graph_exec = nullptr; return 5; }
-        P.graph_key = key;
-        P.graph_x = d_x;
+        if (e != hipSuccess) { ge = nullptr; return 5; }
+        P.graph_key[slot] = key;
+        P.graph_x[slot] = d_x;
     }
-    return hipGraphLaunch(P.graph_exec, P.stream) == hipSuccess ? 0 : 5;
+    return hipGraphLaunch(ge, P.stream) == hipSuccess ? 0 : 5;
+}
+
+// One chunk of n iterations (n even, at most full, the solve's maximum): replayed from captured
+// hipGraphs when enabled -- the full length from one graph, a shorter chunk as a sum of graphs of
+// 2^k iterations launched back to back.
+template <int PRE>
+static int run_chunk(Problem &P, bool fp32, double *d_x, int n, int full) {
+    // level-scheduled IC(0) in natural order has hundreds of sweep launches per iteration: no graph
+    if (!P.use_graph || (PRE == PRE_IC0 && P.ic0.sched.n_levels() > 64)) {
+        for (int k = 0; k < n; ++k) launch_iteration<PRE>(P, fp32, d_x);
+        return hipGetLastError() == hipSuccess ? 0 : 5;
+    }
+    if (n >= full) return run_graph<PRE>(P, fp32, d_x, full, 0);
+    while (n >= 2) {
+        int k = 1;
+        while (k + 1 < Problem::N_GRAPHS && (2 << k) <= n) ++k;
+        if (run_graph<PRE>(P, fp32, d_x, 1 << k, k)) return 5;
+        n -= 1 << k;
+    }
+    return 0;
+}
+
+// Iterations until the next pause of a single running problem (a check or an FP64 refresh: its
+// recurrence residual below the stop test's bound, or in MIXED below delta^2 of the last FP64
+// residual), predicted from the geometric rate of the last chunk, rounded up to even, within
+// [2, full]; full without a rate (the first chunk after a start or refresh, a residual that did
+// not fall).  A chunk that ends at the pause saves the iterations that would run after it as
+// no-ops (every kernel of a paused problem returns at once) until the chunk's end.
+static int predict_chunk(const ProbScalars &S, double rr_prev, long it_prev, int full) {
+    if (it_prev < 0 || S.iter <= it_prev || !(S.rr > 0) || !(rr_prev > S.rr)) return full;
+    const double rate = std::log(S.rr / rr_prev) / (double)(S.iter - it_prev);  // < 0
+    double thr = S.crit_kind == 0 ? S.stop2 : 0.0;
+    if (S.mixed) thr = std::max(thr, S.delta2 * S.rr_ref);
+    if (!(thr > 0)) return full;
+    const double k = std::log(thr / S.rr) / rate;
+    if (!(k < (double)full)) return full;
+    const int n = std::max(2, ((int)std::ceil(k) + 1) & ~1);
+    return std::min(n, full);
 }
 
 // Chebyshev preconditioner: z = p(D K) D r with p(t) = c0 + c1 t, the degree-2 residual
@@ -1116,19 +1161,27 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int chunk) {
 constexpr int CHEB_POWER_STEPS = 20;
 constexpr double CHEB_SAFETY = 1.15, CHEB_LOWER = 0.05;
 
-// Power iteration on D K in FP64 (block-Jacobi factors in d_Minv, vectors in d_p and d_q, dead
-// before a solve), every sub-problem at once; then (c0, c1) per sub-problem into P.cheb_c and
-// the host scalars (written back).  Returns 3 when a sub-problem's quotient is not positive.
-static int cheb_setup(Problem &P) {
+// Power iteration on D K in the solve's precision (FP32: the FP32 copies of K or the FP32
+// matrix-free product and of the block-Jacobi factors; vectors in the direction and product
+// buffers, dead before a solve), every sub-problem at once; then (c0, c1) per sub-problem into
+// P.cheb_c and the host scalars (written back).  Returns 3 when a sub-problem's quotient is not
+// positive.
+template <typename T>
+static void cheb_power(Problem &P, const T *K, T *v, T *u, const T *Minv) {
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
-    hipLaunchKernelGGL(k_power_init, g, blk, 0, P.stream, P.d_p, wt, P.d_partials);
+    hipLaunchKernelGGL(k_power_init<T>, g, blk, 0, P.stream, v, wt, P.d_partials);
     finalize(P, FIN_POWER, SEL_EVERY);
     for (int k = 0; k < CHEB_POWER_STEPS; ++k) {
-        launch_spmv_t<double, MODE_PLAIN>(P, P.d_K, P.d_p, nullptr, P.d_q, SEL_EVERY);
-        hipLaunchKernelGGL(k_power, g, blk, 0, P.stream, P.d_p, P.d_q, P.d_Minv, wt, P.d_ps, P.d_partials);
+        launch_spmv_t<T, MODE_PLAIN>(P, K, v, nullptr, u, SEL_EVERY);
+        hipLaunchKernelGGL(k_power<T>, g, blk, 0, P.stream, v, u, Minv, wt, P.d_ps, P.d_partials);
         finalize(P, FIN_POWER, SEL_EVERY);
     }
+}
+
+static int cheb_setup(Problem &P, bool fp32) {
+    if (fp32) cheb_power<float>(P, P.d_K32, P.d_p32, P.d_q32, P.d_Minv32);
+    else cheb_power<double>(P, P.d_K, P.d_p, P.d_q, P.d_Minv);
     if (hipGetLastError() != hipSuccess || read_scalars(P)) return 5;
     P.cheb_c.assign(2 * (size_t)P.n_sub, 0.0);
     int rc = 0;
@@ -1203,7 +1256,7 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         bool ok = true;
         for (uint32_t s = 0; s < P.n_sub; ++s) ok &= !P.h_ps[s].status;
         if (ok && (built_minv || P.cheb_c.size() != 2 * (size_t)P.n_sub)) {
-            const int rc = cheb_setup(P);
+            const int rc = cheb_setup(P, mixed);
             if (rc == 5) return 5;
             (void)hipEventRecord(P.ev[3], P.stream);
             if (hipEventSynchronize(P.ev[3]) != hipSuccess) return 5;
@@ -1235,7 +1288,13 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     } else {
         launch_restart<PRE, double>(P, P.d_r, P.d_r, P.d_p, P.d_Minv, zbuf<PRE, double>(P), 1);
     }
-    int chunk = 2;
+    const int full = std::max(2, o.chunk);
+    int chunk = P.use_graph ? full : 2;
+    // single problems: chunks end at the predicted pause (a batch keeps full chunks: its paused
+    // sub-problems idle while the others run); without graphs the first chunks double from 2
+    const bool predict = P.n_sub == 1;
+    double rr_prev = 0;
+    long it_prev = -1;
     const long max_chunks = (long)o.max_iter / 2 + 1000;
     // chunk timing: events 4-6 as a ring (the previous chunk's end stays for the gap)
     P.last_chunks = P.last_gaps = 0;
@@ -1251,7 +1310,7 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         if (prev_end == 4) ev_s = 6;
         else if (prev_end == 5) ev_e = 6;
         (void)hipEventRecord(P.ev[ev_s], P.stream);
-        if (run_chunk<PRE>(P, mixed, d_x, P.use_graph ? std::max(2, o.chunk) : chunk)) return 5;
+        if (run_chunk<PRE>(P, mixed, d_x, chunk, full)) return 5;
         (void)hipEventRecord(P.ev[ev_e], P.stream);
         if (read_scalars(P)) return 5;
         float t = 0;
@@ -1262,7 +1321,14 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         }
         ++P.last_chunks;
         prev_end = ev_e;
-        chunk = std::min(chunk * 2, std::max(2, o.chunk));
+        const int grown = P.use_graph ? full : std::min(chunk * 2, full);
+        if (predict) {
+            chunk = std::min(grown, predict_chunk(P.h_ps[0], rr_prev, it_prev, full));
+            rr_prev = P.h_ps[0].rr;
+            it_prev = P.h_ps[0].iter;
+        } else {
+            chunk = grown;
+        }
         bool all_done = true, refresh = false;
         for (uint32_t s = 0; s < P.n_sub; ++s) {
             all_done &= P.h_ps[s].done != 0;
@@ -1283,6 +1349,8 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             if (mixed) launch_restart<PRE, float>(P, P.d_r, P.d_r32, P.d_p32, P.d_Minv32, zbuf<PRE, float>(P), 0);
             else launch_restart<PRE, double>(P, P.d_q, P.d_r, P.d_p, P.d_Minv, zbuf<PRE, double>(P), 0);
             hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
+            it_prev = -1;  // the recurrence restarts from the FP64 residual
+            chunk = P.use_graph ? full : std::max(chunk, 2);
         }
     }
     if (mixed) hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_EVERY);
@@ -1329,7 +1397,7 @@ int precond_build(Problem &P, int precond) {
     bool ok = true;
     for (uint32_t s = 0; s < P.n_sub; ++s) ok &= !P.h_ps[s].status;
     if (ok && precond == 4) {
-        const int rc = cheb_setup(P);
+        const int rc = cheb_setup(P, false);
         if (rc == 5) return 5;
         (void)hipEventRecord(P.ev[3], P.stream);
         if (hipEventSynchronize(P.ev[3]) != hipSuccess) return 5;

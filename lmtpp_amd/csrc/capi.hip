@@ -74,6 +74,7 @@ void free_problem(Problem &P) {
     dfree(P.d_r32); dfree(P.d_p32); dfree(P.d_q32); dfree(P.d_x32); dfree(P.d_Minv32); dfree(P.d_z); dfree(P.d_z32);
     dfree(P.d_partials); dfree(P.d_ps); dfree(P.d_flags);
     dfree(P.d_wg_row0); dfree(P.d_wg_row1); dfree(P.d_wg_sub); dfree(P.d_sub_wg0); dfree(P.d_sub_nwg);
+    dfree(P.d_fwg_row0); dfree(P.d_fwg_row1); dfree(P.d_fwg_sub); dfree(P.d_fsub_wg0); dfree(P.d_fsub_nwg);
     dfree(P.d_gptr); dfree(P.d_gel); dfree(P.d_perm); dfree(P.d_iperm); dfree(P.d_nperm); dfree(P.d_ninv); dfree(P.d_Kb);
     free_gather(P.ga);
     if (P.h_ps) (void)hipHostFree(P.h_ps);
@@ -86,23 +87,23 @@ void free_problem(Problem &P) {
     P.stream = nullptr;
 }
 
-// Workgroup tables: every sub-problem's rows split into workgroups of up to 256 rows; the grid is
-// rounded up to a multiple of 8 with idle workgroups (sub = NO_SUB).
-void build_wg_tables(Problem &P, std::vector<uint32_t> &row0, std::vector<uint32_t> &row1, std::vector<uint32_t> &sub,
-                     std::vector<uint32_t> &swg0, std::vector<uint32_t> &snwg) {
+// Workgroup tables: every sub-problem's rows split into workgroups of up to `rows` rows; the grid
+// is rounded up to a multiple of 8 with idle workgroups (sub = NO_SUB).  Returns the grid size.
+uint32_t build_wg_tables(Problem &P, uint32_t rows, std::vector<uint32_t> &row0, std::vector<uint32_t> &row1,
+                         std::vector<uint32_t> &sub, std::vector<uint32_t> &swg0, std::vector<uint32_t> &snwg) {
     swg0.assign(P.n_sub, 0);
     snwg.assign(P.n_sub, 0);
     for (uint32_t s = 0; s < P.n_sub; ++s) {
         swg0[s] = (uint32_t)row0.size();
-        for (uint32_t r = P.node_off[s]; r < P.node_off[s + 1]; r += ROWS_PER_WG) {
+        for (uint32_t r = P.node_off[s]; r < P.node_off[s + 1]; r += rows) {
             row0.push_back(r);
-            row1.push_back(std::min(P.node_off[s + 1], r + ROWS_PER_WG));
+            row1.push_back(std::min(P.node_off[s + 1], r + rows));
             sub.push_back(s);
         }
         snwg[s] = (uint32_t)row0.size() - swg0[s];
     }
     while (row0.size() % 8 || row0.empty()) { row0.push_back(0); row1.push_back(0); sub.push_back(NO_SUB); }
-    P.n_wg = (uint32_t)row0.size();
+    return (uint32_t)row0.size();
 }
 
 // Classes of ASM_GATHER_BOX: every element an axis-aligned Hexa box (else 1), classed by its exact
@@ -270,12 +271,21 @@ int alloc_system(Problem &P) {
     HIPCHK(upload(&P.d_diag, P.pat.diag.data(), P.pat.diag.size()));
     {
         std::vector<uint32_t> row0, row1, sub, swg0, snwg;
-        build_wg_tables(P, row0, row1, sub, swg0, snwg);
+        P.n_wg = build_wg_tables(P, ROWS_PER_WG, row0, row1, sub, swg0, snwg);
         HIPCHK(upload(&P.d_wg_row0, row0.data(), row0.size()));
         HIPCHK(upload(&P.d_wg_row1, row1.data(), row1.size()));
         HIPCHK(upload(&P.d_wg_sub, sub.data(), sub.size()));
         HIPCHK(upload(&P.d_sub_wg0, swg0.data(), swg0.size()));
         HIPCHK(upload(&P.d_sub_nwg, snwg.data(), snwg.size()));
+    }
+    {
+        std::vector<uint32_t> row0, row1, sub, swg0, snwg;
+        P.n_fwg = build_wg_tables(P, FOLD_ROWS_PER_WG, row0, row1, sub, swg0, snwg);
+        HIPCHK(upload(&P.d_fwg_row0, row0.data(), row0.size()));
+        HIPCHK(upload(&P.d_fwg_row1, row1.data(), row1.size()));
+        HIPCHK(upload(&P.d_fwg_sub, sub.data(), sub.size()));
+        HIPCHK(upload(&P.d_fsub_wg0, swg0.data(), swg0.size()));
+        HIPCHK(upload(&P.d_fsub_nwg, snwg.data(), snwg.size()));
     }
     HIPCHK(dalloc(&P.d_K, 9 * P.ps));
     HIPCHK(dalloc(&P.d_F, n));
@@ -292,7 +302,7 @@ int alloc_system(Problem &P) {
     HIPCHK(dalloc(&P.d_z, n));
     HIPCHK(dalloc(&P.d_z32, n));
     HIPCHK(dalloc(&P.d_partials, 3 * (size_t)P.n_wg));
-    HIPCHK(dalloc(&P.d_ps, P.n_sub));
+    HIPCHK(dalloc(&P.d_ps, 2 * (size_t)P.n_sub));
     HIPCHK(dalloc(&P.d_flags, 4));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&P.h_ps), sizeof(ProbScalars) * P.n_sub, hipHostMallocDefault));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&P.h_flags), sizeof(int) * 4, hipHostMallocDefault));

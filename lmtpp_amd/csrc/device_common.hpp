@@ -54,6 +54,25 @@ __device__ void wg_partial(double (&v)[NV], double *partials, unsigned stride, u
     }
 }
 
+// The same for a workgroup of NW waves (the waves combined in order).
+template <int NV, unsigned MAXMASK, int NW>
+__device__ void wg_partial_n(double (&v)[NV], double *partials, unsigned stride, unsigned L) {
+    __shared__ double sm[NV][NW];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double t = (MAXMASK >> k & 1) ? wave_max(v[k]) : wave_sum(v[k]);
+        if (lane == 0) sm[k][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        double o = sm[k][0];
+        for (int w = 1; w < NW; ++w) o = (MAXMASK >> k & 1) ? fmax(o, sm[k][w]) : o + sm[k][w];
+        partials[(size_t)k * stride + L] = o;
+    }
+}
+
 // One workgroup (up to 1024 lanes) combines the partials of workgroups [w0, w0 + nw) in a fixed
 // order: each round issues the loads of all NV values for four strided elements per lane at once
 // (one memory round trip per round: a finalize after a row kernel of 1024 x 4 workgroups needs
@@ -98,7 +117,90 @@ __device__ void wg_combine(const double *partials, unsigned stride, unsigned w0,
     }
 }
 
+// A 256-lane workgroup combines the partials of workgroups [w0, w0 + nw): lane l takes elements
+// l, l + 256, l + 512, ... in that order (lane_partials, loads issued BATCH at a time ahead of
+// their adds), then a wave tree and the four waves in order (wg_combine_lanes).  The value of
+// index k depends only on the partials of index k, so every workgroup that combines the same
+// partials gets the same bits whatever NV it asks for and however its loads are batched (the
+// two-kernel CG iteration relies on it: every element block forms beta itself).
+template <int NV, unsigned MAXMASK, int BATCH>
+__device__ void lane_partials(const double *partials, unsigned stride, unsigned w0, unsigned nw, double (&a)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) a[k] = 0.0;
+    for (unsigned i0 = threadIdx.x; i0 < nw; i0 += BATCH * 256u) {
+        double t[NV][BATCH];
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const unsigned i = i0 + j * 256u;
+                t[k][j] = i < nw ? partials[(size_t)k * stride + w0 + i] : 0.0;
+            }
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) a[k] = (MAXMASK >> k & 1) ? fmax(a[k], t[k][j]) : a[k] + t[k][j];
+    }
+}
+
+template <int NV, unsigned MAXMASK>
+__device__ void wg_combine_lanes(const double (&a)[NV], double (&out)[NV]) {
+    __shared__ double sm[NV][4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double t = (MAXMASK >> k & 1) ? wave_max(a[k]) : wave_sum(a[k]);
+        if (lane == 0) sm[k][wave] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        out[k] = (MAXMASK >> k & 1) ? fmax(fmax(sm[k][0], sm[k][1]), fmax(sm[k][2], sm[k][3]))
+                                    : ((sm[k][0] + sm[k][1]) + sm[k][2]) + sm[k][3];
+}
+
+template <int NV, unsigned MAXMASK, int BATCH>
+__device__ void wg_combine256(const double *partials, unsigned stride, unsigned w0, unsigned nw, double (&out)[NV]) {
+    double a[NV];
+    lane_partials<NV, MAXMASK, BATCH>(partials, stride, w0, nw, a);
+    wg_combine_lanes<NV, MAXMASK>(a, out);
+}
+
 __device__ inline bool finite(double v) { return v == v && fabs(v) <= 1.7976931348623157e308; }
+
+constexpr int STALL_CHECKS = 8, STALL_ITERS = 1000;
+
+// One sub-problem's scalars copied by the first lanes of a workgroup, one 8-byte word each (no
+// lane holds the whole record).
+__device__ inline void copy_scalars(ProbScalars *dst, const ProbScalars *src) {
+    static_assert(sizeof(ProbScalars) % 8 == 0 && sizeof(ProbScalars) / 8 <= 64, "scalar record layout");
+    if (threadIdx.x < sizeof(ProbScalars) / 8)
+        reinterpret_cast<uint64_t *>(dst)[threadIdx.x] = reinterpret_cast<const uint64_t *>(src)[threadIdx.x];
+}
+
+// End of a CG iteration's vector update from its combined partials o = (r.z, r.r, max|r|) (with_rz
+// false: IC(0), whose r.z follows the sweeps): advance the iteration count and rho, form beta and
+// apply the stop test.  The recurrence residual only requests a check: the solve ends on the true
+// residual b - K x (FIN_RES_CHECK), in FP64 as in MIXED, where an FP32 recurrence that has fallen
+// by delta since the last FP64 residual also pauses for a refresh.
+__device__ inline void fin_update(ProbScalars &P, const double (&o)[3], bool with_rz) {
+    P.iter += 1;
+    P.rho_prev = P.rho;
+    if (with_rz) {
+        P.rho = o[0];
+        P.beta = o[0] / P.rho_prev;
+    }
+    P.rr = o[1];
+    P.rmax = o[2];
+    if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
+    if (with_rz && P.cheb_c1 != 0 && o[0] < 0) { P.status = 3; P.done = 1; return; }
+    const bool conv = P.crit_kind == 0 ? o[1] <= P.stop2 : o[2] <= P.crit;
+    if (conv || (P.mixed && o[1] <= P.delta2 * P.rr_ref)) P.pause = PAUSE_REQ;
+    // no progress of the true residual since the last check that improved it, for as many
+    // iterations as all progress took (at least STALL_ITERS): stagnated above the tolerance
+    if (!P.done && P.checks >= 2 && P.iter - P.iter_best > max(P.iter_best, STALL_ITERS)) { P.status = 8; P.done = 1; }
+    if (!P.done && P.iter >= P.max_iter) { P.status = 4; P.done = 1; }
+}
 
 // det J and J^-1 by cofactors (exmatrix.cpp:98-125), J[j][i] = dx_i/dxi_j, so that
 // Ji[j][i] = dxi_i/dx_j and the physical gradient is grad_j = sum_i Ji[j][i] d/dxi_i.

@@ -31,6 +31,10 @@ struct ProbScalars {
     int iter_best;                // iteration of the check that set res_best
     double cheb_c0, cheb_c1;      // Chebyshev preconditioner: z = c0 D r + c1 D K D r
     double pw_norm, pw_rq;        // power iteration: norm of its vector, Rayleigh quotient
+    // folded iteration (two kernels): the last vector update's partials of r.z, r.r, max|r| are
+    // not yet applied (iter, rho, beta, the stop test); the next element-block kernel or the
+    // chunk's closing pass applies them
+    int fin_pending;
 };
 
 // pause states: 0 running, 1 FP64 refresh requested, 2 refreshed (direction update pending)
@@ -41,6 +45,10 @@ enum { SEL_ACTIVE = 0, SEL_PAUSE_REQ = 1, SEL_PAUSE_DONE = 2, SEL_ALL = 3 };
 struct SubMaterial { double nu, bf0, bf1, bf2; };
 
 constexpr uint32_t ROWS_PER_WG = 256;
+// Rows per workgroup of the vector update of the two-kernel CG iteration (1024 lanes, one row
+// each): the fewer its partials, the fewer bytes every element block of the next product reads
+// to form beta.
+constexpr uint32_t FOLD_ROWS_PER_WG = 1024;
 constexpr uint32_t ASM_STAGED = 0xffffffffu;
 constexpr uint32_t ASM_GATHER = 0xfffffffeu;
 constexpr uint32_t ASM_GATHER_BOX = 0xfffffffdu;
@@ -138,6 +146,8 @@ struct MatFree {
     double *d_fpen = nullptr;        // summed penalty M * P of each DOF of snap_dofs at that assembly
     double *d_partials = nullptr;    // one p.q partial per workgroup (element blocks, then isolated nodes)
     uint32_t *d_wg0 = nullptr, *d_nwg = nullptr;  // per sub-problem: its p.q partial slots (first, count)
+    uint32_t *d_b0 = nullptr;        // per sub-problem: its first block
+    bool fold = false;               // every sub-problem has a block: the CG iteration runs in two kernels
 };
 
 // Device side of the block IC(0) preconditioner (schedule in host_mesh.hpp).
@@ -193,6 +203,10 @@ struct Problem {
     uint32_t n_wg = 0;                                       // grid size, multiple of 8
     uint32_t *d_wg_row0 = nullptr, *d_wg_row1 = nullptr, *d_wg_sub = nullptr;
     uint32_t *d_sub_wg0 = nullptr, *d_sub_nwg = nullptr;     // per sub-problem workgroup range
+    // the same with workgroups of 4 x 256 rows (the vector update of the two-kernel CG iteration)
+    uint32_t n_fwg = 0;
+    uint32_t *d_fwg_row0 = nullptr, *d_fwg_row1 = nullptr, *d_fwg_sub = nullptr;
+    uint32_t *d_fsub_wg0 = nullptr, *d_fsub_nwg = nullptr;
 
     // device: mesh, pattern, scatter map
     double *d_coords = nullptr, *d_E = nullptr;
@@ -235,6 +249,9 @@ struct Problem {
     double *d_z = nullptr;    // z = M^-1 r of the Jacobi preconditioners (written by the update)
     float *d_z32 = nullptr;
     double *d_partials = nullptr;
+    // d_ps: the scalars of every sub-problem (n_sub), then a second copy (n_sub) that the
+    // two-kernel CG iteration passes between its kernels (the element-block kernel writes it from
+    // the first, the vector update writes the first back from it)
     ProbScalars *d_ps = nullptr, *h_ps = nullptr;
     int *d_flags = nullptr, *h_flags = nullptr;  // [0]: element with det J <= 0; [1]: IC(0) pivot <= 0
     int minv_kind = -1;  // preconditioner currently stored in d_Minv
@@ -334,8 +351,19 @@ int mf_prepare(Problem &P);
 // y = K x (pq: on the active sub-problem, with the p.q partials in mf.d_partials).  With z
 // (pq only): y = K p for the CG direction p = z + beta x (beta of the scalars), stored into x.
 // halo: where the shared nodes' partials go (3 values of T per slot); null: the problem's buffer.
-hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z = nullptr, void *halo = nullptr);
-hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z = nullptr, void *halo = nullptr);
+// The partials of r.z, r.r, max|r| a two-kernel CG iteration's element blocks combine: values
+// k * stride + w0[s] + i, i < nw[s], of sub-problem s.
+struct FinParts {
+    const double *part;
+    unsigned stride;
+    const uint32_t *w0, *nw;
+};
+// fin (with z): the two-kernel CG iteration, with the previous vector update's pending partials
+// (fin) applied by the element blocks (P.mf.fold problems only).
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z = nullptr, void *halo = nullptr,
+                     const FinParts *fin = nullptr);
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z = nullptr, void *halo = nullptr,
+                     const FinParts *fin = nullptr);
 // y = K x with the shared nodes' partials left in the halo buffer (summed by the caller's vector
 // kernel); pq: active sub-problems only, with the p.q partials.
 hipError_t mf_launch_partial(Problem &P, const float *x, float *y, bool pq);

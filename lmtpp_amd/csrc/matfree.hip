@@ -41,6 +41,11 @@
 #include <tuple>
 #include <type_traits>
 
+// Multiply-adds fuse only within one source expression (the C FP_CONTRACT rule), never across
+// statements at the compiler's choice: every instantiation of an element routine (the box-only
+// kernel, the mixed box/general one, batches) then rounds the same way, bit for bit.
+#pragma clang fp contract(on)
+
 namespace femgpu {
 
 namespace {
@@ -484,11 +489,17 @@ __device__ inline void tet10_apply(const TetRec<T> &r, const T (&u)[10][3], T (&
     }
 }
 
+// r.z partials per lane the folded element-block kernel loads with its node lists (a single
+// problem's vector update leaves one per 1024 nodes: 4 per lane cover 1M nodes)
+constexpr uint32_t MF_RZ_PER_LANE = 4;
+// Waves per SIMD the box-only element-block kernels are held to (the second __launch_bounds__
+// argument): 6 in FP32 (80 VGPRs) and 4 in FP64 (128), what the kernels without the folded
+// vector-update end reach by themselves.
 #ifndef MF_WG_F32
-#define MF_WG_F32 1
+#define MF_WG_F32 6
 #endif
 #ifndef MF_WG_F64
-#define MF_WG_F64 1
+#define MF_WG_F64 4
 #endif
 // LDS of k_mf_block: per local node its 3 input values and 3 accumulators, each triple at a
 // stride of MF_NS values.  (A stride of 4, one 16-byte access per node, was measured: 29.7 us
@@ -539,7 +550,16 @@ __device__ inline void lds_put3(T *p, T a, T b, T c) {
 // LDS: per local node 3 input values and 3 accumulators.
 // (FP64 Tetra_10: capped at 3 waves per SIMD, 168 VGPRs and 40 B of spills, against 184 VGPRs and
 // 2 waves uncapped: 80 against 84 us per product at 384k elements)
-template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA>
+// FIN (with PQ and DIR: the two-kernel CG iteration): the end of the previous vector update is
+// folded in.  When its partials are pending (fin_pending), every workgroup combines the r.z
+// partials rpart itself (in the fixed order of wg_combine256, so all hold the same bits) and forms
+// beta = r.z / rho; the sub-problem's first block also combines r.r and max|r|, applies the stop
+// test (fin_update) and writes the advanced scalars into S1, the copy the vector update reads (no
+// workgroup reads a value another one writes in the same kernel); without pending partials it
+// copies S into S1 unchanged.  A stop or pause decided here takes effect from the vector update
+// on: this launch still forms the next direction into the other buffer and its product, which
+// nothing reads, and leaves the stored direction as it was.
+template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA, bool FIN = false>
 __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES ? (sizeof(T) == 4 ? MF_WG_F32 : MF_WG_F64) : 1)) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   const uint32_t *__restrict__ bsub,
@@ -551,7 +571,10 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
                                                   const SubMaterial *__restrict__ mat,
                                                   const void *__restrict__ recv, const T *x, T *xb,
                                                   const T *__restrict__ z, T *__restrict__ y, T *__restrict__ halo,
-                                                  const ProbScalars *S, double *partials) {
+                                                  const ProbScalars *S, double *partials, ProbScalars *S1,
+                                                  const double *__restrict__ rpart, unsigned rstride,
+                                                  const uint32_t *__restrict__ rw0, const uint32_t *__restrict__ rnw,
+                                                  const uint32_t *__restrict__ b0) {
     extern __shared__ double mf_lds[];
     constexpr int NPE = K == TETRA10 ? 10 : 8;
     const unsigned b = xcd_slab();
@@ -561,9 +584,28 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         if (PQ && threadIdx.x == 0) partials[b] = 0.0;
         return;
     }
-    if (PQ && !selected(S[sb], SEL_ACTIVE)) return;
+    if (PQ && !selected(S[sb], SEL_ACTIVE)) {
+        if (FIN && b == b0[sb]) copy_scalars(S1 + sb, S + sb);
+        return;
+    }
+    // FIN: the pending update's iteration count is the one this product belongs to
+    const bool fin = FIN && S[sb].fin_pending;
+    // FIN: the r.z partials of the pending update, loaded here with the node lists (their memory
+    // round trip overlaps that of the node lists, ahead of the loads of z and the direction) when
+    // they fit one batch of MF_RZ_PER_LANE per lane; summed in the order of lane_partials
+    double rzt[MF_RZ_PER_LANE];
+    bool rz_pre = false;
+    if constexpr (FIN) {
+        const unsigned nw = rnw[sb], w0 = rw0[sb];
+        rz_pre = fin && nw <= MF_RZ_PER_LANE * MF_BLOCK;
+#pragma unroll
+        for (int j = 0; j < (int)MF_RZ_PER_LANE; ++j) {
+            const unsigned i = threadIdx.x + j * MF_BLOCK;
+            rzt[j] = rz_pre && i < nw ? rpart[w0 + i] : 0.0;
+        }
+    }
     // DIR: stored direction in, updated direction out (the two buffers by iteration parity)
-    const bool odd = DIR && (S[sb].iter & 1);
+    const bool odd = DIR && ((S[sb].iter + (fin ? 1 : 0)) & 1);
     const T *xs = odd ? xb : x;
     T *xo = odd ? const_cast<T *>(x) : xb;
     // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
@@ -571,7 +613,7 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     const size_t l0 = (size_t)b * ls;
     T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][MF_NS] the product's input
     T *acc = Ps + MF_NS * (size_t)ls;       // [ls][MF_NS] summed nodal forces
-    const T beta = DIR ? (T)S[sb].beta : (T)0;
+    T beta = DIR ? (T)S[sb].beta : (T)0;
     // everything the later phases read from memory is loaded here, ahead of the barriers (a load
     // after a barrier cannot be hoisted above it): the penalty range, and the global node and
     // partial slot of the lane's first MF_NODE_REGS local nodes (a brick's ~400; larger blocks
@@ -584,6 +626,30 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         gnode[r] = i < ls ? lnode[l0 + i] : MF_NO_NODE;
         gslot[r] = i < ls ? lslot[l0 + i] : 0u;
     }
+    // the element's block-local node indices (8 u16 per Hexa slot, 16 with 10 used per Tetra_10
+    // slot) and a box's record, loaded here with the node lists: the FIN prologue's barrier comes
+    // before the staging barrier, and no load after it can start earlier
+    const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
+    const uint32_t cbits = color[slot];
+    const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
+    const bool box = BOXES || (cbits != MF_EMPTY && (cbits & MF_BOX));
+    // (every slot, empty ones included, has its entries: the loads need not wait for the colour)
+    uint32_t lo[NPE];
+    BoxRec<T> br;    // a box: edge lengths and Hooke coefficients
+    {
+        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[K == TETRA10 ? 2 * slot : slot];
+        lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
+        lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
+        if constexpr (K == TETRA10) {
+            const uint32_t l89 = reinterpret_cast<const uint32_t *>(loc)[8 * slot + 4];
+            lo[NPE - 2] = l89 & 0xffffu;
+            lo[NPE - 1] = l89 >> 16;
+        } else if (BOXES) {
+            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
+        } else if (box) {
+            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
+        }
+    }
     auto stage = [&](uint32_t i, uint32_t gn) {
         const size_t g = 3 * (size_t)gn;
         T v[3];
@@ -592,33 +658,64 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         lds_put3(Ps + MF_NS * i, v[0], v[1], v[2]);
         lds_put3(acc + MF_NS * i, (T)0, (T)0, (T)0);
     };
+    if constexpr (FIN) {
+        // the operands of the first nodes' direction update are loaded before the combine of the
+        // partials that beta needs, so their latencies overlap
+        T zr[MF_NODE_REGS][3], sr[MF_NODE_REGS][3];
 #pragma unroll
-    for (int r = 0; r < (int)MF_NODE_REGS; ++r)
-        if (gnode[r] != MF_NO_NODE) stage(threadIdx.x + r * MF_BLOCK, gnode[r]);
+        for (int r = 0; r < (int)MF_NODE_REGS; ++r)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const size_t g = gnode[r] != MF_NO_NODE ? 3 * (size_t)gnode[r] + d : 0;
+                zr[r][d] = gnode[r] != MF_NO_NODE ? z[g] : (T)0;
+                sr[r][d] = gnode[r] != MF_NO_NODE ? xs[g] : (T)0;
+            }
+        const bool lead = b == b0[sb];
+        if (lead) copy_scalars(S1 + sb, S + sb);
+        if (fin) {
+            // r.z in every block (its barrier orders the lead's copy before the record below)
+            double a[1], rz[1];
+            if (rz_pre) {
+                a[0] = 0.0;
+#pragma unroll
+                for (int j = 0; j < (int)MF_RZ_PER_LANE; ++j) a[0] = a[0] + rzt[j];
+            } else {
+                lane_partials<1, 0u, MF_RZ_PER_LANE>(rpart, rstride, rw0[sb], rnw[sb], a);
+            }
+            wg_combine_lanes<1, 0u>(a, rz);
+            beta = (T)(rz[0] / S[sb].rho);
+            if (lead) {
+                // r.r and max|r| for the stop test
+                double rm[2];
+                wg_combine256<2, 2u, 2>(rpart + rstride, rstride, rw0[sb], rnw[sb], rm);
+                if (threadIdx.x == 0) {
+                    const double o[3] = {rz[0], rm[0], rm[1]};
+                    fin_update(S1[sb], o, true);
+                    S1[sb].fin_pending = 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < (int)MF_NODE_REGS; ++r)
+            if (gnode[r] != MF_NO_NODE) {
+                const uint32_t i = threadIdx.x + r * MF_BLOCK;
+                lds_put3(Ps + MF_NS * i, dir_value(zr[r][0], beta, sr[r][0]), dir_value(zr[r][1], beta, sr[r][1]),
+                         dir_value(zr[r][2], beta, sr[r][2]));
+                lds_put3(acc + MF_NS * i, (T)0, (T)0, (T)0);
+            }
+    } else {
+#pragma unroll
+        for (int r = 0; r < (int)MF_NODE_REGS; ++r)
+            if (gnode[r] != MF_NO_NODE) stage(threadIdx.x + r * MF_BLOCK, gnode[r]);
+    }
     for (uint32_t i = threadIdx.x + MF_NODE_REGS * MF_BLOCK; i < ls; i += MF_BLOCK)
         if (lnode[l0 + i] != MF_NO_NODE) stage(i, lnode[l0 + i]);
-    const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
-    const uint32_t cbits = color[slot];
-    const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
-    const bool box = BOXES || (cbits != MF_EMPTY && (cbits & MF_BOX));
     double pq[1] = {0.0};
     T f[NPE][3];
-    uint32_t lo[NPE];
     double X[8][3];  // corners of a general element
     double Hd[3] = {0, 0, 0};
-    BoxRec<T> br;    // a box: edge lengths and Hooke coefficients
     if (col != MF_EMPTY) {
-        // block-local node indices: 8 u16 per Hexa slot, 16 (10 used) per Tetra_10 slot
-        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[K == TETRA10 ? 2 * slot : slot];
-        lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
-        lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
-        if constexpr (K == TETRA10) {
-            const uint32_t l89 = reinterpret_cast<const uint32_t *>(loc)[8 * slot + 4];
-            lo[NPE - 2] = l89 & 0xffffu;
-            lo[NPE - 1] = l89 >> 16;
-        } else if (box) {
-            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
-        } else {
+        if constexpr (K != TETRA10) if (!box) {
             const uint4 ca = reinterpret_cast<const uint4 *>(conn)[2 * slot];
             const uint4 cb = reinterpret_cast<const uint4 *>(conn)[2 * slot + 1];
             const uint32_t nd[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
@@ -759,7 +856,7 @@ void mf_free(MatFree &M) {
     dfree(M.d_lnode_pad); dfree(M.d_lslot_pad); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof);
     dfree(M.d_isrc);
     dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
-    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_hbeg); dfree(M.d_bsub); dfree(M.d_wg0); dfree(M.d_nwg);
+    dfree(M.d_fpen); dfree(M.d_partials); dfree(M.d_rec64); dfree(M.d_rec32); dfree(M.d_pb); dfree(M.d_pb32); dfree(M.d_hbeg); dfree(M.d_bsub); dfree(M.d_wg0); dfree(M.d_nwg); dfree(M.d_b0);
     M = MatFree();
 }
 
@@ -1059,6 +1156,12 @@ int mf_build(Problem &P) {
             M.sub_b0[bsub[b]] = b;
             ++M.sub_nb[bsub[b]];
         }
+    // the two-kernel iteration needs a block of every sub-problem to carry its scalars
+    M.fold = std::find(M.sub_nb.begin(), M.sub_nb.end(), 0u) == M.sub_nb.end();
+    if (!upload_vec(&M.d_b0, M.sub_b0)) {
+        mf_free(M);
+        return 5;
+    }
     M.built = true;
     return 0;
 }
@@ -1180,30 +1283,35 @@ int mf_prepare(Problem &P) {
 
 // Element-block kernel of one variant (BOXES: every element of the mesh is an axis-aligned box;
 // Tetra_10 problems take the Tetra_10 kernel).
-template <typename T, bool PQ, bool DIR, bool BOXES>
-static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo) {
+template <typename T, bool PQ, bool DIR, bool BOXES, bool FIN = false>
+static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo, const FinParts *fin = nullptr) {
     const MatFree &M = P.mf;
     const void *rec = sizeof(T) == 4 ? M.d_rec32 : M.d_rec64;
+    const FinParts F = fin ? *fin : FinParts{nullptr, 0u, nullptr, nullptr};
 #define FEMGPU_MF_BLOCK(KIND)                                                                                         \
-    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND>), dim3(M.n_blocks), dim3(MF_BLOCK),        \
+    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND, FIN>), dim3(M.n_blocks), dim3(MF_BLOCK),   \
                        mf_lds_bytes(M.ls, sizeof(T)), P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.d_bsub,      \
                        M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
                        P.d_coords, M.d_E, M.d_mat, rec, x, xb, z, y, halo, P.d_ps,                                      \
-                       M.d_partials)
+                       M.d_partials, P.d_ps + P.n_sub, F.part, F.stride, F.w0, F.nw, M.d_b0)
     if (P.kind == TETRA10) FEMGPU_MF_BLOCK(TETRA10);
     else FEMGPU_MF_BLOCK(HEXA);
 #undef FEMGPU_MF_BLOCK
 }
 
 // halo: the shared-node partial slots (3 values of T each), the problem's own buffer or a caller's
-template <typename T, bool PQ, bool DIR>
-static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo, bool halo_pass = true) {
+// FIN: the two-kernel CG iteration (the nodes-of-no-element pass then reads the scalars the
+// element blocks advanced)
+template <typename T, bool PQ, bool DIR, bool FIN = false>
+static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo, bool halo_pass = true,
+                      const FinParts *fin = nullptr) {
     const MatFree &M = P.mf;
-    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true>(P, x, xb, z, y, halo);
-    else mf_block_launch<T, PQ, DIR, false>(P, x, xb, z, y, halo);
+    if (M.all_boxes) mf_block_launch<T, PQ, DIR, true, FIN>(P, x, xb, z, y, halo, fin);
+    else mf_block_launch<T, PQ, DIR, false, FIN>(P, x, xb, z, y, halo, fin);
     if (M.n_iso)
         hipLaunchKernelGGL((k_mf_iso<T, PQ, DIR>), dim3((3 * M.n_iso + 255) / 256), dim3(256), 0, P.stream, 3 * M.n_iso,
-                           M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, P.d_ps, M.d_partials, M.n_blocks);
+                           M.d_idof, M.d_isrc, M.d_fpen, x, xb, z, y, FIN ? P.d_ps + P.n_sub : P.d_ps, M.d_partials,
+                           M.n_blocks);
     // DIR (or !halo_pass): the caller's vector kernel sums the shared nodes' partials itself
     // (k_update_mf, k_cheb)
     if (M.n_bnodes && !DIR && halo_pass)
@@ -1216,11 +1324,15 @@ static void mf_passes(Problem &P, const T *x, T *xb, const T *z, T *y, T *halo, 
 // count, p stored into the other one (the direction update of the CG step, fused into the
 // product's passes); then y holds the nodes of no shared block only, and the shared nodes'
 // partials are left to the caller's vector update (k_update_mf).
+// fin (with z): the two-kernel CG iteration, the previous vector update's end folded in
+// (k_mf_block FIN, combining the partials fin describes); the problem must have P.mf.fold.
 template <typename T>
-static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z, void *halo) {
+static hipError_t mf_launch_t(Problem &P, const T *x, T *y, bool pq, const T *z, void *halo,
+                              const FinParts *fin = nullptr) {
     T *xb = reinterpret_cast<T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
     T *h = reinterpret_cast<T *>(halo ? halo : (void *)P.mf.d_halo);
-    if (pq && z) mf_passes<T, true, true>(P, x, xb, z, y, h);
+    if (pq && z && fin) mf_passes<T, true, true, true>(P, x, xb, z, y, h, true, fin);
+    else if (pq && z) mf_passes<T, true, true>(P, x, xb, z, y, h);
     else if (pq) mf_passes<T, true, false>(P, x, nullptr, nullptr, y, h);
     else mf_passes<T, false, false>(P, x, nullptr, nullptr, y, h);
     // peek, not get: a failed launch stays pending for the caller's own check (the solver
@@ -1276,11 +1388,12 @@ static hipError_t mf_launch_partial_t(Problem &P, const T *x, T *y, bool pq) {
 hipError_t mf_launch_partial(Problem &P, const float *x, float *y, bool pq) { return mf_launch_partial_t<float>(P, x, y, pq); }
 hipError_t mf_launch_partial(Problem &P, const double *x, double *y, bool pq) { return mf_launch_partial_t<double>(P, x, y, pq); }
 
-hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z, void *halo) {
-    return mf_launch_t<float>(P, x, y, pq, z, halo);
+hipError_t mf_launch(Problem &P, const float *x, float *y, bool pq, const float *z, void *halo, const FinParts *fin) {
+    return mf_launch_t<float>(P, x, y, pq, z, halo, fin);
 }
-hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z, void *halo) {
-    return mf_launch_t<double>(P, x, y, pq, z, halo);
+hipError_t mf_launch(Problem &P, const double *x, double *y, bool pq, const double *z, void *halo,
+                     const FinParts *fin) {
+    return mf_launch_t<double>(P, x, y, pq, z, halo, fin);
 }
 
 bool mf_ready(const Problem &P) {

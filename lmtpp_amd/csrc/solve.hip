@@ -293,33 +293,41 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
 // the fixed order of wg_combine, so every workgroup of a sub-problem holds the same alpha; the
 // sub-problem's first workgroup records it), and a node shared by several blocks takes q as the
 // sum of its partials halo[hbeg[n] .. hbeg[n + 1]) in block order, any other node the product's y.
-template <int PRE, typename T>
-__global__ __launch_bounds__(256) void k_update_mf(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
+// FOLD (two-kernel iteration): the scalars are read from Sin, the copy the element blocks wrote,
+// and the first workgroup writes them back into Sout with p.q recorded and the partials of this
+// update marked pending (fin_pending: the next element-block kernel applies them); else Sin and
+// Sout are the same array and only p.q is recorded.  FOLD workgroups take the tables of 1024-row
+// workgroups: a quarter of the partials for every element block to combine, and a quarter of the
+// workgroups combining the blocks' p.q partials.
+template <int PRE, typename T, bool FOLD>
+__global__ __launch_bounds__(FOLD && sizeof(T) == 8 ? 1024 : 256, FOLD && sizeof(T) == 8 ? 8 : 1) void k_update_mf(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
                                                    const T *__restrict__ q, const T *__restrict__ halo,
                                                    const uint32_t *__restrict__ hbeg, const double *__restrict__ mfpart,
                                                    const uint32_t *__restrict__ mf_w0, const uint32_t *__restrict__ mf_nw,
                                                    const T *__restrict__ Minv, T *__restrict__ z, WgTables wt,
-                                                   ProbScalars *S, double *partials) {
+                                                   const ProbScalars *Sin, ProbScalars *Sout, double *partials) {
     const unsigned L = xcd_slab();
     const uint32_t s = wt.sub[L];
-    if (s == NO_SUB || !selected(S[s], SEL_ACTIVE)) return;
-    // FP64: the node's operands are loaded before the combine of the p.q partials (which alpha
-    // needs), so that their latency overlaps it (48.2 against 50.1 us per launch at Hexa8 100^3);
-    // FP32 combines first (25.6 against 27.7 us: memory returns in order, and the combine's L2
-    // reads then wait behind the operand loads)
+    if (s == NO_SUB) return;
+    const ProbScalars *S = Sin;
+    const bool lead = L == wt.sub_wg0[s];
+    if (!selected(S[s], SEL_ACTIVE)) {
+        if (FOLD && lead) copy_scalars(Sout + s, Sin + s);
+        return;
+    }
+    if (FOLD && lead) copy_scalars(Sout + s, Sin + s);  // ordered before the record below by the combine's barrier
     if (!(S[s].iter & 1)) p = p_alt;
-    const uint32_t n = wt.row0[L] + threadIdx.x;
-    const bool valid = n < wt.row1[L];
-    T qn[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, mv[6] = {0, 0, 0, 0, 0, 0};
-    double o[1];
-    if (sizeof(T) == 4) wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
-    if (valid) {
+    // the node's operands: q (a shared node: the sum of its partials in block order), x, r, the
+    // direction, the preconditioner's values
+    auto load = [&](uint32_t n, T (&qn)[3], T (&xv)[3], T (&rv)[3], T (&pv)[3], T (&mv)[6]) {
         load_m<PRE, T>(Minv, n, mv);
         const uint32_t k0 = hbeg[n], k1 = hbeg[n + 1];
         if (k0 == k1) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) qn[d] = q[3 * (size_t)n + d];
         } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) qn[d] = 0;
             for (uint32_t k = k0; k < k1; ++k) {
                 qn[0] += halo[3 * (size_t)k];
                 qn[1] += halo[3 * (size_t)k + 1];
@@ -332,13 +340,9 @@ __global__ __launch_bounds__(256) void k_update_mf(T *__restrict__ x, T *__restr
             rv[d] = r[3 * (size_t)n + d];
             pv[d] = p[3 * (size_t)n + d];
         }
-    }
-    if (sizeof(T) == 8) wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
-    if (L == wt.sub_wg0[s] && threadIdx.x == 0) finish_pq(S[s], o[0]);
-    if (!finite(o[0]) || !(o[0] > 0)) return;  // recorded by workgroup 0: the solve stops
-    const T alpha = (T)(S[s].rho / o[0]);
-    double acc[3] = {0, 0, 0};
-    if (valid) {
+    };
+    auto update = [&](uint32_t n, T alpha, const T (&qn)[3], const T (&xv)[3], T (&rv)[3], const T (&pv)[3],
+                      const T (&mv)[6], double (&acc)[3]) {
         T zv[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -354,8 +358,53 @@ __global__ __launch_bounds__(256) void k_update_mf(T *__restrict__ x, T *__restr
             acc[1] += (double)rv[d] * (double)rv[d];
             acc[2] = fmax(acc[2], fabs((double)rv[d]));
         }
+    };
+    auto record = [&](double pq) {
+        if (lead && threadIdx.x == 0) {
+            finish_pq(Sout[s], pq);
+            if (FOLD) Sout[s].fin_pending = Sout[s].done ? 0 : 1;
+        }
+    };
+    double o[1];
+    double acc[3] = {0, 0, 0};
+    // FOLD: 1024 rows per workgroup, as 1024 lanes of one row (FP64) or 256 lanes of 4 rows,
+    // rows row0 + lane + 256 j (FP32); measured at Hexa8 100^3: FP64 45.9 against 51.4 us, FP32
+    // 24.1 against 25.5 us
+    constexpr int NT = FOLD && sizeof(T) == 8 ? 1024 : 256;
+    constexpr int RPL = FOLD ? 1024 / NT : 1;
+    if constexpr (RPL == 1) {
+        // FP64: the node's operands are loaded before the combine of the p.q partials (which alpha
+        // needs), so that their latency overlaps it (48.2 against 50.1 us per launch at Hexa8
+        // 100^3); FP32 combines first (25.6 against 27.7 us: memory returns in order, and the
+        // combine's L2 reads then wait behind the operand loads)
+        const uint32_t n = wt.row0[L] + threadIdx.x;
+        const bool valid = n < wt.row1[L];
+        T qn[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, mv[6] = {0, 0, 0, 0, 0, 0};
+        if (sizeof(T) == 4) wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
+        if (valid) load(n, qn, xv, rv, pv, mv);
+        if (sizeof(T) == 8) wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
+        record(o[0]);
+        if (!finite(o[0]) || !(o[0] > 0)) return;  // recorded by workgroup 0: the solve stops
+        const T alpha = (T)(S[s].rho / o[0]);
+        if (valid) update(n, alpha, qn, xv, rv, pv, mv, acc);
+    } else {
+        wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
+        record(o[0]);
+        if (!finite(o[0]) || !(o[0] > 0)) return;
+        const T alpha = (T)(S[s].rho / o[0]);
+        for (int j = 0; j < RPL; ++j) {
+            const uint32_t n = wt.row0[L] + threadIdx.x + 256u * j;
+            if (n < wt.row1[L]) {
+                T qn[3], xv[3], rv[3], pv[3], mv[6];
+                load(n, qn, xv, rv, pv, mv);
+                update(n, alpha, qn, xv, rv, pv, mv, acc);
+            }
+        }
     }
-    if (PRE != PRE_CHEB) wg_partial<3, 4u>(acc, partials, gridDim.x, L);
+    if (PRE != PRE_CHEB) {
+        if (NT == 1024) wg_partial_n<3, 4u, 16>(acc, partials, gridDim.x, L);
+        else wg_partial<3, 4u>(acc, partials, gridDim.x, L);
+    }
 }
 
 // p = z + beta p, or p = z when first, with z = M^-1 r as the update, the restart or the IC(0)
@@ -550,7 +599,6 @@ __global__ __launch_bounds__(256) void k_power(T *__restrict__ v, const T *__res
 // its largest eigenvalue).
 enum { FIN_PQ = 0, FIN_UPD = 1, FIN_RES = 2, FIN_RES_CHECK = 3, FIN_RESTART_FIRST = 4, FIN_RESTART = 5,
        FIN_UPD_IC = 6, FIN_RHO = 7, FIN_POWER = 8 };
-constexpr int STALL_CHECKS = 8, STALL_ITERS = 1000;
 
 __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, const double *__restrict__ partials,
                                                   unsigned stride, const uint32_t *__restrict__ sub_wg0,
@@ -572,25 +620,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         double o[3];
         wg_combine<3, 4u>(partials, stride, w0, nw, o);
         if (threadIdx.x) return;
-        P.iter += 1;
-        P.rho_prev = P.rho;
-        if (what == FIN_UPD) {
-            P.rho = o[0];
-            P.beta = o[0] / P.rho_prev;
-        }
-        P.rr = o[1];
-        P.rmax = o[2];
-        if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
-        if (what == FIN_UPD && P.cheb_c1 != 0 && o[0] < 0) { P.status = 3; P.done = 1; return; }
-        // The recurrence residual only requests a check: the solve ends on the true residual
-        // b - K x (FIN_RES_CHECK), in FP64 as in MIXED, where an FP32 recurrence that has fallen
-        // by delta since the last FP64 residual also pauses for a refresh.
-        const bool conv = P.crit_kind == 0 ? o[1] <= P.stop2 : o[2] <= P.crit;
-        if (conv || (P.mixed && o[1] <= P.delta2 * P.rr_ref)) P.pause = PAUSE_REQ;
-        // no progress of the true residual since the last check that improved it, for as many
-        // iterations as all progress took (at least STALL_ITERS): stagnated above the tolerance
-        if (!P.done && P.checks >= 2 && P.iter - P.iter_best > max(P.iter_best, STALL_ITERS)) { P.status = 8; P.done = 1; }
-        if (!P.done && P.iter >= P.max_iter) { P.status = 4; P.done = 1; }
+        fin_update(P, o, what == FIN_UPD);
     } else if (what == FIN_RES || what == FIN_RES_CHECK) {
         double o[2];
         wg_combine<2, 2u>(partials, stride, w0, nw, o);
@@ -635,6 +665,23 @@ __global__ __launch_bounds__(1024) void k_finalize(int what, ProbScalars *S, con
         if (!finite(o[0])) { P.status = 6; P.done = 1; }
         else if (P.cheb_c1 != 0 && o[0] < 0) { P.status = 3; P.done = 1; }
     }
+}
+
+// Closing pass of a chunk of two-kernel iterations: applies the last vector update's pending
+// partials (fin_update), combined exactly as the element-block kernel combines them (the same
+// bits whether a chunk ends there or the next product folds them in).  One 256-lane workgroup per
+// sub-problem.
+__global__ __launch_bounds__(256) void k_fin_pending(ProbScalars *S, const double *__restrict__ partials, unsigned stride,
+                                                     const uint32_t *__restrict__ sub_wg0,
+                                                     const uint32_t *__restrict__ sub_nwg) {
+    const uint32_t s = blockIdx.x;
+    ProbScalars &P = S[s];
+    if (!selected(P, SEL_ACTIVE) || !P.fin_pending) return;
+    double o[3];
+    wg_combine256<3, 4u, 4>(partials, stride, sub_wg0[s], sub_nwg[s], o);
+    if (threadIdx.x) return;
+    fin_update(P, o, true);
+    P.fin_pending = 0;
 }
 
 __global__ void k_clear_pause(ProbScalars *S, uint32_t n_sub) {
@@ -722,11 +769,16 @@ static void finalize_pq(Problem &P, int precision) {
 static WgTables tables(const Problem &P) {
     return WgTables{P.d_wg_row0, P.d_wg_row1, P.d_wg_sub, P.d_sub_wg0, P.d_sub_nwg};
 }
+// workgroups of 1024 rows (4 per lane)
+static WgTables tables_fat(const Problem &P) {
+    return WgTables{P.d_fwg_row0, P.d_fwg_row1, P.d_fwg_sub, P.d_fsub_wg0, P.d_fsub_nwg};
+}
 
 // zdir (matrix-free MODE_PQ only): the direction update p = z + beta x is fused into the product.
+// fin (with zdir): the two-kernel iteration (the previous update's end folded into the product).
 template <typename T, int MODE>
 static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T *y, int sel,
-                          const T *zdir = nullptr, void *halo = nullptr) {
+                          const T *zdir = nullptr, void *halo = nullptr, const FinParts *fin = nullptr) {
     const dim3 g(P.n_wg), blk(256);
     const WgTables wt = tables(P);
     const int pr = sizeof(T) == 4 ? 1 : 0;
@@ -734,7 +786,7 @@ static void launch_spmv_t(Problem &P, const T *K, const T *x, const double *b, T
     // residual's partials
     if (P.spmv.format[pr] == FMT_MF) {
         // a launch error stays pending (mf_launch peeks) for run_chunk's and solve_impl's checks
-        (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr, halo);
+        (void)mf_launch(P, x, y, MODE == MODE_PQ, MODE == MODE_PQ ? zdir : nullptr, halo, MODE == MODE_PQ ? fin : nullptr);
         if (MODE == MODE_RES)
             hipLaunchKernelGGL(k_residual, g, blk, 0, P.stream, b, reinterpret_cast<double *>(y), wt, P.d_ps, P.d_partials,
                                sel);
@@ -997,6 +1049,22 @@ static bool fused_direction(const Problem &P, int pre) {
     return pre != PRE_IC0 && P.spmv.format[sizeof(T) == 4 ? 1 : 0] == FMT_MF;
 }
 
+// Two kernels per iteration (fused direction, every sub-problem with an element block): the end
+// of each vector update (k_finalize FIN_UPD) is folded into the next product's element blocks,
+// and k_fin_pending closes each chunk.
+template <typename T>
+static bool folded_iteration(const Problem &P, int pre) {
+    return fused_direction<T>(P, pre) && P.mf.fold;
+}
+
+// The partials a two-kernel iteration's element blocks (and the chunk's closing pass) combine:
+// the vector update's, in the 1024-row workgroup tables, or for the Chebyshev preconditioner
+// k_cheb's, in the 256-row tables.
+static FinParts fin_parts(const Problem &P, int pre) {
+    if (pre == PRE_CHEB) return FinParts{P.d_partials, P.n_wg, P.d_sub_wg0, P.d_sub_nwg};
+    return FinParts{P.d_partials, P.n_fwg, P.d_fsub_wg0, P.d_fsub_nwg};
+}
+
 // Chebyshev preconditioner, after y = D r was stored in z: v = K y into q (matrix-free: the
 // shared nodes' partials stay in the halo buffer for k_cheb), then z = c0 y + c1 D v with the
 // partials of r.z, r.r, max|r| (iteration, NV 3: the active sub-problems) or of r.z (restart, NV
@@ -1020,14 +1088,21 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const WgTables wt = tables(P);
     const dim3 g(P.n_wg), blk(256);
     const bool fused = fused_direction<T>(P, PRE);
-    launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr);
+    const bool fold = folded_iteration<T>(P, PRE);
+    const FinParts fp = fin_parts(P, PRE);
+    launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr, nullptr, fold ? &fp : nullptr);
     if constexpr (PRE != PRE_IC0) {
         if (fused) {
             // the product's shared-node pass and alpha are folded into the update
             const T *p_alt = reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
-            hipLaunchKernelGGL((k_update_mf<PRE, T>), g, blk, 0, P.stream, x, r, p, p_alt, q,
-                               reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
-                               P.mf.d_nwg, Minv, z, wt, P.d_ps, P.d_partials);
+            if (fold)
+                hipLaunchKernelGGL((k_update_mf<PRE, T, true>), dim3(P.n_fwg), dim3(sizeof(T) == 8 ? 1024 : 256), 0, P.stream, x, r, p, p_alt, q,
+                                   reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
+                                   P.mf.d_nwg, Minv, z, tables_fat(P), P.d_ps + P.n_sub, P.d_ps, P.d_partials);
+            else
+                hipLaunchKernelGGL((k_update_mf<PRE, T, false>), g, blk, 0, P.stream, x, r, p, p_alt, q,
+                                   reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
+                                   P.mf.d_nwg, Minv, z, wt, P.d_ps, P.d_ps, P.d_partials);
         }
     }
     if (!fused) {
@@ -1041,7 +1116,7 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
         ic0_apply(P, r, z, SEL_ACTIVE);
         hipLaunchKernelGGL(k_dot_rz<T>, g, blk, 0, P.stream, r, z, wt, P.d_ps, P.d_partials, (int)SEL_ACTIVE);
         finalize(P, FIN_RHO);
-    } else {
+    } else if (!fold) {
         finalize(P, FIN_UPD);
     }
     if (!fused) hipLaunchKernelGGL(k_update_p<T>, g, blk, 0, P.stream, z, p, wt, P.d_ps, (int)SEL_ACTIVE, 0);
@@ -1118,19 +1193,30 @@ static int run_graph(Problem &P, bool fp32, double *d_x, int n, int slot) {
 // 2^k iterations launched back to back.
 template <int PRE>
 static int run_chunk(Problem &P, bool fp32, double *d_x, int n, int full) {
+    // two-kernel iterations: the chunk's last update is applied by a closing pass, so the host
+    // reads the scalars of a whole iteration
+    const bool fold = fp32 ? folded_iteration<float>(P, PRE) : folded_iteration<double>(P, PRE);
+    auto close = [&]() {
+        if (fold) {
+            const FinParts fp = fin_parts(P, PRE);
+            hipLaunchKernelGGL(k_fin_pending, dim3(P.n_sub), dim3(256), 0, P.stream, P.d_ps, fp.part, fp.stride, fp.w0,
+                               fp.nw);
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 5;
+    };
     // level-scheduled IC(0) in natural order has hundreds of sweep launches per iteration: no graph
     if (!P.use_graph || (PRE == PRE_IC0 && P.ic0.sched.n_levels() > 64)) {
         for (int k = 0; k < n; ++k) launch_iteration<PRE>(P, fp32, d_x);
-        return hipGetLastError() == hipSuccess ? 0 : 5;
+        return close();
     }
-    if (n >= full) return run_graph<PRE>(P, fp32, d_x, full, 0);
+    if (n >= full) return run_graph<PRE>(P, fp32, d_x, full, 0) ? 5 : close();
     while (n >= 2) {
         int k = 1;
         while (k + 1 < Problem::N_GRAPHS && (2 << k) <= n) ++k;
         if (run_graph<PRE>(P, fp32, d_x, 1 << k, k)) return 5;
         n -= 1 << k;
     }
-    return 0;
+    return close();
 }
 
 // Iterations until the next pause of a single running problem (a check or an FP64 refresh: its

@@ -578,6 +578,43 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     extern __shared__ double mf_lds[];
     constexpr int NPE = K == TETRA10 ? 10 : 8;
     const unsigned b = xcd_slab();
+    // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
+    // loads need no other load first
+    const size_t l0 = (size_t)b * ls;
+    // everything the later phases read from memory at the block's own index: the penalty range,
+    // the global node and partial slot of the lane's first MF_NODE_REGS local nodes (a brick's
+    // ~400; larger blocks reload the rest), and the element's block-local node indices (8 u16 per
+    // Hexa slot, 16 with 10 used per Tetra_10 slot) and a box's record.  Every index is in bounds
+    // for every block, padding blocks and empty slots included.  The plain and p.q products load
+    // them ahead of the sub-problem's scalars (a load after a branch or a barrier cannot be
+    // hoisted above it: 24.5 against 25.2 us per FP32 launch at Hexa8 100^3, 38.8 against 39.6 in
+    // FP64); the iteration's FIN kernel after its scalars and r.z partials (loaded ahead, its
+    // iteration took 57.6 against 57.0 us in FP32).
+    uint32_t pk0, pk1, gnode[MF_NODE_REGS], gslot[MF_NODE_REGS], cbits, lo[NPE];
+    const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
+    BoxRec<T> br;    // a box: edge lengths and Hooke coefficients
+    auto load_block = [&]() {
+        pk0 = pptr[b];
+        pk1 = pptr[b + 1];
+#pragma unroll
+        for (int r = 0; r < (int)MF_NODE_REGS; ++r) {
+            const uint32_t i = threadIdx.x + r * MF_BLOCK;
+            gnode[r] = i < ls ? lnode[l0 + i] : MF_NO_NODE;
+            gslot[r] = i < ls ? lslot[l0 + i] : 0u;
+        }
+        cbits = color[slot];
+        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[K == TETRA10 ? 2 * slot : slot];
+        lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
+        lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
+        if constexpr (K == TETRA10) {
+            const uint32_t l89 = reinterpret_cast<const uint32_t *>(loc)[8 * slot + 4];
+            lo[NPE - 2] = l89 & 0xffffu;
+            lo[NPE - 1] = l89 >> 16;
+        } else if (BOXES || (cbits != MF_EMPTY && (cbits & MF_BOX))) {
+            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
+        }
+    };
+    if constexpr (!FIN) load_block();
     const uint32_t sb = bsub[b];  // the block's sub-problem (NO_SUB: an empty padding block)
     if (sb == NO_SUB) {
         // a single problem's p.q combine spans the padding blocks too: they contribute zero
@@ -608,48 +645,12 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     const bool odd = DIR && ((S[sb].iter + (fin ? 1 : 0)) & 1);
     const T *xs = odd ? xb : x;
     T *xo = odd ? const_cast<T *>(x) : xb;
-    // the block's local nodes at a fixed stride ls (MF_NO_NODE past its own count), so their
-    // loads need no other load first
-    const size_t l0 = (size_t)b * ls;
     T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][MF_NS] the product's input
     T *acc = Ps + MF_NS * (size_t)ls;       // [ls][MF_NS] summed nodal forces
     T beta = DIR ? (T)S[sb].beta : (T)0;
-    // everything the later phases read from memory is loaded here, ahead of the barriers (a load
-    // after a barrier cannot be hoisted above it): the penalty range, and the global node and
-    // partial slot of the lane's first MF_NODE_REGS local nodes (a brick's ~400; larger blocks
-    // reload the rest)
-    const uint32_t pk0 = pptr[b], pk1 = pptr[b + 1];
-    uint32_t gnode[MF_NODE_REGS], gslot[MF_NODE_REGS];
-#pragma unroll
-    for (int r = 0; r < (int)MF_NODE_REGS; ++r) {
-        const uint32_t i = threadIdx.x + r * MF_BLOCK;
-        gnode[r] = i < ls ? lnode[l0 + i] : MF_NO_NODE;
-        gslot[r] = i < ls ? lslot[l0 + i] : 0u;
-    }
-    // the element's block-local node indices (8 u16 per Hexa slot, 16 with 10 used per Tetra_10
-    // slot) and a box's record, loaded here with the node lists: the FIN prologue's barrier comes
-    // before the staging barrier, and no load after it can start earlier
-    const size_t slot = (size_t)b * MF_BLOCK + threadIdx.x;
-    const uint32_t cbits = color[slot];
+    if constexpr (FIN) load_block();
     const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
     const bool box = BOXES || (cbits != MF_EMPTY && (cbits & MF_BOX));
-    // (every slot, empty ones included, has its entries: the loads need not wait for the colour)
-    uint32_t lo[NPE];
-    BoxRec<T> br;    // a box: edge lengths and Hooke coefficients
-    {
-        const uint4 lq = reinterpret_cast<const uint4 *>(loc)[K == TETRA10 ? 2 * slot : slot];
-        lo[0] = lq.x & 0xffffu; lo[1] = lq.x >> 16; lo[2] = lq.y & 0xffffu; lo[3] = lq.y >> 16;
-        lo[4] = lq.z & 0xffffu; lo[5] = lq.z >> 16; lo[6] = lq.w & 0xffffu; lo[7] = lq.w >> 16;
-        if constexpr (K == TETRA10) {
-            const uint32_t l89 = reinterpret_cast<const uint32_t *>(loc)[8 * slot + 4];
-            lo[NPE - 2] = l89 & 0xffffu;
-            lo[NPE - 1] = l89 >> 16;
-        } else if (BOXES) {
-            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
-        } else if (box) {
-            br = reinterpret_cast<const BoxRec<T> *>(recv)[slot];
-        }
-    }
     auto stage = [&](uint32_t i, uint32_t gn) {
         const size_t g = 3 * (size_t)gn;
         T v[3];

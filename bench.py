@@ -641,6 +641,38 @@ def main():
     roof32 = mf_roof(True) if spmv_cfg["fp32"][0] == mf else roof_stored32
     roof64 = mf_roof(False) if spmv_cfg["fp64"][0] == mf else roof_stored64
 
+    def iter_roof(prec, tol_p):
+        # A whole CG iteration (block Jacobi, matrix-free product: the element-block kernel with
+        # the direction update and the previous update's end folded in, then the vector update):
+        # its algorithmic bytes over its device time, the iteration chunks' time (HIP events on
+        # the problem's stream, host polls excluded) over the iteration count.
+        from lmtpp_amd.traffic import mf_block_iter_bytes, update_iter_bytes
+        ok, st, err = solve(prec, tol_p)
+        nch, cms, ngaps, gms = f.chunk_stats()
+        fp32 = prec == L.MIXED
+        info = f.matrix_free_info()
+        kb = mf_block_iter_bytes(sz["n_elem"], sz["n_nodes"], info, fp32)
+        ub = update_iter_bytes(sz["n_nodes"], info, fp32, 1)
+        us = cms * 1e3 / max(1, st.iters)
+        gbs = (kb + ub) / (us * 1e-6) / 1e9
+        key = "fp32" if fp32 else "fp64"
+        t1, t2 = load_traffic("mf_block_iter_" + key), load_traffic("update_iter_" + key)
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "us_per_iteration": round(us, 2),
+                "algorithmic_bytes": {"element_blocks": kb, "vector_update": ub},
+                "traffic": (t1 + t2) if t1 and t2 else None,
+                "kernels": "k_mf_block<%s,true,true,true,2,true> + k_update_mf<1,%s,true>" % (
+                    ("float", "float") if fp32 else ("double", "double")),
+                "iters": int(st.iters), "chunks": int(nch), "chunk_ms": round(cms, 2), "host_gap_ms": round(gms, 3)}
+
+    roof_iter = None
+    if spmv_cfg["fp32"][0] == mf and spmv_cfg["fp64"][0] == mf:
+        f.precond = L.BLOCK_JACOBI
+        roof_iter = {"mixed": iter_roof(L.MIXED, mtol), "fp64": iter_roof(L.FP64, tol)}
+        log("iteration: MIXED %.1f us (%.2f of HBM), FP64 %.1f us (%.2f of HBM)" % (
+            roof_iter["mixed"]["us_per_iteration"], roof_iter["mixed"]["frac"], roof_iter["fp64"]["us_per_iteration"],
+            roof_iter["fp64"]["frac"]))
+
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "MDOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -665,6 +697,7 @@ def main():
         "spmv_formats": {"fp64": fmt_name[spmv_cfg["fp64"][0]], "fp32": fmt_name[spmv_cfg["fp32"][0]]},
         "roofline": roof32,
         "roofline_fp64_product": roof64,
+        "roofline_iteration": roof_iter,
         "roofline_stored_spmv": roof_stored32,
         "roofline_stored_spmv_fp64": roof_stored64,
         "preconditioners": precond,

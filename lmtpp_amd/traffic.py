@@ -2,7 +2,8 @@
 WRITE_SIZE counters that compares them with measured HBM traffic.
 
 The algorithmic bytes and flops of a launch (spmv_bytes, mf_block_bytes, mf_halo_bytes, mf_flops)
-are what the roofline fractions are computed on.  The counter workloads that run under rocprofv3
+(and mf_block_iter_bytes, update_iter_bytes for the two kernels of a CG iteration) are what the
+roofline fractions are computed on.  The counter workloads that run under rocprofv3
 live outside the package (profile_workloads.py at the repository root); their counters are read
 back with
 
@@ -79,6 +80,27 @@ def spmv_bytes(nnzb, n_nodes, fp32, fmt, col16=0):
     return nu * (v + 4) + nl * 8 + n_nodes * 8 + vec
 
 
+def mf_block_iter_bytes(n_elem, n_nodes, info, fp32):
+    """Algorithmic bytes of the CG iteration's element-block launch (k_mf_block with the direction
+    update and the end of the previous vector update folded in): mf_block_bytes with the input
+    formed from two vectors (z and the stored direction: one more vector read at every node), the
+    new direction stored once per node, and the p.q partial per block.  The r.z partials of the
+    previous update every block combines come from L2 and are not counted."""
+    s = 12 if fp32 else 24
+    return (mf_block_bytes(n_elem, n_nodes, info, fp32) + n_nodes * s +
+            (n_nodes - info["isolated_nodes"]) * s + info["blocks"] * 8)
+
+
+def update_iter_bytes(n_nodes, info, fp32, precond=1):
+    """Algorithmic bytes of the CG iteration's vector update (k_update_mf): per node x and r read
+    and written, the direction read, the preconditioner's values read (3 for Jacobi, 6 for block
+    Jacobi), z written, its partial range (4 B); q read at the nodes of one block and the partials
+    of the shared nodes read instead."""
+    s = 12 if fp32 else 24
+    m = (3 if precond == 0 else 6) * (4 if fp32 else 8)
+    return (n_nodes * (6 * s + m + 4) - info["shared_nodes"] * s + info["partial_slots"] * s)
+
+
 def _counters(d):
     rows = []
     for p in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
@@ -89,10 +111,16 @@ def _counters(d):
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
-        mf = re.search(r"k_mf_(block)<(double|float), false, false, (?:true|false)(?:, \d+)?>", name) or \
+        mf = re.search(r"k_mf_(block)<(double|float), false, false, (?:true|false)(?:, \d+)?(?:, false)?>", name) or \
             re.search(r"k_mf_(halo)<(double|float)(?:, false)?>", name)
+        it = re.search(r"k_mf_block<(double|float), true, true, (?:true|false), \d+, true>", name)
+        up = re.search(r"k_update_mf<\d, (double|float), true>", name)
         if "k_copy" in name:
             key = "copy"
+        elif it:
+            key = "mf_block_iter_%s" % ("fp64" if it.group(1) == "double" else "fp32")
+        elif up:
+            key = "update_iter_%s" % ("fp64" if up.group(1) == "double" else "fp32")
         elif mf:
             key = "mf_%s_%s" % (mf.group(1), "fp64" if mf.group(2) == "double" else "fp32")
         elif m:
@@ -123,7 +151,8 @@ def parse(fetch_dir, write_dir, nnzb, n_nodes):
                     "traffic_over_algorithmic": (fb + wb) / alg, "dispatches": len(fe[key])}
     # matrix-free kernels: measured bytes (their algorithmic bytes depend on the block layout and
     # are computed where the layout is known: mf_block_bytes, mf_halo_bytes)
-    for key in ("mf_block_fp64", "mf_block_fp32", "mf_halo_fp64", "mf_halo_fp32"):
+    for key in ("mf_block_fp64", "mf_block_fp32", "mf_halo_fp64", "mf_halo_fp32", "mf_block_iter_fp64",
+                "mf_block_iter_fp32", "update_iter_fp64", "update_iter_fp32"):
         if key not in fe or key not in wr:
             continue
         fb = sum(fe[key]) / len(fe[key]) * 1024.0 * f_scale

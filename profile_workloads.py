@@ -8,7 +8,9 @@ clamped and a traction on x=1 (the elastic-cube workload of bench.py):
   traffic    the plain copy kernel (k_copy of lmtpp_amd.traffic.COPY_DOUBLES doubles), then 5 FP64
              and 5 FP32 block SpMV launches per storage variant (full storage with non-temporal
              loads, with 16-bit column offsets, with plain loads; symmetric storage), then the
-             matrix-free product (element-block kernel and shared-node pass);
+             matrix-free product (element-block kernel and shared-node pass), then 64 CG
+             iterations (block Jacobi, matrix-free) in MIXED and in FP64 (the iteration's
+             element-block kernel and vector update);
   assembly   three assemblies with each element-loop variant: staged colour passes, gather, box
              gather;
   matfree    the matrix-free product: its element-block kernel and the whole product, 3 launches
@@ -60,6 +62,14 @@ def traffic(n):
     b32, p32 = f.time_matrix_free(L.MIXED, 5)
     print(json.dumps({"format": A.SPMV_MATRIX_FREE, "info": f.matrix_free_info(), "block_fp64_ms": b64,
                       "product_fp64_ms": p64, "block_fp32_ms": b32, "product_fp32_ms": p32}), flush=True)
+    # the CG iteration's two kernels: 64 block-Jacobi iterations in each precision
+    f.precond = L.BLOCK_JACOBI
+    for prec in (L.MIXED, L.FP64):
+        f.precision = prec
+        f.max_iter = 64
+        f.set_result(np.zeros(3 * len(c)))
+        f.solve_system(rel_tol=1e-30)
+        print("iterations (precision %d): %d, status %d" % (prec, f.last_stats.iters, f.last_stats.status), flush=True)
     f.close()
 
 

@@ -95,9 +95,9 @@ typedef struct {
                                  FEMGPU_MAX_ITER */
     int max_iter;             /* inner iterations cap (the reference has none) */
     double reliable_delta;    /* MIXED: FP64 residual refresh when the FP32 residual fell by this factor */
-    int chunk;                /* at most this many iterations enqueued between host checks (a
-                                 single problem's chunk ends at its predicted next check or
-                                 refresh) */
+    int chunk;                /* iterations enqueued between host checks; a single problem's
+                                 chunk ends at its predicted next check or refresh instead, up to
+                                 8 x chunk once its convergence rate is known */
 } femgpu_solve_opts;
 
 typedef struct {

@@ -1188,9 +1188,9 @@ static int run_graph(Problem &P, bool fp32, double *d_x, int n, int slot) {
     return hipGraphLaunch(ge, P.stream) == hipSuccess ? 0 : 5;
 }
 
-// One chunk of n iterations (n even, at most full, the solve's maximum): replayed from captured
-// hipGraphs when enabled -- the full length from one graph, a shorter chunk as a sum of graphs of
-// 2^k iterations launched back to back.
+// One chunk of n iterations (n even): replayed from captured hipGraphs when enabled -- graphs of
+// the full length (the solve's chunk option), the rest as a sum of graphs of 2^k iterations,
+// launched back to back.
 template <int PRE>
 static int run_chunk(Problem &P, bool fp32, double *d_x, int n, int full) {
     // two-kernel iterations: the chunk's last update is applied by a closing pass, so the host
@@ -1209,7 +1209,8 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int n, int full) {
         for (int k = 0; k < n; ++k) launch_iteration<PRE>(P, fp32, d_x);
         return close();
     }
-    if (n >= full) return run_graph<PRE>(P, fp32, d_x, full, 0) ? 5 : close();
+    for (; n >= full; n -= full)
+        if (run_graph<PRE>(P, fp32, d_x, full, 0)) return 5;
     while (n >= 2) {
         int k = 1;
         while (k + 1 < Problem::N_GRAPHS && (2 << k) <= n) ++k;
@@ -1221,20 +1222,22 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int n, int full) {
 
 // Iterations until the next pause of a single running problem (a check or an FP64 refresh: its
 // recurrence residual below the stop test's bound, or in MIXED below delta^2 of the last FP64
-// residual), predicted from the geometric rate of the last chunk, rounded up to even, within
-// [2, full]; full without a rate (the first chunk after a start or refresh, a residual that did
-// not fall).  A chunk that ends at the pause saves the iterations that would run after it as
-// no-ops (every kernel of a paused problem returns at once) until the chunk's end.
-static int predict_chunk(const ProbScalars &S, double rr_prev, long it_prev, int full) {
-    if (it_prev < 0 || S.iter <= it_prev || !(S.rr > 0) || !(rr_prev > S.rr)) return full;
-    const double rate = std::log(S.rr / rr_prev) / (double)(S.iter - it_prev);  // < 0
+// residual), from its squared residual rr now, the last FP64 one rr_ref and the geometric rate
+// of rr per iteration (log, < 0) measured over the chunks so far, kept across refreshes (the
+// recurrence restarts on the same operator); rounded up to even, within [2, cap]; dflt without a
+// rate.  A chunk that ends at the pause saves the iterations that would run after it as no-ops
+// (every kernel of a paused problem returns at once) until the chunk's end, and a long one the
+// host's polls in between.
+constexpr int PREDICTED_CHUNKS = 8;
+static int predict_chunk(const ProbScalars &S, double rr, double rr_ref, double rate, int dflt, int cap) {
+    if (!(rate < 0) || !(rr > 0)) return dflt;
     double thr = S.crit_kind == 0 ? S.stop2 : 0.0;
-    if (S.mixed) thr = std::max(thr, S.delta2 * S.rr_ref);
-    if (!(thr > 0)) return full;
-    const double k = std::log(thr / S.rr) / rate;
-    if (!(k < (double)full)) return full;
+    if (S.mixed) thr = std::max(thr, S.delta2 * rr_ref);
+    if (!(thr > 0)) return dflt;
+    const double k = std::log(thr / rr) / rate;
+    if (!(k < (double)cap)) return cap;
     const int n = std::max(2, ((int)std::ceil(k) + 1) & ~1);
-    return std::min(n, full);
+    return std::min(n, cap);
 }
 
 // Chebyshev preconditioner: z = p(D K) D r with p(t) = c0 + c1 t, the degree-2 residual
@@ -1376,10 +1379,12 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     }
     const int full = std::max(2, o.chunk);
     int chunk = P.use_graph ? full : 2;
-    // single problems: chunks end at the predicted pause (a batch keeps full chunks: its paused
-    // sub-problems idle while the others run); without graphs the first chunks double from 2
+    // single problems: chunks end at the predicted pause, up to PREDICTED_CHUNKS chunk lengths
+    // once a rate is known (a batch keeps full chunks: its paused sub-problems idle while the
+    // others run); without graphs the first chunks double from 2
     const bool predict = P.n_sub == 1;
-    double rr_prev = 0;
+    const int cap = PREDICTED_CHUNKS * full;
+    double rr_prev = 0, rate = 0;
     long it_prev = -1;
     const long max_chunks = (long)o.max_iter / 2 + 1000;
     // chunk timing: events 4-6 as a ring (the previous chunk's end stays for the gap)
@@ -1409,9 +1414,12 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
         prev_end = ev_e;
         const int grown = P.use_graph ? full : std::min(chunk * 2, full);
         if (predict) {
-            chunk = std::min(grown, predict_chunk(P.h_ps[0], rr_prev, it_prev, full));
-            rr_prev = P.h_ps[0].rr;
-            it_prev = P.h_ps[0].iter;
+            const ProbScalars &S = P.h_ps[0];
+            if (it_prev >= 0 && S.iter > it_prev && S.rr > 0 && rr_prev > S.rr)
+                rate = std::log(S.rr / rr_prev) / (double)(S.iter - it_prev);
+            chunk = predict_chunk(S, S.rr, S.rr_ref, rate, grown, cap);
+            rr_prev = S.rr;
+            it_prev = S.iter;
         } else {
             chunk = grown;
         }
@@ -1437,6 +1445,9 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
             it_prev = -1;  // the recurrence restarts from the FP64 residual
             chunk = P.use_graph ? full : std::max(chunk, 2);
+            // the restarted residual is the true one: the next pause comes after about the
+            // iterations the last cycle took; half of that, so that a faster start wastes nothing
+            if (predict) chunk = std::max(2, (predict_chunk(P.h_ps[0], P.h_ps[0].rr, P.h_ps[0].rr, rate, 2 * chunk, 2 * cap) / 2) & ~1);
         }
     }
     if (mixed) hipLaunchKernelGGL(k_accumulate, g, blk, 0, P.stream, d_x, P.d_x32, wt, P.d_ps, (int)SEL_EVERY);

@@ -180,27 +180,43 @@ __device__ inline void copy_scalars(ProbScalars *dst, const ProbScalars *src) {
 
 // End of a CG iteration's vector update from its combined partials o = (r.z, r.r, max|r|) (with_rz
 // false: IC(0), whose r.z follows the sweeps): advance the iteration count and rho, form beta and
-// apply the stop test.  The recurrence residual only requests a check: the solve ends on the true
-// residual b - K x (FIN_RES_CHECK), in FP64 as in MIXED, where an FP32 recurrence that has fallen
-// by delta since the last FP64 residual also pauses for a refresh.
-__device__ inline void fin_update(ProbScalars &P, const double (&o)[3], bool with_rz) {
-    P.iter += 1;
-    P.rho_prev = P.rho;
-    if (with_rz) {
-        P.rho = o[0];
-        P.beta = o[0] / P.rho_prev;
+// apply the stop test, reading the scalars from in and writing the changed ones into out (the same
+// record, or the copy the next kernel reads: in is then read-only in the kernel, its fields load
+// ahead of the partials' combine).  The recurrence residual only requests a check: the solve ends
+// on the true residual b - K x (FIN_RES_CHECK), in FP64 as in MIXED, where an FP32 recurrence that
+// has fallen by delta since the last FP64 residual also pauses for a refresh.
+__device__ inline void fin_update(const ProbScalars &in, ProbScalars &out, const double (&o)[3], bool with_rz) {
+    const int iter = in.iter + 1, checks = in.checks, iter_best = in.iter_best, max_iter = in.max_iter;
+    const double rho = in.rho, stop2 = in.stop2, crit = in.crit, delta2 = in.delta2, rr_ref = in.rr_ref;
+    const bool mixed = in.mixed != 0, rel = in.crit_kind == 0, cheb = in.cheb_c1 != 0;
+    int done = in.done, status = in.status, pause = in.pause;
+    if (!finite(o[0]) || !finite(o[1])) {
+        status = 6;
+        done = 1;
+    } else if (with_rz && cheb && o[0] < 0) {
+        status = 3;
+        done = 1;
+    } else {
+        const bool conv = rel ? o[1] <= stop2 : o[2] <= crit;
+        if (conv || (mixed && o[1] <= delta2 * rr_ref)) pause = PAUSE_REQ;
+        // no progress of the true residual since the last check that improved it, for as many
+        // iterations as all progress took (at least STALL_ITERS): stagnated above the tolerance
+        if (!done && checks >= 2 && iter - iter_best > max(iter_best, STALL_ITERS)) { status = 8; done = 1; }
+        if (!done && iter >= max_iter) { status = 4; done = 1; }
     }
-    P.rr = o[1];
-    P.rmax = o[2];
-    if (!finite(o[0]) || !finite(o[1])) { P.status = 6; P.done = 1; return; }
-    if (with_rz && P.cheb_c1 != 0 && o[0] < 0) { P.status = 3; P.done = 1; return; }
-    const bool conv = P.crit_kind == 0 ? o[1] <= P.stop2 : o[2] <= P.crit;
-    if (conv || (P.mixed && o[1] <= P.delta2 * P.rr_ref)) P.pause = PAUSE_REQ;
-    // no progress of the true residual since the last check that improved it, for as many
-    // iterations as all progress took (at least STALL_ITERS): stagnated above the tolerance
-    if (!P.done && P.checks >= 2 && P.iter - P.iter_best > max(P.iter_best, STALL_ITERS)) { P.status = 8; P.done = 1; }
-    if (!P.done && P.iter >= P.max_iter) { P.status = 4; P.done = 1; }
+    out.iter = iter;
+    out.rho_prev = rho;
+    if (with_rz) {
+        out.rho = o[0];
+        out.beta = o[0] / rho;
+    }
+    out.rr = o[1];
+    out.rmax = o[2];
+    out.status = status;
+    out.done = done;
+    out.pause = pause;
 }
+__device__ inline void fin_update(ProbScalars &P, const double (&o)[3], bool with_rz) { fin_update(P, P, o, with_rz); }
 
 // det J and J^-1 by cofactors (exmatrix.cpp:98-125), J[j][i] = dx_i/dxi_j, so that
 // Ji[j][i] = dxi_i/dx_j and the physical gradient is grad_j = sum_i Ji[j][i] d/dxi_i.
@@ -222,13 +238,15 @@ __device__ inline T jac_inverse(const T (&J)[3][3], T (&Ji)[3][3]) {
 }
 
 // alpha = rho / p.q once p.q of the iteration is known; a non-finite or non-positive p.q ends the
-// sub-problem (status 6 or 3).
-__device__ inline void finish_pq(ProbScalars &P, double pq) {
-    P.pq = pq;
-    if (!finite(pq)) { P.status = 6; P.done = 1; }
-    else if (!(pq > 0)) { P.status = 3; P.done = 1; }
-    else P.alpha = P.rho / pq;
+// sub-problem (status 6 or 3).  Reads in, writes out (the same record or a copy, as fin_update).
+__device__ inline void finish_pq(const ProbScalars &in, ProbScalars &out, double pq) {
+    const double rho = in.rho;
+    out.pq = pq;
+    if (!finite(pq)) { out.status = 6; out.done = 1; }
+    else if (!(pq > 0)) { out.status = 3; out.done = 1; }
+    else out.alpha = rho / pq;
 }
+__device__ inline void finish_pq(ProbScalars &P, double pq) { finish_pq(P, P, pq); }
 
 // Direction update p = z + beta p of one value, with one rounding (every kernel that forms the
 // direction uses this, so the separate update and the one fused into the matrix-free product

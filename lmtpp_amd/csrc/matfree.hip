@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
                 wg_combine256<2, 2u, 2>(rpart + rstride, rstride, rw0[sb], rnw[sb], rm);
                 if (threadIdx.x == 0) {
                     const double o[3] = {rz[0], rm[0], rm[1]};
-                    fin_update(S1[sb], o, true);
+                    fin_update(S[sb], S1[sb], o, true);
                     S1[sb].fin_pending = 0;
                 }
             }

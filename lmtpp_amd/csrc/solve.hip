@@ -361,7 +361,7 @@ __global__ __launch_bounds__(FOLD && sizeof(T) == 8 ? 1024 : 256, FOLD && sizeof
     };
     auto record = [&](double pq) {
         if (lead && threadIdx.x == 0) {
-            finish_pq(Sout[s], pq);
+            finish_pq(Sin[s], Sout[s], pq);
             if (FOLD) Sout[s].fin_pending = Sout[s].done ? 0 : 1;
         }
     };
@@ -392,6 +392,10 @@ __global__ __launch_bounds__(FOLD && sizeof(T) == 8 ? 1024 : 256, FOLD && sizeof
         record(o[0]);
         if (!finite(o[0]) || !(o[0] > 0)) return;
         const T alpha = (T)(S[s].rho / o[0]);
+#ifndef FEMGPU_UPD_UNROLL
+#define FEMGPU_UPD_UNROLL 1
+#endif
+#pragma unroll FEMGPU_UPD_UNROLL
         for (int j = 0; j < RPL; ++j) {
             const uint32_t n = wt.row0[L] + threadIdx.x + 256u * j;
             if (n < wt.row1[L]) {

@@ -638,8 +638,38 @@ def main():
             r["valu"]["frac_of_unpacked_fp32"] = round(ach / FP64_VECTOR_PEAK_TFLOPS, 4)
         return r
 
-    roof32 = mf_roof(True) if spmv_cfg["fp32"][0] == mf else roof_stored32
-    roof64 = mf_roof(False) if spmv_cfg["fp64"][0] == mf else roof_stored64
+    def iter_kernel_roof(fp32):
+        # The dominant kernel of a step: the CG iteration's element blocks (the matrix-free product
+        # with the direction update and the previous vector update's end folded in), timed with
+        # HIP events around each launch on the problem's stream (femgpu_time_iteration, 50
+        # iterations after 4 warm-up ones); its algorithmic bytes from mf_block_iter_bytes.
+        from lmtpp_amd.traffic import mf_block_iter_bytes
+        tb, tu = f.time_iteration(L.MIXED if fp32 else L.FP64, 50)
+        info = f.matrix_free_info()
+        flops = mf_flops(sz["n_elem"], info)
+        nbytes = mf_block_iter_bytes(sz["n_elem"], sz["n_nodes"], info, fp32)
+        peak = FP32_VECTOR_PEAK_TFLOPS if fp32 else FP64_VECTOR_PEAK_TFLOPS
+        ach = flops / (tb * 1e-3) / 1e12
+        gbs = nbytes / (tb * 1e-3) / 1e9
+        boxes = "true" if info["box_elements"] == sz["n_elem"] else "false"
+        r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(gbs / HBM_PEAK_GBS, 4),
+             "traffic": load_traffic("mf_block_iter_" + ("fp32" if fp32 else "fp64")),
+             "kernel": "k_mf_block<%s,true,true,%s,2,true> (%s CG iteration: matrix-free Hexa product with the "
+                       "direction update and the previous vector update's end folded in)" % (
+                           "float" if fp32 else "double", boxes, "FP32" if fp32 else "FP64"),
+             "avg_launch_ms": round(tb, 4), "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
+             "flop_per_byte": round(flops / nbytes, 2),
+             "valu": {"achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 4)},
+             "vector_update_ms": round(tu, 4), "layout": info}
+        if fp32:
+            r["valu"]["frac_of_unpacked_fp32"] = round(ach / FP64_VECTOR_PEAK_TFLOPS, 4)
+        return r
+
+    prod32 = mf_roof(True) if spmv_cfg["fp32"][0] == mf else None
+    prod64 = mf_roof(False) if spmv_cfg["fp64"][0] == mf else None
+    roof32 = iter_kernel_roof(True) if prod32 else roof_stored32
+    roof64 = iter_kernel_roof(False) if prod64 else roof_stored64
 
     def iter_roof(prec, tol_p):
         # A whole CG iteration (block Jacobi, matrix-free product: the element-block kernel with
@@ -696,7 +726,9 @@ def main():
         "iters_fp64": int(st64.iters),
         "spmv_formats": {"fp64": fmt_name[spmv_cfg["fp64"][0]], "fp32": fmt_name[spmv_cfg["fp32"][0]]},
         "roofline": roof32,
-        "roofline_fp64_product": roof64,
+        "roofline_fp64": roof64,
+        "roofline_product": prod32,
+        "roofline_fp64_product": prod64,
         "roofline_iteration": roof_iter,
         "roofline_stored_spmv": roof_stored32,
         "roofline_stored_spmv_fp64": roof_stored64,

@@ -467,6 +467,17 @@ FEMGPU_API int femgpu_matrix_free_info(femgpu_problem *p, uint64_t info[7]);
  * launches each on the problem's stream (needs an assembled matrix). */
 FEMGPU_API int femgpu_time_matrix_free(femgpu_problem *p, int precision, int reps, double *block_ms,
                                        double *product_ms);
+/* Average device times (ms) of the two kernels of the CG iteration on this problem (block Jacobi;
+ * the matrix-free product of precision FEMGPU_FP64 or the FP32 one with the previous vector
+ * update's end folded in, then the vector update), on the problem's stream with graph replay off:
+ * a solve of iters + 4 iterations with HIP events around each launch gives the two kernels'
+ * shares (the first 4 iterations left out), a second one with events around its chunks only the
+ * iteration's time, which is split in those shares.  The solves run into the problem's scratch
+ * vector from a zero initial guess; the result is left alone, and a preconditioner pinned by
+ * get_precond other than block Jacobi is released.  No counterpart in the reference: a
+ * measurement hook for the roofline of the iteration's dominant kernel. */
+FEMGPU_API int femgpu_time_iteration(femgpu_problem *p, int precision, int iters, double *block_ms,
+                                     double *update_ms);
 /* Times every block-SpMV variant (format x load kind x SELL slots per loop trip 1, 2, 3) on this assembled
  * matrix, for FP64 and for the FP32 copy, and keeps the fastest of each; *unroll receives the
  * FP64 pick (may be NULL).  Then times the element-loop variants (femgpu_set_assembly_rows) and

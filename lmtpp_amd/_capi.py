@@ -116,6 +116,7 @@ _SIGS = {
     "femgpu_apply": (C.c_int, [_P, C.c_int, _P, _P, C.c_size_t]),
     "femgpu_matrix_free_info": (C.c_int, [_P, _P]),
     "femgpu_time_matrix_free": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "femgpu_time_iteration": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "femgpu_autotune": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "femgpu_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "femgpu_stream_read": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),

@@ -1420,6 +1420,24 @@ int femgpu_time_matrix_free(femgpu_problem *h, int precision, int reps, double *
     return FEMGPU_OK;
 }
 
+int femgpu_time_iteration(femgpu_problem *h, int precision, int iters, double *block_ms, double *update_ms) {
+    if (!h || iters < 1 || precision < 0 || precision > 1) return fail(FEMGPU_BAD_INPUT, "time_iteration: bad input");
+    Problem &P = h->P;
+    if (P.n_sub != 1) return fail(FEMGPU_BAD_INPUT, "time_iteration: a single problem only");
+    if (!P.assembled_mat || !P.assembled_vec) return fail(FEMGPU_NOT_READY, "time_iteration: assemble first");
+    int rc = mf_prepare(P);
+    if (rc) return fail(rc == 1 ? FEMGPU_BAD_INPUT : FEMGPU_HIP_ERROR, "time_iteration: no matrix-free product");
+    femgpu_solve_opts bj;
+    femgpu_default_solve_opts(&bj);
+    bj.precond = FEMGPU_BLOCK_JACOBI;
+    unpin_precond(*h, &bj);
+    rc = time_iteration(P, precision, iters, block_ms, update_ms);
+    if (rc == 1) return fail(FEMGPU_BAD_INPUT, "time_iteration: the iteration of that precision is not the two-kernel one "
+                                               "(matrix-free product, one sub-problem)");
+    if (rc) return fail(FEMGPU_HIP_ERROR, "time_iteration: HIP error");
+    return FEMGPU_OK;
+}
+
 int femgpu_matrix_free_info(femgpu_problem *h, uint64_t info[7]) {
     if (!h || !info) return fail(FEMGPU_BAD_INPUT, "matrix_free_info: null");
     const int rc = mf_build(h->P);

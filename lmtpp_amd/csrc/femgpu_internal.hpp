@@ -270,6 +270,11 @@ struct Problem {
     int last_chunks = 0, last_gaps = 0;
     double last_chunk_ms = 0, last_gap_ms = 0;
     bool use_graph = false;        // replay chunks of iterations from a captured hipGraph
+    // kernel timing of the two-kernel iteration (femgpu_time_iteration, graphs off): events before
+    // the element blocks, between them and the vector update, after it, for the first kt_cap
+    // iterations
+    hipEvent_t *kt = nullptr;
+    int kt_cap = 0, kt_used = 0;
     // captured chunks: slot 0 holds a chunk of the solve's maximum length, slot k (1..5) one of
     // 2^k iterations (the shorter chunks that end at a predicted pause are sums of these)
     static constexpr int N_GRAPHS = 6;
@@ -315,6 +320,12 @@ struct SolveOpts { int precision, precond, crit_kind; double tol; int max_iter; 
 struct SolveOut { int status, iters, refreshes; double rel_res, max_abs_res, ms; };
 // Solves every sub-problem; out[s] per sub-problem; returns the first non-zero status.
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out);
+// Average device times (ms) of the two kernels of the CG iteration (block Jacobi, the matrix-free
+// product of precision 0 (FP64) or 1 (FP32) folded with the previous update's end, then the vector
+// update) over iters iterations of a solve of K d = F into the scratch vector d_aux, after
+// warm-up ones; returns 0, 1 (the product is not the matrix-free one for that precision, or the
+// iteration has three kernels) or 5.
+int time_iteration(Problem &P, int precision, int iters, double *block_ms, double *update_ms);
 double time_spmv(Problem &P, int precision, int reps);
 // d_y = K d_x (device, internal order) with the product variant of precision 0 (FP64) or 1 (FP32);
 // returns 0, 1 (the variant cannot serve this problem) or 5.

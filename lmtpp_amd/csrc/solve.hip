@@ -1094,7 +1094,10 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
     const bool fused = fused_direction<T>(P, PRE);
     const bool fold = folded_iteration<T>(P, PRE);
     const FinParts fp = fin_parts(P, PRE);
+    hipEvent_t *kt = fold && P.kt_used < P.kt_cap ? P.kt + 3 * (size_t)P.kt_used++ : nullptr;
+    if (kt) (void)hipEventRecord(kt[0], P.stream);
     launch_spmv_t<T, MODE_PQ>(P, K, p, nullptr, q, SEL_ACTIVE, fused ? z : nullptr, nullptr, fold ? &fp : nullptr);
+    if (kt) (void)hipEventRecord(kt[1], P.stream);
     if constexpr (PRE != PRE_IC0) {
         if (fused) {
             // the product's shared-node pass and alpha are folded into the update
@@ -1107,6 +1110,7 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
                 hipLaunchKernelGGL((k_update_mf<PRE, T, false>), g, blk, 0, P.stream, x, r, p, p_alt, q,
                                    reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
                                    P.mf.d_nwg, Minv, z, wt, P.d_ps, P.d_ps, P.d_partials);
+            if (kt) (void)hipEventRecord(kt[2], P.stream);
         }
     }
     if (!fused) {
@@ -1558,6 +1562,66 @@ int diff_norm(Problem &P, const double *a, const double *b, size_t n, double *no
     if (norm2) *norm2 = std::sqrt(s);
     if (norm_inf) *norm_inf = m;
     return 0;
+}
+
+int time_iteration(Problem &P, int precision, int iters, double *block_ms, double *update_ms) {
+    const int warm = 4, n = iters + warm;
+    if (iters < 1 || P.spmv.format[precision ? 1 : 0] != FMT_MF || !P.mf.fold || !P.assembled_mat || !P.assembled_vec)
+        return 1;
+    std::vector<hipEvent_t> ev(3 * (size_t)n, nullptr);
+    int rc = 0;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = 5;
+    const bool graph = P.use_graph;
+    if (!rc) {
+        P.use_graph = false;  // one launch at a time, the events between them
+        P.kt = ev.data();
+        P.kt_cap = n;
+        P.kt_used = 0;
+        // every iteration runs: a relative residual far below any the solve reaches
+        const SolveOpts o{precision ? 1 : 0, 1, 0, 1e-300, n, 1e-300, 32};
+        std::vector<SolveOut> out(P.n_sub);
+        if (launch_fill(P.d_aux, 3 * (size_t)P.n_nodes, 0.0, P.stream) != hipSuccess) rc = 5;
+        if (!rc) rc = solve_impl<PRE_BLOCK_JACOBI>(P, P.d_F, P.d_aux, o, out.data(), 0);
+        if (rc == 4 || rc == 8) rc = 0;  // the iteration cap (or a stall) ends this solve
+        if (!rc && hipStreamSynchronize(P.stream) != hipSuccess) rc = 5;
+        double tb = 0, tu = 0;
+        int m = 0;
+        for (int k = warm; !rc && k < P.kt_used; ++k) {
+            float a = 0, b = 0;
+            if (hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]) != hipSuccess ||
+                hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]) != hipSuccess) {
+                rc = 5;
+                break;
+            }
+            tb += a;
+            tu += b;
+            ++m;
+        }
+        if (!rc && m == 0) rc = 1;
+        // an event between two dependent kernels adds the gap of its own record to the second
+        // one: the iteration's time comes from a second solve with events around its chunks only
+        // (the closing pass of each chunk included), split in the proportion just measured
+        P.kt = nullptr;
+        P.kt_cap = P.kt_used = 0;
+        if (!rc && launch_fill(P.d_aux, 3 * (size_t)P.n_nodes, 0.0, P.stream) != hipSuccess) rc = 5;
+        if (!rc) {
+            rc = solve_impl<PRE_BLOCK_JACOBI>(P, P.d_F, P.d_aux, o, out.data(), 0);
+            if (rc == 4 || rc == 8) rc = 0;
+        }
+        if (!rc && !(out[0].iters > 0 && P.last_chunk_ms > 0)) rc = 1;
+        if (!rc) {
+            const double t_it = P.last_chunk_ms / out[0].iters, f_b = tb / (tb + tu);
+            if (block_ms) *block_ms = t_it * f_b;
+            if (update_ms) *update_ms = t_it * (1.0 - f_b);
+        }
+    }
+    P.kt = nullptr;
+    P.kt_cap = P.kt_used = 0;
+    P.use_graph = graph;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 int solve_pcg(Problem &P, const double *d_rhs, double *d_x, const SolveOpts &o, SolveOut *out) {

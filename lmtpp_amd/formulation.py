@@ -373,6 +373,14 @@ class Formulation:
                 "time_matrix_free")
         return b.value, p.value
 
+    def time_iteration(self, precision=A.MIXED, iters=50):
+        """(element-block kernel ms, vector update ms) of the two-kernel CG iteration (block
+        Jacobi, matrix-free product), averaged over iters iterations (femgpu_time_iteration)."""
+        b, u = C.c_double(), C.c_double()
+        A.check(self._lib.femgpu_time_iteration(self._h, int(precision), int(iters), C.byref(b), C.byref(u)),
+                "time_iteration")
+        return b.value, u.value
+
     def autotune(self):
         """Pick the fastest block-SpMV variant (storage format, SELL slots per loop trip) for this
         matrix in FP64 and FP32; returns spmv_config()."""

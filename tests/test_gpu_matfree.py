@@ -237,6 +237,31 @@ def test_solutions_match_oracle(oracle, precond, geometry):
     f.close()
 
 
+def test_time_iteration_keeps_the_result(oracle):
+    """femgpu_time_iteration times the two kernels of the CG iteration in a solve of its own (into
+    the scratch vector): positive times, the problem's result and its next solve unchanged, and a
+    refusal where the iteration is not the two-kernel one (the stored product)."""
+    say("time_iteration")
+    c, e, E = _mesh((6, 5, 5), 3)
+    f, (rp, col, vals, F) = _system(oracle, c, e, E, False)
+    f.set_spmv_config(fp64=MF, fp32=MF)
+    f.precond, f.precision = L.BLOCK_JACOBI, L.MIXED
+    f.set_result(np.zeros(3 * len(c)))
+    assert f.solve_system(rel_tol=1e-10)
+    u0, it0 = f.get_result(), f.last_stats.iters
+    for prec in (L.MIXED, L.FP64):
+        tb, tu = f.time_iteration(prec, 20)
+        assert tb > 0 and tu > 0, (prec, tb, tu)
+    assert np.array_equal(f.get_result(), u0)
+    f.set_result(np.zeros(3 * len(c)))
+    assert f.solve_system(rel_tol=1e-10)
+    assert np.array_equal(f.get_result(), u0) and f.last_stats.iters == it0
+    f.set_spmv_config(fp64=FULL, fp32=FULL)
+    with pytest.raises(L.FemGpuError):
+        f.time_iteration(L.MIXED, 20)
+    f.close()
+
+
 def test_refused_where_it_cannot_serve():
     say("matrix-free refused")
     c, e = L.make_rect(L.TETRA, (4, 4, 4))

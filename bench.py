@@ -640,9 +640,10 @@ def main():
 
     def iter_kernel_roof(fp32):
         # The dominant kernel of a step: the CG iteration's element blocks (the matrix-free product
-        # with the direction update and the previous vector update's end folded in), timed with
-        # HIP events around each launch on the problem's stream (femgpu_time_iteration, 50
-        # iterations after 4 warm-up ones); its algorithmic bytes from mf_block_iter_bytes.
+        # with the direction update and the previous vector update's end folded in), its device
+        # time from femgpu_time_iteration (50 iterations after 4 warm-up ones: the kernels' shares
+        # from HIP events around each launch, the iteration's time from events around its chunks
+        # on the problem's stream); its algorithmic bytes from mf_block_iter_bytes.
         from lmtpp_amd.traffic import mf_block_iter_bytes
         tb, tu = f.time_iteration(L.MIXED if fp32 else L.FP64, 50)
         info = f.matrix_free_info()

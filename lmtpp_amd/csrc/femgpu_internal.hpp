@@ -245,7 +245,8 @@ struct Problem {
     double *d_r = nullptr, *d_p = nullptr, *d_q = nullptr, *d_Minv = nullptr;
     double *d_aux = nullptr;  // adjoint state / scratch vector
     double *d_red = nullptr;  // diff_norm partials
-    float *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_x32 = nullptr, *d_Minv32 = nullptr;
+    float *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_x32 = nullptr;
+    uint16_t *d_Minv32 = nullptr;  // the FP32 iterations' preconditioner values, bfloat16 bits
     double *d_z = nullptr;    // z = M^-1 r of the Jacobi preconditioners (written by the update)
     float *d_z32 = nullptr;
     double *d_partials = nullptr;

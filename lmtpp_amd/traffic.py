@@ -94,10 +94,10 @@ def mf_block_iter_bytes(n_elem, n_nodes, info, fp32):
 def update_iter_bytes(n_nodes, info, fp32, precond=1):
     """Algorithmic bytes of the CG iteration's vector update (k_update_mf): per node x and r read
     and written, the direction read, the preconditioner's values read (3 for Jacobi, 6 for block
-    Jacobi), z written, its partial range (4 B); q read at the nodes of one block and the partials
-    of the shared nodes read instead."""
+    Jacobi; bfloat16 in FP32, FP64 otherwise), z written, its partial range (4 B); q read at the
+    nodes of one block and the partials of the shared nodes read instead."""
     s = 12 if fp32 else 24
-    m = (3 if precond == 0 else 6) * (4 if fp32 else 8)
+    m = (3 if precond == 0 else 6) * (2 if fp32 else 8)
     return (n_nodes * (6 * s + m + 4) - info["shared_nodes"] * s + info["partial_slots"] * s)
 
 

@@ -505,8 +505,21 @@ constexpr uint32_t MF_RZ_PER_LANE = 4;
 // stride of MF_NS values.  (A stride of 4, one 16-byte access per node, was measured: 29.7 us
 // against 29.3 us per FP32 element-block launch at Hexa8 100^3, 49.0 against 46.0 us in FP64.)
 constexpr int MF_NS = 3;
-inline size_t mf_lds_bytes(uint32_t max_local, size_t vsize) {
-    return (size_t)std::max<uint32_t>(1, max_local) * 2 * MF_NS * vsize;
+// Accumulator copies of the Hexa element blocks: sub-colour c adds into copy c % NACC in pass
+// c / NACC, so the 8 sub-colours of a brick of boxes take 8 / NACC passes (and barriers), and the
+// write-out sums the copies in order.  FP32 4 copies (2 passes), FP64 2 (4 passes): 6 and 4
+// workgroups of a Hexa8 brick (405 nodes) per CU fit the LDS.  Tetra_10: 1.
+#ifndef MF_ACC_F32
+#define MF_ACC_F32 4
+#endif
+#ifndef MF_ACC_F64
+#define MF_ACC_F64 2
+#endif
+template <typename T, int K>
+constexpr int mf_nacc() { return K == TETRA10 ? 1 : (sizeof(T) == 4 ? MF_ACC_F32 : MF_ACC_F64); }
+// LDS of a block: its input values and nacc accumulator copies, at a stride of lsl nodes
+inline size_t mf_lds_bytes(uint32_t lsl, size_t vsize, int nacc) {
+    return (size_t)std::max<uint32_t>(1, lsl) * (1 + nacc) * MF_NS * vsize;
 }
 
 // The 3 values of local node i at LDS triple p (stride MF_NS), as vector accesses when the stride is 4.
@@ -547,7 +560,7 @@ __device__ inline void lds_put3(T *p, T a, T b, T c) {
 // direction, in x for even iterations and xb for odd ones), and the block stores p into the other
 // buffer at the nodes it owns and at the shared nodes it holds first (lslot bit 31), so every node
 // is written once and no block reads a value another block has already replaced.
-// LDS: per local node 3 input values and 3 accumulators.
+// LDS: per local node 3 input values and mf_nacc copies of 3 accumulators.
 // (FP64 Tetra_10: capped at 3 waves per SIMD, 168 VGPRs and 40 B of spills, against 184 VGPRs and
 // 2 waves uncapped: 80 against 84 us per product at 384k elements)
 // FIN (with PQ and DIR: the two-kernel CG iteration): the end of the previous vector update is
@@ -563,7 +576,7 @@ template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA, bool FIN = fa
 __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES ? (sizeof(T) == 4 ? MF_WG_F32 : MF_WG_F64) : 1)) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   const uint32_t *__restrict__ bsub,
-                                                  uint32_t n_colors, uint32_t ls, const uint32_t *__restrict__ lnode,
+                                                  uint32_t n_colors, uint32_t ls, uint32_t lsl, const uint32_t *__restrict__ lnode,
                                                   const uint32_t *__restrict__ lslot,
                                                   const uint32_t *__restrict__ pptr, const uint32_t *__restrict__ pidx,
                                                   const uint32_t *__restrict__ psrc, const double *__restrict__ fpen,
@@ -645,8 +658,10 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     const bool odd = DIR && ((S[sb].iter + (fin ? 1 : 0)) & 1);
     const T *xs = odd ? xb : x;
     T *xo = odd ? const_cast<T *>(x) : xb;
-    T *Ps = reinterpret_cast<T *>(mf_lds);  // [ls][MF_NS] the product's input
-    T *acc = Ps + MF_NS * (size_t)ls;       // [ls][MF_NS] summed nodal forces
+    constexpr int NACC = mf_nacc<T, K>();
+    T *Ps = reinterpret_cast<T *>(mf_lds);  // [lsl][MF_NS] the product's input
+    T *acc = Ps + MF_NS * (size_t)lsl;      // NACC x [lsl][MF_NS] summed nodal forces
+    const size_t astr = MF_NS * (size_t)lsl;
     T beta = DIR ? (T)S[sb].beta : (T)0;
     if constexpr (FIN) load_block();
     const uint32_t col = cbits == MF_EMPTY ? MF_EMPTY : cbits & MF_COLOR_BITS;
@@ -657,7 +672,8 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
 #pragma unroll
         for (int d = 0; d < 3; ++d) v[d] = DIR ? dir_value(z[g + d], beta, xs[g + d]) : x[g + d];
         lds_put3(Ps + MF_NS * i, v[0], v[1], v[2]);
-        lds_put3(acc + MF_NS * i, (T)0, (T)0, (T)0);
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) lds_put3(acc + k * astr + MF_NS * i, (T)0, (T)0, (T)0);
     };
     if constexpr (FIN) {
         // the operands of the first nodes' direction update are loaded before the combine of the
@@ -702,7 +718,8 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
                 const uint32_t i = threadIdx.x + r * MF_BLOCK;
                 lds_put3(Ps + MF_NS * i, dir_value(zr[r][0], beta, sr[r][0]), dir_value(zr[r][1], beta, sr[r][1]),
                          dir_value(zr[r][2], beta, sr[r][2]));
-                lds_put3(acc + MF_NS * i, (T)0, (T)0, (T)0);
+#pragma unroll
+                for (int k = 0; k < NACC; ++k) lds_put3(acc + k * astr + MF_NS * i, (T)0, (T)0, (T)0);
             }
     } else {
 #pragma unroll
@@ -740,18 +757,22 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         }
     }
     __syncthreads();
-    // no two lanes of a sub-colour touch one entry, so the order of every sum is the sub-colour
-    // order; plain read-add-write (LDS float atomics measured 2.6x slower here in FP32)
+    // no two lanes of a sub-colour touch one entry, and sub-colour c adds into copy c % NACC in
+    // pass c / NACC, so the order of every sum is fixed: each copy in sub-colour order, the copies
+    // summed in order at the write-out; plain read-add-write (LDS float atomics measured 2.6x
+    // slower here in FP32)
     // (an element's nodes are distinct, so all its entries are read before any is written: one
-    // LDS round trip per sub-colour instead of one per node)
-    for (uint32_t c = 0; c < n_colors; ++c) {
-        if (col == c) {
+    // LDS round trip per pass instead of one per node)
+    const uint32_t n_pass = (n_colors + NACC - 1) / NACC;
+    for (uint32_t c = 0; c < n_pass; ++c) {
+        if (col != MF_EMPTY && col / NACC == c) {
+            T *ac = acc + (col % NACC) * astr;
             T cur[NPE][3];
 #pragma unroll
-            for (int a = 0; a < NPE; ++a) lds_get3(acc + MF_NS * lo[a], cur[a][0], cur[a][1], cur[a][2]);
+            for (int a = 0; a < NPE; ++a) lds_get3(ac + MF_NS * lo[a], cur[a][0], cur[a][1], cur[a][2]);
 #pragma unroll
             for (int a = 0; a < NPE; ++a)
-                lds_put3(acc + MF_NS * lo[a], cur[a][0] + f[a][0], cur[a][1] + f[a][1], cur[a][2] + f[a][2]);
+                lds_put3(ac + MF_NS * lo[a], cur[a][0] + f[a][0], cur[a][1] + f[a][1], cur[a][2] + f[a][2]);
         }
         __syncthreads();
     }
@@ -763,6 +784,14 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)(s & MF_SLOT_BITS);
         T a0, a1, a2;
         lds_get3(acc + MF_NS * i, a0, a1, a2);
+#pragma unroll
+        for (int k = 1; k < NACC; ++k) {
+            T b0, b1, b2;
+            lds_get3(acc + k * astr + MF_NS * i, b0, b1, b2);
+            a0 += b0;
+            a1 += b1;
+            a2 += b2;
+        }
         dst[0] = a0;
         dst[1] = a1;
         dst[2] = a2;
@@ -1092,7 +1121,9 @@ int mf_build(Problem &P) {
     }
     lptr[nb] = (uint32_t)lnode.size();
     const uint32_t ls = (max_local + 63) / 64 * 64;  // fixed stride of the blocks' node lists
-    if (mf_lds_bytes(ls, 8) > 64 * 1024) return 1;  // LDS of the FP64 product's blocks
+    // LDS of the FP64 product's blocks
+    if (mf_lds_bytes(max_local, 8, P.kind == TETRA10 ? mf_nacc<double, TETRA10>() : mf_nacc<double, HEXA>()) > 64 * 1024)
+        return 1;
     // shared nodes: a partial slot per (node, block), grouped by node in block order
     std::vector<uint32_t> nshared(nn, 0);
     for (size_t l = 0; l < lnode.size(); ++l)
@@ -1291,8 +1322,8 @@ static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y, T *
     const FinParts F = fin ? *fin : FinParts{nullptr, 0u, nullptr, nullptr};
 #define FEMGPU_MF_BLOCK(KIND)                                                                                         \
     hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND, FIN>), dim3(M.n_blocks), dim3(MF_BLOCK),   \
-                       mf_lds_bytes(M.ls, sizeof(T)), P.stream, M.d_conn, M.d_loc, M.d_color, M.d_elem, M.d_bsub,      \
-                       M.n_colors, M.ls, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
+                       mf_lds_bytes(M.max_local, sizeof(T), mf_nacc<T, KIND>()), P.stream, M.d_conn, M.d_loc,          \
+                       M.d_color, M.d_elem, M.d_bsub, M.n_colors, M.ls, M.max_local, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
                        P.d_coords, M.d_E, M.d_mat, rec, x, xb, z, y, halo, P.d_ps,                                      \
                        M.d_partials, P.d_ps + P.n_sub, F.part, F.stride, F.w0, F.nw, M.d_b0)
     if (P.kind == TETRA10) FEMGPU_MF_BLOCK(TETRA10);

@@ -74,38 +74,43 @@ __device__ void wg_partial_n(double (&v)[NV], double *partials, unsigned stride,
 }
 
 // One workgroup (up to 1024 lanes) combines the partials of workgroups [w0, w0 + nw) in a fixed
-// order: each round issues the loads of all NV values for four strided elements per lane at once
+// order: each round issues the loads of all NV values for PER strided elements per lane at once
 // (one memory round trip per round: a finalize after a row kernel of 1024 x 4 workgroups needs
-// one), then a wave tree and a fixed combination of the waves.  out[] is valid in every thread.
-template <int NV, unsigned MAXMASK>
+// one), then a pairwise tree over the PER accumulators, a wave tree and a fixed combination of
+// the waves.  out[] is valid in every thread.
+template <int NV, unsigned MAXMASK, int PER = 4>
 __device__ void wg_combine(const double *partials, unsigned stride, unsigned w0, unsigned nw, double (&out)[NV]) {
+    static_assert(PER >= 1 && (PER & (PER - 1)) == 0, "PER: a power of two");
     __shared__ double sm[NV][16];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n_waves = (blockDim.x + 63) >> 6;
     const unsigned B = blockDim.x;
-    double a[NV][4];
+    double a[NV][PER];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) a[k][j] = 0.0;
-    for (unsigned i0 = threadIdx.x; i0 < nw; i0 += 4 * B) {
-        double t[NV][4];
+        for (int j = 0; j < PER; ++j) a[k][j] = 0.0;
+    for (unsigned i0 = threadIdx.x; i0 < nw; i0 += PER * B) {
+        double t[NV][PER];
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < PER; ++j) {
                 const unsigned i = i0 + j * B;
                 t[k][j] = i < nw ? partials[(size_t)k * stride + w0 + i] : 0.0;
             }
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) a[k][j] = (MAXMASK >> k & 1) ? fmax(a[k][j], t[k][j]) : a[k][j] + t[k][j];
+            for (int j = 0; j < PER; ++j) a[k][j] = (MAXMASK >> k & 1) ? fmax(a[k][j], t[k][j]) : a[k][j] + t[k][j];
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const double v = (MAXMASK >> k & 1) ? fmax(fmax(a[k][0], a[k][1]), fmax(a[k][2], a[k][3]))
-                                            : (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
-        const double t = (MAXMASK >> k & 1) ? wave_max(v) : wave_sum(v);
+#pragma unroll
+        for (int h = 1; h < PER; h *= 2)
+#pragma unroll
+            for (int j = 0; j + h < PER; j += 2 * h)
+                a[k][j] = (MAXMASK >> k & 1) ? fmax(a[k][j], a[k][j + h]) : a[k][j] + a[k][j + h];
+        const double t = (MAXMASK >> k & 1) ? wave_max(a[k][0]) : wave_sum(a[k][0]);
         if (lane == 0) sm[k][wave] = t;
     }
     __syncthreads();

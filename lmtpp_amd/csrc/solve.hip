@@ -199,6 +199,16 @@ __global__ void k_to_sym(const double *__restrict__ K, uint64_t ps, const uint32
 // 2: block IC(0), whose z the level sweeps write (not applied here), 3: the Chebyshev polynomial
 // z = c0 D r + c1 D K D r in the block-Jacobi D (D r here; k_cheb adds the second term).
 enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2, PRE_CHEB = 3 };
+// FP32 vector update of the two-kernel CG iteration: lanes per 1024-row workgroup (1024 / UPD_NT32
+// rows each) and p.q partials loaded per lane in one round of the combine (Hexa8 100^3, us per
+// CG iteration: 256 lanes and 4 partials 55.6, 8 partials 55.6, 16 60.3; 512 lanes and 4
+// partials 55.1, 8 partials 58.0; 1024 lanes 59.3)
+#ifndef UPD_NT32
+#define UPD_NT32 256
+#endif
+#ifndef UPD_CPL32
+#define UPD_CPL32 4
+#endif
 // the block-Jacobi factors of D serve both
 template <int PRE>
 constexpr int jacobi_kind() { return PRE == PRE_CHEB ? PRE_BLOCK_JACOBI : PRE; }
@@ -340,7 +350,7 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, T *__restrict
 // workgroups: a quarter of the partials for every element block to combine, and a quarter of the
 // workgroups combining the blocks' p.q partials.
 template <int PRE, typename T, bool FOLD>
-__global__ __launch_bounds__(FOLD && sizeof(T) == 8 ? 1024 : 256, FOLD && sizeof(T) == 8 ? 8 : 1) void k_update_mf(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
+__global__ __launch_bounds__(FOLD ? (sizeof(T) == 8 ? 1024 : UPD_NT32) : 256, FOLD && sizeof(T) == 8 ? 8 : 1) void k_update_mf(T *__restrict__ x, T *__restrict__ r, const T *p, const T *p_alt,
                                                    const T *__restrict__ q, const T *__restrict__ halo,
                                                    const uint32_t *__restrict__ hbeg, const double *__restrict__ mfpart,
                                                    const uint32_t *__restrict__ mf_w0, const uint32_t *__restrict__ mf_nw,
@@ -410,8 +420,10 @@ __global__ __launch_bounds__(FOLD && sizeof(T) == 8 ? 1024 : 256, FOLD && sizeof
     // FOLD: 1024 rows per workgroup, as 1024 lanes of one row (FP64) or 256 lanes of 4 rows,
     // rows row0 + lane + 256 j (FP32); measured at Hexa8 100^3: FP64 45.9 against 51.4 us, FP32
     // 24.1 against 25.5 us
-    constexpr int NT = FOLD && sizeof(T) == 8 ? 1024 : 256;
+    constexpr int NT = FOLD ? (sizeof(T) == 8 ? 1024 : UPD_NT32) : 256;
     constexpr int RPL = FOLD ? 1024 / NT : 1;
+    // FP32: the p.q partials (one per element block) in one round of loads per lane
+    constexpr int CPL = FOLD && sizeof(T) == 4 ? UPD_CPL32 : 4;
     if constexpr (RPL == 1) {
         // FP64: the node's operands are loaded before the combine of the p.q partials (which alpha
         // needs), so that their latency overlaps it (48.2 against 50.1 us per launch at Hexa8
@@ -434,21 +446,22 @@ __global__ __launch_bounds__(FOLD && sizeof(T) == 8 ? 1024 : 256, FOLD && sizeof
         T qn[RPL][3], xv[RPL][3], rv[RPL][3], pv[RPL][3], mv[RPL][6];
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
-            const uint32_t n = wt.row0[L] + threadIdx.x + 256u * j;
+            const uint32_t n = wt.row0[L] + threadIdx.x + (uint32_t)NT * j;
             if (n < wt.row1[L]) load(n, qn[j], xv[j], rv[j], pv[j], mv[j]);
         }
-        wg_combine<1, 0u>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
+        wg_combine<1, 0u, CPL>(mfpart, 0u, mf_w0[s], mf_nw[s], o);
         record(o[0]);
         if (!finite(o[0]) || !(o[0] > 0)) return;
         const T alpha = (T)(S[s].rho / o[0]);
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
-            const uint32_t n = wt.row0[L] + threadIdx.x + 256u * j;
+            const uint32_t n = wt.row0[L] + threadIdx.x + (uint32_t)NT * j;
             if (n < wt.row1[L]) update(n, alpha, qn[j], xv[j], rv[j], pv[j], mv[j], acc);
         }
     }
     if (PRE != PRE_CHEB) {
-        if (NT == 1024) wg_partial_n<3, 4u, 16>(acc, partials, gridDim.x, L);
+        if constexpr (NT == 1024) wg_partial_n<3, 4u, 16>(acc, partials, gridDim.x, L);
+        else if constexpr (NT == 512) wg_partial_n<3, 4u, 8>(acc, partials, gridDim.x, L);
         else wg_partial<3, 4u>(acc, partials, gridDim.x, L);
     }
 }
@@ -1162,7 +1175,7 @@ static void launch_iteration_t(Problem &P, const T *K, T *x, T *r, T *p, T *q, c
             // the product's shared-node pass and alpha are folded into the update
             const T *p_alt = reinterpret_cast<const T *>(sizeof(T) == 4 ? (void *)P.mf.d_pb32 : (void *)P.mf.d_pb);
             if (fold)
-                hipLaunchKernelGGL((k_update_mf<PRE, T, true>), dim3(P.n_fwg), dim3(sizeof(T) == 8 ? 1024 : 256), 0, P.stream, x, r, p, p_alt, q,
+                hipLaunchKernelGGL((k_update_mf<PRE, T, true>), dim3(P.n_fwg), dim3(sizeof(T) == 8 ? 1024 : UPD_NT32), 0, P.stream, x, r, p, p_alt, q,
                                    reinterpret_cast<const T *>(P.mf.d_halo), P.mf.d_hbeg, P.mf.d_partials, P.mf.d_wg0,
                                    P.mf.d_nwg, Minv, z, tables_fat(P), P.d_ps + P.n_sub, P.d_ps, P.d_partials);
             else

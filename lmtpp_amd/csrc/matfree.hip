@@ -756,7 +756,8 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
             else hexa_apply<T>(X, u, (T)(Hd[0] - Hd[1]), (T)Hd[1], (T)(0.5 * Hd[2]), f);
         }
     }
-    __syncthreads();
+    // (no barrier here: the element loop reads only the inputs, the passes only the accumulators,
+    // which the staging zeroed before the barrier above)
     // no two lanes of a sub-colour touch one entry, and sub-colour c adds into copy c % NACC in
     // pass c / NACC, so the order of every sum is fixed: each copy in sub-colour order, the copies
     // summed in order at the write-out; plain read-add-write (LDS float atomics measured 2.6x
@@ -776,9 +777,12 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         }
         __syncthreads();
     }
-    // penalty terms of the constrained DOFs assigned to this block (distinct local entries)
-    for (uint32_t k = pk0 + threadIdx.x; k < pk1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * Ps[pidx[k]];
-    __syncthreads();
+    // penalty terms of the constrained DOFs assigned to this block (distinct local entries); the
+    // range is the block's, so a block without any skips the barrier
+    if (pk1 > pk0) {
+        for (uint32_t k = pk0 + threadIdx.x; k < pk1; k += MF_BLOCK) acc[pidx[k]] += (T)fpen[psrc[k]] * Ps[pidx[k]];
+        __syncthreads();
+    }
     auto write_out = [&](uint32_t i, uint32_t s, uint32_t g) {
         const bool own = s == 0xffffffffu;
         T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)(s & MF_SLOT_BITS);

@@ -1464,8 +1464,8 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
     // others run); without graphs the first chunks double from 2
     const bool predict = P.n_sub == 1;
     const int cap = PREDICTED_CHUNKS * full;
-    double rr_prev = 0, rate = 0;
-    long it_prev = -1;
+    double rr_prev = 0, rate = 0, rr_cyc = 0;
+    long it_prev = -1, it_cyc = -1;
     const long max_chunks = (long)o.max_iter / 2 + 1000;
     // chunk timing: events 4-6 as a ring (the previous chunk's end stays for the gap)
     P.last_chunks = P.last_gaps = 0;
@@ -1497,6 +1497,17 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             const ProbScalars &S = P.h_ps[0];
             if (it_prev >= 0 && S.iter > it_prev && S.rr > 0 && rr_prev > S.rr)
                 rate = std::log(S.rr / rr_prev) / (double)(S.iter - it_prev);
+            // the residual of CG falls in steps: a chunk that saw only a plateau predicts a long
+            // way to the pause, so the faster of its rate and the mean rate since the cycle's
+            // first chunk (the solve's start or the last refresh) sets the prediction (Hexa8
+            // 100^3 MIXED: 433 launches past a pause with the chunk's rate alone)
+            if (it_cyc < 0) {
+                it_cyc = S.iter;
+                rr_cyc = S.rr;
+            } else if (S.iter - it_cyc >= 8 && S.rr > 0 && rr_cyc > S.rr) {
+                const double mean = std::log(S.rr / rr_cyc) / (double)(S.iter - it_cyc);
+                if (!(rate < mean)) rate = mean;
+            }
             chunk = predict_chunk(S, S.rr, S.rr_ref, rate, grown, cap);
             rr_prev = S.rr;
             it_prev = S.iter;
@@ -1524,6 +1535,7 @@ static int solve_impl(Problem &P, const double *d_rhs, double *d_x, const SolveO
             else launch_restart<PRE, double>(P, P.d_q, P.d_r, P.d_p, P.d_Minv, zbuf<PRE, double>(P), 0);
             hipLaunchKernelGGL(k_clear_pause, dim3((P.n_sub + 255) / 256), blk, 0, P.stream, P.d_ps, P.n_sub);
             it_prev = -1;  // the recurrence restarts from the FP64 residual
+            it_cyc = -1;
             chunk = P.use_graph ? full : std::max(chunk, 2);
             // the restarted residual is the true one: the next pause comes after about the
             // iterations the last cycle took; half of that, so that a faster start wastes nothing

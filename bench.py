@@ -627,7 +627,7 @@ def main():
         r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(gbs / HBM_PEAK_GBS, 4),
              "traffic": load_traffic("mf_block_" + ("fp32" if fp32 else "fp64")),
-             "kernel": "k_mf_block<%s,false,false,%s,2> (%s matrix-free Hexa product, element blocks)" % (
+             "kernel": "k_mf_block<%s,false,false,%s,2,false,true> (%s matrix-free Hexa product, element blocks)" % (
                  "float" if fp32 else "double", "true" if info["box_elements"] == sz["n_elem"] else "false",
                  "FP32" if fp32 else "FP64"),
              "avg_launch_ms": round(tb, 4), "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
@@ -656,7 +656,7 @@ def main():
         r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(gbs / HBM_PEAK_GBS, 4),
              "traffic": load_traffic("mf_block_iter_" + ("fp32" if fp32 else "fp64")),
-             "kernel": "k_mf_block<%s,true,true,%s,2,true> (%s CG iteration: matrix-free Hexa product with the "
+             "kernel": "k_mf_block<%s,true,true,%s,2,true,true> (%s CG iteration: matrix-free Hexa product with the "
                        "direction update and the previous vector update's end folded in)" % (
                            "float" if fp32 else "double", boxes, "FP32" if fp32 else "FP64"),
              "avg_launch_ms": round(tb, 4), "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
@@ -692,7 +692,7 @@ def main():
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "us_per_iteration": round(us, 2),
                 "algorithmic_bytes": {"element_blocks": kb, "vector_update": ub},
                 "traffic": (t1 + t2) if t1 and t2 else None,
-                "kernels": "k_mf_block<%s,true,true,true,2,true> + k_update_mf<1,%s,true>" % (
+                "kernels": "k_mf_block<%s,true,true,true,2,true,true> + k_update_mf<1,%s,true>" % (
                     ("float", "float") if fp32 else ("double", "double")),
                 "iters": int(st.iters), "chunks": int(nch), "chunk_ms": round(cms, 2), "host_gap_ms": round(gms, 3)}
 

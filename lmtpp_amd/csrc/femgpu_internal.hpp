@@ -120,6 +120,9 @@ struct MatFree {
     uint8_t *d_color = nullptr;      // sub-colour per slot | MF_BOX for axis-aligned boxes (MF_EMPTY: empty slot)
     uint32_t *d_elem = nullptr;      // element per slot (0 for empty slots)
     uint32_t *d_lptr = nullptr;      // block b: local nodes [lptr[b], lptr[b + 1])
+    uint32_t *d_bls = nullptr;       // block b: local node count | log2 accumulator copies << 24 (FP32), << 28 (FP64)
+    size_t lds32 = 0, lds64 = 0;     // LDS of the element-block launches (the largest block's)
+    bool uni32 = false, uni64 = false;  // every block takes the most accumulator copies
     uint32_t ls = 0;                 // stride of the per-block node lists (max_local rounded up to 64)
     // block b, local node l < its count (lptr[b + 1] - lptr[b]): entry b * ls + l holds the global
     // node (MF_NO_NODE past the count) and its partial slot when shared with another block (~0 if not)

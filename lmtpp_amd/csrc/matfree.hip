@@ -505,21 +505,37 @@ constexpr uint32_t MF_RZ_PER_LANE = 4;
 // stride of MF_NS values.  (A stride of 4, one 16-byte access per node, was measured: 29.7 us
 // against 29.3 us per FP32 element-block launch at Hexa8 100^3, 49.0 against 46.0 us in FP64.)
 constexpr int MF_NS = 3;
-// Accumulator copies of the Hexa element blocks: sub-colour c adds into copy c % NACC in pass
-// c / NACC, so the 8 sub-colours of a brick of boxes take 8 / NACC passes (and barriers), and the
-// write-out sums the copies in order.  FP32 4 copies (2 passes), FP64 2 (4 passes): 6 and 4
-// workgroups of a Hexa8 brick (405 nodes) per CU fit the LDS.  Tetra_10: 1.
+// Accumulator copies of the Hexa element blocks: sub-colour c adds into copy c % n in pass c / n,
+// so the 8 sub-colours of a brick of boxes take 8 / n passes (and barriers), and the write-out
+// sums the copies in order.  n is the block's own: the most copies, up to MF_ACC_F32 (FP32) or
+// MF_ACC_F64 (FP64), whose LDS (input and copies, 3 values of T per local node) stays within the
+// share of one of the MF_WG_F32 / MF_WG_F64 workgroups per CU (FP32: 4 copies up to 455 local
+// nodes, a Hexa8 brick has 405; FP64: 2 up to 568).  A function of the block alone, so a
+// sub-problem sums alike alone and in a batch; the launch's LDS is the largest block's.  (A fixed
+// 4 copies sized by the largest block of a batch of uneven cubes left 3-4 workgroups per CU:
+// 153 against 138 ms for the batched solve.)  Tetra_10: 1.
 #ifndef MF_ACC_F32
 #define MF_ACC_F32 4
 #endif
 #ifndef MF_ACC_F64
 #define MF_ACC_F64 2
 #endif
+constexpr uint32_t mf_ilog2(int v) { return v > 1 ? 1 + mf_ilog2(v / 2) : 0; }
+// log2 of the copies of a block whose LDS fits (every block of a lattice): UNI launches
 template <typename T, int K>
-constexpr int mf_nacc() { return K == TETRA10 ? 1 : (sizeof(T) == 4 ? MF_ACC_F32 : MF_ACC_F64); }
-// LDS of a block: its input values and nacc accumulator copies, at a stride of lsl nodes
-inline size_t mf_lds_bytes(uint32_t lsl, size_t vsize, int nacc) {
-    return (size_t)std::max<uint32_t>(1, lsl) * (1 + nacc) * MF_NS * vsize;
+constexpr uint32_t mf_acc_log2_max() { return K == TETRA10 ? 0 : mf_ilog2(sizeof(T) == 4 ? MF_ACC_F32 : MF_ACC_F64); }
+// log2 of block copies for a block of m local nodes (T's size vsize)
+inline uint32_t mf_acc_log2(uint32_t m, size_t vsize, int kind) {
+    if (kind == TETRA10) return 0;
+    const size_t budget = (size_t)160 * 1024 / (vsize == 4 ? MF_WG_F32 : MF_WG_F64);
+    const int cap = vsize == 4 ? MF_ACC_F32 : MF_ACC_F64;
+    uint32_t lg = 0;
+    while ((2 << lg) <= cap && (size_t)std::max<uint32_t>(1, m) * (1 + (2 << lg)) * MF_NS * vsize <= budget) ++lg;
+    return lg;
+}
+// LDS of a block: its input values and 2^lg accumulator copies, at a stride of m nodes
+inline size_t mf_lds_bytes(uint32_t m, size_t vsize, uint32_t lg) {
+    return (size_t)std::max<uint32_t>(1, m) * (1 + (1u << lg)) * MF_NS * vsize;
 }
 
 // The 3 values of local node i at LDS triple p (stride MF_NS), as vector accesses when the stride is 4.
@@ -560,7 +576,7 @@ __device__ inline void lds_put3(T *p, T a, T b, T c) {
 // direction, in x for even iterations and xb for odd ones), and the block stores p into the other
 // buffer at the nodes it owns and at the shared nodes it holds first (lslot bit 31), so every node
 // is written once and no block reads a value another block has already replaced.
-// LDS: per local node 3 input values and mf_nacc copies of 3 accumulators.
+// LDS: per local node 3 input values and the block's copies of 3 accumulators (mf_acc_log2).
 // (FP64 Tetra_10: capped at 3 waves per SIMD, 168 VGPRs and 40 B of spills, against 184 VGPRs and
 // 2 waves uncapped: 80 against 84 us per product at 384k elements)
 // FIN (with PQ and DIR: the two-kernel CG iteration): the end of the previous vector update is
@@ -572,11 +588,15 @@ __device__ inline void lds_put3(T *p, T a, T b, T c) {
 // copies S into S1 unchanged.  A stop or pause decided here takes effect from the vector update
 // on: this launch still forms the next direction into the other buffer and its product, which
 // nothing reads, and leaves the stored direction as it was.
-template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA, bool FIN = false>
+// UNI: every block of the launch takes the most copies (the arithmetic of the per-block choice,
+// with the copy count a compile-time constant; Hexa8 100^3 MIXED 85.8-86.4 against 87.1-87.8 ms
+// read from the block's word).
+template <typename T, bool PQ, bool DIR, bool BOXES, int K = HEXA, bool FIN = false, bool UNI = false>
 __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES ? (sizeof(T) == 4 ? MF_WG_F32 : MF_WG_F64) : 1)) void k_mf_block(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ loc,
                                                   const uint8_t *__restrict__ color, const uint32_t *__restrict__ elem,
                                                   const uint32_t *__restrict__ bsub,
-                                                  uint32_t n_colors, uint32_t ls, uint32_t lsl, const uint32_t *__restrict__ lnode,
+                                                  uint32_t n_colors, uint32_t ls, const uint32_t *__restrict__ bls,
+                                                  uint32_t lsl_all, const uint32_t *__restrict__ lnode,
                                                   const uint32_t *__restrict__ lslot,
                                                   const uint32_t *__restrict__ pptr, const uint32_t *__restrict__ pidx,
                                                   const uint32_t *__restrict__ psrc, const double *__restrict__ fpen,
@@ -658,9 +678,19 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     const bool odd = DIR && ((S[sb].iter + (fin ? 1 : 0)) & 1);
     const T *xs = odd ? xb : x;
     T *xo = odd ? const_cast<T *>(x) : xb;
-    constexpr int NACC = mf_nacc<T, K>();
+    // the block's local node count and accumulator copies (2^lg: bits 24-27 FP32, 28-31 FP64)
+    uint32_t lsl, lg;
+    if constexpr (UNI) {
+        lsl = lsl_all;
+        lg = mf_acc_log2_max<T, K>();
+    } else {
+        const uint32_t bw = bls[b];
+        lsl = bw & 0xffffffu;
+        lg = (bw >> (sizeof(T) == 4 ? 24 : 28)) & 0xfu;
+    }
+    const int nacc = 1 << lg;
     T *Ps = reinterpret_cast<T *>(mf_lds);  // [lsl][MF_NS] the product's input
-    T *acc = Ps + MF_NS * (size_t)lsl;      // NACC x [lsl][MF_NS] summed nodal forces
+    T *acc = Ps + MF_NS * (size_t)lsl;      // nacc x [lsl][MF_NS] summed nodal forces
     const size_t astr = MF_NS * (size_t)lsl;
     T beta = DIR ? (T)S[sb].beta : (T)0;
     if constexpr (FIN) load_block();
@@ -672,8 +702,7 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
 #pragma unroll
         for (int d = 0; d < 3; ++d) v[d] = DIR ? dir_value(z[g + d], beta, xs[g + d]) : x[g + d];
         lds_put3(Ps + MF_NS * i, v[0], v[1], v[2]);
-#pragma unroll
-        for (int k = 0; k < NACC; ++k) lds_put3(acc + k * astr + MF_NS * i, (T)0, (T)0, (T)0);
+        for (int k = 0; k < nacc; ++k) lds_put3(acc + k * astr + MF_NS * i, (T)0, (T)0, (T)0);
     };
     if constexpr (FIN) {
         // the operands of the first nodes' direction update are loaded before the combine of the
@@ -718,8 +747,7 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
                 const uint32_t i = threadIdx.x + r * MF_BLOCK;
                 lds_put3(Ps + MF_NS * i, dir_value(zr[r][0], beta, sr[r][0]), dir_value(zr[r][1], beta, sr[r][1]),
                          dir_value(zr[r][2], beta, sr[r][2]));
-#pragma unroll
-                for (int k = 0; k < NACC; ++k) lds_put3(acc + k * astr + MF_NS * i, (T)0, (T)0, (T)0);
+                for (int k = 0; k < nacc; ++k) lds_put3(acc + k * astr + MF_NS * i, (T)0, (T)0, (T)0);
             }
     } else {
 #pragma unroll
@@ -758,16 +786,16 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
     }
     // (no barrier here: the element loop reads only the inputs, the passes only the accumulators,
     // which the staging zeroed before the barrier above)
-    // no two lanes of a sub-colour touch one entry, and sub-colour c adds into copy c % NACC in
-    // pass c / NACC, so the order of every sum is fixed: each copy in sub-colour order, the copies
+    // no two lanes of a sub-colour touch one entry, and sub-colour c adds into copy c % nacc in
+    // pass c / nacc, so the order of every sum is fixed: each copy in sub-colour order, the copies
     // summed in order at the write-out; plain read-add-write (LDS float atomics measured 2.6x
     // slower here in FP32)
     // (an element's nodes are distinct, so all its entries are read before any is written: one
     // LDS round trip per pass instead of one per node)
-    const uint32_t n_pass = (n_colors + NACC - 1) / NACC;
+    const uint32_t n_pass = (n_colors + nacc - 1) >> lg;
     for (uint32_t c = 0; c < n_pass; ++c) {
-        if (col != MF_EMPTY && col / NACC == c) {
-            T *ac = acc + (col % NACC) * astr;
+        if (col != MF_EMPTY && (col >> lg) == c) {
+            T *ac = acc + (col & (nacc - 1)) * astr;
             T cur[NPE][3];
 #pragma unroll
             for (int a = 0; a < NPE; ++a) lds_get3(ac + MF_NS * lo[a], cur[a][0], cur[a][1], cur[a][2]);
@@ -788,8 +816,7 @@ __global__ __launch_bounds__(256, (K == TETRA10 && sizeof(T) == 8) ? 3 : (BOXES 
         T *dst = own ? y + 3 * (size_t)g : halo + 3 * (size_t)(s & MF_SLOT_BITS);
         T a0, a1, a2;
         lds_get3(acc + MF_NS * i, a0, a1, a2);
-#pragma unroll
-        for (int k = 1; k < NACC; ++k) {
+        for (int k = 1; k < nacc; ++k) {
             T b0, b1, b2;
             lds_get3(acc + k * astr + MF_NS * i, b0, b1, b2);
             a0 += b0;
@@ -886,7 +913,7 @@ bool mf_eligible(const Problem &P) {
 }
 
 void mf_free(MatFree &M) {
-    dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr);
+    dfree(M.d_conn); dfree(M.d_loc); dfree(M.d_color); dfree(M.d_elem); dfree(M.d_lptr); dfree(M.d_bls);
     dfree(M.d_lnode_pad); dfree(M.d_lslot_pad); dfree(M.d_bnode); dfree(M.d_bptr); dfree(M.d_halo); dfree(M.d_idof);
     dfree(M.d_isrc);
     dfree(M.d_pptr); dfree(M.d_pidx); dfree(M.d_psrc); dfree(M.d_E); dfree(M.d_mat);
@@ -1125,9 +1152,21 @@ int mf_build(Problem &P) {
     }
     lptr[nb] = (uint32_t)lnode.size();
     const uint32_t ls = (max_local + 63) / 64 * 64;  // fixed stride of the blocks' node lists
-    // LDS of the FP64 product's blocks
-    if (mf_lds_bytes(max_local, 8, P.kind == TETRA10 ? mf_nacc<double, TETRA10>() : mf_nacc<double, HEXA>()) > 64 * 1024)
-        return 1;
+    // per block its local node count and accumulator copies; the launches' LDS, the largest block's
+    std::vector<uint32_t> bls(nb);
+    size_t lds32 = 0, lds64 = 0;
+    const uint32_t gmax32 = P.kind == TETRA10 ? mf_acc_log2_max<float, TETRA10>() : mf_acc_log2_max<float, HEXA>();
+    const uint32_t gmax64 = P.kind == TETRA10 ? mf_acc_log2_max<double, TETRA10>() : mf_acc_log2_max<double, HEXA>();
+    bool uni32 = true, uni64 = true;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t m = lptr[b + 1] - lptr[b], g32 = mf_acc_log2(m, 4, P.kind), g64 = mf_acc_log2(m, 8, P.kind);
+        bls[b] = m | g32 << 24 | g64 << 28;
+        lds32 = std::max(lds32, mf_lds_bytes(m, 4, g32));
+        lds64 = std::max(lds64, mf_lds_bytes(m, 8, g64));
+        uni32 &= g32 == gmax32;
+        uni64 &= g64 == gmax64;
+    }
+    if (lds64 > 64 * 1024) return 1;  // LDS of the FP64 product's blocks
     // shared nodes: a partial slot per (node, block), grouped by node in block order
     std::vector<uint32_t> nshared(nn, 0);
     for (size_t l = 0; l < lnode.size(); ++l)
@@ -1162,7 +1201,7 @@ int mf_build(Problem &P) {
         }
     const std::vector<uint32_t> zero(1, 0);
     if (!upload_vec(&M.d_conn, sconn) || !upload_vec(&M.d_loc, sloc) || !upload_vec(&M.d_color, scol) ||
-        !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_lnode_pad, lnode_pad) ||
+        !upload_vec(&M.d_elem, selem) || !upload_vec(&M.d_lptr, lptr) || !upload_vec(&M.d_bls, bls) || !upload_vec(&M.d_lnode_pad, lnode_pad) ||
         !upload_vec(&M.d_lslot_pad, lslot_pad) || !upload_vec(&M.d_hbeg, hbeg) || !upload_vec(&M.d_bnode, bnode) || !upload_vec(&M.d_bptr, bptr) ||
         !upload_vec(&M.d_idof, idof) || !upload_vec(&M.d_isrc, std::vector<uint32_t>(idof.size(), 0xffffffffu)) ||
         !upload_vec(&M.d_bsub, bsub) || !upload_vec(&M.d_wg0, std::vector<uint32_t>(P.n_sub, 0)) ||
@@ -1180,6 +1219,10 @@ int mf_build(Problem &P) {
     M.n_boxes = (uint32_t)std::count_if(scol.begin(), scol.end(), [](uint8_t c) { return c != MF_EMPTY && (c & MF_BOX); });
     M.all_boxes = M.n_boxes == ne;
     M.max_local = max_local;
+    M.lds32 = lds32;
+    M.lds64 = lds64;
+    M.uni32 = uni32;
+    M.uni64 = uni64;
     M.ls = ls;
     M.n_bnodes = (uint32_t)bnode.size();
     M.n_halo = bptr.back();
@@ -1324,14 +1367,15 @@ static void mf_block_launch(Problem &P, const T *x, T *xb, const T *z, T *y, T *
     const MatFree &M = P.mf;
     const void *rec = sizeof(T) == 4 ? M.d_rec32 : M.d_rec64;
     const FinParts F = fin ? *fin : FinParts{nullptr, 0u, nullptr, nullptr};
-#define FEMGPU_MF_BLOCK(KIND)                                                                                         \
-    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND, FIN>), dim3(M.n_blocks), dim3(MF_BLOCK),   \
-                       mf_lds_bytes(M.max_local, sizeof(T), mf_nacc<T, KIND>()), P.stream, M.d_conn, M.d_loc,          \
-                       M.d_color, M.d_elem, M.d_bsub, M.n_colors, M.ls, M.max_local, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
+#define FEMGPU_MF_BLOCK(KIND, UNI)                                                                                    \
+    hipLaunchKernelGGL((k_mf_block<T, PQ, DIR, BOXES && KIND == HEXA, KIND, FIN, UNI>), dim3(M.n_blocks), dim3(MF_BLOCK), \
+                       sizeof(T) == 4 ? M.lds32 : M.lds64, P.stream, M.d_conn, M.d_loc,                                 \
+                       M.d_color, M.d_elem, M.d_bsub, M.n_colors, M.ls, M.d_bls, M.max_local, M.d_lnode_pad, M.d_lslot_pad, M.d_pptr, M.d_pidx, M.d_psrc, M.d_fpen,         \
                        P.d_coords, M.d_E, M.d_mat, rec, x, xb, z, y, halo, P.d_ps,                                      \
                        M.d_partials, P.d_ps + P.n_sub, F.part, F.stride, F.w0, F.nw, M.d_b0)
-    if (P.kind == TETRA10) FEMGPU_MF_BLOCK(TETRA10);
-    else FEMGPU_MF_BLOCK(HEXA);
+    if (P.kind == TETRA10) FEMGPU_MF_BLOCK(TETRA10, true);
+    else if (sizeof(T) == 4 ? M.uni32 : M.uni64) FEMGPU_MF_BLOCK(HEXA, true);
+    else FEMGPU_MF_BLOCK(HEXA, false);
 #undef FEMGPU_MF_BLOCK
 }
 

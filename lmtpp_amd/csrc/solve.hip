@@ -1307,8 +1307,9 @@ static int run_chunk(Problem &P, bool fp32, double *d_x, int n, int full) {
 // recurrence restarts on the same operator); rounded up to even, within [2, cap]; dflt without a
 // rate.  A chunk that ends at the pause saves the iterations that would run after it as no-ops
 // (every kernel of a paused problem returns at once) until the chunk's end, and a long one the
-// host's polls in between.
-constexpr int PREDICTED_CHUNKS = 8;
+// host's polls in between.  Up to 4 chunk lengths: a longer prediction is the likelier to
+// overshoot (Hexa8 100^3 MIXED 85.8-86.4 ms against 86.7-87.4 with up to 8).
+constexpr int PREDICTED_CHUNKS = 4;
 static int predict_chunk(const ProbScalars &S, double rr, double rr_ref, double rate, int dflt, int cap) {
     if (!(rate < 0) || !(rr > 0)) return dflt;
     double thr = S.crit_kind == 0 ? S.stop2 : 0.0;

@@ -111,9 +111,9 @@ def _counters(d):
     for r in rows:
         name = r.get("Kernel_Name", "")
         m = pat.search(name)
-        mf = re.search(r"k_mf_(block)<(double|float), false, false, (?:true|false)(?:, \d+)?(?:, false)?>", name) or \
-            re.search(r"k_mf_(halo)<(double|float)(?:, false)?>", name)
-        it = re.search(r"k_mf_block<(double|float), true, true, (?:true|false), \d+, true>", name)
+        mf = re.search(r"k_mf_(block)<(double|float), false, false, (?:true|false)(?:, \d+)?(?:, false)?(?:, (?:true|false))?>",
+                       name) or re.search(r"k_mf_(halo)<(double|float)(?:, false)?>", name)
+        it = re.search(r"k_mf_block<(double|float), true, true, (?:true|false), \d+, true(?:, (?:true|false))?>", name)
         up = re.search(r"k_update_mf<\d, (double|float), true>", name)
         if "k_copy" in name:
             key = "copy"

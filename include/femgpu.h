@@ -97,7 +97,7 @@ typedef struct {
     double reliable_delta;    /* MIXED: FP64 residual refresh when the FP32 residual fell by this factor */
     int chunk;                /* iterations enqueued between host checks; a single problem's
                                  chunk ends at its predicted next check or refresh instead, up to
-                                 8 x chunk once its convergence rate is known */
+                                 4 x chunk once its convergence rate is known */
 } femgpu_solve_opts;
 
 typedef struct {

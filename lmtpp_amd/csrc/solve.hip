@@ -202,9 +202,10 @@ enum { PRE_JACOBI = 0, PRE_BLOCK_JACOBI = 1, PRE_IC0 = 2, PRE_CHEB = 3 };
 // FP32 vector update of the two-kernel CG iteration: lanes per 1024-row workgroup (1024 / UPD_NT32
 // rows each) and p.q partials loaded per lane in one round of the combine (Hexa8 100^3, us per
 // CG iteration: 256 lanes and 4 partials 55.6, 8 partials 55.6, 16 60.3; 512 lanes and 4
-// partials 55.1, 8 partials 58.0; 1024 lanes 59.3)
+// partials 55.1, 8 partials 58.0; 1024 lanes 59.3.  512 lanes: MIXED solve 84.7-85.7 against
+// 85.7-86.5 ms with 256, the batched 256-cube solve 141 against 148 ms)
 #ifndef UPD_NT32
-#define UPD_NT32 256
+#define UPD_NT32 512
 #endif
 #ifndef UPD_CPL32
 #define UPD_CPL32 4
@@ -417,9 +418,9 @@ __global__ __launch_bounds__(FOLD ? (sizeof(T) == 8 ? 1024 : UPD_NT32) : 256, FO
     };
     double o[1];
     double acc[3] = {0, 0, 0};
-    // FOLD: 1024 rows per workgroup, as 1024 lanes of one row (FP64) or 256 lanes of 4 rows,
-    // rows row0 + lane + 256 j (FP32); measured at Hexa8 100^3: FP64 45.9 against 51.4 us, FP32
-    // 24.1 against 25.5 us
+    // FOLD: 1024 rows per workgroup, as 1024 lanes of one row (FP64) or UPD_NT32 lanes of
+    // 1024 / UPD_NT32 rows, rows row0 + lane + UPD_NT32 j (FP32); measured at Hexa8 100^3 against
+    // 256-row workgroups: FP64 45.9 against 51.4 us, FP32 (256 lanes) 24.1 against 25.5 us
     constexpr int NT = FOLD ? (sizeof(T) == 8 ? 1024 : UPD_NT32) : 256;
     constexpr int RPL = FOLD ? 1024 / NT : 1;
     // FP32: the p.q partials (one per element block) in one round of loads per lane
